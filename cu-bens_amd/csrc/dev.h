@@ -1,0 +1,115 @@
+// Device state of one model and the launchers of the hot-path kernels.
+//
+// HBM layout:
+//  * per-element state keeps the reference's flat layouts (three values per
+//    element for triads, 18 per shell for element forces, ...) so the state can
+//    be exchanged with the reference-shaped host arrays by plain copies; each
+//    thread of an element kernel reads its own contiguous 24-144 byte records;
+//  * the global tangent is joint-block CSR (BSR) with bs x bs row-major blocks,
+//    both triangles stored, over bs DOF slots per joint; slots of fixed DOFs
+//    carry an identity row/column so the block system is nonsingular;
+//  * driver vectors (d, f, r, ...) live in the reference's equation order
+//    (NEQ), the solver vectors in slot order (NJ*bs);
+//  * the band factor is stored as dense T x T tiles (tile rows of w+1 tiles).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "model.h"
+
+namespace cb {
+
+#define CB_CHECK(x)                                                                       \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess) throw ::cb::DeviceError(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct DeviceError {
+    std::string msg;
+    explicit DeviceError(std::string m) : msg(std::move(m)) {}
+};
+
+// Generations of per-element state, as in the reference: base (last converged),
+// _i (current iterate), _ip (previous iterate).
+enum Gen { G_BASE = 0, G_I = 1, G_IP = 2 };
+
+struct Dev {
+    hipStream_t s = nullptr;
+    long nj = 0, ntr = 0, nfr = 0, nsh = 0, neq = 0, nslot = 0, nnzb = 0;
+    int bs = 6;
+    int anaflag = 2;
+    // topology (int32)
+    int32_t *minc = nullptr, *mcode = nullptr, *jcode = nullptr, *eq_of_slot = nullptr;
+    int32_t *rowptr = nullptr, *colidx = nullptr, *diagblk = nullptr;
+    int32_t *blk_tr = nullptr, *blk_sh = nullptr, *col_el_tr = nullptr, *col_el_sh = nullptr;
+    std::vector<int32_t> col_ptr_tr, col_ptr_sh;
+    // properties
+    double *emod = nullptr, *carea = nullptr, *llen = nullptr;
+    double *nu = nullptr, *thick = nullptr, *xlocal = nullptr, *farea = nullptr, *slength = nullptr;
+    // state
+    double *x[3] = {nullptr, nullptr, nullptr};  // [G_BASE]=x, [G_I]=x_temp, [G_IP]=x_ip
+    double *c1[3] = {}, *c2[3] = {}, *c3[3] = {};
+    double *ef[3] = {}, *defllen[3] = {}, *deffarea[3] = {}, *defslen[3] = {};
+    // tangent values (FP64 and optional FP32 copy)
+    double* K = nullptr;
+    float* K32 = nullptr;
+    // reduction scratch
+    double* red = nullptr;
+    double* red_host = nullptr;   // pinned
+    int red_blocks = 0;
+};
+
+// ---- setup ---------------------------------------------------------------
+void dev_create(Dev& d, const Model& m);
+void dev_destroy(Dev& d);
+void dev_reset_state(Dev& d, const Model& m);   // all generations <- model geometry
+
+template <class T> T* dalloc(size_t n) {
+    T* p = nullptr;
+    CB_CHECK(hipMalloc(&p, sizeof(T) * (n ? n : 1)));
+    return p;
+}
+template <class T> void h2d(T* dst, const T* src, size_t n, hipStream_t s) {
+    if (n) CB_CHECK(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyHostToDevice, s));
+}
+template <class T> void d2h(T* dst, const T* src, size_t n, hipStream_t s) {
+    if (n) CB_CHECK(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+}
+template <class T> void d2d(T* dst, const T* src, size_t n, hipStream_t s) {
+    if (n) CB_CHECK(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+}
+
+// ---- element kernels (kernels.hip) -----------------------------------------
+// K <- tangent at generation g (coords x[gx], triads c[g], forces ef[g], ...).
+void launch_tangent(Dev& d, int gx, int g, bool want_fp32);
+// f += internal forces; reads ef_ip (clears its membrane entries, shell.c:1773),
+// writes ef_i; triads c_ip (previous) and c_i (current), coordinates x[gx].
+void launch_forces(Dev& d, int gx, double* f, const double* dd);
+// coordinates x[gx] += dd (x_ip <- old x[gx]); element lengths/areas/triads -> _i.
+void launch_updatc(Dev& d, int gx, const double* dd);
+// copy generations of per-element state: dst <- src (ef optional)
+void copy_gen(Dev& d, int dst, int src, bool with_ef, bool with_x);
+
+// ---- vectors -----------------------------------------------------------------
+// out[k] = sum_i a_k[i]*b_k[i], k < n (n <= 8), deterministic two-level reduction.
+void multi_dot(Dev& d, int n, const double* const* a, const double* const* b, long len, double* out);
+void vfill(Dev& d, double* y, double v, long n);
+void vcopy(Dev& d, double* y, const double* x, long n);
+// y = a*x + b*y
+void vaxpby(Dev& d, double* y, double a, const double* x, double b, long n);
+// r = lam*q - f
+void vresidual(Dev& d, double* r, double lam, const double* q, const double* f, long n);
+// neq <-> slot layouts
+void to_slots(Dev& d, double* ext, const double* v);
+void from_slots(Dev& d, double* v, const double* ext);
+
+// ---- sparse block operator -----------------------------------------------------
+void bsr_spmv(Dev& d, const double* x, double* y);                 // FP64 values
+void bsr_spmv32(Dev& d, const double* x, double* y);               // FP32 values, FP64 vectors
+
+}  // namespace cb

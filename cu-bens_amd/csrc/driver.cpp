@@ -1,0 +1,557 @@
+// Nonlinear drivers over device state.  The control flow restates the
+// reference's drivers (NR/MNR main.c:1800-2157, MSAL main.c:2158-3141,
+// arc.c:70-157, misc.c:187-250); all vectors and element state stay on the
+// device, the host sees scalars (dots, pivots summaries, flags).
+#include "driver.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace cb {
+namespace {
+
+double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Timer {
+    double* acc;
+    hipStream_t s;
+    double t0;
+    Timer(double* a, hipStream_t st) : acc(a), s(st), t0(now_s()) {}
+    ~Timer() {
+        (void)hipStreamSynchronize(s);
+        *acc += now_s() - t0;
+    }
+};
+
+}  // namespace
+
+Analysis::Analysis(Model& m, Dev& d, const RunOptions& o) : m_(m), d_(d), o_(o) {
+    const long n = m.neq;
+    double** vs[] = {&q, &dv, &dp, &dpp, &dt, &dd, &ddq, &ddr, &f, &fp, &ft, &fip, &r, &tmp};
+    for (double** p : vs) {
+        *p = dalloc<double>(size_t(n));
+        CB_CHECK(hipMemsetAsync(*p, 0, sizeof(double) * size_t(n), d.s));
+    }
+    eb = dalloc<double>(size_t(d.nslot));
+    ex = dalloc<double>(size_t(d.nslot));
+    h2d(q, m.q.data(), size_t(n), d.s);
+    const bool need64 = o.solver == SOLVER_BAND64 || o.judge || o.solver != SOLVER_BAND32_FGMRES;
+    if (need64) b64_.setup(n, m.bandwidth, d.s);
+    if (o.solver == SOLVER_BAND32_FGMRES) b32_.setup(n, m.bandwidth, d.s);
+    if (o.solver != SOLVER_BAND64) gm_.setup(d.nslot, o.restart);
+    CB_CHECK(hipStreamSynchronize(d.s));
+}
+
+Analysis::~Analysis() {
+    double* vs[] = {q, dv, dp, dpp, dt, dd, ddq, ddr, f, fp, ft, fip, r, tmp, eb, ex};
+    for (double* p : vs)
+        if (p) (void)hipFree(p);
+    b64_.release();
+    b32_.release();
+    gm_.release();
+    bj_.release();
+}
+
+double* Analysis::dev_vec(int which) {
+    double* v[] = {q, dv, dp, dpp, dt, dd, ddq, ddr, f, fp, ft, fip, r};
+    return (which >= 0 && which < 13) ? v[which] : nullptr;
+}
+
+void Analysis::record(double lam, int ite, const double* dvec) {
+    lpf.push_back(lam);
+    itecnt.push_back(ite);
+    CB_CHECK(hipStreamSynchronize(d_.s));
+    step_t.push_back(now_s() - t_start_);
+    step_judge.push_back(st_->t_judge);
+    step_newton.push_back(st_->newton);
+    if (o_.keep_d) {
+        size_t off = dpath.size();
+        dpath.resize(off + size_t(m_.neq));
+        d2h(dpath.data() + off, dvec, size_t(m_.neq), d_.s);
+        CB_CHECK(hipStreamSynchronize(d_.s));
+    }
+}
+
+void Analysis::update_and_forces(int gx, double* fout, const double* ddv) {
+    Timer t(&st_->t_forces, d_.s);
+    launch_updatc(d_, gx, ddv);
+    launch_forces(d_, gx, fout, ddv);
+}
+
+void Analysis::reset_from_base() {
+    copy_gen(d_, G_I, G_BASE, true, true);
+    copy_gen(d_, G_IP, G_BASE, true, false);
+}
+
+void Analysis::commit() { copy_gen(d_, G_BASE, G_I, true, true); }
+
+// Factorization (or preconditioner set-up) of the tangent in d.K, pivots for
+// the arc-length decisions.  Returns 0 or a nonzero code on a singular factor.
+int Analysis::factor_current(int* det) {
+    int rc = 0;
+    bool have_piv = false;
+    {
+        Timer t(&st_->t_factor, d_.s);
+        if (o_.solver == SOLVER_BAND64) {
+            rc = b64_.factor(d_);
+            pivots = b64_.piv_host;
+            have_piv = true;
+        } else if (o_.solver == SOLVER_BAND32_FGMRES) {
+            rc = b32_.factor(d_);
+            pivots = b32_.piv_host;
+            have_piv = true;
+        } else if (o_.solver == SOLVER_BJ_FGMRES) {
+            bj_.setup(d_);
+        } else if (learned.setup) {
+            learned.setup(d_);
+        } else {
+            bj_.setup(d_);
+        }
+        st_->factors++;
+    }
+    if (o_.solver != SOLVER_BAND64 && o_.judge) {
+        // exact FP64 pivots for det/psi (and the direct-solve fallback)
+        Timer t(&st_->t_judge, d_.s);
+        int rj = b64_.factor(d_);
+        pivots = b64_.piv_host;
+        have_piv = true;
+        if (rc == 0) rc = rj;
+    }
+    if (det) {
+        *det = 0;
+        if (have_piv)
+            for (long i = 0; i < m_.neq; ++i)
+                if (!(pivots[i] > 0)) { *det = 1; break; }
+    }
+    return rc;
+}
+
+double Analysis::psi_now() const {
+    double psi = 1;
+    if (pivots.size() < size_t(m_.neq) || piv0_.size() < size_t(m_.neq)) return psi;
+    for (long i = 0; i < m_.neq; ++i) {
+        double t = std::fabs(pivots[i] / piv0_[i]);
+        if (t < psi) psi = t;
+    }
+    return psi;
+}
+
+int Analysis::tangent_and_factor(int gx, int g, int* det) {
+    RunStats tmpst;
+    RunStats* keep = st_;
+    if (!st_) st_ = &tmpst;
+    {
+        Timer t(&st_->t_tangent, d_.s);
+        launch_tangent(d_, gx, g, false);
+    }
+    int rc = factor_current(det);
+    st_ = keep;
+    return rc;
+}
+
+int Analysis::factor_only(int* det) {
+    RunStats tmpst;
+    RunStats* keep = st_;
+    if (!st_) st_ = &tmpst;
+    int rc = factor_current(det);
+    st_ = keep;
+    return rc;
+}
+
+int Analysis::linear_solve(const double* rhs, double* x) {
+    RunStats tmpst;
+    RunStats* keep = st_;
+    if (!st_) st_ = &tmpst;
+    Timer t(&st_->t_solve, d_.s);
+    st_->solves++;
+    int rc = 0;
+    if (o_.solver == SOLVER_BAND64) {
+        b64_.solve(rhs, x);
+    } else {
+        std::function<void(const double*, double*)> prec;
+        if (o_.solver == SOLVER_BAND32_FGMRES) {
+            prec = [&](const double* v, double* z) {
+                from_slots(d_, tmp, v);
+                b32_.solve(tmp, tmp);
+                to_slots(d_, z, tmp);
+            };
+        } else if (o_.solver == SOLVER_LEARNED_FGMRES && learned.apply) {
+            prec = [&](const double* v, double* z) { learned.apply(d_, v, z); };
+        } else {
+            prec = [&](const double* v, double* z) { bj_.apply(d_, v, z); };
+        }
+        to_slots(d_, eb, rhs);
+        double rel = 0;
+        int it = gm_.solve(d_, prec, eb, ex, o_.krylov_tol, o_.maxit, &rel);
+        st_->krylov_iters += it;
+        st_->worst_relres = std::max(st_->worst_relres, rel);
+        if (!(rel <= 10 * o_.krylov_tol) && b64_.tiles) {
+            // fall back to the direct solve (the judge's factor)
+            st_->krylov_fallbacks++;
+            b64_.solve(rhs, x);
+        } else {
+            from_slots(d_, x, ex);
+            if (!(rel <= 10 * o_.krylov_tol)) rc = 3;
+        }
+    }
+    st_ = keep;
+    return rc;
+}
+
+int Analysis::run(RunStats& st) {
+    st_ = &st;
+    lpf.clear();
+    itecnt.clear();
+    dpath.clear();
+    step_t.clear();
+    step_judge.clear();
+    step_newton.clear();
+    dev_reset_state(d_, m_);
+    for (double* v : {q, dv, dp, dpp, dt, dd, ddq, ddr, f, fp, ft, fip, r})
+        CB_CHECK(hipMemsetAsync(v, 0, sizeof(double) * size_t(m_.neq), d_.s));
+    h2d(q, m_.q.data(), size_t(m_.neq), d_.s);
+    CB_CHECK(hipStreamSynchronize(d_.s));
+    double t0 = now_s();
+    t_start_ = t0;
+    int rc = 0;
+    try {
+        if (m_.algflag == 1 || m_.algflag == 2) rc = run_nr();
+        else if (m_.algflag == 3) rc = run_msal();
+        else { rc = 2; st.msg = "driver not available for this ALGFLAG"; }
+    } catch (const DeviceError& e) {
+        rc = 4;
+        st.msg = e.msg;
+    }
+    CB_CHECK(hipStreamSynchronize(d_.s));
+    st.t_total = now_s() - t0;
+    st.status = rc;
+    st_ = nullptr;
+    return rc;
+}
+
+#define FAIL(code, text) do { st_->msg = (text); return (code); } while (0)
+
+// misc.c:187-250 on the fused sums; returns 1 on a zero-norm error.
+static int test_decide(const double s[5], double intener1, double toldisp, double tolforc,
+                       double tolener, int* convchk) {
+    *convchk = 0;
+    if (toldisp < 1) {
+        if (s[1] == 0) return 1;
+        if (std::sqrt(s[0]) / std::sqrt(s[1]) > toldisp) *convchk += 10;
+    }
+    if (tolforc < 1) {
+        if (s[3] == 0) return 1;
+        if (std::sqrt(s[2]) / std::sqrt(s[3]) > tolforc) *convchk += 100;
+    }
+    if (tolener < 1) {
+        if (intener1 == 0) return 1;
+        if (std::fabs(s[4] / intener1) > tolener) *convchk += 1000;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- NR / MNR
+int Analysis::run_nr() {
+    const long n = m_.neq;
+    double lam = m_.lpf0, dlam = m_.dlpf0, dlamp;
+    int solcnt = 0, subcnt = 0, itecnt_ = 0, convchk = 0, det = 0;
+    double intener1 = 0;
+    copy_gen(d_, G_I, G_BASE, true, true);
+    copy_gen(d_, G_IP, G_BASE, true, true);
+    do {
+        if (lam > m_.lpfmax) lam = m_.lpfmax;
+        vcopy(d_, fp, f, n);
+        vcopy(d_, dt, dv, n);
+        vcopy(d_, ft, f, n);
+        dlamp = dlam;
+        reset_from_base();
+        itecnt_ = 0;
+        do {
+            vresidual(d_, r, lam, q, ft, n);
+            const bool full = (m_.algflag == 1 || itecnt_ == 0);
+            if (full) {
+                {
+                    Timer t(&st_->t_tangent, d_.s);
+                    launch_tangent(d_, G_I, G_IP, false);
+                }
+                // solve() discards the factor status (solve.c:86-88, 535)
+                factor_current(&det);
+            }
+            linear_solve(r, dd);
+            vaxpby(d_, dt, 1.0, dd, 1.0, n);
+            vcopy(d_, fip, ft, n);
+            vfill(d_, ft, 0.0, n);
+            update_and_forces(G_I, ft, dd);
+            d2d(d_.ef[G_IP], d_.ef[G_I], size_t(d_.ntr * 2 + d_.nfr * 14 + d_.nsh * 18), d_.s);
+            double s[5];
+            test_sums(d_, dd, dt, q, lam, ft, fp, fip, n, s);
+            if (itecnt_ == 0) {
+                // intener1 = dd . (qtot - fp): the fifth sum with fp in place of f_ip
+                double s2[5];
+                test_sums(d_, dd, dt, q, lam, ft, fp, fp, n, s2);
+                intener1 = s2[4];
+            }
+            if (test_decide(s, intener1, m_.toldisp, m_.tolforc, m_.tolener, &convchk))
+                FAIL(1, "zero norm in convergence test");
+            shift_ip(true);
+            itecnt_++;
+            st_->newton++;
+        } while (convchk != 0 && itecnt_ <= m_.itemax);
+
+        if (convchk != 0 && subcnt <= m_.submax) {
+            if (lam == m_.lpfmax) FAIL(1, "maximum load factor attempted without convergence");
+            if (dlamp == m_.dlpfmin) FAIL(1, "minimum load factor increment reached");
+            dlam = dlamp / 2;
+            if (dlam < m_.dlpfmin) dlam = m_.dlpfmin;
+            lam = lam - dlamp + dlam;
+            subcnt++;
+            solcnt = 0;
+        } else if (subcnt > m_.submax) {
+            FAIL(1, "maximum number of subdivisions exceeded");
+        } else {
+            vcopy(d_, dv, dt, n);
+            vcopy(d_, f, ft, n);
+            commit();
+            record(lam, itecnt_, dv);
+            if (long(lpf.size()) >= o_.max_steps) return 0;
+            solcnt++;
+            subcnt = 0;
+            if (solcnt >= m_.solmin) {
+                dlam *= 2;
+                if (dlam > m_.dlpfmax) dlam = m_.dlpfmax;
+                solcnt = 0;
+            }
+            lam += dlam;
+        }
+    } while (lam <= m_.lpfmax);
+    return 0;
+}
+
+// ---------------------------------------------------------------- MSAL
+int Analysis::run_msal() {
+    const long n = m_.neq, k = m_.dkdof;
+    int det = 0, convchk = 0, itecnt_ = 0;
+    double lam, dlam, intener1, lamp = 0, lampp = 0;
+    copy_gen(d_, G_I, G_BASE, true, false);
+    copy_gen(d_, G_IP, G_BASE, true, false);
+    auto fetch = [&](const double* v, long i) {
+        double h;
+        d2h(&h, v + i, 1, d_.s);
+        CB_CHECK(hipStreamSynchronize(d_.s));
+        return h;
+    };
+    auto dots = [&](int cnt, std::initializer_list<const double*> a, std::initializer_list<const double*> b,
+                    double* out) {
+        std::vector<const double*> va(a), vb(b);
+        Timer t(&st_->t_other, d_.s);
+        multi_dot(d_, cnt, va.data(), vb.data(), n, out);
+    };
+
+    // first increment: displacement control at the arc-length DOF
+    {
+        Timer t(&st_->t_tangent, d_.s);
+        launch_tangent(d_, G_BASE, G_BASE, false);
+    }
+    factor_current(&det);
+    piv0_ = pivots;
+    linear_solve(q, ddq);
+    lam = m_.dk / fetch(ddq, k);
+    vaxpby(d_, dv, lam, ddq, 0.0, n);
+    vaxpby(d_, dd, lam, ddq, 0.0, n);
+    {
+        double s;
+        dots(1, {dd}, {q}, &s);
+        intener1 = lam * s;
+    }
+    vfill(d_, f, 0.0, n);
+    update_and_forces(G_BASE, f, dd);
+    itecnt_ = 1;
+    shift_ip(true);
+    do {
+        // qtot is formed before the load factor moves and test() sees that
+        // qtot (main.c:2415-2485)
+        const double lam_q = lam;
+        vresidual(d_, r, lam, q, f, n);
+        linear_solve(r, ddr);
+        dlam = -fetch(ddr, k) / fetch(ddq, k);
+        lam += dlam;
+        vcopy(d_, dd, ddr, n);
+        vaxpby(d_, dd, dlam, ddq, 1.0, n);
+        vaxpby(d_, dv, 1.0, dd, 1.0, n);
+        vcopy(d_, fip, f, n);
+        vfill(d_, f, 0.0, n);
+        update_and_forces(G_BASE, f, dd);
+        double s[5];
+        test_sums(d_, dd, dv, q, lam_q, f, fp, fip, n, s);
+        if (test_decide(s, intener1, m_.toldisp, m_.tolforc, m_.tolener, &convchk))
+            FAIL(1, "zero norm in convergence test");
+        itecnt_++;
+        st_->newton++;
+        shift_ip(true);
+    } while (convchk != 0 && itecnt_ <= m_.itemax);
+    if (convchk != 0) FAIL(1, "initially prescribed displacement too large");
+    copy_gen(d_, G_BASE, G_I, true, false);
+    record(lam, itecnt_, dv);
+    if (long(lpf.size()) >= o_.max_steps) return 0;
+
+    double dkc = std::fabs(fetch(dv, k)), lamc = std::fabs(lam);
+    vcopy(d_, dpp, dp, n);
+    vcopy(d_, dp, dv, n);
+    vcopy(d_, fp, f, n);
+    lampp = lamp;
+    lamp = lam;
+    reset_from_base();
+    double dpdp;
+    dots(1, {dp}, {dp}, &dpdp);
+    double dnorm = std::sqrt(dpdp), dnormallow = m_.alpha * std::sqrt(dpdp);
+    double beta = std::sqrt(double(m_.iteopt) / double(itecnt_)) * (dnormallow / dnorm);
+    double psi = psi_now();
+
+    int errchk2 = 0, subcnt = 0, imagcnt = 0, negcnt = 0;
+    double arc = 0, a = 0, b, c, lamt;
+    double qq;
+    dots(1, {q}, {q}, &qq);
+    while (lamc <= m_.lpfmax && dkc <= m_.dkimax) {
+        vcopy(d_, dt, dv, n);
+        lamt = lam;
+        if (errchk2 == 0) {
+            // |dp - dpp|^2
+            double sq[3];
+            quad_sums(d_, dp, dpp, dpp, dpp, n, sq);
+            if (psi >= m_.psi_thresh)
+                arc = beta * std::sqrt(sq[0] + (lamp - lampp) * (lamp - lampp));
+            else
+                arc = beta * std::sqrt(sq[0]);
+        } else {
+            reset_from_base();
+            errchk2 = 0;
+        }
+        {
+            Timer t(&st_->t_tangent, d_.s);
+            launch_tangent(d_, G_BASE, G_BASE, false);
+        }
+        factor_current(&det);
+        linear_solve(q, ddq);
+        double qd;
+        dots(1, {ddq}, {ddq}, &qd);
+        a = (psi >= m_.psi_thresh) ? qq + qd : qd;
+        dlam = (det == 0) ? arc * std::sqrt(1 / a) : -arc * std::sqrt(1 / a);
+        lamt += dlam;
+        vaxpby(d_, dd, dlam, ddq, 0.0, n);
+        vaxpby(d_, dt, 1.0, dd, 1.0, n);
+        {
+            double s;
+            dots(1, {dd}, {q}, &s);
+            intener1 = dlam * s;
+        }
+        vfill(d_, ft, 0.0, n);
+        update_and_forces(G_I, ft, dd);
+        shift_ip(true);
+        itecnt_ = 1;
+        st_->newton++;
+        do {
+            const double lam_q = lamt;   // qtot of this iteration (main.c:2845-2855)
+            vresidual(d_, r, lamt, q, ft, n);
+            vcopy(d_, fip, ft, n);
+            vfill(d_, ft, 0.0, n);
+            linear_solve(r, ddr);
+            double s8[8], s2[2];
+            dots(8, {dt, dp, ddr, dt, dp, dt, dt, dp}, {ddq, ddq, ddq, ddr, ddr, dp, dt, dp}, s8);
+            dots(1, {ddr}, {ddr}, s2);
+            const double dtq = s8[0], dpq = s8[1], drq = s8[2], dtr = s8[3], dpr = s8[4];
+            const double dtp = s8[5], dtt = s8[6], dpp_ = s8[7], drr = s2[0];
+            if (psi >= m_.psi_thresh) {
+                b = 2 * (dtq - dpq + drq + (lamt - lamp) * qq);
+                c = 2 * (dtr - dpr - dtp) + dtt + dpp_ + drr + (lamt - lamp) * (lamt - lamp) * qq - arc * arc;
+            } else {
+                b = 2 * (dtq - dpq + drq);
+                c = 2 * (dtr - dpr - dtp) + dtt + dpp_ + drr - arc * arc;
+            }
+            // quad (arc.c:70-157)
+            double disc = b * b - 4 * a * c;
+            if (!(disc > 0)) {
+                errchk2 = 2;
+            } else {
+                double r1 = (-b + std::sqrt(b * b - 4 * a * c)) / (2 * a);
+                double r2 = (-b - std::sqrt(b * b - 4 * a * c)) / (2 * a);
+                double sq[3];
+                quad_sums(d_, dt, dp, ddr, ddq, n, sq);
+                double g1 = sq[0] + sq[1] + r1 * sq[2], g2 = sq[0] + sq[1] + r2 * sq[2];
+                double root;
+                if (g1 > g2 && g1 > 0) { root = r1; errchk2 = 0; }
+                else if (g2 > g1 && g2 > 0) { root = r2; errchk2 = 0; }
+                else { errchk2 = 3; root = 0; }
+                if (errchk2 == 0) {
+                    quad_update(d_, dt, dp, ddr, ddq, dd, root, n);
+                    dlam = root;
+                    lamt += dlam;
+                }
+            }
+            if (errchk2 == 0) {
+                update_and_forces(G_I, ft, dd);
+                double s[5];
+                test_sums(d_, dd, dt, q, lam_q, ft, fp, fip, n, s);
+                if (test_decide(s, intener1, m_.toldisp, m_.tolforc, m_.tolener, &convchk))
+                    FAIL(1, "zero norm in convergence test");
+                st_->newton++;
+                if (convchk == 0) {
+                    double sq[3];
+                    quad_sums(d_, dt, dp, dp, dp, n, sq);
+                    dnorm = std::sqrt(sq[0]);
+                    if (dnorm > 100 * dnormallow) {
+                        arc /= beta;
+                        beta = dnormallow / dnorm;
+                        arc *= beta;
+                        vcopy(d_, ft, f, n);
+                        errchk2 = 1;
+                    }
+                } else {
+                    itecnt_++;
+                    if (itecnt_ > m_.itemax) {
+                        arc *= 0.5;
+                        vcopy(d_, ft, f, n);
+                        subcnt++;
+                        errchk2 = 1;
+                    }
+                    shift_ip(true);
+                }
+            } else if (errchk2 == 2) {
+                arc *= 0.5;
+                vcopy(d_, ft, f, n);
+                imagcnt++;
+            } else if (errchk2 == 3) {
+                arc *= 0.5;
+                vcopy(d_, ft, f, n);
+                negcnt++;
+            }
+        } while (convchk != 0 && errchk2 == 0 && subcnt <= m_.submax && imagcnt <= m_.imagmax &&
+                 negcnt <= m_.negmax);
+        if (subcnt > m_.submax || imagcnt > m_.imagmax || negcnt > m_.negmax)
+            FAIL(1, "maximum allowable number of subdivisions exceeded");
+        if (errchk2 == 0) {
+            beta = std::sqrt(double(m_.iteopt) / double(itecnt_)) * (dnormallow / dnorm);
+            vcopy(d_, dpp, dp, n);
+            vcopy(d_, dv, dt, n);
+            vcopy(d_, dp, dt, n);
+            vcopy(d_, f, ft, n);
+            vcopy(d_, fp, ft, n);
+            lampp = lamp;
+            lam = lamp = lamt;
+            commit();
+            dkc = std::fabs(fetch(dv, k));
+            lamc = std::fabs(lam);
+            psi = psi_now();
+            subcnt = imagcnt = negcnt = 0;
+            record(lam, itecnt_, dv);
+            if (long(lpf.size()) >= o_.max_steps) return 0;
+        }
+    }
+    return 0;
+}
+
+}  // namespace cb

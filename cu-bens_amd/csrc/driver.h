@@ -1,0 +1,99 @@
+// Nonlinear drivers on the device: Newton-Raphson / modified NR load control
+// (main.c:1800-2157) and the modified spherical arc-length method
+// (main.c:2158-3141), over the device element kernels and linear solvers.
+#pragma once
+
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "band.h"
+#include "dev.h"
+#include "model.h"
+#include "solver.h"
+
+namespace cb {
+
+enum SolverKind {
+    SOLVER_BAND64 = 0,          // FP64 band LDL^T, direct (the reference's algorithm)
+    SOLVER_BAND32_FGMRES = 1,   // FGMRES(FP64 K) preconditioned by an FP32 band LDL^T
+    SOLVER_BJ_FGMRES = 2,       // FGMRES with joint-block Jacobi
+    SOLVER_LEARNED_FGMRES = 3,  // FGMRES with the learned block approximate inverse
+};
+
+struct RunOptions {
+    int solver = SOLVER_BAND64;
+    double krylov_tol = 1e-8;
+    int restart = 30;
+    int maxit = 20000;
+    int judge = 1;          // exact FP64 pivots for det/psi when the solver has none
+    int max_steps = 1 << 30;
+    int keep_d = 1;         // copy d to the host at every converged step
+};
+
+struct RunStats {
+    double t_total = 0, t_tangent = 0, t_factor = 0, t_solve = 0, t_forces = 0, t_judge = 0;
+    double t_other = 0;
+    long newton = 0, solves = 0, factors = 0, krylov_iters = 0, krylov_fallbacks = 0;
+    double worst_relres = 0;
+    int status = 0;
+    std::string msg;
+};
+
+// Hook for the learned preconditioner: called after each tangent assembly
+// (setup) and for each application z = M v on slot-layout vectors.
+struct PrecHook {
+    std::function<void(Dev&)> setup;
+    std::function<void(Dev&, const double*, double*)> apply;
+};
+
+class Analysis {
+public:
+    Analysis(Model& m, Dev& d, const RunOptions& o);
+    ~Analysis();
+    // Runs the driver of ALGFLAG; records each converged step.
+    int run(RunStats& st);
+    std::vector<double> lpf;
+    std::vector<int> itecnt;
+    std::vector<double> dpath;   // nsteps * NEQ
+    // per converged step: wall time since the run started, judge time and
+    // Newton iterations so far
+    std::vector<double> step_t, step_judge;
+    std::vector<long> step_newton;
+    PrecHook learned;
+    // one-shot operations used by the C ABI
+    int tangent_and_factor(int gx, int g, int* det);
+    int factor_only(int* det);   // factor the tangent already in d.K
+    int linear_solve(const double* rhs, double* x);   // device NEQ vectors
+    double* dev_vec(int which);
+    BandLDLT<double>& band64() { return b64_; }
+    BandLDLT<float>& band32() { return b32_; }
+    std::vector<double> pivots;   // pivots of the last factorization (host)
+
+private:
+    Model& m_;
+    Dev& d_;
+    RunOptions o_;
+    RunStats* st_ = nullptr;
+    double t_start_ = 0;
+    BandLDLT<double> b64_;
+    BandLDLT<float> b32_;
+    Fgmres gm_;
+    BlockJacobi bj_;
+    double *q = nullptr, *dv = nullptr, *dp = nullptr, *dpp = nullptr, *dt = nullptr, *dd = nullptr;
+    double *ddq = nullptr, *ddr = nullptr, *f = nullptr, *fp = nullptr, *ft = nullptr, *fip = nullptr;
+    double *r = nullptr, *eb = nullptr, *ex = nullptr, *tmp = nullptr;
+    std::vector<double> piv0_;
+
+    int run_nr();
+    int run_msal();
+    void record(double lam, int ite, const double* dvec);
+    void update_and_forces(int gx, double* fout, const double* ddv);
+    int factor_current(int* det);
+    double psi_now() const;
+    void shift_ip(bool with_ef) { copy_gen(d_, G_IP, G_I, with_ef, false); }
+    void reset_from_base();
+    void commit();
+};
+
+}  // namespace cb

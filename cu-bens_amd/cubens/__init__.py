@@ -1,0 +1,247 @@
+"""Python view of the MI355X CU-BENs solve path (libcubens_amd.so, include/cubens.h).
+
+The product is the C ABI; this module is a thin ctypes mirror used by tests and
+the benchmark.  There is no CPU fallback: every call goes through the HIP
+library, and opening a model fails loudly when the library or a GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libcubens_amd.so")
+
+SOLVER_BAND64 = 0
+SOLVER_BAND32_FGMRES = 1
+SOLVER_BJ_FGMRES = 2
+SOLVER_LEARNED_FGMRES = 3
+
+GEN_BASE, GEN_I, GEN_IP = 0, 1, 2
+
+
+class Info(C.Structure):
+    _fields_ = [("nj", C.c_long), ("ne_tr", C.c_long), ("ne_fr", C.c_long), ("ne_sh", C.c_long),
+                ("neq", C.c_long), ("lss", C.c_long), ("bandwidth", C.c_long),
+                ("nnz_blocks", C.c_long), ("bs", C.c_int), ("anaflag", C.c_int),
+                ("algflag", C.c_int), ("ncolours_tr", C.c_int), ("ncolours_sh", C.c_int)]
+
+
+class Options(C.Structure):
+    _fields_ = [("solver", C.c_int), ("krylov_tol", C.c_double), ("restart", C.c_int),
+                ("maxit", C.c_int), ("judge", C.c_int), ("max_steps", C.c_int),
+                ("keep_d", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("t_total", C.c_double), ("t_tangent", C.c_double), ("t_factor", C.c_double),
+                ("t_solve", C.c_double), ("t_forces", C.c_double), ("t_judge", C.c_double),
+                ("t_other", C.c_double), ("newton", C.c_long), ("solves", C.c_long),
+                ("factors", C.c_long), ("krylov_iters", C.c_long),
+                ("krylov_fallbacks", C.c_long), ("worst_relres", C.c_double),
+                ("status", C.c_int), ("nsteps", C.c_int)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def options(solver: int = SOLVER_BAND64, krylov_tol: float = 1e-8, restart: int = 30,
+            maxit: int = 20000, judge: int = 1, max_steps: int = 1 << 30, keep_d: int = 1) -> Options:
+    return Options(solver, krylov_tol, restart, maxit, judge, max_steps, keep_d)
+
+
+_lib = None
+
+# Every entry point of include/cubens.h with its ctypes signature.
+PD = C.POINTER(C.c_double)
+PL = C.POINTER(C.c_long)
+PI = C.POINTER(C.c_int)
+PI32 = C.POINTER(C.c_int32)
+ABI = {
+    "cb_last_error": (C.c_char_p, []),
+    "cb_open": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "cb_close": (None, [C.c_void_p]),
+    "cb_get_info": (C.c_int, [C.c_void_p, C.POINTER(Info)]),
+    "cb_get_dofmaps": (C.c_int, [C.c_void_p, PL, PL, PL]),
+    "cb_get_pattern": (C.c_int, [C.c_void_p, PI32, PI32]),
+    "cb_get_load": (C.c_int, [C.c_void_p, PD]),
+    "cb_state_set": (C.c_int, [C.c_void_p, C.c_char_p, PD, C.c_long]),
+    "cb_state_get": (C.c_int, [C.c_void_p, C.c_char_p, PD, C.c_long]),
+    "cb_tangent": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "cb_get_tangent": (C.c_int, [C.c_void_p, PD]),
+    "cb_internal_force": (C.c_int, [C.c_void_p, C.c_int, PD, PD]),
+    "cb_update_geometry": (C.c_int, [C.c_void_p, C.c_int, PD]),
+    "cb_factor": (C.c_int, [C.c_void_p, C.POINTER(Options), PI, PD]),
+    "cb_solve": (C.c_int, [C.c_void_p, PD, PD]),
+    "cb_run": (C.c_int, [C.c_void_p, C.POINTER(Options), C.POINTER(Stats)]),
+    "cb_get_path": (C.c_int, [C.c_void_p, PD, PI, PD]),
+    "cb_get_step_stats": (C.c_int, [C.c_void_p, PD, PD, PL]),
+    "cb_write_results2": (C.c_int, [C.c_void_p, C.c_char_p]),
+}
+
+
+def lib():
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with python cu-bens_amd/build.py")
+        _lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in ABI.items():
+            fn = getattr(_lib, name)
+            fn.restype = res
+            fn.argtypes = args
+    return _lib
+
+
+def _p(a: np.ndarray, kind=PD):
+    return a.ctypes.data_as(kind)
+
+
+class CubensError(RuntimeError):
+    pass
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise CubensError(lib().cb_last_error().decode(errors="replace") or f"error {rc}")
+
+
+class GpuModel:
+    """A model opened on the GPU (cb_open)."""
+
+    def __init__(self, path: str):
+        h = C.c_void_p()
+        _check(lib().cb_open(path.encode(), C.byref(h)))
+        self.h = h
+        inf = Info()
+        _check(lib().cb_get_info(h, C.byref(inf)))
+        self.info = inf
+        self.neq = inf.neq
+        self.nj = inf.nj
+        self.ntr, self.nfr, self.nsh = inf.ne_tr, inf.ne_fr, inf.ne_sh
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cb_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- maps --------------------------------------------------------------
+    def dofmaps(self):
+        jc = np.zeros(self.nj * 7, np.int64)
+        mc = np.zeros(max(1, self.ntr * 6 + self.nfr * 14 + self.nsh * 18), np.int64)
+        mx = np.zeros(self.neq + 1, np.int64)
+        _check(lib().cb_get_dofmaps(self.h, _p(jc, PL), _p(mc, PL), _p(mx, PL)))
+        return jc, mc[: self.ntr * 6 + self.nfr * 14 + self.nsh * 18], mx
+
+    def pattern(self):
+        rp = np.zeros(self.nj + 1, np.int32)
+        ci = np.zeros(self.info.nnz_blocks, np.int32)
+        _check(lib().cb_get_pattern(self.h, _p(rp, PI32), _p(ci, PI32)))
+        return rp, ci
+
+    def load(self):
+        q = np.zeros(self.neq)
+        _check(lib().cb_get_load(self.h, _p(q)))
+        return q
+
+    # ---- state -------------------------------------------------------------
+    def set(self, name: str, a: np.ndarray):
+        a = np.ascontiguousarray(a, np.float64)
+        _check(lib().cb_state_set(self.h, name.encode(), _p(a), a.size))
+
+    def get(self, name: str, n: int) -> np.ndarray:
+        a = np.zeros(n)
+        _check(lib().cb_state_get(self.h, name.encode(), _p(a), n))
+        return a
+
+    # ---- operations --------------------------------------------------------
+    def tangent(self, gen_x: int = GEN_BASE, gen_state: int = GEN_BASE) -> np.ndarray:
+        _check(lib().cb_tangent(self.h, gen_x, gen_state))
+        v = np.zeros(self.info.nnz_blocks * self.info.bs * self.info.bs)
+        _check(lib().cb_get_tangent(self.h, _p(v)))
+        return v
+
+    def internal_force(self, dd: np.ndarray, gen_x: int = GEN_I) -> np.ndarray:
+        dd = np.ascontiguousarray(dd, np.float64)
+        f = np.zeros(self.neq)
+        _check(lib().cb_internal_force(self.h, gen_x, _p(dd), _p(f)))
+        return f
+
+    def update_geometry(self, dd: np.ndarray, gen_x: int = GEN_I):
+        dd = np.ascontiguousarray(dd, np.float64)
+        _check(lib().cb_update_geometry(self.h, gen_x, _p(dd)))
+
+    def factor(self, opt: Options | None = None):
+        det = C.c_int()
+        piv = np.zeros(self.neq)
+        _check(lib().cb_factor(self.h, C.byref(opt) if opt is not None else None, C.byref(det), _p(piv)))
+        return det.value, piv
+
+    def solve(self, rhs: np.ndarray) -> np.ndarray:
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        x = np.zeros(self.neq)
+        _check(lib().cb_solve(self.h, _p(rhs), _p(x)))
+        return x
+
+    def run(self, opt: Options | None = None) -> dict:
+        st = Stats()
+        _check(lib().cb_run(self.h, C.byref(opt) if opt is not None else None, C.byref(st)))
+        n = st.nsteps
+        lpf = np.zeros(max(n, 1))
+        ite = np.zeros(max(n, 1), np.int32)
+        keep = opt is None or opt.keep_d
+        d = np.zeros((max(n, 1), self.neq)) if keep else None
+        _check(lib().cb_get_path(self.h, _p(lpf), _p(ite, PI), _p(d) if keep else None))
+        tw = np.zeros(max(n, 1))
+        tj = np.zeros(max(n, 1))
+        nw = np.zeros(max(n, 1), np.int64)
+        _check(lib().cb_get_step_stats(self.h, _p(tw), _p(tj), _p(nw, PL)))
+        out = st.as_dict()
+        out.update(step_t=tw[:n], step_judge=tj[:n], step_newton=nw[:n])
+        out.update(lpf=lpf[:n], itecnt=ite[:n], d=d[:n] if keep else None,
+                   msg=lib().cb_last_error().decode(errors="replace") if st.status else "")
+        return out
+
+    def write_results2(self, path: str):
+        _check(lib().cb_write_results2(self.h, path.encode()))
+
+
+def bsr_to_dense(vals: np.ndarray, rowptr: np.ndarray, colidx: np.ndarray, bs: int,
+                 eq_of_slot: np.ndarray, neq: int) -> np.ndarray:
+    """Dense NEQ x NEQ matrix from joint-block values (fixed slots dropped)."""
+    K = np.zeros((neq, neq))
+    nj = len(rowptr) - 1
+    v = vals.reshape(-1, bs, bs)
+    for a in range(nj):
+        for k in range(rowptr[a], rowptr[a + 1]):
+            b = colidx[k]
+            ia = eq_of_slot[a * bs:(a + 1) * bs]
+            ib = eq_of_slot[b * bs:(b + 1) * bs]
+            for p in range(bs):
+                if ia[p] < 0:
+                    continue
+                for q in range(bs):
+                    if ib[q] < 0:
+                        continue
+                    K[ia[p], ib[q]] = v[k, p, q]
+    return K
+
+
+def eq_of_slot(jcode: np.ndarray, nj: int, bs: int) -> np.ndarray:
+    j = jcode.reshape(nj, 7)[:, :bs].astype(np.int64) - 1
+    return j.reshape(-1)

@@ -1,0 +1,130 @@
+/*
+ * cubens.h -- C ABI of the MI355X CU-BENs nonlinear solve path.
+ *
+ * The reference has no library boundary: its drivers in main.c call the element,
+ * geometry and solver routines declared in prototypes.h directly, over arrays
+ * allocated in main().  Each entry point below replaces one of those calls (or
+ * a whole driver) and is marked with the reference interface it stands in for.
+ * Plain C types only; host arrays use the reference's flat layouts:
+ *   triads c1/c2/c3 [NE_TR | NE_FR*3 | NE_SH*3], element forces ef
+ *   [NE_TR*2 | NE_FR*14 | NE_SH*18], coordinates x [NJ*3], vectors [NEQ].
+ * All hot-path state lives on the GPU; a handle is not thread-safe.
+ * Every function returns 0 on success; on failure a nonzero code and
+ * cb_last_error() describes the error.  No entry point falls back to a CPU
+ * implementation: without a usable GPU cb_open fails.
+ */
+#ifndef CUBENS_H
+#define CUBENS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cb_model cb_model;
+
+typedef struct {
+    long nj, ne_tr, ne_fr, ne_sh, neq;
+    long lss;          /* skyline length (model.c:1276-1281)            */
+    long bandwidth;    /* max column height of the skyline             */
+    long nnz_blocks;   /* joint blocks of the device tangent           */
+    int bs;            /* DOF slots per joint in the device layout     */
+    int anaflag, algflag;
+    int ncolours_tr, ncolours_sh;
+} cb_info;
+
+/* Linear solvers replacing solve()/skyfact/skysolve (solve.c:51-89, 539-698). */
+enum {
+    CB_SOLVER_BAND64 = 0,         /* FP64 band LDL^T on the GPU                  */
+    CB_SOLVER_BAND32_FGMRES = 1,  /* FGMRES, FP32 band LDL^T preconditioner      */
+    CB_SOLVER_BJ_FGMRES = 2,      /* FGMRES, 6x6 joint-block Jacobi              */
+    CB_SOLVER_LEARNED_FGMRES = 3  /* FGMRES, learned block approximate inverse   */
+};
+
+typedef struct {
+    int solver;
+    double krylov_tol;   /* relative residual target of FGMRES            */
+    int restart;
+    int maxit;
+    int judge;           /* 1: exact FP64 pivots for det/psi               */
+    int max_steps;       /* stop after this many converged steps           */
+    int keep_d;          /* keep the displacement path for cb_path         */
+} cb_options;
+
+typedef struct {
+    double t_total, t_tangent, t_factor, t_solve, t_forces, t_judge, t_other;
+    long newton, solves, factors, krylov_iters, krylov_fallbacks;
+    double worst_relres;
+    int status;          /* 0: the driver ran to its end or to max_steps   */
+    int nsteps;
+} cb_stats;
+
+const char *cb_last_error(void);
+
+/* Reads model_def.txt and builds device state.  Replaces main.c:330-1653:
+ * the header reads, struc (model.c:42), codes (model.c:937), skylin
+ * (model.c:1145), coordinates, prop_tr/fr/sh (truss.c:41, frame.c:43,
+ * shell.c:43) and load (model.c:1285). */
+int cb_open(const char *path, cb_model **out);
+void cb_close(cb_model *m);
+int cb_get_info(const cb_model *m, cb_info *out);
+
+/* DOF maps as produced by codes()/skylin(): jcode [NJ*7], mcode
+ * [NE_TR*6 | NE_FR*14 | NE_SH*18], maxa [NEQ+1] (model.c:937-1283). */
+int cb_get_dofmaps(const cb_model *m, long *jcode, long *mcode, long *maxa);
+/* Joint-block pattern of the device tangent: rowptr [NJ+1], colidx [nnz_blocks]. */
+int cb_get_pattern(const cb_model *m, int32_t *rowptr, int32_t *colidx);
+/* Reference load vector q [NEQ] (model.c:1331). */
+int cb_get_load(const cb_model *m, double *q);
+
+/* Per-element / per-joint state in the reference layouts.  Names: "x", "x_temp",
+ * "x_ip", "c1","c2","c3" (converged), "c1_i".., "c1_ip".., "ef", "ef_i",
+ * "ef_ip", "defllen[_i|_ip]", "deffarea[_i|_ip]", "defslen[_i|_ip]". */
+int cb_state_set(cb_model *m, const char *name, const double *host, long n);
+int cb_state_get(cb_model *m, const char *name, double *host, long n);
+
+/* Tangent assembly: replaces ss = 0; stiff_tr/stiff_sh (truss.c:82,
+ * shell.c:110).  gen_x / gen_state select the coordinates and element state:
+ * 0 converged, 1 current (_i / x_temp), 2 previous (_ip).  The MSAL call of
+ * main.c:2717-2734 is (0,0); the NR call of main.c:1903-1921 is (1,2). */
+int cb_tangent(cb_model *m, int gen_x, int gen_state);
+/* Copies the assembled joint-block values [nnz_blocks*bs*bs] (rows/columns of
+ * fixed DOFs carry identity). */
+int cb_get_tangent(cb_model *m, double *vals);
+
+/* Internal force: replaces forces_tr/forces_sh (truss.c:231, shell.c:1593).
+ * f_host [NEQ] receives the force (accumulated from zero); dd_host [NEQ] is the
+ * increment; reads ef_ip (clears its membrane entries as shell.c:1773 does) and
+ * writes ef_i; coordinates of generation gen_x. */
+int cb_internal_force(cb_model *m, int gen_x, const double *dd_host, double *f_host);
+
+/* Geometry update: replaces updatc (misc.c:71): x[gen_x] += dd, x_ip <- old,
+ * lengths/areas/triads of generation _i. */
+int cb_update_geometry(cb_model *m, int gen_x, const double *dd_host);
+
+/* Factorization of the last assembled tangent with the solver of the options
+ * (skyfact, solve.c:539): det = 1 when a pivot is not positive; pivots [NEQ]
+ * may be NULL. */
+int cb_factor(cb_model *m, const cb_options *opt, int *det, double *pivots);
+/* Solve with the last factorization / preconditioner (skysolve, solve.c:655). */
+int cb_solve(cb_model *m, const double *rhs_host, double *x_host);
+
+/* Runs the driver of the model's ALGFLAG: NR / modified NR (main.c:1800-2157)
+ * or modified spherical arc length (main.c:2158-3141). */
+int cb_run(cb_model *m, const cb_options *opt, cb_stats *st);
+/* Load-displacement path of the last run, the triple misc.c:414-417 writes
+ * to results2.txt: lpf [nsteps], itecnt [nsteps], d [nsteps*NEQ] (full
+ * precision). */
+int cb_get_path(const cb_model *m, double *lpf, int *itecnt, double *d);
+/* Per converged step of the last run: wall seconds since the run started,
+ * seconds spent in the FP64 pivot judge, Newton iterations so far (no
+ * reference counterpart: the reference keeps no timers). */
+int cb_get_step_stats(const cb_model *m, double *t_wall, double *t_judge, long *newton);
+/* Writes the results2.txt rows of the last run (format of misc.c:350-417). */
+int cb_write_results2(const cb_model *m, const char *path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

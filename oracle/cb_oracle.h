@@ -1,0 +1,182 @@
+/*
+ * cb_oracle.h -- CPU restatement of the CU-BENs nonlinear solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path is
+ * compared against; it is never linked into, loaded by or called from the
+ * product library.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may use it.
+ *
+ * The restatement follows the reference's algorithms (file:line cited at each
+ * function in the .c files) and keeps the reference's flat array layouts so
+ * that the same numpy buffers can be handed to the oracle and to the reference
+ * element routines compiled from source into oracle/_ref:
+ *   per-element properties   [NE_TR | NE_FR | NE_SH]   (emod, yield)
+ *   carea, llength           [NE_TR | NE_FR]
+ *   nu, thick, dens(shell)   [NE_SH]           (no type offset, as in shell.c:61)
+ *   triads c1/c2/c3          [NE_TR | NE_FR*3 | NE_SH*3]
+ *   element forces ef        [NE_TR*2 | NE_FR*14 | NE_SH*18]
+ *   minc                     [NE_TR*2 | NE_FR*2 | NE_SH*3]   (1-based joints)
+ *   mcode                    [NE_TR*6 | NE_FR*14 | NE_SH*18] (1-based eqs, 0 = fixed)
+ *   jcode                    NJ*7                            (1-based eqs, 0 = fixed)
+ */
+#ifndef CB_ORACLE_H
+#define CB_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Sizes and flags the reference keeps in globals (main.c:323-328). */
+typedef struct {
+    long NJ, NE_TR, NE_FR, NE_SH, NEQ;
+    int ANAFLAG, ALGFLAG, SLVFLAG, OPTFLAG;
+} orc_dims;
+
+/* Parsed model_def.txt plus everything derived from it before the driver runs. */
+typedef struct orc_model {
+    orc_dims dm;
+    int chkpt, rflag;
+    long *minc, *jcode, *mcode, *maxa, *kht, *jnt;
+    int *wrpres;
+    long lss;
+    double *x;                        /* NJ*3 */
+    /* element properties (layouts in the header comment) */
+    double *emod, *yield, *carea, *llength, *dens_tr;
+    double *gmod, *dens_fr, *istrong, *iweak, *ipolar, *iwarp, *zstrong, *zweak;
+    double *offset, *auxpt, *xfr;     /* NE_FR*6, NE_FR*3, NE_FR*6 */
+    int *osflag, *mendrel;            /* NE_FR, NE_FR*5 */
+    double *nu, *thick, *dens_sh, *xlocal, *farea, *slength;
+    double *c1, *c2, *c3;
+    double *q;                        /* NEQ reference load */
+    /* dynamic input (ALGFLAG > 3) */
+    long ntstpsinpt; double ttot, dt;
+    double *pinpt;                    /* NEQ*ntstpsinpt load histories */
+    double *um, *vm, *am;             /* initial conditions */
+    /* driver parameters */
+    double lpfmax, lpf0, dlpf0, dlpfmax, dlpfmin;          /* NR / Newmark */
+    int itemax, submax, solmin, imagmax, negmax, iteopt;
+    double toldisp, tolforc, tolener;
+    double alpha, psi_thresh, dkimax, dk;                   /* MSAL */
+    long dkdof;                                             /* 0-based */
+    int numopt, spectrds;                                   /* Newmark */
+} orc_model;
+
+/* ---- model input (model.c, main.c:330-1653) ---------------------------- */
+orc_model *orc_open(const char *path, char *err, int errlen);
+void orc_free(orc_model *m);
+
+/* ---- element routines; argument lists mirror prototypes.h ------------- */
+void orc_stiff_tr(const orc_dims *d, double *ss, const double *emod, const double *carea,
+                  const double *llength, const double *defllen_ip, const double *c1_ip,
+                  const double *c2_ip, const double *c3_ip, const double *ef_ip,
+                  const long *maxa, const long *mcode);
+void orc_forces_tr(const orc_dims *d, double *f, double *ef_i, const double *emod,
+                   const double *carea, const double *llength, const double *defllen_i,
+                   const double *c1_i, const double *c2_i, const double *c3_i,
+                   const long *mcode);
+void orc_stiff_sh(const orc_dims *d, double *ss, const double *emod, const double *nu,
+                  const double *x_temp, const double *xlocal, const double *thick,
+                  const double *farea, const double *deffarea_ip, const double *slength,
+                  const double *c1_ip, const double *c2_ip, const double *c3_ip,
+                  const long *maxa, const long *minc, const long *mcode);
+void orc_forces_sh(const orc_dims *d, double *f, double *ef_ip, double *ef_i,
+                   const double *dd, const double *x_temp, const double *emod,
+                   const double *nu, const double *xlocal, const double *thick,
+                   const double *farea, const double *slength,
+                   const double *c1_ip, const double *c2_ip, const double *c3_ip,
+                   const double *c1_i, const double *c2_i, const double *c3_i,
+                   const long *minc, const long *mcode);
+void orc_updatc(const orc_dims *d, double *x_temp, double *x_ip, double *xfr_temp,
+                const double *dd, double *defllen_i, double *deffarea_i, double *defslen_i,
+                const double *offset, const int *osflag, const double *auxpt,
+                double *c1_i, double *c2_i, double *c3_i, const long *minc,
+                const long *jcode);
+/* One element's 18x18 global tangent (stiff_sh body, shell.c:140-305). */
+void orc_shell_elem_tangent(double E, double nu, double t, const double xl[3], double A0,
+                            const double sl[3], double A_ip, const double dm[6],
+                            const double c1[3], const double c2[3], const double c3[3],
+                            int geometric, double K[18 * 18]);
+
+/* ---- frame (frame.c) ---------------------------------------------------- */
+void orc_stiff_fr(const orc_dims *d, double *ss, const orc_model *m, const double *llength,
+                  const double *defllen_ip, const double *c1_ip, const double *c2_ip,
+                  const double *c3_ip, const double *ef_ip, const double *efFE_ip);
+void orc_forces_fr(const orc_dims *d, double *f, double *ef_ip, double *ef_i,
+                   const double *efFE_ip, double *efFE_i, const double *dd,
+                   const orc_model *m, const double *llength, const double *defllen_ip,
+                   const double *c1_ip, const double *c2_ip, const double *c3_ip,
+                   const double *c1_i, const double *c2_i, const double *c3_i);
+void orc_mass(const orc_dims *d, double *sm, const orc_model *m, const double *x,
+              double *llength_out);
+
+/* ---- solver pieces (solve.c, misc.c, arc.c) ------------------------------ */
+int orc_skyfact(long neq, const long *maxa, double *ss, double *ssd, int msal, int *det);
+void orc_skysolve(long neq, const long *maxa, const double *ss, double *rhs);
+int orc_test(long neq, const double *d_temp, const double *dd, const double *f_temp,
+             const double *fp, const double *qtot, const double *f_ip, double intener1,
+             int *convchk, double toldisp, double tolforc, double tolener);
+int orc_quad(long neq, double a, double b, double c, double *dt, const double *dp,
+             const double *ddr, const double *ddq, double *dd, double *dlpf, double *lpft);
+
+/* ---- element backend: lets the driver run on the oracle's own element
+ *      routines or on the reference's (oracle/_ref) ------------------------ */
+typedef struct orc_backend {
+    void *ctx;
+    void (*stiff_tr)(void *ctx, const orc_dims *d, double *ss, const double *emod,
+                     const double *carea, const double *llength, const double *defllen_ip,
+                     const double *yield, const double *c1_ip, const double *c2_ip,
+                     const double *c3_ip, const double *ef_ip, const long *maxa,
+                     const long *mcode);
+    void (*forces_tr)(void *ctx, const orc_dims *d, double *f, double *ef_i,
+                      const double *emod, const double *carea, const double *llength,
+                      const double *defllen_i, const double *yield, const double *c1_i,
+                      const double *c2_i, const double *c3_i, const long *mcode);
+    void (*stiff_sh)(void *ctx, const orc_dims *d, double *ss, const double *emod,
+                     const double *nu, const double *x_temp, const double *xlocal,
+                     const double *thick, const double *farea, const double *deffarea_ip,
+                     const double *slength, const double *defslen_ip, const double *yield,
+                     const double *c1_ip, const double *c2_ip, const double *c3_ip,
+                     const double *ef_ip, const long *maxa, const long *minc,
+                     const long *mcode);
+    int (*forces_sh)(void *ctx, const orc_dims *d, double *f, double *ef_ip, double *ef_i,
+                     const double *dd, const double *x_temp, const double *x_ip,
+                     const double *emod, const double *nu, const double *xlocal,
+                     const double *thick, const double *farea, const double *deffarea_ip,
+                     const double *slength, const double *defslen_ip, const double *yield,
+                     const double *c1_ip, const double *c2_ip, const double *c3_ip,
+                     const double *c1_i, const double *c2_i, const double *c3_i,
+                     const long *minc, const long *mcode, const long *jcode);
+    void (*updatc)(void *ctx, const orc_dims *d, double *x_temp, double *x_ip,
+                   double *xfr_temp, const double *dd, double *defllen_i,
+                   double *deffarea_i, double *defslen_i, const double *offset,
+                   const int *osflag, const double *auxpt, double *c1_i, double *c2_i,
+                   double *c3_i, const long *minc, const long *jcode);
+} orc_backend;
+
+void orc_default_backend(orc_backend *b);
+
+/* ---- drivers -------------------------------------------------------------- */
+/* One converged step of the load-displacement path (misc.c:414-417 writes the
+ * same triple to results2.txt). */
+typedef struct {
+    int max_steps;       /* capacity of the arrays below                      */
+    int nsteps;          /* rows written                                      */
+    double *lpf;         /* [max_steps]                                       */
+    int *itecnt;         /* [max_steps]                                       */
+    double *d;           /* [max_steps*NEQ] full-precision displacements      */
+    double *f;           /* [max_steps*NEQ] internal force at convergence (opt) */
+    int status;          /* 0 = ran to its end or to max_steps, else error code */
+    char msg[160];
+    double t_tangent, t_factor, t_solve, t_forces, t_other;  /* seconds */
+    long n_factor, n_solve, n_newton;
+} orc_path;
+
+/* Runs the analysis selected by ALGFLAG (1/2 NR/MNR main.c:1800-2157,
+ * 3 MSAL main.c:2158-3141, 5 nonlinear Newmark main.c:3305-3960) with the
+ * element routines of backend b (NULL = oracle's own). */
+int orc_run(orc_model *m, const orc_backend *b, orc_path *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

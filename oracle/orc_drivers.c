@@ -1,0 +1,630 @@
+/*
+ * orc_drivers.c -- oracle: the nonlinear drivers of main.c restated over the
+ * skyline LDL^T, with the element routines taken from a backend table so the
+ * same driver runs on the oracle's restatement or on the reference's own
+ * element routines (oracle/_ref).
+ *
+ * TEST INFRASTRUCTURE ONLY (see cb_oracle.h).
+ *
+ * NR / modified NR: main.c:1800-2157.  MSAL arc-length: main.c:2158-3141.
+ * Nonlinear Newmark: main.c:3305-3960 with solve.c:139-245, 469-533.
+ */
+#include "cb_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* ---- default backend: the oracle's own element routines ---------------- */
+static void be_stiff_tr(void *ctx, const orc_dims *d, double *ss, const double *emod,
+                        const double *carea, const double *llength, const double *defllen_ip,
+                        const double *yield, const double *c1_ip, const double *c2_ip,
+                        const double *c3_ip, const double *ef_ip, const long *maxa,
+                        const long *mcode)
+{
+    (void)ctx; (void)yield;
+    orc_stiff_tr(d, ss, emod, carea, llength, defllen_ip, c1_ip, c2_ip, c3_ip, ef_ip, maxa, mcode);
+}
+static void be_forces_tr(void *ctx, const orc_dims *d, double *f, double *ef_i,
+                         const double *emod, const double *carea, const double *llength,
+                         const double *defllen_i, const double *yield, const double *c1_i,
+                         const double *c2_i, const double *c3_i, const long *mcode)
+{
+    (void)ctx; (void)yield;
+    orc_forces_tr(d, f, ef_i, emod, carea, llength, defllen_i, c1_i, c2_i, c3_i, mcode);
+}
+static void be_stiff_sh(void *ctx, const orc_dims *d, double *ss, const double *emod,
+                        const double *nu, const double *x_temp, const double *xlocal,
+                        const double *thick, const double *farea, const double *deffarea_ip,
+                        const double *slength, const double *defslen_ip, const double *yield,
+                        const double *c1_ip, const double *c2_ip, const double *c3_ip,
+                        const double *ef_ip, const long *maxa, const long *minc,
+                        const long *mcode)
+{
+    (void)ctx; (void)defslen_ip; (void)yield; (void)ef_ip;
+    orc_stiff_sh(d, ss, emod, nu, x_temp, xlocal, thick, farea, deffarea_ip, slength, c1_ip,
+                 c2_ip, c3_ip, maxa, minc, mcode);
+}
+static int be_forces_sh(void *ctx, const orc_dims *d, double *f, double *ef_ip, double *ef_i,
+                        const double *dd, const double *x_temp, const double *x_ip,
+                        const double *emod, const double *nu, const double *xlocal,
+                        const double *thick, const double *farea, const double *deffarea_ip,
+                        const double *slength, const double *defslen_ip, const double *yield,
+                        const double *c1_ip, const double *c2_ip, const double *c3_ip,
+                        const double *c1_i, const double *c2_i, const double *c3_i,
+                        const long *minc, const long *mcode, const long *jcode)
+{
+    (void)ctx; (void)x_ip; (void)deffarea_ip; (void)defslen_ip; (void)yield; (void)jcode;
+    orc_forces_sh(d, f, ef_ip, ef_i, dd, x_temp, emod, nu, xlocal, thick, farea, slength,
+                  c1_ip, c2_ip, c3_ip, c1_i, c2_i, c3_i, minc, mcode);
+    return 0;
+}
+static void be_updatc(void *ctx, const orc_dims *d, double *x_temp, double *x_ip,
+                      double *xfr_temp, const double *dd, double *defllen_i,
+                      double *deffarea_i, double *defslen_i, const double *offset,
+                      const int *osflag, const double *auxpt, double *c1_i, double *c2_i,
+                      double *c3_i, const long *minc, const long *jcode)
+{
+    (void)ctx;
+    orc_updatc(d, x_temp, x_ip, xfr_temp, dd, defllen_i, deffarea_i, defslen_i, offset, osflag,
+               auxpt, c1_i, c2_i, c3_i, minc, jcode);
+}
+
+void orc_default_backend(orc_backend *b)
+{
+    b->ctx = NULL;
+    b->stiff_tr = be_stiff_tr;
+    b->forces_tr = be_forces_tr;
+    b->stiff_sh = be_stiff_sh;
+    b->forces_sh = be_forces_sh;
+    b->updatc = be_updatc;
+}
+
+/* ---- state of one analysis -------------------------------------------------
+ * Three generations of every per-element quantity, as in the reference:
+ * base (last converged), _i (current iterate), _ip (previous iterate). */
+typedef struct {
+    orc_model *m;
+    const orc_backend *be;
+    long neq, nj, ntri, nef, nlen, nsh, nfr;
+    double *ss, *sm, *ssd, *ssd_o;
+    double *d, *dp, *dpp, *d_temp, *dd, *ddq, *ddr, *f, *fp, *f_temp, *f_ip, *qtot, *r;
+    double *x, *x_temp, *x_ip, *xfr, *xfr_temp;
+    double *ef, *ef_i, *ef_ip;
+    double *c1, *c2, *c3, *c1_i, *c2_i, *c3_i, *c1_ip, *c2_ip, *c3_ip;
+    double *defllen, *defllen_i, *defllen_ip, *llen_ref;
+    double *deffarea, *deffarea_i, *deffarea_ip, *defslen, *defslen_i, *defslen_ip;
+    double *efFE, *efFE_i, *efFE_ip;
+    orc_path *out;
+} run_t;
+
+static double *dz(long n) { return calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
+#define CP(dst, src, n) memcpy((dst), (src), sizeof(double) * (size_t)(n))
+
+static void run_init(run_t *R, orc_model *m, const orc_backend *be, orc_path *out)
+{
+    memset(R, 0, sizeof *R);
+    const orc_dims *d = &m->dm;
+    R->m = m; R->be = be; R->out = out;
+    R->neq = d->NEQ; R->nj = d->NJ; R->nsh = d->NE_SH; R->nfr = d->NE_FR;
+    R->ntri = d->NE_TR + d->NE_FR * 3 + d->NE_SH * 3;
+    R->nef = d->NE_TR * 2 + d->NE_FR * 14 + d->NE_SH * 18;
+    R->nlen = d->NE_TR + d->NE_FR;
+    long n = R->neq;
+    R->ss = dz(m->lss); R->sm = dz(n); R->ssd = dz(n); R->ssd_o = dz(n);
+    R->d = dz(n); R->dp = dz(n); R->dpp = dz(n); R->d_temp = dz(n); R->dd = dz(n);
+    R->ddq = dz(n); R->ddr = dz(n); R->f = dz(n); R->fp = dz(n); R->f_temp = dz(n);
+    R->f_ip = dz(n); R->qtot = dz(n); R->r = dz(n);
+    R->x = dz(R->nj * 3); R->x_temp = dz(R->nj * 3); R->x_ip = dz(R->nj * 3);
+    R->xfr = dz(R->nfr * 6); R->xfr_temp = dz(R->nfr * 6);
+    R->ef = dz(R->nef); R->ef_i = dz(R->nef); R->ef_ip = dz(R->nef);
+    R->c1 = dz(R->ntri); R->c2 = dz(R->ntri); R->c3 = dz(R->ntri);
+    R->c1_i = dz(R->ntri); R->c2_i = dz(R->ntri); R->c3_i = dz(R->ntri);
+    R->c1_ip = dz(R->ntri); R->c2_ip = dz(R->ntri); R->c3_ip = dz(R->ntri);
+    R->defllen = dz(R->nlen); R->defllen_i = dz(R->nlen); R->defllen_ip = dz(R->nlen);
+    R->llen_ref = dz(R->nlen);
+    R->deffarea = dz(R->nsh); R->deffarea_i = dz(R->nsh); R->deffarea_ip = dz(R->nsh);
+    R->defslen = dz(R->nsh * 3); R->defslen_i = dz(R->nsh * 3); R->defslen_ip = dz(R->nsh * 3);
+    R->efFE = dz(R->nfr * 14); R->efFE_i = dz(R->nfr * 14); R->efFE_ip = dz(R->nfr * 14);
+
+    CP(R->x, m->x, R->nj * 3);
+    CP(R->x_temp, m->x, R->nj * 3);
+    CP(R->xfr, m->xfr, R->nfr * 6);
+    CP(R->xfr_temp, m->xfr, R->nfr * 6);
+    CP(R->c1, m->c1, R->ntri); CP(R->c2, m->c2, R->ntri); CP(R->c3, m->c3, R->ntri);
+    CP(R->llen_ref, m->llength, R->nlen);
+    CP(R->defllen, m->llength, R->nlen);
+    CP(R->deffarea, m->farea, R->nsh);
+    CP(R->defslen, m->slength, R->nsh * 3);
+}
+
+static void run_free(run_t *R)
+{
+    double **p = &R->ss, **e = &R->efFE_ip;
+    for (; p <= e; ++p) free(*p);
+}
+
+/* _ip <- _i for the per-element generations (main.c:2390-2411). */
+static void shift_ip(run_t *R, int with_ef)
+{
+    if (with_ef) CP(R->ef_ip, R->ef_i, R->nef);
+    CP(R->c1_ip, R->c1_i, R->ntri); CP(R->c2_ip, R->c2_i, R->ntri); CP(R->c3_ip, R->c3_i, R->ntri);
+    CP(R->defllen_ip, R->defllen_i, R->nlen);
+    CP(R->efFE_ip, R->efFE_i, R->nfr * 14);
+    CP(R->deffarea_ip, R->deffarea_i, R->nsh);
+    CP(R->defslen_ip, R->defslen_i, R->nsh * 3);
+}
+
+/* _i, _ip <- base (main.c:2583-2622). */
+static void reset_from_base(run_t *R)
+{
+    CP(R->x_temp, R->x, R->nj * 3);
+    CP(R->ef_i, R->ef, R->nef); CP(R->ef_ip, R->ef, R->nef);
+    CP(R->c1_i, R->c1, R->ntri); CP(R->c2_i, R->c2, R->ntri); CP(R->c3_i, R->c3, R->ntri);
+    CP(R->c1_ip, R->c1, R->ntri); CP(R->c2_ip, R->c2, R->ntri); CP(R->c3_ip, R->c3, R->ntri);
+    CP(R->defllen_i, R->defllen, R->nlen); CP(R->defllen_ip, R->defllen, R->nlen);
+    CP(R->xfr_temp, R->xfr, R->nfr * 6);
+    CP(R->efFE_i, R->efFE, R->nfr * 14); CP(R->efFE_ip, R->efFE, R->nfr * 14);
+    CP(R->deffarea_i, R->deffarea, R->nsh); CP(R->deffarea_ip, R->deffarea, R->nsh);
+    CP(R->defslen_i, R->defslen, R->nsh * 3); CP(R->defslen_ip, R->defslen, R->nsh * 3);
+}
+
+/* base <- _i (main.c:3055-3110). */
+static void commit(run_t *R)
+{
+    CP(R->x, R->x_temp, R->nj * 3);
+    CP(R->ef, R->ef_i, R->nef);
+    CP(R->c1, R->c1_i, R->ntri); CP(R->c2, R->c2_i, R->ntri); CP(R->c3, R->c3_i, R->ntri);
+    CP(R->defllen, R->defllen_i, R->nlen);
+    CP(R->xfr, R->xfr_temp, R->nfr * 6);
+    CP(R->efFE, R->efFE_i, R->nfr * 14);
+    CP(R->deffarea, R->deffarea_i, R->nsh);
+    CP(R->defslen, R->defslen_i, R->nsh * 3);
+}
+
+/* ss = 0; stiff_tr/fr/sh over the chosen state. */
+static void assemble(run_t *R, const double *x, const double *c1, const double *c2,
+                     const double *c3, const double *ef, const double *defllen,
+                     const double *deffarea, const double *defslen, const double *efFE)
+{
+    orc_model *m = R->m;
+    const orc_dims *d = &m->dm;
+    double t0 = now_s();
+    memset(R->ss, 0, sizeof(double) * (size_t)m->lss);
+    if (d->NE_TR > 0)
+        R->be->stiff_tr(R->be->ctx, d, R->ss, m->emod, m->carea, R->llen_ref, defllen, m->yield,
+                        c1, c2, c3, ef, m->maxa, m->mcode);
+    if (d->NE_FR > 0)
+        orc_stiff_fr(d, R->ss, m, R->llen_ref, defllen, c1, c2, c3, ef, efFE);
+    if (d->NE_SH > 0)
+        R->be->stiff_sh(R->be->ctx, d, R->ss, m->emod, m->nu, x, m->xlocal, m->thick, m->farea,
+                        deffarea, m->slength, defslen, m->yield, c1, c2, c3, ef, m->maxa,
+                        m->minc, m->mcode);
+    if (R->out) { R->out->t_tangent += now_s() - t0; }
+}
+
+/* updatc + forces_* into f_out (zeroed by the caller as in the reference). */
+static int update_and_forces(run_t *R, double *x_temp, double *xfr_temp, double *f_out,
+                             const double *dd)
+{
+    orc_model *m = R->m;
+    const orc_dims *d = &m->dm;
+    double t0 = now_s();
+    R->be->updatc(R->be->ctx, d, x_temp, R->x_ip, xfr_temp, dd, R->defllen_i, R->deffarea_i,
+                  R->defslen_i, m->offset, m->osflag, m->auxpt, R->c1_i, R->c2_i, R->c3_i,
+                  m->minc, m->jcode);
+    if (d->NE_TR > 0)
+        R->be->forces_tr(R->be->ctx, d, f_out, R->ef_i, m->emod, m->carea, R->llen_ref,
+                         R->defllen_i, m->yield, R->c1_i, R->c2_i, R->c3_i, m->mcode);
+    if (d->NE_FR > 0)
+        orc_forces_fr(d, f_out, R->ef_ip, R->ef_i, R->efFE_ip, R->efFE_i, dd, m, R->llen_ref,
+                      R->defllen_ip, R->c1_ip, R->c2_ip, R->c3_ip, R->c1_i, R->c2_i, R->c3_i);
+    int chk = 0;
+    if (d->NE_SH > 0)
+        chk = R->be->forces_sh(R->be->ctx, d, f_out, R->ef_ip, R->ef_i, dd, x_temp, R->x_ip,
+                               m->emod, m->nu, m->xlocal, m->thick, m->farea, R->deffarea_ip,
+                               m->slength, R->defslen_ip, m->yield, R->c1_ip, R->c2_ip,
+                               R->c3_ip, R->c1_i, R->c2_i, R->c3_i, m->minc, m->mcode,
+                               m->jcode);
+    if (R->out) R->out->t_forces += now_s() - t0;
+    return chk;
+}
+
+static int factor(run_t *R, int *det)
+{
+    double t0 = now_s();
+    int e = orc_skyfact(R->neq, R->m->maxa, R->ss, R->ssd, R->m->dm.ALGFLAG == 3, det);
+    if (R->out) { R->out->t_factor += now_s() - t0; R->out->n_factor++; }
+    return e;
+}
+
+static void backsolve(run_t *R, const double *rhs, double *x)
+{
+    double t0 = now_s();
+    CP(x, rhs, R->neq);
+    orc_skysolve(R->neq, R->m->maxa, R->ss, x);
+    if (R->out) { R->out->t_solve += now_s() - t0; R->out->n_solve++; }
+}
+
+static void record(run_t *R, double lpf, int itecnt, const double *d, const double *f)
+{
+    orc_path *o = R->out;
+    if (!o || o->nsteps >= o->max_steps) return;
+    if (o->lpf) o->lpf[o->nsteps] = lpf;
+    if (o->itecnt) o->itecnt[o->nsteps] = itecnt;
+    if (o->d) CP(o->d + (long)o->nsteps * R->neq, d, R->neq);
+    if (o->f && f) CP(o->f + (long)o->nsteps * R->neq, f, R->neq);
+    o->nsteps++;
+}
+
+static double vdot(long n, const double *a, const double *b)
+{
+    double s = 0;
+    for (long i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+
+#define FAIL(code, txt) do { if (R->out) { R->out->status = (code); snprintf(R->out->msg, sizeof R->out->msg, "%s", (txt)); } return (code); } while (0)
+
+/* ---- NR / modified NR with load control (main.c:1800-2157) --------------- */
+static int run_nr(run_t *R)
+{
+    orc_model *m = R->m;
+    long n = R->neq;
+    double lpf = m->lpf0, dlpf = m->dlpf0, dlpfp;
+    int inccnt = 0, solcnt = 0, subcnt = 0, itecnt = 0, convchk = 0, det = 0;
+    double intener1 = 0;
+    CP(R->defllen_i, R->defllen, R->nlen); CP(R->defllen_ip, R->defllen, R->nlen);
+    do {
+        if (lpf > m->lpfmax) lpf = m->lpfmax;
+        for (long i = 0; i < n; ++i) {
+            R->qtot[i] = m->q[i] * lpf;
+            R->fp[i] = R->f[i];
+            R->d_temp[i] = R->d[i];
+            R->f_temp[i] = R->f[i];
+        }
+        dlpfp = dlpf;
+        reset_from_base(R);
+        itecnt = 0;
+        int frc = 0;
+        do {
+            for (long i = 0; i < n; ++i) R->r[i] = R->qtot[i] - R->f_temp[i];
+            int full = (m->dm.ALGFLAG == 1 || itecnt == 0);
+            if (full)
+                assemble(R, R->x_temp, R->c1_ip, R->c2_ip, R->c3_ip, R->ef_ip, R->defllen_ip,
+                         R->deffarea_ip, R->defslen_ip, R->efFE_ip);
+            if (m->lss == 1) {
+                R->dd[0] = R->r[0] / R->ss[0];
+            } else {
+                /* solve() discards skyfact's status (solve.c:86-88, 535): a
+                   non-positive pivot leaves a partial factor that is used as is */
+                if (full) factor(R, &det);
+                backsolve(R, R->r, R->dd);
+            }
+            for (long i = 0; i < n; ++i) {
+                R->d_temp[i] += R->dd[i];
+                R->f_ip[i] = R->f_temp[i];
+                R->f_temp[i] = 0;
+            }
+            frc = update_and_forces(R, R->x_temp, R->xfr_temp, R->f_temp, R->dd);
+            CP(R->ef_ip, R->ef_i, R->nef);
+            if (itecnt == 0) {
+                intener1 = 0;
+                for (long i = 0; i < n; ++i) intener1 += R->dd[i] * (R->qtot[i] - R->fp[i]);
+            }
+            if (orc_test(n, R->d_temp, R->dd, R->f_temp, R->fp, R->qtot, R->f_ip, intener1,
+                         &convchk, m->toldisp, m->tolforc, m->tolener))
+                FAIL(1, "zero norm in convergence test");
+            if (getenv("ORC_VERBOSE"))
+                fprintf(stderr, "NR lpf=%g it=%d convchk=%d |dd|=%g\n", lpf, itecnt, convchk,
+                        sqrt(vdot(n, R->dd, R->dd)));
+            shift_ip(R, 0);
+            itecnt++;
+            if (R->out) R->out->n_newton++;
+        } while (convchk != 0 && frc == 0 && itecnt <= m->itemax);
+
+        if ((convchk != 0 || frc != 0) && subcnt <= m->submax) {
+            if (lpf == m->lpfmax) FAIL(1, "maximum load factor attempted without convergence");
+            if (dlpfp == m->dlpfmin) FAIL(1, "minimum load factor increment reached");
+            dlpf = dlpfp / 2;
+            if (dlpf < m->dlpfmin) dlpf = m->dlpfmin;
+            lpf = lpf - dlpfp + dlpf;
+            subcnt++;
+            solcnt = 0;
+        } else if (subcnt > m->submax) {
+            FAIL(1, "maximum number of subdivisions exceeded");
+        } else {
+            inccnt++;
+            CP(R->d, R->d_temp, n);
+            CP(R->f, R->f_temp, n);
+            commit(R);
+            record(R, lpf, itecnt, R->d, R->f);
+            if (R->out && R->out->nsteps >= R->out->max_steps) return 0;
+            solcnt++;
+            subcnt = 0;
+            if (solcnt >= m->solmin) {
+                dlpf *= 2;
+                if (dlpf > m->dlpfmax) dlpf = m->dlpfmax;
+                solcnt = 0;
+            }
+            lpf += dlpf;
+        }
+    } while (lpf <= m->lpfmax);
+    (void)inccnt;
+    return 0;
+}
+
+/* ---- modified spherical arc length (main.c:2158-3141) -------------------- */
+static int run_msal(run_t *R)
+{
+    orc_model *m = R->m;
+    long n = R->neq, k = m->dkdof;
+    int det = 0, convchk = 0, itecnt = 0, frc = 0;
+    double lpf, dlpf, intener1, lpfp = 0, lpfpp = 0;
+
+    CP(R->defllen_i, R->defllen, R->nlen); CP(R->defllen_ip, R->defllen, R->nlen);
+    CP(R->deffarea_i, R->deffarea, R->nsh); CP(R->deffarea_ip, R->deffarea, R->nsh);
+    CP(R->defslen_i, R->defslen, R->nsh * 3); CP(R->defslen_ip, R->defslen, R->nsh * 3);
+    CP(R->c1_i, R->c1, R->ntri); CP(R->c2_i, R->c2, R->ntri); CP(R->c3_i, R->c3, R->ntri);
+    CP(R->c1_ip, R->c1, R->ntri); CP(R->c2_ip, R->c2, R->ntri); CP(R->c3_ip, R->c3, R->ntri);
+
+    /* first increment: displacement control at the MSAL DOF (Bathe-Dvorkin) */
+    assemble(R, R->x, R->c1, R->c2, R->c3, R->ef, R->defllen, R->deffarea, R->defslen, R->efFE);
+    if (m->lss == 1) {
+        R->ddq[0] = m->q[0] / R->ss[0];
+        R->ssd[0] = R->ss[0];
+    } else {
+        factor(R, &det);
+        backsolve(R, m->q, R->ddq);
+    }
+    lpf = m->dk / R->ddq[k];
+    intener1 = 0;
+    for (long i = 0; i < n; ++i) {
+        R->d[i] = R->dd[i] = lpf * R->ddq[i];
+        intener1 += R->dd[i] * (lpf * m->q[i]);
+        R->ssd_o[i] = R->ssd[i];
+    }
+    memset(R->f, 0, sizeof(double) * (size_t)n);
+    frc = update_and_forces(R, R->x, R->xfr, R->f, R->dd);
+    itecnt = 1;
+    shift_ip(R, 1);
+    frc = 0;
+    do {
+        for (long i = 0; i < n; ++i) {
+            R->qtot[i] = m->q[i] * lpf;
+            R->r[i] = R->qtot[i] - R->f[i];
+        }
+        if (m->lss == 1) {
+            R->ddr[0] = R->r[0] / R->ss[0];
+            R->ssd[0] = R->ss[0];
+        } else {
+            backsolve(R, R->r, R->ddr);
+        }
+        dlpf = -R->ddr[k] / R->ddq[k];
+        lpf += dlpf;
+        for (long i = 0; i < n; ++i) {
+            R->dd[i] = R->ddr[i] + dlpf * R->ddq[i];
+            R->d[i] += R->dd[i];
+            R->f_ip[i] = R->f[i];
+            R->f[i] = 0;
+        }
+        frc = update_and_forces(R, R->x, R->xfr, R->f, R->dd);
+        if (orc_test(n, R->d, R->dd, R->f, R->fp, R->qtot, R->f_ip, intener1, &convchk,
+                     m->toldisp, m->tolforc, m->tolener))
+            FAIL(1, "zero norm in convergence test");
+        itecnt++;
+        if (R->out) R->out->n_newton++;
+        shift_ip(R, 1);
+    } while (convchk != 0 && frc == 0 && itecnt <= m->itemax);
+    if (convchk != 0 || frc != 0) FAIL(1, "initially prescribed displacement too large");
+    CP(R->ef, R->ef_i, R->nef);
+    CP(R->c1, R->c1_i, R->ntri); CP(R->c2, R->c2_i, R->ntri); CP(R->c3, R->c3_i, R->ntri);
+    CP(R->defllen, R->defllen_i, R->nlen);
+    CP(R->efFE, R->efFE_i, R->nfr * 14);
+    CP(R->deffarea, R->deffarea_i, R->nsh);
+    CP(R->defslen, R->defslen_i, R->nsh * 3);
+    record(R, lpf, itecnt, R->d, R->f);
+    if (R->out && R->out->nsteps >= R->out->max_steps) return 0;
+
+    double dkc = fabs(R->d[k]), lpfc = fabs(lpf);
+    for (long i = 0; i < n; ++i) {
+        R->dpp[i] = R->dp[i];
+        R->dp[i] = R->d[i];
+        R->fp[i] = R->f[i];
+    }
+    lpfpp = lpfp;
+    lpfp = lpf;
+    reset_from_base(R);
+
+    double dnorm = sqrt(vdot(n, R->dp, R->dp));
+    double dnormallow = m->alpha * sqrt(vdot(n, R->dp, R->dp));
+    double beta = sqrt(((double)m->iteopt) / (double)itecnt) * (dnormallow / dnorm);
+    double psi = 1;
+    for (long i = 0; i < n; ++i) {
+        double tq = fabs(R->ssd[i] / R->ssd_o[i]);
+        if (tq < psi) psi = tq;
+    }
+
+    int errchk2 = 0, subcnt = 0, imagcnt = 0, negcnt = 0;
+    double arc = 0, a = 0, b, c, lpf_temp;
+    while (lpfc <= m->lpfmax && dkc <= m->dkimax) {
+        CP(R->d_temp, R->d, n);
+        lpf_temp = lpf;
+        if (errchk2 == 0) {
+            double s = 0;
+            for (long i = 0; i < n; ++i) s += (R->dp[i] - R->dpp[i]) * (R->dp[i] - R->dpp[i]);
+            if (psi >= m->psi_thresh)
+                arc = beta * sqrt(s + (lpfp - lpfpp) * (lpfp - lpfpp));
+            else
+                arc = beta * sqrt(s);
+        } else {
+            reset_from_base(R);
+            errchk2 = 0;
+        }
+        /* tangent at the last converged state, factorised once per increment */
+        assemble(R, R->x, R->c1, R->c2, R->c3, R->ef, R->defllen, R->deffarea, R->defslen,
+                 R->efFE);
+        if (m->lss == 1) {
+            R->ddq[0] = m->q[0] / R->ss[0];
+            R->ssd[0] = R->ss[0];
+            det = R->ss[0] > 0 ? 0 : 1;
+        } else {
+            factor(R, &det);
+            backsolve(R, m->q, R->ddq);
+        }
+        if (psi >= m->psi_thresh)
+            a = vdot(n, m->q, m->q) + vdot(n, R->ddq, R->ddq);
+        else
+            a = vdot(n, R->ddq, R->ddq);
+        dlpf = (det == 0) ? arc * sqrt(1 / a) : -arc * sqrt(1 / a);
+        lpf_temp += dlpf;
+        intener1 = 0;
+        for (long i = 0; i < n; ++i) {
+            R->dd[i] = dlpf * R->ddq[i];
+            R->d_temp[i] += R->dd[i];
+            intener1 += R->dd[i] * (dlpf * m->q[i]);
+            R->f_temp[i] = 0;
+        }
+        update_and_forces(R, R->x_temp, R->xfr_temp, R->f_temp, R->dd);
+        shift_ip(R, 1);
+        itecnt = 1;
+        frc = 0;
+        if (R->out) R->out->n_newton++;
+        do {
+            for (long i = 0; i < n; ++i) {
+                R->qtot[i] = m->q[i] * lpf_temp;
+                R->r[i] = R->qtot[i] - R->f_temp[i];
+                R->f_ip[i] = R->f_temp[i];
+                R->f_temp[i] = 0;
+            }
+            if (m->lss == 1)
+                R->ddr[0] = R->r[0] / R->ss[0];
+            else
+                backsolve(R, R->r, R->ddr);
+            double qq = vdot(n, m->q, m->q);
+            if (psi >= m->psi_thresh) {
+                b = 2 * (vdot(n, R->d_temp, R->ddq) - vdot(n, R->dp, R->ddq) +
+                         vdot(n, R->ddr, R->ddq) + (lpf_temp - lpfp) * qq);
+                c = 2 * (vdot(n, R->d_temp, R->ddr) - vdot(n, R->dp, R->ddr) -
+                         vdot(n, R->d_temp, R->dp)) +
+                    vdot(n, R->d_temp, R->d_temp) + vdot(n, R->dp, R->dp) +
+                    vdot(n, R->ddr, R->ddr) + (lpf_temp - lpfp) * (lpf_temp - lpfp) * qq -
+                    arc * arc;
+            } else {
+                b = 2 * (vdot(n, R->d_temp, R->ddq) - vdot(n, R->dp, R->ddq) +
+                         vdot(n, R->ddr, R->ddq));
+                c = 2 * (vdot(n, R->d_temp, R->ddr) - vdot(n, R->dp, R->ddr) -
+                         vdot(n, R->d_temp, R->dp)) +
+                    vdot(n, R->d_temp, R->d_temp) + vdot(n, R->dp, R->dp) +
+                    vdot(n, R->ddr, R->ddr) - arc * arc;
+            }
+            errchk2 = orc_quad(n, a, b, c, R->d_temp, R->dp, R->ddr, R->ddq, R->dd, &dlpf,
+                               &lpf_temp);
+            if (getenv("ORC_VERBOSE"))
+                fprintf(stderr, "msal arc=%.4e psi=%.3f a=%.3e b=%.3e c=%.3e quad=%d lpf=%.6e it=%d\n",
+                        arc, psi, a, b, c, errchk2, lpf_temp, itecnt);
+            if (errchk2 == 0) {
+                frc = update_and_forces(R, R->x_temp, R->xfr_temp, R->f_temp, R->dd);
+                if (orc_test(n, R->d_temp, R->dd, R->f_temp, R->fp, R->qtot, R->f_ip, intener1,
+                             &convchk, m->toldisp, m->tolforc, m->tolener))
+                    FAIL(1, "zero norm in convergence test");
+                if (R->out) R->out->n_newton++;
+                if (convchk == 0) {
+                    dnorm = 0;
+                    for (long i = 0; i < n; ++i)
+                        dnorm += (R->d_temp[i] - R->dp[i]) * (R->d_temp[i] - R->dp[i]);
+                    dnorm = sqrt(dnorm);
+                    if (dnorm > 100 * dnormallow) {
+                        arc /= beta;
+                        beta = dnormallow / dnorm;
+                        arc *= beta;
+                        CP(R->f_temp, R->f, n);
+                        errchk2 = 1;
+                    }
+                } else if (frc != 0 && subcnt <= m->submax) {
+                    arc *= 0.5;
+                    CP(R->f_temp, R->f, n);
+                    subcnt++;
+                    errchk2 = 1;
+                } else {
+                    itecnt++;
+                    if (itecnt > m->itemax) {
+                        arc *= 0.5;
+                        CP(R->f_temp, R->f, n);
+                        subcnt++;
+                        errchk2 = 1;
+                    }
+                    shift_ip(R, 1);
+                }
+            } else if (errchk2 == 2) {
+                arc *= 0.5;
+                CP(R->f_temp, R->f, n);
+                imagcnt++;
+            } else if (errchk2 == 3) {
+                arc *= 0.5;
+                CP(R->f_temp, R->f, n);
+                negcnt++;
+            }
+        } while (convchk != 0 && errchk2 == 0 && subcnt <= m->submax &&
+                 imagcnt <= m->imagmax && negcnt <= m->negmax);
+
+        if (subcnt > m->submax || imagcnt > m->imagmax || negcnt > m->negmax)
+            FAIL(1, "maximum allowable number of subdivisions exceeded");
+        if (errchk2 == 0) {
+            beta = sqrt(((double)m->iteopt) / ((double)itecnt)) * (dnormallow / dnorm);
+            for (long i = 0; i < n; ++i) {
+                R->dpp[i] = R->dp[i];
+                R->d[i] = R->dp[i] = R->d_temp[i];
+                R->f[i] = R->fp[i] = R->f_temp[i];
+            }
+            lpfpp = lpfp;
+            lpf = lpfp = lpf_temp;
+            commit(R);
+            dkc = fabs(R->d[k]);
+            lpfc = fabs(lpf);
+            psi = 1;
+            for (long i = 0; i < n; ++i) {
+                double tq = fabs(R->ssd[i] / R->ssd_o[i]);
+                if (tq < psi) psi = tq;
+            }
+            subcnt = imagcnt = negcnt = 0;
+            record(R, lpf, itecnt, R->d, R->f);
+            if (R->out && R->out->nsteps >= R->out->max_steps) return 0;
+        }
+    }
+    return 0;
+}
+
+/* ---- nonlinear Newmark (main.c:3305-3960) ------------------------------ */
+static int run_newmark(run_t *R)
+{
+    if (R->out) { R->out->status = 2; snprintf(R->out->msg, sizeof R->out->msg, "Newmark not restated yet"); }
+    return 2;
+}
+
+int orc_run(orc_model *m, const orc_backend *b, orc_path *out)
+{
+    orc_backend own;
+    if (!b) { orc_default_backend(&own); b = &own; }
+    run_t R;
+    run_init(&R, m, b, out);
+    if (out) { out->nsteps = 0; out->status = 0; out->msg[0] = 0; }
+    int rc;
+    switch (m->dm.ALGFLAG) {
+    case 1: case 2: rc = run_nr(&R); break;
+    case 3: rc = run_msal(&R); break;
+    case 5: rc = run_newmark(&R); break;
+    default: rc = 2; if (out) { out->status = 2; snprintf(out->msg, sizeof out->msg, "unsupported ALGFLAG"); }
+    }
+    run_free(&R);
+    return rc;
+}
