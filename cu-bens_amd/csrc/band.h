@@ -17,7 +17,9 @@ struct BandLDLT {
     double* piv = nullptr;   // pivots D (padded to nt*T)
     double* xw = nullptr;    // work vectors (padded)
     double* yw = nullptr;
-    int* flag = nullptr;
+    int* flag = nullptr;       // error flag (bad band entry / wait bound hit)
+    int* flags = nullptr;      // per tile row completion epochs: forward [nt], backward [nt]
+    int epoch = 0;
     std::vector<double> piv_host;
 
     void setup(long n_, long bandwidth, hipStream_t s_);

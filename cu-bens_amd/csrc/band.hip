@@ -58,50 +58,59 @@ __global__ void k_band_pad(R* __restrict__ tiles, long n, long nt, int w) {
     tiles[(I * (w + 1) + w) * (long)TT + (i % T) * T + (i % T)] = R(1);
 }
 
-// LDL^T of the diagonal tile in LDS; writes L (unit, strictly lower part),
-// pivots D and the inverse of the unit lower factor.  at[] holds the tile
-// column-major (at[col][row]) and inv[] the inverse row-major, so that both
-// the column updates and the column-per-thread inversion read LDS without
-// bank conflicts; 2 x 32 KB of LDS in double.
+// LDL^T of the diagonal tile by one wavefront: lane i keeps row i of the tile
+// in registers (fully unrolled, static indices); column c is broadcast through
+// LDS at step c.  The inverse of the unit lower factor is formed the same way:
+// lane j builds column j by forward substitution, reading the rows of L as LDS
+// broadcasts.  Writes L (unit lower), pivots D and Linv = L^{-1}.
 template <class R>
-__global__ void __launch_bounds__(256) k_tile_diag(R* __restrict__ tiles, R* __restrict__ linv,
-                                                   double* __restrict__ piv, long Kt, int w) {
-    __shared__ R at[T][T];
-    __shared__ R inv[T][T];
+__global__ void __launch_bounds__(64) k_tile_diag(R* __restrict__ tiles, R* __restrict__ linv,
+                                                  double* __restrict__ piv, long Kt, int w) {
+    __shared__ R col[T];
+    __shared__ R Ls[T][T + 1];
+    const int lane = threadIdx.x;
     R* tile = tiles + (Kt * (w + 1) + w) * (long)TT;
-    for (int idx = threadIdx.x; idx < TT; idx += blockDim.x) at[idx / T][idx % T] = tile[idx];
-    __syncthreads();
+    R a[T];
+#pragma unroll
+    for (int j = 0; j < T; ++j) a[j] = tile[j * T + lane];   // (lane, j), column-major
+#pragma unroll
     for (int c = 0; c < T; ++c) {
-        const R dc = at[c][c];
-        const int m = T - 1 - c;
-        // a[i][j] -= a[i][c] a[j][c] / d_c for c < j <= i; consecutive threads
-        // walk down a column (at[j][i], i fastest)
-        for (int idx = threadIdx.x; idx < m * m; idx += blockDim.x) {
-            const int i = c + 1 + idx % m, j = c + 1 + idx / m;
-            if (j <= i) at[j][i] -= at[c][i] * (at[c][j] / dc);
+        if (lane >= c) col[lane] = a[c];
+        __syncthreads();
+        const R d = col[c];
+        if (lane > c) {
+            const R l = a[c] / d;
+#pragma unroll
+            for (int j = c + 1; j < T; ++j)
+                if (j <= lane) a[j] -= l * col[j];
+            a[c] = l;
         }
         __syncthreads();
-        for (int i = c + 1 + threadIdx.x; i < T; i += blockDim.x) at[c][i] = at[c][i] / dc;
-        __syncthreads();
     }
-    for (int idx = threadIdx.x; idx < TT; idx += blockDim.x) {
-        const int i = idx % T, j = idx / T;
-        tile[idx] = (i >= j) ? at[j][i] : R(0);
-    }
-    if (threadIdx.x < T) piv[Kt * T + threadIdx.x] = double(at[threadIdx.x][threadIdx.x]);
-    // inverse of the unit lower factor, column j by thread j (forward
-    // substitution; L entries are broadcast reads, inv[k][j] is the thread's own)
-    if (threadIdx.x < T) {
-        const int j = threadIdx.x;
-        for (int i = 0; i < T; ++i) {
-            R v = (i == j) ? R(1) : R(0);
-            for (int k = j; k < i; ++k) v -= at[k][i] * inv[k][j];
-            inv[i][j] = (i < j) ? R(0) : v;
-        }
+    // pivots and L (unit lower, zero upper) back to the tile and to LDS
+    piv[Kt * T + lane] = double(col[lane]);   // d_lane, written by lane at step c = lane
+#pragma unroll
+    for (int j = 0; j < T; ++j) {
+        R v = (j < lane) ? a[j] : (j == lane ? col[lane] : R(0));
+        tile[j * T + lane] = v;
+        Ls[lane][j] = (j < lane) ? a[j] : R(0);
     }
     __syncthreads();
+    // column lane of L^{-1}: v_i = [i == lane] - sum_{k<i} L[i][k] v_k
+    R v[T];
+#pragma unroll
+    for (int i = 0; i < T; ++i) {
+        R s0 = R(0), s1 = R(0);
+#pragma unroll
+        for (int k = 0; k < i; ++k) {
+            if (k & 1) s1 += Ls[i][k] * v[k];
+            else s0 += Ls[i][k] * v[k];
+        }
+        v[i] = ((i == lane) ? R(1) : R(0)) - (s0 + s1);
+    }
     R* out = linv + Kt * (long)TT;
-    for (int idx = threadIdx.x; idx < TT; idx += blockDim.x) out[idx] = inv[idx % T][idx / T];
+#pragma unroll
+    for (int i = 0; i < T; ++i) out[lane * T + i] = v[i];   // column lane, row i
 }
 
 // LDS tiles below are column-major like the global tiles: t[col][row].  The
@@ -258,6 +267,142 @@ __global__ void __launch_bounds__(256) k_bwd_step(const R* __restrict__ tiles, c
     if (tid < T) z[J * T + tid] -= part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
 }
 
+// ---- pipelined sweeps ---------------------------------------------------------
+// One workgroup per tile row, launched in tile order (forward) or reverse tile
+// order (backward).  A workgroup waits only on workgroups with a lower index,
+// which the hardware dispatched before it, so the oldest unfinished workgroup
+// is always resident and the sweep makes progress.  Completion of a tile row
+// is published by a per-row flag holding the sweep's epoch: the values are
+// stored with agent-scope atomics, fenced, then the flag is released; readers
+// acquire the flag and read the values with agent-scope atomics.  Each wait is
+// bounded: past spin_limit polls the workgroup sets *err and leaves, so every
+// wave exits whatever happens.  Contributions of earlier rows are applied
+// while the newest one is still pending; the newest off-diagonal tile and
+// Linv are prefetched into registers so the critical path per row is one flag
+// handoff and two 64x64 products from registers.
+__device__ inline bool wait_flag(int* flags, long J, int epoch, long spin_limit, int* err, int* ok_s) {
+    if (threadIdx.x == 0) {
+        // poll without cache maintenance; one acquire once the flag is seen
+        long n = 0;
+        int f;
+        while ((f = __hip_atomic_load(&flags[J], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) != epoch &&
+               ++n < spin_limit)
+            __builtin_amdgcn_s_sleep(2);
+        __atomic_thread_fence(__ATOMIC_ACQUIRE);
+        *ok_s = (f == epoch);
+        if (!*ok_s) atomicExch(err, 1);
+    }
+    __syncthreads();
+    return *ok_s != 0;
+}
+
+__device__ inline void publish(double* v, long I, double val, int* flags, int epoch) {
+    if (threadIdx.x < T) __hip_atomic_store(&v[I * T + threadIdx.x], val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(&flags[I], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// y_I = Linv_I (r_I - sum_{J<I} L_IJ y_J)
+template <class R>
+__global__ void __launch_bounds__(256) k_fwd_pipe(const R* __restrict__ tiles, const R* __restrict__ linv,
+                                                  const double* __restrict__ r, double* y, int* flags, int epoch,
+                                                  long nt, int w, long spin_limit, int* err) {
+    __shared__ double vin[T];
+    __shared__ double part[4][T];
+    __shared__ int ok_s;
+    const long I = blockIdx.x;
+    const int tid = threadIdx.x, row = tid % T, q = tid / T, k0 = 16 * q;
+    // prefetch: Linv_I and the newest off-diagonal tile (I, I-1), rows of this thread
+    R lv[16], ln[16];
+    const R* lvp = linv + I * (long)TT;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) lv[k] = lvp[(k0 + k) * T + row];
+    const bool has_prev = I > 0;
+    if (has_prev) {
+        const R* lnp = tiles + (I * (w + 1) + (w - 1)) * (long)TT;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) ln[k] = lnp[(k0 + k) * T + row];
+    }
+    double acc = (q == 0) ? r[I * T + row] : 0.0;
+    const long J0 = I - w > 0 ? I - w : 0;
+    for (long J = J0; J < I; ++J) {
+        if (!wait_flag(flags, J, epoch, spin_limit, err, &ok_s)) return;
+        if (tid < T) vin[tid] = __hip_atomic_load(&y[J * T + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        if (J == I - 1) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) acc -= double(ln[k]) * vin[k0 + k];
+        } else {
+            const R* l = tiles + (I * (w + 1) + (J - I + w)) * (long)TT;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) acc -= double(l[(k0 + k) * T + row]) * vin[k0 + k];
+        }
+        __syncthreads();
+    }
+    part[q][row] = acc;
+    __syncthreads();
+    if (tid < T) vin[tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s += double(lv[k]) * vin[k0 + k];
+    part[q][row] = s;
+    __syncthreads();
+    double yi = (tid < T) ? part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid] : 0.0;
+    publish(y, I, yi, flags, epoch);
+}
+
+// x_I = Linv_I^T (y_I / D_I - sum_{J>I} L_JI^T x_J); workgroup g handles I = nt-1-g
+template <class R>
+__global__ void __launch_bounds__(256) k_bwd_pipe(const R* __restrict__ tiles, const R* __restrict__ linv,
+                                                  const double* __restrict__ y, const double* __restrict__ piv,
+                                                  double* x, int* flags, int epoch, long nt, int w,
+                                                  long spin_limit, int* err) {
+    __shared__ double vin[T];
+    __shared__ double part[4][T];
+    __shared__ int ok_s;
+    const long I = nt - 1 - blockIdx.x;
+    const int tid = threadIdx.x, col = tid % T, q = tid / T, k0 = 16 * q;
+    R lv[16], ln[16];
+    const R* lvp = linv + I * (long)TT;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) lv[k] = lvp[col * T + k0 + k];   // Linv[k][col]
+    const bool has_next = I + 1 < nt;
+    if (has_next) {
+        const R* lnp = tiles + ((I + 1) * (w + 1) + (w - 1)) * (long)TT;   // tile (I+1, I)
+#pragma unroll
+        for (int k = 0; k < 16; ++k) ln[k] = lnp[col * T + k0 + k];
+    }
+    double acc = (q == 0) ? y[I * T + col] / piv[I * T + col] : 0.0;
+    const long J1 = (I + w < nt - 1) ? I + w : nt - 1;
+    for (long J = J1; J > I; --J) {
+        if (!wait_flag(flags, J, epoch, spin_limit, err, &ok_s)) return;
+        if (tid < T) vin[tid] = __hip_atomic_load(&x[J * T + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        if (J == I + 1) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) acc -= double(ln[k]) * vin[k0 + k];
+        } else {
+            const R* l = tiles + (J * (w + 1) + (I - J + w)) * (long)TT;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) acc -= double(l[col * T + k0 + k]) * vin[k0 + k];
+        }
+        __syncthreads();
+    }
+    part[q][col] = acc;
+    __syncthreads();
+    if (tid < T) vin[tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s += double(lv[k]) * vin[k0 + k];
+    part[q][col] = s;
+    __syncthreads();
+    double xi = (tid < T) ? part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid] : 0.0;
+    publish(x, I, xi, flags, epoch);
+}
+
 __global__ void k_scale_piv(double* __restrict__ x, const double* __restrict__ piv, long n) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i < n) x[i] /= piv[i];
@@ -284,15 +429,20 @@ void BandLDLT<R>::setup(long n_, long bandwidth, hipStream_t s_) {
     xw = dalloc<double>(size_t(nt) * T);
     yw = dalloc<double>(size_t(nt) * T);
     flag = dalloc<int>(1);
+    flags = dalloc<int>(size_t(2 * nt));
+    CB_CHECK(hipMemsetAsync(flags, 0, sizeof(int) * size_t(2 * nt), s));
+    CB_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
+    epoch = 0;
     piv_host.resize(size_t(n));
 }
 
 template <class R>
 void BandLDLT<R>::release() {
-    void* ps[] = {tiles, linv, piv, xw, yw, flag};
+    void* ps[] = {tiles, linv, piv, xw, yw, flag, flags};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     tiles = nullptr; linv = nullptr; piv = nullptr; xw = nullptr; yw = nullptr; flag = nullptr;
+    flags = nullptr;
 }
 
 template <class R>
@@ -305,7 +455,7 @@ int BandLDLT<R>::factor(const Dev& d) {
         hipLaunchKernelGGL((k_band_pad<R>), dim3(1), dim3(T), 0, s, tiles, n, nt, w);
     CB_CHECK(hipGetLastError());
     for (long Kt = 0; Kt < nt; ++Kt) {
-        hipLaunchKernelGGL((k_tile_diag<R>), dim3(1), dim3(256), 0, s, tiles, linv, piv, Kt, w);
+        hipLaunchKernelGGL((k_tile_diag<R>), dim3(1), dim3(64), 0, s, tiles, linv, piv, Kt, w);
         long m = std::min<long>(w, nt - 1 - Kt);
         if (m > 0) {
             hipLaunchKernelGGL((k_tile_panel<R>), dim3(int(m)), dim3(256), 0, s, tiles, linv, piv, Kt, w);
@@ -327,17 +477,18 @@ int BandLDLT<R>::factor(const Dev& d) {
 template <class R>
 void BandLDLT<R>::solve(const double* rhs, double* out) {
     hipLaunchKernelGGL(k_copy_pad, dim3(int((nt * T + 255) / 256)), dim3(256), 0, s, xw, rhs, n, nt * T);
-    for (long I = 0; I < nt; ++I) {
-        int m = int(std::min<long>(w, nt - 1 - I));
-        hipLaunchKernelGGL((k_fwd_step<R>), dim3(m + 1), dim3(256), 0, s, tiles, linv, xw, yw, I, w);
-    }
-    hipLaunchKernelGGL(k_scale_piv, dim3(int((nt * T + 255) / 256)), dim3(256), 0, s, yw, piv, nt * T);
-    for (long I = nt - 1; I >= 0; --I) {
-        int m = int(std::min<long>(w, I));
-        hipLaunchKernelGGL((k_bwd_step<R>), dim3(m + 1), dim3(256), 0, s, tiles, linv, yw, xw, I, w);
-    }
+    const long spin_limit = 1L << 24;
+    epoch += 1;
+    hipLaunchKernelGGL((k_fwd_pipe<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, xw, yw, flags, epoch, nt, w,
+                       spin_limit, flag);
+    hipLaunchKernelGGL((k_bwd_pipe<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, yw, piv, xw, flags + nt,
+                       epoch, nt, w, spin_limit, flag);
     CB_CHECK(hipGetLastError());
     CB_CHECK(hipMemcpyAsync(out, xw, sizeof(double) * size_t(n), hipMemcpyDeviceToDevice, s));
+    int bad = 0;
+    CB_CHECK(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    CB_CHECK(hipStreamSynchronize(s));
+    if (bad) throw DeviceError("band solve: a tile-row wait exceeded its bound");
 }
 
 template struct BandLDLT<double>;
