@@ -43,16 +43,82 @@ def workload_file(n: int) -> str:
     return path
 
 
-def run_path(path: str, solver: int, warmup: int, steps: int, tol: float) -> dict:
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+TILE = 64
+
+
+def kernel_model(info) -> dict:
+    """Algorithmic work of one launch of each band kernel family at this model.
+
+    nt tile rows of 64, w sub-diagonal tiles (w = ceil(bandwidth/64)):
+      diag   : LDL^T of a 64x64 tile (2/3 T^3) + inverse of its unit factor (T^3/3)
+      panel  : m tile products of 2 T^3, m = w (interior)
+      update : m(m+1)/2 tile products of 2 T^3
+      sweeps : every factor tile and every Linv tile read once (8 B entries)
+    """
+    nt = (info.neq + TILE - 1) // TILE
+    w = min((info.bandwidth + TILE - 1) // TILE, nt - 1)
+    t3 = TILE ** 3
+    sweep_bytes = 8 * TILE * TILE * (nt * (w + 1) + nt) + 3 * 8 * nt * TILE
+    return {
+        "band_diag": ("mfma", float(t3), "TFLOP/s"),
+        "band_panel": ("mfma", 2.0 * t3 * w, "TFLOP/s"),
+        "band_update": ("mfma", 2.0 * t3 * w * (w + 1) / 2, "TFLOP/s"),
+        "band_fwd_sweep": ("hbm", float(sweep_bytes), "GB/s"),
+        "band_bwd_sweep": ("hbm", float(sweep_bytes), "GB/s"),
+    }
+
+
+def roofline(info, ktimes: dict) -> dict:
+    name = max(ktimes, key=lambda k: ktimes[k][0])
+    ms, launches = ktimes[name]
+    bound, work, unit = kernel_model(info)[name]
+    avg_s = ms / 1000.0 / max(launches, 1)
+    if bound == "hbm":
+        achieved, peak = work / avg_s / 1e9, PEAK_HBM_GBS
+    else:
+        achieved, peak = work / avg_s / 1e12, PEAK_FP64_TFLOPS
+    return {"kernel": name, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": achieved / peak, "traffic": None, "avg_launch_us": 1e6 * avg_s, "launches": launches,
+            "share_of_kernel_time": ms / sum(v[0] for v in ktimes.values())}
+
+
+def cpu_baseline(path: str, newton: int, increments: int, budget: float) -> dict:
+    """Single-core timing of the oracle's serial skyline path on a bounded sample
+    of the same workload, projected onto the GPU window's work."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    om = O.OracleModel(path)
+    smp = om.time_sample(budget)
+    per_incr = smp["t_tangent"] + smp["t_factor_est"] + smp["t_solve"]
+    per_newton = smp["t_solve"] + smp["t_forces"]
+    t = increments * per_incr + newton * per_newton
+    return {"value": newton / t, "unit": "Newton iters/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle (C restatement of the serial skyline path) on 1 core, same model: one tangent "
+                       f"{smp['t_tangent']:.2f}s, one internal-force pass {smp['t_forces']:.2f}s, one skyline "
+                       f"back-substitution {smp['t_solve']:.2f}s, skyline LDL^T of the first {smp['cols_done']} "
+                       f"columns {smp['t_factor_part']:.1f}s = {100 * smp['factor_fraction']:.2f}% of its "
+                       f"sum(kht^2) work -> {smp['t_factor_est']:.0f}s per factorization; projected onto the "
+                       f"GPU window ({increments} increments, {newton} Newton iterations)")}
+
+
+def run_path(path: str, solver: int, warmup: int, steps: int, tol: float, timing: int = 0) -> dict:
     import cubens
     t0 = time.time()
     with cubens.GpuModel(path) as g:
         t_open = time.time() - t0
         opt = cubens.options(solver, krylov_tol=tol, restart=30, maxit=400,
                              judge=0 if solver == cubens.SOLVER_BAND64 else 1,
-                             max_steps=warmup + steps, keep_d=1)
+                             max_steps=warmup + steps, keep_d=1, kernel_timing=timing)
         r = g.run(opt)
         info = g.info
+        kt = None
+        if timing and r["nsteps"] >= warmup + steps:
+            prec = 64 if solver == cubens.SOLVER_BAND64 else 32
+            end = g.kernel_times(prec, warmup + steps - 1)
+            start = g.kernel_times(prec, warmup - 1) if warmup > 0 else {k: (0.0, 0) for k in end}
+            kt = {k: (end[k][0] - start[k][0], end[k][1] - start[k][1]) for k in end}
     n = r["nsteps"]
     if n < warmup + steps:
         raise RuntimeError(f"run stopped after {n} steps: {r['msg']}")
@@ -61,16 +127,18 @@ def run_path(path: str, solver: int, warmup: int, steps: int, tol: float) -> dic
     t = (tw[-1] - tj[-1]) - ((tw[a] - tj[a]) if a >= 0 else 0.0)
     newton = int(nw[-1] - (nw[a] if a >= 0 else 0))
     return {"rate": newton / t, "seconds": t, "newton": newton, "open_s": t_open, "stats": r,
-            "neq": info.neq, "bandwidth": info.bandwidth, "lpf": r["lpf"], "d": r["d"]}
+            "neq": info.neq, "bandwidth": info.bandwidth, "lpf": r["lpf"], "d": r["d"], "info": info,
+            "ktimes": kt}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256, help="panel mesh (256 = the 396k-DOF workload)")
-    ap.add_argument("--paths", default="fp64,mixed")
+    ap.add_argument("--paths", default="fp64", help="fp64 (headline) and/or mixed")
+    ap.add_argument("--cpu-budget", type=float, default=8.0, help="seconds of partial CPU factorization")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -84,22 +152,29 @@ def main():
     for name in args.paths.split(","):
         solver = {"fp64": cubens.SOLVER_BAND64, "mixed": cubens.SOLVER_BAND32_FGMRES}[name]
         log(f"[bench] path {name}: {args.warmup} warmup + {args.steps} timed increments")
-        res = run_path(path, solver, args.warmup, args.steps, args.tol)
+        res = run_path(path, solver, args.warmup, args.steps, args.tol, timing=31)
         st = res["stats"]
         log(f"[bench] {name}: {res['rate']:.2f} Newton it/s ({res['newton']} its in {res['seconds']:.2f}s); "
             f"tangent {st['t_tangent']:.2f}s factor {st['t_factor']:.2f}s solve {st['t_solve']:.2f}s "
             f"forces {st['t_forces']:.2f}s judge {st['t_judge']:.2f}s krylov {st['krylov_iters']} "
             f"fallbacks {st['krylov_fallbacks']} worst relres {st['worst_relres']:.2e}")
         results[name] = res
-    head = results.get("mixed") or results["fp64"]
     plain = results.get("fp64")
+    head = plain if plain is not None else results["mixed"]
     extra = {}
     if plain is not None and "mixed" in results:
         m = results["mixed"]
-        extra["plain_fp64_value"] = plain["rate"]
-        extra["ratio_vs_plain_fp64"] = m["rate"] / plain["rate"]
+        extra["mixed_fp32_factor_value"] = m["rate"]
+        extra["mixed_vs_plain_fp64_ratio"] = m["rate"] / plain["rate"]
         extra["mixed_vs_fp64_rel_disp_err"] = float(
             np.abs(m["d"] - plain["d"]).max() / np.abs(plain["d"]).max())
+    roof = roofline(head["info"], head["ktimes"])
+    log(f"[bench] kernel times (ms, launches): {head['ktimes']}")
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline: oracle sample on 1 core")
+        cpu = cpu_baseline(path, head["newton"], args.steps, args.cpu_budget)
+        log(f"[bench] cpu: {cpu['value']:.4f} Newton it/s; {cpu['sample']}")
     out = {
         "metric": METRIC,
         "value": head["rate"],
@@ -115,8 +190,10 @@ def main():
         "data": "synthetic (seeded SplitMix64 thickness field and imperfection)",
         "config": {"workload": f"cylindrical shell panel {args.n}x{args.n} MSAL, "
                                f"{2 * args.n * args.n} CST+DKT triangles, NEQ {head['neq']}",
-                   "path": "mixed" if head is not plain else "fp64",
+                   "path": "fp64 band LDL^T" if head is plain else "fp32-factor FGMRES",
                    "increments_timed": args.steps},
+        "roofline": roof,
+        "cpu_baseline": cpu,
         **extra,
     }
     print(json.dumps(out))

@@ -7,8 +7,26 @@
 
 namespace cb {
 
+// Kernel families of the band solver whose launches can be timed with HIP events.
+enum { KT_DIAG = 0, KT_PANEL = 1, KT_UPDATE = 2, KT_FWD = 3, KT_BWD = 4, KT_N = 5 };
+
+struct KernelClock {
+    int mask = 0;                        // bit f: time family f
+    double ms[KT_N] = {};
+    long launches[KT_N] = {};
+    std::vector<hipEvent_t> ev;
+    std::vector<int> fam;                // family of each recorded pair
+    size_t used = 0;
+    bool on(int f) const { return (mask >> f) & 1; }
+    void begin(int f, hipStream_t s);
+    void end(hipStream_t s);
+    void collect();                      // after a stream sync
+    void release();
+};
+
 template <class R>
 struct BandLDLT {
+    KernelClock clock;
     long n = 0, nt = 0;
     int w = 0;
     hipStream_t s = nullptr;

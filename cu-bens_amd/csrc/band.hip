@@ -415,6 +415,36 @@ __global__ void k_copy_pad(double* __restrict__ dst, const double* __restrict__ 
 
 }  // namespace
 
+void KernelClock::begin(int f, hipStream_t s) {
+    if (used + 2 > ev.size()) {
+        size_t old = ev.size();
+        ev.resize(old + 4096);
+        for (size_t i = old; i < ev.size(); ++i) CB_CHECK(hipEventCreate(&ev[i]));
+    }
+    fam.push_back(f);
+    CB_CHECK(hipEventRecord(ev[used++], s));
+}
+
+void KernelClock::end(hipStream_t s) { CB_CHECK(hipEventRecord(ev[used++], s)); }
+
+void KernelClock::collect() {
+    for (size_t i = 0; i < fam.size(); ++i) {
+        float t = 0;
+        CB_CHECK(hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]));
+        ms[fam[i]] += t;
+        launches[fam[i]] += 1;
+    }
+    fam.clear();
+    used = 0;
+}
+
+void KernelClock::release() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    ev.clear();
+    fam.clear();
+    used = 0;
+}
+
 template <class R>
 void BandLDLT<R>::setup(long n_, long bandwidth, hipStream_t s_) {
     release();
@@ -443,6 +473,7 @@ void BandLDLT<R>::release() {
         if (p) (void)hipFree(p);
     tiles = nullptr; linv = nullptr; piv = nullptr; xw = nullptr; yw = nullptr; flag = nullptr;
     flags = nullptr;
+    clock.release();
 }
 
 template <class R>
@@ -455,12 +486,18 @@ int BandLDLT<R>::factor(const Dev& d) {
         hipLaunchKernelGGL((k_band_pad<R>), dim3(1), dim3(T), 0, s, tiles, n, nt, w);
     CB_CHECK(hipGetLastError());
     for (long Kt = 0; Kt < nt; ++Kt) {
+        if (clock.on(KT_DIAG)) clock.begin(KT_DIAG, s);
         hipLaunchKernelGGL((k_tile_diag<R>), dim3(1), dim3(64), 0, s, tiles, linv, piv, Kt, w);
+        if (clock.on(KT_DIAG)) clock.end(s);
         long m = std::min<long>(w, nt - 1 - Kt);
         if (m > 0) {
+            if (clock.on(KT_PANEL)) clock.begin(KT_PANEL, s);
             hipLaunchKernelGGL((k_tile_panel<R>), dim3(int(m)), dim3(256), 0, s, tiles, linv, piv, Kt, w);
+            if (clock.on(KT_PANEL)) clock.end(s);
+            if (clock.on(KT_UPDATE)) clock.begin(KT_UPDATE, s);
             hipLaunchKernelGGL((k_tile_update<R>), dim3(int(m * (m + 1) / 2)), dim3(256), 0, s, tiles, piv,
                                Kt, w);
+            if (clock.on(KT_UPDATE)) clock.end(s);
         }
     }
     CB_CHECK(hipGetLastError());
@@ -468,6 +505,7 @@ int BandLDLT<R>::factor(const Dev& d) {
     CB_CHECK(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, s));
     CB_CHECK(hipMemcpyAsync(piv_host.data(), piv, sizeof(double) * size_t(n), hipMemcpyDeviceToHost, s));
     CB_CHECK(hipStreamSynchronize(s));
+    clock.collect();
     if (bad) return 2;
     for (long i = 0; i < n; ++i)
         if (piv_host[i] == 0.0 || !std::isfinite(piv_host[i])) return 1;
@@ -479,15 +517,20 @@ void BandLDLT<R>::solve(const double* rhs, double* out) {
     hipLaunchKernelGGL(k_copy_pad, dim3(int((nt * T + 255) / 256)), dim3(256), 0, s, xw, rhs, n, nt * T);
     const long spin_limit = 1L << 24;
     epoch += 1;
+    if (clock.on(KT_FWD)) clock.begin(KT_FWD, s);
     hipLaunchKernelGGL((k_fwd_pipe<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, xw, yw, flags, epoch, nt, w,
                        spin_limit, flag);
+    if (clock.on(KT_FWD)) clock.end(s);
+    if (clock.on(KT_BWD)) clock.begin(KT_BWD, s);
     hipLaunchKernelGGL((k_bwd_pipe<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, yw, piv, xw, flags + nt,
                        epoch, nt, w, spin_limit, flag);
+    if (clock.on(KT_BWD)) clock.end(s);
     CB_CHECK(hipGetLastError());
     CB_CHECK(hipMemcpyAsync(out, xw, sizeof(double) * size_t(n), hipMemcpyDeviceToDevice, s));
     int bad = 0;
     CB_CHECK(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, s));
     CB_CHECK(hipStreamSynchronize(s));
+    clock.collect();
     if (bad) throw DeviceError("band solve: a tile-row wait exceeded its bound");
 }
 

@@ -28,6 +28,8 @@ struct cb_model {
     std::vector<double> path_lpf, path_d, step_t, step_judge;
     std::vector<int> path_ite;
     std::vector<long> step_newton;
+    std::vector<std::vector<double>> step_kt_ms;
+    std::vector<std::vector<long>> step_kt_n;
 };
 
 namespace {
@@ -41,6 +43,7 @@ cb_options default_options() {
     o.judge = 1;
     o.max_steps = 1 << 30;
     o.keep_d = 1;
+    o.kernel_timing = 0;
     return o;
 }
 
@@ -53,12 +56,14 @@ cb::RunOptions to_run(const cb_options& o) {
     r.judge = o.judge;
     r.max_steps = o.max_steps;
     r.keep_d = o.keep_d;
+    r.kernel_timing = o.kernel_timing;
     return r;
 }
 
 bool same_opts(const cb_options& a, const cb_options& b) {
     return a.solver == b.solver && a.krylov_tol == b.krylov_tol && a.restart == b.restart &&
-           a.maxit == b.maxit && a.judge == b.judge && a.max_steps == b.max_steps && a.keep_d == b.keep_d;
+           a.maxit == b.maxit && a.judge == b.judge && a.max_steps == b.max_steps && a.keep_d == b.keep_d &&
+           a.kernel_timing == b.kernel_timing;
 }
 
 cb::Analysis& analysis(cb_model* h, const cb_options* o) {
@@ -293,6 +298,8 @@ int cb_run(cb_model* h, const cb_options* opt, cb_stats* st) {
     h->step_t = an.step_t;
     h->step_judge = an.step_judge;
     h->step_newton = an.step_newton;
+    h->step_kt_ms = an.step_kt_ms;
+    h->step_kt_n = an.step_kt_n;
     if (st) {
         st->t_total = rs.t_total; st->t_tangent = rs.t_tangent; st->t_factor = rs.t_factor;
         st->t_solve = rs.t_solve; st->t_forces = rs.t_forces; st->t_judge = rs.t_judge;
@@ -323,6 +330,25 @@ int cb_get_step_stats(const cb_model* h, double* t_wall, double* t_judge, long* 
     if (t_wall) std::memcpy(t_wall, h->step_t.data(), sizeof(double) * n);
     if (t_judge) std::memcpy(t_judge, h->step_judge.data(), sizeof(double) * n);
     if (newton) std::memcpy(newton, h->step_newton.data(), sizeof(long) * n);
+    return 0;
+}
+
+int cb_get_kernel_times(const cb_model* h, int precision, int step, double* ms, long* launches) {
+    if (!h) return fail(-1, "null handle");
+    if (!h->an) return fail(-1, "no analysis has run");
+    if (step >= 0) {
+        if (size_t(step) >= h->step_kt_ms.size()) return fail(-1, "no such step");
+        for (int f = 0; f < cb::KT_N; ++f) {
+            if (ms) ms[f] = h->step_kt_ms[size_t(step)][size_t(f)];
+            if (launches) launches[f] = h->step_kt_n[size_t(step)][size_t(f)];
+        }
+        return 0;
+    }
+    const cb::KernelClock& c = precision == 32 ? h->an->band32().clock : h->an->band64().clock;
+    for (int f = 0; f < cb::KT_N; ++f) {
+        if (ms) ms[f] = c.ms[f];
+        if (launches) launches[f] = c.launches[f];
+    }
     return 0;
 }
 

@@ -43,6 +43,8 @@ Analysis::Analysis(Model& m, Dev& d, const RunOptions& o) : m_(m), d_(d), o_(o) 
     const bool need64 = o.solver == SOLVER_BAND64 || o.judge || o.solver != SOLVER_BAND32_FGMRES;
     if (need64) b64_.setup(n, m.bandwidth, d.s);
     if (o.solver == SOLVER_BAND32_FGMRES) b32_.setup(n, m.bandwidth, d.s);
+    b64_.clock.mask = o.kernel_timing;
+    b32_.clock.mask = o.kernel_timing;
     if (o.solver != SOLVER_BAND64) gm_.setup(d.nslot, o.restart);
     CB_CHECK(hipStreamSynchronize(d.s));
 }
@@ -69,6 +71,9 @@ void Analysis::record(double lam, int ite, const double* dvec) {
     step_t.push_back(now_s() - t_start_);
     step_judge.push_back(st_->t_judge);
     step_newton.push_back(st_->newton);
+    const KernelClock& kc = (o_.solver == SOLVER_BAND32_FGMRES) ? b32_.clock : b64_.clock;
+    step_kt_ms.emplace_back(kc.ms, kc.ms + KT_N);
+    step_kt_n.emplace_back(kc.launches, kc.launches + KT_N);
     if (o_.keep_d) {
         size_t off = dpath.size();
         dpath.resize(off + size_t(m_.neq));
@@ -211,6 +216,8 @@ int Analysis::run(RunStats& st) {
     step_t.clear();
     step_judge.clear();
     step_newton.clear();
+    step_kt_ms.clear();
+    step_kt_n.clear();
     dev_reset_state(d_, m_);
     for (double* v : {q, dv, dp, dpp, dt, dd, ddq, ddr, f, fp, ft, fip, r})
         CB_CHECK(hipMemsetAsync(v, 0, sizeof(double) * size_t(m_.neq), d_.s));

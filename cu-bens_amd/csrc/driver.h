@@ -29,6 +29,7 @@ struct RunOptions {
     int judge = 1;          // exact FP64 pivots for det/psi when the solver has none
     int max_steps = 1 << 30;
     int keep_d = 1;         // copy d to the host at every converged step
+    int kernel_timing = 0;  // KT_* bitmask of band kernels timed with HIP events
 };
 
 struct RunStats {
@@ -60,6 +61,9 @@ public:
     // Newton iterations so far
     std::vector<double> step_t, step_judge;
     std::vector<long> step_newton;
+    // cumulative band-kernel clocks at each converged step: [step][family]
+    std::vector<std::vector<double>> step_kt_ms;
+    std::vector<std::vector<long>> step_kt_n;
     PrecHook learned;
     // one-shot operations used by the C ABI
     int tangent_and_factor(int gx, int g, int* det);

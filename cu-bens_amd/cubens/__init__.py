@@ -32,7 +32,7 @@ class Info(C.Structure):
 class Options(C.Structure):
     _fields_ = [("solver", C.c_int), ("krylov_tol", C.c_double), ("restart", C.c_int),
                 ("maxit", C.c_int), ("judge", C.c_int), ("max_steps", C.c_int),
-                ("keep_d", C.c_int)]
+                ("keep_d", C.c_int), ("kernel_timing", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -47,9 +47,14 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+KT_DIAG, KT_PANEL, KT_UPDATE, KT_FWD, KT_BWD = range(5)
+KT_NAMES = ("band_diag", "band_panel", "band_update", "band_fwd_sweep", "band_bwd_sweep")
+
+
 def options(solver: int = SOLVER_BAND64, krylov_tol: float = 1e-8, restart: int = 30,
-            maxit: int = 20000, judge: int = 1, max_steps: int = 1 << 30, keep_d: int = 1) -> Options:
-    return Options(solver, krylov_tol, restart, maxit, judge, max_steps, keep_d)
+            maxit: int = 20000, judge: int = 1, max_steps: int = 1 << 30, keep_d: int = 1,
+            kernel_timing: int = 0) -> Options:
+    return Options(solver, krylov_tol, restart, maxit, judge, max_steps, keep_d, kernel_timing)
 
 
 _lib = None
@@ -78,6 +83,7 @@ ABI = {
     "cb_run": (C.c_int, [C.c_void_p, C.POINTER(Options), C.POINTER(Stats)]),
     "cb_get_path": (C.c_int, [C.c_void_p, PD, PI, PD]),
     "cb_get_step_stats": (C.c_int, [C.c_void_p, PD, PD, PL]),
+    "cb_get_kernel_times": (C.c_int, [C.c_void_p, C.c_int, C.c_int, PD, PL]),
     "cb_write_results2": (C.c_int, [C.c_void_p, C.c_char_p]),
 }
 
@@ -216,6 +222,12 @@ class GpuModel:
         out.update(lpf=lpf[:n], itecnt=ite[:n], d=d[:n] if keep else None,
                    msg=lib().cb_last_error().decode(errors="replace") if st.status else "")
         return out
+
+    def kernel_times(self, precision: int = 64, step: int = -1) -> dict:
+        ms = np.zeros(5)
+        n = np.zeros(5, np.int64)
+        _check(lib().cb_get_kernel_times(self.h, precision, step, _p(ms), _p(n, PL)))
+        return {KT_NAMES[f]: (float(ms[f]), int(n[f])) for f in range(5)}
 
     def write_results2(self, path: str):
         _check(lib().cb_write_results2(self.h, path.encode()))

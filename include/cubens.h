@@ -50,7 +50,11 @@ typedef struct {
     int judge;           /* 1: exact FP64 pivots for det/psi               */
     int max_steps;       /* stop after this many converged steps           */
     int keep_d;          /* keep the displacement path for cb_path         */
+    int kernel_timing;   /* bitmask of CB_KT_* kernel families timed with HIP events */
 } cb_options;
+
+/* Kernel families of the band solver (timed with HIP events on their stream). */
+enum { CB_KT_DIAG = 0, CB_KT_PANEL = 1, CB_KT_UPDATE = 2, CB_KT_FWD = 3, CB_KT_BWD = 4 };
 
 typedef struct {
     double t_total, t_tangent, t_factor, t_solve, t_forces, t_judge, t_other;
@@ -121,6 +125,11 @@ int cb_get_path(const cb_model *m, double *lpf, int *itecnt, double *d);
  * seconds spent in the FP64 pivot judge, Newton iterations so far (no
  * reference counterpart: the reference keeps no timers). */
 int cb_get_step_stats(const cb_model *m, double *t_wall, double *t_judge, long *newton);
+/* Accumulated HIP-event times (ms) and launch counts of the band kernel
+ * families [5] selected by cb_options.kernel_timing, for the FP64 (precision
+ * 64) or FP32 (32) factor; step >= 0 returns the totals at that converged step
+ * of the last run, step < 0 the totals now (no reference counterpart). */
+int cb_get_kernel_times(const cb_model *m, int precision, int step, double *ms, long *launches);
 /* Writes the results2.txt rows of the last run (format of misc.c:350-417). */
 int cb_write_results2(const cb_model *m, const char *path);
 

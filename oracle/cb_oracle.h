@@ -118,6 +118,20 @@ int orc_test(long neq, const double *d_temp, const double *dd, const double *f_t
 int orc_quad(long neq, double a, double b, double c, double *dt, const double *dp,
              const double *ddr, const double *ddq, double *dd, double *dlpf, double *lpft);
 
+/* Skyline LDL^T of the leading columns only, stopping once budget_s seconds
+ * have passed; *cols receives the number of columns factorized. */
+int orc_skyfact_partial(long neq, const long *maxa, double *ss, double budget_s, long *cols);
+
+/* Bounded CPU timing sample of the serial path on a model: one tangent
+ * assembly, one internal-force pass (updatc + forces), one skyline
+ * back-substitution and a time-bounded partial factorization whose full cost
+ * is extrapolated by the column-height weights sum(kht^2). */
+typedef struct {
+    double t_tangent, t_forces, t_solve, t_factor_part, factor_fraction, t_factor_est;
+    long cols_done;
+} orc_sample;
+int orc_time_sample(orc_model *m, double factor_budget_s, orc_sample *out);
+
 /* ---- element backend: lets the driver run on the oracle's own element
  *      routines or on the reference's (oracle/_ref) ------------------------ */
 typedef struct orc_backend {

@@ -56,6 +56,12 @@ class Path(C.Structure):
                 ("n_factor", C.c_long), ("n_solve", C.c_long), ("n_newton", C.c_long)]
 
 
+class Sample(C.Structure):
+    _fields_ = [("t_tangent", C.c_double), ("t_forces", C.c_double), ("t_solve", C.c_double),
+                ("t_factor_part", C.c_double), ("factor_fraction", C.c_double),
+                ("t_factor_est", C.c_double), ("cols_done", C.c_long)]
+
+
 class Backend(C.Structure):
     _fields_ = [("ctx", C.c_void_p), ("stiff_tr", C.c_void_p), ("forces_tr", C.c_void_p),
                 ("stiff_sh", C.c_void_p), ("forces_sh", C.c_void_p), ("updatc", C.c_void_p)]
@@ -89,6 +95,8 @@ def lib():
         _lib.orc_test.argtypes = [C.c_long, PD, PD, PD, PD, PD, PD, C.c_double, PI, C.c_double,
                                   C.c_double, C.c_double]
         _lib.orc_test.restype = C.c_int
+        _lib.orc_time_sample.argtypes = [C.POINTER(Model), C.c_double, C.POINTER(Sample)]
+        _lib.orc_time_sample.restype = C.c_int
         _lib.orc_shell_elem_tangent.argtypes = [C.c_double, C.c_double, C.c_double, PD,
                                                 C.c_double, PD, C.c_double, PD, PD, PD, PD,
                                                 C.c_int, PD]
@@ -171,6 +179,13 @@ class OracleModel:
             self.close()
         except Exception:
             pass
+
+    def time_sample(self, factor_budget_s: float) -> dict:
+        """Bounded single-core timing sample of the serial path (orc_time_sample)."""
+        smp = Sample()
+        if lib().orc_time_sample(self.h, factor_budget_s, C.byref(smp)):
+            raise MemoryError("oracle sample allocation failed")
+        return {k: getattr(smp, k) for k, _ in smp._fields_}
 
     def run(self, max_steps: int, backend: str = "oracle", keep_f: bool = False) -> dict:
         """Runs the analysis; returns the converged path and timings."""

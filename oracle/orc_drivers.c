@@ -628,3 +628,56 @@ int orc_run(orc_model *m, const orc_backend *b, orc_path *out)
     run_free(&R);
     return rc;
 }
+
+/* Bounded timing sample of the serial CPU path (see cb_oracle.h). */
+int orc_time_sample(orc_model *m, double factor_budget_s, orc_sample *out)
+{
+    const orc_dims *d = &m->dm;
+    long neq = d->NEQ, nj = d->NJ, nsh = d->NE_SH, ntri = d->NE_TR + d->NE_FR * 3 + d->NE_SH * 3;
+    memset(out, 0, sizeof *out);
+    double *ss = calloc((size_t)m->lss, sizeof(double));
+    double *f = calloc((size_t)neq, sizeof(double));
+    double *dd = calloc((size_t)neq, sizeof(double));
+    double *x = malloc(sizeof(double) * (size_t)nj * 3), *xip = malloc(sizeof(double) * (size_t)nj * 3);
+    double *efp = calloc((size_t)nsh * 18 + 1, sizeof(double)), *efi = calloc((size_t)nsh * 18 + 1, sizeof(double));
+    double *c1 = malloc(sizeof(double) * (size_t)ntri), *c2 = malloc(sizeof(double) * (size_t)ntri);
+    double *c3 = malloc(sizeof(double) * (size_t)ntri);
+    double *da = calloc((size_t)nsh + 1, sizeof(double)), *ds = calloc((size_t)nsh * 3 + 1, sizeof(double));
+    double *dl = calloc((size_t)(d->NE_TR + d->NE_FR) + 1, sizeof(double));
+    double *xfr = calloc((size_t)d->NE_FR * 6 + 1, sizeof(double));
+    if (!ss || !f || !dd || !x || !xip || !efp || !efi || !c1 || !c2 || !c3 || !da || !ds || !dl || !xfr) return 1;
+    memcpy(x, m->x, sizeof(double) * (size_t)nj * 3);
+    double t0 = now_s();
+    if (d->NE_TR > 0) orc_stiff_tr(d, ss, m->emod, m->carea, m->llength, m->llength, m->c1, m->c2, m->c3, efp, m->maxa, m->mcode);
+    if (nsh > 0)
+        orc_stiff_sh(d, ss, m->emod, m->nu, m->x, m->xlocal, m->thick, m->farea, m->farea, m->slength,
+                     m->c1, m->c2, m->c3, m->maxa, m->minc, m->mcode);
+    out->t_tangent = now_s() - t0;
+    t0 = now_s();
+    orc_updatc(d, x, xip, xfr, dd, dl, da, ds, m->offset, m->osflag, m->auxpt, c1, c2, c3, m->minc, m->jcode);
+    if (d->NE_TR > 0) orc_forces_tr(d, f, efi, m->emod, m->carea, m->llength, dl, c1, c2, c3, m->mcode);
+    if (nsh > 0)
+        orc_forces_sh(d, f, efp, efi, dd, x, m->emod, m->nu, m->xlocal, m->thick, m->farea, m->slength,
+                      m->c1, m->c2, m->c3, c1, c2, c3, m->minc, m->mcode);
+    out->t_forces = now_s() - t0;
+    /* a back-substitution sweeps the same skyline array whether or not it is
+       factorized; time it on the assembled array */
+    memcpy(f, m->q, sizeof(double) * (size_t)neq);
+    t0 = now_s();
+    orc_skysolve(neq, m->maxa, ss, f);
+    out->t_solve = now_s() - t0;
+    t0 = now_s();
+    orc_skyfact_partial(neq, m->maxa, ss, factor_budget_s, &out->cols_done);
+    out->t_factor_part = now_s() - t0;
+    double wall = 0, wdone = 0;
+    for (long i = 0; i < neq; ++i) {
+        double h = (double)m->kht[i];
+        wall += h * h + 1;
+        if (i < out->cols_done) wdone += h * h + 1;
+    }
+    out->factor_fraction = wall > 0 ? wdone / wall : 1.0;
+    out->t_factor_est = out->factor_fraction > 0 ? out->t_factor_part / out->factor_fraction : 0;
+    free(ss); free(f); free(dd); free(x); free(xip); free(efp); free(efi); free(c1); free(c2); free(c3);
+    free(da); free(ds); free(dl); free(xfr);
+    return 0;
+}

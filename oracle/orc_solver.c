@@ -66,6 +66,49 @@ int orc_skyfact(long neq, const long *maxa, double *ss, double *ssd, int msal, i
     return 0;
 }
 
+#include <time.h>
+static double wall_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+int orc_skyfact_partial(long neq, const long *maxa, double *ss, double budget_s, long *cols)
+{
+    double t0 = wall_s();
+    long n;
+    for (n = 1; n <= neq; ++n) {
+        long diag = maxa[n - 1], top = maxa[n] - 1, height = top - diag;
+        if (height >= 2) {
+            long k = n - (height - 1);
+            for (long j = 1; j <= height - 1; ++j, ++k) {
+                long at = top - j, kd = maxa[k - 1], kh = maxa[k] - kd - 1;
+                if (kh > 0) {
+                    long cnt = kh < j ? kh : j;
+                    double c = 0;
+                    for (long l = 1; l <= cnt; ++l) c += ss[kd + l - 1] * ss[at + l - 1];
+                    ss[at - 1] -= c;
+                }
+            }
+        }
+        if (height >= 1) {
+            double b = 0;
+            long k = n;
+            for (long a = diag + 1; a <= top; ++a) {
+                k--;
+                double c = ss[a - 1] / ss[maxa[k - 1] - 1];
+                b += c * ss[a - 1];
+                ss[a - 1] = c;
+            }
+            ss[diag - 1] -= b;
+        }
+        if ((n & 255) == 0 && wall_s() - t0 > budget_s) break;
+    }
+    *cols = n > neq ? neq : n;
+    return 0;
+}
+
 /* Forward reduction, diagonal scaling and back substitution on a factor from
  * orc_skyfact; rhs is overwritten by the solution. */
 void orc_skysolve(long neq, const long *maxa, const double *ss, double *rhs)
