@@ -224,12 +224,156 @@ def smooth_field(rng: SplitMix64, tris, ii, jj, n: int, modes: int = 16, kmax: i
     return g * np.sqrt(2.0 / modes)
 
 
+# --------------------------------------------------------------------------
+# C4: 3-D corotational space frames under harmonic lateral load, Newmark
+# --------------------------------------------------------------------------
+def chs_section(r_o: float, E: float = 2e11):
+    """Circular hollow section with wall r_o/10: A, I, J = 2I, warping 0."""
+    r_i = 0.9 * r_o
+    A = math.pi * (r_o ** 2 - r_i ** 2)
+    I = math.pi * (r_o ** 4 - r_i ** 4) / 4
+    return {"E": E, "G": E / 2.6, "A": A, "I": I, "J": 2 * I, "Iw": 0.0}
+
+
+def space_frame(n_elem: int = 400, kind: str = "tower", seed: int | None = None,
+                instance: int = 0, steps: int = 200, dt: float = 1e-3, amp: float | None = None,
+                tol=(1e-8, 1e-8, 1e-8), algflag: int = 5, numopt: int = 0,
+                spectrds: float = 1.0, itemax: int = 20) -> ModelText:
+    """A random tower or lattice girder of about ``n_elem`` frame members.
+
+    tower : square storeys of 4 columns, 4 beams and 8 face X-braces; plan
+            width and storey height U(1, 4) m; base joints fixed.
+    grid  : a long 3-D lattice girder of ny x nz joints per slice and nx
+            slices, members along the three lattice directions at spacings
+            U(1, 4) m; the joints of the first slice are fixed.
+    Every joint has its warping DOF fixed (restraint 0), so each member has 12
+    active DOF.  Circular hollow sections r_o ~ U(0.05, 0.2) m, E 200 GPa,
+    density 7850.  The top joints (tower) or the free-end slice (grid) carry a
+    lateral load amp * sin(2 pi f t) with f ~ U(2, 10) Hz over ``steps`` steps
+    of ``dt`` (average acceleration Newmark with numopt 0, spectral radius 1).
+    Joints are numbered level by level so the skyline band stays narrow.
+    """
+    rng = SplitMix64(workload_seed(4, instance) if seed is None else seed)
+    joints, members, fixed, loaded = [], [], [], []
+    if kind == "tower":
+        b = float(rng.uniform(1, 1.0, 4.0)[0])
+        h = float(rng.uniform(1, 1.0, 4.0)[0])
+        ns = max(1, int(math.ceil(n_elem / 16)))
+        plan = [(0.0, 0.0), (b, 0.0), (b, b), (0.0, b)]
+        for lv in range(ns + 1):
+            for (px, py) in plan:
+                joints.append((px, py, lv * h))
+        jid = lambda lv, c: lv * 4 + c + 1      # noqa: E731
+        fixed = [jid(0, c) for c in range(4)]
+        loaded = [jid(ns, c) for c in range(4)]
+        for lv in range(ns):
+            for c in range(4):
+                members.append((jid(lv, c), jid(lv + 1, c)))                 # column
+            for c in range(4):
+                members.append((jid(lv + 1, c), jid(lv + 1, (c + 1) % 4)))   # beam
+            for c in range(4):
+                members.append((jid(lv, c), jid(lv + 1, (c + 1) % 4)))       # X-brace
+                members.append((jid(lv, (c + 1) % 4), jid(lv + 1, c)))
+    elif kind == "grid":
+        ny = 2 + int(rng.uniform(1, 0, 3)[0])
+        nz = 2 + int(rng.uniform(1, 0, 3)[0])
+        per_slice = ny * nz * 3 - ny - nz          # members between and within slices
+        nx = max(2, int(math.ceil(n_elem / per_slice)))
+        sx, sy, sz = (float(v) for v in rng.uniform(3, 1.0, 4.0))
+        jid = lambda a, b_, c: a * ny * nz + b_ * nz + c + 1   # noqa: E731
+        for a in range(nx):
+            for b_ in range(ny):
+                for c in range(nz):
+                    joints.append((a * sx, b_ * sy, c * sz))
+        fixed = [jid(0, b_, c) for b_ in range(ny) for c in range(nz)]
+        loaded = [jid(nx - 1, b_, c) for b_ in range(ny) for c in range(nz)]
+        for a in range(nx):
+            for b_ in range(ny):
+                for c in range(nz):
+                    if a + 1 < nx:
+                        members.append((jid(a, b_, c), jid(a + 1, b_, c)))
+                    if b_ + 1 < ny:
+                        members.append((jid(a, b_, c), jid(a, b_ + 1, c)))
+                    if c + 1 < nz:
+                        members.append((jid(a, b_, c), jid(a, b_, c + 1)))
+    else:
+        raise ValueError(kind)
+    nj, ne = len(joints), len(members)
+    r_o = rng.uniform(ne, 0.05, 0.2)
+    freq = float(rng.uniform(1, 2.0, 10.0)[0])
+    if amp is None:
+        amp = float(rng.uniform(1, 1e3, 5e3)[0])
+    X = np.array(joints)
+    out = io.StringIO()
+    w = out.write
+    w(f"2\n{algflag}\n")
+    if algflag > 3:
+        # checkpoint interval beyond the last step: no checkpoint is written
+        # (CHKPT 0 would divide by zero in the reference's k % CHKPT)
+        w(f"{steps + 1},0\n")
+    w(f"0\n1\n{nj}\n0,{ne},0,0,0\n")
+    for a, b_ in members:
+        w(f"{a},{b_}\n")
+    for j in fixed:
+        for d in range(1, 7):
+            w(f"{j},{d}\n")
+    for j in range(1, nj + 1):
+        w(f"{j},7,0\n")
+    w("0,0\n")
+    if algflag > 3:
+        w("0,0\n")
+    for p in X:
+        w(f"{_fmt(p[0])},{_fmt(p[1])},{_fmt(p[2])}\n")
+    secs = [chs_section(r) for r in r_o]
+    for s_ in secs:
+        w(f"{_fmt(s_['E'])},{_fmt(s_['G'])},7850.0,{_fmt(s_['A'])},{_fmt(s_['I'])},"
+          f"{_fmt(s_['I'])},{_fmt(s_['J'])},0.0\n")
+    w("0,0,0,0,0,0,0\n")
+    for a, b_ in members:
+        pa, pb = X[a - 1], X[b_ - 1]
+        e = (pb - pa) / np.linalg.norm(pb - pa)
+        aux = pa + (np.array([1.0, 0.0, 0.0]) if abs(e[2]) > 0.9 else np.array([0.0, 0.0, 1.0]))
+        w(f"{_fmt(aux[0])},{_fmt(aux[1])},{_fmt(aux[2])}\n")
+    for _ in members:
+        w("1e+30,1.0,1.0\n")
+    w("0,0,0,0,0\n")
+    t = np.arange(steps + 1) * dt
+    if algflag > 3:
+        w(f"{steps},{_fmt(steps * dt)}\n")
+        hist = amp * np.sin(2 * np.pi * freq * t)
+        for j in loaded:
+            for v in hist:
+                w(f"{j},1,{_fmt(v)}\n")
+        w("0,0,0\n0,0,0\n0,0,0,0,0\n")
+        w(f"{numopt},{_fmt(spectrds)}\n1.0,1.0,1.0,1.0,1.0\n{itemax},10,1000\n")
+        w(f"{tol[0]},{tol[1]},{tol[2]}\n")
+    else:
+        for j in loaded:
+            w(f"{j},1,{_fmt(amp)}\n")
+        w("0,0,0\n0,0,0\n")
+        w(f"1.0,0.05,0.05,0.05,0.05\n{itemax},10,1000\n{tol[0]},{tol[1]},{tol[2]}\n")
+    return ModelText(out.getvalue(), nj, n_fr=ne,
+                     info={"kind": kind, "freq": freq, "amp": amp, "fixed": fixed,
+                           "loaded": loaded, "neq": 6 * (nj - len(fixed))})
+
+
+def frame_batch(count: int = 512, lo: int = 200, hi: int = 5000, **kw) -> list[ModelText]:
+    """``count`` frames of U{lo..hi} members, towers and girders alternating."""
+    rng = SplitMix64(workload_seed(4, 100000))
+    sizes = lo + np.floor(rng.uniform(count, 0, hi - lo + 1)).astype(int)
+    return [space_frame(int(sz), kind=("tower", "grid")[i % 2], instance=i, **kw)
+            for i, sz in enumerate(sizes)]
+
+
 def config_inputs(index: int, **kw) -> ModelText:
-    """The workloads by index: 1 truss dome, 2 panel 64^2, 3 panel 256^2."""
+    """The workloads by index: 1 truss dome, 2 panel 64^2, 3 panel 256^2,
+    4 one space frame of the frame batch."""
     if index == 1:
         return truss_dome(**kw)
     if index == 2:
         return shell_panel(n=64, config_index=2, **kw)
     if index == 3:
         return shell_panel(n=256, thick_sigma=0.1, imperfection=0.1, config_index=3, **kw)
+    if index == 4:
+        return space_frame(**kw)
     raise ValueError(f"no generator for workload {index}")

@@ -46,6 +46,8 @@ typedef struct orc_model {
     double *offset, *auxpt, *xfr;     /* NE_FR*6, NE_FR*3, NE_FR*6 */
     int *osflag, *mendrel;            /* NE_FR, NE_FR*5 */
     double *nu, *thick, *dens_sh, *xlocal, *farea, *slength;
+    double *dens;                     /* the reference's one density array [NE_TR+NE_FR+NE_SH],
+                                         written by prop_tr, prop_fr, prop_sh in turn, each from 0 */
     double *c1, *c2, *c3;
     double *q;                        /* NEQ reference load */
     /* dynamic input (ALGFLAG > 3) */
@@ -58,7 +60,7 @@ typedef struct orc_model {
     double toldisp, tolforc, tolener;
     double alpha, psi_thresh, dkimax, dk;                   /* MSAL */
     long dkdof;                                             /* 0-based */
-    int numopt, spectrds;                                   /* Newmark */
+    int numopt; double spectrds;                            /* Newmark */
 } orc_model;
 
 /* ---- model input (model.c, main.c:330-1653) ---------------------------- */
@@ -98,16 +100,26 @@ void orc_shell_elem_tangent(double E, double nu, double t, const double xl[3], d
                             int geometric, double K[18 * 18]);
 
 /* ---- frame (frame.c) ---------------------------------------------------- */
+/* Elastic and geometric local 14x14 frame matrices (frame.c:354-579). */
+void orc_frame_ke(double E, double G, double A, double L, double Is, double Iw, double J,
+                  double Cw, double k[14][14]);
+void orc_frame_kg(const double f[14], double L, double A, double Ip, double k[14][14]);
 void orc_stiff_fr(const orc_dims *d, double *ss, const orc_model *m, const double *llength,
                   const double *defllen_ip, const double *c1_ip, const double *c2_ip,
                   const double *c3_ip, const double *ef_ip, const double *efFE_ip);
-void orc_forces_fr(const orc_dims *d, double *f, double *ef_ip, double *ef_i,
-                   const double *efFE_ip, double *efFE_i, const double *dd,
-                   const orc_model *m, const double *llength, const double *defllen_ip,
-                   const double *c1_ip, const double *c2_ip, const double *c3_ip,
-                   const double *c1_i, const double *c2_i, const double *c3_i);
-void orc_mass(const orc_dims *d, double *sm, const orc_model *m, const double *x,
-              double *llength_out);
+int orc_forces_fr(const orc_dims *d, double *f, const double *ef_ip, double *ef_i,
+                  const double *efFE_ip, double *efFE_i, const double *dd,
+                  const orc_model *m, const double *llength, const double *defllen_ip,
+                  const double *c1_ip, const double *c2_ip, const double *c3_ip,
+                  const double *c1_i, const double *c2_i, const double *c3_i);
+/* Lumped diagonal masses into sm[NEQ]; overwrite the geometry arrays they are
+ * given, as the reference's mass_* do. */
+void orc_mass_tr(const orc_dims *d, double *sm, const orc_model *m, const double *x,
+                 double *llength, const double *dens);
+void orc_mass_fr(const orc_dims *d, double *sm, const orc_model *m, const double *x,
+                 double *llength, double *xfr, const double *dens);
+void orc_mass_sh(const orc_dims *d, double *sm, const orc_model *m, const double *x,
+                 double *farea, double *slength, const double *dens);
 
 /* ---- solver pieces (solve.c, misc.c, arc.c) ------------------------------ */
 int orc_skyfact(long neq, const long *maxa, double *ss, double *ssd, int msal, int *det);
@@ -165,6 +177,22 @@ typedef struct orc_backend {
                    double *deffarea_i, double *defslen_i, const double *offset,
                    const int *osflag, const double *auxpt, double *c1_i, double *c2_i,
                    double *c3_i, const long *minc, const long *jcode);
+    void (*stiff_fr)(void *ctx, const orc_dims *d, double *ss, const orc_model *m,
+                     const double *llength, const double *defllen_ip, const double *c1_ip,
+                     const double *c2_ip, const double *c3_ip, const double *ef_ip,
+                     const double *efFE_ip);
+    int (*forces_fr)(void *ctx, const orc_dims *d, double *f, double *ef_ip, double *ef_i,
+                     const double *efFE_ip, double *efFE_i, const double *dd,
+                     const orc_model *m, const double *llength, const double *defllen_ip,
+                     const double *c1_ip, const double *c2_ip, const double *c3_ip,
+                     const double *c1_i, const double *c2_i, const double *c3_i,
+                     double *dlpf, int itecnt);
+    void (*mass_tr)(void *ctx, const orc_dims *d, double *sm, const orc_model *m,
+                    const double *x, double *llength, const double *dens);
+    void (*mass_fr)(void *ctx, const orc_dims *d, double *sm, const orc_model *m,
+                    const double *x, double *llength, double *xfr, const double *dens);
+    void (*mass_sh)(void *ctx, const orc_dims *d, double *sm, const orc_model *m,
+                    const double *x, double *farea, double *slength, const double *dens);
 } orc_backend;
 
 void orc_default_backend(orc_backend *b);

@@ -35,7 +35,7 @@ class Model(C.Structure):
                 ("iwarp", PD), ("zstrong", PD), ("zweak", PD),
                 ("offset", PD), ("auxpt", PD), ("xfr", PD), ("osflag", PI), ("mendrel", PI),
                 ("nu", PD), ("thick", PD), ("dens_sh", PD), ("xlocal", PD), ("farea", PD),
-                ("slength", PD), ("c1", PD), ("c2", PD), ("c3", PD), ("q", PD),
+                ("slength", PD), ("dens", PD), ("c1", PD), ("c2", PD), ("c3", PD), ("q", PD),
                 ("ntstpsinpt", C.c_long), ("ttot", C.c_double), ("dt", C.c_double),
                 ("pinpt", PD), ("um", PD), ("vm", PD), ("am", PD),
                 ("lpfmax", C.c_double), ("lpf0", C.c_double), ("dlpf0", C.c_double),
@@ -45,7 +45,7 @@ class Model(C.Structure):
                 ("toldisp", C.c_double), ("tolforc", C.c_double), ("tolener", C.c_double),
                 ("alpha", C.c_double), ("psi_thresh", C.c_double), ("dkimax", C.c_double),
                 ("dk", C.c_double), ("dkdof", C.c_long),
-                ("numopt", C.c_int), ("spectrds", C.c_int)]
+                ("numopt", C.c_int), ("spectrds", C.c_double)]
 
 
 class Path(C.Structure):
@@ -64,7 +64,9 @@ class Sample(C.Structure):
 
 class Backend(C.Structure):
     _fields_ = [("ctx", C.c_void_p), ("stiff_tr", C.c_void_p), ("forces_tr", C.c_void_p),
-                ("stiff_sh", C.c_void_p), ("forces_sh", C.c_void_p), ("updatc", C.c_void_p)]
+                ("stiff_sh", C.c_void_p), ("forces_sh", C.c_void_p), ("updatc", C.c_void_p),
+                ("stiff_fr", C.c_void_p), ("forces_fr", C.c_void_p), ("mass_tr", C.c_void_p),
+                ("mass_fr", C.c_void_p), ("mass_sh", C.c_void_p)]
 
 
 def ptr(a: np.ndarray, kind=PD):
@@ -346,3 +348,22 @@ def skysolve(maxa: np.ndarray, fac: np.ndarray, rhs: np.ndarray) -> np.ndarray:
     x = np.array(rhs, dtype=np.float64, copy=True)
     L.orc_skysolve(len(maxa) - 1, ptr(np.ascontiguousarray(maxa, np.int64), PL), ptr(fac), ptr(x))
     return x
+
+
+def frame_ke(E, G, A, L, Is, Iw, J, Cw) -> np.ndarray:
+    """Local 14x14 elastic frame stiffness (orc_frame_ke)."""
+    k = np.zeros((14, 14))
+    L_ = lib()
+    L_.orc_frame_ke.argtypes = [C.c_double] * 8 + [PD]
+    L_.orc_frame_ke(E, G, A, L, Is, Iw, J, Cw, ptr(k))
+    return k
+
+
+def frame_kg(f, L, A, Ip) -> np.ndarray:
+    """Local 14x14 geometric frame stiffness for end forces f[14] (orc_frame_kg)."""
+    k = np.zeros((14, 14))
+    f = np.ascontiguousarray(f, np.float64)
+    L_ = lib()
+    L_.orc_frame_kg.argtypes = [PD, C.c_double, C.c_double, C.c_double, PD]
+    L_.orc_frame_kg(ptr(f), L, A, Ip, ptr(k))
+    return k

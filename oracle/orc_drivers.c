@@ -79,6 +79,44 @@ static void be_updatc(void *ctx, const orc_dims *d, double *x_temp, double *x_ip
                auxpt, c1_i, c2_i, c3_i, minc, jcode);
 }
 
+static void be_stiff_fr(void *ctx, const orc_dims *d, double *ss, const orc_model *m,
+                        const double *llength, const double *defllen_ip, const double *c1_ip,
+                        const double *c2_ip, const double *c3_ip, const double *ef_ip,
+                        const double *efFE_ip)
+{
+    (void)ctx;
+    orc_stiff_fr(d, ss, m, llength, defllen_ip, c1_ip, c2_ip, c3_ip, ef_ip, efFE_ip);
+}
+static int be_forces_fr(void *ctx, const orc_dims *d, double *f, double *ef_ip, double *ef_i,
+                        const double *efFE_ip, double *efFE_i, const double *dd,
+                        const orc_model *m, const double *llength, const double *defllen_ip,
+                        const double *c1_ip, const double *c2_ip, const double *c3_ip,
+                        const double *c1_i, const double *c2_i, const double *c3_i,
+                        double *dlpf, int itecnt)
+{
+    (void)ctx; (void)dlpf; (void)itecnt;
+    return orc_forces_fr(d, f, ef_ip, ef_i, efFE_ip, efFE_i, dd, m, llength, defllen_ip, c1_ip,
+                         c2_ip, c3_ip, c1_i, c2_i, c3_i);
+}
+static void be_mass_tr(void *ctx, const orc_dims *d, double *sm, const orc_model *m,
+                       const double *x, double *llength, const double *dens)
+{
+    (void)ctx;
+    orc_mass_tr(d, sm, m, x, llength, dens);
+}
+static void be_mass_fr(void *ctx, const orc_dims *d, double *sm, const orc_model *m,
+                       const double *x, double *llength, double *xfr, const double *dens)
+{
+    (void)ctx;
+    orc_mass_fr(d, sm, m, x, llength, xfr, dens);
+}
+static void be_mass_sh(void *ctx, const orc_dims *d, double *sm, const orc_model *m,
+                       const double *x, double *farea, double *slength, const double *dens)
+{
+    (void)ctx;
+    orc_mass_sh(d, sm, m, x, farea, slength, dens);
+}
+
 void orc_default_backend(orc_backend *b)
 {
     b->ctx = NULL;
@@ -87,6 +125,11 @@ void orc_default_backend(orc_backend *b)
     b->stiff_sh = be_stiff_sh;
     b->forces_sh = be_forces_sh;
     b->updatc = be_updatc;
+    b->stiff_fr = be_stiff_fr;
+    b->forces_fr = be_forces_fr;
+    b->mass_tr = be_mass_tr;
+    b->mass_fr = be_mass_fr;
+    b->mass_sh = be_mass_sh;
 }
 
 /* ---- state of one analysis -------------------------------------------------
@@ -104,6 +147,8 @@ typedef struct {
     double *defllen, *defllen_i, *defllen_ip, *llen_ref;
     double *deffarea, *deffarea_i, *deffarea_ip, *defslen, *defslen_i, *defslen_ip;
     double *efFE, *efFE_i, *efFE_ip;
+    double *farea_w, *slen_w;          /* face areas / side lengths the elements read */
+    double *keff, *uc, *vc, *ac, *uc_i, *vc_i, *ac_i, *um, *vm, *am, *dyn;
     orc_path *out;
 } run_t;
 
@@ -135,6 +180,12 @@ static void run_init(run_t *R, orc_model *m, const orc_backend *be, orc_path *ou
     R->deffarea = dz(R->nsh); R->deffarea_i = dz(R->nsh); R->deffarea_ip = dz(R->nsh);
     R->defslen = dz(R->nsh * 3); R->defslen_i = dz(R->nsh * 3); R->defslen_ip = dz(R->nsh * 3);
     R->efFE = dz(R->nfr * 14); R->efFE_i = dz(R->nfr * 14); R->efFE_ip = dz(R->nfr * 14);
+    R->farea_w = dz(R->nsh); R->slen_w = dz(R->nsh * 3);
+    R->keff = dz(m->dm.ALGFLAG == 5 ? m->lss : 1);
+    R->uc = dz(n); R->vc = dz(n); R->ac = dz(n); R->uc_i = dz(n); R->vc_i = dz(n);
+    R->ac_i = dz(n); R->um = dz(n); R->vm = dz(n); R->am = dz(n); R->dyn = dz(n);
+    CP(R->farea_w, m->farea, R->nsh);
+    CP(R->slen_w, m->slength, R->nsh * 3);
 
     CP(R->x, m->x, R->nj * 3);
     CP(R->x_temp, m->x, R->nj * 3);
@@ -149,7 +200,7 @@ static void run_init(run_t *R, orc_model *m, const orc_backend *be, orc_path *ou
 
 static void run_free(run_t *R)
 {
-    double **p = &R->ss, **e = &R->efFE_ip;
+    double **p = &R->ss, **e = &R->dyn;
     for (; p <= e; ++p) free(*p);
 }
 
@@ -204,10 +255,10 @@ static void assemble(run_t *R, const double *x, const double *c1, const double *
         R->be->stiff_tr(R->be->ctx, d, R->ss, m->emod, m->carea, R->llen_ref, defllen, m->yield,
                         c1, c2, c3, ef, m->maxa, m->mcode);
     if (d->NE_FR > 0)
-        orc_stiff_fr(d, R->ss, m, R->llen_ref, defllen, c1, c2, c3, ef, efFE);
+        R->be->stiff_fr(R->be->ctx, d, R->ss, m, R->llen_ref, defllen, c1, c2, c3, ef, efFE);
     if (d->NE_SH > 0)
-        R->be->stiff_sh(R->be->ctx, d, R->ss, m->emod, m->nu, x, m->xlocal, m->thick, m->farea,
-                        deffarea, m->slength, defslen, m->yield, c1, c2, c3, ef, m->maxa,
+        R->be->stiff_sh(R->be->ctx, d, R->ss, m->emod, m->nu, x, m->xlocal, m->thick, R->farea_w,
+                        deffarea, R->slen_w, defslen, m->yield, c1, c2, c3, ef, m->maxa,
                         m->minc, m->mcode);
     if (R->out) { R->out->t_tangent += now_s() - t0; }
 }
@@ -225,14 +276,17 @@ static int update_and_forces(run_t *R, double *x_temp, double *xfr_temp, double 
     if (d->NE_TR > 0)
         R->be->forces_tr(R->be->ctx, d, f_out, R->ef_i, m->emod, m->carea, R->llen_ref,
                          R->defllen_i, m->yield, R->c1_i, R->c2_i, R->c3_i, m->mcode);
-    if (d->NE_FR > 0)
-        orc_forces_fr(d, f_out, R->ef_ip, R->ef_i, R->efFE_ip, R->efFE_i, dd, m, R->llen_ref,
-                      R->defllen_ip, R->c1_ip, R->c2_ip, R->c3_ip, R->c1_i, R->c2_i, R->c3_i);
     int chk = 0;
+    if (d->NE_FR > 0) {
+        double dl = 0;
+        chk = R->be->forces_fr(R->be->ctx, d, f_out, R->ef_ip, R->ef_i, R->efFE_ip, R->efFE_i, dd,
+                               m, R->llen_ref, R->defllen_ip, R->c1_ip, R->c2_ip, R->c3_ip,
+                               R->c1_i, R->c2_i, R->c3_i, &dl, 1);
+    }
     if (d->NE_SH > 0)
         chk = R->be->forces_sh(R->be->ctx, d, f_out, R->ef_ip, R->ef_i, dd, x_temp, R->x_ip,
-                               m->emod, m->nu, m->xlocal, m->thick, m->farea, R->deffarea_ip,
-                               m->slength, R->defslen_ip, m->yield, R->c1_ip, R->c2_ip,
+                               m->emod, m->nu, m->xlocal, m->thick, R->farea_w, R->deffarea_ip,
+                               R->slen_w, R->defslen_ip, m->yield, R->c1_ip, R->c2_ip,
                                R->c3_ip, R->c1_i, R->c2_i, R->c3_i, m->minc, m->mcode,
                                m->jcode);
     if (R->out) R->out->t_forces += now_s() - t0;
@@ -605,10 +659,229 @@ static int run_msal(run_t *R)
 }
 
 /* ---- nonlinear Newmark (main.c:3305-3960) ------------------------------ */
+/* ---- nonlinear Newmark (main.c:3305-3960, solve.c:139-245, 469-533) -------
+ * Without prescribed support motion (NBC = 0; the input reader rejects it) and
+ * with the skyline solver.  The reference discards the initial conditions
+ * read from the input here (uc = vc = ac = 0, main.c:3443-3445), and so does
+ * this restatement.  One path row per time step: (time, itecnt, d). */
 static int run_newmark(run_t *R)
 {
-    if (R->out) { R->out->status = 2; snprintf(R->out->msg, sizeof R->out->msg, "Newmark not restated yet"); }
-    return 2;
+    orc_model *m = R->m;
+    const orc_dims *d = &m->dm;
+    long n = R->neq;
+    double am_, af_;
+    double rho = m->spectrds;
+    switch (m->numopt) {
+    case 0:
+        if (rho != 1) FAIL(1, "invalid spectral radius for Newmark without dissipation");
+        am_ = 0; af_ = 0; break;
+    case 1: am_ = (2 * rho - 1) / (rho + 1); af_ = rho / (rho + 1); break;
+    case 2: am_ = 0; af_ = (1 - rho) / (1 + rho); break;
+    case 3: am_ = (rho - 1) / (rho + 1); af_ = 0; break;
+    default: FAIL(1, "unknown Newmark dissipation option");
+    }
+    double alpha = pow(1 - am_ + af_, 2) / 4, delta = 0.5 - am_ + af_;
+    double dt = m->dt;
+    long ntst = (long)(m->ttot / dt + 1);
+    long ns_in = m->ntstpsinpt;
+    if (ntst > ns_in) FAIL(1, "time steps exceed the load history");
+    const double *P = m->pinpt;                /* P[i*ns_in + k] */
+    double ddt = 1, sub_dt = 1, dt_temp = dt;
+    int tsflag = 0;
+    double lpfi = m->lpf0, dlpfi = m->dlpf0;
+    int convchk = 1, frcchk = 0, itecnt = 0, det = 0;
+    double intener1 = 0;
+    /* the frame/truss lengths and the shell areas/side lengths the elements
+       read are the run's working copies, overwritten by the mass routines */
+    double *llen = R->llen_ref;
+    long k = 0;
+    do {
+        dt_temp = ddt * dt;
+        sub_dt = ddt;
+        tsflag = 0;
+        do {
+            double a0 = 1 / (alpha * pow(dt_temp, 2));
+            double a2 = 1 / (alpha * dt_temp);
+            double a3 = 1 / (2 * alpha) - 1;
+            double a6 = dt_temp * (1 - delta);
+            double a7 = delta * dt_temp;
+            int solcnt = 0, subcnt = 0;
+            double lpf = lpfi, dlpf = dlpfi, dlpfp;
+            do {
+                if (lpf > m->lpfmax) lpf = m->lpfmax;
+                for (long i = 0; i < n; ++i) {
+                    R->fp[i] = R->f[i];
+                    R->d_temp[i] = R->d[i];
+                    R->f_temp[i] = R->f[i];
+                }
+                dlpfp = dlpf;
+                reset_from_base(R);
+                CP(R->um, R->uc, n); CP(R->uc_i, R->uc, n);
+                CP(R->vm, R->vc, n); CP(R->vc_i, R->vc, n);
+                CP(R->am, R->ac, n); CP(R->ac_i, R->ac, n);
+                itecnt = 0;
+                frcchk = 0;
+                do {
+                    if (itecnt == 0) {
+                        if (k == 0) {
+                            for (long i = 0; i < n; ++i) {
+                                R->qtot[i] = P[i * ns_in + k] * sub_dt * lpf;
+                                R->r[i] = R->qtot[i] - R->f_temp[i];
+                            }
+                        } else {
+                            for (long i = 0; i < n; ++i) {
+                                double p0 = P[i * ns_in + k - 1], p1 = P[i * ns_in + k];
+                                R->qtot[i] = (p0 + (p1 - p0) * sub_dt) * lpf;
+                                R->r[i] = (R->qtot[i] - R->f_temp[i]) +
+                                          af_ / (1 - af_) * (p0 - R->f_temp[i]);
+                            }
+                        }
+                    } else {
+                        for (long i = 0; i < n; ++i) {
+                            double r = R->f_temp[i] - R->qtot[i];
+                            R->r[i] = r + R->sm[i] * R->ac_i[i] -
+                                      R->sm[i] * ((a2 * R->vc_i[i] + a3 * R->ac_i[i]) * (1 - am_) -
+                                                  R->ac_i[i] * am_) / (1 - af_);
+                        }
+                    }
+                    /* tangent and lumped mass, type by type (main.c:3592-3620) */
+                    double t0 = now_s();
+                    memset(R->ss, 0, sizeof(double) * (size_t)m->lss);
+                    memset(R->sm, 0, sizeof(double) * (size_t)n);
+                    if (d->NE_TR > 0) {
+                        R->be->stiff_tr(R->be->ctx, d, R->ss, m->emod, m->carea, llen, R->defllen_ip,
+                                        m->yield, R->c1_ip, R->c2_ip, R->c3_ip, R->ef_ip, m->maxa,
+                                        m->mcode);
+                        R->be->mass_tr(R->be->ctx, d, R->sm, m, R->x, llen, m->dens);
+                    }
+                    if (d->NE_FR > 0) {
+                        R->be->stiff_fr(R->be->ctx, d, R->ss, m, llen, R->defllen_ip, R->c1_ip,
+                                        R->c2_ip, R->c3_ip, R->ef_ip, R->efFE_ip);
+                        R->be->mass_fr(R->be->ctx, d, R->sm, m, R->x, llen, R->xfr, m->dens);
+                    }
+                    if (d->NE_SH > 0) {
+                        R->be->stiff_sh(R->be->ctx, d, R->ss, m->emod, m->nu, R->x_temp, m->xlocal,
+                                        m->thick, R->farea_w, R->deffarea_ip, R->slen_w,
+                                        R->defslen_ip, m->yield, R->c1_ip, R->c2_ip, R->c3_ip,
+                                        R->ef_ip, m->maxa, m->minc, m->mcode);
+                        R->be->mass_sh(R->be->ctx, d, R->sm, m, R->x, R->farea_w, R->slen_w, m->dens);
+                    }
+                    if (R->out) R->out->t_tangent += now_s() - t0;
+                    if (m->lss == 1) {
+                        R->dd[0] = R->r[0] / R->ss[0];
+                    } else {
+                        /* solve.c:189-207, 469-533: Keff = K + a0(1-am)/(1-af) M on the
+                           diagonal, Reff = r + M((1-am)(a2 v + a3 a) - am a)/(1-af) */
+                        CP(R->keff, R->ss, m->lss);
+                        for (long i = 0; i < n; ++i)
+                            R->keff[m->maxa[i] - 1] += a0 * (1 - am_) * R->sm[i] / (1 - af_);
+                        double t1 = now_s();
+                        orc_skyfact(n, m->maxa, R->keff, R->ssd, 0, &det);
+                        if (R->out) { R->out->t_factor += now_s() - t1; R->out->n_factor++; }
+                        for (long i = 0; i < n; ++i)
+                            R->dd[i] = R->r[i] + R->sm[i] * ((1 - am_) * (R->vc_i[i] * a2 +
+                                                                        R->ac_i[i] * a3) -
+                                                             am_ * R->ac_i[i]) / (1 - af_);
+                        t1 = now_s();
+                        orc_skysolve(n, m->maxa, R->keff, R->dd);
+                        if (R->out) { R->out->t_solve += now_s() - t1; R->out->n_solve++; }
+                    }
+                    for (long i = 0; i < n; ++i) {
+                        if (itecnt > 0) R->dd[i] = -R->dd[i];
+                        R->d_temp[i] += R->dd[i];
+                        R->f_ip[i] = R->f_temp[i];
+                        R->f_temp[i] = 0;
+                    }
+                    for (long i = 0; i < n; ++i) {
+                        R->uc_i[i] = R->d_temp[i];
+                        R->ac_i[i] = (R->uc_i[i] - R->um[i]) * a0 - a2 * R->vm[i] - a3 * R->am[i];
+                        R->vc_i[i] = R->vm[i] + a6 * R->am[i] + a7 * R->ac_i[i];
+                    }
+                    /* updatc + forces with the working lengths (main.c:3656-3680) */
+                    t0 = now_s();
+                    R->be->updatc(R->be->ctx, d, R->x_temp, R->x_ip, R->xfr_temp, R->dd,
+                                  R->defllen_i, R->deffarea_i, R->defslen_i, m->offset, m->osflag,
+                                  m->auxpt, R->c1_i, R->c2_i, R->c3_i, m->minc, m->jcode);
+                    if (d->NE_TR > 0)
+                        R->be->forces_tr(R->be->ctx, d, R->f_temp, R->ef_i, m->emod, m->carea, llen,
+                                         R->defllen_i, m->yield, R->c1_i, R->c2_i, R->c3_i,
+                                         m->mcode);
+                    int frc_fr = 0, frc_sh = 0;
+                    if (d->NE_FR > 0)
+                        frc_fr = R->be->forces_fr(R->be->ctx, d, R->f_temp, R->ef_ip, R->ef_i,
+                                                  R->efFE_ip, R->efFE_i, R->dd, m, llen,
+                                                  R->defllen_ip, R->c1_ip, R->c2_ip, R->c3_ip,
+                                                  R->c1_i, R->c2_i, R->c3_i, &dlpf, itecnt);
+                    if (d->NE_SH > 0)
+                        frc_sh = R->be->forces_sh(R->be->ctx, d, R->f_temp, R->ef_ip, R->ef_i,
+                                                  R->dd, R->x_temp, R->x_ip, m->emod, m->nu,
+                                                  m->xlocal, m->thick, R->farea_w,
+                                                  R->deffarea_ip, R->slen_w, R->defslen_ip,
+                                                  m->yield, R->c1_ip, R->c2_ip, R->c3_ip,
+                                                  R->c1_i, R->c2_i, R->c3_i, m->minc, m->mcode,
+                                                  m->jcode);
+                    if (R->out) R->out->t_forces += now_s() - t0;
+                    frcchk = frc_fr || frc_sh;
+                    for (long i = 0; i < n; ++i) R->dyn[i] = R->qtot[i] - R->sm[i] * R->ac_i[i];
+                    if (itecnt == 0) {
+                        intener1 = 0;
+                        for (long i = 0; i < n; ++i) intener1 += R->dd[i] * (R->dyn[i] - R->fp[i]);
+                    }
+                    if (orc_test(n, R->d_temp, R->dd, R->f_temp, R->fp, R->dyn, R->f_ip, intener1,
+                                 &convchk, m->toldisp, m->tolforc, m->tolener))
+                        FAIL(1, "zero norm in convergence test");
+                    if (getenv("ORC_VERBOSE"))
+                        fprintf(stderr, "NM k=%ld it=%d convchk=%d |dd|=%g\n", k, itecnt, convchk,
+                                sqrt(vdot(n, R->dd, R->dd)));
+                    shift_ip(R, 1);
+                    if (R->out) R->out->n_newton++;
+                    itecnt++;
+                } while (convchk != 0 && frcchk == 0 && itecnt <= m->itemax);
+                if ((convchk != 0 || frcchk) && subcnt <= m->submax) {
+                    if (lpf == m->lpfmax) break;          /* main.c:3783-3788 */
+                    if (dlpfp == m->dlpfmin) break;
+                    dlpf = dlpfp / 2;
+                    if (dlpf < m->dlpfmin) dlpf = m->dlpfmin;
+                    lpf = lpf - dlpfp + dlpf;
+                    subcnt++;
+                    solcnt = 0;
+                } else if (subcnt > m->submax) {
+                    break;
+                } else {
+                    solcnt++;
+                    subcnt = 0;
+                    if (solcnt >= m->solmin) { dlpf *= 2; solcnt = 0; }
+                    lpf += dlpf;
+                }
+            } while (lpf <= m->lpfmax);
+            if (convchk != 0) {
+                /* halve the time increment (main.c:3826-3846) */
+                ddt = ddt / 2;
+                dt_temp = ddt * dt;
+                if (tsflag == 0) sub_dt = ddt;
+                else if (tsflag == 1) sub_dt = sub_dt - ddt;
+            } else {
+                CP(R->d, R->d_temp, n);
+                CP(R->f, R->f_temp, n);
+                CP(R->uc, R->uc_i, n); CP(R->vc, R->vc_i, n); CP(R->ac, R->ac_i, n);
+                commit(R);
+            }
+            if (!frcchk) {
+                if (ddt < 1) {
+                    if (sub_dt <= 1) { sub_dt = sub_dt + ddt; tsflag = 1; }
+                    if (sub_dt > 1) { ddt = ddt * 2; tsflag = 2; }
+                } else if (ddt == 1) {
+                    tsflag = 2;
+                }
+            }
+        } while (ddt >= 0.0001 && tsflag != 2);
+        if (convchk != 0 || frcchk)
+            FAIL(3, "minimum time increment interval reached without convergence");
+        record(R, (double)k * dt, itecnt, R->d, R->f);
+        if (R->out && R->out->nsteps >= R->out->max_steps) break;
+        ++k;
+    } while (k < ntst);
+    return 0;
 }
 
 int orc_run(orc_model *m, const orc_backend *b, orc_path *out)

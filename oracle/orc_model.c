@@ -64,7 +64,7 @@ void orc_free(orc_model *m)
                     m->istrong, m->iweak, m->ipolar, m->iwarp, m->zstrong, m->zweak,
                     m->offset, m->auxpt, m->xfr, m->osflag, m->mendrel, m->nu, m->thick,
                     m->dens_sh, m->xlocal, m->farea, m->slength, m->c1, m->c2, m->c3, m->q,
-                    m->pinpt, m->um, m->vm, m->am};
+                    m->pinpt, m->um, m->vm, m->am, m->dens};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
     free(m);
 }
@@ -366,6 +366,7 @@ orc_model *orc_open(const char *path, char *err, int errlen)
             long j = tk_long(&t);
             int r[4];
             for (int k = 0; k < 4; ++k) r[k] = (int)tk_long(&t);
+            (void)r;
             if (t.bad) FAIL("truncated frame releases");
             if (j == 0) break;
             FAIL("frame end releases are not on this path");
@@ -414,6 +415,12 @@ orc_model *orc_open(const char *path, char *err, int errlen)
         local_coords(p1, p2, p3, lx, ly, m->xlocal + e * 3);
     }
     if (t.bad) FAIL("truncated element properties");
+    /* the reference keeps one density array that prop_tr, prop_fr and prop_sh
+       each fill from index 0 (truss.c:66, frame.c:61, shell.c:61) */
+    m->dens = zalloc((size_t)nprop, sizeof(double));
+    for (long e = 0; e < ntr; ++e) m->dens[e] = m->dens_tr[e];
+    for (long e = 0; e < nfr; ++e) m->dens[e] = m->dens_fr[e];
+    for (long e = 0; e < nsh; ++e) m->dens[e] = m->dens_sh[e];
 
     /* load vectors (model.c:1285-1606) */
     m->q = zalloc((size_t)neq, sizeof(double));
@@ -498,7 +505,7 @@ orc_model *orc_open(const char *path, char *err, int errlen)
         m->imagmax = (int)tk_long(&t); m->negmax = (int)tk_long(&t);
         m->toldisp = tk_dbl(&t); m->tolforc = tk_dbl(&t); m->tolener = tk_dbl(&t);
     } else if (d->ALGFLAG == 5) {
-        m->numopt = (int)tk_long(&t); m->spectrds = (int)tk_long(&t);
+        m->numopt = (int)tk_dbl(&t); m->spectrds = tk_dbl(&t);
         m->lpfmax = tk_dbl(&t); m->lpf0 = tk_dbl(&t); m->dlpf0 = tk_dbl(&t);
         m->dlpfmax = tk_dbl(&t); m->dlpfmin = tk_dbl(&t);
         m->itemax = (int)tk_long(&t); m->submax = (int)tk_long(&t); m->solmin = (int)tk_long(&t);

@@ -124,6 +124,69 @@ static void w_updatc(void *ctx, const orc_dims *d, double *x_temp, double *x_ip,
            (int *)osflag, U(auxpt), c1_i, c2_i, c3_i, UL(minc), UL(jcode));
 }
 
+static int *zeros_int(long n)
+{
+    static int *buf = NULL;
+    static long cap = 0;
+    if (n > cap) { free(buf); buf = calloc((size_t)n, sizeof(int)); cap = n; }
+    return buf;
+}
+
+static void w_stiff_fr(void *ctx, const orc_dims *d, double *ss, const orc_model *m,
+                       const double *llength, const double *defllen_ip, const double *c1_ip,
+                       const double *c2_ip, const double *c3_ip, const double *ef_ip,
+                       const double *efFE_ip)
+{
+    (void)ctx;
+    ref_set_dims(d);
+    stiff_fr(ss, m->emod, m->gmod, m->carea, m->offset, m->osflag, U(llength), U(defllen_ip),
+             m->istrong, m->iweak, m->ipolar, m->iwarp, zeros_int(d->NE_FR * 2 + 2), m->yield,
+             m->zstrong, m->zweak, U(c1_ip), U(c2_ip), U(c3_ip), U(ef_ip), U(efFE_ip),
+             m->mendrel, m->maxa, m->mcode);
+}
+
+static int w_forces_fr(void *ctx, const orc_dims *d, double *f, double *ef_ip, double *ef_i,
+                       const double *efFE_ip, double *efFE_i, const double *dd,
+                       const orc_model *m, const double *llength, const double *defllen_ip,
+                       const double *c1_ip, const double *c2_ip, const double *c3_ip,
+                       const double *c1_i, const double *c2_i, const double *c3_i,
+                       double *dlpf, int itecnt)
+{
+    (void)ctx;
+    ref_set_dims(d);
+    int it = itecnt;
+    return forces_fr(f, ef_ip, ef_i, zeros(d->NE_FR * 14 + 1), U(efFE_ip), efFE_i,
+                     zeros_int(d->NE_FR * 2 + 2), U(dd), m->emod, m->gmod, m->carea, m->offset,
+                     m->osflag, U(llength), U(defllen_ip), m->istrong, m->iweak, m->ipolar,
+                     m->iwarp, m->yield, m->zstrong, m->zweak, U(c1_ip), U(c2_ip), U(c3_ip),
+                     U(c1_i), U(c2_i), U(c3_i), m->mendrel, m->mcode, dlpf, &it);
+}
+
+static void w_mass_tr(void *ctx, const orc_dims *d, double *sm, const orc_model *m,
+                      const double *x, double *llength, const double *dens)
+{
+    (void)ctx;
+    ref_set_dims(d);
+    mass_tr(sm, m->carea, llength, U(dens), U(x), m->minc, m->mcode, zeros(1));
+}
+
+static void w_mass_fr(void *ctx, const orc_dims *d, double *sm, const orc_model *m,
+                      const double *x, double *llength, double *xfr, const double *dens)
+{
+    (void)ctx;
+    ref_set_dims(d);
+    mass_fr(sm, m->carea, llength, m->istrong, m->iweak, m->ipolar, m->iwarp, U(dens),
+            m->osflag, m->offset, U(x), xfr, m->minc, m->mcode, zeros(1));
+}
+
+static void w_mass_sh(void *ctx, const orc_dims *d, double *sm, const orc_model *m,
+                      const double *x, double *farea, double *slength, const double *dens)
+{
+    (void)ctx;
+    ref_set_dims(d);
+    mass_sh(sm, m->carea, U(dens), m->thick, farea, slength, U(x), m->minc, m->mcode, zeros(1));
+}
+
 void ref_fill_backend(orc_backend *b)
 {
     b->ctx = NULL;
@@ -132,6 +195,11 @@ void ref_fill_backend(orc_backend *b)
     b->stiff_sh = w_stiff_sh;
     b->forces_sh = w_forces_sh;
     b->updatc = w_updatc;
+    b->stiff_fr = w_stiff_fr;
+    b->forces_fr = w_forces_fr;
+    b->mass_tr = w_mass_tr;
+    b->mass_fr = w_mass_fr;
+    b->mass_sh = w_mass_sh;
 }
 
 /* Flat entry points with the reference signatures, sizes passed explicitly. */
@@ -353,6 +421,13 @@ int ref_get_double(const char *name, double *out, long n)
     else if (!strcmp(name, "pinpt")) src = RM.pinpt;
     else if (!strcmp(name, "auxpt")) src = RM.auxpt;
     else if (!strcmp(name, "gmod")) src = RM.gmod;
+    else if (!strcmp(name, "dens")) src = RM.dens;
+    else if (!strcmp(name, "xfr")) src = RM.xfr;
+    else if (!strcmp(name, "offset")) src = RM.offset;
+    else if (!strcmp(name, "istrong")) src = RM.istrong;
+    else if (!strcmp(name, "iweak")) src = RM.iweak;
+    else if (!strcmp(name, "ipolar")) src = RM.ipolar;
+    else if (!strcmp(name, "iwarp")) src = RM.iwarp;
     if (!src || !RM.open) return 1;
     memcpy(out, src, sizeof(double) * (size_t)n);
     return 0;

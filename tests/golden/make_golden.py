@@ -106,6 +106,27 @@ def path_fixture(path, steps):
     return {"lpf": r["lpf"], "itecnt": r["itecnt"], "d": r["d"]}
 
 
+def frame_fixtures(tmp):
+    """Frame paths on the reference's frame/mass routines: a tower under
+    harmonic lateral load (nonlinear Newmark) and a girder with member-end
+    offsets under static load (Newton-Raphson)."""
+    tw = inputs.space_frame(100, kind="tower", amp=1.5e5, steps=30)
+    ptw = tw.write(os.path.join(tmp, "tower.txt"))
+    r = O.OracleModel(ptw).run(30, backend="ref", keep_f=True)
+    assert r["rc"] == 0 and r["nsteps"] == 30, r["msg"]
+    np.savez_compressed(os.path.join(HERE, "frame_tower_newmark.npz"), model=np.array(tw.text),
+                        path_lpf=r["lpf"], path_itecnt=r["itecnt"], path_d=r["d"], path_f=r["f"])
+    gd = inputs.space_frame(60, kind="grid", algflag=1, amp=2.5e4, tol=(1e-10, 1e-8, 1e-12))
+    txt = gd.text.replace("0,0,0,0,0,0,0\n", "3,0.05,-0.02,0.01,0.03,0.04,-0.05\n"
+                          "7,0.1,0.0,0.0,0.0,0.0,0.1\n0,0,0,0,0,0,0\n", 1)
+    pgd = os.path.join(tmp, "girder.txt")
+    open(pgd, "w").write(txt)
+    r = O.OracleModel(pgd).run(25, backend="ref", keep_f=True)
+    assert r["rc"] == 0, r["msg"]
+    np.savez_compressed(os.path.join(HERE, "frame_girder_offsets_nr.npz"), model=np.array(txt),
+                        path_lpf=r["lpf"], path_itecnt=r["itecnt"], path_d=r["d"], path_f=r["f"])
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("oracle/_ref is not built (make -C oracle with /root/reference present)")
@@ -126,6 +147,7 @@ def main():
     _, _, s = model_arrays(s5a)
     s.update({"path_" + k: v for k, v in path_fixture(s5a, 50).items()})
     np.savez_compressed(os.path.join(HERE, "sample_5a.npz"), **s)
+    frame_fixtures(tmp)
     for f in os.listdir(tmp):
         os.remove(os.path.join(tmp, f))
     os.rmdir(tmp)
