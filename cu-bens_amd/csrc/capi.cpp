@@ -85,6 +85,10 @@ bool state_ref(cb_model* h, const std::string& name, double** p, long* n) {
     if (name == "x_temp") { *p = d.x[cb::G_I]; *n = d.nj * 3; return true; }
     if (name == "x_ip") { *p = d.x[cb::G_IP]; *n = d.nj * 3; return true; }
     if (name == "x") { *p = d.x[cb::G_BASE]; *n = d.nj * 3; return true; }
+    // working element geometry the kernels read (the mass routines rewrite it)
+    if (name == "llength") { *p = d.llen; *n = d.ntr + d.nfr; return true; }
+    if (name == "farea") { *p = d.farea; *n = d.nsh; return true; }
+    if (name == "slength") { *p = d.slength; *n = d.nsh * 3; return true; }
     if (base.size() > 3 && base.compare(base.size() - 3, 3, "_ip") == 0) {
         g = cb::G_IP;
         base = base.substr(0, base.size() - 3);
@@ -236,6 +240,24 @@ int cb_internal_force(cb_model* h, int gen_x, const double* dd_host, double* f_h
     CB_CHECK(hipStreamSynchronize(h->d.s));
     (void)hipFree(dd);
     (void)hipFree(f);
+    return 0;
+    GUARD_END
+}
+
+int cb_mass(cb_model* h, double* sm_host) {
+    if (!h || !sm_host) return fail(-1, "null argument");
+    GUARD_BEGIN
+    const long n = h->m.neq;
+    double* smv = cb::dalloc<double>(size_t(n));
+    try {
+        analysis(h, nullptr).mass(smv);
+        cb::d2h(sm_host, smv, size_t(n), h->d.s);
+        CB_CHECK(hipStreamSynchronize(h->d.s));
+    } catch (...) {
+        (void)hipFree(smv);
+        throw;
+    }
+    (void)hipFree(smv);
     return 0;
     GUARD_END
 }

@@ -47,9 +47,13 @@ struct Dev {
     int32_t *minc = nullptr, *mcode = nullptr, *jcode = nullptr, *eq_of_slot = nullptr;
     int32_t *rowptr = nullptr, *colidx = nullptr, *diagblk = nullptr;
     int32_t *blk_tr = nullptr, *blk_sh = nullptr, *col_el_tr = nullptr, *col_el_sh = nullptr;
-    std::vector<int32_t> col_ptr_tr, col_ptr_sh;
-    // properties
-    double *emod = nullptr, *carea = nullptr, *llen = nullptr;
+    int32_t *blk_fr = nullptr, *col_el_fr = nullptr;
+    std::vector<int32_t> col_ptr_tr, col_ptr_sh, col_ptr_fr;
+    // properties; llen, farea and slength are the working copies the element
+    // kernels read (the Newmark mass routines overwrite them)
+    double *emod = nullptr, *carea = nullptr, *llen = nullptr, *dens = nullptr;
+    double *gmod = nullptr, *istrong = nullptr, *iweak = nullptr, *ipolar = nullptr, *iwarp = nullptr;
+    double *auxpt = nullptr;
     double *nu = nullptr, *thick = nullptr, *xlocal = nullptr, *farea = nullptr, *slength = nullptr;
     // state
     double *x[3] = {nullptr, nullptr, nullptr};  // [G_BASE]=x, [G_I]=x_temp, [G_IP]=x_ip
@@ -94,6 +98,31 @@ void launch_forces(Dev& d, int gx, double* f, const double* dd);
 void launch_updatc(Dev& d, int gx, const double* dd);
 // copy generations of per-element state: dst <- src (ef optional)
 void copy_gen(Dev& d, int dst, int src, bool with_ef, bool with_x);
+
+// frame members (frame.hip): tangent of generation g into K, internal forces
+// (ef_ip -> ef_i, triads _ip and _i), member lengths/triads after updatc
+void launch_frame_tangent(Dev& d, int g);
+void launch_frame_forces(Dev& d, double* f, const double* dd);
+void launch_frame_update(Dev& d, int gx);
+
+// ---- nonlinear Newmark (newmark.hip) -------------------------------------------
+// sm += lumped masses of element type (0 truss, 1 frame, 2 shell) at the
+// converged coordinates; overwrites the working lengths / areas.
+void launch_mass(Dev& d, double* sm, int type);
+// K(slot diagonal) += cnum * sm[eq] / cden
+void add_mass_diag(Dev& d, const double* sm, double cnum, double cden);
+
+enum NewmarkVec { NM_PREDICT0 = 0, NM_PREDICT, NM_CORRECT, NM_REFF, NM_UPDATE, NM_DYN };
+struct NewmarkArgs {
+    const double* P = nullptr;   // load histories [NEQ][ld]
+    long ld = 0, k = 0;
+    double sub_dt = 1, lpf = 1, am = 0, af = 0, a0 = 0, a2 = 0, a3 = 0, a6 = 0, a7 = 0;
+    int flip = 0;
+    double *qtot = nullptr, *r = nullptr, *ft = nullptr, *fip = nullptr, *dt = nullptr, *dd = nullptr;
+    double *sm = nullptr, *ac = nullptr, *vc = nullptr, *out = nullptr;
+    const double *um = nullptr, *vm = nullptr, *amv = nullptr;
+};
+void newmark_vec(Dev& d, NewmarkVec op, const NewmarkArgs& a);
 
 // ---- vectors -----------------------------------------------------------------
 // out[k] = sum_i a_k[i]*b_k[i], k < n (n <= 8), deterministic two-level reduction.

@@ -50,7 +50,8 @@ Analysis::Analysis(Model& m, Dev& d, const RunOptions& o) : m_(m), d_(d), o_(o) 
 }
 
 Analysis::~Analysis() {
-    double* vs[] = {q, dv, dp, dpp, dt, dd, ddq, ddr, f, fp, ft, fip, r, tmp, eb, ex};
+    double* vs[] = {q, dv, dp, dpp, dt, dd, ddq, ddr, f, fp, ft, fip, r, tmp, eb, ex,
+                    sm, qtot, uc, vc, ac, vci, aci, um, vm, amv, dyn, hist};
     for (double* p : vs)
         if (p) (void)hipFree(p);
     b64_.release();
@@ -229,6 +230,7 @@ int Analysis::run(RunStats& st) {
     try {
         if (m_.algflag == 1 || m_.algflag == 2) rc = run_nr();
         else if (m_.algflag == 3) rc = run_msal();
+        else if (m_.algflag == 5) rc = run_newmark();
         else { rc = 2; st.msg = "driver not available for this ALGFLAG"; }
     } catch (const DeviceError& e) {
         rc = 4;
@@ -558,6 +560,157 @@ int Analysis::run_msal() {
             if (long(lpf.size()) >= o_.max_steps) return 0;
         }
     }
+    return 0;
+}
+
+// ---------------------------------------------------------------- Newmark
+// Lumped masses of every element type at the converged coordinates.
+int Analysis::mass(double* smv) {
+    const int types = (m_.ntr > 0) + (m_.nfr > 0) + (m_.nsh > 0);
+    if (types > 1)
+        throw DeviceError("lumped masses of mixed element types index the per-type arrays from 0 "
+                          "in the reference (frame.c:1356-1368); single-type models only");
+    vfill(d_, smv, 0.0, m_.neq);
+    if (m_.ntr) launch_mass(d_, smv, 0);
+    if (m_.nfr) launch_mass(d_, smv, 1);
+    if (m_.nsh) launch_mass(d_, smv, 2);
+    return 0;
+}
+
+// main.c:3305-3960 with solve.c:139-245, 469-533, without prescribed support
+// motion; one path row per time step (time, itecnt, d).  The initial
+// conditions read from the input are discarded here as in the reference
+// (main.c:3443-3445).
+int Analysis::run_newmark() {
+    const long n = m_.neq;
+    if (!sm) {
+        double** vs[] = {&sm, &qtot, &uc, &vc, &ac, &vci, &aci, &um, &vm, &amv, &dyn};
+        for (double** p : vs) *p = dalloc<double>(size_t(n));
+        hist = dalloc<double>(size_t(n * m_.ntstpsinpt));
+        h2d(hist, m_.pinpt.data(), size_t(n * m_.ntstpsinpt), d_.s);
+    }
+    for (double* v : {sm, qtot, uc, vc, ac, vci, aci, um, vm, amv, dyn})
+        CB_CHECK(hipMemsetAsync(v, 0, sizeof(double) * size_t(n), d_.s));
+    double am_ = 0, af_ = 0;
+    const double rho = m_.spectrds;
+    switch (m_.numopt) {
+    case 0:
+        if (rho != 1) FAIL(1, "invalid spectral radius for Newmark without dissipation");
+        break;
+    case 1: am_ = (2 * rho - 1) / (rho + 1); af_ = rho / (rho + 1); break;
+    case 2: af_ = (1 - rho) / (1 + rho); break;
+    case 3: am_ = (rho - 1) / (rho + 1); break;
+    default: FAIL(1, "unknown Newmark dissipation option");
+    }
+    const double alpha = std::pow(1 - am_ + af_, 2) / 4, delta = 0.5 - am_ + af_;
+    const double dtime = m_.dt;
+    const long ntst = long(m_.ttot / dtime + 1);
+    if (ntst > m_.ntstpsinpt) FAIL(1, "time steps exceed the load history");
+    double ddt = 1, sub_dt = 1, dt_temp = dtime;
+    int tsflag = 0, convchk = 1, itecnt_ = 0, det = 0;
+    double intener1 = 0;
+    copy_gen(d_, G_I, G_BASE, true, true);
+    copy_gen(d_, G_IP, G_BASE, true, true);
+    NewmarkArgs A;
+    A.P = hist; A.ld = m_.ntstpsinpt; A.am = am_; A.af = af_;
+    A.qtot = qtot; A.r = r; A.ft = ft; A.fip = fip; A.dt = dt; A.dd = dd; A.sm = sm;
+    A.ac = aci; A.vc = vci; A.um = um; A.vm = vm; A.amv = amv;
+    long k = 0;
+    do {
+        dt_temp = ddt * dtime;
+        sub_dt = ddt;
+        tsflag = 0;
+        do {
+            A.a0 = 1 / (alpha * std::pow(dt_temp, 2));
+            A.a2 = 1 / (alpha * dt_temp);
+            A.a3 = 1 / (2 * alpha) - 1;
+            A.a6 = dt_temp * (1 - delta);
+            A.a7 = delta * dt_temp;
+            int solcnt = 0, subcnt = 0;
+            double lam = m_.lpf0, dlam = m_.dlpf0, dlamp;
+            do {
+                if (lam > m_.lpfmax) lam = m_.lpfmax;
+                vcopy(d_, fp, f, n);
+                vcopy(d_, dt, dv, n);
+                vcopy(d_, ft, f, n);
+                dlamp = dlam;
+                reset_from_base();
+                vcopy(d_, um, uc, n); vcopy(d_, vm, vc, n); vcopy(d_, amv, ac, n);
+                vcopy(d_, vci, vc, n); vcopy(d_, aci, ac, n);
+                itecnt_ = 0;
+                do {
+                    A.k = k; A.sub_dt = sub_dt; A.lpf = lam;
+                    if (itecnt_ == 0) newmark_vec(d_, k == 0 ? NM_PREDICT0 : NM_PREDICT, A);
+                    else newmark_vec(d_, NM_CORRECT, A);
+                    {
+                        Timer t(&st_->t_tangent, d_.s);
+                        launch_tangent(d_, G_I, G_IP, false);
+                        mass(sm);
+                        add_mass_diag(d_, sm, A.a0 * (1 - am_), 1 - af_);
+                    }
+                    // solve() discards the factor status (solve.c:208-212, 535)
+                    factor_current(&det);
+                    A.out = tmp;
+                    newmark_vec(d_, NM_REFF, A);
+                    linear_solve(tmp, dd);
+                    A.flip = itecnt_ > 0;
+                    newmark_vec(d_, NM_UPDATE, A);
+                    update_and_forces(G_I, ft, dd);
+                    A.out = dyn;
+                    newmark_vec(d_, NM_DYN, A);
+                    double s[5];
+                    test_sums(d_, dd, dt, dyn, 1.0, ft, fp, fip, n, s);
+                    if (itecnt_ == 0) {
+                        double s2[5];
+                        test_sums(d_, dd, dt, dyn, 1.0, ft, fp, fp, n, s2);
+                        intener1 = s2[4];
+                    }
+                    if (test_decide(s, intener1, m_.toldisp, m_.tolforc, m_.tolener, &convchk))
+                        FAIL(1, "zero norm in convergence test");
+                    shift_ip(true);
+                    itecnt_++;
+                    st_->newton++;
+                } while (convchk != 0 && itecnt_ <= m_.itemax);
+                if (convchk != 0 && subcnt <= m_.submax) {
+                    if (lam == m_.lpfmax) break;          // main.c:3783-3788
+                    if (dlamp == m_.dlpfmin) break;
+                    dlam = dlamp / 2;
+                    if (dlam < m_.dlpfmin) dlam = m_.dlpfmin;
+                    lam = lam - dlamp + dlam;
+                    subcnt++;
+                    solcnt = 0;
+                } else if (subcnt > m_.submax) {
+                    break;
+                } else {
+                    solcnt++;
+                    subcnt = 0;
+                    if (solcnt >= m_.solmin) { dlam *= 2; solcnt = 0; }
+                    lam += dlam;
+                }
+            } while (lam <= m_.lpfmax);
+            if (convchk != 0) {
+                ddt = ddt / 2;                          // main.c:3826-3846
+                dt_temp = ddt * dtime;
+                if (tsflag == 0) sub_dt = ddt;
+                else if (tsflag == 1) sub_dt = sub_dt - ddt;
+            } else {
+                vcopy(d_, dv, dt, n);
+                vcopy(d_, f, ft, n);
+                vcopy(d_, uc, dt, n); vcopy(d_, vc, vci, n); vcopy(d_, ac, aci, n);
+                commit();
+            }
+            if (ddt < 1) {
+                if (sub_dt <= 1) { sub_dt = sub_dt + ddt; tsflag = 1; }
+                if (sub_dt > 1) { ddt = ddt * 2; tsflag = 2; }
+            } else if (ddt == 1) {
+                tsflag = 2;
+            }
+        } while (ddt >= 0.0001 && tsflag != 2);
+        if (convchk != 0) FAIL(3, "minimum time increment interval reached without convergence");
+        record(double(k) * dtime, itecnt_, dv);
+        if (long(lpf.size()) >= o_.max_steps) return 0;
+        ++k;
+    } while (k < ntst);
     return 0;
 }
 

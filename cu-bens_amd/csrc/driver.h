@@ -1,6 +1,7 @@
 // Nonlinear drivers on the device: Newton-Raphson / modified NR load control
-// (main.c:1800-2157) and the modified spherical arc-length method
-// (main.c:2158-3141), over the device element kernels and linear solvers.
+// (main.c:1800-2157), the modified spherical arc-length method
+// (main.c:2158-3141) and nonlinear Newmark time integration (main.c:3305-3960),
+// over the device element kernels and linear solvers.
 #pragma once
 
 #include <functional>
@@ -68,6 +69,7 @@ public:
     // one-shot operations used by the C ABI
     int tangent_and_factor(int gx, int g, int* det);
     int factor_only(int* det);   // factor the tangent already in d.K
+    int mass(double* sm_dev);    // lumped masses at the converged coordinates
     int linear_solve(const double* rhs, double* x);   // device NEQ vectors
     double* dev_vec(int which);
     BandLDLT<double>& band64() { return b64_; }
@@ -91,6 +93,11 @@ private:
 
     int run_nr();
     int run_msal();
+    int run_newmark();
+    // nonlinear Newmark state (allocated on first use)
+    double *sm = nullptr, *qtot = nullptr, *uc = nullptr, *vc = nullptr, *ac = nullptr;
+    double *vci = nullptr, *aci = nullptr, *um = nullptr, *vm = nullptr, *amv = nullptr;
+    double *dyn = nullptr, *hist = nullptr;
     void record(double lam, int ite, const double* dvec);
     void update_and_forces(int gx, double* fout, const double* ddv);
     int factor_current(int* det);

@@ -1,0 +1,334 @@
+// Corotational 3-D frame element (elastic + geometric, ANAFLAG 2) on the device.
+//
+// One thread per member.  The 14x14 local matrix is never stored: every entry
+// is a closed-form expression of ~40 per-member scalars (stiffe_fr /
+// stiffg_fr, frame.c:354-579), evaluated where the fully unrolled transform
+// loops ask for it, so the compiler folds the zero pattern away.  The device
+// layout carries 6 slots per joint; warping DOFs must be fixed (every member
+// then has 12 active DOF).  Members of one colour share no joint, so the
+// block adds and force scatters of one launch never collide.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "dev.h"
+
+namespace cb {
+namespace {
+
+constexpr int TPB = 128;
+inline int nblk(long n, int tpb = TPB) { return int((n + tpb - 1) / tpb); }
+
+// Scalars of the local tangent of one member: elastic terms from E, G, A, L0,
+// I_strong, I_weak, J, C_w and geometric terms from the previous iterate's end
+// forces (P = f[7], My1 = f[4], Mz1 = f[5], T = f[10], My2 = f[11],
+// Mz2 = f[12]) over its deformed length L.
+struct FrameCoef {
+    double ax, s12, s6, s4, s2, w12, w6, w4, w2, t33, t63, t66, t136;
+    double gP, pv, pt, pr, pw, p10, pq, pl, pw30, tl, th;
+    double a, b, my1, my2, c, e, mz1, mz2;
+    double x34, x35, x311, x312, x410, x510, x1011, x1012;
+    double w46, w56, w413, w513, w611, w612, w1113, w1213;
+};
+
+__device__ inline void frame_coef(FrameCoef& k, double E, double G, double A, double L0,
+                                  double Is, double Iw, double J, double Cw, const double* f,
+                                  double L, bool geometric) {
+    const double L2 = L0 * L0, L3 = L0 * L0 * L0;
+    k.ax = E * A / L0;
+    k.s12 = 12 * E * Is / L3; k.s6 = 6 * E * Is / L2; k.s4 = 4 * E * Is / L0; k.s2 = 2 * E * Is / L0;
+    k.w12 = 12 * E * Iw / L3; k.w6 = 6 * E * Iw / L2; k.w4 = 4 * E * Iw / L0; k.w2 = 2 * E * Iw / L0;
+    k.t33 = 6 * G * J / (5 * L0) + 12 * E * Cw / L3;
+    k.t63 = G * J / 10 + 6 * E * Cw / L2;
+    k.t66 = 2 * G * J * L0 / 15 + 4 * E * Cw / L0;
+    k.t136 = -(G * J * L0 / 30 - 2 * E * Cw / L0);
+    const double P = geometric ? f[7] : 0.0, My1 = geometric ? f[4] : 0.0;
+    const double Mz1 = geometric ? f[5] : 0.0, T = geometric ? f[10] : 0.0;
+    const double My2 = geometric ? f[11] : 0.0, Mz2 = geometric ? f[12] : 0.0;
+    k.gP = P / L; k.pv = 6 * P / (5 * L); k.pt = 6 * P * J / (5 * A * L);
+    k.pr = 2 * P * L / 15; k.pw = 2 * P * J / (15 * A); k.p10 = P / 10;
+    k.pq = P * J / (10 * A); k.pl = P * L / 30; k.pw30 = P * J / (30 * A);
+    k.tl = T / L; k.th = T / 2;
+    k.a = (11 * My1 - My2) / (10 * L); k.b = (My1 - 11 * My2) / (10 * L);
+    k.my1 = My1 / 10; k.my2 = My2 / 10;
+    k.c = (11 * Mz1 - Mz2) / (10 * L); k.e = (Mz1 - 11 * Mz2) / (10 * L);
+    k.mz1 = Mz1 / 10; k.mz2 = Mz2 / 10;
+    k.x34 = -(2 * Mz1 - Mz2) / 5; k.x35 = (2 * My1 - My2) / 5;
+    k.x311 = -(2 * Mz1 + Mz2) / 10; k.x312 = (2 * My1 + My2) / 10;
+    k.x410 = -(Mz1 + 2 * Mz2) / 10; k.x510 = (My1 + 2 * My2) / 10;
+    k.x1011 = (Mz1 - 2 * Mz2) / 5; k.x1012 = -(My1 - 2 * My2) / 5;
+    k.w46 = -(3 * Mz1 - Mz2) * L / 30; k.w56 = (3 * My1 - My2) * L / 30;
+    k.w413 = Mz1 * L / 30; k.w513 = -My1 * L / 30;
+    k.w611 = -Mz2 * L / 30; k.w612 = My2 * L / 30;
+    k.w1113 = -(Mz1 - 3 * Mz2) * L / 30; k.w1213 = (My1 - 3 * My2) * L / 30;
+}
+
+// Entry (i, j) of the symmetric local tangent, DOF order per end u, v, w,
+// theta_x, theta_y, theta_z, warping (end 2 at +7).
+__device__ __forceinline__ double kf(const FrameCoef& k, int i, int j) {
+    if (i > j) { int t = i; i = j; j = t; }
+    switch (i * 14 + j) {
+    case 0 * 14 + 0: case 7 * 14 + 7: return k.ax + k.gP;
+    case 0 * 14 + 7: return -k.ax + -k.gP;
+    case 1 * 14 + 1: case 8 * 14 + 8: return k.s12 + k.pv;
+    case 1 * 14 + 8: return -k.s12 + -k.pv;
+    case 2 * 14 + 2: case 9 * 14 + 9: return k.w12 + k.pv;
+    case 2 * 14 + 9: return -k.w12 + -k.pv;
+    case 3 * 14 + 3: case 10 * 14 + 10: return k.t33 + k.pt;
+    case 3 * 14 + 10: return -k.t33 + -k.pt;
+    case 4 * 14 + 4: case 11 * 14 + 11: return k.w4 + k.pr;
+    case 5 * 14 + 5: case 12 * 14 + 12: return k.s4 + k.pr;
+    case 6 * 14 + 6: case 13 * 14 + 13: return k.t66 + k.pw;
+    case 1 * 14 + 5: case 1 * 14 + 12: return k.s6 + k.p10;
+    case 5 * 14 + 8: case 8 * 14 + 12: return -k.s6 + -k.p10;
+    case 4 * 14 + 9: case 9 * 14 + 11: return k.w6 + k.p10;
+    case 2 * 14 + 4: case 2 * 14 + 11: return -k.w6 + -k.p10;
+    case 3 * 14 + 6: case 3 * 14 + 13: return k.t63 + k.pq;
+    case 6 * 14 + 10: case 10 * 14 + 13: return -k.t63 + -k.pq;
+    case 5 * 14 + 12: return k.s2 + -k.pl;
+    case 4 * 14 + 11: return k.w2 + -k.pl;
+    case 6 * 14 + 13: return k.t136 + -k.pw30;
+    case 1 * 14 + 4: case 2 * 14 + 5: case 8 * 14 + 11: case 9 * 14 + 12: return k.tl;
+    case 1 * 14 + 11: case 2 * 14 + 12: case 4 * 14 + 8: case 5 * 14 + 9: return -k.tl;
+    case 4 * 14 + 12: return k.th;
+    case 5 * 14 + 11: return -k.th;
+    case 1 * 14 + 3: return k.a;
+    case 3 * 14 + 8: return -k.a;
+    case 8 * 14 + 10: return k.b;
+    case 1 * 14 + 10: return -k.b;
+    case 1 * 14 + 6: return k.my1;
+    case 6 * 14 + 8: return -k.my1;
+    case 8 * 14 + 13: return k.my2;
+    case 1 * 14 + 13: return -k.my2;
+    case 2 * 14 + 3: return k.c;
+    case 3 * 14 + 9: return -k.c;
+    case 9 * 14 + 10: return k.e;
+    case 2 * 14 + 10: return -k.e;
+    case 2 * 14 + 6: return k.mz1;
+    case 6 * 14 + 9: return -k.mz1;
+    case 9 * 14 + 13: return k.mz2;
+    case 2 * 14 + 13: return -k.mz2;
+    case 3 * 14 + 4: return k.x34;
+    case 3 * 14 + 5: return k.x35;
+    case 3 * 14 + 11: return k.x311;
+    case 3 * 14 + 12: return k.x312;
+    case 4 * 14 + 10: return k.x410;
+    case 5 * 14 + 10: return k.x510;
+    case 10 * 14 + 11: return k.x1011;
+    case 10 * 14 + 12: return k.x1012;
+    case 4 * 14 + 6: return k.w46;
+    case 5 * 14 + 6: return k.w56;
+    case 4 * 14 + 13: return k.w413;
+    case 5 * 14 + 13: return k.w513;
+    case 6 * 14 + 11: return k.w611;
+    case 6 * 14 + 12: return k.w612;
+    case 11 * 14 + 13: return k.w1113;
+    case 12 * 14 + 13: return k.w1213;
+    default: return 0.0;
+    }
+}
+
+// local index of slot s (0..5) of end a
+__device__ __forceinline__ constexpr int lidx(int a, int s) { return 7 * a + s; }
+
+__device__ inline void load_triad(const double* c1, const double* c2, const double* c3, long t,
+                                  double R[3][3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        R[0][k] = c1[t + k];
+        R[1][k] = c2[t + k];
+        R[2][k] = c3[t + k];
+    }
+}
+
+// local index of element of triple t (0: u1, 1: theta1, 2: u2, 3: theta2)
+__device__ __forceinline__ constexpr int tidx(int t, int m) { return lidx(t / 2, (t % 2) * 3 + m); }
+
+// stiff_fr (frame.c:226-349) for the members of one colour: T^T k T with
+// T = blockdiag(R, R, 1, R, R, 1), one 3x3 sub-block (pair of triples) at a
+// time, added into the four joint blocks of the member.
+__global__ void __launch_bounds__(TPB) k_frame_tangent(
+    const int32_t* __restrict__ els, int n, const int32_t* __restrict__ blk,
+    const double* __restrict__ c1, const double* __restrict__ c2, const double* __restrict__ c3,
+    const double* __restrict__ emod, const double* __restrict__ gmod, const double* __restrict__ carea,
+    const double* __restrict__ llen, const double* __restrict__ istrong, const double* __restrict__ iweak,
+    const double* __restrict__ ipolar, const double* __restrict__ iwarp,
+    const double* __restrict__ defllen, const double* __restrict__ ef, double* __restrict__ K,
+    long ntr, int geometric) {
+    int it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= n) return;
+    const long e = els[it], p = ntr + e;
+    const double* fe = ef + ntr * 2 + e * 14;
+    FrameCoef k;
+    frame_coef(k, emod[p], gmod[e], carea[p], llen[p], istrong[e], iweak[e], ipolar[e], iwarp[e], fe,
+               defllen[p], geometric != 0);
+    double R[3][3];
+    load_triad(c1, c2, c3, ntr + e * 3, R);
+    int32_t bk[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bk[q] = blk[e * 4 + q];
+#pragma unroll
+    for (int t1 = 0; t1 < 4; ++t1)
+#pragma unroll
+        for (int t2 = 0; t2 < 4; ++t2) {
+            double tmp[3][3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int l = 0; l < 3; ++l)
+                    tmp[r][l] = R[0][r] * kf(k, tidx(t1, 0), tidx(t2, l)) +
+                                R[1][r] * kf(k, tidx(t1, 1), tidx(t2, l)) +
+                                R[2][r] * kf(k, tidx(t1, 2), tidx(t2, l));
+            double* Kb = K + (long)bk[(t1 / 2) * 2 + t2 / 2] * 36 + ((t1 % 2) * 3) * 6 + (t2 % 2) * 3;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int s = 0; s < 3; ++s)
+                    Kb[r * 6 + s] += tmp[r][0] * R[0][s] + tmp[r][1] * R[1][s] + tmp[r][2] * R[2][s];
+        }
+}
+
+// forces_fr, elastic branch (frame.c:902-1096, 1273-1309):
+//   dl = T_ip DD;  def = k(ef_ip) dl;  ef_i = T_i T_ip^T (ef_ip + def);  f += T_i^T ef_i,
+// one output triple at a time.
+__global__ void __launch_bounds__(TPB) k_frame_forces(
+    const int32_t* __restrict__ els, int n, const int32_t* __restrict__ mcode,
+    const double* __restrict__ dd, const double* __restrict__ c1p, const double* __restrict__ c2p,
+    const double* __restrict__ c3p, const double* __restrict__ c1c, const double* __restrict__ c2c,
+    const double* __restrict__ c3c, const double* __restrict__ emod, const double* __restrict__ gmod,
+    const double* __restrict__ carea, const double* __restrict__ llen,
+    const double* __restrict__ istrong, const double* __restrict__ iweak,
+    const double* __restrict__ ipolar, const double* __restrict__ iwarp,
+    const double* __restrict__ defllen_ip, const double* __restrict__ ef_ip,
+    double* __restrict__ ef_i, double* __restrict__ f, long ntr, int geometric) {
+    int it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= n) return;
+    const long e = els[it], p = ntr + e;
+    const int32_t* mc = mcode + ntr * 6 + e * 14;
+    const double* fe = ef_ip + ntr * 2 + e * 14;
+    FrameCoef k;
+    frame_coef(k, emod[p], gmod[e], carea[p], llen[p], istrong[e], iweak[e], ipolar[e], iwarp[e], fe,
+               defllen_ip[p], geometric != 0);
+    double Rp[3][3], Ri[3][3];
+    load_triad(c1p, c2p, c3p, ntr + e * 3, Rp);
+    load_triad(c1c, c2c, c3c, ntr + e * 3, Ri);
+    // local increments of the four triples; warping increments stay 0
+    double dl[4][3];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        double D[3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            int32_t q = mc[tidx(t, m)];
+            D[m] = q ? dd[q - 1] : 0.0;
+        }
+#pragma unroll
+        for (int m = 0; m < 3; ++m) dl[t][m] = Rp[m][0] * D[0] + Rp[m][1] * D[1] + Rp[m][2] * D[2];
+    }
+    double Q[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Q[r][c] = Ri[r][0] * Rp[c][0] + Ri[r][1] * Rp[c][1] + Ri[r][2] * Rp[c][2];
+    double* ei = ef_i + ntr * 2 + e * 14;
+#pragma unroll
+    for (int to = 0; to < 4; ++to) {
+        double v[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            double s = 0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int m = 0; m < 3; ++m) s += kf(k, tidx(to, r), tidx(t, m)) * dl[t][m];
+            v[r] = fe[tidx(to, r)] + s;
+        }
+        double g[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            g[r] = Q[r][0] * v[0] + Q[r][1] * v[1] + Q[r][2] * v[2];
+            ei[tidx(to, r)] = g[r];
+        }
+        // T_i^T ef_i on this triple, scattered by mcode
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            int32_t q = mc[tidx(to, m)];
+            if (q) f[q - 1] += Ri[0][m] * g[0] + Ri[1][m] * g[1] + Ri[2][m] * g[2];
+        }
+    }
+    // warping rows (bimoments): T_i T_ip^T is 1 there
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+        const int i = 7 * w + 6;
+        double s = 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int m = 0; m < 3; ++m) s += kf(k, i, tidx(t, m)) * dl[t][m];
+        ei[i] = fe[i] + s;
+    }
+}
+
+// misc.c:122-144: member length and triad from the current chord and the
+// fixed input auxiliary point.
+__global__ void k_update_frame(const int32_t* __restrict__ minc, const double* __restrict__ x,
+                               const double* __restrict__ auxpt, double* __restrict__ defllen,
+                               double* __restrict__ c1, double* __restrict__ c2,
+                               double* __restrict__ c3, long n, long ntr) {
+    long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const double* a = x + (minc[ntr * 2 + e * 2] - 1) * 3;
+    const double* b = x + (minc[ntr * 2 + e * 2 + 1] - 1) * 3;
+    double el[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    double L = sqrt(el[0] * el[0] + el[1] * el[1] + el[2] * el[2]);
+    defllen[ntr + e] = L;
+    double lx[3] = {el[0] / L, el[1] / L, el[2] / L};
+    double tp[3] = {auxpt[e * 3] - a[0], auxpt[e * 3 + 1] - a[1], auxpt[e * 3 + 2] - a[2]};
+    double lz[3] = {lx[1] * tp[2] - lx[2] * tp[1], lx[2] * tp[0] - lx[0] * tp[2], lx[0] * tp[1] - lx[1] * tp[0]};
+    double nz = sqrt(lz[0] * lz[0] + lz[1] * lz[1] + lz[2] * lz[2]);
+    lz[0] /= nz; lz[1] /= nz; lz[2] /= nz;
+    double ly[3] = {lz[1] * lx[2] - lz[2] * lx[1], lz[2] * lx[0] - lz[0] * lx[2], lz[0] * lx[1] - lz[1] * lx[0]};
+    double ny = sqrt(ly[0] * ly[0] + ly[1] * ly[1] + ly[2] * ly[2]);
+    const long t = ntr + e * 3;
+    for (int k = 0; k < 3; ++k) {
+        c1[t + k] = lx[k];
+        c2[t + k] = ly[k] / ny;
+        c3[t + k] = lz[k];
+    }
+}
+
+}  // namespace
+
+void launch_frame_tangent(Dev& d, int g) {
+    for (size_t c = 0; c + 1 < d.col_ptr_fr.size(); ++c) {
+        int n = d.col_ptr_fr[c + 1] - d.col_ptr_fr[c];
+        if (!n) continue;
+        hipLaunchKernelGGL(k_frame_tangent, dim3(nblk(n)), dim3(TPB), 0, d.s,
+                           d.col_el_fr + d.col_ptr_fr[c], n, d.blk_fr, d.c1[g], d.c2[g], d.c3[g],
+                           d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp,
+                           d.defllen[g], d.ef[g], d.K, d.ntr, int(d.anaflag == 2));
+    }
+    CB_CHECK(hipGetLastError());
+}
+
+void launch_frame_forces(Dev& d, double* f, const double* dd) {
+    for (size_t c = 0; c + 1 < d.col_ptr_fr.size(); ++c) {
+        int n = d.col_ptr_fr[c + 1] - d.col_ptr_fr[c];
+        if (!n) continue;
+        hipLaunchKernelGGL(k_frame_forces, dim3(nblk(n)), dim3(TPB), 0, d.s,
+                           d.col_el_fr + d.col_ptr_fr[c], n, d.mcode, dd, d.c1[G_IP], d.c2[G_IP],
+                           d.c3[G_IP], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.emod, d.gmod, d.carea,
+                           d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[G_IP],
+                           d.ef[G_IP], d.ef[G_I], f, d.ntr, int(d.anaflag == 2));
+    }
+    CB_CHECK(hipGetLastError());
+}
+
+void launch_frame_update(Dev& d, int gx) {
+    if (!d.nfr) return;
+    hipLaunchKernelGGL(k_update_frame, dim3(nblk(d.nfr, 256)), dim3(256), 0, d.s, d.minc, d.x[gx],
+                       d.auxpt, d.defllen[G_I], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.nfr, d.ntr);
+    CB_CHECK(hipGetLastError());
+}
+
+}  // namespace cb

@@ -608,6 +608,7 @@ void launch_tangent(Dev& d, int gx, int g, bool want_fp32) {
                            d.col_el_tr + d.col_ptr_tr[c], n, d.blk_tr, d.c1[g], d.c2[g], d.c3[g],
                            d.emod, d.carea, d.llen, d.defllen[g], d.ef[g], d.K, d.bs);
     }
+    if (d.nfr) launch_frame_tangent(d, g);
     for (size_t c = 0; c + 1 < d.col_ptr_sh.size(); ++c) {
         int n = d.col_ptr_sh[c + 1] - d.col_ptr_sh[c];
         if (!n) continue;
@@ -639,6 +640,7 @@ void launch_forces(Dev& d, int gx, double* f, const double* dd) {
                            d.col_el_tr + d.col_ptr_tr[c], n, d.mcode, d.c1[G_I], d.c2[G_I], d.c3[G_I],
                            d.emod, d.carea, d.llen, d.defllen[G_I], d.ef[G_I], f);
     }
+    if (d.nfr) launch_frame_forces(d, f, dd);
     for (size_t c = 0; c + 1 < d.col_ptr_sh.size(); ++c) {
         int n = d.col_ptr_sh[c + 1] - d.col_ptr_sh[c];
         if (!n) continue;
@@ -657,6 +659,7 @@ void launch_updatc(Dev& d, int gx, const double* dd) {
     if (d.ntr)
         hipLaunchKernelGGL(k_update_truss, dim3(nblk(d.ntr)), dim3(TPB), 0, d.s, d.minc, d.x[gx],
                            d.defllen[G_I], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.ntr);
+    launch_frame_update(d, gx);
     if (d.nsh)
         hipLaunchKernelGGL(k_update_shell, dim3(nblk(d.nsh)), dim3(TPB), 0, d.s, d.minc, d.x[gx],
                            d.defslen[G_I], d.deffarea[G_I], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.nsh,
@@ -754,13 +757,23 @@ void dev_create(Dev& d, const Model& m) {
     d.diagblk = upi(m.diagblk);
     d.blk_tr = upi(m.blk_tr);
     d.blk_sh = upi(m.blk_sh);
+    d.blk_fr = upi(m.blk_fr);
     d.col_el_tr = upi(m.col_el_tr);
     d.col_el_sh = upi(m.col_el_sh);
+    d.col_el_fr = upi(m.col_el_fr);
     d.col_ptr_tr = m.col_ptr_tr;
     d.col_ptr_sh = m.col_ptr_sh;
+    d.col_ptr_fr = m.col_ptr_fr;
     d.emod = upd(m.emod);
     d.carea = upd(m.carea);
     d.llen = upd(m.llength);
+    d.dens = upd(m.dens);
+    d.gmod = upd(m.gmod);
+    d.istrong = upd(m.istrong);
+    d.iweak = upd(m.iweak);
+    d.ipolar = upd(m.ipolar);
+    d.iwarp = upd(m.iwarp);
+    d.auxpt = upd(m.auxpt);
     d.nu = upd(m.nu);
     d.thick = upd(m.thick);
     d.xlocal = upd(m.xlocal);
@@ -796,13 +809,17 @@ void dev_reset_state(Dev& d, const Model& m) {
         h2d(d.deffarea[g], m.farea.data(), size_t(m.nsh), d.s);
         h2d(d.defslen[g], m.slength.data(), size_t(m.nsh * 3), d.s);
     }
+    h2d(d.llen, m.llength.data(), m.llength.size(), d.s);
+    h2d(d.farea, m.farea.data(), m.farea.size(), d.s);
+    h2d(d.slength, m.slength.data(), m.slength.size(), d.s);
     CB_CHECK(hipStreamSynchronize(d.s));
 }
 
 void dev_destroy(Dev& d) {
     void* ps[] = {d.minc, d.mcode, d.jcode, d.eq_of_slot, d.rowptr, d.colidx, d.diagblk, d.blk_tr,
-                  d.blk_sh, d.col_el_tr, d.col_el_sh, d.emod, d.carea, d.llen, d.nu, d.thick,
-                  d.xlocal, d.farea, d.slength, d.K, d.K32, d.red};
+                  d.blk_sh, d.blk_fr, d.col_el_tr, d.col_el_sh, d.col_el_fr, d.emod, d.carea,
+                  d.llen, d.dens, d.gmod, d.istrong, d.iweak, d.ipolar, d.iwarp, d.auxpt, d.nu,
+                  d.thick, d.xlocal, d.farea, d.slength, d.K, d.K32, d.red};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     for (int g = 0; g < 3; ++g) {

@@ -200,6 +200,9 @@ bool load_model(const char* path, Model& m, std::string& err) {
         m.jcode[(j - 1) * 7 + (k - 1)] = 0;
     }
     number_dofs(m);
+    // the device layout carries 6 slots per joint: warping DOFs must be fixed
+    for (long j = 0; j < nj; ++j)
+        if (m.jcode[j * 7 + 6] != 0) FAIL("free warping DOFs are not carried by the device layout");
     if (m.algflag > 3) {
         long j = t.l(), k = t.l();
         if (j != 0 || k != 0) FAIL("prescribed displacement DOFs (NBC > 0) are not on this path");
@@ -241,6 +244,7 @@ bool load_model(const char* path, Model& m, std::string& err) {
             if (t.bad()) FAIL("truncated frame offsets");
             if (j == 0) break;
             if (j < 1 || j > nfr) FAIL("frame offset member out of range");
+            FAIL("frame member-end offsets are not carried by the device kernels");
             m.osflag[j - 1] = 1;
             std::copy(o, o + 6, &m.offset[(j - 1) * 6]);
         }
@@ -311,6 +315,12 @@ bool load_model(const char* path, Model& m, std::string& err) {
         m.xlocal[e * 3 + 1] = lx[0] * e31[0] + lx[1] * e31[1] + lx[2] * e31[2];
         m.xlocal[e * 3 + 2] = ly[0] * e31[0] + ly[1] * e31[1] + ly[2] * e31[2];
     }
+    // one density array, filled by the truss, frame and shell readers in turn,
+    // each from index 0 (truss.c:66, frame.c:61, shell.c:61)
+    m.dens.assign(size_t(np > 0 ? np : 1), 0.0);
+    for (long e = 0; e < ntr; ++e) m.dens[e] = m.dens_tr[e];
+    for (long e = 0; e < nfr; ++e) m.dens[e] = m.dens_fr[e];
+    for (long e = 0; e < nsh; ++e) m.dens[e] = m.dens_sh[e];
     if (t.bad()) FAIL("truncated element properties");
 
     const long neq = m.neq;
@@ -377,7 +387,7 @@ bool load_model(const char* path, Model& m, std::string& err) {
         if (!any) FAIL("no forces present in input file");
     }
     if (m.algflag == 1 || m.algflag == 2 || m.algflag == 5) {
-        if (m.algflag == 5) { m.numopt = int(t.l()); m.spectrds = int(t.l()); }
+        if (m.algflag == 5) { m.numopt = int(t.d()); m.spectrds = t.d(); }
         m.lpfmax = t.d(); m.lpf0 = t.d(); m.dlpf0 = t.d(); m.dlpfmax = t.d(); m.dlpfmin = t.d();
         m.itemax = int(t.l()); m.submax = int(t.l()); m.solmin = int(t.l());
         m.toldisp = t.d(); m.tolforc = t.d(); m.tolener = t.d();

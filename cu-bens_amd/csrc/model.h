@@ -24,6 +24,7 @@ struct Model {
     std::vector<double> offset, auxpt, xfr;
     std::vector<int> osflag;
     std::vector<double> nu, thick, dens_sh, xlocal, farea, slength;
+    std::vector<double> dens;        // the reference's one density array (see load_model)
     std::vector<double> c1, c2, c3;  // [NE_TR | NE_FR*3 | NE_SH*3]
     std::vector<double> q;           // NEQ reference load
     // dynamic input
@@ -36,7 +37,8 @@ struct Model {
     double toldisp = 0, tolforc = 0, tolener = 0;
     double alpha = 0, psi_thresh = 0, dkimax = 0, dk = 0;
     long dkdof = 0;  // 0-based
-    int numopt = 0, spectrds = 0;
+    int numopt = 0;
+    double spectrds = 1;
 
     // ---- device-facing pattern --------------------------------------------
     int bs = 6;                       // DOF slots per joint in the solver layout

@@ -84,6 +84,7 @@ ABI = {
     "cb_get_path": (C.c_int, [C.c_void_p, PD, PI, PD]),
     "cb_get_step_stats": (C.c_int, [C.c_void_p, PD, PD, PL]),
     "cb_get_kernel_times": (C.c_int, [C.c_void_p, C.c_int, C.c_int, PD, PL]),
+    "cb_mass": (C.c_int, [C.c_void_p, PD]),
     "cb_write_results2": (C.c_int, [C.c_void_p, C.c_char_p]),
 }
 
@@ -187,6 +188,12 @@ class GpuModel:
         f = np.zeros(self.neq)
         _check(lib().cb_internal_force(self.h, gen_x, _p(dd), _p(f)))
         return f
+
+    def mass(self) -> np.ndarray:
+        """Lumped diagonal masses at the converged coordinates (cb_mass)."""
+        sm = np.zeros(self.neq)
+        _check(lib().cb_mass(self.h, _p(sm)))
+        return sm
 
     def update_geometry(self, dd: np.ndarray, gen_x: int = GEN_I):
         dd = np.ascontiguousarray(dd, np.float64)

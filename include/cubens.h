@@ -103,6 +103,13 @@ int cb_get_tangent(cb_model *m, double *vals);
  * writes ef_i; coordinates of generation gen_x. */
 int cb_internal_force(cb_model *m, int gen_x, const double *dd_host, double *f_host);
 
+/* Lumped diagonal masses at the converged coordinates into sm[NEQ]: replaces
+ * mass_tr (truss.c:381-441), mass_fr (frame.c:1314-1393) and mass_sh
+ * (shell.c:1505-1591).  Like them it rewrites the working member lengths,
+ * face areas and side lengths the element kernels read ("llength", "farea",
+ * "slength" in cb_state_get).  Models of one element type only. */
+int cb_mass(cb_model *m, double *sm);
+
 /* Geometry update: replaces updatc (misc.c:71): x[gen_x] += dd, x_ip <- old,
  * lengths/areas/triads of generation _i. */
 int cb_update_geometry(cb_model *m, int gen_x, const double *dd_host);

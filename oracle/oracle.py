@@ -170,6 +170,9 @@ class OracleModel:
         self.carea = _arr(m.carea, self.ntr + self.nfr)
         self.llength = _arr(m.llength, self.ntr + self.nfr)
         self.dkdof = m.dkdof
+        self.auxpt = _arr(m.auxpt, self.nfr * 3)
+        self.offset = _arr(m.offset, self.nfr * 6)
+        self.osflag = _arr(m.osflag, self.nfr, np.int32)
 
     def close(self):
         if self.h:
@@ -326,6 +329,10 @@ def updatc(om: "OracleModel", x_temp, dd):
     off = np.zeros(max(1, nfr * 6))
     osf = np.zeros(max(1, nfr), np.int32)
     aux = np.zeros(max(1, nfr * 3))
+    if nfr:
+        off[:nfr * 6] = om.offset
+        osf[:nfr] = om.osflag
+        aux[:nfr * 3] = om.auxpt
     L.orc_updatc(C.byref(om.dims), ptr(x), ptr(xip), ptr(xfr), ptr(np.ascontiguousarray(dd)),
                  ptr(defllen), ptr(deffarea), ptr(defslen), ptr(off), ptr(osf, PI), ptr(aux),
                  ptr(c1), ptr(c2), ptr(c3), ptr(om.minc, PL), ptr(om.jcode, PL))
@@ -367,3 +374,57 @@ def frame_kg(f, L, A, Ip) -> np.ndarray:
     L_.orc_frame_kg.argtypes = [PD, C.c_double, C.c_double, C.c_double, PD]
     L_.orc_frame_kg(ptr(f), L, A, Ip, ptr(k))
     return k
+
+
+def _c(a):
+    return ptr(np.ascontiguousarray(a, np.float64))
+
+
+def stiff_fr(om: "OracleModel", llength, defllen_ip, c1, c2, c3, ef_ip) -> np.ndarray:
+    """Skyline array of stiff_fr at the given state (ss zeroed first)."""
+    L_ = lib()
+    L_.orc_stiff_fr.argtypes = [C.POINTER(Dims), PD, C.c_void_p] + [PD] * 7
+    ss = np.zeros(om.lss)
+    efFE = np.zeros(max(om.nfr * 14, 1))
+    L_.orc_stiff_fr(C.byref(om.dims), ptr(ss), om.h, _c(llength), _c(defllen_ip), _c(c1), _c(c2),
+                    _c(c3), _c(ef_ip), ptr(efFE))
+    return ss
+
+
+def forces_fr(om: "OracleModel", dd, llength, defllen_ip, c_ip, c_i, ef_ip):
+    """forces_fr: returns (f, ef_i)."""
+    L_ = lib()
+    L_.orc_forces_fr.argtypes = [C.POINTER(Dims), PD, PD, PD, PD, PD, PD, C.c_void_p] + [PD] * 8
+    L_.orc_forces_fr.restype = C.c_int
+    f = np.zeros(om.neq)
+    efi = np.zeros(len(ef_ip))
+    efFE = np.zeros(max(om.nfr * 14, 1))
+    efFE_i = np.zeros_like(efFE)
+    L_.orc_forces_fr(C.byref(om.dims), ptr(f), _c(ef_ip), ptr(efi), ptr(efFE), ptr(efFE_i), _c(dd), om.h,
+                     _c(llength), _c(defllen_ip), *[_c(v) for v in (*c_ip, *c_i)])
+    return f, efi
+
+
+def mass(om: "OracleModel", x):
+    """Lumped masses of every element type at coordinates x; returns
+    (sm, llength, farea, slength) with the rewritten working geometry."""
+    L_ = lib()
+    D = C.POINTER(Dims)
+    L_.orc_mass_tr.argtypes = [D, PD, C.c_void_p, PD, PD, PD]
+    L_.orc_mass_fr.argtypes = [D, PD, C.c_void_p, PD, PD, PD, PD]
+    L_.orc_mass_sh.argtypes = [D, PD, C.c_void_p, PD, PD, PD, PD]
+    m = C.cast(om.h, C.POINTER(Model)).contents
+    dens = _arr(m.dens, om.ntr + om.nfr + om.nsh)
+    sm = np.zeros(om.neq)
+    llen = om.llength.copy() if len(om.llength) else np.zeros(1)
+    farea = om.farea.copy() if om.nsh else np.zeros(1)
+    slen = om.slength.copy() if om.nsh else np.zeros(1)
+    xfr = np.zeros(max(om.nfr * 6, 1))
+    x = np.ascontiguousarray(x, np.float64)
+    if om.ntr:
+        L_.orc_mass_tr(C.byref(om.dims), ptr(sm), om.h, ptr(x), ptr(llen), ptr(dens))
+    if om.nfr:
+        L_.orc_mass_fr(C.byref(om.dims), ptr(sm), om.h, ptr(x), ptr(llen), ptr(xfr), ptr(dens))
+    if om.nsh:
+        L_.orc_mass_sh(C.byref(om.dims), ptr(sm), om.h, ptr(x), ptr(farea), ptr(slen), ptr(dens))
+    return sm, llen, farea, slen
