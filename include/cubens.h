@@ -53,6 +53,11 @@ typedef struct {
     int kernel_timing;   /* bitmask of CB_KT_* kernel families timed with HIP events */
 } cb_options;
 
+/* Generations of the per-element state (base = last converged, _i = current
+ * iterate, _ip = previous iterate), as the reference's x/x_temp/x_ip, c1/c1_i/
+ * c1_ip, ef/ef_i/ef_ip arrays (main.c:497-510, 760-790). */
+enum { CB_GEN_BASE = 0, CB_GEN_I = 1, CB_GEN_IP = 2 };
+
 /* Kernel families of the band solver (timed with HIP events on their stream). */
 enum { CB_KT_DIAG = 0, CB_KT_PANEL = 1, CB_KT_UPDATE = 2, CB_KT_FWD = 3, CB_KT_BWD = 4 };
 
