@@ -1,12 +1,14 @@
 // C ABI over the host model, device state and drivers (include/cubens.h).
 #include "../../include/cubens.h"
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
 #include <string>
 
 #include "driver.h"
+#include "learned.h"
 #include "model.h"
 
 namespace {
@@ -20,9 +22,34 @@ int fail(int code, const std::string& msg) {
 
 }  // namespace
 
+struct cb_learned {
+    std::unique_ptr<cb::Learned> net;
+    // online training state
+    bool training = false;
+    int probes = 4;
+    float lr = 1e-3f;
+    uint64_t rng = 0x243F6A8885A308D3ull;
+    std::vector<double> losses;
+    uint64_t bound = 0;   // uid of the model the training buffers are sized for
+    std::vector<double*> rb, ub;   // slot-layout pairs
+    double *zq = nullptr, *uq = nullptr;   // NEQ-layout probe and its solution
+    void release_buffers() {
+        for (double* p : rb) (void)hipFree(p);
+        for (double* p : ub) (void)hipFree(p);
+        rb.clear();
+        ub.clear();
+        if (zq) (void)hipFree(zq);
+        if (uq) (void)hipFree(uq);
+        zq = uq = nullptr;
+        bound = 0;
+    }
+    ~cb_learned() { release_buffers(); }
+};
+
 struct cb_model {
     cb::Model m;
     cb::Dev d;
+    cb_learned* learned = nullptr;
     std::unique_ptr<cb::Analysis> an;
     cb_options opt{};
     std::vector<double> path_lpf, path_d, step_t, step_judge;
@@ -66,12 +93,73 @@ bool same_opts(const cb_options& a, const cb_options& b) {
            a.kernel_timing == b.kernel_timing;
 }
 
+double normal01(uint64_t& st) {
+    auto u01 = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    };
+    const double a = u01(), b = u01();
+    return std::sqrt(-2.0 * std::log(a)) * std::cos(6.283185307179586 * b);
+}
+
+// slot-layout buffers of the training batch for model h
+void training_buffers(cb_learned* L, cb_model* h) {
+    if (L->bound == h->d.uid && int(L->rb.size()) == L->probes + 1) return;
+    L->release_buffers();
+    for (int b = 0; b <= L->probes; ++b) {
+        L->rb.push_back(cb::dalloc<double>(size_t(h->d.nslot)));
+        L->ub.push_back(cb::dalloc<double>(size_t(h->d.nslot)));
+    }
+    L->zq = cb::dalloc<double>(size_t(h->m.neq));
+    L->uq = cb::dalloc<double>(size_t(h->m.neq));
+    L->bound = h->d.uid;
+}
+
+// One online training step on a (rhs, solution) pair of the FP64 band solve
+// plus random probes solved with the same factor.
+void train_on_pair(cb_model* h, cb::Dev& d, const double* rhs, const double* sol) {
+    cb_learned* L = h->learned;
+    training_buffers(L, h);
+    cb::Learned& net = *L->net;
+    net.setup(d);
+    cb::to_slots(d, L->rb[0], rhs);
+    cb::to_slots(d, L->ub[0], sol);
+    std::vector<double> z(size_t(h->m.neq));
+    for (int b = 1; b <= L->probes; ++b) {
+        for (double& v : z) v = normal01(L->rng);
+        cb::h2d(L->zq, z.data(), z.size(), d.s);
+        h->an->band64().solve(L->zq, L->uq);
+        cb::to_slots(d, L->rb[size_t(b)], L->zq);
+        cb::to_slots(d, L->ub[size_t(b)], L->uq);
+    }
+    std::vector<const double*> r(L->rb.begin(), L->rb.end()), u(L->ub.begin(), L->ub.end());
+    const double loss = net.loss_grad(d, r.data(), u.data(), L->probes + 1);
+    net.adam_step(L->lr);
+    L->losses.push_back(loss);
+}
+
+void install_hooks(cb_model* h) {
+    cb::PrecHook hk;
+    if (cb_learned* L = h->learned) {
+        hk.setup = [L](cb::Dev& d) { L->net->setup(d); };
+        hk.apply = [L](cb::Dev& d, const double* v, double* z) { L->net->apply(d, v, z); };
+        if (L->training)
+            hk.observe = [h](cb::Dev& d, const double* rhs, const double* sol) { train_on_pair(h, d, rhs, sol); };
+    }
+    h->an->learned = hk;
+}
+
 cb::Analysis& analysis(cb_model* h, const cb_options* o) {
     cb_options want = o ? *o : (h->an ? h->opt : default_options());
     if (!h->an || !same_opts(want, h->opt)) {
         h->an.reset();
         h->opt = want;
         h->an.reset(new cb::Analysis(h->m, h->d, to_run(want)));
+        install_hooks(h);
     }
     return *h->an;
 }
@@ -392,6 +480,143 @@ int cb_write_results2(const cb_model* h, const char* path) {
     }
     std::fclose(fp);
     return 0;
+}
+
+
+/* ---- learned preconditioner ---------------------------------------------- */
+int cb_learned_create(unsigned seed, cb_learned** out) {
+    if (!out) return fail(-1, "null argument");
+    GUARD_BEGIN
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd < 1)
+        return fail(-2, "no HIP device available (the learned preconditioner runs on the GPU only)");
+    auto* L = new cb_learned();
+    L->net.reset(new cb::Learned(seed));
+    *out = L;
+    return 0;
+    GUARD_END
+}
+
+void cb_learned_free(cb_learned* L) { delete L; }
+
+long cb_learned_nparams(const cb_learned* L) { return L ? L->net->nparams() : -1; }
+
+int cb_learned_get_params(const cb_learned* L, float* p) {
+    if (!L || !p) return fail(-1, "null argument");
+    GUARD_BEGIN
+    L->net->get_params(p);
+    return 0;
+    GUARD_END
+}
+
+int cb_learned_set_params(cb_learned* L, const float* p) {
+    if (!L || !p) return fail(-1, "null argument");
+    GUARD_BEGIN
+    L->net->set_params(p);
+    return 0;
+    GUARD_END
+}
+
+int cb_learned_set_objective(cb_learned* L, int kind) {
+    if (!L) return fail(-1, "null argument");
+    if (kind < 0 || kind > 2) return fail(-1, "objective must be 0, 1 or 2");
+    L->net->set_objective(kind);
+    return 0;
+}
+
+int cb_learned_attach(cb_model* h, cb_learned* L) {
+    if (!h) return fail(-1, "null handle");
+    h->learned = L;
+    if (h->an) install_hooks(h);
+    return 0;
+}
+
+int cb_learned_train(cb_learned* L, cb_model* h, int max_steps, int probes, float lr, cb_train_stats* st) {
+    if (!L || !h) return fail(-1, "null argument");
+    if (probes < 0 || probes > 15) return fail(-1, "probes must be 0..15");
+    GUARD_BEGIN
+    cb_learned* keep = h->learned;
+    h->learned = L;
+    L->training = true;
+    L->probes = probes;
+    L->lr = lr;
+    L->losses.clear();
+    cb_options o = default_options();
+    o.solver = CB_SOLVER_BAND64;
+    o.judge = 0;
+    o.max_steps = max_steps;
+    o.keep_d = 0;
+    h->an.reset();
+    int rc = 0;
+    std::string msg;
+    try {
+        cb::Analysis& an = analysis(h, &o);
+        cb::RunStats rs;
+        rc = an.run(rs);
+        msg = rs.msg;
+    } catch (...) {
+        L->training = false;
+        h->learned = keep;
+        h->an.reset();
+        throw;
+    }
+    L->training = false;
+    h->learned = keep;
+    h->an.reset();
+    if (st) {
+        st->steps = long(L->losses.size());
+        st->loss_first = L->losses.empty() ? 0 : L->losses.front();
+        st->loss_last = L->losses.empty() ? 0 : L->losses.back();
+        double m = 0;
+        const size_t k = std::min<size_t>(L->losses.size(), 10);
+        for (size_t i = L->losses.size() - k; i < L->losses.size(); ++i) m += L->losses[i];
+        st->loss_recent = k ? m / double(k) : 0;
+    }
+    if (rc) return fail(rc, "training analysis stopped: " + msg);
+    return 0;
+    GUARD_END
+}
+
+int cb_learned_loss_grad(cb_learned* L, cb_model* h, const double* r_host, const double* u_host, int nvec,
+                         double* loss, float* grad) {
+    if (!L || !h || !r_host || !u_host) return fail(-1, "null argument");
+    if (nvec < 1 || nvec > 16) return fail(-1, "nvec must be 1..16");
+    GUARD_BEGIN
+    cb::Dev& d = h->d;
+    const long n = h->m.neq;
+    cb::Learned& net = *L->net;
+    net.setup(d);
+    std::vector<double*> rs, us;
+    double* tq = cb::dalloc<double>(size_t(n));
+    for (int b = 0; b < nvec; ++b) {
+        rs.push_back(cb::dalloc<double>(size_t(d.nslot)));
+        us.push_back(cb::dalloc<double>(size_t(d.nslot)));
+        cb::h2d(tq, r_host + long(b) * n, size_t(n), d.s);
+        cb::to_slots(d, rs.back(), tq);
+        cb::h2d(tq, u_host + long(b) * n, size_t(n), d.s);
+        cb::to_slots(d, us.back(), tq);
+        CB_CHECK(hipStreamSynchronize(d.s));
+    }
+    std::vector<const double*> rc(rs.begin(), rs.end()), uc(us.begin(), us.end());
+    const double l = net.loss_grad(d, rc.data(), uc.data(), nvec);
+    if (loss) *loss = l;
+    if (grad) net.get_grad(grad);
+    for (double* p : rs) (void)hipFree(p);
+    for (double* p : us) (void)hipFree(p);
+    (void)hipFree(tq);
+    return 0;
+    GUARD_END
+}
+
+int cb_learned_blocks(cb_learned* L, cb_model* h, float* gd, float* go, long* nv, long* ne) {
+    if (!L || !h) return fail(-1, "null argument");
+    GUARD_BEGIN
+    L->net->setup(h->d);
+    if (nv) *nv = L->net->nv();
+    if (ne) *ne = L->net->ne();
+    L->net->get_blocks(gd, go);
+    return 0;
+    GUARD_END
 }
 
 }  // extern "C"

@@ -39,6 +39,7 @@ struct DeviceError {
 enum Gen { G_BASE = 0, G_I = 1, G_IP = 2 };
 
 struct Dev {
+    uint64_t uid = 0;   // unique per dev_create: caches keyed on a model use this, never its address
     hipStream_t s = nullptr;
     long nj = 0, ntr = 0, nfr = 0, nsh = 0, neq = 0, nslot = 0, nnzb = 0;
     int bs = 6;

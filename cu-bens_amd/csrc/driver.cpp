@@ -120,6 +120,7 @@ int Analysis::factor_current(int* det) {
         }
         st_->factors++;
     }
+    if (o_.solver == SOLVER_BAND64 && learned.after_factor) learned.after_factor(d_);
     if (o_.solver != SOLVER_BAND64 && o_.judge) {
         // exact FP64 pivots for det/psi (and the direct-solve fallback)
         Timer t(&st_->t_judge, d_.s);
@@ -178,6 +179,7 @@ int Analysis::linear_solve(const double* rhs, double* x) {
     int rc = 0;
     if (o_.solver == SOLVER_BAND64) {
         b64_.solve(rhs, x);
+        if (learned.observe) learned.observe(d_, rhs, x);
     } else {
         std::function<void(const double*, double*)> prec;
         if (o_.solver == SOLVER_BAND32_FGMRES) {

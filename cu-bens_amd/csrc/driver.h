@@ -42,11 +42,15 @@ struct RunStats {
     std::string msg;
 };
 
-// Hook for the learned preconditioner: called after each tangent assembly
-// (setup) and for each application z = M v on slot-layout vectors.
+// Hooks for the learned preconditioner: set-up after each tangent assembly and
+// application z = M v on slot-layout vectors (CB_SOLVER_LEARNED_FGMRES); and,
+// for online training, a call after every FP64 band factorization and after
+// every FP64 band solve with its (right-hand side, solution) in NEQ layout.
 struct PrecHook {
     std::function<void(Dev&)> setup;
     std::function<void(Dev&, const double*, double*)> apply;
+    std::function<void(Dev&)> after_factor;
+    std::function<void(Dev&, const double*, const double*)> observe;
 };
 
 class Analysis {

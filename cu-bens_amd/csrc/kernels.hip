@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 
 #include "dev.h"
@@ -728,6 +729,8 @@ void bsr_spmv32(Dev& d, const double* x, double* y) {
 
 // ============================================================== setup
 void dev_create(Dev& d, const Model& m) {
+    static std::atomic<uint64_t> next_uid{1};
+    d.uid = next_uid.fetch_add(1);
     CB_CHECK(hipStreamCreateWithFlags(&d.s, hipStreamNonBlocking));
     d.nj = m.nj; d.ntr = m.ntr; d.nfr = m.nfr; d.nsh = m.nsh; d.neq = m.neq;
     d.bs = m.bs; d.nslot = m.nj * m.bs; d.nnzb = long(m.colidx.size());

@@ -64,6 +64,14 @@ PD = C.POINTER(C.c_double)
 PL = C.POINTER(C.c_long)
 PI = C.POINTER(C.c_int)
 PI32 = C.POINTER(C.c_int32)
+PF = C.POINTER(C.c_float)
+
+
+class TrainStats(C.Structure):
+    _fields_ = [("steps", C.c_long), ("loss_first", C.c_double), ("loss_last", C.c_double),
+                ("loss_recent", C.c_double)]
+
+
 ABI = {
     "cb_last_error": (C.c_char_p, []),
     "cb_open": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
@@ -85,6 +93,16 @@ ABI = {
     "cb_get_step_stats": (C.c_int, [C.c_void_p, PD, PD, PL]),
     "cb_get_kernel_times": (C.c_int, [C.c_void_p, C.c_int, C.c_int, PD, PL]),
     "cb_mass": (C.c_int, [C.c_void_p, PD]),
+    "cb_learned_create": (C.c_int, [C.c_uint, C.POINTER(C.c_void_p)]),
+    "cb_learned_free": (None, [C.c_void_p]),
+    "cb_learned_nparams": (C.c_long, [C.c_void_p]),
+    "cb_learned_get_params": (C.c_int, [C.c_void_p, PF]),
+    "cb_learned_set_params": (C.c_int, [C.c_void_p, PF]),
+    "cb_learned_attach": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cb_learned_set_objective": (C.c_int, [C.c_void_p, C.c_int]),
+    "cb_learned_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_void_p]),
+    "cb_learned_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, PD, PD, C.c_int, PD, PF]),
+    "cb_learned_blocks": (C.c_int, [C.c_void_p, C.c_void_p, PF, PF, PL, PL]),
     "cb_write_results2": (C.c_int, [C.c_void_p, C.c_char_p]),
 }
 
@@ -264,3 +282,65 @@ def bsr_to_dense(vals: np.ndarray, rowptr: np.ndarray, colidx: np.ndarray, bs: i
 def eq_of_slot(jcode: np.ndarray, nj: int, bs: int) -> np.ndarray:
     j = jcode.reshape(nj, 7)[:, :bs].astype(np.int64) - 1
     return j.reshape(-1)
+
+
+class LearnedPrecond:
+    """The learned block approximate-inverse preconditioner (cb_learned_*)."""
+
+    def __init__(self, seed: int = 0):
+        h = C.c_void_p()
+        _check(lib().cb_learned_create(seed, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().cb_learned_free(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def nparams(self) -> int:
+        return int(lib().cb_learned_nparams(self.h))
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.nparams, np.float32)
+        _check(lib().cb_learned_get_params(self.h, _p(p, PF)))
+        return p
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.nparams:
+            raise ValueError("parameter vector size mismatch")
+        _check(lib().cb_learned_set_params(self.h, _p(p, PF)))
+
+    def set_objective(self, kind: int):
+        _check(lib().cb_learned_set_objective(self.h, kind))
+
+    def attach(self, g: "GpuModel"):
+        _check(lib().cb_learned_attach(g.h, self.h))
+
+    def train(self, g: "GpuModel", max_steps: int, probes: int = 4, lr: float = 1e-3) -> dict:
+        st = TrainStats()
+        _check(lib().cb_learned_train(self.h, g.h, max_steps, probes, lr, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def loss_grad(self, g: "GpuModel", r: np.ndarray, u: np.ndarray):
+        r = np.ascontiguousarray(np.atleast_2d(r), np.float64)
+        u = np.ascontiguousarray(np.atleast_2d(u), np.float64)
+        loss = C.c_double()
+        grad = np.zeros(self.nparams, np.float32)
+        _check(lib().cb_learned_loss_grad(self.h, g.h, _p(r), _p(u), r.shape[0], C.byref(loss), _p(grad, PF)))
+        return loss.value, grad
+
+    def blocks(self, g: "GpuModel"):
+        nv, ne = C.c_long(), C.c_long()
+        _check(lib().cb_learned_blocks(self.h, g.h, None, None, C.byref(nv), C.byref(ne)))
+        gd = np.zeros(nv.value * 36, np.float32)
+        go = np.zeros(max(ne.value, 1) * 36, np.float32)
+        _check(lib().cb_learned_blocks(self.h, g.h, _p(gd, PF), _p(go, PF), C.byref(nv), C.byref(ne)))
+        return gd.reshape(-1, 6, 6), go[: ne.value * 36].reshape(-1, 6, 6)

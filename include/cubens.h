@@ -145,6 +145,45 @@ int cb_get_kernel_times(const cb_model *m, int precision, int step, double *ms, 
 /* Writes the results2.txt rows of the last run (format of misc.c:350-417). */
 int cb_write_results2(const cb_model *m, const char *path);
 
+/* ---- learned preconditioner (no reference counterpart) ------------------
+ * A message-passing network on the joint graph of the tangent emits a
+ * factorized block approximate inverse M = S G^T G S used by
+ * CB_SOLVER_LEARNED_FGMRES in place of block-Jacobi; FGMRES keeps the true
+ * FP64 residual and falls back to the FP64 band factor (the judge) when it
+ * misses its tolerance.  The network is trained on the GPU by its own
+ * forward/backward kernels on (tangent, right-hand side, solution) triples of
+ * the FP64 solves of an analysis. */
+typedef struct cb_learned cb_learned;
+typedef struct {
+    long steps;                 /* training steps (one per FP64 solve)      */
+    double loss_first, loss_last, loss_recent;   /* recent = mean of the last 10 */
+} cb_train_stats;
+
+int cb_learned_create(unsigned seed, cb_learned **out);
+void cb_learned_free(cb_learned *L);
+long cb_learned_nparams(const cb_learned *L);
+int cb_learned_get_params(const cb_learned *L, float *params);
+int cb_learned_set_params(cb_learned *L, const float *params);
+/* Training objective on the error t = M r - K^{-1} r in the Jacobi-scaled
+ * space: 1 energy norm t.K t / r.K^{-1}r (default), 0 |K t|^2/|r|^2,
+ * 2 |t|^2/|K^{-1}r|^2. */
+int cb_learned_set_objective(cb_learned *L, int kind);
+/* Use L for CB_SOLVER_LEARNED_FGMRES runs of model m (NULL detaches). */
+int cb_learned_attach(cb_model *m, cb_learned *L);
+/* Runs m's analysis with the FP64 band solver for max_steps converged steps and
+ * takes one Adam step (learning rate lr) per FP64 solve on the pair
+ * (right-hand side, solution) plus `probes` random right-hand sides solved
+ * with the same factor. */
+int cb_learned_train(cb_learned *L, cb_model *m, int max_steps, int probes, float lr,
+                     cb_train_stats *st);
+/* Loss and gradient (all parameters) at the tangent currently held by m (after
+ * cb_tangent) for nvec NEQ-layout pairs r[b*NEQ..], u[b*NEQ..] with K u = r. */
+int cb_learned_loss_grad(cb_learned *L, cb_model *m, const double *r, const double *u, int nvec,
+                         double *loss, float *grad);
+/* Preconditioner blocks at m's current tangent: G diagonal [nv*36] and edge
+ * [ne*36] blocks (either pointer may be NULL; nv, ne receive the sizes). */
+int cb_learned_blocks(cb_learned *L, cb_model *m, float *gd, float *go, long *nv, long *ne);
+
 #ifdef __cplusplus
 }
 #endif
