@@ -51,17 +51,22 @@ TILE = 64
 def kernel_model(info) -> dict:
     """Algorithmic work of one launch of each band kernel family at this model.
 
-    nt tile rows of 64, w sub-diagonal tiles (w = ceil(bandwidth/64)):
+    nt tile rows of 64, w sub-diagonal tiles (w = ceil(bandwidth/64)); column K
+    has m = min(w, nt-1-K) sub-diagonal tiles:
       diag   : LDL^T of a 64x64 tile (2/3 T^3) + inverse of its unit factor (T^3/3)
-      panel  : m tile products of 2 T^3, m = w (interior)
+      panel  : m tile products of 2 T^3
       update : m(m+1)/2 tile products of 2 T^3
+      column : one fused launch = diag + panel + update of a column (average
+               over the nt launches of a factorization)
       sweeps : every factor tile and every Linv tile read once (8 B entries)
     """
     nt = (info.neq + TILE - 1) // TILE
     w = min((info.bandwidth + TILE - 1) // TILE, nt - 1)
     t3 = TILE ** 3
     sweep_bytes = 8 * TILE * TILE * (nt * (w + 1) + nt) + 3 * 8 * nt * TILE
+    whole = sum(t3 + 2.0 * t3 * m + t3 * m * (m + 1) for m in (min(w, nt - 1 - k) for k in range(nt)))
     return {
+        "band_column": ("mfma", whole / nt, "TFLOP/s"),
         "band_diag": ("mfma", float(t3), "TFLOP/s"),
         "band_panel": ("mfma", 2.0 * t3 * w, "TFLOP/s"),
         "band_update": ("mfma", 2.0 * t3 * w * (w + 1) / 2, "TFLOP/s"),
@@ -103,14 +108,16 @@ def cpu_baseline(path: str, newton: int, increments: int, budget: float) -> dict
                        f"GPU window ({increments} increments, {newton} Newton iterations)")}
 
 
-def run_path(path: str, solver: int, warmup: int, steps: int, tol: float, timing: int = 0) -> dict:
+def run_path(path: str, solver: int, warmup: int, steps: int, tol: float, timing: int = 0,
+             schedule: int = 0) -> dict:
     import cubens
     t0 = time.time()
     with cubens.GpuModel(path) as g:
         t_open = time.time() - t0
         opt = cubens.options(solver, krylov_tol=tol, restart=30, maxit=400,
                              judge=0 if solver == cubens.SOLVER_BAND64 else 1,
-                             max_steps=warmup + steps, keep_d=1, kernel_timing=timing)
+                             max_steps=warmup + steps, keep_d=1, kernel_timing=timing,
+                             factor_schedule=schedule)
         r = g.run(opt)
         info = g.info
         kt = None
@@ -141,6 +148,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=8.0, help="seconds of partial CPU factorization")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--schedule", default="fused", choices=("fused", "split"),
+                    help="band factorization schedule: one fused launch per tile column, or "
+                         "diag/panel/update launches")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -149,10 +159,11 @@ def main():
     import cubens
     path = workload_file(args.n)
     results = {}
+    schedule = cubens.FACTOR_SPLIT if args.schedule == "split" else cubens.FACTOR_FUSED
     for name in args.paths.split(","):
         solver = {"fp64": cubens.SOLVER_BAND64, "mixed": cubens.SOLVER_BAND32_FGMRES}[name]
         log(f"[bench] path {name}: {args.warmup} warmup + {args.steps} timed increments")
-        res = run_path(path, solver, args.warmup, args.steps, args.tol, timing=31)
+        res = run_path(path, solver, args.warmup, args.steps, args.tol, timing=63, schedule=schedule)
         st = res["stats"]
         log(f"[bench] {name}: {res['rate']:.2f} Newton it/s ({res['newton']} its in {res['seconds']:.2f}s); "
             f"tangent {st['t_tangent']:.2f}s factor {st['t_factor']:.2f}s solve {st['t_solve']:.2f}s "
@@ -191,7 +202,7 @@ def main():
         "config": {"workload": f"cylindrical shell panel {args.n}x{args.n} MSAL, "
                                f"{2 * args.n * args.n} CST+DKT triangles, NEQ {head['neq']}",
                    "path": "fp64 band LDL^T" if head is plain else "fp32-factor FGMRES",
-                   "increments_timed": args.steps},
+                   "increments_timed": args.steps, "factor_schedule": args.schedule},
         "roofline": roof,
         "cpu_baseline": cpu,
         **extra,

@@ -7,8 +7,10 @@
 
 namespace cb {
 
-// Kernel families of the band solver whose launches can be timed with HIP events.
-enum { KT_DIAG = 0, KT_PANEL = 1, KT_UPDATE = 2, KT_FWD = 3, KT_BWD = 4, KT_N = 5 };
+// Kernel families of the band solver whose launches can be timed with HIP
+// events: KT_COLUMN is the fused per-column factorization launch; KT_DIAG,
+// KT_PANEL and KT_UPDATE are the launches of the split schedule.
+enum { KT_COLUMN = 0, KT_DIAG = 1, KT_PANEL = 2, KT_UPDATE = 3, KT_FWD = 4, KT_BWD = 5, KT_N = 6 };
 
 struct KernelClock {
     int mask = 0;                        // bit f: time family f
@@ -35,9 +37,14 @@ struct BandLDLT {
     double* piv = nullptr;   // pivots D (padded to nt*T)
     double* xw = nullptr;    // work vectors (padded)
     double* yw = nullptr;
-    int* flag = nullptr;       // error flag (bad band entry / wait bound hit)
+    int* flag = nullptr;       // error bits: 1 a wait bound hit, 2 an entry outside the band
     int* flags = nullptr;      // per tile row completion epochs: forward [nt], backward [nt]
     int epoch = 0;
+    int* pflags = nullptr;     // panel completion epochs of the fused factorization launch
+    int cepoch = 0;            // epoch of the last fused launch
+    // factorization schedule: one fused launch per tile column (default), or
+    // separate diag/panel/update launches
+    bool split = false;
     std::vector<double> piv_host;
 
     void setup(long n_, long bandwidth, hipStream_t s_);

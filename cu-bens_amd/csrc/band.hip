@@ -9,11 +9,12 @@
 // Storage: the lower band (half-bandwidth b = max column height) in dense T x T
 // tiles; tile row I holds tiles J = I-w .. I (w = ceil(b/T)), column-major.
 // Factorization is right-looking by tile column K:
-//   diag : L_KK D_K L_KK^T = A_KK in LDS, plus Linv_K = L_KK^{-1}
+//   diag : D_K and Linv_K = L_KK^{-1} of A_KK = L_KK D_K L_KK^T, in registers
 //   panel: L_IK = A_IK Linv_K^T D_K^{-1},            K < I <= K+w
 //   update: A_IJ -= (L_IK D_K) L_JK^T,               K < J <= I <= K+w
-// Solves: forward with Linv (one launch per tile row, the launch also pushes
-// y_I into the next w tile rows), pivot scaling, backward likewise.
+// Solves: pipelined forward and backward sweeps, one workgroup per tile row,
+// with Linv_I for the diagonal blocks and pivot scaling folded into the
+// backward sweep.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,7 +44,7 @@ __global__ void k_band_from_bsr(R* __restrict__ tiles, const double* __restrict_
             int32_t j = eq[B * bs + q];
             if (j < 0 || j > i) continue;
             long I = i / T, J = j / T;
-            if (I - J > w) { *bad = 1; continue; }
+            if (I - J > w) { *bad = 2; continue; }
             tiles[(I * (w + 1) + (J - I + w)) * (long)TT + (j % T) * T + (i % T)] =
                 R(K[(long)k * bs * bs + p * bs + q]);
         }
@@ -58,213 +59,333 @@ __global__ void k_band_pad(R* __restrict__ tiles, long n, long nt, int w) {
     tiles[(I * (w + 1) + w) * (long)TT + (i % T) * T + (i % T)] = R(1);
 }
 
-// LDL^T of the diagonal tile by one wavefront: lane i keeps row i of the tile
-// in registers (fully unrolled, static indices); column c is broadcast through
-// LDS at step c.  The inverse of the unit lower factor is formed the same way:
-// lane j builds column j by forward substitution, reading the rows of L as LDS
-// broadcasts.  Writes L (unit lower), pivots D and Linv = L^{-1}.
-template <class R>
-__global__ void __launch_bounds__(64) k_tile_diag(R* __restrict__ tiles, R* __restrict__ linv,
-                                                  double* __restrict__ piv, long Kt, int w) {
-    __shared__ R col[T];
-    __shared__ R Ls[T][T + 1];
-    const int lane = threadIdx.x;
-    R* tile = tiles + (Kt * (w + 1) + w) * (long)TT;
-    R a[T];
+// ---- factorization --------------------------------------------------------
+// Tile tasks of column K:
+//   D_K          D_K and Linv_K of tile (K,K)
+//   P_{I,K}      L_IK = A_IK Linv_K^T D_K^{-1},            K < I <= K+m
+//   U_{I,J,K}    A_IJ -= (L_IK D_K) L_JK^T,               K < J <= I <= K+m
+// (m = min(w, nt-1-K)).  Two schedules run the same tile arithmetic:
+//  - fused (default): one launch per column, k_band_column(K) for K = -1 ..
+//    nt-2.  Workgroup 0 runs the critical chain P_{K+1,K} -> U_{K+1,K+1,K} ->
+//    D_{K+1} entirely in its LDS; workgroups 1..m-1 run the other panels and
+//    the rest run the updates, each waiting on the flags of the two panels it
+//    reads.  A workgroup only waits on workgroups with a lower index, which
+//    were dispatched before it, so every wait ends; waits are bounded anyway.
+//  - split: three launches per column (diag, panels, updates), for timing
+//    the tile tasks one by one.
+// Tiles handed between workgroups of one launch (the panels L_IK) are stored
+// and loaded with system-scope coherent accesses (written through to memory,
+// read past the XCD caches): a producer waits for its stores to complete
+// (vmcnt 0) before the flag store, a consumer reads the data after it has
+// seen the flag.  Everything else crosses a launch boundary.
+constexpr int NTH = 512;   // threads of every factorization workgroup (8 wavefronts)
+constexpr int NWAVE = NTH / 64;
+
+__device__ inline long tile_index(long I, long J, int w) { return I * (w + 1) + (J - I + w); }
+
+template <bool CO, class R> __device__ __forceinline__ R cld(const R* p) {
+    if constexpr (CO) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    else return *p;
+}
+template <bool CO, class R> __device__ __forceinline__ void cst(R* p, R v) {
+    if constexpr (CO) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    else *p = v;
+}
+
+// Panel flags sit one per 256-byte line so that pollers of different panels
+// do not share a memory channel.
+constexpr long FLAG_STRIDE = 64;
+
+// thread 0 waits until flag i holds epoch (or an error is raised anywhere)
+__device__ inline bool wait_flag_bounded(const int* flags, long i, int epoch, long spin_limit, int* err,
+                                         int* ok_s) {
+    if (threadIdx.x == 0) {
+        long n = 0;
+        bool good = true;
+        const int* f = flags + i * FLAG_STRIDE;
+        while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != epoch) {
+            if (++n >= spin_limit) { good = false; break; }
+            if ((n & 63) == 0 && (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) & 1)) {
+                good = false;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        if (!good) atomicOr(err, 1);
+        *ok_s = good;
+    }
+    __syncthreads();
+    return *ok_s != 0;
+}
+
+__device__ inline void raise_flag(int* flags, long i, int epoch) {
+    __builtin_amdgcn_s_waitcnt(0);   // this wave's coherent stores have completed
+    __syncthreads();
+    if (threadIdx.x == 0)
+        __hip_atomic_store(flags + i * FLAG_STRIDE, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// ---- tile product C = X Y^T -------------------------------------------------
+// Operands staged in LDS column-major: xs[k][row] = X[row][k], ys[k][col] =
+// Y[col][k]; the result stays in registers and is handed out entry by entry.
+template <class R> struct TileAcc;
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// FP64: FP64 MFMA 16x16x4, two 16x16 output blocks per wavefront.  The MFMA
+// is fed Y as its A operand and X as its B operand (lane l: Y[16bj + l%16][k]
+// and X[16bi + l%16][k], k = k0 + l/16), so element e of lane l is
+// C[16bi + l%16][16bj + l/16 + 4e]: sixteen consecutive rows per column, which
+// makes the column-major tile stores coalesced.
+template <> struct TileAcc<double> {
+    d4 acc[2];
+    __device__ __forceinline__ void product(const double (*xs)[T], const double (*ys)[T]) {
+        const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        acc[0] = d4{0.0, 0.0, 0.0, 0.0};
+        acc[1] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+        for (int k0 = 0; k0 < T; k0 += 4) {
+            const int k = k0 + (l >> 4);
 #pragma unroll
-    for (int j = 0; j < T; ++j) a[j] = tile[j * T + lane];   // (lane, j), column-major
+            for (int b = 0; b < 2; ++b) {
+                const int blk = 2 * wv + b, bi = blk & 3, bj = blk >> 2;
+                acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ys[k][16 * bj + (l & 15)], xs[k][16 * bi + (l & 15)],
+                                                             acc[b], 0, 0, 0);
+            }
+        }
+    }
+    template <class F> __device__ __forceinline__ void each(F&& f) const {
+        const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int blk = 2 * wv + b, bi = blk & 3, bj = blk >> 2;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) f(16 * bi + (l & 15), 16 * bj + (l >> 4) + 4 * e, acc[b][e]);
+        }
+    }
+};
+
+// FP32: vector FMA, thread (tr, tc) owns rows tr + 16u and columns tc + 32v.
+template <> struct TileAcc<float> {
+    float acc[4][2];
+    __device__ __forceinline__ void product(const float (*xs)[T], const float (*ys)[T]) {
+        const int tr = threadIdx.x & 15, tc = threadIdx.x >> 4;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u][0] = acc[u][1] = 0.0f;
+        for (int k = 0; k < T; ++k) {
+            float av[4], bv[2];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) av[u] = xs[k][tr + 16 * u];
+#pragma unroll
+            for (int v = 0; v < 2; ++v) bv[v] = ys[k][tc + 32 * v];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 2; ++v) acc[u][v] += av[u] * bv[v];
+        }
+    }
+    template <class F> __device__ __forceinline__ void each(F&& f) const {
+        const int tr = threadIdx.x & 15, tc = threadIdx.x >> 4;
+#pragma unroll
+        for (int v = 0; v < 2; ++v)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) f(tr + 16 * u, tc + 32 * v, acc[u][v]);
+    }
+};
+
+// ---- D_K --------------------------------------------------------------------
+// Thread (r = tid % 64, g = tid / 64) keeps the entries (r, j), j = g + NWAVE m,
+// of the tile in registers: the Schur complement a[r][j] while j is still
+// ahead of the elimination step c, afterwards the entry B[r][j] of B = L^{-1},
+// which the same elimination applied to the identity produces (B[r][:] -=
+// l_r B[c][:]).  Step c broadcasts one vector through LDS: column c of the
+// Schur complement in its entries j >= c and row c of B in its entries j < c
+// (double-buffered, one barrier per step).  The steps are unrolled, so every
+// register index is static; the multiplier is zero on the rows r <= c, which
+// makes the update branch-free.  Entries above the diagonal (j > r) are
+// updated too but never read; they start at zero.  Writes D to piv and
+// Linv = L^{-1} (zeros above the diagonal).  L_KK itself is read by neither
+// the panels and updates nor the solves and is not stored.
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+template <class R, class Load>
+__device__ __forceinline__ void diag_core(Load&& load, R* __restrict__ lout, double* __restrict__ piv, long Kt,
+                                          R (*bc)[T]) {
+    constexpr int S = T / NWAVE;
+    const int r = threadIdx.x & 63;
+    const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    R v[S];
+#pragma unroll
+    for (int m = 0; m < S; ++m) {
+        const int j = g + NWAVE * m;
+        const R x = load(j, r);
+        v[m] = (j <= r) ? x : R(0);
+    }
+    R dr = R(0);
 #pragma unroll
     for (int c = 0; c < T; ++c) {
-        if (lane >= c) col[lane] = a[c];
-        __syncthreads();
-        const R d = col[c];
-        if (lane > c) {
-            const R l = a[c] / d;
+        const int p = c & 1;
+        if (g == c % NWAVE && r >= c) bc[p][r] = v[c / NWAVE];
+        if (r == c) {
 #pragma unroll
-            for (int j = c + 1; j < T; ++j)
-                if (j <= lane) a[j] -= l * col[j];
-            a[c] = l;
+            for (int m = 0; m < S; ++m)
+                if (g + NWAVE * m < c) bc[p][g + NWAVE * m] = v[m];
         }
         __syncthreads();
+        const R d = bc[p][c];
+        const R l = bc[p][r] / d;
+        const R le = (r > c) ? l : R(0);
+#pragma unroll
+        for (int m = 0; m < S; ++m) v[m] = fmadd(-le, bc[p][g + NWAVE * m], v[m]);
+        if (g == c % NWAVE) v[c / NWAVE] = (r > c) ? -l : ((r == c) ? R(1) : v[c / NWAVE]);
+        dr = (r == c) ? d : dr;
     }
-    // pivots and L (unit lower, zero upper) back to the tile and to LDS
-    piv[Kt * T + lane] = double(col[lane]);   // d_lane, written by lane at step c = lane
+    if (g == 0) piv[Kt * T + r] = double(dr);
 #pragma unroll
-    for (int j = 0; j < T; ++j) {
-        R v = (j < lane) ? a[j] : (j == lane ? col[lane] : R(0));
-        tile[j * T + lane] = v;
-        Ls[lane][j] = (j < lane) ? a[j] : R(0);
+    for (int m = 0; m < S; ++m) {
+        const int j = g + NWAVE * m;
+        lout[j * T + r] = (j <= r) ? v[m] : R(0);
     }
-    __syncthreads();
-    // column lane of L^{-1}: v_i = [i == lane] - sum_{k<i} L[i][k] v_k
-    R v[T];
-#pragma unroll
-    for (int i = 0; i < T; ++i) {
-        R s0 = R(0), s1 = R(0);
-#pragma unroll
-        for (int k = 0; k < i; ++k) {
-            if (k & 1) s1 += Ls[i][k] * v[k];
-            else s0 += Ls[i][k] * v[k];
-        }
-        v[i] = ((i == lane) ? R(1) : R(0)) - (s0 + s1);
-    }
-    R* out = linv + Kt * (long)TT;
-#pragma unroll
-    for (int i = 0; i < T; ++i) out[lane * T + i] = v[i];   // column lane, row i
 }
 
-// LDS tiles below are column-major like the global tiles: t[col][row].  The
-// 16x16 thread grid owns 4x4 outputs each; at step k a thread reads row
-// entries tr+16u of column k (consecutive across threads, no bank conflicts).
+// ---- staging ----------------------------------------------------------------
+template <bool CO, class R>
+__device__ __forceinline__ void stage(R (*dst)[T], const R* __restrict__ src) {
+    for (int idx = threadIdx.x; idx < TT; idx += NTH) dst[idx / T][idx % T] = cld<CO>(&src[idx]);
+}
+// dst[k][row] = src[k][row] * d[k]
+template <bool CO, class R>
+__device__ __forceinline__ void stage_scaled(R (*dst)[T], const R* __restrict__ src, const double* __restrict__ d) {
+    for (int idx = threadIdx.x; idx < TT; idx += NTH) dst[idx / T][idx % T] = cld<CO>(&src[idx]) * R(d[idx / T]);
+}
 
-// L_IK = A_IK Linv_K^T D_K^{-1} for I = Kt+1+blockIdx.x.
+// P_{I,K}: out = A_IK Linv_K^T D_K^{-1} (out may alias a)
+template <bool CO, class R>
+__device__ __forceinline__ void panel_task(R* __restrict__ out, const R* __restrict__ a, const R* __restrict__ lv,
+                                           const double* __restrict__ dk, R (*xs)[T], R (*ys)[T]) {
+    stage<false>(xs, a);
+    stage<false>(ys, lv);
+    __syncthreads();
+    TileAcc<R> acc;
+    acc.product(xs, ys);
+    acc.each([&](int row, int col, R v) { cst<CO>(&out[col * T + row], v * (R(1) / R(dk[col]))); });
+}
+
+// U_{I,J,K}: out -= (L_IK D_K) L_JK^T
+template <bool CO, class R>
+__device__ __forceinline__ void update_task(R* __restrict__ out, const R* __restrict__ lik, const R* __restrict__ ljk,
+                                            const double* __restrict__ dk, R (*xs)[T], R (*ys)[T]) {
+    stage_scaled<CO>(xs, lik, dk);
+    stage<CO>(ys, ljk);
+    __syncthreads();
+    TileAcc<R> acc;
+    acc.product(xs, ys);
+    acc.each([&](int row, int col, R v) { out[col * T + row] -= v; });
+}
+
+// ---- split schedule ----------------------------------------------------------
 template <class R>
-__global__ void __launch_bounds__(256) k_tile_panel(R* __restrict__ tiles, const R* __restrict__ linv,
+__global__ void __launch_bounds__(NTH) k_tile_diag(const R* __restrict__ tiles, R* __restrict__ linv,
+                                                   double* __restrict__ piv, long Kt, int w) {
+    __shared__ R bc[2][T];
+    const R* tile = tiles + tile_index(Kt, Kt, w) * TT;
+    diag_core<R>([&](int j, int r) { return tile[j * T + r]; }, linv + Kt * (long)TT, piv, Kt, bc);
+}
+
+// P_{I,K} for I = Kt+1+blockIdx.x
+template <class R>
+__global__ void __launch_bounds__(NTH) k_tile_panel(R* __restrict__ tiles, const R* __restrict__ linv,
                                                     const double* __restrict__ piv, long Kt, int w) {
-    __shared__ R at[T][T];   // A_IK
-    __shared__ R lt[T][T];   // Linv_K
-    const long I = Kt + 1 + blockIdx.x;
-    R* tile = tiles + (I * (w + 1) + (Kt - I + w)) * (long)TT;
-    const R* lv = linv + Kt * (long)TT;
-    for (int idx = threadIdx.x; idx < TT; idx += blockDim.x) {
-        at[idx / T][idx % T] = tile[idx];
-        lt[idx / T][idx % T] = lv[idx];
-    }
-    __syncthreads();
-    // out[r][c] = sum_k A[r][k] Linv[c][k] / d[c]
-    const int tr = threadIdx.x % 16, tc = threadIdx.x / 16;
-    R acc[4][4] = {};
-    for (int k = 0; k < T; ++k) {
-        R av[4], bv[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) { av[u] = at[k][tr + 16 * u]; bv[u] = lt[k][tc + 16 * u]; }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[u][v] += av[u] * bv[v];
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const int c = tc + 16 * v;
-        const R dinv = R(1) / R(piv[Kt * T + c]);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) tile[c * T + tr + 16 * u] = acc[u][v] * dinv;
-    }
+    __shared__ R xs[T][T];
+    __shared__ R ys[T][T];
+    R* tile = tiles + tile_index(Kt + 1 + blockIdx.x, Kt, w) * TT;
+    panel_task<false, R>(tile, tile, linv + Kt * (long)TT, piv + Kt * T, xs, ys);
 }
 
-// A_IJ -= (L_IK D_K) L_JK^T for the pairs K < J <= I <= K+m.
-template <class R>
-__global__ void __launch_bounds__(256) k_tile_update(R* __restrict__ tiles, const double* __restrict__ piv,
-                                                     long Kt, int w) {
-    __shared__ R at[T][T];   // L_IK D_K
-    __shared__ R bt[T][T];   // L_JK
-    // blockIdx.x -> (r, s) with s <= r in a triangle of side m
-    const long t = blockIdx.x;
-    long r = long((sqrt(8.0 * double(t) + 1.0) - 1.0) / 2.0);
+// (r, s), s <= r, of entry t of the triangle enumerated row by row
+__device__ inline void tri_pair(long t, long& r, long& s) {
+    r = long((sqrt(8.0 * double(t) + 1.0) - 1.0) / 2.0);
     while ((r + 1) * (r + 2) / 2 <= t) ++r;
     while (r * (r + 1) / 2 > t) --r;
-    const long s = t - r * (r + 1) / 2;
+    s = t - r * (r + 1) / 2;
+}
+
+// U_{I,J,K} for the pairs K < J <= I <= K+m, blockIdx.x -> (I, J) row by row
+template <class R>
+__global__ void __launch_bounds__(NTH) k_tile_update(R* __restrict__ tiles, const double* __restrict__ piv,
+                                                     long Kt, int w) {
+    __shared__ R xs[T][T];
+    __shared__ R ys[T][T];
+    long r, s;
+    tri_pair(blockIdx.x, r, s);
     const long I = Kt + 1 + r, J = Kt + 1 + s;
-    const R* lik = tiles + (I * (w + 1) + (Kt - I + w)) * (long)TT;
-    const R* ljk = tiles + (J * (w + 1) + (Kt - J + w)) * (long)TT;
-    R* out = tiles + (I * (w + 1) + (J - I + w)) * (long)TT;
-    for (int idx = threadIdx.x; idx < TT; idx += blockDim.x) {
-        const int col = idx / T;
-        at[col][idx % T] = lik[idx] * R(piv[Kt * T + col]);
-        bt[col][idx % T] = ljk[idx];
-    }
-    __syncthreads();
-    const int tr = threadIdx.x % 16, tc = threadIdx.x / 16;
-    R acc[4][4] = {};
-    for (int k = 0; k < T; ++k) {
-        R av[4], bv[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) { av[u] = at[k][tr + 16 * u]; bv[u] = bt[k][tc + 16 * u]; }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[u][v] += av[u] * bv[v];
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-#pragma unroll
-        for (int u = 0; u < 4; ++u) out[(tc + 16 * v) * T + tr + 16 * u] -= acc[u][v];
+    update_task<false, R>(tiles + tile_index(I, J, w) * TT, tiles + tile_index(I, Kt, w) * TT,
+                          tiles + tile_index(J, Kt, w) * TT, piv + Kt * T, xs, ys);
 }
 
-// Solve steps use 256 threads: the 64x64 tiles are staged through LDS with
-// coalesced loads; a row of the result is formed by 4 lanes groups of 16
-// columns and reduced through LDS.
-
-// Forward step I: y_I = Linv_I r_I (every block), block 0 stores it, block j>0
-// subtracts L_{I+j,I} y_I from r_{I+j}.
+// ---- fused schedule -----------------------------------------------------------
+// Launch K (K = -1 .. nt-2), m = min(w, nt-1-K) (0 for K = -1):
+//   workgroup 0          P_{K+1,K}, U_{K+1,K+1,K} in LDS, D_{K+1}; raises flag 0
+//   workgroups 1..m-1    P_{K+1+b,K}; raise flag b
+//   the rest             U_{I,J,K} for the pairs other than (K+1,K+1), row by row
+// Grid: 1 + (m - 1) + (m(m+1)/2 - 1) workgroups for m > 0, 1 for m = 0.
 template <class R>
-__global__ void __launch_bounds__(256) k_fwd_step(const R* __restrict__ tiles, const R* __restrict__ linv,
-                                                  double* __restrict__ x, double* __restrict__ yout, long It,
-                                                  int w) {
-    __shared__ R ts[T][T + 1];        // ts[row][col]
-    __shared__ double vin[T], y[T];
-    __shared__ double part[4][T];
-    const int tid = threadIdx.x, row = tid % T, q = tid / T;   // q: column quarter
-    if (tid < T) vin[tid] = x[It * T + tid];
-    const R* lv = linv + It * (long)TT;
-    for (int idx = tid; idx < TT; idx += 256) ts[idx % T][idx / T] = lv[idx];
-    __syncthreads();
-    double s = 0.0;
-    for (int k = 16 * q; k < 16 * q + 16; ++k) s += double(ts[row][k]) * vin[k];
-    part[q][row] = s;
-    __syncthreads();
-    if (tid < T) y[tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
-    __syncthreads();
-    if (blockIdx.x == 0) {
-        if (tid < T) yout[It * T + tid] = y[tid];
+__global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* __restrict__ linv,
+                                                     double* __restrict__ piv, long K, int w, long nt,
+                                                     int* flags, int epoch, long spin_limit, int* err) {
+    __shared__ R xs[T][T];
+    __shared__ R ys[T][T];
+    __shared__ double dk[T];
+    __shared__ int ok_s;
+    const long m = (K < 0) ? 0 : ((nt - 1 - K < w) ? nt - 1 - K : w);
+    const long b = blockIdx.x;
+    if (m > 0 && threadIdx.x < T) dk[threadIdx.x] = piv[K * T + threadIdx.x];
+    if (b == 0) {
+        const long D = K + 1;
+        const R* dtile = tiles + tile_index(D, D, w) * TT;
+        R(*bc)[T] = ys;   // the diagonal step's broadcast buffers
+        if (m == 0) {
+            diag_core<R>([&](int j, int r) { return dtile[j * T + r]; }, linv + D * (long)TT, piv, D, bc);
+            return;
+        }
+        R* ptile = tiles + tile_index(D, K, w) * TT;
+        stage<false>(xs, ptile);
+        stage<false>(ys, linv + K * (long)TT);
+        __syncthreads();
+        TileAcc<R> acc;
+        acc.product(xs, ys);
+        __syncthreads();
+        // L = P_{K+1,K}: to memory for the updates, (L D, L) to LDS for U_{K+1,K+1,K}
+        acc.each([&](int row, int col, R v) {
+            const R lv = v * (R(1) / R(dk[col]));
+            cst<true>(&ptile[col * T + row], lv);
+            xs[col][row] = lv * R(dk[col]);
+            ys[col][row] = lv;
+        });
+        raise_flag(flags, 0, epoch);
+        acc.product(xs, ys);
+        __syncthreads();
+        acc.each([&](int row, int col, R v) { xs[col][row] = dtile[col * T + row] - v; });
+        __syncthreads();
+        diag_core<R>([&](int j, int r) { return xs[j][r]; }, linv + D * (long)TT, piv, D, bc);
         return;
     }
-    const long J = It + blockIdx.x;
-    const R* l = tiles + (J * (w + 1) + (It - J + w)) * (long)TT;
-    for (int idx = tid; idx < TT; idx += 256) ts[idx % T][idx / T] = l[idx];
-    __syncthreads();
-    s = 0.0;
-    for (int k = 16 * q; k < 16 * q + 16; ++k) s += double(ts[row][k]) * y[k];
-    part[q][row] = s;
-    __syncthreads();
-    if (tid < T) x[J * T + tid] -= part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
-}
-
-// Backward step I: x_I = Linv_I^T z_I (every block), block 0 stores it, block
-// j>0 subtracts L_{I,I-j}^T x_I from z_{I-j}.  The transposes are read as
-// columns of the LDS copies.
-template <class R>
-__global__ void __launch_bounds__(256) k_bwd_step(const R* __restrict__ tiles, const R* __restrict__ linv,
-                                                  double* __restrict__ z, double* __restrict__ xout, long It,
-                                                  int w) {
-    __shared__ R ts[T][T + 1];        // ts[row][col]
-    __shared__ double vin[T], xs[T];
-    __shared__ double part[4][T];
-    const int tid = threadIdx.x, col = tid % T, q = tid / T;
-    if (tid < T) vin[tid] = z[It * T + tid];
-    const R* lv = linv + It * (long)TT;
-    for (int idx = tid; idx < TT; idx += 256) ts[idx % T][idx / T] = lv[idx];
-    __syncthreads();
-    // (Linv^T v)[col] = sum_k Linv[k][col] v[k]
-    double s = 0.0;
-    for (int k = 16 * q; k < 16 * q + 16; ++k) s += double(ts[k][col]) * vin[k];
-    part[q][col] = s;
-    __syncthreads();
-    if (tid < T) xs[tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
-    __syncthreads();
-    if (blockIdx.x == 0) {
-        if (tid < T) xout[It * T + tid] = xs[tid];
+    __syncthreads();   // dk
+    if (b < m) {
+        const long I = K + 1 + b;
+        R* tile = tiles + tile_index(I, K, w) * TT;
+        panel_task<true, R>(tile, tile, linv + K * (long)TT, dk, xs, ys);
+        raise_flag(flags, b, epoch);
         return;
     }
-    const long J = It - blockIdx.x;
-    const R* l = tiles + (It * (w + 1) + (J - It + w)) * (long)TT;
-    for (int idx = tid; idx < TT; idx += 256) ts[idx % T][idx / T] = l[idx];
-    __syncthreads();
-    s = 0.0;
-    for (int k = 16 * q; k < 16 * q + 16; ++k) s += double(ts[k][col]) * xs[k];
-    part[q][col] = s;
-    __syncthreads();
-    if (tid < T) z[J * T + tid] -= part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+    long r, s;
+    tri_pair(b - m + 1, r, s);
+    if (!wait_flag_bounded(flags, r, epoch, spin_limit, err, &ok_s)) return;
+    if (s != r && !wait_flag_bounded(flags, s, epoch, spin_limit, err, &ok_s)) return;
+    const long I = K + 1 + r, J = K + 1 + s;
+    update_task<true, R>(tiles + tile_index(I, J, w) * TT, tiles + tile_index(I, K, w) * TT,
+                         tiles + tile_index(J, K, w) * TT, dk, xs, ys);
 }
 
 // ---- pipelined sweeps ---------------------------------------------------------
@@ -403,11 +524,6 @@ __global__ void __launch_bounds__(256) k_bwd_pipe(const R* __restrict__ tiles, c
     publish(x, I, xi, flags, epoch);
 }
 
-__global__ void k_scale_piv(double* __restrict__ x, const double* __restrict__ piv, long n) {
-    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i < n) x[i] /= piv[i];
-}
-
 __global__ void k_copy_pad(double* __restrict__ dst, const double* __restrict__ src, long n, long ntot) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i < ntot) dst[i] = i < n ? src[i] : 0.0;
@@ -460,19 +576,23 @@ void BandLDLT<R>::setup(long n_, long bandwidth, hipStream_t s_) {
     yw = dalloc<double>(size_t(nt) * T);
     flag = dalloc<int>(1);
     flags = dalloc<int>(size_t(2 * nt));
+    pflags = dalloc<int>(size_t(w + 1) * FLAG_STRIDE);
     CB_CHECK(hipMemsetAsync(flags, 0, sizeof(int) * size_t(2 * nt), s));
+    CB_CHECK(hipMemsetAsync(pflags, 0, sizeof(int) * size_t(w + 1) * FLAG_STRIDE, s));
     CB_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
     epoch = 0;
+    cepoch = 0;
     piv_host.resize(size_t(n));
+    CB_CHECK(hipStreamSynchronize(s));
 }
 
 template <class R>
 void BandLDLT<R>::release() {
-    void* ps[] = {tiles, linv, piv, xw, yw, flag, flags};
+    void* ps[] = {tiles, linv, piv, xw, yw, flag, flags, pflags};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     tiles = nullptr; linv = nullptr; piv = nullptr; xw = nullptr; yw = nullptr; flag = nullptr;
-    flags = nullptr;
+    flags = nullptr; pflags = nullptr;
     clock.release();
 }
 
@@ -485,19 +605,36 @@ int BandLDLT<R>::factor(const Dev& d) {
     if (nt * T > n)
         hipLaunchKernelGGL((k_band_pad<R>), dim3(1), dim3(T), 0, s, tiles, n, nt, w);
     CB_CHECK(hipGetLastError());
-    for (long Kt = 0; Kt < nt; ++Kt) {
-        if (clock.on(KT_DIAG)) clock.begin(KT_DIAG, s);
-        hipLaunchKernelGGL((k_tile_diag<R>), dim3(1), dim3(64), 0, s, tiles, linv, piv, Kt, w);
-        if (clock.on(KT_DIAG)) clock.end(s);
-        long m = std::min<long>(w, nt - 1 - Kt);
-        if (m > 0) {
-            if (clock.on(KT_PANEL)) clock.begin(KT_PANEL, s);
-            hipLaunchKernelGGL((k_tile_panel<R>), dim3(int(m)), dim3(256), 0, s, tiles, linv, piv, Kt, w);
-            if (clock.on(KT_PANEL)) clock.end(s);
-            if (clock.on(KT_UPDATE)) clock.begin(KT_UPDATE, s);
-            hipLaunchKernelGGL((k_tile_update<R>), dim3(int(m * (m + 1) / 2)), dim3(256), 0, s, tiles, piv,
-                               Kt, w);
-            if (clock.on(KT_UPDATE)) clock.end(s);
+    if (!split) {
+        if (cepoch + nt + 1 >= (1 << 30)) {
+            CB_CHECK(hipMemsetAsync(pflags, 0, sizeof(int) * size_t(w + 1) * FLAG_STRIDE, s));
+            cepoch = 0;
+        }
+        const long spin_limit = 1L << 22;
+        for (long K = -1; K < nt - 1; ++K) {
+            const long m = (K < 0) ? 0 : std::min<long>(w, nt - 1 - K);
+            const long grid = m > 0 ? m + m * (m + 1) / 2 - 1 : 1;
+            cepoch += 1;
+            if (clock.on(KT_COLUMN)) clock.begin(KT_COLUMN, s);
+            hipLaunchKernelGGL((k_band_column<R>), dim3(int(grid)), dim3(NTH), 0, s, tiles, linv, piv, K, w, nt,
+                               pflags, cepoch, spin_limit, flag + 0);
+            if (clock.on(KT_COLUMN)) clock.end(s);
+        }
+    } else {
+        for (long Kt = 0; Kt < nt; ++Kt) {
+            if (clock.on(KT_DIAG)) clock.begin(KT_DIAG, s);
+            hipLaunchKernelGGL((k_tile_diag<R>), dim3(1), dim3(NTH), 0, s, tiles, linv, piv, Kt, w);
+            if (clock.on(KT_DIAG)) clock.end(s);
+            const long m = std::min<long>(w, nt - 1 - Kt);
+            if (m > 0) {
+                if (clock.on(KT_PANEL)) clock.begin(KT_PANEL, s);
+                hipLaunchKernelGGL((k_tile_panel<R>), dim3(int(m)), dim3(NTH), 0, s, tiles, linv, piv, Kt, w);
+                if (clock.on(KT_PANEL)) clock.end(s);
+                if (clock.on(KT_UPDATE)) clock.begin(KT_UPDATE, s);
+                hipLaunchKernelGGL((k_tile_update<R>), dim3(int(m * (m + 1) / 2)), dim3(NTH), 0, s, tiles, piv,
+                                   Kt, w);
+                if (clock.on(KT_UPDATE)) clock.end(s);
+            }
         }
     }
     CB_CHECK(hipGetLastError());
@@ -506,7 +643,8 @@ int BandLDLT<R>::factor(const Dev& d) {
     CB_CHECK(hipMemcpyAsync(piv_host.data(), piv, sizeof(double) * size_t(n), hipMemcpyDeviceToHost, s));
     CB_CHECK(hipStreamSynchronize(s));
     clock.collect();
-    if (bad) return 2;
+    if (bad & 1) throw DeviceError("band factorization: a panel wait exceeded its bound");
+    if (bad & 2) return 2;
     for (long i = 0; i < n; ++i)
         if (piv_host[i] == 0.0 || !std::isfinite(piv_host[i])) return 1;
     return 0;
@@ -531,7 +669,7 @@ void BandLDLT<R>::solve(const double* rhs, double* out) {
     CB_CHECK(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, s));
     CB_CHECK(hipStreamSynchronize(s));
     clock.collect();
-    if (bad) throw DeviceError("band solve: a tile-row wait exceeded its bound");
+    if (bad & 1) throw DeviceError("band solve: a tile-row wait exceeded its bound");
 }
 
 template struct BandLDLT<double>;

@@ -71,6 +71,7 @@ cb_options default_options() {
     o.max_steps = 1 << 30;
     o.keep_d = 1;
     o.kernel_timing = 0;
+    o.factor_schedule = CB_FACTOR_FUSED;
     return o;
 }
 
@@ -84,13 +85,14 @@ cb::RunOptions to_run(const cb_options& o) {
     r.max_steps = o.max_steps;
     r.keep_d = o.keep_d;
     r.kernel_timing = o.kernel_timing;
+    r.factor_schedule = o.factor_schedule;
     return r;
 }
 
 bool same_opts(const cb_options& a, const cb_options& b) {
     return a.solver == b.solver && a.krylov_tol == b.krylov_tol && a.restart == b.restart &&
            a.maxit == b.maxit && a.judge == b.judge && a.max_steps == b.max_steps && a.keep_d == b.keep_d &&
-           a.kernel_timing == b.kernel_timing;
+           a.kernel_timing == b.kernel_timing && a.factor_schedule == b.factor_schedule;
 }
 
 double normal01(uint64_t& st) {

@@ -32,7 +32,7 @@ class Info(C.Structure):
 class Options(C.Structure):
     _fields_ = [("solver", C.c_int), ("krylov_tol", C.c_double), ("restart", C.c_int),
                 ("maxit", C.c_int), ("judge", C.c_int), ("max_steps", C.c_int),
-                ("keep_d", C.c_int), ("kernel_timing", C.c_int)]
+                ("keep_d", C.c_int), ("kernel_timing", C.c_int), ("factor_schedule", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -47,14 +47,16 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-KT_DIAG, KT_PANEL, KT_UPDATE, KT_FWD, KT_BWD = range(5)
-KT_NAMES = ("band_diag", "band_panel", "band_update", "band_fwd_sweep", "band_bwd_sweep")
+KT_COLUMN, KT_DIAG, KT_PANEL, KT_UPDATE, KT_FWD, KT_BWD = range(6)
+KT_NAMES = ("band_column", "band_diag", "band_panel", "band_update", "band_fwd_sweep", "band_bwd_sweep")
+FACTOR_FUSED, FACTOR_SPLIT = 0, 1
 
 
 def options(solver: int = SOLVER_BAND64, krylov_tol: float = 1e-8, restart: int = 30,
             maxit: int = 20000, judge: int = 1, max_steps: int = 1 << 30, keep_d: int = 1,
-            kernel_timing: int = 0) -> Options:
-    return Options(solver, krylov_tol, restart, maxit, judge, max_steps, keep_d, kernel_timing)
+            kernel_timing: int = 0, factor_schedule: int = 0) -> Options:
+    return Options(solver, krylov_tol, restart, maxit, judge, max_steps, keep_d, kernel_timing,
+                   factor_schedule)
 
 
 _lib = None
@@ -249,10 +251,11 @@ class GpuModel:
         return out
 
     def kernel_times(self, precision: int = 64, step: int = -1) -> dict:
-        ms = np.zeros(5)
-        n = np.zeros(5, np.int64)
+        nf = len(KT_NAMES)
+        ms = np.zeros(nf)
+        n = np.zeros(nf, np.int64)
         _check(lib().cb_get_kernel_times(self.h, precision, step, _p(ms), _p(n, PL)))
-        return {KT_NAMES[f]: (float(ms[f]), int(n[f])) for f in range(5)}
+        return {KT_NAMES[f]: (float(ms[f]), int(n[f])) for f in range(nf)}
 
     def write_results2(self, path: str):
         _check(lib().cb_write_results2(self.h, path.encode()))

@@ -51,15 +51,25 @@ typedef struct {
     int max_steps;       /* stop after this many converged steps           */
     int keep_d;          /* keep the displacement path for cb_path         */
     int kernel_timing;   /* bitmask of CB_KT_* kernel families timed with HIP events */
+    int factor_schedule; /* CB_FACTOR_* schedule of the band factorization  */
 } cb_options;
+
+/* Schedules of the band LDL^T factorization (same tile arithmetic, bitwise
+ * the same result): one fused launch per tile column whose workgroups hand
+ * the panels to the updates through flags, or a diag, a panel and an update
+ * launch per tile column (for timing the tile tasks one by one). */
+enum { CB_FACTOR_FUSED = 0, CB_FACTOR_SPLIT = 1 };
 
 /* Generations of the per-element state (base = last converged, _i = current
  * iterate, _ip = previous iterate), as the reference's x/x_temp/x_ip, c1/c1_i/
  * c1_ip, ef/ef_i/ef_ip arrays (main.c:497-510, 760-790). */
 enum { CB_GEN_BASE = 0, CB_GEN_I = 1, CB_GEN_IP = 2 };
 
-/* Kernel families of the band solver (timed with HIP events on their stream). */
-enum { CB_KT_DIAG = 0, CB_KT_PANEL = 1, CB_KT_UPDATE = 2, CB_KT_FWD = 3, CB_KT_BWD = 4 };
+/* Kernel families of the band solver (timed with HIP events on their stream):
+ * the fused per-column factorization launch, the diag/panel/update launches
+ * of the split schedule, the forward and backward sweeps. */
+enum { CB_KT_COLUMN = 0, CB_KT_DIAG = 1, CB_KT_PANEL = 2, CB_KT_UPDATE = 3, CB_KT_FWD = 4, CB_KT_BWD = 5,
+       CB_KT_N = 6 };
 
 typedef struct {
     double t_total, t_tangent, t_factor, t_solve, t_forces, t_judge, t_other;
@@ -138,7 +148,7 @@ int cb_get_path(const cb_model *m, double *lpf, int *itecnt, double *d);
  * reference counterpart: the reference keeps no timers). */
 int cb_get_step_stats(const cb_model *m, double *t_wall, double *t_judge, long *newton);
 /* Accumulated HIP-event times (ms) and launch counts of the band kernel
- * families [5] selected by cb_options.kernel_timing, for the FP64 (precision
+ * families [CB_KT_N] selected by cb_options.kernel_timing, for the FP64 (precision
  * 64) or FP32 (32) factor; step >= 0 returns the totals at that converged step
  * of the last run, step < 0 the totals now (no reference counterpart). */
 int cb_get_kernel_times(const cb_model *m, int precision, int step, double *ms, long *launches);

@@ -154,11 +154,13 @@ def test_shell_internal_force_matches_oracle():
     g.close()
 
 
-def test_band_factor_pivots_and_solve():
-    path = panel(8, thick_sigma=0.1, imperfection=0.1)
+@pytest.mark.parametrize("n", [8, 16])
+@pytest.mark.parametrize("schedule", [cubens.FACTOR_FUSED, cubens.FACTOR_SPLIT])
+def test_band_factor_pivots_and_solve(n, schedule):
+    path = panel(n, thick_sigma=0.1, imperfection=0.1)
     g, om = _open_both(path)
     g.tangent(cubens.GEN_BASE, cubens.GEN_BASE)
-    det, piv = g.factor(cubens.options(cubens.SOLVER_BAND64))
+    det, piv = g.factor(cubens.options(cubens.SOLVER_BAND64, factor_schedule=schedule))
     ss = O.stiff_sh(om, om.x, om.c1, om.c2, om.c3, om.farea)
     rc, fac, opiv, odet = O.skyfact(om.maxa, ss)
     assert rc == 0 and det == odet == 0
@@ -169,6 +171,23 @@ def test_band_factor_pivots_and_solve():
     K = skyline_dense(ss, om.maxa)
     assert np.linalg.norm(K @ x - b) / np.linalg.norm(b) < 1e-10
     assert relerr(x, O.skysolve(om.maxa, fac, b)) < 1e-9
+    g.close()
+
+
+@pytest.mark.parametrize("n", [16, 96])
+def test_band_factor_schedules_agree_bitwise(n):
+    """The fused and split schedules apply the same tile operations in the
+    same order per tile: identical pivots and solutions (the fused launch's
+    flag hand-offs deliver every panel before its updates read it)."""
+    g = cubens.GpuModel(panel(n, thick_sigma=0.1, imperfection=0.1))
+    g.tangent(cubens.GEN_BASE, cubens.GEN_BASE)
+    b = np.random.default_rng(3).standard_normal(g.neq)
+    out = []
+    for sch in (cubens.FACTOR_SPLIT, cubens.FACTOR_FUSED, cubens.FACTOR_SPLIT, cubens.FACTOR_FUSED):
+        _, piv = g.factor(cubens.options(cubens.SOLVER_BAND64, factor_schedule=sch))
+        out.append((piv.copy(), g.solve(b)))
+    for piv, x in out[1:]:
+        assert np.array_equal(piv, out[0][0]) and np.array_equal(x, out[0][1])
     g.close()
 
 
