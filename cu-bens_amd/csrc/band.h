@@ -38,8 +38,6 @@ struct BandLDLT {
     double* xw = nullptr;    // work vectors (padded)
     double* yw = nullptr;
     int* flag = nullptr;       // error bits: 1 a wait bound hit, 2 an entry outside the band
-    int* flags = nullptr;      // per tile row completion epochs: forward [nt], backward [nt]
-    int epoch = 0;
     int* pflags = nullptr;     // panel completion epochs of the fused factorization launch
     int cepoch = 0;            // epoch of the last fused launch
     // factorization schedule: one fused launch per tile column (default), or

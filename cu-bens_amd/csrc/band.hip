@@ -392,73 +392,103 @@ __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* _
 // One workgroup per tile row, launched in tile order (forward) or reverse tile
 // order (backward).  A workgroup waits only on workgroups with a lower index,
 // which the hardware dispatched before it, so the oldest unfinished workgroup
-// is always resident and the sweep makes progress.  Completion of a tile row
-// is published by a per-row flag holding the sweep's epoch: the values are
-// stored with agent-scope atomics, fenced, then the flag is released; readers
-// acquire the flag and read the values with agent-scope atomics.  Each wait is
-// bounded: past spin_limit polls the workgroup sets *err and leaves, so every
-// wave exits whatever happens.  Contributions of earlier rows are applied
-// while the newest one is still pending; the newest off-diagonal tile and
-// Linv are prefetched into registers so the critical path per row is one flag
-// handoff and two 64x64 products from registers.
-__device__ inline bool wait_flag(int* flags, long J, int epoch, long spin_limit, int* err, int* ok_s) {
-    if (threadIdx.x == 0) {
-        // poll without cache maintenance; one acquire once the flag is seen
-        long n = 0;
-        int f;
-        while ((f = __hip_atomic_load(&flags[J], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) != epoch &&
-               ++n < spin_limit)
-            __builtin_amdgcn_s_sleep(2);
-        __atomic_thread_fence(__ATOMIC_ACQUIRE);
-        *ok_s = (f == epoch);
-        if (!*ok_s) atomicExch(err, 1);
+// is always resident and the sweep makes progress.  A tile row's values are
+// their own completion signal: before a sweep every slot holds SENT, a
+// signalling-NaN bit pattern that no arithmetic produces; the producer writes
+// the 64 values with system-scope stores (written through to memory) and a
+// consumer polls them with system-scope loads until none is SENT.  There is
+// no flag and no cache maintenance on the critical path.  Polling backs off
+// the longer a wait lasts, so the workgroups far ahead of the front do not
+// load memory.  Each wait is bounded: past spin_limit polls the value reads as
+// a quiet NaN and *err is raised, so every workgroup still finishes and the
+// sweep drains.  Contributions of earlier rows are applied while the newest
+// one is still pending, with the next off-diagonal tile prefetched; the
+// newest off-diagonal tile and Linv stay in registers, so the critical path
+// per row is one store-to-load hand-off and two 64x64 products.
+constexpr unsigned long long SENT = 0x7FF4C0DE5E471E11ull;
+
+__device__ __forceinline__ double poll_value(const double* p, long spin_limit, int* err) {
+    const unsigned long long* q = reinterpret_cast<const unsigned long long*>(p);
+    unsigned long long v;
+    long n = 0;
+    while ((v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) == SENT) {
+        if (++n >= spin_limit) {
+            atomicOr(err, 1);
+            return __builtin_nan("");
+        }
+        if (n < 64) __builtin_amdgcn_s_sleep(1);
+        else if (n < 1024) __builtin_amdgcn_s_sleep(8);
+        else __builtin_amdgcn_s_sleep(64);
+    }
+    return __builtin_bit_cast(double, v);
+}
+
+// vin[0..63] = the values of tile row J of v (wave 0 polls, one lane first)
+__device__ __forceinline__ void fetch_row(const double* v, long J, double* vin, long spin_limit, int* err) {
+    if (threadIdx.x < T) {
+        const double* p = v + J * T;
+        if (threadIdx.x == T - 1) (void)poll_value(p + T - 1, spin_limit, err);
+        vin[threadIdx.x] = poll_value(p + threadIdx.x, spin_limit, err);
     }
     __syncthreads();
-    return *ok_s != 0;
 }
 
-__device__ inline void publish(double* v, long I, double val, int* flags, int epoch) {
-    if (threadIdx.x < T) __hip_atomic_store(&v[I * T + threadIdx.x], val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(&flags[I], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void put_value(double* p, double x) {
+    __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// y_I = Linv_I (r_I - sum_{J<I} L_IJ y_J)
+// y_I = Linv_I (r_I - sum_{J<I} L_IJ y_J).  r is read from xr, whose slot I
+// is set to SENT for the backward sweep; y must hold SENT.
 template <class R>
-__global__ void __launch_bounds__(256) k_fwd_pipe(const R* __restrict__ tiles, const R* __restrict__ linv,
-                                                  const double* __restrict__ r, double* y, int* flags, int epoch,
-                                                  long nt, int w, long spin_limit, int* err) {
+__global__ void __launch_bounds__(256) k_fwd_sweep(const R* __restrict__ tiles, const R* __restrict__ linv,
+                                                   double* __restrict__ xr, double* y, long nt, int w,
+                                                   long spin_limit, int* err) {
     __shared__ double vin[T];
     __shared__ double part[4][T];
-    __shared__ int ok_s;
     const long I = blockIdx.x;
     const int tid = threadIdx.x, row = tid % T, q = tid / T, k0 = 16 * q;
-    // prefetch: Linv_I and the newest off-diagonal tile (I, I-1), rows of this thread
-    R lv[16], ln[16];
+    R lv[16], ln[16], lc[16];
     const R* lvp = linv + I * (long)TT;
 #pragma unroll
     for (int k = 0; k < 16; ++k) lv[k] = lvp[(k0 + k) * T + row];
-    const bool has_prev = I > 0;
+    const long J0 = I - w > 0 ? I - w : 0;
+    const bool has_prev = J0 <= I - 1;
     if (has_prev) {
-        const R* lnp = tiles + (I * (w + 1) + (w - 1)) * (long)TT;
+        const R* lnp = tiles + (I * (w + 1) + (w - 1)) * (long)TT;   // tile (I, I-1)
 #pragma unroll
         for (int k = 0; k < 16; ++k) ln[k] = lnp[(k0 + k) * T + row];
     }
-    double acc = (q == 0) ? r[I * T + row] : 0.0;
-    const long J0 = I - w > 0 ? I - w : 0;
-    for (long J = J0; J < I; ++J) {
-        if (!wait_flag(flags, J, epoch, spin_limit, err, &ok_s)) return;
-        if (tid < T) vin[tid] = __hip_atomic_load(&y[J * T + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        if (J == I - 1) {
+    if (J0 < I - 1) {
+        const R* l = tiles + (I * (w + 1) + (J0 - I + w)) * (long)TT;
 #pragma unroll
-            for (int k = 0; k < 16; ++k) acc -= double(ln[k]) * vin[k0 + k];
-        } else {
-            const R* l = tiles + (I * (w + 1) + (J - I + w)) * (long)TT;
+        for (int k = 0; k < 16; ++k) lc[k] = l[(k0 + k) * T + row];
+    }
+    double acc = 0.0;
+    if (q == 0) {
+        acc = xr[I * T + row];
+        xr[I * T + row] = __builtin_bit_cast(double, SENT);
+    }
+    for (long J = J0; J < I - 1; ++J) {
+        R nx[16];
+        const bool more = J + 1 < I - 1;
+        if (more) {
+            const R* l = tiles + (I * (w + 1) + (J + 1 - I + w)) * (long)TT;
 #pragma unroll
-            for (int k = 0; k < 16; ++k) acc -= double(l[(k0 + k) * T + row]) * vin[k0 + k];
+            for (int k = 0; k < 16; ++k) nx[k] = l[(k0 + k) * T + row];
         }
+        fetch_row(y, J, vin, spin_limit, err);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc -= double(lc[k]) * vin[k0 + k];
+        __syncthreads();
+        if (more) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) lc[k] = nx[k];
+        }
+    }
+    if (has_prev) {
+        fetch_row(y, I - 1, vin, spin_limit, err);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc -= double(ln[k]) * vin[k0 + k];
         __syncthreads();
     }
     part[q][row] = acc;
@@ -470,45 +500,57 @@ __global__ void __launch_bounds__(256) k_fwd_pipe(const R* __restrict__ tiles, c
     for (int k = 0; k < 16; ++k) s += double(lv[k]) * vin[k0 + k];
     part[q][row] = s;
     __syncthreads();
-    double yi = (tid < T) ? part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid] : 0.0;
-    publish(y, I, yi, flags, epoch);
+    if (tid < T) put_value(&y[I * T + tid], part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid]);
 }
 
-// x_I = Linv_I^T (y_I / D_I - sum_{J>I} L_JI^T x_J); workgroup g handles I = nt-1-g
+// x_I = Linv_I^T (y_I / D_I - sum_{J>I} L_JI^T x_J); workgroup g handles
+// I = nt-1-g.  x must hold SENT.
 template <class R>
-__global__ void __launch_bounds__(256) k_bwd_pipe(const R* __restrict__ tiles, const R* __restrict__ linv,
-                                                  const double* __restrict__ y, const double* __restrict__ piv,
-                                                  double* x, int* flags, int epoch, long nt, int w,
-                                                  long spin_limit, int* err) {
+__global__ void __launch_bounds__(256) k_bwd_sweep(const R* __restrict__ tiles, const R* __restrict__ linv,
+                                                   const double* __restrict__ y, const double* __restrict__ piv,
+                                                   double* x, long nt, int w, long spin_limit, int* err) {
     __shared__ double vin[T];
     __shared__ double part[4][T];
-    __shared__ int ok_s;
     const long I = nt - 1 - blockIdx.x;
     const int tid = threadIdx.x, col = tid % T, q = tid / T, k0 = 16 * q;
-    R lv[16], ln[16];
+    R lv[16], ln[16], lc[16];
     const R* lvp = linv + I * (long)TT;
 #pragma unroll
     for (int k = 0; k < 16; ++k) lv[k] = lvp[col * T + k0 + k];   // Linv[k][col]
-    const bool has_next = I + 1 < nt;
+    const long J1 = (I + w < nt - 1) ? I + w : nt - 1;
+    const bool has_next = I + 1 <= J1;
     if (has_next) {
         const R* lnp = tiles + ((I + 1) * (w + 1) + (w - 1)) * (long)TT;   // tile (I+1, I)
 #pragma unroll
         for (int k = 0; k < 16; ++k) ln[k] = lnp[col * T + k0 + k];
     }
+    if (J1 > I + 1) {
+        const R* l = tiles + (J1 * (w + 1) + (I - J1 + w)) * (long)TT;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) lc[k] = l[col * T + k0 + k];
+    }
     double acc = (q == 0) ? y[I * T + col] / piv[I * T + col] : 0.0;
-    const long J1 = (I + w < nt - 1) ? I + w : nt - 1;
-    for (long J = J1; J > I; --J) {
-        if (!wait_flag(flags, J, epoch, spin_limit, err, &ok_s)) return;
-        if (tid < T) vin[tid] = __hip_atomic_load(&x[J * T + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        if (J == I + 1) {
+    for (long J = J1; J > I + 1; --J) {
+        R nx[16];
+        const bool more = J - 1 > I + 1;
+        if (more) {
+            const R* l = tiles + ((J - 1) * (w + 1) + (I - (J - 1) + w)) * (long)TT;
 #pragma unroll
-            for (int k = 0; k < 16; ++k) acc -= double(ln[k]) * vin[k0 + k];
-        } else {
-            const R* l = tiles + (J * (w + 1) + (I - J + w)) * (long)TT;
-#pragma unroll
-            for (int k = 0; k < 16; ++k) acc -= double(l[col * T + k0 + k]) * vin[k0 + k];
+            for (int k = 0; k < 16; ++k) nx[k] = l[col * T + k0 + k];
         }
+        fetch_row(x, J, vin, spin_limit, err);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc -= double(lc[k]) * vin[k0 + k];
+        __syncthreads();
+        if (more) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) lc[k] = nx[k];
+        }
+    }
+    if (has_next) {
+        fetch_row(x, I + 1, vin, spin_limit, err);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc -= double(ln[k]) * vin[k0 + k];
         __syncthreads();
     }
     part[q][col] = acc;
@@ -520,13 +562,17 @@ __global__ void __launch_bounds__(256) k_bwd_pipe(const R* __restrict__ tiles, c
     for (int k = 0; k < 16; ++k) s += double(lv[k]) * vin[k0 + k];
     part[q][col] = s;
     __syncthreads();
-    double xi = (tid < T) ? part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid] : 0.0;
-    publish(x, I, xi, flags, epoch);
+    if (tid < T) put_value(&x[I * T + tid], part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid]);
 }
 
-__global__ void k_copy_pad(double* __restrict__ dst, const double* __restrict__ src, long n, long ntot) {
+// dst = rhs padded with zeros to ntot; y = SENT
+__global__ void k_sweep_prep(double* __restrict__ dst, const double* __restrict__ src, double* __restrict__ y,
+                             long n, long ntot) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i < ntot) dst[i] = i < n ? src[i] : 0.0;
+    if (i < ntot) {
+        dst[i] = i < n ? src[i] : 0.0;
+        y[i] = __builtin_bit_cast(double, SENT);
+    }
 }
 
 }  // namespace
@@ -575,12 +621,9 @@ void BandLDLT<R>::setup(long n_, long bandwidth, hipStream_t s_) {
     xw = dalloc<double>(size_t(nt) * T);
     yw = dalloc<double>(size_t(nt) * T);
     flag = dalloc<int>(1);
-    flags = dalloc<int>(size_t(2 * nt));
     pflags = dalloc<int>(size_t(w + 1) * FLAG_STRIDE);
-    CB_CHECK(hipMemsetAsync(flags, 0, sizeof(int) * size_t(2 * nt), s));
     CB_CHECK(hipMemsetAsync(pflags, 0, sizeof(int) * size_t(w + 1) * FLAG_STRIDE, s));
     CB_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
-    epoch = 0;
     cepoch = 0;
     piv_host.resize(size_t(n));
     CB_CHECK(hipStreamSynchronize(s));
@@ -588,11 +631,11 @@ void BandLDLT<R>::setup(long n_, long bandwidth, hipStream_t s_) {
 
 template <class R>
 void BandLDLT<R>::release() {
-    void* ps[] = {tiles, linv, piv, xw, yw, flag, flags, pflags};
+    void* ps[] = {tiles, linv, piv, xw, yw, flag, pflags};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     tiles = nullptr; linv = nullptr; piv = nullptr; xw = nullptr; yw = nullptr; flag = nullptr;
-    flags = nullptr; pflags = nullptr;
+    pflags = nullptr;
     clock.release();
 }
 
@@ -652,16 +695,16 @@ int BandLDLT<R>::factor(const Dev& d) {
 
 template <class R>
 void BandLDLT<R>::solve(const double* rhs, double* out) {
-    hipLaunchKernelGGL(k_copy_pad, dim3(int((nt * T + 255) / 256)), dim3(256), 0, s, xw, rhs, n, nt * T);
-    const long spin_limit = 1L << 24;
-    epoch += 1;
+    CB_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_sweep_prep, dim3(int((nt * T + 255) / 256)), dim3(256), 0, s, xw, rhs, yw, n, nt * T);
+    const long spin_limit = 1L << 22;
     if (clock.on(KT_FWD)) clock.begin(KT_FWD, s);
-    hipLaunchKernelGGL((k_fwd_pipe<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, xw, yw, flags, epoch, nt, w,
-                       spin_limit, flag);
+    hipLaunchKernelGGL((k_fwd_sweep<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, xw, yw, nt, w, spin_limit,
+                       flag);
     if (clock.on(KT_FWD)) clock.end(s);
     if (clock.on(KT_BWD)) clock.begin(KT_BWD, s);
-    hipLaunchKernelGGL((k_bwd_pipe<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, yw, piv, xw, flags + nt,
-                       epoch, nt, w, spin_limit, flag);
+    hipLaunchKernelGGL((k_bwd_sweep<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, yw, piv, xw, nt, w,
+                       spin_limit, flag);
     if (clock.on(KT_BWD)) clock.end(s);
     CB_CHECK(hipGetLastError());
     CB_CHECK(hipMemcpyAsync(out, xw, sizeof(double) * size_t(n), hipMemcpyDeviceToDevice, s));
