@@ -44,6 +44,9 @@ def workload_file(n: int) -> str:
 
 
 PEAK_FP64_TFLOPS = 78.6
+# every 16th factorization launch is bracketed by HIP events (the sweeps every
+# time): the average launch duration with little event overhead in the window
+TIMING_EVERY = 16
 PEAK_HBM_GBS = 8000.0
 TILE = 64
 
@@ -117,6 +120,7 @@ def run_path(path: str, solver: int, warmup: int, steps: int, tol: float, timing
         opt = cubens.options(solver, krylov_tol=tol, restart=30, maxit=400,
                              judge=0 if solver == cubens.SOLVER_BAND64 else 1,
                              max_steps=warmup + steps, keep_d=1, kernel_timing=timing,
+                             kernel_timing_every=TIMING_EVERY,
                              factor_schedule=schedule)
         r = g.run(opt)
         info = g.info

@@ -12,14 +12,30 @@ namespace cb {
 // KT_PANEL and KT_UPDATE are the launches of the split schedule.
 enum { KT_COLUMN = 0, KT_DIAG = 1, KT_PANEL = 2, KT_UPDATE = 3, KT_FWD = 4, KT_BWD = 5, KT_N = 6 };
 
+// Launches of a selected family are bracketed by two events.  Of the
+// factorization families (launched per tile column) only every `every`-th
+// launch is timed, which keeps the event overhead out of the factorization;
+// the sweeps are always timed.  launches counts every launch issued and ms is
+// the timed total scaled to all of them, so ms / launches is the average
+// launch duration.
 struct KernelClock {
     int mask = 0;                        // bit f: time family f
+    int every = 1;
     double ms[KT_N] = {};
     long launches[KT_N] = {};
+    double ms_timed[KT_N] = {};
+    long n_timed[KT_N] = {};
     std::vector<hipEvent_t> ev;
     std::vector<int> fam;                // family of each recorded pair
     size_t used = 0;
     bool on(int f) const { return (mask >> f) & 1; }
+    // whether to time this launch of family f
+    bool take(int f) {
+        if (!on(f)) return false;
+        const long k = launches[f]++;
+        const long e = (f == KT_FWD || f == KT_BWD || every < 1) ? 1 : every;
+        return k % e == 0;
+    }
     void begin(int f, hipStream_t s);
     void end(hipStream_t s);
     void collect();                      // after a stream sync

@@ -154,14 +154,18 @@ template <> struct TileAcc<double> {
             }
         }
     }
-    template <class F> __device__ __forceinline__ void each(F&& f) const {
+    static constexpr int N = 8;   // entries per thread
+    template <class F> __device__ __forceinline__ void each_i(F&& f) const {
         const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
             const int blk = 2 * wv + b, bi = blk & 3, bj = blk >> 2;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) f(16 * bi + (l & 15), 16 * bj + (l >> 4) + 4 * e, acc[b][e]);
+            for (int e = 0; e < 4; ++e) f(16 * bi + (l & 15), 16 * bj + (l >> 4) + 4 * e, acc[b][e], 4 * b + e);
         }
+    }
+    template <class F> __device__ __forceinline__ void each(F&& f) const {
+        each_i([&](int row, int col, double v, int) { f(row, col, v); });
     }
 };
 
@@ -184,14 +188,25 @@ template <> struct TileAcc<float> {
                 for (int v = 0; v < 2; ++v) acc[u][v] += av[u] * bv[v];
         }
     }
-    template <class F> __device__ __forceinline__ void each(F&& f) const {
+    static constexpr int N = 8;   // entries per thread
+    template <class F> __device__ __forceinline__ void each_i(F&& f) const {
         const int tr = threadIdx.x & 15, tc = threadIdx.x >> 4;
 #pragma unroll
         for (int v = 0; v < 2; ++v)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) f(tr + 16 * u, tc + 32 * v, acc[u][v]);
+            for (int u = 0; u < 4; ++u) f(tr + 16 * u, tc + 32 * v, acc[u][v], 4 * v + u);
+    }
+    template <class F> __device__ __forceinline__ void each(F&& f) const {
+        each_i([&](int row, int col, float v, int) { f(row, col, v); });
     }
 };
+
+// The tile entries at the positions TileAcc<R>::each_i hands out, in its order.
+template <class R>
+__device__ __forceinline__ void gather_tile(const R* __restrict__ tile, R* buf) {
+    TileAcc<R> pos;
+    pos.each_i([&](int row, int col, R, int i) { buf[i] = tile[col * T + row]; });
+}
 
 // ---- D_K --------------------------------------------------------------------
 // Thread (r = tid % 64, g = tid / 64) keeps the entries (r, j), j = g + NWAVE m,
@@ -326,10 +341,14 @@ __global__ void __launch_bounds__(NTH) k_tile_update(R* __restrict__ tiles, cons
 
 // ---- fused schedule -----------------------------------------------------------
 // Launch K (K = -1 .. nt-2), m = min(w, nt-1-K) (0 for K = -1):
-//   workgroup 0          P_{K+1,K}, U_{K+1,K+1,K} in LDS, D_{K+1}; raises flag 0
+//   workgroup 0          P_{K+1,K} and U_{K+1,K+1,K} in LDS only, then D_{K+1};
+//                        raises flag w once it has read A_{K+1,K}
 //   workgroups 1..m-1    P_{K+1+b,K}; raise flag b
+//   workgroup m          P_{K+1,K} again (the same arithmetic), stored once
+//                        flag w is up; raises flag 0
 //   the rest             U_{I,J,K} for the pairs other than (K+1,K+1), row by row
-// Grid: 1 + (m - 1) + (m(m+1)/2 - 1) workgroups for m > 0, 1 for m = 0.
+// Grid: m + m(m+1)/2 workgroups for m > 0, 1 for m = 0.  Workgroup 0 neither
+// stores nor waits before its diagonal step.
 template <class R>
 __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* __restrict__ linv,
                                                      double* __restrict__ piv, long K, int w, long nt,
@@ -349,38 +368,49 @@ __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* _
             diag_core<R>([&](int j, int r) { return dtile[j * T + r]; }, linv + D * (long)TT, piv, D, bc);
             return;
         }
-        R* ptile = tiles + tile_index(D, K, w) * TT;
-        stage<false>(xs, ptile);
+        R pre[TileAcc<R>::N];
+        gather_tile(dtile, pre);
+        stage<false>(xs, tiles + tile_index(D, K, w) * TT);
         stage<false>(ys, linv + K * (long)TT);
-        __syncthreads();
+        raise_flag(flags, w, epoch);   // A_{K+1,K} has been read (includes the barrier)
         TileAcc<R> acc;
         acc.product(xs, ys);
         __syncthreads();
-        // L = P_{K+1,K}: to memory for the updates, (L D, L) to LDS for U_{K+1,K+1,K}
+        // (L D, L) of L = P_{K+1,K} to LDS for U_{K+1,K+1,K}
         acc.each([&](int row, int col, R v) {
             const R lv = v * (R(1) / R(dk[col]));
-            cst<true>(&ptile[col * T + row], lv);
             xs[col][row] = lv * R(dk[col]);
             ys[col][row] = lv;
         });
-        raise_flag(flags, 0, epoch);
+        __syncthreads();
         acc.product(xs, ys);
         __syncthreads();
-        acc.each([&](int row, int col, R v) { xs[col][row] = dtile[col * T + row] - v; });
+        acc.each_i([&](int row, int col, R v, int i) { xs[col][row] = pre[i] - v; });
         __syncthreads();
         diag_core<R>([&](int j, int r) { return xs[j][r]; }, linv + D * (long)TT, piv, D, bc);
         return;
     }
     __syncthreads();   // dk
-    if (b < m) {
-        const long I = K + 1 + b;
+    if (b <= m) {
+        const long pb = (b == m) ? 0 : b;
+        const long I = K + 1 + pb;
         R* tile = tiles + tile_index(I, K, w) * TT;
-        panel_task<true, R>(tile, tile, linv + K * (long)TT, dk, xs, ys);
-        raise_flag(flags, b, epoch);
+        if (pb == 0) {
+            // stage, then wait until workgroup 0 has read the tile before overwriting it
+            stage<false>(xs, tile);
+            stage<false>(ys, linv + K * (long)TT);
+            const bool ok = wait_flag_bounded(flags, w, epoch, spin_limit, err, &ok_s);
+            TileAcc<R> acc;
+            acc.product(xs, ys);
+            if (ok) acc.each([&](int row, int col, R v) { cst<true>(&tile[col * T + row], v * (R(1) / R(dk[col]))); });
+        } else {
+            panel_task<true, R>(tile, tile, linv + K * (long)TT, dk, xs, ys);
+        }
+        raise_flag(flags, pb, epoch);
         return;
     }
     long r, s;
-    tri_pair(b - m + 1, r, s);
+    tri_pair(b - m, r, s);
     if (!wait_flag_bounded(flags, r, epoch, spin_limit, err, &ok_s)) return;
     if (s != r && !wait_flag_bounded(flags, s, epoch, spin_limit, err, &ok_s)) return;
     const long I = K + 1 + r, J = K + 1 + s;
@@ -593,9 +623,11 @@ void KernelClock::collect() {
     for (size_t i = 0; i < fam.size(); ++i) {
         float t = 0;
         CB_CHECK(hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]));
-        ms[fam[i]] += t;
-        launches[fam[i]] += 1;
+        ms_timed[fam[i]] += t;
+        n_timed[fam[i]] += 1;
     }
+    for (int f = 0; f < KT_N; ++f)
+        ms[f] = n_timed[f] > 0 ? ms_timed[f] * double(launches[f]) / double(n_timed[f]) : 0.0;
     fam.clear();
     used = 0;
 }
@@ -656,27 +688,31 @@ int BandLDLT<R>::factor(const Dev& d) {
         const long spin_limit = 1L << 22;
         for (long K = -1; K < nt - 1; ++K) {
             const long m = (K < 0) ? 0 : std::min<long>(w, nt - 1 - K);
-            const long grid = m > 0 ? m + m * (m + 1) / 2 - 1 : 1;
+            const long grid = m > 0 ? m + m * (m + 1) / 2 : 1;
             cepoch += 1;
-            if (clock.on(KT_COLUMN)) clock.begin(KT_COLUMN, s);
+            const bool tk_column = clock.take(KT_COLUMN);
+            if (tk_column) clock.begin(KT_COLUMN, s);
             hipLaunchKernelGGL((k_band_column<R>), dim3(int(grid)), dim3(NTH), 0, s, tiles, linv, piv, K, w, nt,
                                pflags, cepoch, spin_limit, flag + 0);
-            if (clock.on(KT_COLUMN)) clock.end(s);
+            if (tk_column) clock.end(s);
         }
     } else {
         for (long Kt = 0; Kt < nt; ++Kt) {
-            if (clock.on(KT_DIAG)) clock.begin(KT_DIAG, s);
+            const bool tk_diag = clock.take(KT_DIAG);
+            if (tk_diag) clock.begin(KT_DIAG, s);
             hipLaunchKernelGGL((k_tile_diag<R>), dim3(1), dim3(NTH), 0, s, tiles, linv, piv, Kt, w);
-            if (clock.on(KT_DIAG)) clock.end(s);
+            if (tk_diag) clock.end(s);
             const long m = std::min<long>(w, nt - 1 - Kt);
             if (m > 0) {
-                if (clock.on(KT_PANEL)) clock.begin(KT_PANEL, s);
+                const bool tk_panel = clock.take(KT_PANEL);
+                if (tk_panel) clock.begin(KT_PANEL, s);
                 hipLaunchKernelGGL((k_tile_panel<R>), dim3(int(m)), dim3(NTH), 0, s, tiles, linv, piv, Kt, w);
-                if (clock.on(KT_PANEL)) clock.end(s);
-                if (clock.on(KT_UPDATE)) clock.begin(KT_UPDATE, s);
+                if (tk_panel) clock.end(s);
+                const bool tk_update = clock.take(KT_UPDATE);
+                if (tk_update) clock.begin(KT_UPDATE, s);
                 hipLaunchKernelGGL((k_tile_update<R>), dim3(int(m * (m + 1) / 2)), dim3(NTH), 0, s, tiles, piv,
                                    Kt, w);
-                if (clock.on(KT_UPDATE)) clock.end(s);
+                if (tk_update) clock.end(s);
             }
         }
     }
@@ -698,14 +734,16 @@ void BandLDLT<R>::solve(const double* rhs, double* out) {
     CB_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_sweep_prep, dim3(int((nt * T + 255) / 256)), dim3(256), 0, s, xw, rhs, yw, n, nt * T);
     const long spin_limit = 1L << 22;
-    if (clock.on(KT_FWD)) clock.begin(KT_FWD, s);
+    const bool tk_fwd = clock.take(KT_FWD);
+    if (tk_fwd) clock.begin(KT_FWD, s);
     hipLaunchKernelGGL((k_fwd_sweep<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, xw, yw, nt, w, spin_limit,
                        flag);
-    if (clock.on(KT_FWD)) clock.end(s);
-    if (clock.on(KT_BWD)) clock.begin(KT_BWD, s);
+    if (tk_fwd) clock.end(s);
+    const bool tk_bwd = clock.take(KT_BWD);
+    if (tk_bwd) clock.begin(KT_BWD, s);
     hipLaunchKernelGGL((k_bwd_sweep<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, yw, piv, xw, nt, w,
                        spin_limit, flag);
-    if (clock.on(KT_BWD)) clock.end(s);
+    if (tk_bwd) clock.end(s);
     CB_CHECK(hipGetLastError());
     CB_CHECK(hipMemcpyAsync(out, xw, sizeof(double) * size_t(n), hipMemcpyDeviceToDevice, s));
     int bad = 0;

@@ -72,6 +72,7 @@ cb_options default_options() {
     o.keep_d = 1;
     o.kernel_timing = 0;
     o.factor_schedule = CB_FACTOR_FUSED;
+    o.kernel_timing_every = 1;
     return o;
 }
 
@@ -86,13 +87,15 @@ cb::RunOptions to_run(const cb_options& o) {
     r.keep_d = o.keep_d;
     r.kernel_timing = o.kernel_timing;
     r.factor_schedule = o.factor_schedule;
+    r.kernel_timing_every = o.kernel_timing_every;
     return r;
 }
 
 bool same_opts(const cb_options& a, const cb_options& b) {
     return a.solver == b.solver && a.krylov_tol == b.krylov_tol && a.restart == b.restart &&
            a.maxit == b.maxit && a.judge == b.judge && a.max_steps == b.max_steps && a.keep_d == b.keep_d &&
-           a.kernel_timing == b.kernel_timing && a.factor_schedule == b.factor_schedule;
+           a.kernel_timing == b.kernel_timing && a.factor_schedule == b.factor_schedule &&
+           a.kernel_timing_every == b.kernel_timing_every;
 }
 
 double normal01(uint64_t& st) {

@@ -45,6 +45,8 @@ Analysis::Analysis(Model& m, Dev& d, const RunOptions& o) : m_(m), d_(d), o_(o) 
     if (o.solver == SOLVER_BAND32_FGMRES) b32_.setup(n, m.bandwidth, d.s);
     b64_.clock.mask = o.kernel_timing;
     b32_.clock.mask = o.kernel_timing;
+    b64_.clock.every = o.kernel_timing_every;
+    b32_.clock.every = o.kernel_timing_every;
     b64_.split = o.factor_schedule == 1;
     b32_.split = o.factor_schedule == 1;
     if (o.solver != SOLVER_BAND64) gm_.setup(d.nslot, o.restart);

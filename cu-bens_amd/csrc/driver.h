@@ -32,6 +32,7 @@ struct RunOptions {
     int keep_d = 1;         // copy d to the host at every converged step
     int kernel_timing = 0;  // KT_* bitmask of band kernels timed with HIP events
     int factor_schedule = 0;  // 0 fused launch per tile column, 1 split diag/panel/update launches
+    int kernel_timing_every = 1;  // time every n-th launch of a timed family
 };
 
 struct RunStats {

@@ -32,7 +32,8 @@ class Info(C.Structure):
 class Options(C.Structure):
     _fields_ = [("solver", C.c_int), ("krylov_tol", C.c_double), ("restart", C.c_int),
                 ("maxit", C.c_int), ("judge", C.c_int), ("max_steps", C.c_int),
-                ("keep_d", C.c_int), ("kernel_timing", C.c_int), ("factor_schedule", C.c_int)]
+                ("keep_d", C.c_int), ("kernel_timing", C.c_int), ("factor_schedule", C.c_int),
+                ("kernel_timing_every", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -54,9 +55,9 @@ FACTOR_FUSED, FACTOR_SPLIT = 0, 1
 
 def options(solver: int = SOLVER_BAND64, krylov_tol: float = 1e-8, restart: int = 30,
             maxit: int = 20000, judge: int = 1, max_steps: int = 1 << 30, keep_d: int = 1,
-            kernel_timing: int = 0, factor_schedule: int = 0) -> Options:
+            kernel_timing: int = 0, factor_schedule: int = 0, kernel_timing_every: int = 1) -> Options:
     return Options(solver, krylov_tol, restart, maxit, judge, max_steps, keep_d, kernel_timing,
-                   factor_schedule)
+                   factor_schedule, kernel_timing_every)
 
 
 _lib = None

@@ -52,6 +52,7 @@ typedef struct {
     int keep_d;          /* keep the displacement path for cb_path         */
     int kernel_timing;   /* bitmask of CB_KT_* kernel families timed with HIP events */
     int factor_schedule; /* CB_FACTOR_* schedule of the band factorization  */
+    int kernel_timing_every; /* time every n-th launch of a timed family (<= 1: all) */
 } cb_options;
 
 /* Schedules of the band LDL^T factorization (same tile arithmetic, bitwise
@@ -147,8 +148,9 @@ int cb_get_path(const cb_model *m, double *lpf, int *itecnt, double *d);
  * seconds spent in the FP64 pivot judge, Newton iterations so far (no
  * reference counterpart: the reference keeps no timers). */
 int cb_get_step_stats(const cb_model *m, double *t_wall, double *t_judge, long *newton);
-/* Accumulated HIP-event times (ms) and launch counts of the band kernel
- * families [CB_KT_N] selected by cb_options.kernel_timing, for the FP64 (precision
+/* Accumulated HIP-event times (ms) and counts of the timed launches of the
+ * band kernel families [CB_KT_N] selected by cb_options.kernel_timing (every
+ * kernel_timing_every-th launch is timed; ms / launches is the average), for the FP64 (precision
  * 64) or FP32 (32) factor; step >= 0 returns the totals at that converged step
  * of the last run, step < 0 the totals now (no reference counterpart). */
 int cb_get_kernel_times(const cb_model *m, int precision, int step, double *ms, long *launches);
