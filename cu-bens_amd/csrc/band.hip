@@ -209,58 +209,253 @@ __device__ __forceinline__ void gather_tile(const R* __restrict__ tile, R* buf) 
 }
 
 // ---- D_K --------------------------------------------------------------------
-// Thread (r = tid % 64, g = tid / 64) keeps the entries (r, j), j = g + NWAVE m,
-// of the tile in registers: the Schur complement a[r][j] while j is still
-// ahead of the elimination step c, afterwards the entry B[r][j] of B = L^{-1},
-// which the same elimination applied to the identity produces (B[r][:] -=
-// l_r B[c][:]).  Step c broadcasts one vector through LDS: column c of the
-// Schur complement in its entries j >= c and row c of B in its entries j < c
-// (double-buffered, one barrier per step).  The steps are unrolled, so every
-// register index is static; the multiplier is zero on the rows r <= c, which
-// makes the update branch-free.  Entries above the diagonal (j > r) are
-// updated too but never read; they start at zero.  Writes D to piv and
-// Linv = L^{-1} (zeros above the diagonal).  L_KK itself is read by neither
-// the panels and updates nor the solves and is not stored.
-__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
-__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+// Both versions take the 64x64 tile S (lower part; the part above the
+// diagonal is never read) in LDS as xs[col][row], write D to piv and
+// Linv = L^{-1} (zeros above the diagonal) to lout, and overwrite xs.  L_KK
+// itself is read by neither the panels and updates nor the solves and is not
+// stored.
 
-template <class R, class Load>
-__device__ __forceinline__ void diag_core(Load&& load, R* __restrict__ lout, double* __restrict__ piv, long Kt,
-                                          R (*bc)[T]) {
+// FP32: unblocked.  Thread (r = tid % 64, g = tid / 64) keeps the entries
+// (r, j), j = g + NWAVE m, in registers: the Schur complement a[r][j] while j
+// is still ahead of the elimination step c, afterwards the entry B[r][j] of
+// B = L^{-1}, which the same elimination applied to the identity produces
+// (B[r][:] -= l_r B[c][:]).  Step c broadcasts one vector through LDS: column
+// c of the Schur complement in its entries j >= c and row c of B in its
+// entries j < c (double-buffered, one barrier per step).  The steps are
+// unrolled, so every register index is static; the multiplier is zero on the
+// rows r <= c, which makes the update branch-free.
+template <class R> struct DiagScratch;
+template <> struct DiagScratch<float> { float bc[2][T]; };
+template <> struct DiagScratch<double> { double tsc[5][256]; double ub[16]; };
+
+__device__ __forceinline__ void diag_task(float (*xs)[T], float* /*ys*/, double* /*dd*/, DiagScratch<float>& sm,
+                                          float* __restrict__ lout, double* __restrict__ piv, long Kt) {
     constexpr int S = T / NWAVE;
     const int r = threadIdx.x & 63;
     const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    R v[S];
+    float v[S];
 #pragma unroll
     for (int m = 0; m < S; ++m) {
         const int j = g + NWAVE * m;
-        const R x = load(j, r);
-        v[m] = (j <= r) ? x : R(0);
+        const float x = xs[j][r];
+        v[m] = (j <= r) ? x : 0.0f;
     }
-    R dr = R(0);
+    float dr = 0.0f;
 #pragma unroll
     for (int c = 0; c < T; ++c) {
         const int p = c & 1;
-        if (g == c % NWAVE && r >= c) bc[p][r] = v[c / NWAVE];
+        if (g == c % NWAVE && r >= c) sm.bc[p][r] = v[c / NWAVE];
         if (r == c) {
 #pragma unroll
             for (int m = 0; m < S; ++m)
-                if (g + NWAVE * m < c) bc[p][g + NWAVE * m] = v[m];
+                if (g + NWAVE * m < c) sm.bc[p][g + NWAVE * m] = v[m];
         }
         __syncthreads();
-        const R d = bc[p][c];
-        const R l = bc[p][r] / d;
-        const R le = (r > c) ? l : R(0);
+        const float d = sm.bc[p][c];
+        const float l = sm.bc[p][r] / d;
+        const float le = (r > c) ? l : 0.0f;
 #pragma unroll
-        for (int m = 0; m < S; ++m) v[m] = fmadd(-le, bc[p][g + NWAVE * m], v[m]);
-        if (g == c % NWAVE) v[c / NWAVE] = (r > c) ? -l : ((r == c) ? R(1) : v[c / NWAVE]);
+        for (int m = 0; m < S; ++m) v[m] = __builtin_fmaf(-le, sm.bc[p][g + NWAVE * m], v[m]);
+        if (g == c % NWAVE) v[c / NWAVE] = (r > c) ? -l : ((r == c) ? 1.0f : v[c / NWAVE]);
         dr = (r == c) ? d : dr;
     }
     if (g == 0) piv[Kt * T + r] = double(dr);
 #pragma unroll
     for (int m = 0; m < S; ++m) {
         const int j = g + NWAVE * m;
-        lout[j * T + r] = (j <= r) ? v[m] : R(0);
+        lout[j * T + r] = (j <= r) ? v[m] : 0.0f;
+    }
+}
+
+// FP64: blocked by 16.  For each column block k:
+//  - wavefront 0 factorizes the 64x16 panel (rows 16k..63, lane = row):
+//    16 pivot steps, each broadcasting column c of the block through LDS; the
+//    pivot and the next step's critical entry travel by readlane, and the
+//    reciprocal of the next pivot is formed while the rest of the step's
+//    updates are issued.  The block lanes also form column p of Linv_kk by
+//    the same elimination applied to the identity.  Multipliers are
+//    l = a * (1/d) with the reciprocal refined by two Newton steps.
+//  - the other wavefronts meanwhile form the off-diagonal blocks of Linv
+//    that are ready, Linv_ij = -Linv_ii sum_{k=j}^{i-1} L_ik Linv_kj, as
+//    16x16x16 FP64 MFMA products, avoiding wavefront 4 (which shares a SIMD
+//    with wavefront 0);
+//  - after a barrier, up to six wavefronts update the trailing blocks
+//    A_ij -= L_ik D_k L_jk^T (16x16x16 FP64 MFMA).
+// Linv is kept in ys column-major with the row index swizzled by the column
+// (ysw), which makes both row and column accesses free of bank conflicts.
+
+__device__ __forceinline__ int ysw(int col, int row) { return col * T + (row ^ (col & 15)); }
+
+__device__ __forceinline__ double lane_value(double x, int lane) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = __builtin_amdgcn_readlane(unsigned(u), lane);
+    const unsigned hi = __builtin_amdgcn_readlane(unsigned(u >> 32), lane);
+    return __builtin_bit_cast(double, (unsigned long long)hi << 32 | lo);
+}
+
+__device__ __forceinline__ double recip(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-d, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+// D[m][n] = sum_t A(m, t) B(t, n) over 16x16 blocks on one wavefront (FP64
+// MFMA 16x16x4: lane l supplies A(l%16, t) and B(t, l%16), t = t0 + l/16);
+// element e of lane l is D[l/16 + 4e][l%16].
+template <class FA, class FB>
+__device__ __forceinline__ d4 mm16(FA A, FB B, d4 acc) {
+    const int l = threadIdx.x & 63, rr = l & 15, lg = l >> 4;
+#pragma unroll
+    for (int t0 = 0; t0 < 16; t0 += 4) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(A(rr, t0 + lg), B(t0 + lg, rr), acc, 0, 0, 0);
+    return acc;
+}
+
+__device__ __forceinline__ void diag_task(double (*xs)[T], double* ys, double* dd, DiagScratch<double>& sm,
+                                          double* __restrict__ lout, double* __restrict__ piv, long Kt) {
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, rr = l & 15, lg = l >> 4;
+    double(*tsc)[256] = sm.tsc;
+    // operand views: L_ik (rows r, cols t), Linv_ij as (t, c) and as (r, t), T slot s as (t, c)
+    auto Lb = [&](int i, int k) { return [&, i, k](int r, int t) { return xs[16 * k + t][16 * i + r]; }; };
+    auto Vb = [&](int i, int j) { return [&, i, j](int t, int c) { return ys[ysw(16 * j + c, 16 * i + t)]; }; };
+    auto Vr = [&](int i, int j) { return [&, i, j](int r, int t) { return ys[ysw(16 * j + t, 16 * i + r)]; }; };
+    auto Tb = [&](int s) { return [&, s](int t, int c) { return tsc[s][t * 16 + c]; }; };
+    auto getT = [&](int s) {
+        d4 acc;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[e] = tsc[s][(lg + 4 * e) * 16 + rr];
+        return acc;
+    };
+    auto putT = [&](int s, d4 acc) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) tsc[s][(lg + 4 * e) * 16 + rr] = acc[e];
+        __builtin_amdgcn_wave_barrier();
+    };
+    auto putV = [&](int i, int j, d4 acc) {   // Linv_ij = -acc
+#pragma unroll
+        for (int e = 0; e < 4; ++e) ys[ysw(16 * j + rr, 16 * i + lg + 4 * e)] = -acc[e];
+        __builtin_amdgcn_wave_barrier();
+    };
+    const d4 z = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int k = 0; k < 4; ++k) {
+        if (wv == 0) {
+            const int r = l, base = 16 * k, p = r - base;
+            double v[16], bcol[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                v[q] = xs[base + q][r];
+                bcol[q] = (q == p) ? 1.0 : 0.0;
+            }
+            if (p >= 0 && p < 16) sm.ub[p] = v[0];
+            __builtin_amdgcn_wave_barrier();
+            double d = lane_value(v[0], base);
+            double rd = recip(d);
+            double dr = 0.0;
+            double U[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) U[q] = sm.ub[q];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                int po = p;
+                __asm__ volatile("" : "+v"(po));   // keeps the per-step lane tests in the step
+                const bool below = po > c;
+                dr = (po == c) ? d : dr;
+                const double le = below ? v[c] * rd : 0.0;
+                const double fb = below ? 0.0 : bcol[c] * rd;
+                double dn = 0.0, rdn = 0.0, un = 0.0;
+                if (c + 1 < 16) {
+                    v[c + 1] = __builtin_fma(-le, U[c + 1], v[c + 1]);
+                    __builtin_amdgcn_wave_barrier();
+                    if (po >= 0 && po < 16) sm.ub[po] = v[c + 1];
+                    __builtin_amdgcn_wave_barrier();
+                    dn = lane_value(v[c + 1], base + c + 1);
+                    if (c + 2 < 16) un = lane_value(v[c + 1], base + c + 2);
+                    rdn = recip(dn);
+                }
+#pragma unroll
+                for (int q = c + 2; q < 16; ++q) v[q] = __builtin_fma(-le, U[q], v[q]);
+#pragma unroll
+                for (int q = c + 1; q < 16; ++q) bcol[q] = __builtin_fma(-fb, U[q], bcol[q]);
+                v[c] = below ? le : v[c];
+                if (c + 1 < 16) {
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) U[q] = (q == c + 2) ? un : sm.ub[q];
+                    d = dn;
+                    rd = rdn;
+                }
+                // keep every update in its own step (no deferred chains)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    __asm__ volatile("" : "+v"(v[q]));
+                    __asm__ volatile("" : "+v"(bcol[q]));
+                }
+            }
+            if (p >= 0) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) xs[base + q][r] = v[q];
+            }
+            if (p >= 0 && p < 16) {
+                dd[r] = dr;
+                piv[Kt * T + r] = dr;
+#pragma unroll
+                for (int q = 0; q < 16; ++q) ys[ysw(base + p, base + q)] = (q >= p) ? bcol[q] : 0.0;
+            }
+        } else if (k == 1) {
+            if (wv == 1) putT(0, mm16(Lb(1, 0), Vb(0, 0), z));                      // T_10
+        } else if (k == 2) {
+            if (wv == 1) {
+                putV(1, 0, mm16(Vr(1, 1), Tb(0), z));                                // Linv_10
+                putT(1, mm16(Lb(2, 1), Vb(1, 0), mm16(Lb(2, 0), Vb(0, 0), z)));      // T_20
+            } else if (wv == 2) {
+                putT(2, mm16(Lb(2, 1), Vb(1, 1), z));                                // T_21
+            } else if (wv == 3) {
+                putT(3, mm16(Lb(3, 0), Vb(0, 0), z));                                // T_30, first term
+            } else if (wv == 5) {
+                putT(4, mm16(Lb(3, 1), Vb(1, 1), z));                                // T_31, first term
+            }
+        } else if (k == 3) {
+            if (wv == 1) putV(2, 0, mm16(Vr(2, 2), Tb(1), z));                       // Linv_20
+            else if (wv == 2) putV(2, 1, mm16(Vr(2, 2), Tb(2), z));                  // Linv_21
+            else if (wv == 3) putT(3, mm16(Lb(3, 1), Vb(1, 0), getT(3)));            // T_30, second term
+            else if (wv == 5) putT(0, mm16(Lb(3, 2), Vb(2, 2), z));                  // T_32
+        }
+        __syncthreads();
+        if (k == 3) break;
+        // trailing update of the blocks k < j <= i <= 3, one per wavefront
+        {
+            const int nb = 3 - k;
+            int pp = wv, i = -1, j = -1;
+            for (int a = 0; a < nb && i < 0; ++a)
+                for (int b = 0; b <= a; ++b) {
+                    if (pp == 0) { i = k + 1 + a; j = k + 1 + b; break; }
+                    --pp;
+                }
+            if (i >= 0) {
+                // D[m = c][n = r] = sum_t L_jk[c][t] (L_ik[r][t] d_t)
+                const d4 acc = mm16([&](int c, int t) { return xs[16 * k + t][16 * j + c]; },
+                                    [&](int t, int r) { return xs[16 * k + t][16 * i + r] * dd[16 * k + t]; }, z);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) xs[16 * j + lg + 4 * e][16 * i + rr] -= acc[e];
+            }
+        }
+        __syncthreads();
+    }
+    // the last row of blocks: T_3j completed, Linv_3j = -Linv_33 T_3j
+    if (wv == 0) {
+        putT(1, mm16(Lb(3, 2), Vb(2, 0), getT(3)));
+        putV(3, 0, mm16(Vr(3, 3), Tb(1), z));
+    } else if (wv == 1) {
+        putT(2, mm16(Lb(3, 2), Vb(2, 1), getT(4)));
+        putV(3, 1, mm16(Vr(3, 3), Tb(2), z));
+    } else if (wv == 2) {
+        putV(3, 2, mm16(Vr(3, 3), Tb(0), z));
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < TT; idx += NTH) {
+        const int col = idx / T, row = idx % T;
+        lout[idx] = (row >= col) ? ys[ysw(col, row)] : 0.0;
     }
 }
 
@@ -303,9 +498,13 @@ __device__ __forceinline__ void update_task(R* __restrict__ out, const R* __rest
 template <class R>
 __global__ void __launch_bounds__(NTH) k_tile_diag(const R* __restrict__ tiles, R* __restrict__ linv,
                                                    double* __restrict__ piv, long Kt, int w) {
-    __shared__ R bc[2][T];
-    const R* tile = tiles + tile_index(Kt, Kt, w) * TT;
-    diag_core<R>([&](int j, int r) { return tile[j * T + r]; }, linv + Kt * (long)TT, piv, Kt, bc);
+    __shared__ R xs[T][T];
+    __shared__ R ys[TT];
+    __shared__ double dd[T];
+    __shared__ DiagScratch<R> sm;
+    stage<false>(xs, tiles + tile_index(Kt, Kt, w) * TT);
+    __syncthreads();
+    diag_task(xs, ys, dd, sm, linv + Kt * (long)TT, piv, Kt);
 }
 
 // P_{I,K} for I = Kt+1+blockIdx.x
@@ -361,11 +560,13 @@ __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* _
     const long b = blockIdx.x;
     if (m > 0 && threadIdx.x < T) dk[threadIdx.x] = piv[K * T + threadIdx.x];
     if (b == 0) {
+        __shared__ DiagScratch<R> sm;
         const long D = K + 1;
         const R* dtile = tiles + tile_index(D, D, w) * TT;
-        R(*bc)[T] = ys;   // the diagonal step's broadcast buffers
         if (m == 0) {
-            diag_core<R>([&](int j, int r) { return dtile[j * T + r]; }, linv + D * (long)TT, piv, D, bc);
+            stage<false>(xs, dtile);
+            __syncthreads();
+            diag_task(xs, &ys[0][0], dk, sm, linv + D * (long)TT, piv, D);
             return;
         }
         R pre[TileAcc<R>::N];
@@ -387,7 +588,7 @@ __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* _
         __syncthreads();
         acc.each_i([&](int row, int col, R v, int i) { xs[col][row] = pre[i] - v; });
         __syncthreads();
-        diag_core<R>([&](int j, int r) { return xs[j][r]; }, linv + D * (long)TT, piv, D, bc);
+        diag_task(xs, &ys[0][0], dk, sm, linv + D * (long)TT, piv, D);
         return;
     }
     __syncthreads();   // dk
