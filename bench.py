@@ -44,8 +44,9 @@ def workload_file(n: int) -> str:
 
 
 PEAK_FP64_TFLOPS = 78.6
-# every 16th factorization launch is bracketed by HIP events (the sweeps every
-# time): the average launch duration with little event overhead in the window
+# the fused factorization's launches are timed as one HIP-event span per
+# factorization and the split schedule's every 16th launch (the sweeps every
+# time), which keeps the event overhead out of the measured window
 TIMING_EVERY = 16
 PEAK_HBM_GBS = 8000.0
 TILE = 64
@@ -60,7 +61,8 @@ def kernel_model(info) -> dict:
       panel  : m tile products of 2 T^3
       update : m(m+1)/2 tile products of 2 T^3
       column : one fused launch = diag + panel + update of a column (average
-               over the nt launches of a factorization)
+               over the nt launches of a factorization); its time is the launch
+               period (the nt launches of a factorization timed as one span)
       sweeps : every factor tile and every Linv tile read once (8 B entries)
     """
     nt = (info.neq + TILE - 1) // TILE
@@ -152,10 +154,23 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=8.0, help="seconds of partial CPU factorization")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="MIB",
+                    help="only run 20 elementwise copies of MIB MiB (the reference kernel for "
+                         "scaling the FETCH_SIZE/WRITE_SIZE counters) and exit")
     ap.add_argument("--schedule", default="fused", choices=("fused", "split"),
                     help="band factorization schedule: one fused launch per tile column, or "
                          "diag/panel/update launches")
     args = ap.parse_args()
+    if args.calibrate_copy > 0:
+        import torch
+        n = int(args.calibrate_copy * (1 << 20)) // 8
+        src = torch.rand(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(20):
+            torch.mul(src, 1.0, out=dst)   # an elementwise kernel: reads and writes 8n bytes
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": 8 * n, "copies": 20}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))

@@ -12,9 +12,10 @@ namespace cb {
 // KT_PANEL and KT_UPDATE are the launches of the split schedule.
 enum { KT_COLUMN = 0, KT_DIAG = 1, KT_PANEL = 2, KT_UPDATE = 3, KT_FWD = 4, KT_BWD = 5, KT_N = 6 };
 
-// Launches of a selected family are bracketed by two events.  Of the
-// factorization families (launched per tile column) only every `every`-th
-// launch is timed, which keeps the event overhead out of the factorization;
+// Launches of a selected family are bracketed by two events.  The fused
+// factorization's nt launches are bracketed as one span (its launch period:
+// kernel time plus the gaps between the launches, without event overhead per
+// launch); of the split schedule's launches only every `every`-th is timed;
 // the sweeps are always timed.  launches counts every launch issued and ms is
 // the timed total scaled to all of them, so ms / launches is the average
 // launch duration.
@@ -27,6 +28,7 @@ struct KernelClock {
     long n_timed[KT_N] = {};
     std::vector<hipEvent_t> ev;
     std::vector<int> fam;                // family of each recorded pair
+    std::vector<long> count;             // launches covered by each recorded pair
     size_t used = 0;
     bool on(int f) const { return (mask >> f) & 1; }
     // whether to time this launch of family f
@@ -36,7 +38,7 @@ struct KernelClock {
         const long e = (f == KT_FWD || f == KT_BWD || every < 1) ? 1 : every;
         return k % e == 0;
     }
-    void begin(int f, hipStream_t s);
+    void begin(int f, hipStream_t s, long n = 1);
     void end(hipStream_t s);
     void collect();                      // after a stream sync
     void release();

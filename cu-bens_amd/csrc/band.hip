@@ -808,13 +808,14 @@ __global__ void k_sweep_prep(double* __restrict__ dst, const double* __restrict_
 
 }  // namespace
 
-void KernelClock::begin(int f, hipStream_t s) {
+void KernelClock::begin(int f, hipStream_t s, long n) {
     if (used + 2 > ev.size()) {
         size_t old = ev.size();
         ev.resize(old + 4096);
         for (size_t i = old; i < ev.size(); ++i) CB_CHECK(hipEventCreate(&ev[i]));
     }
     fam.push_back(f);
+    count.push_back(n);
     CB_CHECK(hipEventRecord(ev[used++], s));
 }
 
@@ -825,11 +826,12 @@ void KernelClock::collect() {
         float t = 0;
         CB_CHECK(hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]));
         ms_timed[fam[i]] += t;
-        n_timed[fam[i]] += 1;
+        n_timed[fam[i]] += count[i];
     }
     for (int f = 0; f < KT_N; ++f)
         ms[f] = n_timed[f] > 0 ? ms_timed[f] * double(launches[f]) / double(n_timed[f]) : 0.0;
     fam.clear();
+    count.clear();
     used = 0;
 }
 
@@ -837,6 +839,7 @@ void KernelClock::release() {
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     ev.clear();
     fam.clear();
+    count.clear();
     used = 0;
 }
 
@@ -887,16 +890,19 @@ int BandLDLT<R>::factor(const Dev& d) {
             cepoch = 0;
         }
         const long spin_limit = 1L << 22;
+        const bool tk_column = clock.on(KT_COLUMN);
+        if (tk_column) {
+            clock.launches[KT_COLUMN] += nt;
+            clock.begin(KT_COLUMN, s, nt);
+        }
         for (long K = -1; K < nt - 1; ++K) {
             const long m = (K < 0) ? 0 : std::min<long>(w, nt - 1 - K);
             const long grid = m > 0 ? m + m * (m + 1) / 2 : 1;
             cepoch += 1;
-            const bool tk_column = clock.take(KT_COLUMN);
-            if (tk_column) clock.begin(KT_COLUMN, s);
             hipLaunchKernelGGL((k_band_column<R>), dim3(int(grid)), dim3(NTH), 0, s, tiles, linv, piv, K, w, nt,
                                pflags, cepoch, spin_limit, flag + 0);
-            if (tk_column) clock.end(s);
         }
+        if (tk_column) clock.end(s);
     } else {
         for (long Kt = 0; Kt < nt; ++Kt) {
             const bool tk_diag = clock.take(KT_DIAG);
