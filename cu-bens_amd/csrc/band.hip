@@ -628,9 +628,11 @@ __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* _
 // signalling-NaN bit pattern that no arithmetic produces; the producer writes
 // the 64 values with system-scope stores (written through to memory) and a
 // consumer polls them with system-scope loads until none is SENT.  There is
-// no flag and no cache maintenance on the critical path.  Polling backs off
-// the longer a wait lasts, so the workgroups far ahead of the front do not
-// load memory.  Each wait is bounded: past spin_limit polls the value reads as
+// no flag and no cache maintenance on the critical path.  Every lane polls
+// its own value with several loads in flight, so a row is seen about one
+// memory round trip after it lands; polling slows down once a wait has
+// lasted long, so the workgroups far ahead of the front load memory little.
+// Each wait is bounded: past spin_limit polls the value reads as
 // a quiet NaN and *err is raised, so every workgroup still finishes and the
 // sweep drains.  Contributions of earlier rows are applied while the newest
 // one is still pending, with the next off-diagonal tile prefetched; the
@@ -640,27 +642,32 @@ constexpr unsigned long long SENT = 0x7FF4C0DE5E471E11ull;
 
 __device__ __forceinline__ double poll_value(const double* p, long spin_limit, int* err) {
     const unsigned long long* q = reinterpret_cast<const unsigned long long*>(p);
-    unsigned long long v;
-    long n = 0;
-    while ((v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) == SENT) {
-        if (++n >= spin_limit) {
+    auto ld = [q] { return __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM); };
+    unsigned long long a = ld();
+    if (a != SENT) return __builtin_bit_cast(double, a);
+    // four loads in flight: each check waits for the oldest one only and
+    // issues the next, so a value is seen about one round trip after it lands
+    unsigned long long b = ld(), c = ld(), d = ld();
+    for (long n = 0;; n += 4) {
+        if (a != SENT) return __builtin_bit_cast(double, a);
+        a = ld();
+        if (b != SENT) return __builtin_bit_cast(double, b);
+        b = ld();
+        if (c != SENT) return __builtin_bit_cast(double, c);
+        c = ld();
+        if (d != SENT) return __builtin_bit_cast(double, d);
+        d = ld();
+        if (n >= spin_limit) {
             atomicOr(err, 1);
             return __builtin_nan("");
         }
-        if (n < 64) __builtin_amdgcn_s_sleep(1);
-        else if (n < 1024) __builtin_amdgcn_s_sleep(8);
-        else __builtin_amdgcn_s_sleep(64);
+        if (n >= 1024) __builtin_amdgcn_s_sleep(32);   // far from the front: poll less often
     }
-    return __builtin_bit_cast(double, v);
 }
 
-// vin[0..63] = the values of tile row J of v (wave 0 polls, one lane first)
+// vin[0..63] = the values of tile row J of v (wave 0 polls, a value per lane)
 __device__ __forceinline__ void fetch_row(const double* v, long J, double* vin, long spin_limit, int* err) {
-    if (threadIdx.x < T) {
-        const double* p = v + J * T;
-        if (threadIdx.x == T - 1) (void)poll_value(p + T - 1, spin_limit, err);
-        vin[threadIdx.x] = poll_value(p + threadIdx.x, spin_limit, err);
-    }
+    if (threadIdx.x < T) vin[threadIdx.x] = poll_value(v + J * T + threadIdx.x, spin_limit, err);
     __syncthreads();
 }
 
