@@ -545,8 +545,10 @@ __global__ void __launch_bounds__(NTH) k_tile_update(R* __restrict__ tiles, cons
 //   workgroups 1..m-1    P_{K+1+b,K}; raise flag b
 //   workgroup m          P_{K+1,K} again (the same arithmetic), stored once
 //                        flag w is up; raises flag 0
-//   the rest             U_{I,J,K} for the pairs other than (K+1,K+1), row by row
-// Grid: m + m(m+1)/2 workgroups for m > 0, 1 for m = 0.  Workgroup 0 neither
+//   the rest             U_{I,J,K} for the pairs other than (K+1,K+1), two tiles
+//                        of one tile row I per workgroup (they share L_IK)
+// Grid: m + 1 + sum_{r=1}^{m-1} ceil((r+1)/2) workgroups for m > 0 (194 at
+// m = 25, so that all of them are resident at once), 1 for m = 0.  Workgroup 0 neither
 // stores nor waits before its diagonal step.
 template <class R>
 __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* __restrict__ linv,
@@ -610,13 +612,28 @@ __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* _
         raise_flag(flags, pb, epoch);
         return;
     }
-    long r, s;
-    tri_pair(b - m, r, s);
+    // row r >= 1 of the update triangle holds ceil((r+1)/2) workgroups
+    long t = b - (m + 1), r = 1;
+    for (long c = (r + 2) / 2; t >= c; c = (r + 2) / 2) {
+        t -= c;
+        ++r;
+    }
+    const long s0 = 2 * t, ns = (s0 + 1 <= r) ? 2 : 1;
     if (!wait_flag_bounded(flags, r, epoch, spin_limit, err, &ok_s)) return;
-    if (s != r && !wait_flag_bounded(flags, s, epoch, spin_limit, err, &ok_s)) return;
-    const long I = K + 1 + r, J = K + 1 + s;
-    update_task<true, R>(tiles + tile_index(I, J, w) * TT, tiles + tile_index(I, K, w) * TT,
-                         tiles + tile_index(J, K, w) * TT, dk, xs, ys);
+    for (long j = 0; j < ns; ++j)
+        if (s0 + j != r && !wait_flag_bounded(flags, s0 + j, epoch, spin_limit, err, &ok_s)) return;
+    const long I = K + 1 + r;
+    stage_scaled<true>(xs, tiles + tile_index(I, K, w) * TT, dk);
+    for (long j = 0; j < ns; ++j) {
+        const long J = K + 1 + s0 + j;
+        if (j > 0) __syncthreads();   // the previous product has read ys
+        stage<true>(ys, tiles + tile_index(J, K, w) * TT);
+        __syncthreads();
+        TileAcc<R> acc;
+        acc.product(xs, ys);
+        R* out = tiles + tile_index(I, J, w) * TT;
+        acc.each([&](int row, int col, R v) { out[col * T + row] -= v; });
+    }
 }
 
 // ---- pipelined sweeps ---------------------------------------------------------
@@ -904,7 +921,11 @@ int BandLDLT<R>::factor(const Dev& d) {
         }
         for (long K = -1; K < nt - 1; ++K) {
             const long m = (K < 0) ? 0 : std::min<long>(w, nt - 1 - K);
-            const long grid = m > 0 ? m + m * (m + 1) / 2 : 1;
+            long grid = 1;
+            if (m > 0) {
+                grid = m + 1;
+                for (long rr = 1; rr < m; ++rr) grid += (rr + 2) / 2;
+            }
             cepoch += 1;
             hipLaunchKernelGGL((k_band_column<R>), dim3(int(grid)), dim3(NTH), 0, s, tiles, linv, piv, K, w, nt,
                                pflags, cepoch, spin_limit, flag + 0);
