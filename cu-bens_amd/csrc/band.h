@@ -73,4 +73,43 @@ struct BandLDLT {
     size_t bytes() const { return sizeof(R) * size_t(nt) * size_t(w + 1) * 64 * 64; }
 };
 
+// A batch of independent models sharing one FP64 band: model b owns the tile
+// rows [t0, t0 + nt) (its equations start on a tile boundary, the unused
+// equations of its last tile are identity rows) and has w tiles below the
+// diagonal, w <= the band's.  One workgroup factors and solves one model, so
+// the models of a batch run side by side in one launch.
+struct BandSeg {
+    long t0 = 0, nt = 0;
+    long neq = 0;   // equations of the model (t0 * 64 is its first)
+    int w = 0;
+    int pad_ = 0;
+};
+
+struct BandBatch {
+    long n = 0, nt = 0;
+    int w = 0;
+    int nseg = 0;
+    hipStream_t s = nullptr;
+    double* tiles = nullptr;
+    double* linv = nullptr;
+    double* piv = nullptr;
+    double* yw = nullptr;           // forward-sweep values
+    int* flag = nullptr;            // 2: an entry outside the band
+    int* flag_host = nullptr;       // pinned copy, valid after the stream syncs
+    BandSeg* segs = nullptr;        // device copy
+    int* order = nullptr;           // workgroup -> model, most work first
+    int32_t* gaps = nullptr;        // equations that no DOF slot maps to
+    long ngap = 0;
+    std::vector<BandSeg> segs_host;
+
+    void setup(const std::vector<BandSeg>& sg, long n_total, int w_band, const std::vector<int32_t>& gap_eqs,
+               hipStream_t s_);
+    void release();
+    // Copies the lower band of d.K and factors every model; enqueued only.
+    void factor(const Dev& d);
+    // out = (L D L^T)^{-1} rhs for every model (device vectors of n, no alias).
+    void solve(const double* rhs, double* out);
+    size_t bytes() const { return sizeof(double) * size_t(nt) * size_t(w + 1) * 64 * 64; }
+};
+
 }  // namespace cb

@@ -830,6 +830,127 @@ __global__ void k_sweep_prep(double* __restrict__ dst, const double* __restrict_
     }
 }
 
+// ---- batches of models ---------------------------------------------------------
+// Identity rows for the equations no DOF slot maps to (the tail of each
+// model's last tile).
+__global__ void k_band_gap_pad(double* __restrict__ tiles, const int32_t* __restrict__ gaps, long ngap, int w) {
+    long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= ngap) return;
+    const long i = gaps[k], I = i / T;
+    tiles[tile_index(I, I, w) * TT + (i % T) * T + (i % T)] = 1.0;
+}
+
+// One workgroup factors one model, column by column, with the tile tasks of
+// the single-model schedules (the same arithmetic, so the same factor): D_K,
+// then P_{I,K} for the m tiles below it, then U_{I,J,K} row by row.  Tiles
+// pass between the tasks through memory inside the workgroup; the barrier
+// after each task orders its stores before the next task's loads.
+__global__ void __launch_bounds__(NTH) k_band_factor_models(double* __restrict__ tiles, double* __restrict__ linv,
+                                                            double* __restrict__ piv, int w,
+                                                            const BandSeg* __restrict__ segs,
+                                                            const int* __restrict__ order) {
+    __shared__ double xs[T][T];
+    __shared__ double ys[TT];
+    __shared__ double dd[T];
+    __shared__ DiagScratch<double> sm;
+    const BandSeg g = segs[order[blockIdx.x]];
+    double(*yt)[T] = reinterpret_cast<double(*)[T]>(ys);
+    const long Kend = g.t0 + g.nt;
+    for (long K = g.t0; K < Kend; ++K) {
+        stage<false>(xs, tiles + tile_index(K, K, w) * TT);
+        __syncthreads();
+        diag_task(xs, ys, dd, sm, linv + K * (long)TT, piv, K);
+        __syncthreads();
+        const long m = (Kend - 1 - K < g.w) ? Kend - 1 - K : g.w;
+        for (long r = 1; r <= m; ++r) {
+            double* tile = tiles + tile_index(K + r, K, w) * TT;
+            panel_task<false, double>(tile, tile, linv + K * (long)TT, piv + K * T, xs, yt);
+            __syncthreads();
+        }
+        for (long r = 1; r <= m; ++r)
+            for (long c = 1; c <= r; ++c) {
+                update_task<false, double>(tiles + tile_index(K + r, K + c, w) * TT,
+                                           tiles + tile_index(K + r, K, w) * TT,
+                                           tiles + tile_index(K + c, K, w) * TT, piv + K * T, xs, yt);
+                __syncthreads();
+            }
+    }
+}
+
+// One 256-thread workgroup solves one model: the forward sweep into y, then
+// the backward sweep into out, tile row by tile row.  Thread (row, q) keeps the
+// partial sum over columns 16q..16q+15 and the partial sums are combined in
+// the order of k_fwd_sweep / k_bwd_sweep, so a model's solution is the one
+// those kernels give.
+__global__ void __launch_bounds__(256) k_band_solve_models(const double* __restrict__ tiles,
+                                                           const double* __restrict__ linv,
+                                                           const double* __restrict__ piv, int w,
+                                                           const BandSeg* __restrict__ segs,
+                                                           const int* __restrict__ order,
+                                                           const double* __restrict__ rhs, double* y, double* out) {
+    __shared__ double vin[T];
+    __shared__ double part[4][T];
+    const BandSeg g = segs[order[blockIdx.x]];
+    const int tid = threadIdx.x, row = tid % T, q = tid / T, k0 = 16 * q;
+    const long Iend = g.t0 + g.nt, e0 = g.t0 * T;
+    auto combine = [&](double v) {
+        part[q][row] = v;
+        __syncthreads();
+        double s = 0.0;
+        if (tid < T) s = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+        return s;
+    };
+    for (long I = g.t0; I < Iend; ++I) {
+        double acc = 0.0;
+        if (q == 0) {
+            const long e = I * T + row;
+            acc = (e - e0 < g.neq) ? rhs[e] : 0.0;
+        }
+        const long J0 = (I - g.w > g.t0) ? I - g.w : g.t0;
+        for (long J = J0; J < I; ++J) {
+            const double* l = tiles + tile_index(I, J, w) * TT;
+            if (tid < T) vin[tid] = y[J * T + tid];
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < 16; ++k) acc -= l[(k0 + k) * T + row] * vin[k0 + k];
+            __syncthreads();
+        }
+        double s = combine(acc);
+        if (tid < T) vin[tid] = s;
+        __syncthreads();
+        const double* lv = linv + I * (long)TT;
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) t += lv[(k0 + k) * T + row] * vin[k0 + k];
+        s = combine(t);
+        if (tid < T) y[I * T + tid] = s;
+        __syncthreads();
+    }
+    for (long I = Iend - 1; I >= g.t0; --I) {
+        const int col = row;
+        double acc = (q == 0) ? y[I * T + col] / piv[I * T + col] : 0.0;
+        const long J1 = (I + g.w < Iend - 1) ? I + g.w : Iend - 1;
+        for (long J = J1; J > I; --J) {
+            const double* l = tiles + tile_index(J, I, w) * TT;
+            if (tid < T) vin[tid] = out[J * T + tid];
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < 16; ++k) acc -= l[col * T + k0 + k] * vin[k0 + k];
+            __syncthreads();
+        }
+        double s = combine(acc);
+        if (tid < T) vin[tid] = s;
+        __syncthreads();
+        const double* lv = linv + I * (long)TT;
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) t += lv[col * T + k0 + k] * vin[k0 + k];
+        s = combine(t);
+        if (tid < T) out[I * T + tid] = s;
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 void KernelClock::begin(int f, hipStream_t s, long n) {
@@ -990,5 +1111,75 @@ void BandLDLT<R>::solve(const double* rhs, double* out) {
 
 template struct BandLDLT<double>;
 template struct BandLDLT<float>;
+
+void BandBatch::setup(const std::vector<BandSeg>& sg, long n_total, int w_band, const std::vector<int32_t>& gap_eqs,
+                      hipStream_t s_) {
+    release();
+    s = s_;
+    n = n_total;
+    if (n % T) throw DeviceError("band batch: the equations of a batch fill whole tiles");
+    nt = n / T;
+    w = w_band;
+    segs_host = sg;
+    nseg = int(sg.size());
+    for (const BandSeg& g : sg)
+        if (g.t0 < 0 || g.t0 + g.nt > nt || g.w > w || g.w < 0 || g.neq > g.nt * T)
+            throw DeviceError("band batch: a model's tile range does not fit the band");
+    // workgroups in decreasing order of the model's tile work, so the longest
+    // factorizations start first
+    std::vector<int> ord(sg.size());
+    for (size_t i = 0; i < ord.size(); ++i) ord[i] = int(i);
+    auto work = [&](int i) { return double(sg[i].nt) * double((sg[i].w + 1) * (sg[i].w + 2) / 2); };
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return work(a) > work(b); });
+    tiles = dalloc<double>(size_t(nt) * size_t(w + 1) * TT);
+    linv = dalloc<double>(size_t(nt) * TT);
+    piv = dalloc<double>(size_t(nt) * T);
+    yw = dalloc<double>(size_t(nt) * T);
+    flag = dalloc<int>(1);
+    CB_CHECK(hipHostMalloc(&flag_host, sizeof(int), hipHostMallocDefault));
+    *flag_host = 0;
+    segs = dalloc<BandSeg>(sg.size());
+    order = dalloc<int>(ord.size());
+    ngap = long(gap_eqs.size());
+    gaps = dalloc<int32_t>(size_t(ngap));
+    h2d(segs, sg.data(), sg.size(), s);
+    h2d(order, ord.data(), ord.size(), s);
+    h2d(gaps, gap_eqs.data(), gap_eqs.size(), s);
+    CB_CHECK(hipMemsetAsync(piv, 0, sizeof(double) * size_t(nt) * T, s));
+    CB_CHECK(hipStreamSynchronize(s));
+}
+
+void BandBatch::release() {
+    void* ps[] = {tiles, linv, piv, yw, flag, segs, order, gaps};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    if (flag_host) (void)hipHostFree(flag_host);
+    tiles = linv = piv = yw = nullptr;
+    flag = nullptr;
+    flag_host = nullptr;
+    segs = nullptr;
+    order = nullptr;
+    gaps = nullptr;
+}
+
+void BandBatch::factor(const Dev& d) {
+    if (!nseg) return;
+    CB_CHECK(hipMemsetAsync(tiles, 0, sizeof(double) * size_t(nt) * (w + 1) * TT, s));
+    CB_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
+    hipLaunchKernelGGL((k_band_from_bsr<double>), dim3(int((d.nslot + 255) / 256)), dim3(256), 0, s, tiles, d.K,
+                       d.rowptr, d.colidx, d.eq_of_slot, d.nj, d.bs, w, flag);
+    if (ngap)
+        hipLaunchKernelGGL(k_band_gap_pad, dim3(int((ngap + 255) / 256)), dim3(256), 0, s, tiles, gaps, ngap, w);
+    hipLaunchKernelGGL(k_band_factor_models, dim3(nseg), dim3(NTH), 0, s, tiles, linv, piv, w, segs, order);
+    CB_CHECK(hipGetLastError());
+    CB_CHECK(hipMemcpyAsync(flag_host, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+}
+
+void BandBatch::solve(const double* rhs, double* out) {
+    if (!nseg) return;
+    hipLaunchKernelGGL(k_band_solve_models, dim3(nseg), dim3(256), 0, s, tiles, linv, piv, w, segs, order, rhs, yw,
+                       out);
+    CB_CHECK(hipGetLastError());
+}
 
 }  // namespace cb

@@ -7,6 +7,7 @@
 #include <memory>
 #include <string>
 
+#include "batch.h"
 #include "driver.h"
 #include "learned.h"
 #include "model.h"
@@ -57,6 +58,10 @@ struct cb_model {
     std::vector<long> step_newton;
     std::vector<std::vector<double>> step_kt_ms;
     std::vector<std::vector<long>> step_kt_n;
+};
+
+struct cb_batch {
+    std::unique_ptr<cb::Batch> bt;
 };
 
 namespace {
@@ -620,6 +625,98 @@ int cb_learned_blocks(cb_learned* L, cb_model* h, float* gd, float* go, long* nv
     if (nv) *nv = L->net->nv();
     if (ne) *ne = L->net->ne();
     L->net->get_blocks(gd, go);
+    return 0;
+    GUARD_END
+}
+
+int cb_batch_open(const char* const* paths, int count, cb_batch** out) {
+    if (!out || !paths || count < 1) return fail(-1, "null argument or empty batch");
+    *out = nullptr;
+    GUARD_BEGIN
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(-5, "no HIP device available: the CU-BENs GPU path needs an MI355X (gfx950)");
+    std::vector<std::string> ps;
+    for (int i = 0; i < count; ++i) {
+        if (!paths[i]) return fail(-1, "null model path");
+        ps.emplace_back(paths[i]);
+    }
+    std::unique_ptr<cb_batch> h(new cb_batch());
+    try {
+        h->bt.reset(new cb::Batch(ps));
+    } catch (const std::runtime_error& e) {
+        return fail(-4, e.what());
+    }
+    *out = h.release();
+    return 0;
+    GUARD_END
+}
+
+void cb_batch_close(cb_batch* b) {
+    if (!b) return;
+    try {
+        b->bt.reset();
+    } catch (...) {
+    }
+    delete b;
+}
+
+int cb_batch_count(const cb_batch* b) { return (b && b->bt) ? b->bt->count() : 0; }
+
+int cb_batch_get_info(const cb_batch* b, int i, cb_info* out) {
+    if (!b || !b->bt || !out) return fail(-1, "null argument");
+    if (i < 0 || i >= b->bt->count()) return fail(-1, "model index out of range");
+    const cb::Model& m = b->bt->model(i);
+    out->nj = m.nj; out->ne_tr = m.ntr; out->ne_fr = m.nfr; out->ne_sh = m.nsh;
+    out->neq = m.neq; out->lss = m.lss; out->bandwidth = m.bandwidth;
+    out->nnz_blocks = long(m.colidx.size());
+    out->bs = m.bs;
+    out->anaflag = m.anaflag; out->algflag = m.algflag;
+    out->ncolours_tr = int(m.col_ptr_tr.size()) - 1;
+    out->ncolours_sh = int(m.col_ptr_sh.size()) - 1;
+    return 0;
+}
+
+int cb_batch_run(cb_batch* b, const cb_options* opt, cb_batch_stats* st) {
+    if (!b || !b->bt) return fail(-1, "null handle");
+    const cb_options o = opt ? *opt : default_options();
+    if (o.solver != CB_SOLVER_BAND64) return fail(-1, "a batch runs the FP64 band solver only");
+    GUARD_BEGIN
+    cb::BatchStats bs;
+    b->bt->run(o.max_steps, o.keep_d != 0, bs);
+    if (st) {
+        st->t_total = bs.t_total;
+        st->rounds = bs.rounds;
+        st->newton = bs.newton;
+        st->steps = bs.steps;
+        st->failed = bs.failed;
+    }
+    return 0;
+    GUARD_END
+}
+
+int cb_batch_get_status(const cb_batch* b, int* status, int* nsteps) {
+    if (!b || !b->bt) return fail(-1, "null handle");
+    const cb::Batch& bt = *b->bt;
+    if (bt.status.size() != size_t(bt.count())) return fail(-1, "the batch has not run");
+    for (int i = 0; i < bt.count(); ++i) {
+        if (status) status[i] = bt.status[size_t(i)];
+        if (nsteps) nsteps[i] = int(bt.times[size_t(i)].size());
+    }
+    return 0;
+}
+
+int cb_batch_get_path(const cb_batch* b, int i, double* t, int* itecnt, double* d) {
+    if (!b || !b->bt) return fail(-1, "null handle");
+    const cb::Batch& bt = *b->bt;
+    if (bt.status.size() != size_t(bt.count())) return fail(-1, "the batch has not run");
+    if (i < 0 || i >= bt.count()) return fail(-1, "model index out of range");
+    GUARD_BEGIN
+    const auto& tv = bt.times[size_t(i)];
+    const auto& iv = bt.itecnt[size_t(i)];
+    if (t) std::memcpy(t, tv.data(), sizeof(double) * tv.size());
+    if (itecnt) std::memcpy(itecnt, iv.data(), sizeof(int) * iv.size());
+    if (d && !tv.empty()) bt.path(i, d);
     return 0;
     GUARD_END
 }

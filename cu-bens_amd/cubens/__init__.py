@@ -70,6 +70,14 @@ PI32 = C.POINTER(C.c_int32)
 PF = C.POINTER(C.c_float)
 
 
+class BatchStats(C.Structure):
+    _fields_ = [("t_total", C.c_double), ("rounds", C.c_long), ("newton", C.c_long),
+                ("steps", C.c_long), ("failed", C.c_int)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class TrainStats(C.Structure):
     _fields_ = [("steps", C.c_long), ("loss_first", C.c_double), ("loss_last", C.c_double),
                 ("loss_recent", C.c_double)]
@@ -107,6 +115,13 @@ ABI = {
     "cb_learned_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, PD, PD, C.c_int, PD, PF]),
     "cb_learned_blocks": (C.c_int, [C.c_void_p, C.c_void_p, PF, PF, PL, PL]),
     "cb_write_results2": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "cb_batch_open": (C.c_int, [C.POINTER(C.c_char_p), C.c_int, C.POINTER(C.c_void_p)]),
+    "cb_batch_close": (None, [C.c_void_p]),
+    "cb_batch_count": (C.c_int, [C.c_void_p]),
+    "cb_batch_get_info": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Info)]),
+    "cb_batch_run": (C.c_int, [C.c_void_p, C.POINTER(Options), C.POINTER(BatchStats)]),
+    "cb_batch_get_status": (C.c_int, [C.c_void_p, PI, PI]),
+    "cb_batch_get_path": (C.c_int, [C.c_void_p, C.c_int, PD, PI, PD]),
 }
 
 
@@ -260,6 +275,62 @@ class GpuModel:
 
     def write_results2(self, path: str):
         _check(lib().cb_write_results2(self.h, path.encode()))
+
+
+class GpuBatch:
+    """Many frame models of the nonlinear Newmark driver run side by side
+    (cb_batch_open); each model's path is the one cb_run gives for it alone."""
+
+    def __init__(self, paths):
+        arr = (C.c_char_p * len(paths))(*[p.encode() for p in paths])
+        h = C.c_void_p()
+        _check(lib().cb_batch_open(arr, len(paths), C.byref(h)))
+        self.h = h
+        self.count = lib().cb_batch_count(h)
+        self.info = []
+        for i in range(self.count):
+            inf = Info()
+            _check(lib().cb_batch_get_info(h, i, C.byref(inf)))
+            self.info.append(inf)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cb_batch_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, max_steps: int = 1 << 30, keep_d: bool = True) -> dict:
+        st = BatchStats()
+        opt = options(max_steps=max_steps, keep_d=int(keep_d))
+        _check(lib().cb_batch_run(self.h, C.byref(opt), C.byref(st)))
+        status = np.zeros(self.count, np.int32)
+        nsteps = np.zeros(self.count, np.int32)
+        _check(lib().cb_batch_get_status(self.h, _p(status, PI), _p(nsteps, PI)))
+        out = st.as_dict()
+        out.update(status=status, nsteps=nsteps)
+        return out
+
+    def path(self, i: int, with_d: bool = True) -> dict:
+        """Model i's recorded steps: time, itecnt and (with_d) d [nsteps, NEQ_i]."""
+        nsteps = np.zeros(self.count, np.int32)
+        _check(lib().cb_batch_get_status(self.h, None, _p(nsteps, PI)))
+        n = int(nsteps[i])
+        t = np.zeros(max(n, 1))
+        ite = np.zeros(max(n, 1), np.int32)
+        d = np.zeros((max(n, 1), self.info[i].neq)) if with_d else None
+        _check(lib().cb_batch_get_path(self.h, i, _p(t), _p(ite, PI), _p(d) if with_d else None))
+        return {"lpf": t[:n], "itecnt": ite[:n], "d": d[:n] if with_d else None}
 
 
 def bsr_to_dense(vals: np.ndarray, rowptr: np.ndarray, colidx: np.ndarray, bs: int,

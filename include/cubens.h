@@ -196,6 +196,38 @@ int cb_learned_loss_grad(cb_learned *L, cb_model *m, const double *r, const doub
  * [ne*36] blocks (either pointer may be NULL; nv, ne receive the sizes). */
 int cb_learned_blocks(cb_learned *L, cb_model *m, float *gd, float *go, long *nv, long *ne);
 
+/* ---- batches of independent models ---------------------------------------
+ * Many frame models of the nonlinear Newmark driver (main.c:3305-3960, one
+ * model_def.txt each, ALGFLAG 5) run side by side on one GPU.  Each round
+ * takes one Newton iteration of every running model; each model keeps its own
+ * time steps, step halving and convergence tests, and its path is the one
+ * cb_run gives for it alone.  The batch has no reference counterpart (the
+ * reference runs one model per process); it replaces running main() once per
+ * model. */
+typedef struct cb_batch cb_batch;
+typedef struct {
+    double t_total;      /* seconds of the run (device work and host decisions) */
+    long rounds;         /* Newton rounds over the batch                  */
+    long newton;         /* Newton iterations summed over the models      */
+    long steps;          /* recorded time steps summed over the models    */
+    int failed;          /* models whose driver stopped with an error     */
+} cb_batch_stats;
+/* Reads every model (as cb_open does) and builds the batch's device state;
+ * frame-only models without member-end offsets or free warping DOFs. */
+int cb_batch_open(const char *const *paths, int count, cb_batch **out);
+void cb_batch_close(cb_batch *b);
+int cb_batch_count(const cb_batch *b);
+int cb_batch_get_info(const cb_batch *b, int i, cb_info *out);
+/* Runs every model for at most opt->max_steps recorded steps (keep_d: keep
+ * the displacement rows on the GPU for cb_batch_get_path). */
+int cb_batch_run(cb_batch *b, const cb_options *opt, cb_batch_stats *st);
+/* Per model [count]: driver status (0 ran to its end or to max_steps, 1 zero
+ * norm in the convergence test, 3 minimum time increment reached without
+ * convergence) and recorded steps. */
+int cb_batch_get_status(const cb_batch *b, int *status, int *nsteps);
+/* Model i's path as cb_get_path gives it: time [nsteps], itecnt [nsteps],
+ * d [nsteps*NEQ_i] (any pointer may be NULL). */
+int cb_batch_get_path(const cb_batch *b, int i, double *t, int *itecnt, double *d);
 #ifdef __cplusplus
 }
 #endif

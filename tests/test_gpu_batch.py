@@ -1,0 +1,112 @@
+"""Batches of frame models under the nonlinear Newmark driver (main.c:3305-3960),
+side by side on the GPU, through the C ABI (cb_batch_*).
+
+- A mixed batch: towers and lattice girders of 60-300 members, without and
+  with numerical dissipation, and models with itemax 1, whose Newton loops
+  fail and halve the time step (main.c:3826-3846) while the other models of
+  the batch step on.  Every model's path is the path of the same model run
+  alone (cb_run): identical iteration counts, displacements to 1e-12
+  relative; and the oracle's path to 1e-9 with identical iteration counts.
+- The 512-frame workload (200-5000 members each, about 2.66 million
+  equations in all) for 20 time steps: every model runs to the step limit
+  with finite displacements, and the largest and smallest models match their
+  single runs to 1e-12 with identical iteration counts.
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import cubens
+import oracle as O
+from cubens import inputs
+from helpers import model_file, relerr
+
+pytestmark = pytest.mark.gpu
+
+MIXED = [
+    dict(n=60, kind="tower", amp=8e4),
+    dict(n=150, kind="grid", amp=8e4, itemax=1),
+    dict(n=300, kind="tower", amp=3e5, numopt=1, spectrds=0.8),
+    dict(n=90, kind="grid", amp=1e6),
+    dict(n=200, kind="tower", amp=1e6, itemax=1),
+    dict(n=120, kind="grid", amp=3e5, numopt=2, spectrds=0.7),
+]
+
+
+def _mixed_files(steps):
+    paths = []
+    for i, kw in enumerate(MIXED):
+        kw = dict(kw)
+        n = kw.pop("n")
+        txt = inputs.space_frame(n, instance=100 + i, steps=steps, **kw).text
+        paths.append(model_file(f"batch_{i}.txt", txt))
+    return paths
+
+
+def _single(path, steps):
+    with cubens.GpuModel(path) as g:
+        r = g.run(cubens.options(cubens.SOLVER_BAND64, max_steps=steps))
+    assert r["status"] == 0, r["msg"]
+    return r
+
+
+def test_mixed_batch_matches_single_runs_and_oracle():
+    steps = 15
+    paths = _mixed_files(steps)
+    with cubens.GpuBatch(paths) as b:
+        st = b.run(max_steps=steps)
+        assert st["failed"] == 0 and np.all(st["status"] == 0), st
+        assert np.all(st["nsteps"] == steps)
+        halving = 0
+        for i, p in enumerate(paths):
+            rb = b.path(i)
+            rs = _single(p, steps)
+            ro = O.OracleModel(p).run(steps)
+            assert ro["rc"] == 0, ro["msg"]
+            halving += ro["n_newton"] > ro["itecnt"].sum()
+            print(f"model {i}: neq {b.info[i].neq}, Newton {ro['n_newton']}, "
+                  f"batch vs single {relerr(rb['d'], rs['d']):.1e}, vs oracle {relerr(rb['d'], ro['d']):.1e}")
+            assert np.array_equal(rb["itecnt"], rs["itecnt"])
+            assert np.array_equal(rb["lpf"], rs["lpf"])
+            assert relerr(rb["d"], rs["d"]) < 1e-12
+            assert np.array_equal(rb["itecnt"], ro["itecnt"])
+            assert relerr(rb["d"], ro["d"]) < 1e-9
+        assert halving >= 2, "the batch should hold models that halve their time step"
+        assert st["newton"] == sum(O.OracleModel(p).run(steps)["n_newton"] for p in paths)
+
+
+def test_batch_rejects_models_it_cannot_run():
+    shell = model_file("batch_shell.txt", inputs.shell_panel(n=4).text)
+    tower = model_file("batch_tower.txt", inputs.space_frame(40, kind="tower", steps=5).text)
+    with pytest.raises(cubens.CubensError, match="frame-only|Newmark"):
+        cubens.GpuBatch([tower, shell])
+
+
+def test_frame_workload_batch_512_models():
+    steps = 20
+    d = tempfile.mkdtemp(prefix="cubens_frames_")
+    models = inputs.frame_batch(steps=steps)
+    paths = []
+    for i, m in enumerate(models):
+        p = os.path.join(d, f"f{i:03d}.txt")
+        with open(p, "w") as fh:
+            fh.write(m.text)
+        paths.append(p)
+    with cubens.GpuBatch(paths) as b:
+        neq = np.array([inf.neq for inf in b.info])
+        nfr = np.array([inf.ne_fr for inf in b.info])
+        assert b.count == 512 and nfr.min() >= 200 and nfr.max() <= 5000
+        st = b.run(max_steps=steps)
+        print(f"512 frames, {nfr.sum()} members, {neq.sum()} equations: {st['rounds']} rounds, "
+              f"{st['newton']} Newton iterations in {st['t_total']:.2f} s")
+        assert st["failed"] == 0 and np.all(st["nsteps"] == steps), st
+        for i in (int(np.argmax(neq)), int(np.argmin(neq))):
+            rb = b.path(i)
+            assert np.all(np.isfinite(rb["d"]))
+            rs = _single(paths[i], steps)
+            assert np.array_equal(rb["itecnt"], rs["itecnt"])
+            assert relerr(rb["d"], rs["d"]) < 1e-12
+        for i in range(0, 512, 37):
+            assert np.all(np.isfinite(b.path(i)["d"]))
