@@ -113,6 +113,96 @@ def cpu_baseline(path: str, newton: int, increments: int, budget: float) -> dict
                        f"GPU window ({increments} increments, {newton} Newton iterations)")}
 
 
+FRAME_METRIC = "Newton iters/s, 512 space frames under nonlinear Newmark (batched)"
+
+
+def frame_files(steps: int) -> list[str]:
+    """The 512-frame workload (towers and lattice girders of U{200..5000}
+    members, 200 time steps of average-acceleration Newmark), written once."""
+    from cubens import inputs
+    d = os.path.join(os.environ.get("CUBENS_BENCH_DIR", "/tmp/cubens_bench"), f"frames{steps}")
+    os.makedirs(d, exist_ok=True)
+    done = os.path.join(d, "complete")
+    paths = [os.path.join(d, f"f{i:03d}.txt") for i in range(512)]
+    if not os.path.exists(done):
+        for p, m in zip(paths, inputs.frame_batch(steps=steps)):
+            with open(p, "w") as fh:
+                fh.write(m.text)
+        open(done, "w").close()
+    return paths
+
+
+def frame_cpu_baseline(paths: list[str], steps: int, stride: int = 32) -> dict:
+    """The oracle's serial Newmark driver on 1 core over every stride-th model
+    of the batch for `steps` time steps: Newton iterations per second."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    t = 0.0
+    newton = 0
+    sample = paths[::stride]
+    for p in sample:
+        om = O.OracleModel(p)
+        t0 = time.perf_counter()
+        r = om.run(steps)
+        t += time.perf_counter() - t0
+        newton += int(r["n_newton"])
+    return {"value": newton / t, "unit": "Newton iters/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle (C restatement of the serial Newmark driver with the skyline solver) on 1 core: "
+                       f"{len(sample)} of the 512 models (one in {stride}), first {steps} time steps each, "
+                       f"{newton} Newton iterations in {t:.1f}s")}
+
+
+def run_frames(args) -> None:
+    import cubens
+    steps_total = 200
+    log("[bench] frames: writing / reading the 512-model workload")
+    paths = frame_files(steps_total)
+    t0 = time.time()
+    b = cubens.GpuBatch(paths)
+    t_open = time.time() - t0
+    neq = sum(i.neq for i in b.info)
+    nfr = sum(i.ne_fr for i in b.info)
+    log(f"[bench] frames: {b.count} models, {nfr} members, NEQ {neq}, opened in {t_open:.1f}s")
+    if args.warmup > 0:
+        b.run(max_steps=args.warmup, keep_d=False)
+    k = min(args.steps, steps_total)
+    r = b.run(max_steps=k, keep_d=False, timing=True)
+    if r["failed"] or int(r["nsteps"].min()) < k:
+        raise RuntimeError(f"frame batch stopped early: {r}")
+    rate = r["newton"] / r["t_total"]
+    log(f"[bench] frames: {rate:.0f} Newton it/s ({r['newton']} its, {r['rounds']} rounds in {r['t_total']:.2f}s); "
+        f"band factor {r['ms_factor']:.1f} ms, solve {r['ms_solve']:.1f} ms")
+    rounds = max(r["rounds"], 1)
+    fam = {"band_factor_models": (r["ms_factor"], "mfma", r["factor_flops"] / 1e12, "TFLOP/s", PEAK_FP64_TFLOPS),
+           "band_solve_models": (r["ms_solve"], "hbm", r["solve_bytes"] / 1e9, "GB/s", PEAK_HBM_GBS)}
+    name = max(fam, key=lambda f: fam[f][0])
+    ms, bound, work, unit, peak = fam[name]
+    avg_s = ms / 1000.0 / rounds
+    achieved = work / avg_s
+    roof = {"kernel": name, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": achieved / peak, "traffic": None, "avg_launch_us": 1e6 * avg_s, "launches": rounds,
+            "share_of_run": ms / 1000.0 / r["t_total"]}
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline: oracle on a sample of the models, 1 core")
+        cpu = frame_cpu_baseline(paths, min(k, 10))
+        log(f"[bench] cpu: {cpu['value']:.1f} Newton it/s; {cpu['sample']}")
+    out = {
+        "metric": FRAME_METRIC, "value": rate, "unit": "Newton iters/s", "n_gpus": 1, "steps": k,
+        "warmup": args.warmup, "ms_per_step": 1000.0 * r["t_total"] / k, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 topologies, sections and harmonic loads)",
+        "config": {"workload": f"512 space frames (towers and lattice girders, 200-5000 members), {nfr} members, "
+                               f"NEQ {neq}, average-acceleration Newmark, dt 1e-3",
+                   "path": "batched fp64 band LDL^T, one workgroup per model", "time_steps_timed": k,
+                   "rounds": r["rounds"]},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    b.close()
+    print(json.dumps(out))
+
+
 def run_path(path: str, solver: int, warmup: int, steps: int, tol: float, timing: int = 0,
              schedule: int = 0) -> dict:
     import cubens
@@ -157,6 +247,9 @@ def main():
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="MIB",
                     help="only run 20 elementwise copies of MIB MiB (the reference kernel for "
                          "scaling the FETCH_SIZE/WRITE_SIZE counters) and exit")
+    ap.add_argument("--workload", default="panel", choices=("panel", "frames"),
+                    help="panel: the 256x256 shell panel MSAL (the headline); frames: the 512-frame "
+                         "nonlinear Newmark batch (--steps time steps)")
     ap.add_argument("--schedule", default="fused", choices=("fused", "split"),
                     help="band factorization schedule: one fused launch per tile column, or "
                          "diag/panel/update launches")
@@ -175,6 +268,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "frames":
+        run_frames(args)
+        return
     import cubens
     path = workload_file(args.n)
     results = {}

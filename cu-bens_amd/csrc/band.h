@@ -101,6 +101,10 @@ struct BandBatch {
     int32_t* gaps = nullptr;        // equations that no DOF slot maps to
     long ngap = 0;
     std::vector<BandSeg> segs_host;
+    // HIP events around the factor and the solve kernel of the last round
+    // (recorded when timing is on): [0, 1] factor, [2, 3] solve
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
     void setup(const std::vector<BandSeg>& sg, long n_total, int w_band, const std::vector<int32_t>& gap_eqs,
                hipStream_t s_);

@@ -313,7 +313,8 @@ __device__ __forceinline__ d4 mm16(FA A, FB B, d4 acc) {
 }
 
 __device__ __forceinline__ void diag_task(double (*xs)[T], double* ys, double* dd, DiagScratch<double>& sm,
-                                          double* __restrict__ lout, double* __restrict__ piv, long Kt) {
+                                          double* __restrict__ lout, double* __restrict__ piv, long Kt,
+                                          double* lds_out = nullptr) {
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, rr = l & 15, lg = l >> 4;
     double(*tsc)[256] = sm.tsc;
     // operand views: L_ik (rows r, cols t), Linv_ij as (t, c) and as (r, t), T slot s as (t, c)
@@ -455,7 +456,9 @@ __device__ __forceinline__ void diag_task(double (*xs)[T], double* ys, double* d
     __syncthreads();
     for (int idx = threadIdx.x; idx < TT; idx += NTH) {
         const int col = idx / T, row = idx % T;
-        lout[idx] = (row >= col) ? ys[ysw(col, row)] : 0.0;
+        const double v = (row >= col) ? ys[ysw(col, row)] : 0.0;
+        lout[idx] = v;
+        if (lds_out) lds_out[idx] = v;
     }
 }
 
@@ -840,40 +843,98 @@ __global__ void k_band_gap_pad(double* __restrict__ tiles, const int32_t* __rest
     tiles[tile_index(I, I, w) * TT + (i % T) * T + (i % T)] = 1.0;
 }
 
-// One workgroup factors one model, column by column, with the tile tasks of
-// the single-model schedules (the same arithmetic, so the same factor): D_K,
-// then P_{I,K} for the m tiles below it, then U_{I,J,K} row by row.  Tiles
-// pass between the tasks through memory inside the workgroup; the barrier
-// after each task orders its stores before the next task's loads.
+// One workgroup factors one model, column by column, with the tile arithmetic
+// of the single-model schedules (so the same factor): D_K, then P_{I,K} for
+// the m tiles below it, then U_{I,J,K}.  Linv_K stays in LDS for the panels.
+// While m <= 2 the panels also stay in LDS (P_{K+1,K} in ys, P_{K+2,K} in lk
+// once Linv_K is no longer needed), the updates take their operands from
+// there, and U_{K+1,K+1,K} is applied last and left in xs, where D_{K+1}
+// starts.  Wider columns take the operands of the updates from memory.  The
+// barrier after each task orders its LDS and memory writes before the next
+// task reads them.
 __global__ void __launch_bounds__(NTH) k_band_factor_models(double* __restrict__ tiles, double* __restrict__ linv,
                                                             double* __restrict__ piv, int w,
                                                             const BandSeg* __restrict__ segs,
                                                             const int* __restrict__ order) {
     __shared__ double xs[T][T];
     __shared__ double ys[TT];
+    __shared__ double lk[T][T];   // Linv_K as the panels' second operand (lk[k][row] = Linv(row, k))
     __shared__ double dd[T];
     __shared__ DiagScratch<double> sm;
     const BandSeg g = segs[order[blockIdx.x]];
     double(*yt)[T] = reinterpret_cast<double(*)[T]>(ys);
     const long Kend = g.t0 + g.nt;
+    bool staged = false;   // xs already holds A_KK
+    // xs = src * D_K by column (the scaled operand of the updates)
+    auto scale_into_xs = [&](const double (*src)[T], const double* dk) {
+        for (int idx = threadIdx.x; idx < TT; idx += NTH) xs[idx / T][idx % T] = src[idx / T][idx % T] * dk[idx / T];
+    };
     for (long K = g.t0; K < Kend; ++K) {
-        stage<false>(xs, tiles + tile_index(K, K, w) * TT);
-        __syncthreads();
-        diag_task(xs, ys, dd, sm, linv + K * (long)TT, piv, K);
-        __syncthreads();
-        const long m = (Kend - 1 - K < g.w) ? Kend - 1 - K : g.w;
-        for (long r = 1; r <= m; ++r) {
-            double* tile = tiles + tile_index(K + r, K, w) * TT;
-            panel_task<false, double>(tile, tile, linv + K * (long)TT, piv + K * T, xs, yt);
+        if (!staged) {
+            stage<false>(xs, tiles + tile_index(K, K, w) * TT);
             __syncthreads();
         }
-        for (long r = 1; r <= m; ++r)
-            for (long c = 1; c <= r; ++c) {
-                update_task<false, double>(tiles + tile_index(K + r, K + c, w) * TT,
-                                           tiles + tile_index(K + r, K, w) * TT,
-                                           tiles + tile_index(K + c, K, w) * TT, piv + K * T, xs, yt);
+        staged = false;
+        diag_task(xs, ys, dd, sm, linv + K * (long)TT, piv, K, &lk[0][0]);
+        __syncthreads();
+        const long m = (Kend - 1 - K < g.w) ? Kend - 1 - K : g.w;
+        const double* dk = piv + K * T;
+        const bool in_lds = m <= 2;
+        for (long r = 1; r <= m; ++r) {
+            // P_{K+r,K} as panel_task computes it, with Linv_K from LDS
+            double* tile = tiles + tile_index(K + r, K, w) * TT;
+            stage<false>(xs, tile);
+            __syncthreads();
+            TileAcc<double> acc;
+            acc.product(xs, lk);
+            if (in_lds) {
+                __syncthreads();   // lk is free after the last panel's product
+                double(*dst)[T] = (r == 1) ? yt : lk;
+                acc.each([&](int row, int col, double v) {
+                    const double p = v * (1.0 / dk[col]);
+                    tile[col * T + row] = p;
+                    dst[col][row] = p;
+                });
+            } else {
+                acc.each([&](int row, int col, double v) { tile[col * T + row] = v * (1.0 / dk[col]); });
+            }
+            __syncthreads();
+        }
+        if (in_lds) {
+            if (m == 2) {
+                // U_{K+2,K+1,K} and U_{K+2,K+2,K}: (P_{K+2} D_K) P_{K+1}^T and (P_{K+2} D_K) P_{K+2}^T
+                scale_into_xs(lk, dk);
+                __syncthreads();
+                TileAcc<double> acc;
+                acc.product(xs, yt);
+                double* o21 = tiles + tile_index(K + 2, K + 1, w) * TT;
+                acc.each([&](int row, int col, double v) { o21[col * T + row] -= v; });
+                acc.product(xs, lk);
+                double* o22 = tiles + tile_index(K + 2, K + 2, w) * TT;
+                acc.each([&](int row, int col, double v) { o22[col * T + row] -= v; });
                 __syncthreads();
             }
+            if (m >= 1) {
+                // U_{K+1,K+1,K}, left in xs for D_{K+1}
+                scale_into_xs(yt, dk);
+                __syncthreads();
+                TileAcc<double> acc;
+                acc.product(xs, yt);
+                __syncthreads();
+                const double* a11 = tiles + tile_index(K + 1, K + 1, w) * TT;
+                acc.each([&](int row, int col, double v) { xs[col][row] = a11[col * T + row] - v; });
+                __syncthreads();
+                staged = true;
+            }
+        } else {
+            for (long r = 1; r <= m; ++r)
+                for (long c = 1; c <= r; ++c) {
+                    update_task<false, double>(tiles + tile_index(K + r, K + c, w) * TT,
+                                               tiles + tile_index(K + r, K, w) * TT,
+                                               tiles + tile_index(K + c, K, w) * TT, piv + K * T, xs, yt);
+                    __syncthreads();
+                }
+        }
     }
 }
 
@@ -900,14 +961,37 @@ __global__ void __launch_bounds__(256) k_band_solve_models(const double* __restr
         if (tid < T) s = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
         return s;
     };
+    // the first RW off-diagonal tiles of a row and its Linv are loaded into
+    // registers before the row waits on the rows it depends on
+    constexpr int RW = 2;
+    double lt[RW][16], lv[16];
     for (long I = g.t0; I < Iend; ++I) {
+        const long J0 = (I - g.w > g.t0) ? I - g.w : g.t0;
+        const int nJ = int(I - J0);
+#pragma unroll
+        for (int j = 0; j < RW; ++j)
+            if (j < nJ) {
+                const double* l = tiles + tile_index(I, J0 + j, w) * TT;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) lt[j][k] = l[(k0 + k) * T + row];
+            }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) lv[k] = linv[I * (long)TT + (k0 + k) * T + row];
         double acc = 0.0;
         if (q == 0) {
             const long e = I * T + row;
             acc = (e - e0 < g.neq) ? rhs[e] : 0.0;
         }
-        const long J0 = (I - g.w > g.t0) ? I - g.w : g.t0;
-        for (long J = J0; J < I; ++J) {
+#pragma unroll
+        for (int j = 0; j < RW; ++j)
+            if (j < nJ) {
+                if (tid < T) vin[tid] = y[(J0 + j) * T + tid];
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < 16; ++k) acc -= lt[j][k] * vin[k0 + k];
+                __syncthreads();
+            }
+        for (long J = J0 + RW; J < I; ++J) {
             const double* l = tiles + tile_index(I, J, w) * TT;
             if (tid < T) vin[tid] = y[J * T + tid];
             __syncthreads();
@@ -918,19 +1002,37 @@ __global__ void __launch_bounds__(256) k_band_solve_models(const double* __restr
         double s = combine(acc);
         if (tid < T) vin[tid] = s;
         __syncthreads();
-        const double* lv = linv + I * (long)TT;
         double t = 0.0;
 #pragma unroll
-        for (int k = 0; k < 16; ++k) t += lv[(k0 + k) * T + row] * vin[k0 + k];
+        for (int k = 0; k < 16; ++k) t += lv[k] * vin[k0 + k];
         s = combine(t);
         if (tid < T) y[I * T + tid] = s;
         __syncthreads();
     }
     for (long I = Iend - 1; I >= g.t0; --I) {
         const int col = row;
-        double acc = (q == 0) ? y[I * T + col] / piv[I * T + col] : 0.0;
         const long J1 = (I + g.w < Iend - 1) ? I + g.w : Iend - 1;
-        for (long J = J1; J > I; --J) {
+        const int nJ = int(J1 - I);
+#pragma unroll
+        for (int j = 0; j < RW; ++j)
+            if (j < nJ) {
+                const double* l = tiles + tile_index(J1 - j, I, w) * TT;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) lt[j][k] = l[col * T + k0 + k];
+            }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) lv[k] = linv[I * (long)TT + col * T + k0 + k];
+        double acc = (q == 0) ? y[I * T + col] / piv[I * T + col] : 0.0;
+#pragma unroll
+        for (int j = 0; j < RW; ++j)
+            if (j < nJ) {
+                if (tid < T) vin[tid] = out[(J1 - j) * T + tid];
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < 16; ++k) acc -= lt[j][k] * vin[k0 + k];
+                __syncthreads();
+            }
+        for (long J = J1 - RW; J > I; --J) {
             const double* l = tiles + tile_index(J, I, w) * TT;
             if (tid < T) vin[tid] = out[J * T + tid];
             __syncthreads();
@@ -941,10 +1043,9 @@ __global__ void __launch_bounds__(256) k_band_solve_models(const double* __restr
         double s = combine(acc);
         if (tid < T) vin[tid] = s;
         __syncthreads();
-        const double* lv = linv + I * (long)TT;
         double t = 0.0;
 #pragma unroll
-        for (int k = 0; k < 16; ++k) t += lv[col * T + k0 + k] * vin[k0 + k];
+        for (int k = 0; k < 16; ++k) t += lv[k] * vin[k0 + k];
         s = combine(t);
         if (tid < T) out[I * T + tid] = s;
         __syncthreads();
@@ -1146,10 +1247,16 @@ void BandBatch::setup(const std::vector<BandSeg>& sg, long n_total, int w_band, 
     h2d(order, ord.data(), ord.size(), s);
     h2d(gaps, gap_eqs.data(), gap_eqs.size(), s);
     CB_CHECK(hipMemsetAsync(piv, 0, sizeof(double) * size_t(nt) * T, s));
+    for (hipEvent_t& e : ev) CB_CHECK(hipEventCreate(&e));
     CB_CHECK(hipStreamSynchronize(s));
 }
 
 void BandBatch::release() {
+    for (hipEvent_t& e : ev)
+        if (e) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
     void* ps[] = {tiles, linv, piv, yw, flag, segs, order, gaps};
     for (void* p : ps)
         if (p) (void)hipFree(p);
@@ -1170,15 +1277,19 @@ void BandBatch::factor(const Dev& d) {
                        d.rowptr, d.colidx, d.eq_of_slot, d.nj, d.bs, w, flag);
     if (ngap)
         hipLaunchKernelGGL(k_band_gap_pad, dim3(int((ngap + 255) / 256)), dim3(256), 0, s, tiles, gaps, ngap, w);
+    if (timing) CB_CHECK(hipEventRecord(ev[0], s));
     hipLaunchKernelGGL(k_band_factor_models, dim3(nseg), dim3(NTH), 0, s, tiles, linv, piv, w, segs, order);
+    if (timing) CB_CHECK(hipEventRecord(ev[1], s));
     CB_CHECK(hipGetLastError());
     CB_CHECK(hipMemcpyAsync(flag_host, flag, sizeof(int), hipMemcpyDeviceToHost, s));
 }
 
 void BandBatch::solve(const double* rhs, double* out) {
     if (!nseg) return;
+    if (timing) CB_CHECK(hipEventRecord(ev[2], s));
     hipLaunchKernelGGL(k_band_solve_models, dim3(nseg), dim3(256), 0, s, tiles, linv, piv, w, segs, order, rhs, yw,
                        out);
+    if (timing) CB_CHECK(hipEventRecord(ev[3], s));
     CB_CHECK(hipGetLastError());
 }
 

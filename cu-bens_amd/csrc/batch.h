@@ -62,6 +62,11 @@ struct BatchStats {
     double t_total = 0;
     long rounds = 0, newton = 0, steps = 0;
     int failed = 0;
+    // HIP-event time (ms) of the batched band factor and solve kernels, summed
+    // over the rounds (when timing is on), and the tile work of one factor
+    // launch (flops, T^3 + 2 T^3 m + T^3 m (m+1) per tile column of every model)
+    double ms_factor = 0, ms_solve = 0;
+    double factor_flops = 0, solve_bytes = 0;
 };
 
 class Batch {
@@ -74,7 +79,7 @@ public:
     const Model& model(int b) const { return models_[b]; }
     const Model& all() const { return all_; }
     // Runs every model to its end (or max_steps recorded steps each).
-    void run(int max_steps, bool keep_path, BatchStats& st);
+    void run(int max_steps, bool keep_path, BatchStats& st, bool timing = false);
     // after run(): per model status (0 ran to its end or to max_steps, 1 zero
     // norm in the convergence test, 3 minimum time increment reached), time and
     // Newton iterations of each recorded step, and the displacement rows

@@ -360,9 +360,11 @@ Batch::~Batch() {
     }
 }
 
-void Batch::run(int max_steps, bool keep_path, BatchStats& st) {
+void Batch::run(int max_steps, bool keep_path, BatchStats& st, bool timing) {
     const size_t nb = models_.size();
     const double t0 = now_s();
+    band_.timing = timing;
+    double ms_factor = 0, ms_solve = 0;
     std::vector<Machine> mc(nb);
     status.assign(nb, 0);
     message.assign(nb, "");
@@ -461,11 +463,32 @@ void Batch::run(int max_steps, bool keep_path, BatchStats& st) {
         d2h(sums_host_, sums_dev_, nb * 6, d_.s);
         CB_CHECK(hipStreamSynchronize(d_.s));
         if (*band_.flag_host & 2) throw DeviceError("batched band factorization: an entry outside a model's band");
+        if (timing) {
+            float a = 0, c = 0;
+            CB_CHECK(hipEventElapsedTime(&a, band_.ev[0], band_.ev[1]));
+            CB_CHECK(hipEventElapsedTime(&c, band_.ev[2], band_.ev[3]));
+            ms_factor += a;
+            ms_solve += c;
+        }
         ++rounds;
     }
     CB_CHECK(hipStreamSynchronize(d_.s));
     st.t_total = now_s() - t0;
     st.rounds = rounds;
+    st.ms_factor = ms_factor;
+    st.ms_solve = ms_solve;
+    // algorithmic work of one factor / solve launch over all models
+    double fl = 0, by = 0;
+    const double t3 = double(TILE * TILE * TILE);
+    for (const BandSeg& g : band_.segs_host) {
+        for (long K = 0; K < g.nt; ++K) {
+            const double m = double(std::min<long>(g.w, g.nt - 1 - K));
+            fl += t3 + 2 * t3 * m + t3 * m * (m + 1);
+        }
+        by += 2.0 * 8 * TILE * TILE * double(g.nt * (g.w + 1) + g.nt);   // two sweeps: factor and Linv tiles
+    }
+    st.factor_flops = fl;
+    st.solve_bytes = by;
     st.newton = 0;
     st.steps = 0;
     st.failed = 0;

@@ -683,13 +683,17 @@ int cb_batch_run(cb_batch* b, const cb_options* opt, cb_batch_stats* st) {
     if (o.solver != CB_SOLVER_BAND64) return fail(-1, "a batch runs the FP64 band solver only");
     GUARD_BEGIN
     cb::BatchStats bs;
-    b->bt->run(o.max_steps, o.keep_d != 0, bs);
+    b->bt->run(o.max_steps, o.keep_d != 0, bs, o.kernel_timing != 0);
     if (st) {
         st->t_total = bs.t_total;
         st->rounds = bs.rounds;
         st->newton = bs.newton;
         st->steps = bs.steps;
         st->failed = bs.failed;
+        st->ms_factor = bs.ms_factor;
+        st->ms_solve = bs.ms_solve;
+        st->factor_flops = bs.factor_flops;
+        st->solve_bytes = bs.solve_bytes;
     }
     return 0;
     GUARD_END

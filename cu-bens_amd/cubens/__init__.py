@@ -72,7 +72,8 @@ PF = C.POINTER(C.c_float)
 
 class BatchStats(C.Structure):
     _fields_ = [("t_total", C.c_double), ("rounds", C.c_long), ("newton", C.c_long),
-                ("steps", C.c_long), ("failed", C.c_int)]
+                ("steps", C.c_long), ("failed", C.c_int), ("ms_factor", C.c_double),
+                ("ms_solve", C.c_double), ("factor_flops", C.c_double), ("solve_bytes", C.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -310,9 +311,9 @@ class GpuBatch:
         except Exception:
             pass
 
-    def run(self, max_steps: int = 1 << 30, keep_d: bool = True) -> dict:
+    def run(self, max_steps: int = 1 << 30, keep_d: bool = True, timing: bool = False) -> dict:
         st = BatchStats()
-        opt = options(max_steps=max_steps, keep_d=int(keep_d))
+        opt = options(max_steps=max_steps, keep_d=int(keep_d), kernel_timing=int(timing))
         _check(lib().cb_batch_run(self.h, C.byref(opt), C.byref(st)))
         status = np.zeros(self.count, np.int32)
         nsteps = np.zeros(self.count, np.int32)

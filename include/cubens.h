@@ -211,6 +211,12 @@ typedef struct {
     long newton;         /* Newton iterations summed over the models      */
     long steps;          /* recorded time steps summed over the models    */
     int failed;          /* models whose driver stopped with an error     */
+    /* with opt->kernel_timing != 0: HIP-event time (ms) of the batched band
+     * factor and solve kernels summed over the rounds, and the algorithmic
+     * work of one launch of each (flops; bytes of the factor and Linv tiles
+     * the two sweeps read) */
+    double ms_factor, ms_solve;
+    double factor_flops, solve_bytes;
 } cb_batch_stats;
 /* Reads every model (as cb_open does) and builds the batch's device state;
  * frame-only models without member-end offsets or free warping DOFs. */
@@ -219,7 +225,8 @@ void cb_batch_close(cb_batch *b);
 int cb_batch_count(const cb_batch *b);
 int cb_batch_get_info(const cb_batch *b, int i, cb_info *out);
 /* Runs every model for at most opt->max_steps recorded steps (keep_d: keep
- * the displacement rows on the GPU for cb_batch_get_path). */
+ * the displacement rows on the GPU for cb_batch_get_path; kernel_timing:
+ * time the band kernels). */
 int cb_batch_run(cb_batch *b, const cb_options *opt, cb_batch_stats *st);
 /* Per model [count]: driver status (0 ran to its end or to max_steps, 1 zero
  * norm in the convergence test, 3 minimum time increment reached without
