@@ -49,6 +49,7 @@ struct Dev {
     int32_t *rowptr = nullptr, *colidx = nullptr, *diagblk = nullptr;
     int32_t *blk_tr = nullptr, *blk_sh = nullptr, *col_el_tr = nullptr, *col_el_sh = nullptr;
     int32_t *blk_fr = nullptr, *col_el_fr = nullptr;
+    int32_t *blkrow = nullptr, *gat_ptr_fr = nullptr, *gat_item_fr = nullptr;   // frame tangent gather
     std::vector<int32_t> col_ptr_tr, col_ptr_sh, col_ptr_fr;
     // properties; llen, farea and slength are the working copies the element
     // kernels read (the Newmark mass routines overwrite them)
@@ -103,6 +104,9 @@ void copy_gen(Dev& d, int dst, int src, bool with_ef, bool with_x);
 // frame members (frame.hip): tangent of generation g into K, internal forces
 // (ef_ip -> ef_i, triads _ip and _i), member lengths/triads after updatc
 void launch_frame_tangent(Dev& d, int g);
+// frame-only models: K <- tangent of generation g, one thread per joint block
+// summing its members' contributions in colour order, fixed slots as identity
+void launch_frame_tangent_gather(Dev& d, int g);
 void launch_frame_forces(Dev& d, double* f, const double* dd);
 void launch_frame_update(Dev& d, int gx);
 

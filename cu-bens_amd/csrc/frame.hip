@@ -188,6 +188,82 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent(
         }
 }
 
+// The same tangent as k_frame_tangent, gathered: two threads per joint block,
+// each summing rows 3u..3u+2 of the block over the block's members (each
+// member adds the 3 x 3 sub-blocks of its end pair (RA, CB)) in colour order,
+// which is the order the colour launches add them in, and writing them once
+// with the rows and columns of fixed slots as identity (k_fix_slots).  No
+// zeroing and no read-modify-write of K; a member's scalars are evaluated by
+// each thread whose block it touches.
+template <int RA, int CB, int U>
+__device__ __forceinline__ void frame_half_add(double acc[3][6], const FrameCoef& k, const double R[3][3]) {
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+        const int t1 = 2 * RA + U, t2 = 2 * CB + v;
+        double tmp[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int l = 0; l < 3; ++l)
+                tmp[r][l] = R[0][r] * kf(k, tidx(t1, 0), tidx(t2, l)) + R[1][r] * kf(k, tidx(t1, 1), tidx(t2, l)) +
+                            R[2][r] * kf(k, tidx(t1, 2), tidx(t2, l));
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int s = 0; s < 3; ++s)
+                acc[r][v * 3 + s] += tmp[r][0] * R[0][s] + tmp[r][1] * R[1][s] + tmp[r][2] * R[2][s];
+    }
+}
+
+__global__ void __launch_bounds__(TPB) k_frame_tangent_gather(
+    const int32_t* __restrict__ gptr, const int32_t* __restrict__ gitem, const int32_t* __restrict__ blkrow,
+    const int32_t* __restrict__ colidx, const int32_t* __restrict__ eq, long nnzb,
+    const double* __restrict__ c1, const double* __restrict__ c2, const double* __restrict__ c3,
+    const double* __restrict__ emod, const double* __restrict__ gmod, const double* __restrict__ carea,
+    const double* __restrict__ llen, const double* __restrict__ istrong, const double* __restrict__ iweak,
+    const double* __restrict__ ipolar, const double* __restrict__ iwarp,
+    const double* __restrict__ defllen, const double* __restrict__ ef, double* __restrict__ K, int geometric) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long kb = t >> 1;
+    const int u = int(t & 1);
+    if (kb >= nnzb) return;
+    double acc[3][6];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) acc[i][j] = 0.0;
+    for (int32_t it = gptr[kb]; it < gptr[kb + 1]; ++it) {
+        const int32_t item = gitem[it];
+        const long e = item >> 2;
+        FrameCoef k;
+        frame_coef(k, emod[e], gmod[e], carea[e], llen[e], istrong[e], iweak[e], ipolar[e], iwarp[e], ef + e * 14,
+                   defllen[e], geometric != 0);
+        double R[3][3];
+        load_triad(c1, c2, c3, e * 3, R);
+        switch ((item & 3) * 2 + u) {
+        case 0: frame_half_add<0, 0, 0>(acc, k, R); break;
+        case 1: frame_half_add<0, 0, 1>(acc, k, R); break;
+        case 2: frame_half_add<0, 1, 0>(acc, k, R); break;
+        case 3: frame_half_add<0, 1, 1>(acc, k, R); break;
+        case 4: frame_half_add<1, 0, 0>(acc, k, R); break;
+        case 5: frame_half_add<1, 0, 1>(acc, k, R); break;
+        case 6: frame_half_add<1, 1, 0>(acc, k, R); break;
+        default: frame_half_add<1, 1, 1>(acc, k, R); break;
+        }
+    }
+    const long A = blkrow[kb], B = colidx[kb];
+    double* Kb = K + kb * 36 + u * 18;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const bool rowfix = eq[A * 6 + u * 3 + i] < 0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const bool colfix = eq[B * 6 + j] < 0;
+            Kb[i * 6 + j] = (rowfix || colfix) ? ((A == B && u * 3 + i == j) ? 1.0 : 0.0) : acc[i][j];
+        }
+    }
+}
+
 // forces_fr, elastic branch (frame.c:902-1096, 1273-1309):
 //   dl = T_ip DD;  def = k(ef_ip) dl;  ef_i = T_i T_ip^T (ef_ip + def);  f += T_i^T ef_i,
 // one output triple at a time.
@@ -308,6 +384,15 @@ void launch_frame_tangent(Dev& d, int g) {
                            d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp,
                            d.defllen[g], d.ef[g], d.K, d.ntr, int(d.anaflag == 2));
     }
+    CB_CHECK(hipGetLastError());
+}
+
+void launch_frame_tangent_gather(Dev& d, int g) {
+    if (d.bs != 6) throw DeviceError("frame tangent gather: frame models carry 6 slots per joint");
+    hipLaunchKernelGGL(k_frame_tangent_gather, dim3(nblk(2 * d.nnzb)), dim3(TPB), 0, d.s, d.gat_ptr_fr, d.gat_item_fr,
+                       d.blkrow, d.colidx, d.eq_of_slot, d.nnzb, d.c1[g], d.c2[g], d.c3[g], d.emod, d.gmod, d.carea,
+                       d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[g], d.ef[g], d.K,
+                       int(d.anaflag == 2));
     CB_CHECK(hipGetLastError());
 }
 

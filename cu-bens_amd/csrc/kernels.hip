@@ -600,6 +600,15 @@ __global__ void __launch_bounds__(TPB) k_bsr_spmv(const T* __restrict__ K, const
 // ============================================================== launchers
 void launch_tangent(Dev& d, int gx, int g, bool want_fp32) {
     const long nb = d.nnzb * d.bs * d.bs;
+    if (d.nfr && !d.ntr && !d.nsh) {
+        // frame-only: every block is summed and written once, fixed slots included
+        launch_frame_tangent_gather(d, g);
+        if (want_fp32) {
+            hipLaunchKernelGGL(k_cast32, dim3(nblk(nb)), dim3(TPB), 0, d.s, d.K32, d.K, nb);
+            CB_CHECK(hipGetLastError());
+        }
+        return;
+    }
     CB_CHECK(hipMemsetAsync(d.K, 0, sizeof(double) * nb, d.s));
     const long to = d.ntr + d.nfr * 3, io = d.ntr * 2 + d.nfr * 2, po = d.ntr + d.nfr;
     for (size_t c = 0; c + 1 < d.col_ptr_tr.size(); ++c) {
@@ -767,6 +776,9 @@ void dev_create(Dev& d, const Model& m) {
     d.col_ptr_tr = m.col_ptr_tr;
     d.col_ptr_sh = m.col_ptr_sh;
     d.col_ptr_fr = m.col_ptr_fr;
+    d.blkrow = upi(m.blkrow);
+    d.gat_ptr_fr = upi(m.gat_ptr_fr);
+    d.gat_item_fr = upi(m.gat_item_fr);
     d.emod = upd(m.emod);
     d.carea = upd(m.carea);
     d.llen = upd(m.llength);
@@ -820,7 +832,8 @@ void dev_reset_state(Dev& d, const Model& m) {
 
 void dev_destroy(Dev& d) {
     void* ps[] = {d.minc, d.mcode, d.jcode, d.eq_of_slot, d.rowptr, d.colidx, d.diagblk, d.blk_tr,
-                  d.blk_sh, d.blk_fr, d.col_el_tr, d.col_el_sh, d.col_el_fr, d.emod, d.carea,
+                  d.blk_sh, d.blk_fr, d.col_el_tr, d.col_el_sh, d.col_el_fr, d.blkrow, d.gat_ptr_fr,
+                  d.gat_item_fr, d.emod, d.carea,
                   d.llen, d.dens, d.gmod, d.istrong, d.iweak, d.ipolar, d.iwarp, d.auxpt, d.nu,
                   d.thick, d.xlocal, d.farea, d.slength, d.K, d.K32, d.red};
     for (void* p : ps)

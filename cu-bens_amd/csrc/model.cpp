@@ -487,6 +487,22 @@ void build_pattern(Model& m) {
     colour(m.ntr, 2, 0, m.col_ptr_tr, m.col_el_tr);
     colour(m.nfr, 2, m.ntr * 2, m.col_ptr_fr, m.col_el_fr);
     colour(m.nsh, 3, m.ntr * 2 + m.nfr * 2, m.col_ptr_sh, m.col_el_sh);
+
+    // block rows, and the frame contributions of each block in colour order
+    // (the order in which the colour launches add them)
+    const size_t nnzb = m.colidx.size();
+    m.blkrow.assign(nnzb, 0);
+    for (long j = 0; j < m.nj; ++j)
+        for (int32_t k = m.rowptr[j]; k < m.rowptr[j + 1]; ++k) m.blkrow[size_t(k)] = int32_t(j);
+    m.gat_ptr_fr.assign(nnzb + 1, 0);
+    for (long e = 0; e < m.nfr; ++e)
+        for (int q = 0; q < 4; ++q) m.gat_ptr_fr[size_t(m.blk_fr[size_t(e * 4 + q)]) + 1]++;
+    for (size_t k = 0; k < nnzb; ++k) m.gat_ptr_fr[k + 1] += m.gat_ptr_fr[k];
+    m.gat_item_fr.assign(size_t(m.nfr * 4), 0);
+    std::vector<int32_t> fill(m.gat_ptr_fr.begin(), m.gat_ptr_fr.end() - 1);
+    for (int32_t e : m.col_el_fr)
+        for (int q = 0; q < 4; ++q)
+            m.gat_item_fr[size_t(fill[size_t(m.blk_fr[size_t(e) * 4 + q])]++)] = e * 4 + q;
 }
 
 }  // namespace cb

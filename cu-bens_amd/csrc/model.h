@@ -51,6 +51,9 @@ struct Model {
     std::vector<int32_t> blk_sh;  // NE_SH*9
     // colour classes: elements of one class share no joint
     std::vector<int32_t> col_ptr_tr, col_el_tr, col_ptr_fr, col_el_fr, col_ptr_sh, col_el_sh;
+    // row joint of every block, and per block the frame contributions that add
+    // into it, in colour order: item = member * 4 + end_row * 2 + end_col
+    std::vector<int32_t> blkrow, gat_ptr_fr, gat_item_fr;
     long bandwidth = 0;  // max column height of the skyline
 };
 
