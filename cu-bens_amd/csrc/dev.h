@@ -50,6 +50,8 @@ struct Dev {
     int32_t *blk_tr = nullptr, *blk_sh = nullptr, *col_el_tr = nullptr, *col_el_sh = nullptr;
     int32_t *blk_fr = nullptr, *col_el_fr = nullptr;
     int32_t *blkrow = nullptr, *gat_ptr_fr = nullptr, *gat_item_fr = nullptr;   // frame tangent gather
+    int32_t *jnt_ptr_fr = nullptr, *jnt_item_fr = nullptr;                       // frame force gather
+    double* fh = nullptr;   // frame end forces in global axes, 12 per member
     std::vector<int32_t> col_ptr_tr, col_ptr_sh, col_ptr_fr;
     // properties; llen, farea and slength are the working copies the element
     // kernels read (the Newmark mass routines overwrite them)

@@ -267,20 +267,18 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent_gather(
 // forces_fr, elastic branch (frame.c:902-1096, 1273-1309):
 //   dl = T_ip DD;  def = k(ef_ip) dl;  ef_i = T_i T_ip^T (ef_ip + def);  f += T_i^T ef_i,
 // one output triple at a time.
-__global__ void __launch_bounds__(TPB) k_frame_forces(
-    const int32_t* __restrict__ els, int n, const int32_t* __restrict__ mcode,
-    const double* __restrict__ dd, const double* __restrict__ c1p, const double* __restrict__ c2p,
-    const double* __restrict__ c3p, const double* __restrict__ c1c, const double* __restrict__ c2c,
-    const double* __restrict__ c3c, const double* __restrict__ emod, const double* __restrict__ gmod,
-    const double* __restrict__ carea, const double* __restrict__ llen,
-    const double* __restrict__ istrong, const double* __restrict__ iweak,
-    const double* __restrict__ ipolar, const double* __restrict__ iwarp,
-    const double* __restrict__ defllen_ip, const double* __restrict__ ef_ip,
-    double* __restrict__ ef_i, double* __restrict__ f, long ntr, int geometric) {
-    int it = blockIdx.x * blockDim.x + threadIdx.x;
-    if (it >= n) return;
-    const long e = els[it], p = ntr + e;
-    const int32_t* mc = mcode + ntr * 6 + e * 14;
+// One member of forces_fr: ef_i, and the member's end forces in global axes
+// (T_i^T ef_i) handed to out(triple, m, value).
+template <class Out>
+__device__ __forceinline__ void frame_member_forces(
+    long e, const int32_t* __restrict__ mc, const double* __restrict__ dd, const double* __restrict__ c1p,
+    const double* __restrict__ c2p, const double* __restrict__ c3p, const double* __restrict__ c1c,
+    const double* __restrict__ c2c, const double* __restrict__ c3c, const double* __restrict__ emod,
+    const double* __restrict__ gmod, const double* __restrict__ carea, const double* __restrict__ llen,
+    const double* __restrict__ istrong, const double* __restrict__ iweak, const double* __restrict__ ipolar,
+    const double* __restrict__ iwarp, const double* __restrict__ defllen_ip, const double* __restrict__ ef_ip,
+    double* __restrict__ ef_i, long ntr, int geometric, Out&& out) {
+    const long p = ntr + e;
     const double* fe = ef_ip + ntr * 2 + e * 14;
     FrameCoef k;
     frame_coef(k, emod[p], gmod[e], carea[p], llen[p], istrong[e], iweak[e], ipolar[e], iwarp[e], fe,
@@ -325,12 +323,9 @@ __global__ void __launch_bounds__(TPB) k_frame_forces(
             g[r] = Q[r][0] * v[0] + Q[r][1] * v[1] + Q[r][2] * v[2];
             ei[tidx(to, r)] = g[r];
         }
-        // T_i^T ef_i on this triple, scattered by mcode
+        // T_i^T ef_i on this triple
 #pragma unroll
-        for (int m = 0; m < 3; ++m) {
-            int32_t q = mc[tidx(to, m)];
-            if (q) f[q - 1] += Ri[0][m] * g[0] + Ri[1][m] * g[1] + Ri[2][m] * g[2];
-        }
+        for (int m = 0; m < 3; ++m) out(to, m, Ri[0][m] * g[0] + Ri[1][m] * g[1] + Ri[2][m] * g[2]);
     }
     // warping rows (bimoments): T_i T_ip^T is 1 there
 #pragma unroll
@@ -343,6 +338,66 @@ __global__ void __launch_bounds__(TPB) k_frame_forces(
             for (int m = 0; m < 3; ++m) s += kf(k, i, tidx(t, m)) * dl[t][m];
         ei[i] = fe[i] + s;
     }
+}
+
+// forces_fr, elastic branch (frame.c:902-1096, 1273-1309), for the members of
+// one colour, scattered by mcode into f.
+__global__ void __launch_bounds__(TPB) k_frame_forces(
+    const int32_t* __restrict__ els, int n, const int32_t* __restrict__ mcode,
+    const double* __restrict__ dd, const double* __restrict__ c1p, const double* __restrict__ c2p,
+    const double* __restrict__ c3p, const double* __restrict__ c1c, const double* __restrict__ c2c,
+    const double* __restrict__ c3c, const double* __restrict__ emod, const double* __restrict__ gmod,
+    const double* __restrict__ carea, const double* __restrict__ llen,
+    const double* __restrict__ istrong, const double* __restrict__ iweak,
+    const double* __restrict__ ipolar, const double* __restrict__ iwarp,
+    const double* __restrict__ defllen_ip, const double* __restrict__ ef_ip,
+    double* __restrict__ ef_i, double* __restrict__ f, long ntr, int geometric) {
+    int it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= n) return;
+    const long e = els[it];
+    const int32_t* mc = mcode + ntr * 6 + e * 14;
+    frame_member_forces(e, mc, dd, c1p, c2p, c3p, c1c, c2c, c3c, emod, gmod, carea, llen, istrong, iweak, ipolar,
+                        iwarp, defllen_ip, ef_ip, ef_i, ntr, geometric, [&](int to, int m, double val) {
+                            const int32_t q = mc[tidx(to, m)];
+                            if (q) f[q - 1] += val;
+                        });
+}
+
+// Frame-only models: every member in one launch, its global end forces into
+// h[e * 12 + 3 * triple + m] ...
+__global__ void __launch_bounds__(TPB) k_frame_forces_members(
+    long n, const int32_t* __restrict__ mcode, const double* __restrict__ dd, const double* __restrict__ c1p,
+    const double* __restrict__ c2p, const double* __restrict__ c3p, const double* __restrict__ c1c,
+    const double* __restrict__ c2c, const double* __restrict__ c3c, const double* __restrict__ emod,
+    const double* __restrict__ gmod, const double* __restrict__ carea, const double* __restrict__ llen,
+    const double* __restrict__ istrong, const double* __restrict__ iweak, const double* __restrict__ ipolar,
+    const double* __restrict__ iwarp, const double* __restrict__ defllen_ip, const double* __restrict__ ef_ip,
+    double* __restrict__ ef_i, double* __restrict__ h, int geometric) {
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double* he = h + e * 12;
+    frame_member_forces(e, mcode + e * 14, dd, c1p, c2p, c3p, c1c, c2c, c3c, emod, gmod, carea, llen, istrong,
+                        iweak, ipolar, iwarp, defllen_ip, ef_ip, ef_i, 0, geometric,
+                        [&](int to, int m, double val) { he[3 * to + m] = val; });
+}
+
+// ... then one thread per free slot adds the end forces of the members at its
+// joint in colour order, the order in which the colour launches add them.
+__global__ void k_frame_forces_gather(const int32_t* __restrict__ jptr, const int32_t* __restrict__ jitem,
+                                      const int32_t* __restrict__ eq, long nslot, const double* __restrict__ h,
+                                      double* __restrict__ f) {
+    const long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (r >= nslot) return;
+    const int32_t q = eq[r];
+    if (q < 0) return;
+    const long j = r / 6;
+    const int s = int(r % 6);
+    double acc = f[q];
+    for (int32_t it = jptr[j]; it < jptr[j + 1]; ++it) {
+        const int32_t item = jitem[it];
+        acc += h[long(item >> 1) * 12 + 6 * (item & 1) + s];
+    }
+    f[q] = acc;
 }
 
 // misc.c:122-144: member length and triad from the current chord and the
@@ -397,6 +452,16 @@ void launch_frame_tangent_gather(Dev& d, int g) {
 }
 
 void launch_frame_forces(Dev& d, double* f, const double* dd) {
+    if (!d.ntr && !d.nsh && d.bs == 6) {
+        hipLaunchKernelGGL(k_frame_forces_members, dim3(nblk(d.nfr)), dim3(TPB), 0, d.s, d.nfr, d.mcode, dd,
+                           d.c1[G_IP], d.c2[G_IP], d.c3[G_IP], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.emod, d.gmod,
+                           d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[G_IP], d.ef[G_IP],
+                           d.ef[G_I], d.fh, int(d.anaflag == 2));
+        hipLaunchKernelGGL(k_frame_forces_gather, dim3(nblk(d.nslot, 256)), dim3(256), 0, d.s, d.jnt_ptr_fr,
+                           d.jnt_item_fr, d.eq_of_slot, d.nslot, d.fh, f);
+        CB_CHECK(hipGetLastError());
+        return;
+    }
     for (size_t c = 0; c + 1 < d.col_ptr_fr.size(); ++c) {
         int n = d.col_ptr_fr[c + 1] - d.col_ptr_fr[c];
         if (!n) continue;

@@ -779,6 +779,9 @@ void dev_create(Dev& d, const Model& m) {
     d.blkrow = upi(m.blkrow);
     d.gat_ptr_fr = upi(m.gat_ptr_fr);
     d.gat_item_fr = upi(m.gat_item_fr);
+    d.jnt_ptr_fr = upi(m.jnt_ptr_fr);
+    d.jnt_item_fr = upi(m.jnt_item_fr);
+    d.fh = dalloc<double>(size_t(m.nfr * 12));
     d.emod = upd(m.emod);
     d.carea = upd(m.carea);
     d.llen = upd(m.llength);
@@ -833,7 +836,7 @@ void dev_reset_state(Dev& d, const Model& m) {
 void dev_destroy(Dev& d) {
     void* ps[] = {d.minc, d.mcode, d.jcode, d.eq_of_slot, d.rowptr, d.colidx, d.diagblk, d.blk_tr,
                   d.blk_sh, d.blk_fr, d.col_el_tr, d.col_el_sh, d.col_el_fr, d.blkrow, d.gat_ptr_fr,
-                  d.gat_item_fr, d.emod, d.carea,
+                  d.gat_item_fr, d.jnt_ptr_fr, d.jnt_item_fr, d.fh, d.emod, d.carea,
                   d.llen, d.dens, d.gmod, d.istrong, d.iweak, d.ipolar, d.iwarp, d.auxpt, d.nu,
                   d.thick, d.xlocal, d.farea, d.slength, d.K, d.K32, d.red};
     for (void* p : ps)

@@ -503,6 +503,16 @@ void build_pattern(Model& m) {
     for (int32_t e : m.col_el_fr)
         for (int q = 0; q < 4; ++q)
             m.gat_item_fr[size_t(fill[size_t(m.blk_fr[size_t(e) * 4 + q])]++)] = e * 4 + q;
+    const long fo = m.ntr * 2;
+    m.jnt_ptr_fr.assign(size_t(m.nj) + 1, 0);
+    for (long e = 0; e < m.nfr; ++e)
+        for (int a = 0; a < 2; ++a) m.jnt_ptr_fr[size_t(m.minc[size_t(fo + e * 2 + a)])]++;   // 1-based joint
+    for (long j = 0; j < m.nj; ++j) m.jnt_ptr_fr[size_t(j) + 1] += m.jnt_ptr_fr[size_t(j)];
+    m.jnt_item_fr.assign(size_t(m.nfr * 2), 0);
+    std::vector<int32_t> jf(m.jnt_ptr_fr.begin(), m.jnt_ptr_fr.end() - 1);
+    for (int32_t e : m.col_el_fr)
+        for (int a = 0; a < 2; ++a)
+            m.jnt_item_fr[size_t(jf[size_t(m.minc[size_t(fo + long(e) * 2 + a)] - 1)]++)] = e * 2 + a;
 }
 
 }  // namespace cb

@@ -54,6 +54,8 @@ struct Model {
     // row joint of every block, and per block the frame contributions that add
     // into it, in colour order: item = member * 4 + end_row * 2 + end_col
     std::vector<int32_t> blkrow, gat_ptr_fr, gat_item_fr;
+    // per joint, the frame member ends at it in colour order: item = member * 2 + end
+    std::vector<int32_t> jnt_ptr_fr, jnt_item_fr;
     long bandwidth = 0;  // max column height of the skyline
 };
 
