@@ -942,28 +942,52 @@ __global__ void __launch_bounds__(NTH) k_band_factor_models(double* __restrict__
 // the backward sweep into out, tile row by tile row.  Thread (row, q) keeps the
 // partial sum over columns 16q..16q+15 and the partial sums are combined in
 // the order of k_fwd_sweep / k_bwd_sweep, so a model's solution is the one
-// those kernels give.
+// those kernels give.  The last RING rows of the solution stay in LDS, where
+// every thread reads the values it needs; the first RW off-diagonal tiles of
+// a row and its Linv are loaded into registers when the row starts.  A row
+// then costs three barriers.
 __global__ void __launch_bounds__(256) k_band_solve_models(const double* __restrict__ tiles,
                                                            const double* __restrict__ linv,
                                                            const double* __restrict__ piv, int w,
                                                            const BandSeg* __restrict__ segs,
                                                            const int* __restrict__ order,
                                                            const double* __restrict__ rhs, double* y, double* out) {
+    constexpr int RING = 8, RW = 2;
     __shared__ double vin[T];
     __shared__ double part[4][T];
+    __shared__ double ring[RING][T];
     const BandSeg g = segs[order[blockIdx.x]];
     const int tid = threadIdx.x, row = tid % T, q = tid / T, k0 = 16 * q;
     const long Iend = g.t0 + g.nt, e0 = g.t0 * T;
-    auto combine = [&](double v) {
-        part[q][row] = v;
+    // acc -= sum_k a[k] * (row J of the solution)[k0 + k], from the ring when
+    // row J is one of the last RING rows
+    auto sub_row = [&](double& acc, const double* a, const double* v, long I, long J) {
+        if (I - J < RING && J - I < RING) {
+            const double* rj = &ring[J % RING][k0];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) acc -= a[k] * rj[k];
+        } else {
+            const double* vj = v + J * T + k0;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) acc -= a[k] * vj[k];
+        }
+    };
+    // acc -> combined over q -> Linv-product (lv) -> combined; thread tid < T
+    // returns entry tid of the row's result
+    auto finish_row = [&](double acc, const double* lv) {
+        part[q][row] = acc;
+        __syncthreads();
+        if (tid < T) vin[tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+        __syncthreads();
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) t += lv[k] * vin[k0 + k];
+        part[q][row] = t;
         __syncthreads();
         double s = 0.0;
         if (tid < T) s = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
         return s;
     };
-    // the first RW off-diagonal tiles of a row and its Linv are loaded into
-    // registers before the row waits on the rows it depends on
-    constexpr int RW = 2;
     double lt[RW][16], lv[16];
     for (long I = g.t0; I < Iend; ++I) {
         const long J0 = (I - g.w > g.t0) ? I - g.w : g.t0;
@@ -984,29 +1008,19 @@ __global__ void __launch_bounds__(256) k_band_solve_models(const double* __restr
         }
 #pragma unroll
         for (int j = 0; j < RW; ++j)
-            if (j < nJ) {
-                if (tid < T) vin[tid] = y[(J0 + j) * T + tid];
-                __syncthreads();
-#pragma unroll
-                for (int k = 0; k < 16; ++k) acc -= lt[j][k] * vin[k0 + k];
-                __syncthreads();
-            }
+            if (j < nJ) sub_row(acc, lt[j], y, I, J0 + j);
         for (long J = J0 + RW; J < I; ++J) {
             const double* l = tiles + tile_index(I, J, w) * TT;
-            if (tid < T) vin[tid] = y[J * T + tid];
-            __syncthreads();
+            double a[16];
 #pragma unroll
-            for (int k = 0; k < 16; ++k) acc -= l[(k0 + k) * T + row] * vin[k0 + k];
-            __syncthreads();
+            for (int k = 0; k < 16; ++k) a[k] = l[(k0 + k) * T + row];
+            sub_row(acc, a, y, I, J);
         }
-        double s = combine(acc);
-        if (tid < T) vin[tid] = s;
-        __syncthreads();
-        double t = 0.0;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) t += lv[k] * vin[k0 + k];
-        s = combine(t);
-        if (tid < T) y[I * T + tid] = s;
+        const double s = finish_row(acc, lv);
+        if (tid < T) {
+            y[I * T + tid] = s;
+            ring[I % RING][tid] = s;
+        }
         __syncthreads();
     }
     for (long I = Iend - 1; I >= g.t0; --I) {
@@ -1025,29 +1039,19 @@ __global__ void __launch_bounds__(256) k_band_solve_models(const double* __restr
         double acc = (q == 0) ? y[I * T + col] / piv[I * T + col] : 0.0;
 #pragma unroll
         for (int j = 0; j < RW; ++j)
-            if (j < nJ) {
-                if (tid < T) vin[tid] = out[(J1 - j) * T + tid];
-                __syncthreads();
-#pragma unroll
-                for (int k = 0; k < 16; ++k) acc -= lt[j][k] * vin[k0 + k];
-                __syncthreads();
-            }
+            if (j < nJ) sub_row(acc, lt[j], out, I, J1 - j);
         for (long J = J1 - RW; J > I; --J) {
             const double* l = tiles + tile_index(J, I, w) * TT;
-            if (tid < T) vin[tid] = out[J * T + tid];
-            __syncthreads();
+            double a[16];
 #pragma unroll
-            for (int k = 0; k < 16; ++k) acc -= l[col * T + k0 + k] * vin[k0 + k];
-            __syncthreads();
+            for (int k = 0; k < 16; ++k) a[k] = l[col * T + k0 + k];
+            sub_row(acc, a, out, I, J);
         }
-        double s = combine(acc);
-        if (tid < T) vin[tid] = s;
-        __syncthreads();
-        double t = 0.0;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) t += lv[k] * vin[k0 + k];
-        s = combine(t);
-        if (tid < T) out[I * T + tid] = s;
+        const double s = finish_row(acc, lv);
+        if (tid < T) {
+            out[I * T + tid] = s;
+            ring[I % RING][tid] = s;
+        }
         __syncthreads();
     }
 }
