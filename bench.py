@@ -63,12 +63,14 @@ def kernel_model(info) -> dict:
       column : one fused launch = diag + panel + update of a column (average
                over the nt launches of a factorization); its time is the launch
                period (the nt launches of a factorization timed as one span)
-      sweeps : every factor tile and every Linv tile read once (8 B entries)
+      sweeps : every off-diagonal factor tile and every Linv tile read once (8 B
+               entries), the right-hand side read and the solution written
     """
     nt = (info.neq + TILE - 1) // TILE
     w = min((info.bandwidth + TILE - 1) // TILE, nt - 1)
     t3 = TILE ** 3
-    sweep_bytes = 8 * TILE * TILE * (nt * (w + 1) + nt) + 3 * 8 * nt * TILE
+    offd = sum(min(w, i) for i in range(nt))
+    sweep_bytes = 8 * TILE * TILE * (offd + nt) + 3 * 8 * nt * TILE
     whole = sum(t3 + 2.0 * t3 * m + t3 * m * (m + 1) for m in (min(w, nt - 1 - k) for k in range(nt)))
     return {
         "band_column": ("mfma", whole / nt, "TFLOP/s"),

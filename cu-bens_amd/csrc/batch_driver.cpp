@@ -485,7 +485,10 @@ void Batch::run(int max_steps, bool keep_path, BatchStats& st, bool timing) {
             const double m = double(std::min<long>(g.w, g.nt - 1 - K));
             fl += t3 + 2 * t3 * m + t3 * m * (m + 1);
         }
-        by += 2.0 * 8 * TILE * TILE * double(g.nt * (g.w + 1) + g.nt);   // two sweeps: factor and Linv tiles
+        // two sweeps, each reading the off-diagonal factor tiles and the Linv tiles once
+        long offd = 0;
+        for (long I = 0; I < g.nt; ++I) offd += std::min<long>(g.w, I);
+        by += 2.0 * 8 * TILE * TILE * double(offd + g.nt);
     }
     st.factor_flops = fl;
     st.solve_bytes = by;
