@@ -51,6 +51,7 @@ struct BandLDLT {
     int w = 0;
     hipStream_t s = nullptr;
     R* tiles = nullptr;
+    R* tiles_t = nullptr;    // the factor's off-diagonal tiles transposed (row-major), for the backward sweep
     R* linv = nullptr;
     double* piv = nullptr;   // pivots D (padded to nt*T)
     double* xw = nullptr;    // work vectors (padded)
@@ -70,7 +71,7 @@ struct BandLDLT {
     int factor(const Dev& d);
     // out = (L D L^T)^{-1} rhs, both device NEQ vectors (may alias).
     void solve(const double* rhs, double* out);
-    size_t bytes() const { return sizeof(R) * size_t(nt) * size_t(w + 1) * 64 * 64; }
+    size_t bytes() const { return 2 * sizeof(R) * size_t(nt) * size_t(w + 1) * 64 * 64; }
 };
 
 // A batch of independent models sharing one FP64 band: model b owns the tile

@@ -473,16 +473,22 @@ __device__ __forceinline__ void stage_scaled(R (*dst)[T], const R* __restrict__ 
     for (int idx = threadIdx.x; idx < TT; idx += NTH) dst[idx / T][idx % T] = cld<CO>(&src[idx]) * R(d[idx / T]);
 }
 
-// P_{I,K}: out = A_IK Linv_K^T D_K^{-1} (out may alias a)
+// P_{I,K}: out = A_IK Linv_K^T D_K^{-1} (out may alias a); tout receives the
+// transposed tile (row-major), which the backward sweep reads
 template <bool CO, class R>
 __device__ __forceinline__ void panel_task(R* __restrict__ out, const R* __restrict__ a, const R* __restrict__ lv,
-                                           const double* __restrict__ dk, R (*xs)[T], R (*ys)[T]) {
+                                           const double* __restrict__ dk, R (*xs)[T], R (*ys)[T],
+                                           R* __restrict__ tout) {
     stage<false>(xs, a);
     stage<false>(ys, lv);
     __syncthreads();
     TileAcc<R> acc;
     acc.product(xs, ys);
-    acc.each([&](int row, int col, R v) { cst<CO>(&out[col * T + row], v * (R(1) / R(dk[col]))); });
+    acc.each([&](int row, int col, R v) {
+        const R p = v * (R(1) / R(dk[col]));
+        cst<CO>(&out[col * T + row], p);
+        tout[row * T + col] = p;
+    });
 }
 
 // U_{I,J,K}: out -= (L_IK D_K) L_JK^T
@@ -512,12 +518,13 @@ __global__ void __launch_bounds__(NTH) k_tile_diag(const R* __restrict__ tiles, 
 
 // P_{I,K} for I = Kt+1+blockIdx.x
 template <class R>
-__global__ void __launch_bounds__(NTH) k_tile_panel(R* __restrict__ tiles, const R* __restrict__ linv,
-                                                    const double* __restrict__ piv, long Kt, int w) {
+__global__ void __launch_bounds__(NTH) k_tile_panel(R* __restrict__ tiles, R* __restrict__ tiles_t,
+                                                    const R* __restrict__ linv, const double* __restrict__ piv,
+                                                    long Kt, int w) {
     __shared__ R xs[T][T];
     __shared__ R ys[T][T];
-    R* tile = tiles + tile_index(Kt + 1 + blockIdx.x, Kt, w) * TT;
-    panel_task<false, R>(tile, tile, linv + Kt * (long)TT, piv + Kt * T, xs, ys);
+    const long ti = tile_index(Kt + 1 + blockIdx.x, Kt, w) * TT;
+    panel_task<false, R>(tiles + ti, tiles + ti, linv + Kt * (long)TT, piv + Kt * T, xs, ys, tiles_t + ti);
 }
 
 // (r, s), s <= r, of entry t of the triangle enumerated row by row
@@ -554,7 +561,8 @@ __global__ void __launch_bounds__(NTH) k_tile_update(R* __restrict__ tiles, cons
 // m = 25, so that all of them are resident at once), 1 for m = 0.  Workgroup 0 neither
 // stores nor waits before its diagonal step.
 template <class R>
-__global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* __restrict__ linv,
+__global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* __restrict__ tiles_t,
+                                                     R* __restrict__ linv,
                                                      double* __restrict__ piv, long K, int w, long nt,
                                                      int* flags, int epoch, long spin_limit, int* err) {
     __shared__ R xs[T][T];
@@ -601,18 +609,20 @@ __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* _
         const long pb = (b == m) ? 0 : b;
         const long I = K + 1 + pb;
         R* tile = tiles + tile_index(I, K, w) * TT;
-        if (pb == 0) {
-            // stage, then wait until workgroup 0 has read the tile before overwriting it
-            stage<false>(xs, tile);
-            stage<false>(ys, linv + K * (long)TT);
-            const bool ok = wait_flag_bounded(flags, w, epoch, spin_limit, err, &ok_s);
-            TileAcc<R> acc;
-            acc.product(xs, ys);
-            if (ok) acc.each([&](int row, int col, R v) { cst<true>(&tile[col * T + row], v * (R(1) / R(dk[col]))); });
-        } else {
-            panel_task<true, R>(tile, tile, linv + K * (long)TT, dk, xs, ys);
-        }
+        R* ttile = tiles_t + tile_index(I, K, w) * TT;
+        // stage (workgroup m then waits until workgroup 0 has read the tile
+        // before overwriting it), product, coherent stores, flag; the
+        // transposed copy for the backward sweep is stored after the flag
+        stage<false>(xs, tile);
+        stage<false>(ys, linv + K * (long)TT);
+        bool ok = true;
+        if (pb == 0) ok = wait_flag_bounded(flags, w, epoch, spin_limit, err, &ok_s);
+        else __syncthreads();
+        TileAcc<R> acc;
+        acc.product(xs, ys);
+        if (ok) acc.each([&](int row, int col, R v) { cst<true>(&tile[col * T + row], v * (R(1) / R(dk[col]))); });
         raise_flag(flags, pb, epoch);
+        if (ok) acc.each([&](int row, int col, R v) { ttile[row * T + col] = v * (R(1) / R(dk[col])); });
         return;
     }
     // row r >= 1 of the update triangle holds ceil((r+1)/2) workgroups
@@ -762,9 +772,10 @@ __global__ void __launch_bounds__(256) k_fwd_sweep(const R* __restrict__ tiles, 
 }
 
 // x_I = Linv_I^T (y_I / D_I - sum_{J>I} L_JI^T x_J); workgroup g handles
-// I = nt-1-g.  x must hold SENT.
+// I = nt-1-g.  x must hold SENT.  The factor tiles are read from their
+// transposed copies, so that consecutive lanes read consecutive entries.
 template <class R>
-__global__ void __launch_bounds__(256) k_bwd_sweep(const R* __restrict__ tiles, const R* __restrict__ linv,
+__global__ void __launch_bounds__(256) k_bwd_sweep(const R* __restrict__ tiles_t, const R* __restrict__ linv,
                                                    const double* __restrict__ y, const double* __restrict__ piv,
                                                    double* x, long nt, int w, long spin_limit, int* err) {
     __shared__ double vin[T];
@@ -778,23 +789,23 @@ __global__ void __launch_bounds__(256) k_bwd_sweep(const R* __restrict__ tiles, 
     const long J1 = (I + w < nt - 1) ? I + w : nt - 1;
     const bool has_next = I + 1 <= J1;
     if (has_next) {
-        const R* lnp = tiles + ((I + 1) * (w + 1) + (w - 1)) * (long)TT;   // tile (I+1, I)
+        const R* lnp = tiles_t + ((I + 1) * (w + 1) + (w - 1)) * (long)TT;   // tile (I+1, I), transposed
 #pragma unroll
-        for (int k = 0; k < 16; ++k) ln[k] = lnp[col * T + k0 + k];
+        for (int k = 0; k < 16; ++k) ln[k] = lnp[(k0 + k) * T + col];
     }
     if (J1 > I + 1) {
-        const R* l = tiles + (J1 * (w + 1) + (I - J1 + w)) * (long)TT;
+        const R* l = tiles_t + (J1 * (w + 1) + (I - J1 + w)) * (long)TT;
 #pragma unroll
-        for (int k = 0; k < 16; ++k) lc[k] = l[col * T + k0 + k];
+        for (int k = 0; k < 16; ++k) lc[k] = l[(k0 + k) * T + col];
     }
     double acc = (q == 0) ? y[I * T + col] / piv[I * T + col] : 0.0;
     for (long J = J1; J > I + 1; --J) {
         R nx[16];
         const bool more = J - 1 > I + 1;
         if (more) {
-            const R* l = tiles + ((J - 1) * (w + 1) + (I - (J - 1) + w)) * (long)TT;
+            const R* l = tiles_t + ((J - 1) * (w + 1) + (I - (J - 1) + w)) * (long)TT;
 #pragma unroll
-            for (int k = 0; k < 16; ++k) nx[k] = l[col * T + k0 + k];
+            for (int k = 0; k < 16; ++k) nx[k] = l[(k0 + k) * T + col];
         }
         fetch_row(x, J, vin, spin_limit, err);
 #pragma unroll
@@ -1102,6 +1113,7 @@ void BandLDLT<R>::setup(long n_, long bandwidth, hipStream_t s_) {
     w = int((std::max(bandwidth, 1L) + T - 1) / T);
     if (w > nt - 1) w = int(std::max(nt - 1, 0L));
     tiles = dalloc<R>(size_t(nt) * size_t(w + 1) * TT);
+    tiles_t = dalloc<R>(size_t(nt) * size_t(w + 1) * TT);
     linv = dalloc<R>(size_t(nt) * TT);
     piv = dalloc<double>(size_t(nt) * T);
     xw = dalloc<double>(size_t(nt) * T);
@@ -1117,10 +1129,10 @@ void BandLDLT<R>::setup(long n_, long bandwidth, hipStream_t s_) {
 
 template <class R>
 void BandLDLT<R>::release() {
-    void* ps[] = {tiles, linv, piv, xw, yw, flag, pflags};
+    void* ps[] = {tiles, tiles_t, linv, piv, xw, yw, flag, pflags};
     for (void* p : ps)
         if (p) (void)hipFree(p);
-    tiles = nullptr; linv = nullptr; piv = nullptr; xw = nullptr; yw = nullptr; flag = nullptr;
+    tiles = nullptr; tiles_t = nullptr; linv = nullptr; piv = nullptr; xw = nullptr; yw = nullptr; flag = nullptr;
     pflags = nullptr;
     clock.release();
 }
@@ -1153,8 +1165,8 @@ int BandLDLT<R>::factor(const Dev& d) {
                 for (long rr = 1; rr < m; ++rr) grid += (rr + 2) / 2;
             }
             cepoch += 1;
-            hipLaunchKernelGGL((k_band_column<R>), dim3(int(grid)), dim3(NTH), 0, s, tiles, linv, piv, K, w, nt,
-                               pflags, cepoch, spin_limit, flag + 0);
+            hipLaunchKernelGGL((k_band_column<R>), dim3(int(grid)), dim3(NTH), 0, s, tiles, tiles_t, linv, piv, K, w,
+                               nt, pflags, cepoch, spin_limit, flag + 0);
         }
         if (tk_column) clock.end(s);
     } else {
@@ -1167,7 +1179,8 @@ int BandLDLT<R>::factor(const Dev& d) {
             if (m > 0) {
                 const bool tk_panel = clock.take(KT_PANEL);
                 if (tk_panel) clock.begin(KT_PANEL, s);
-                hipLaunchKernelGGL((k_tile_panel<R>), dim3(int(m)), dim3(NTH), 0, s, tiles, linv, piv, Kt, w);
+                hipLaunchKernelGGL((k_tile_panel<R>), dim3(int(m)), dim3(NTH), 0, s, tiles, tiles_t, linv, piv, Kt,
+                                   w);
                 if (tk_panel) clock.end(s);
                 const bool tk_update = clock.take(KT_UPDATE);
                 if (tk_update) clock.begin(KT_UPDATE, s);
@@ -1202,7 +1215,7 @@ void BandLDLT<R>::solve(const double* rhs, double* out) {
     if (tk_fwd) clock.end(s);
     const bool tk_bwd = clock.take(KT_BWD);
     if (tk_bwd) clock.begin(KT_BWD, s);
-    hipLaunchKernelGGL((k_bwd_sweep<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, yw, piv, xw, nt, w,
+    hipLaunchKernelGGL((k_bwd_sweep<R>), dim3(int(nt)), dim3(256), 0, s, tiles_t, linv, yw, piv, xw, nt, w,
                        spin_limit, flag);
     if (tk_bwd) clock.end(s);
     CB_CHECK(hipGetLastError());
