@@ -95,10 +95,14 @@ struct BandBatch {
     double* linv = nullptr;
     double* piv = nullptr;
     double* yw = nullptr;           // forward-sweep values
-    int* flag = nullptr;            // 2: an entry outside the band
+    int* flag = nullptr;            // 1: a factor wait exceeded its bound; 2: an entry outside the band
     int* flag_host = nullptr;       // pinned copy, valid after the stream syncs
     BandSeg* segs = nullptr;        // device copy
-    int* order = nullptr;           // workgroup -> model, most work first
+    int* order = nullptr;           // factor workgroup -> model: pairs (main, helper) first, most work first
+    int* role = nullptr;            // factor workgroup -> single, main or helper
+    int* sorder = nullptr;          // solve workgroup -> model, most work first
+    int* prog = nullptr;            // progress counters of the paired models
+    int nwg = 0;                    // factor workgroups
     int32_t* gaps = nullptr;        // equations that no DOF slot maps to
     long ngap = 0;
     std::vector<BandSeg> segs_host;

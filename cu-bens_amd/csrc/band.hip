@@ -863,16 +863,65 @@ __global__ void k_band_gap_pad(double* __restrict__ tiles, const int32_t* __rest
 // starts.  Wider columns take the operands of the updates from memory.  The
 // barrier after each task orders its LDS and memory writes before the next
 // task reads them.
+//
+// A model with two tiles below the diagonal may instead be factored by a pair
+// of workgroups (role ROLE_MAIN and ROLE_HELPER).  The main workgroup keeps the
+// critical chain D_K -> P_{K+1,K} -> U_{K+1,K+1,K} -> D_{K+1}; the helper
+// computes P_{K+2,K}, U_{K+2,K+1,K} and U_{K+2,K+2,K} meanwhile.  They hand
+// over through memory with system-scope stores and loads and four progress
+// counters per model (the main's D_K and P_{K+1,K}, the helper's two
+// updates), each raised after the writer's stores have completed.  The main
+// reads A_{K+1,K} after the helper's U_{K+1,K,K-1} and A_{K+1,K+1} after its
+// U_{K+1,K+1,K-1}; the helper reads Linv_K, D_K and P_{K+1,K} after the main
+// has published them.  Every task keeps its arithmetic, so the factor is the
+// one a single workgroup computes.  The pairs come first in the grid, so both
+// workgroups of a pair are resident together; every wait is bounded and ends
+// early once any wait has failed (error bit 1).
+constexpr int ROLE_SINGLE = 0, ROLE_MAIN = 1, ROLE_HELPER = 2;
+enum { C_DIAG = 0, C_P1 = 1, C_U21 = 2, C_U22 = 3 };
+
+// thread 0 waits until *c >= target (or an error is raised anywhere)
+__device__ inline bool wait_count(const int* c, int target, long spin_limit, int* err, int* ok_s) {
+    if (threadIdx.x == 0) {
+        long n = 0;
+        bool good = true;
+        while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) < target) {
+            if (++n >= spin_limit) { good = false; break; }
+            if ((n & 63) == 0 && (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) & 1)) {
+                good = false;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        if (!good) atomicOr(err, 1);
+        *ok_s = good;
+    }
+    __syncthreads();
+    return *ok_s != 0;
+}
+
+// *c = value once every thread's stores have completed
+__device__ inline void publish(int* c, int value) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(c, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ void __launch_bounds__(NTH) k_band_factor_models(double* __restrict__ tiles, double* __restrict__ linv,
                                                             double* __restrict__ piv, int w,
                                                             const BandSeg* __restrict__ segs,
-                                                            const int* __restrict__ order) {
+                                                            const int* __restrict__ order,
+                                                            const int* __restrict__ role, int* prog,
+                                                            long spin_limit, int* err) {
     __shared__ double xs[T][T];
     __shared__ double ys[TT];
     __shared__ double lk[T][T];   // Linv_K as the panels' second operand (lk[k][row] = Linv(row, k))
     __shared__ double dd[T];
     __shared__ DiagScratch<double> sm;
-    const BandSeg g = segs[order[blockIdx.x]];
+    __shared__ int ok_s;
+    const int mdl = order[blockIdx.x], rl = role[blockIdx.x];
+    const BandSeg g = segs[mdl];
+    int* pc = prog + long(mdl) * 4 * FLAG_STRIDE;   // the model's progress counters
     double(*yt)[T] = reinterpret_cast<double(*)[T]>(ys);
     const long Kend = g.t0 + g.nt;
     bool staged = false;   // xs already holds A_KK
@@ -880,6 +929,39 @@ __global__ void __launch_bounds__(NTH) k_band_factor_models(double* __restrict__
     auto scale_into_xs = [&](const double (*src)[T], const double* dk) {
         for (int idx = threadIdx.x; idx < TT; idx += NTH) xs[idx / T][idx % T] = src[idx / T][idx % T] * dk[idx / T];
     };
+    if (rl == ROLE_HELPER) {
+        // P_{K+2,K}, U_{K+2,K+1,K}, U_{K+2,K+2,K} for the columns with two tiles below
+        for (long K = g.t0; K + 2 < Kend; ++K) {
+            const int rel = int(K - g.t0);
+            if (!wait_count(pc + C_DIAG * FLAG_STRIDE, rel + 1, spin_limit, err, &ok_s)) return;
+            stage<true>(lk, linv + K * (long)TT);
+            if (threadIdx.x < T) dd[threadIdx.x] = cld<true>(&piv[K * T + threadIdx.x]);
+            double* p2 = tiles + tile_index(K + 2, K, w) * TT;
+            stage<false>(xs, p2);
+            __syncthreads();
+            TileAcc<double> acc;
+            acc.product(xs, lk);
+            acc.each([&](int row, int col, double v) {
+                const double p = v * (1.0 / dd[col]);
+                p2[col * T + row] = p;
+                yt[col][row] = p;
+            });
+            __syncthreads();
+            if (!wait_count(pc + C_P1 * FLAG_STRIDE, rel + 1, spin_limit, err, &ok_s)) return;
+            stage<true>(lk, tiles + tile_index(K + 1, K, w) * TT);   // P_{K+1,K}
+            scale_into_xs(yt, dd);
+            __syncthreads();
+            acc.product(xs, lk);
+            double* o21 = tiles + tile_index(K + 2, K + 1, w) * TT;
+            acc.each([&](int row, int col, double v) { cst<true>(&o21[col * T + row], o21[col * T + row] - v); });
+            publish(pc + C_U21 * FLAG_STRIDE, rel + 1);
+            acc.product(xs, yt);
+            double* o22 = tiles + tile_index(K + 2, K + 2, w) * TT;
+            acc.each([&](int row, int col, double v) { cst<true>(&o22[col * T + row], o22[col * T + row] - v); });
+            publish(pc + C_U22 * FLAG_STRIDE, rel + 1);
+        }
+        return;
+    }
     for (long K = g.t0; K < Kend; ++K) {
         if (!staged) {
             stage<false>(xs, tiles + tile_index(K, K, w) * TT);
@@ -890,6 +972,43 @@ __global__ void __launch_bounds__(NTH) k_band_factor_models(double* __restrict__
         __syncthreads();
         const long m = (Kend - 1 - K < g.w) ? Kend - 1 - K : g.w;
         const double* dk = piv + K * T;
+        if (rl == ROLE_MAIN) {
+            const int rel = int(K - g.t0);
+            if (m == 2) {
+                // Linv_K and D_K for the helper, stored again through to memory
+                for (int idx = threadIdx.x; idx < TT; idx += NTH)
+                    cst<true>(&linv[K * (long)TT + idx], lk[idx / T][idx % T]);
+                if (threadIdx.x < T) cst<true>(&piv[K * T + threadIdx.x], piv[K * T + threadIdx.x]);
+                publish(pc + C_DIAG * FLAG_STRIDE, rel + 1);
+            }
+            if (m >= 1) {
+                // P_{K+1,K}, after the helper's U_{K+1,K,K-1}
+                if (K > g.t0 && !wait_count(pc + C_U21 * FLAG_STRIDE, rel, spin_limit, err, &ok_s)) return;
+                double* p1 = tiles + tile_index(K + 1, K, w) * TT;
+                stage<true>(xs, p1);
+                __syncthreads();
+                TileAcc<double> acc;
+                acc.product(xs, lk);
+                acc.each([&](int row, int col, double v) {
+                    const double p = v * (1.0 / dk[col]);
+                    cst<true>(&p1[col * T + row], p);
+                    yt[col][row] = p;
+                });
+                if (m == 2) publish(pc + C_P1 * FLAG_STRIDE, rel + 1);
+                else __syncthreads();
+                // U_{K+1,K+1,K}, after the helper's U_{K+1,K+1,K-1}; left in xs for D_{K+1}
+                if (K > g.t0 && !wait_count(pc + C_U22 * FLAG_STRIDE, rel, spin_limit, err, &ok_s)) return;
+                scale_into_xs(yt, dk);
+                __syncthreads();
+                acc.product(xs, yt);
+                __syncthreads();
+                const double* a11 = tiles + tile_index(K + 1, K + 1, w) * TT;
+                acc.each([&](int row, int col, double v) { xs[col][row] = cld<true>(&a11[col * T + row]) - v; });
+                __syncthreads();
+                staged = true;
+            }
+            continue;
+        }
         const bool in_lds = m <= 2;
         for (long r = 1; r <= m; ++r) {
             // P_{K+r,K} as panel_task computes it, with Linv_K from LDS
@@ -1243,12 +1362,33 @@ void BandBatch::setup(const std::vector<BandSeg>& sg, long n_total, int w_band, 
     for (const BandSeg& g : sg)
         if (g.t0 < 0 || g.t0 + g.nt > nt || g.w > w || g.w < 0 || g.neq > g.nt * T)
             throw DeviceError("band batch: a model's tile range does not fit the band");
-    // workgroups in decreasing order of the model's tile work, so the longest
-    // factorizations start first
-    std::vector<int> ord(sg.size());
-    for (size_t i = 0; i < ord.size(); ++i) ord[i] = int(i);
+    // models in decreasing order of their tile work, so the longest
+    // factorizations start first; the models with two tiles below the
+    // diagonal (as many as fit with their helpers into the first workgroups
+    // the GPU holds at once) get a helper workgroup and come first, in pairs
+    std::vector<int> mo(sg.size());
+    for (size_t i = 0; i < mo.size(); ++i) mo[i] = int(i);
     auto work = [&](int i) { return double(sg[i].nt) * double((sg[i].w + 1) * (sg[i].w + 2) / 2); };
-    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return work(a) > work(b); });
+    std::stable_sort(mo.begin(), mo.end(), [&](int a, int b) { return work(a) > work(b); });
+    std::vector<int> ord, rol;
+    std::vector<char> paired(sg.size(), 0);
+    constexpr int MAX_PAIRS = 96;   // 192 workgroups, fewer than the CUs (one factor workgroup per CU)
+    int npair = 0;
+    for (int i : mo)
+        if (sg[size_t(i)].w == 2 && sg[size_t(i)].nt >= 3 && npair < MAX_PAIRS) {
+            ord.push_back(i);
+            rol.push_back(ROLE_MAIN);
+            ord.push_back(i);
+            rol.push_back(ROLE_HELPER);
+            paired[size_t(i)] = 1;
+            ++npair;
+        }
+    for (int i : mo)
+        if (!paired[size_t(i)]) {
+            ord.push_back(i);
+            rol.push_back(ROLE_SINGLE);
+        }
+    nwg = int(ord.size());
     tiles = dalloc<double>(size_t(nt) * size_t(w + 1) * TT);
     linv = dalloc<double>(size_t(nt) * TT);
     piv = dalloc<double>(size_t(nt) * T);
@@ -1258,10 +1398,15 @@ void BandBatch::setup(const std::vector<BandSeg>& sg, long n_total, int w_band, 
     *flag_host = 0;
     segs = dalloc<BandSeg>(sg.size());
     order = dalloc<int>(ord.size());
+    role = dalloc<int>(rol.size());
+    sorder = dalloc<int>(mo.size());
+    prog = dalloc<int>(sg.size() * 4 * FLAG_STRIDE);
     ngap = long(gap_eqs.size());
     gaps = dalloc<int32_t>(size_t(ngap));
     h2d(segs, sg.data(), sg.size(), s);
     h2d(order, ord.data(), ord.size(), s);
+    h2d(role, rol.data(), rol.size(), s);
+    h2d(sorder, mo.data(), mo.size(), s);
     h2d(gaps, gap_eqs.data(), gap_eqs.size(), s);
     CB_CHECK(hipMemsetAsync(piv, 0, sizeof(double) * size_t(nt) * T, s));
     for (hipEvent_t& e : ev) CB_CHECK(hipEventCreate(&e));
@@ -1274,7 +1419,7 @@ void BandBatch::release() {
             (void)hipEventDestroy(e);
             e = nullptr;
         }
-    void* ps[] = {tiles, linv, piv, yw, flag, segs, order, gaps};
+    void* ps[] = {tiles, linv, piv, yw, flag, segs, order, role, sorder, prog, gaps};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (flag_host) (void)hipHostFree(flag_host);
@@ -1283,6 +1428,9 @@ void BandBatch::release() {
     flag_host = nullptr;
     segs = nullptr;
     order = nullptr;
+    role = nullptr;
+    sorder = nullptr;
+    prog = nullptr;
     gaps = nullptr;
 }
 
@@ -1294,8 +1442,11 @@ void BandBatch::factor(const Dev& d) {
                        d.rowptr, d.colidx, d.eq_of_slot, d.nj, d.bs, w, flag);
     if (ngap)
         hipLaunchKernelGGL(k_band_gap_pad, dim3(int((ngap + 255) / 256)), dim3(256), 0, s, tiles, gaps, ngap, w);
+    CB_CHECK(hipMemsetAsync(prog, 0, sizeof(int) * size_t(nseg) * 4 * FLAG_STRIDE, s));
+    const long spin_limit = 1L << 22;
     if (timing) CB_CHECK(hipEventRecord(ev[0], s));
-    hipLaunchKernelGGL(k_band_factor_models, dim3(nseg), dim3(NTH), 0, s, tiles, linv, piv, w, segs, order);
+    hipLaunchKernelGGL(k_band_factor_models, dim3(nwg), dim3(NTH), 0, s, tiles, linv, piv, w, segs, order, role, prog,
+                       spin_limit, flag);
     if (timing) CB_CHECK(hipEventRecord(ev[1], s));
     CB_CHECK(hipGetLastError());
     CB_CHECK(hipMemcpyAsync(flag_host, flag, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1304,7 +1455,7 @@ void BandBatch::factor(const Dev& d) {
 void BandBatch::solve(const double* rhs, double* out) {
     if (!nseg) return;
     if (timing) CB_CHECK(hipEventRecord(ev[2], s));
-    hipLaunchKernelGGL(k_band_solve_models, dim3(nseg), dim3(256), 0, s, tiles, linv, piv, w, segs, order, rhs, yw,
+    hipLaunchKernelGGL(k_band_solve_models, dim3(nseg), dim3(256), 0, s, tiles, linv, piv, w, segs, sorder, rhs, yw,
                        out);
     if (timing) CB_CHECK(hipEventRecord(ev[3], s));
     CB_CHECK(hipGetLastError());

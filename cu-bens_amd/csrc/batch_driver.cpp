@@ -462,6 +462,7 @@ void Batch::run(int max_steps, bool keep_path, BatchStats& st, bool timing) {
         copy_gen(d_, G_IP, G_I, true, false);
         d2h(sums_host_, sums_dev_, nb * 6, d_.s);
         CB_CHECK(hipStreamSynchronize(d_.s));
+        if (*band_.flag_host & 1) throw DeviceError("batched band factorization: a wait exceeded its bound");
         if (*band_.flag_host & 2) throw DeviceError("batched band factorization: an entry outside a model's band");
         if (timing) {
             float a = 0, c = 0;

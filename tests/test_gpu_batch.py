@@ -2,15 +2,17 @@
 side by side on the GPU, through the C ABI (cb_batch_*).
 
 - A mixed batch: towers and lattice girders of 60-300 members, without and
-  with numerical dissipation, and models with itemax 1, whose Newton loops
+  with numerical dissipation, girders whose band has two tiles below the
+  diagonal (factored by a pair of workgroups), and models with itemax 1, whose Newton loops
   fail and halve the time step (main.c:3826-3846) while the other models of
   the batch step on.  Every model's path is the path of the same model run
   alone (cb_run): identical iteration counts, displacements to 1e-12
   relative; and the oracle's path to 1e-9 with identical iteration counts.
 - The 512-frame workload (200-5000 members each, about 2.66 million
   equations in all) for 20 time steps: every model runs to the step limit
-  with finite displacements, and the largest and smallest models match their
-  single runs to 1e-12 with identical iteration counts.
+  with finite displacements, and the largest and smallest models and the
+  largest model with a half-bandwidth above 64 match their single runs to
+  1e-12 with identical iteration counts.
 """
 import os
 import tempfile
@@ -25,6 +27,8 @@ from helpers import model_file, relerr
 
 pytestmark = pytest.mark.gpu
 
+# the last two are girders with a half-bandwidth above 64 (two tiles below the
+# diagonal), which the batch factors with a pair of workgroups
 MIXED = [
     dict(n=60, kind="tower", amp=8e4),
     dict(n=150, kind="grid", amp=8e4, itemax=1),
@@ -32,6 +36,8 @@ MIXED = [
     dict(n=90, kind="grid", amp=1e6),
     dict(n=200, kind="tower", amp=1e6, itemax=1),
     dict(n=120, kind="grid", amp=3e5, numopt=2, spectrds=0.7),
+    dict(n=150, kind="grid", amp=3e5, instance=114),
+    dict(n=150, kind="grid", amp=8e4, itemax=1, instance=104),
 ]
 
 
@@ -40,7 +46,8 @@ def _mixed_files(steps):
     for i, kw in enumerate(MIXED):
         kw = dict(kw)
         n = kw.pop("n")
-        txt = inputs.space_frame(n, instance=100 + i, steps=steps, **kw).text
+        inst = kw.pop("instance", 100 + i)
+        txt = inputs.space_frame(n, instance=inst, steps=steps, **kw).text
         paths.append(model_file(f"batch_{i}.txt", txt))
     return paths
 
@@ -74,6 +81,7 @@ def test_mixed_batch_matches_single_runs_and_oracle():
             assert np.array_equal(rb["itecnt"], ro["itecnt"])
             assert relerr(rb["d"], ro["d"]) < 1e-9
         assert halving >= 2, "the batch should hold models that halve their time step"
+        assert sum(inf.bandwidth > 64 for inf in b.info) >= 2, "the batch should hold two-tile-band models"
         assert st["newton"] == sum(O.OracleModel(p).run(steps)["n_newton"] for p in paths)
 
 
@@ -102,7 +110,9 @@ def test_frame_workload_batch_512_models():
         print(f"512 frames, {nfr.sum()} members, {neq.sum()} equations: {st['rounds']} rounds, "
               f"{st['newton']} Newton iterations in {st['t_total']:.2f} s")
         assert st["failed"] == 0 and np.all(st["nsteps"] == steps), st
-        for i in (int(np.argmax(neq)), int(np.argmin(neq))):
+        bw = np.array([inf.bandwidth for inf in b.info])
+        wide = np.flatnonzero(bw > 64)
+        for i in (int(np.argmax(neq)), int(np.argmin(neq)), int(wide[np.argmax(neq[wide])])):
             rb = b.path(i)
             assert np.all(np.isfinite(rb["d"]))
             rs = _single(paths[i], steps)
