@@ -116,6 +116,11 @@ struct BandBatch {
     void release();
     // Copies the lower band of d.K and factors every model; enqueued only.
     void factor(const Dev& d);
+    // The same in two parts, for a caller that writes the band itself:
+    // zero() clears the tiles and the error bits; factor_filled() adds the
+    // identity rows of the unused equations and factors.
+    void zero();
+    void factor_filled();
     // out = (L D L^T)^{-1} rhs for every model (device vectors of n, no alias).
     void solve(const double* rhs, double* out);
     size_t bytes() const { return sizeof(double) * size_t(nt) * size_t(w + 1) * 64 * 64; }

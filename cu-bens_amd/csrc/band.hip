@@ -1434,12 +1434,21 @@ void BandBatch::release() {
     gaps = nullptr;
 }
 
-void BandBatch::factor(const Dev& d) {
-    if (!nseg) return;
+void BandBatch::zero() {
     CB_CHECK(hipMemsetAsync(tiles, 0, sizeof(double) * size_t(nt) * (w + 1) * TT, s));
     CB_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
+}
+
+void BandBatch::factor(const Dev& d) {
+    if (!nseg) return;
+    zero();
     hipLaunchKernelGGL((k_band_from_bsr<double>), dim3(int((d.nslot + 255) / 256)), dim3(256), 0, s, tiles, d.K,
                        d.rowptr, d.colidx, d.eq_of_slot, d.nj, d.bs, w, flag);
+    factor_filled();
+}
+
+void BandBatch::factor_filled() {
+    if (!nseg) return;
     if (ngap)
         hipLaunchKernelGGL(k_band_gap_pad, dim3(int((ngap + 255) / 256)), dim3(256), 0, s, tiles, gaps, ngap, w);
     CB_CHECK(hipMemsetAsync(prog, 0, sizeof(int) * size_t(nseg) * 4 * FLAG_STRIDE, s));

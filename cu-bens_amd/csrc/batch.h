@@ -52,6 +52,9 @@ void batch_masked_copy(hipStream_t s, const CopyJob* jobs_dev, int njobs, long n
 constexpr int NM_MODEL_OP = -1;
 void batch_nm_vec(hipStream_t s, int op, const NewmarkArgs& a, const NmParam* prm, const int32_t* own_eq, long n);
 void batch_add_mass_diag(Dev& d, const double* sm, const NmParam* prm, const int32_t* own_joint);
+// the same on the diagonal of band tiles of width 64 (w tiles below the diagonal)
+void batch_add_mass_band(Dev& d, double* tiles, int w, const double* sm, const NmParam* prm,
+                         const int32_t* own_joint);
 // per model: sum dd^2, dt^2, (dyn - f)^2, (dyn - fp)^2, dd.(dyn - fip), dd.(dyn - fp)
 void batch_test_sums(hipStream_t s, const BatchSeg* segs, int nmod, const double* dd, const double* dt,
                      const double* dyn, const double* f, const double* fp, const double* fip, double* out);

@@ -59,6 +59,20 @@ __global__ void k_add_mass_diag_models(double* __restrict__ K, const int32_t* __
     K[(long)diagblk[j] * bs * bs + k * bs + k] += p.cnum * sm[q] / p.cden;
 }
 
+// the same on the band diagonal: tile (I, I) entry (q % 64, q % 64) of every
+// free equation q
+__global__ void k_add_mass_band_models(double* __restrict__ tiles, int w, const int32_t* __restrict__ eq,
+                                       const double* __restrict__ sm, const NmParam* __restrict__ prm,
+                                       const int32_t* __restrict__ own_joint, long nslot, int bs) {
+    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (s >= nslot) return;
+    const int32_t q = eq[s];
+    if (q < 0) return;
+    const NmParam& p = prm[own_joint[s / bs]];
+    const long I = q / 64;
+    tiles[(I * (w + 1) + w) * 4096 + (q % 64) * 64 + (q % 64)] += p.cnum * sm[q] / p.cden;
+}
+
 __device__ inline double wave_sum64(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -135,6 +149,13 @@ void batch_nm_vec(hipStream_t s, int op, const NewmarkArgs& a, const NmParam* pr
 void batch_add_mass_diag(Dev& d, const double* sm, const NmParam* prm, const int32_t* own_joint) {
     hipLaunchKernelGGL(k_add_mass_diag_models, dim3(nblk(d.nslot)), dim3(TPB), 0, d.s, d.K, d.diagblk, d.eq_of_slot,
                        sm, prm, own_joint, d.nslot, d.bs);
+    CB_CHECK(hipGetLastError());
+}
+
+void batch_add_mass_band(Dev& d, double* tiles, int w, const double* sm, const NmParam* prm,
+                         const int32_t* own_joint) {
+    hipLaunchKernelGGL(k_add_mass_band_models, dim3(nblk(d.nslot)), dim3(TPB), 0, d.s, tiles, w, d.eq_of_slot, sm, prm,
+                       own_joint, d.nslot, d.bs);
     CB_CHECK(hipGetLastError());
 }
 

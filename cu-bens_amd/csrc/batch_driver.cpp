@@ -445,11 +445,14 @@ void Batch::run(int max_steps, bool keep_path, BatchStats& st, bool timing) {
         if (any_start) batch_masked_copy(d_.s, start_jobs_, nstart_, nmax, own_eq_, own_joint_, own_el_, prm_dev_, 1);
         // one Newton iteration of every model (Analysis::run_newmark's L4 body)
         batch_nm_vec(d_.s, NM_MODEL_OP, A, prm_dev_, own_eq_, n_);   // predictor or corrector
-        launch_tangent(d_, G_I, G_IP, false);
+        // the tangent goes straight into the band (the copy the single-model
+        // solver makes from the joint blocks), then Keff's mass term
+        band_.zero();
+        launch_frame_tangent_band(d_, G_IP, band_.tiles, band_.w, band_.flag);
         vfill(d_, sm, 0.0, n_);
         launch_mass(d_, sm, 1);
-        batch_add_mass_diag(d_, sm, prm_dev_, own_joint_);
-        band_.factor(d_);
+        batch_add_mass_band(d_, band_.tiles, band_.w, sm, prm_dev_, own_joint_);
+        band_.factor_filled();
         A.out = tmp;
         batch_nm_vec(d_.s, NM_REFF, A, prm_dev_, own_eq_, n_);
         band_.solve(tmp, dd);

@@ -109,6 +109,10 @@ void launch_frame_tangent(Dev& d, int g);
 // frame-only models: K <- tangent of generation g, one thread per joint block
 // summing its members' contributions in colour order, fixed slots as identity
 void launch_frame_tangent_gather(Dev& d, int g);
+// frame-only models: the same tangent written straight into band tiles of
+// tile width 64 (free entries on or below the diagonal; the tiles must be zero
+// elsewhere), w tiles below the diagonal; *bad = 2 for an entry outside
+void launch_frame_tangent_band(Dev& d, int g, double* tiles, int w, int* bad);
 void launch_frame_forces(Dev& d, double* f, const double* dd);
 void launch_frame_update(Dev& d, int gx);
 

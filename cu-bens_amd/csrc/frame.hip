@@ -215,6 +215,12 @@ __device__ __forceinline__ void frame_half_add(double acc[3][6], const FrameCoef
     }
 }
 
+// TO_BAND: instead of the joint block, the entries of free rows and columns on
+// or below the diagonal go straight into the band tiles (the copy
+// BandLDLT / BandBatch make from K, with the same values): tile (I, J) of the
+// equation pair (i, j) is tiles[(I (w + 1) + J - I + w) T^2], entry (j % T, i % T)
+// column-major; an entry more than w tiles below the diagonal sets *bad = 2.
+template <bool TO_BAND>
 __global__ void __launch_bounds__(TPB) k_frame_tangent_gather(
     const int32_t* __restrict__ gptr, const int32_t* __restrict__ gitem, const int32_t* __restrict__ blkrow,
     const int32_t* __restrict__ colidx, const int32_t* __restrict__ eq, long nnzb,
@@ -222,7 +228,8 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent_gather(
     const double* __restrict__ emod, const double* __restrict__ gmod, const double* __restrict__ carea,
     const double* __restrict__ llen, const double* __restrict__ istrong, const double* __restrict__ iweak,
     const double* __restrict__ ipolar, const double* __restrict__ iwarp,
-    const double* __restrict__ defllen, const double* __restrict__ ef, double* __restrict__ K, int geometric) {
+    const double* __restrict__ defllen, const double* __restrict__ ef, double* __restrict__ K, int geometric,
+    int w, int* __restrict__ bad) {
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     const long kb = t >> 1;
     const int u = int(t & 1);
@@ -252,14 +259,34 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent_gather(
         }
     }
     const long A = blkrow[kb], B = colidx[kb];
-    double* Kb = K + kb * 36 + u * 18;
+    if constexpr (TO_BAND) {
+        constexpr long BT = 64;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const bool rowfix = eq[A * 6 + u * 3 + i] < 0;
+        for (int i = 0; i < 3; ++i) {
+            const int32_t qi = eq[A * 6 + u * 3 + i];
+            if (qi < 0) continue;
 #pragma unroll
-        for (int j = 0; j < 6; ++j) {
-            const bool colfix = eq[B * 6 + j] < 0;
-            Kb[i * 6 + j] = (rowfix || colfix) ? ((A == B && u * 3 + i == j) ? 1.0 : 0.0) : acc[i][j];
+            for (int j = 0; j < 6; ++j) {
+                const int32_t qj = eq[B * 6 + j];
+                if (qj < 0 || qj > qi) continue;
+                const long I = qi / BT, J = qj / BT;
+                if (I - J > w) {
+                    *bad = 2;
+                    continue;
+                }
+                K[(I * (w + 1) + (J - I + w)) * BT * BT + (qj % BT) * BT + (qi % BT)] = acc[i][j];
+            }
+        }
+    } else {
+        double* Kb = K + kb * 36 + u * 18;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const bool rowfix = eq[A * 6 + u * 3 + i] < 0;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                const bool colfix = eq[B * 6 + j] < 0;
+                Kb[i * 6 + j] = (rowfix || colfix) ? ((A == B && u * 3 + i == j) ? 1.0 : 0.0) : acc[i][j];
+            }
         }
     }
 }
@@ -444,10 +471,19 @@ void launch_frame_tangent(Dev& d, int g) {
 
 void launch_frame_tangent_gather(Dev& d, int g) {
     if (d.bs != 6) throw DeviceError("frame tangent gather: frame models carry 6 slots per joint");
-    hipLaunchKernelGGL(k_frame_tangent_gather, dim3(nblk(2 * d.nnzb)), dim3(TPB), 0, d.s, d.gat_ptr_fr, d.gat_item_fr,
-                       d.blkrow, d.colidx, d.eq_of_slot, d.nnzb, d.c1[g], d.c2[g], d.c3[g], d.emod, d.gmod, d.carea,
-                       d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[g], d.ef[g], d.K,
-                       int(d.anaflag == 2));
+    hipLaunchKernelGGL(k_frame_tangent_gather<false>, dim3(nblk(2 * d.nnzb)), dim3(TPB), 0, d.s, d.gat_ptr_fr,
+                       d.gat_item_fr, d.blkrow, d.colidx, d.eq_of_slot, d.nnzb, d.c1[g], d.c2[g], d.c3[g], d.emod,
+                       d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[g], d.ef[g], d.K,
+                       int(d.anaflag == 2), 0, nullptr);
+    CB_CHECK(hipGetLastError());
+}
+
+void launch_frame_tangent_band(Dev& d, int g, double* tiles, int w, int* bad) {
+    if (d.bs != 6 || d.ntr || d.nsh) throw DeviceError("frame tangent into the band: frame-only models");
+    hipLaunchKernelGGL(k_frame_tangent_gather<true>, dim3(nblk(2 * d.nnzb)), dim3(TPB), 0, d.s, d.gat_ptr_fr,
+                       d.gat_item_fr, d.blkrow, d.colidx, d.eq_of_slot, d.nnzb, d.c1[g], d.c2[g], d.c3[g], d.emod,
+                       d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[g], d.ef[g], tiles,
+                       int(d.anaflag == 2), w, bad);
     CB_CHECK(hipGetLastError());
 }
 

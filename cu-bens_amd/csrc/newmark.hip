@@ -74,6 +74,38 @@ __global__ void k_mass_frame(const int32_t* __restrict__ els, int n, const int32
     }
 }
 
+// Frame-only models: the same masses in two passes, every member in one
+// launch (its length, and its translational and rotational masses into fm)...
+__global__ void k_mass_frame_members(long n, const int32_t* __restrict__ minc, const double* __restrict__ x,
+                                     const double* __restrict__ dens, const double* __restrict__ carea,
+                                     double* __restrict__ llen, double* __restrict__ fm) {
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const double* a = x + (minc[e * 2] - 1) * 3;
+    const double* b = x + (minc[e * 2 + 1] - 1) * 3;
+    double el0 = b[0] - a[0], el1 = b[1] - a[1], el2 = b[2] - a[2];
+    const double L = sqrt(el0 * el0 + el1 * el1 + el2 * el2);
+    llen[e] = L;
+    fm[e * 2] = dens[e] * carea[e] * L / 24 * 12;
+    fm[e * 2 + 1] = dens[e] * carea[e] * L / 24 * (L * L);
+}
+
+// ... then one thread per free slot adds the masses of the members at its
+// joint in colour order, the order in which the colour launches add them.
+__global__ void k_mass_frame_gather(const int32_t* __restrict__ jptr, const int32_t* __restrict__ jitem,
+                                    const int32_t* __restrict__ eq, long nslot, const double* __restrict__ fm,
+                                    double* __restrict__ sm) {
+    const long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (r >= nslot) return;
+    const int32_t q = eq[r];
+    if (q < 0) return;
+    const long j = r / 6;
+    const int s = int(r % 6);
+    double acc = sm[q];
+    for (int32_t it = jptr[j]; it < jptr[j + 1]; ++it) acc += fm[long(jitem[it] >> 1) * 2 + (s >= 3)];
+    sm[q] = acc;
+}
+
 // shell: rho A t/3 per translation, (rho A t/3) t^2/12 per rotation
 // (shell.c:1544-1573); overwrites the side lengths and face area
 __global__ void k_mass_shell(const int32_t* __restrict__ els, int n, const int32_t* __restrict__ minc,
@@ -140,6 +172,11 @@ void launch_mass(Dev& d, double* sm, int type) {
                                d.col_el_tr + d.col_ptr_tr[c], n, d.minc, d.mcode, xb, d.dens,
                                d.carea, d.llen, sm);
         }
+    } else if (type == 1 && !d.ntr && !d.nsh && d.bs == 6) {
+        hipLaunchKernelGGL(k_mass_frame_members, dim3(nblk(d.nfr)), dim3(TPB), 0, d.s, d.nfr, d.minc, xb, d.dens,
+                           d.carea, d.llen, d.fh);
+        hipLaunchKernelGGL(k_mass_frame_gather, dim3(nblk(d.nslot)), dim3(TPB), 0, d.s, d.jnt_ptr_fr, d.jnt_item_fr,
+                           d.eq_of_slot, d.nslot, d.fh, sm);
     } else if (type == 1) {
         for (size_t c = 0; c + 1 < d.col_ptr_fr.size(); ++c) {
             int n = d.col_ptr_fr[c + 1] - d.col_ptr_fr[c];
