@@ -757,8 +757,9 @@ __global__ void __launch_bounds__(256) k_fwd_sweep(const R* __restrict__ tiles, 
         fetch_row(y, I - 1, vin, spin_limit, err);
 #pragma unroll
         for (int k = 0; k < 16; ++k) acc -= double(ln[k]) * vin[k0 + k];
-        __syncthreads();
     }
+    // every thread has read vin before it passes this barrier, so vin may be
+    // overwritten after it
     part[q][row] = acc;
     __syncthreads();
     if (tid < T) vin[tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
@@ -820,7 +821,6 @@ __global__ void __launch_bounds__(256) k_bwd_sweep(const R* __restrict__ tiles_t
         fetch_row(x, I + 1, vin, spin_limit, err);
 #pragma unroll
         for (int k = 0; k < 16; ++k) acc -= double(ln[k]) * vin[k0 + k];
-        __syncthreads();
     }
     part[q][col] = acc;
     __syncthreads();
