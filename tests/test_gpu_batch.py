@@ -120,3 +120,28 @@ def test_frame_workload_batch_512_models():
             assert relerr(rb["d"], rs["d"]) < 1e-12
         for i in range(0, 512, 37):
             assert np.all(np.isfinite(b.path(i)["d"]))
+
+
+def test_batch_edge_cases():
+    """A batch of one model equals its single run; zero steps record nothing;
+    a run without the displacement path refuses to return one; a second run
+    of the same batch repeats the first bitwise."""
+    p = model_file("batch_one.txt", inputs.space_frame(80, kind="tower", instance=7, steps=8, amp=1e6).text)
+    with cubens.GpuBatch([p]) as b:
+        st = b.run(max_steps=0)
+        assert st["rounds"] == 0 and np.all(st["nsteps"] == 0)
+        st = b.run(max_steps=8)
+        assert np.all(st["status"] == 0) and np.all(st["nsteps"] == 8)
+        first = b.path(0)
+        rs = _single(p, 8)
+        assert np.array_equal(first["itecnt"], rs["itecnt"]) and np.array_equal(first["d"], rs["d"])
+        again = b.run(max_steps=8)
+        assert again["newton"] == st["newton"]
+        assert np.array_equal(b.path(0)["d"], first["d"])
+        b.run(max_steps=4, keep_d=False)
+        with pytest.raises(cubens.CubensError, match="path"):
+            b.path(0)
+        with pytest.raises(cubens.CubensError, match="range"):
+            lib = cubens.lib()
+            import ctypes as C
+            cubens._check(lib.cb_batch_get_path(b.h, 5, None, None, None))
