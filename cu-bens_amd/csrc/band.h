@@ -82,6 +82,7 @@ struct BandLDLT {
 struct BandSeg {
     long t0 = 0, nt = 0;
     long neq = 0;   // equations of the model (t0 * 64 is its first)
+    long bw = 0;    // half-bandwidth: entries (i, j) with i - j > bw are zero
     int w = 0;
     int pad_ = 0;
 };

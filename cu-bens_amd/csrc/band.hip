@@ -1118,19 +1118,34 @@ __global__ void __launch_bounds__(256) k_band_solve_models(const double* __restr
         if (tid < T) s = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
         return s;
     };
+    // The entries of a model's band more than bw below the diagonal are zero
+    // in A and in L, and Linv is lower triangular: a thread whose 16 entries
+    // of a tile are all such zeros takes zeros instead of reading them (the
+    // products are the same up to the sign of zero).
+    // a[k] = tile entry (r0, c0 + k) of a tile d tiles below the diagonal
+    auto load16 = [&](double* a, const double* l, long d, int r0, int c0, int stride_r, int stride_c) {
+        if (64 * d + r0 - (c0 + 15) > g.bw) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) a[k] = 0.0;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) a[k] = l[(c0 + k) * stride_c + r0 * stride_r];
+        }
+    };
     double lt[RW][16], lv[16];
     for (long I = g.t0; I < Iend; ++I) {
         const long J0 = (I - g.w > g.t0) ? I - g.w : g.t0;
         const int nJ = int(I - J0);
 #pragma unroll
         for (int j = 0; j < RW; ++j)
-            if (j < nJ) {
-                const double* l = tiles + tile_index(I, J0 + j, w) * TT;
+            if (j < nJ) load16(lt[j], tiles + tile_index(I, J0 + j, w) * TT, I - J0 - j, row, k0, 1, T);
+        if (k0 > row) {
 #pragma unroll
-                for (int k = 0; k < 16; ++k) lt[j][k] = l[(k0 + k) * T + row];
-            }
+            for (int k = 0; k < 16; ++k) lv[k] = 0.0;
+        } else {
 #pragma unroll
-        for (int k = 0; k < 16; ++k) lv[k] = linv[I * (long)TT + (k0 + k) * T + row];
+            for (int k = 0; k < 16; ++k) lv[k] = linv[I * (long)TT + (k0 + k) * T + row];
+        }
         double acc = 0.0;
         if (q == 0) {
             const long e = I * T + row;
@@ -1140,10 +1155,8 @@ __global__ void __launch_bounds__(256) k_band_solve_models(const double* __restr
         for (int j = 0; j < RW; ++j)
             if (j < nJ) sub_row(acc, lt[j], y, I, J0 + j);
         for (long J = J0 + RW; J < I; ++J) {
-            const double* l = tiles + tile_index(I, J, w) * TT;
             double a[16];
-#pragma unroll
-            for (int k = 0; k < 16; ++k) a[k] = l[(k0 + k) * T + row];
+            load16(a, tiles + tile_index(I, J, w) * TT, I - J, row, k0, 1, T);
             sub_row(acc, a, y, I, J);
         }
         const double s = finish_row(acc, lv);
@@ -1157,15 +1170,28 @@ __global__ void __launch_bounds__(256) k_band_solve_models(const double* __restr
         const int col = row;
         const long J1 = (I + g.w < Iend - 1) ? I + g.w : Iend - 1;
         const int nJ = int(J1 - I);
+        // tile (J, I) read by columns: entry (k0 + k, col), 64 d + k0 + k - col
+        // below the diagonal
 #pragma unroll
         for (int j = 0; j < RW; ++j)
             if (j < nJ) {
                 const double* l = tiles + tile_index(J1 - j, I, w) * TT;
+                const long d = J1 - j - I;
+                if (64 * d + k0 - col > g.bw) {
 #pragma unroll
-                for (int k = 0; k < 16; ++k) lt[j][k] = l[col * T + k0 + k];
+                    for (int k = 0; k < 16; ++k) lt[j][k] = 0.0;
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) lt[j][k] = l[col * T + k0 + k];
+                }
             }
+        if (k0 + 15 < col) {
 #pragma unroll
-        for (int k = 0; k < 16; ++k) lv[k] = linv[I * (long)TT + col * T + k0 + k];
+            for (int k = 0; k < 16; ++k) lv[k] = 0.0;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) lv[k] = linv[I * (long)TT + col * T + k0 + k];
+        }
         double acc = (q == 0) ? y[I * T + col] / piv[I * T + col] : 0.0;
 #pragma unroll
         for (int j = 0; j < RW; ++j)
@@ -1173,8 +1199,13 @@ __global__ void __launch_bounds__(256) k_band_solve_models(const double* __restr
         for (long J = J1 - RW; J > I; --J) {
             const double* l = tiles + tile_index(J, I, w) * TT;
             double a[16];
+            if (64 * (J - I) + k0 - col > g.bw) {
 #pragma unroll
-            for (int k = 0; k < 16; ++k) a[k] = l[col * T + k0 + k];
+                for (int k = 0; k < 16; ++k) a[k] = 0.0;
+            } else {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) a[k] = l[col * T + k0 + k];
+            }
             sub_row(acc, a, out, I, J);
         }
         const double s = finish_row(acc, lv);

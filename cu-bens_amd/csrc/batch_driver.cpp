@@ -240,6 +240,7 @@ Batch::Batch(const std::vector<std::string>& paths) {
         bseg[b].t0 = qb / TILE;
         bseg[b].nt = ntb;
         bseg[b].neq = m.neq;
+        bseg[b].bw = std::max(m.bandwidth, 0L);
         // the single-model band's tiles below the diagonal (BandLDLT::setup)
         int wb = int((std::max(m.bandwidth, 1L) + TILE - 1) / TILE);
         if (wb > ntb - 1) wb = int(std::max(ntb - 1, 0L));
