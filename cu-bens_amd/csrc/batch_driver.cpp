@@ -490,10 +490,27 @@ void Batch::run(int max_steps, bool keep_path, BatchStats& st, bool timing) {
             const double m = double(std::min<long>(g.w, g.nt - 1 - K));
             fl += t3 + 2 * t3 * m + t3 * m * (m + 1);
         }
-        // two sweeps, each reading the off-diagonal factor tiles and the Linv tiles once
-        long offd = 0;
-        for (long I = 0; I < g.nt; ++I) offd += std::min<long>(g.w, I);
-        by += 2.0 * 8 * TILE * TILE * double(offd + g.nt);
+        // the two sweeps read every off-diagonal factor tile and every Linv
+        // tile once, except the 16-entry chunks that are structurally zero
+        // (beyond the half-bandwidth, above Linv's diagonal)
+        double chunks = 0;
+        for (long d = 1; d <= g.w; ++d) {
+            long fwd = 0, bwd = 0;
+            for (int r = 0; r < TILE; ++r)
+                for (int q = 0; q < 4; ++q) {
+                    fwd += (TILE * d + r - 16 * q - 15 <= g.bw);
+                    bwd += (TILE * d + 16 * q - r <= g.bw);
+                }
+            chunks += double(fwd + bwd) * double(g.nt - d);
+        }
+        long lfwd = 0, lbwd = 0;
+        for (int r = 0; r < TILE; ++r)
+            for (int q = 0; q < 4; ++q) {
+                lfwd += (16 * q <= r);
+                lbwd += (16 * q + 15 >= r);
+            }
+        chunks += double(lfwd + lbwd) * double(g.nt);
+        by += chunks * 16 * 8;
     }
     st.factor_flops = fl;
     st.solve_bytes = by;

@@ -213,8 +213,8 @@ typedef struct {
     int failed;          /* models whose driver stopped with an error     */
     /* with opt->kernel_timing != 0: HIP-event time (ms) of the batched band
      * factor and solve kernels summed over the rounds, and the algorithmic
-     * work of one launch of each (flops; bytes of the factor and Linv tiles
-     * the two sweeps read) */
+     * work of one launch of each (flops; bytes of the factor and Linv tile
+     * entries the two sweeps read, structurally zero chunks left out) */
     double ms_factor, ms_solve;
     double factor_flops, solve_bytes;
 } cb_batch_stats;
