@@ -1403,10 +1403,15 @@ void BandBatch::setup(const std::vector<BandSeg>& sg, long n_total, int w_band, 
     std::stable_sort(mo.begin(), mo.end(), [&](int a, int b) { return work(a) > work(b); });
     std::vector<int> ord, rol;
     std::vector<char> paired(sg.size(), 0);
-    constexpr int MAX_PAIRS = 96;   // 192 workgroups, fewer than the CUs (one factor workgroup per CU)
+    // the factor kernel's LDS allows one workgroup per CU: the pairs stay
+    // within the first CUs - 32 workgroups of the grid
+    int dev = 0, ncu = 0;
+    CB_CHECK(hipGetDevice(&dev));
+    CB_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int max_pairs = std::max(0, std::min(96, (ncu - 32) / 2));
     int npair = 0;
     for (int i : mo)
-        if (sg[size_t(i)].w == 2 && sg[size_t(i)].nt >= 3 && npair < MAX_PAIRS) {
+        if (sg[size_t(i)].w == 2 && sg[size_t(i)].nt >= 3 && npair < max_pairs) {
             ord.push_back(i);
             rol.push_back(ROLE_MAIN);
             ord.push_back(i);
