@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "band.h"
 
@@ -154,6 +155,28 @@ template <> struct TileAcc<double> {
             }
         }
     }
+    // The same product for a lower-triangular Y (Y[col][k] = 0 for k > col):
+    // the k-steps past the last column of the output block column add exact
+    // zeros and are skipped.  Both blocks of wavefront wv lie in block column
+    // wv / 2.
+    __device__ __forceinline__ void product_ylower(const double (*xs)[T], const double (*ys)[T]) {
+        const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        const int nkb = (__builtin_amdgcn_readfirstlane(wv) >> 1) + 1;   // 16-wide k blocks
+        acc[0] = d4{0.0, 0.0, 0.0, 0.0};
+        acc[1] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int kb = 0; kb < nkb; ++kb) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int k = 16 * kb + 4 * t + (l >> 4);
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    const int blk = 2 * wv + b, bi = blk & 3, bj = blk >> 2;
+                    acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ys[k][16 * bj + (l & 15)],
+                                                                 xs[k][16 * bi + (l & 15)], acc[b], 0, 0, 0);
+                }
+            }
+        }
+    }
     static constexpr int N = 8;   // entries per thread
     template <class F> __device__ __forceinline__ void each_i(F&& f) const {
         const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -188,6 +211,7 @@ template <> struct TileAcc<float> {
                 for (int v = 0; v < 2; ++v) acc[u][v] += av[u] * bv[v];
         }
     }
+    __device__ __forceinline__ void product_ylower(const float (*xs)[T], const float (*ys)[T]) { product(xs, ys); }
     static constexpr int N = 8;   // entries per thread
     template <class F> __device__ __forceinline__ void each_i(F&& f) const {
         const int tr = threadIdx.x & 15, tc = threadIdx.x >> 4;
@@ -207,6 +231,52 @@ __device__ __forceinline__ void gather_tile(const R* __restrict__ tile, R* buf) 
     TileAcc<R> pos;
     pos.each_i([&](int row, int col, R, int i) { buf[i] = tile[col * T + row]; });
 }
+
+// C = X Y^T (operands as for TileAcc) for a symmetric C of which only the ten
+// 16x16 blocks on and below the diagonal are wanted: one or two blocks per
+// wavefront, three blocks' work on two of the four SIMDs and two on the
+// others (wavefronts wv and wv + 4 share a SIMD), so the product takes 48
+// MFMA steps per SIMD instead of 64.  Element e of slot s of lane l is
+// C[16 bi_s + l % 16][16 bj_s + l / 16 + 4e], as in TileAcc<double>.
+struct LowerAcc {
+    d4 acc[2];
+    int bi[2], bj[2], n;
+    __device__ __forceinline__ LowerAcc() {
+        const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        // per wavefront: (bi, bj) of slot 0, then of slot 1 (-1: none)
+        constexpr signed char tab[8][4] = {{0, 0, 3, 3}, {1, 1, 2, 2}, {2, 1, -1, -1}, {3, 1, -1, -1},
+                                           {1, 0, -1, -1}, {2, 0, -1, -1}, {3, 0, -1, -1}, {3, 2, -1, -1}};
+        bi[0] = tab[wv][0];
+        bj[0] = tab[wv][1];
+        bi[1] = tab[wv][2];
+        bj[1] = tab[wv][3];
+        n = bi[1] >= 0 ? 2 : 1;
+    }
+    __device__ __forceinline__ void product(const double (*xs)[T], const double (*ys)[T]) {
+        const int l = threadIdx.x & 63;
+        acc[0] = d4{0.0, 0.0, 0.0, 0.0};
+        acc[1] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+        for (int k0 = 0; k0 < T; k0 += 4) {
+            const int k = k0 + (l >> 4);
+            acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ys[k][16 * bj[0] + (l & 15)], xs[k][16 * bi[0] + (l & 15)],
+                                                         acc[0], 0, 0, 0);
+            if (n == 2)
+                acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ys[k][16 * bj[1] + (l & 15)],
+                                                             xs[k][16 * bi[1] + (l & 15)], acc[1], 0, 0, 0);
+        }
+    }
+    // f(row, col, value, i), i = 4 s + e
+    template <class F> __device__ __forceinline__ void each_i(F&& f) const {
+        const int l = threadIdx.x & 63;
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+            if (s < n) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) f(16 * bi[s] + (l & 15), 16 * bj[s] + (l >> 4) + 4 * e, acc[s][e], 4 * s + e);
+            }
+    }
+};
 
 // ---- D_K --------------------------------------------------------------------
 // Both versions take the 64x64 tile S (lower part; the part above the
@@ -583,12 +653,17 @@ __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* _
             return;
         }
         R pre[TileAcc<R>::N];
-        gather_tile(dtile, pre);
+        if constexpr (std::is_same<R, double>::value) {
+            LowerAcc pos;
+            pos.each_i([&](int row, int col, double, int i) { pre[i] = dtile[col * T + row]; });
+        } else {
+            gather_tile(dtile, pre);
+        }
         stage<false>(xs, tiles + tile_index(D, K, w) * TT);
         stage<false>(ys, linv + K * (long)TT);
         raise_flag(flags, w, epoch);   // A_{K+1,K} has been read (includes the barrier)
         TileAcc<R> acc;
-        acc.product(xs, ys);
+        acc.product_ylower(xs, ys);   // Linv_K is lower triangular
         __syncthreads();
         // (L D, L) of L = P_{K+1,K} to LDS for U_{K+1,K+1,K}
         acc.each([&](int row, int col, R v) {
@@ -597,9 +672,18 @@ __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* _
             ys[col][row] = lv;
         });
         __syncthreads();
-        acc.product(xs, ys);
-        __syncthreads();
-        acc.each_i([&](int row, int col, R v, int i) { xs[col][row] = pre[i] - v; });
+        // U_{K+1,K+1,K} is symmetric and the diagonal task reads only its
+        // lower part
+        if constexpr (std::is_same<R, double>::value) {
+            LowerAcc u;
+            u.product(xs, ys);
+            __syncthreads();
+            u.each_i([&](int row, int col, double v, int i) { xs[col][row] = pre[i] - v; });
+        } else {
+            acc.product(xs, ys);
+            __syncthreads();
+            acc.each_i([&](int row, int col, R v, int i) { xs[col][row] = pre[i] - v; });
+        }
         __syncthreads();
         diag_task(xs, &ys[0][0], dk, sm, linv + D * (long)TT, piv, D);
         return;
