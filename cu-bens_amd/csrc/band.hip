@@ -382,9 +382,17 @@ __device__ __forceinline__ d4 mm16(FA A, FB B, d4 acc) {
     return acc;
 }
 
+struct NoHook {
+    __device__ void operator()() const {}
+};
+
+// tail_hook runs once the pivot steps are done (every thread), where few
+// registers are live: a caller issues loads there that it consumes after the
+// task.
+template <class Hook = NoHook>
 __device__ __forceinline__ void diag_task(double (*xs)[T], double* ys, double* dd, DiagScratch<double>& sm,
                                           double* __restrict__ lout, double* __restrict__ piv, long Kt,
-                                          double* lds_out = nullptr) {
+                                          double* lds_out = nullptr, Hook tail_hook = Hook()) {
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, rr = l & 15, lg = l >> 4;
     double(*tsc)[256] = sm.tsc;
     // operand views: L_ik (rows r, cols t), Linv_ij as (t, c) and as (r, t), T slot s as (t, c)
@@ -513,6 +521,7 @@ __device__ __forceinline__ void diag_task(double (*xs)[T], double* ys, double* d
         }
         __syncthreads();
     }
+    tail_hook();
     // the last row of blocks: T_3j completed, Linv_3j = -Linv_33 T_3j
     if (wv == 0) {
         putT(1, mm16(Lb(3, 2), Vb(2, 0), getT(3)));
@@ -1024,7 +1033,7 @@ __global__ void __launch_bounds__(NTH) k_band_factor_models(double* __restrict__
             stage<false>(xs, p2);
             __syncthreads();
             TileAcc<double> acc;
-            acc.product(xs, lk);
+            acc.product_ylower(xs, lk);
             acc.each([&](int row, int col, double v) {
                 const double p = v * (1.0 / dd[col]);
                 p2[col * T + row] = p;
@@ -1047,14 +1056,27 @@ __global__ void __launch_bounds__(NTH) k_band_factor_models(double* __restrict__
         return;
     }
     for (long K = g.t0; K < Kend; ++K) {
+        const long m = (Kend - 1 - K < g.w) ? Kend - 1 - K : g.w;
+        const bool in_lds = m <= 2;
+        // A workgroup that factors its model alone stages A_{K+1,K} after the
+        // diagonal task; the previous column has written it, so it is loaded
+        // into registers (in the staging layout) once the pivot steps are
+        // done, and its latency overlaps the rest of the diagonal task.
+        const bool pre = rl != ROLE_MAIN && in_lds && m >= 1;
+        double pa[TT / NTH];
         if (!staged) {
             stage<false>(xs, tiles + tile_index(K, K, w) * TT);
             __syncthreads();
         }
         staged = false;
-        diag_task(xs, ys, dd, sm, linv + K * (long)TT, piv, K, &lk[0][0]);
+        diag_task(xs, ys, dd, sm, linv + K * (long)TT, piv, K, &lk[0][0], [&] {
+            if (pre) {
+                const double* t1 = tiles + tile_index(K + 1, K, w) * TT;
+#pragma unroll
+                for (int i = 0; i < TT / NTH; ++i) pa[i] = t1[threadIdx.x + NTH * i];
+            }
+        });
         __syncthreads();
-        const long m = (Kend - 1 - K < g.w) ? Kend - 1 - K : g.w;
         const double* dk = piv + K * T;
         if (rl == ROLE_MAIN) {
             const int rel = int(K - g.t0);
@@ -1072,7 +1094,7 @@ __global__ void __launch_bounds__(NTH) k_band_factor_models(double* __restrict__
                 stage<true>(xs, p1);
                 __syncthreads();
                 TileAcc<double> acc;
-                acc.product(xs, lk);
+                acc.product_ylower(xs, lk);
                 acc.each([&](int row, int col, double v) {
                     const double p = v * (1.0 / dk[col]);
                     cst<true>(&p1[col * T + row], p);
@@ -1084,23 +1106,32 @@ __global__ void __launch_bounds__(NTH) k_band_factor_models(double* __restrict__
                 if (K > g.t0 && !wait_count(pc + C_U22 * FLAG_STRIDE, rel, spin_limit, err, &ok_s)) return;
                 scale_into_xs(yt, dk);
                 __syncthreads();
-                acc.product(xs, yt);
+                // symmetric, and the diagonal task reads only the lower part
+                LowerAcc u;
+                u.product(xs, yt);
                 __syncthreads();
                 const double* a11 = tiles + tile_index(K + 1, K + 1, w) * TT;
-                acc.each([&](int row, int col, double v) { xs[col][row] = cld<true>(&a11[col * T + row]) - v; });
+                u.each_i([&](int row, int col, double v, int) { xs[col][row] = cld<true>(&a11[col * T + row]) - v; });
                 __syncthreads();
                 staged = true;
             }
             continue;
         }
-        const bool in_lds = m <= 2;
         for (long r = 1; r <= m; ++r) {
             // P_{K+r,K} as panel_task computes it, with Linv_K from LDS
             double* tile = tiles + tile_index(K + r, K, w) * TT;
-            stage<false>(xs, tile);
+            if (pre && r == 1) {
+#pragma unroll
+                for (int i = 0; i < TT / NTH; ++i) {
+                    const int idx = threadIdx.x + NTH * i;
+                    xs[idx / T][idx % T] = pa[i];
+                }
+            } else {
+                stage<false>(xs, tile);
+            }
             __syncthreads();
             TileAcc<double> acc;
-            acc.product(xs, lk);
+            acc.product_ylower(xs, lk);
             if (in_lds) {
                 __syncthreads();   // lk is free after the last panel's product
                 double(*dst)[T] = (r == 1) ? yt : lk;
@@ -1129,14 +1160,15 @@ __global__ void __launch_bounds__(NTH) k_band_factor_models(double* __restrict__
                 __syncthreads();
             }
             if (m >= 1) {
-                // U_{K+1,K+1,K}, left in xs for D_{K+1}
+                // U_{K+1,K+1,K}, left in xs for D_{K+1}: symmetric, and the
+                // diagonal task reads only the lower part
                 scale_into_xs(yt, dk);
                 __syncthreads();
-                TileAcc<double> acc;
-                acc.product(xs, yt);
+                LowerAcc u;
+                u.product(xs, yt);
                 __syncthreads();
                 const double* a11 = tiles + tile_index(K + 1, K + 1, w) * TT;
-                acc.each([&](int row, int col, double v) { xs[col][row] = a11[col * T + row] - v; });
+                u.each_i([&](int row, int col, double v, int) { xs[col][row] = a11[col * T + row] - v; });
                 __syncthreads();
                 staged = true;
             }
