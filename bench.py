@@ -82,7 +82,21 @@ def kernel_model(info) -> dict:
     }
 
 
-def roofline(info, ktimes: dict) -> dict:
+# HBM bytes per launch of the dominant kernels on the default workloads, from
+# two rocprofv3 counter runs each (FETCH_SIZE, then WRITE_SIZE, KiB per
+# dispatch), scaled by what the same counters read for an elementwise copy of
+# 256 MiB (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000): k_band_column over the
+# interior launches 3000-3099 of the 256x256 panel, k_band_factor_models over
+# the 13 rounds of 5 time steps of the 512 frames.
+MEASURED_TRAFFIC = {
+    "band_column": (8644.92 * 2.0 + 12001.31) * 1024.0,
+    "band_factor_models": (1679014.64 * 2.0 + 3575412.50) * 1024.0,
+}
+TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, KiB per dispatch scaled by a 256 MiB "
+                  "copy (x2.000, x1.000)")
+
+
+def roofline(info, ktimes: dict, default_workload: bool) -> dict:
     name = max(ktimes, key=lambda k: ktimes[k][0])
     ms, launches = ktimes[name]
     bound, work, unit = kernel_model(info)[name]
@@ -91,9 +105,14 @@ def roofline(info, ktimes: dict) -> dict:
         achieved, peak = work / avg_s / 1e9, PEAK_HBM_GBS
     else:
         achieved, peak = work / avg_s / 1e12, PEAK_FP64_TFLOPS
-    return {"kernel": name, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
-            "frac": achieved / peak, "traffic": None, "avg_launch_us": 1e6 * avg_s, "launches": launches,
-            "share_of_kernel_time": ms / sum(v[0] for v in ktimes.values())}
+    traffic = MEASURED_TRAFFIC.get(name) if default_workload else None
+    out = {"kernel": name, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+           "frac": achieved / peak, "traffic": traffic, "avg_launch_us": 1e6 * avg_s, "launches": launches,
+           "share_of_kernel_time": ms / sum(v[0] for v in ktimes.values())}
+    if traffic is not None:
+        out["traffic_unit"] = "B per launch"
+        out["traffic_source"] = TRAFFIC_SOURCE
+    return out
 
 
 def cpu_baseline(path: str, newton: int, increments: int, budget: float) -> dict:
@@ -181,9 +200,13 @@ def run_frames(args) -> None:
     ms, bound, work, unit, peak = fam[name]
     avg_s = ms / 1000.0 / rounds
     achieved = work / avg_s
+    traffic = MEASURED_TRAFFIC.get(name)
     roof = {"kernel": name, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
-            "frac": achieved / peak, "traffic": None, "avg_launch_us": 1e6 * avg_s, "launches": rounds,
+            "frac": achieved / peak, "traffic": traffic, "avg_launch_us": 1e6 * avg_s, "launches": rounds,
             "share_of_run": ms / 1000.0 / r["t_total"]}
+    if traffic is not None:
+        roof["traffic_unit"] = "B per launch"
+        roof["traffic_source"] = TRAFFIC_SOURCE
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline: oracle on a sample of the models, 1 core")
@@ -352,7 +375,9 @@ def main():
         extra["mixed_vs_plain_fp64_ratio"] = m["rate"] / plain["rate"]
         extra["mixed_vs_fp64_rel_disp_err"] = float(
             np.abs(m["d"] - plain["d"]).max() / np.abs(plain["d"]).max())
-    roof = roofline(head["info"], head["ktimes"])
+    # the counters were read on the 256x256 panel's FP64 band with the fused schedule
+    roof = roofline(head["info"], head["ktimes"],
+                    default_workload=(args.n == 256 and plain is not None and args.schedule == "fused"))
     log(f"[bench] kernel times (ms, launches): {head['ktimes']}")
     cpu = None
     if not args.no_cpu:
