@@ -647,7 +647,9 @@ __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* _
     __shared__ R xs[T][T];
     __shared__ R ys[T][T];
     __shared__ double dk[T];
-    __shared__ int ok_s;
+    // one slot per consecutive wait: a slow thread still reads its wait's
+    // result after thread 0 has gone on to the next wait
+    __shared__ int ok_s[3];
     const long m = (K < 0) ? 0 : ((nt - 1 - K < w) ? nt - 1 - K : w);
     const long b = blockIdx.x;
     if (m > 0 && threadIdx.x < T) dk[threadIdx.x] = piv[K * T + threadIdx.x];
@@ -709,7 +711,7 @@ __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* _
         stage<false>(xs, tile);
         stage<false>(ys, linv + K * (long)TT);
         bool ok = true;
-        if (pb == 0) ok = wait_flag_bounded(flags, w, epoch, spin_limit, err, &ok_s);
+        if (pb == 0) ok = wait_flag_bounded(flags, w, epoch, spin_limit, err, &ok_s[0]);
         else __syncthreads();
         TileAcc<R> acc;
         acc.product(xs, ys);
@@ -725,9 +727,9 @@ __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* _
         ++r;
     }
     const long s0 = 2 * t, ns = (s0 + 1 <= r) ? 2 : 1;
-    if (!wait_flag_bounded(flags, r, epoch, spin_limit, err, &ok_s)) return;
+    if (!wait_flag_bounded(flags, r, epoch, spin_limit, err, &ok_s[0])) return;
     for (long j = 0; j < ns; ++j)
-        if (s0 + j != r && !wait_flag_bounded(flags, s0 + j, epoch, spin_limit, err, &ok_s)) return;
+        if (s0 + j != r && !wait_flag_bounded(flags, s0 + j, epoch, spin_limit, err, &ok_s[1 + j])) return;
     const long I = K + 1 + r;
     stage_scaled<true>(xs, tiles + tile_index(I, K, w) * TT, dk);
     for (long j = 0; j < ns; ++j) {
