@@ -238,7 +238,7 @@ def chs_section(r_o: float, E: float = 2e11):
 def space_frame(n_elem: int = 400, kind: str = "tower", seed: int | None = None,
                 instance: int = 0, steps: int = 200, dt: float = 1e-3, amp: float | None = None,
                 tol=(1e-8, 1e-8, 1e-8), algflag: int = 5, numopt: int = 0,
-                spectrds: float = 1.0, itemax: int = 20) -> ModelText:
+                spectrds: float = 1.0, itemax: int = 20, slice_shape: tuple | None = None) -> ModelText:
     """A random tower or lattice girder of about ``n_elem`` frame members.
 
     tower : square storeys of 4 columns, 4 beams and 8 face X-braces; plan
@@ -252,6 +252,9 @@ def space_frame(n_elem: int = 400, kind: str = "tower", seed: int | None = None,
     lateral load amp * sin(2 pi f t) with f ~ U(2, 10) Hz over ``steps`` steps
     of ``dt`` (average acceleration Newmark with numopt 0, spectral radius 1).
     Joints are numbered level by level so the skyline band stays narrow.
+    slice_shape = (ny, nz) replaces the drawn girder cross-section (the draws
+    are still made, so the rest of the model is the same); a wide slice gives
+    a wide band.
     """
     rng = SplitMix64(workload_seed(4, instance) if seed is None else seed)
     joints, members, fixed, loaded = [], [], [], []
@@ -277,6 +280,8 @@ def space_frame(n_elem: int = 400, kind: str = "tower", seed: int | None = None,
     elif kind == "grid":
         ny = 2 + int(rng.uniform(1, 0, 3)[0])
         nz = 2 + int(rng.uniform(1, 0, 3)[0])
+        if slice_shape is not None:
+            ny, nz = int(slice_shape[0]), int(slice_shape[1])
         per_slice = ny * nz * 3 - ny - nz          # members between and within slices
         nx = max(2, int(math.ceil(n_elem / per_slice)))
         sx, sy, sz = (float(v) for v in rng.uniform(3, 1.0, 4.0))

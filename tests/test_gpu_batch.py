@@ -8,6 +8,8 @@ side by side on the GPU, through the C ABI (cb_batch_*).
   the batch step on.  Every model's path is the path of the same model run
   alone (cb_run): identical iteration counts, displacements to 1e-12
   relative; and the oracle's path to 1e-9 with identical iteration counts.
+- Girders with three tiles below the diagonal, factored through memory
+  rather than in LDS, against their single runs and the oracle.
 - The 512-frame workload (200-5000 members each, about 2.66 million
   equations in all) for 20 time steps: every model runs to the step limit
   with finite displacements, and the largest and smallest models and the
@@ -120,6 +122,36 @@ def test_frame_workload_batch_512_models():
             assert relerr(rb["d"], rs["d"]) < 1e-12
         for i in range(0, 512, 37):
             assert np.all(np.isfinite(b.path(i)["d"]))
+
+
+def test_batch_wide_band_models():
+    """Girders whose band has three tiles below the diagonal: their workgroup
+    factors them with the panels and updates through memory (not in LDS).
+    Batch paths equal the single runs and the oracle's."""
+    steps = 10
+    specs = [dict(n=300, shape=(5, 5), inst=131, amp=3e5), dict(n=400, shape=(6, 5), inst=132, amp=8e4),
+             dict(n=120, shape=None, inst=133, amp=1e6)]
+    paths = []
+    for i, sp in enumerate(specs):
+        kind = "grid" if sp["shape"] else "tower"
+        txt = inputs.space_frame(sp["n"], kind=kind, instance=sp["inst"], steps=steps, amp=sp["amp"],
+                                 slice_shape=sp["shape"]).text
+        paths.append(model_file(f"batch_wide_{i}.txt", txt))
+    with cubens.GpuBatch(paths) as b:
+        assert sum(inf.bandwidth > 128 for inf in b.info) == 2, [inf.bandwidth for inf in b.info]
+        st = b.run(max_steps=steps)
+        assert st["failed"] == 0 and np.all(st["status"] == 0) and np.all(st["nsteps"] == steps), st
+        for i, p in enumerate(paths):
+            rb = b.path(i)
+            rs = _single(p, steps)
+            ro = O.OracleModel(p).run(steps)
+            assert ro["rc"] == 0, ro["msg"]
+            print(f"model {i}: neq {b.info[i].neq}, bandwidth {b.info[i].bandwidth}, "
+                  f"batch vs single {relerr(rb['d'], rs['d']):.1e}, vs oracle {relerr(rb['d'], ro['d']):.1e}")
+            assert np.array_equal(rb["itecnt"], rs["itecnt"])
+            assert relerr(rb["d"], rs["d"]) < 1e-12
+            assert np.array_equal(rb["itecnt"], ro["itecnt"])
+            assert relerr(rb["d"], ro["d"]) < 1e-9
 
 
 def test_batch_edge_cases():
