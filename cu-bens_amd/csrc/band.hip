@@ -760,9 +760,12 @@ __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* _
 // Each wait is bounded: past spin_limit polls the value reads as
 // a quiet NaN and *err is raised, so every workgroup still finishes and the
 // sweep drains.  Contributions of earlier rows are applied while the newest
-// one is still pending, with the next off-diagonal tile prefetched; the
-// newest off-diagonal tile and Linv stay in registers, so the critical path
-// per row is one store-to-load hand-off and two 64x64 products.
+// one is still pending, four rows at a time: each wavefront polls one of
+// them, so a workgroup that is catching up pays one memory round trip per
+// four rows rather than per row (one per row made the workgroups finish
+// their older rows only just before the newest one arrived).  The newest
+// off-diagonal tile and Linv stay in registers, so the critical path per row
+// is one store-to-load hand-off and two 64x64 products.
 constexpr unsigned long long SENT = 0x7FF4C0DE5E471E11ull;
 
 __device__ __forceinline__ double poll_value(const double* p, long spin_limit, int* err) {
@@ -796,6 +799,17 @@ __device__ __forceinline__ void fetch_row(const double* v, long J, double* vin, 
     __syncthreads();
 }
 
+// Rows that the sweeps' older contributions need, polled side by side:
+// wavefront c polls row J + dir * c (c < nc) into vch[c], so a chunk costs
+// one memory round trip instead of one per row.
+constexpr int NCH = 4;
+__device__ __forceinline__ void fetch_rows(const double* v, long J, int dir, int nc, double (*vch)[T],
+                                           long spin_limit, int* err) {
+    const int c = threadIdx.x >> 6;
+    if (c < nc) vch[c][threadIdx.x & 63] = poll_value(v + (J + dir * c) * T + (threadIdx.x & 63), spin_limit, err);
+    __syncthreads();
+}
+
 __device__ __forceinline__ void put_value(double* p, double x) {
     __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
@@ -807,10 +821,11 @@ __global__ void __launch_bounds__(256) k_fwd_sweep(const R* __restrict__ tiles, 
                                                    double* __restrict__ xr, double* y, long nt, int w,
                                                    long spin_limit, int* err) {
     __shared__ double vin[T];
+    __shared__ double vch[NCH][T];
     __shared__ double part[4][T];
     const long I = blockIdx.x;
     const int tid = threadIdx.x, row = tid % T, q = tid / T, k0 = 16 * q;
-    R lv[16], ln[16], lc[16];
+    R lv[16], ln[16];
     const R* lvp = linv + I * (long)TT;
 #pragma unroll
     for (int k = 0; k < 16; ++k) lv[k] = lvp[(k0 + k) * T + row];
@@ -821,32 +836,30 @@ __global__ void __launch_bounds__(256) k_fwd_sweep(const R* __restrict__ tiles, 
 #pragma unroll
         for (int k = 0; k < 16; ++k) ln[k] = lnp[(k0 + k) * T + row];
     }
-    if (J0 < I - 1) {
-        const R* l = tiles + (I * (w + 1) + (J0 - I + w)) * (long)TT;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) lc[k] = l[(k0 + k) * T + row];
-    }
     double acc = 0.0;
     if (q == 0) {
         acc = xr[I * T + row];
         xr[I * T + row] = __builtin_bit_cast(double, SENT);
     }
-    for (long J = J0; J < I - 1; ++J) {
-        R nx[16];
-        const bool more = J + 1 < I - 1;
-        if (more) {
-            const R* l = tiles + (I * (w + 1) + (J + 1 - I + w)) * (long)TT;
+    // the older rows J < I - 1 in chunks of NCH, wavefront c polling row J + c
+    for (long J = J0; J < I - 1; J += NCH) {
+        const int nc = (I - 1 - J < NCH) ? int(I - 1 - J) : NCH;
+        R lx[NCH][16];
 #pragma unroll
-            for (int k = 0; k < 16; ++k) nx[k] = l[(k0 + k) * T + row];
-        }
-        fetch_row(y, J, vin, spin_limit, err);
+        for (int c = 0; c < NCH; ++c)
+            if (c < nc) {
+                const R* l = tiles + (I * (w + 1) + (J + c - I + w)) * (long)TT;
 #pragma unroll
-        for (int k = 0; k < 16; ++k) acc -= double(lc[k]) * vin[k0 + k];
+                for (int k = 0; k < 16; ++k) lx[c][k] = l[(k0 + k) * T + row];
+            }
+        fetch_rows(y, J, 1, nc, vch, spin_limit, err);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            if (c < nc) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) acc -= double(lx[c][k]) * vch[c][k0 + k];
+            }
         __syncthreads();
-        if (more) {
-#pragma unroll
-            for (int k = 0; k < 16; ++k) lc[k] = nx[k];
-        }
     }
     if (has_prev) {
         fetch_row(y, I - 1, vin, spin_limit, err);
@@ -875,10 +888,11 @@ __global__ void __launch_bounds__(256) k_bwd_sweep(const R* __restrict__ tiles_t
                                                    const double* __restrict__ y, const double* __restrict__ piv,
                                                    double* x, long nt, int w, long spin_limit, int* err) {
     __shared__ double vin[T];
+    __shared__ double vch[NCH][T];
     __shared__ double part[4][T];
     const long I = nt - 1 - blockIdx.x;
     const int tid = threadIdx.x, col = tid % T, q = tid / T, k0 = 16 * q;
-    R lv[16], ln[16], lc[16];
+    R lv[16], ln[16];
     const R* lvp = linv + I * (long)TT;
 #pragma unroll
     for (int k = 0; k < 16; ++k) lv[k] = lvp[col * T + k0 + k];   // Linv[k][col]
@@ -889,28 +903,26 @@ __global__ void __launch_bounds__(256) k_bwd_sweep(const R* __restrict__ tiles_t
 #pragma unroll
         for (int k = 0; k < 16; ++k) ln[k] = lnp[(k0 + k) * T + col];
     }
-    if (J1 > I + 1) {
-        const R* l = tiles_t + (J1 * (w + 1) + (I - J1 + w)) * (long)TT;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) lc[k] = l[(k0 + k) * T + col];
-    }
     double acc = (q == 0) ? y[I * T + col] / piv[I * T + col] : 0.0;
-    for (long J = J1; J > I + 1; --J) {
-        R nx[16];
-        const bool more = J - 1 > I + 1;
-        if (more) {
-            const R* l = tiles_t + ((J - 1) * (w + 1) + (I - (J - 1) + w)) * (long)TT;
+    // the later rows J > I + 1 in chunks of NCH, wavefront c polling row J - c
+    for (long J = J1; J > I + 1; J -= NCH) {
+        const int nc = (J - (I + 1) < NCH) ? int(J - (I + 1)) : NCH;
+        R lx[NCH][16];
 #pragma unroll
-            for (int k = 0; k < 16; ++k) nx[k] = l[(k0 + k) * T + col];
-        }
-        fetch_row(x, J, vin, spin_limit, err);
+        for (int c = 0; c < NCH; ++c)
+            if (c < nc) {
+                const R* l = tiles_t + ((J - c) * (w + 1) + (I - (J - c) + w)) * (long)TT;
 #pragma unroll
-        for (int k = 0; k < 16; ++k) acc -= double(lc[k]) * vin[k0 + k];
+                for (int k = 0; k < 16; ++k) lx[c][k] = l[(k0 + k) * T + col];
+            }
+        fetch_rows(x, J, -1, nc, vch, spin_limit, err);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            if (c < nc) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) acc -= double(lx[c][k]) * vch[c][k0 + k];
+            }
         __syncthreads();
-        if (more) {
-#pragma unroll
-            for (int k = 0; k < 16; ++k) lc[k] = nx[k];
-        }
     }
     if (has_next) {
         fetch_row(x, I + 1, vin, spin_limit, err);
