@@ -647,8 +647,7 @@ __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* _
     __shared__ R xs[T][T];
     __shared__ R ys[T][T];
     __shared__ double dk[T];
-    // one slot per consecutive wait: a slow thread still reads its wait's
-    // result after thread 0 has gone on to the next wait
+    // wait results: one slot per flag waited on
     __shared__ int ok_s[3];
     const long m = (K < 0) ? 0 : ((nt - 1 - K < w) ? nt - 1 - K : w);
     const long b = blockIdx.x;
@@ -727,9 +726,30 @@ __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* _
         ++r;
     }
     const long s0 = 2 * t, ns = (s0 + 1 <= r) ? 2 : 1;
-    if (!wait_flag_bounded(flags, r, epoch, spin_limit, err, &ok_s[0])) return;
-    for (long j = 0; j < ns; ++j)
-        if (s0 + j != r && !wait_flag_bounded(flags, s0 + j, epoch, spin_limit, err, &ok_s[1 + j])) return;
+    // the (up to three) panel flags of the pair, polled side by side: lane i
+    // of wavefront 0 polls flag i, so the waits cost one round trip together
+    if (threadIdx.x < 3) {
+        const int i = threadIdx.x;
+        const long f = (i == 0) ? r : s0 + (i - 1);
+        const bool need = (i == 0) || (i - 1 < ns && f != r);
+        bool good = true;
+        if (need) {
+            const int* fp = flags + f * FLAG_STRIDE;
+            long n = 0;
+            while (__hip_atomic_load(fp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != epoch) {
+                if (++n >= spin_limit) { good = false; break; }
+                if ((n & 63) == 0 && (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) & 1)) {
+                    good = false;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            if (!good) atomicOr(err, 1);
+        }
+        ok_s[i] = good;
+    }
+    __syncthreads();
+    if (!(ok_s[0] && ok_s[1] && ok_s[2])) return;
     const long I = K + 1 + r;
     stage_scaled<true>(xs, tiles + tile_index(I, K, w) * TT, dk);
     for (long j = 0; j < ns; ++j) {
