@@ -646,6 +646,7 @@ __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* _
                                                      int* flags, int epoch, long spin_limit, int* err) {
     __shared__ R xs[T][T];
     __shared__ R ys[T][T];
+    __shared__ R zs[T][T];   // an update workgroup's second L_JK
     __shared__ double dk[T];
     // wait results: one slot per flag waited on
     __shared__ int ok_s[3];
@@ -751,15 +752,15 @@ __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* _
     __syncthreads();
     if (!(ok_s[0] && ok_s[1] && ok_s[2])) return;
     const long I = K + 1 + r;
+    // L_IK D_K and both L_JK staged together
     stage_scaled<true>(xs, tiles + tile_index(I, K, w) * TT, dk);
+    stage<true>(ys, tiles + tile_index(K + 1 + s0, K, w) * TT);
+    if (ns == 2) stage<true>(zs, tiles + tile_index(K + 2 + s0, K, w) * TT);
+    __syncthreads();
     for (long j = 0; j < ns; ++j) {
-        const long J = K + 1 + s0 + j;
-        if (j > 0) __syncthreads();   // the previous product has read ys
-        stage<true>(ys, tiles + tile_index(J, K, w) * TT);
-        __syncthreads();
         TileAcc<R> acc;
-        acc.product(xs, ys);
-        R* out = tiles + tile_index(I, J, w) * TT;
+        acc.product(xs, j == 0 ? ys : zs);
+        R* out = tiles + tile_index(I, K + 1 + s0 + j, w) * TT;
         acc.each([&](int row, int col, R v) { out[col * T + row] -= v; });
     }
 }
