@@ -714,7 +714,7 @@ __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* _
         if (pb == 0) ok = wait_flag_bounded(flags, w, epoch, spin_limit, err, &ok_s[0]);
         else __syncthreads();
         TileAcc<R> acc;
-        acc.product(xs, ys);
+        acc.product_ylower(xs, ys);   // Linv_K is lower triangular
         if (ok) acc.each([&](int row, int col, R v) { cst<true>(&tile[col * T + row], v * (R(1) / R(dk[col]))); });
         raise_flag(flags, pb, epoch);
         if (ok) acc.each([&](int row, int col, R v) { ttile[row * T + col] = v * (R(1) / R(dk[col])); });
