@@ -89,7 +89,7 @@ def kernel_model(info) -> dict:
 # interior launches 3000-3099 of the 256x256 panel, k_band_factor_models over
 # the 13 rounds of 5 time steps of the 512 frames.
 MEASURED_TRAFFIC = {
-    "band_column": (8644.92 * 2.0 + 12001.31) * 1024.0,
+    "band_column": (8647.29 * 2.0 + 12001.31) * 1024.0,
     "band_factor_models": (1679014.64 * 2.0 + 3575412.50) * 1024.0,
 }
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, KiB per dispatch scaled by a 256 MiB "
