@@ -41,3 +41,16 @@ def test_open_without_gpu_fails_loudly(tmp_path):
     p.write_text(inputs.shell_panel(n=2).text)
     with pytest.raises(cubens.CubensError):
         cubens.GpuModel(str(p))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py is a one-GPU benchmark: any other --gpus exits 2 with a JSON error line."""
+    import json
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], cwd=root,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 2
+    line = json.loads(r.stdout.strip().splitlines()[-1])
+    assert line["n_gpus"] == 2 and "one GPU" in line["error"]
