@@ -75,7 +75,7 @@ class ModelText:
 # --------------------------------------------------------------------------
 def truss_dome(n: int = 10, rise: float = 0.5, seed: int | None = None,
                load: float = -80.0, steps: int = 20, tol=(1e-9, 1e-6, 1e-12),
-               algflag: int = 1) -> ModelText:
+               algflag: int = 1, msal: dict | None = None) -> ModelText:
     """n x n bays at 1 m, paraboloid crown rise, boundary pinned, centre load.
 
     The crown of a 0.5 m paraboloid over 10 m is locally flat (bar slope 0.01),
@@ -85,6 +85,11 @@ def truss_dome(n: int = 10, rise: float = 0.5, seed: int | None = None,
     displacement criteria are set above their round-off floors (the bar force
     comes from L - L0, which cancels).  lpfmax is a hair above 1 so the twentieth
     step runs although 20 * 0.05 rounds above 1.
+
+    algflag 3 runs the arc-length method instead, with ``load`` as the
+    reference load and the centre joint's direction 3 as the control DOF;
+    ``msal`` overrides its parameters (dk, alpha, psi_thresh, iteopt, lpfmax,
+    dkimax, itemax, submax, imagmax, negmax).
     """
     rng = SplitMix64(workload_seed(1) if seed is None else seed)
     m = n + 1
@@ -124,12 +129,83 @@ def truss_dome(n: int = 10, rise: float = 0.5, seed: int | None = None,
         w(f"200000000000.0,{_fmt(a)},7850.0,1e30\n")
     ctr = node(n // 2, n // 2)
     w(f"{ctr},3,{_fmt(load)}\n0,0,0\n")
-    dl = 1.0 / steps
-    w(f"{_fmt(1.0 + 1e-9)},{_fmt(dl)},{_fmt(dl)},{_fmt(dl)},{_fmt(dl)}\n")
-    w("50,10,1000\n")
+    if algflag == 3:
+        p = {"dk": -0.01, "alpha": 1.0, "psi_thresh": 0.1, "iteopt": 6, "lpfmax": 1e30,
+             "dkimax": 1.2, "itemax": 30, "submax": 10, "imagmax": 10, "negmax": 10}
+        p.update(msal or {})
+        w(f"{ctr},3,{_fmt(p['dk'])}\n")
+        w(f"{_fmt(p['alpha'])}\n{_fmt(p['psi_thresh'])}\n{p['iteopt']}\n")
+        w(f"{_fmt(p['lpfmax'])},{_fmt(p['dkimax'])}\n")
+        w(f"{p['itemax']},{p['submax']},{p['imagmax']},{p['negmax']}\n")
+    else:
+        dl = 1.0 / steps
+        w(f"{_fmt(1.0 + 1e-9)},{_fmt(dl)},{_fmt(dl)},{_fmt(dl)},{_fmt(dl)}\n")
+        w("50,10,1000\n")
     w(f"{tol[0]},{tol[1]},{tol[2]}\n")
     return ModelText(out.getvalue(), m * m, n_tr=len(bars),
                      info={"centre_joint": ctr, "neq": 3 * (n - 1) ** 2})
+
+
+def star_truss(bars: int = 4, radius: float = 1.0, rise: float = 0.1, emod: float = 2e11,
+               area: float = 1e-4, load: float = -1.0, dk: float = -0.005, alpha: float = 1.0,
+               psi_thresh: float = 0.1, iteopt: int = 6, dkimax: float | None = None,
+               lpfmax: float = 1e30, itemax: int = 30, submax: int = 10, imagmax: int = 10,
+               negmax: int = 10, tol=(1e-12, 1e-10, 1.0), tilt: float = 0.0) -> ModelText:
+    """A shallow star truss under arc-length control: ``bars`` equal bars from
+    pinned supports on a circle of ``radius`` (at height 0, angles 2 pi k/bars)
+    to one apex at height ``rise``, loaded by ``load`` in direction 3 at the
+    apex, which is also the arc-length control DOF.  The apex has its three
+    translations free (NEQ = 3).
+
+    Under a vertical load the apex stays on the axis and the load factor
+    follows the closed form (truss.c:327-331 for the bar force)
+        lambda(w) = bars * N(L) * H / (L * load),   H = rise + w,
+        L = sqrt(radius^2 + H^2),  N = E A (e + e^2/2) L / L0,  e = (L - L0)/L0,
+    which rises to a limit point, falls through zero at the flat position
+    (H = 0), reaches a minimum and rises again once the apex has snapped
+    through.  ``tilt`` > 0 moves the apex sideways by tilt * radius so the
+    three DOFs couple (no closed form then).
+    """
+    h = float(rise)
+    if dkimax is None:
+        dkimax = 2.6 * h
+    out = io.StringIO()
+    w = out.write
+    nj = bars + 1
+    w(f"2\n3\n0\n1\n{nj}\n{bars},0,0,0,0\n")
+    for k in range(bars):
+        w(f"{k + 2},1\n")             # support -> apex (apex is joint 1)
+    for k in range(bars):
+        for d in (1, 2, 3):
+            w(f"{k + 2},{d}\n")
+    w("0,0\n")
+    w(f"{_fmt(tilt * radius)},0.0,{_fmt(h)}\n")
+    for k in range(bars):
+        t = 2 * math.pi * k / bars
+        w(f"{_fmt(radius * math.cos(t))},{_fmt(radius * math.sin(t))},0.0\n")
+    for _ in range(bars):
+        w(f"{_fmt(emod)},{_fmt(area)},7850.0,1e30\n")
+    w(f"1,3,{_fmt(load)}\n0,0,0\n")
+    w(f"1,3,{_fmt(dk)}\n")
+    w(f"{_fmt(alpha)}\n{_fmt(psi_thresh)}\n{iteopt}\n")
+    w(f"{_fmt(lpfmax)},{_fmt(dkimax)}\n")
+    w(f"{itemax},{submax},{imagmax},{negmax}\n")
+    w(f"{tol[0]},{tol[1]},{tol[2]}\n")
+    return ModelText(out.getvalue(), nj, n_tr=bars,
+                     info={"bars": bars, "radius": radius, "rise": h, "emod": emod, "area": area,
+                           "load": load, "neq": 3})
+
+
+def star_truss_lambda(w, bars: int = 4, radius: float = 1.0, rise: float = 0.1,
+                      emod: float = 2e11, area: float = 1e-4, load: float = -1.0):
+    """Closed-form load factor of star_truss at apex displacement w (direction 3)."""
+    w = np.asarray(w, dtype=np.float64)
+    L0 = math.sqrt(radius * radius + rise * rise)
+    H = rise + w
+    L = np.sqrt(radius * radius + H * H)
+    e = (L - L0) / L0
+    N = emod * area * (e + 0.5 * e * e) * L / L0
+    return bars * N * H / (L * load)
 
 
 # --------------------------------------------------------------------------

@@ -121,6 +121,12 @@ void orc_mass_fr(const orc_dims *d, double *sm, const orc_model *m, const double
 void orc_mass_sh(const orc_dims *d, double *sm, const orc_model *m, const double *x,
                  double *farea, double *slength, const double *dens);
 
+/* Support reactions r[NJ*7] of an element state (ef, triads): T_i^T ef_i of
+ * every element summed at the constrained joint slots (jcode 0), zero at the
+ * free ones (the reference computes none; orc_reactions.c). */
+void orc_reactions(const orc_model *m, const double *ef, const double *c1, const double *c2,
+                   const double *c3, double *r);
+
 /* ---- solver pieces (solve.c, misc.c, arc.c) ------------------------------ */
 int orc_skyfact(long neq, const long *maxa, double *ss, double *ssd, int msal, int *det);
 void orc_skysolve(long neq, const long *maxa, const double *ss, double *rhs);
@@ -211,7 +217,21 @@ typedef struct {
     char msg[160];
     double t_tangent, t_factor, t_solve, t_forces, t_other;  /* seconds */
     long n_factor, n_solve, n_newton;
+    /* optional per-step driver events [max_steps*ORC_NEV] (ORC_EV_* slots) and
+     * the arc-length psi the step was taken with [max_steps] (NULL: not kept) */
+    int *events;
+    double *psi;
+    double *react;       /* optional [max_steps*NJ*7] reactions of each converged step */
 } orc_path;
+
+/* Driver events of one converged step: the pivot-sign flag of the step's last
+ * factorization (det, solve.c:608-621), and how often within the step quad
+ * found imaginary roots (arc.c:145) or no root with a positive gamma
+ * (arc.c:134), the dnorm test restarted the increment (main.c:2936-2953), and
+ * the arc or load step was halved after itemax iterations (main.c:3000-3012,
+ * 2096-2110). */
+enum { ORC_EV_DET = 0, ORC_EV_IMAG = 1, ORC_EV_NEG = 2, ORC_EV_DNORM = 3, ORC_EV_HALVE = 4,
+       ORC_NEV = 5 };
 
 /* Runs the analysis selected by ALGFLAG (1/2 NR/MNR main.c:1800-2157,
  * 3 MSAL main.c:2158-3141, 5 nonlinear Newmark main.c:3305-3960) with the

@@ -53,7 +53,12 @@ class Path(C.Structure):
                 ("d", PD), ("f", PD), ("status", C.c_int), ("msg", C.c_char * 160),
                 ("t_tangent", C.c_double), ("t_factor", C.c_double), ("t_solve", C.c_double),
                 ("t_forces", C.c_double), ("t_other", C.c_double),
-                ("n_factor", C.c_long), ("n_solve", C.c_long), ("n_newton", C.c_long)]
+                ("n_factor", C.c_long), ("n_solve", C.c_long), ("n_newton", C.c_long),
+                ("events", PI), ("psi", PD), ("react", PD)]
+
+# Slots of the per-step driver events (ORC_EV_* in cb_oracle.h).
+EV_DET, EV_IMAG, EV_NEG, EV_DNORM, EV_HALVE = range(5)
+NEV = 5
 
 
 class Sample(C.Structure):
@@ -192,8 +197,18 @@ class OracleModel:
             raise MemoryError("oracle sample allocation failed")
         return {k: getattr(smp, k) for k, _ in smp._fields_}
 
-    def run(self, max_steps: int, backend: str = "oracle", keep_f: bool = False) -> dict:
-        """Runs the analysis; returns the converged path and timings."""
+    def reactions(self, ef, c1, c2, c3) -> np.ndarray:
+        """Support reactions [NJ, 7] of an element state (orc_reactions)."""
+        L = lib()
+        L.orc_reactions.argtypes = [C.POINTER(Model), PD, PD, PD, PD, PD]
+        r = np.zeros(self.nj * 7)
+        L.orc_reactions(self.h, _c(ef), _c(c1), _c(c2), _c(c3), ptr(r))
+        return r.reshape(self.nj, 7)
+
+    def run(self, max_steps: int, backend: str = "oracle", keep_f: bool = False,
+            keep_react: bool = False) -> dict:
+        """Runs the analysis; returns the converged path and timings (and, with
+        keep_react, the reactions [nsteps, NJ, 7] of every converged step)."""
         P = Path()
         P.max_steps = max_steps
         lpf = np.zeros(max_steps)
@@ -204,6 +219,12 @@ class OracleModel:
         P.itecnt = ptr(ite, PI)
         P.d = ptr(dmat)
         P.f = ptr(fmat) if keep_f else None
+        ev = np.zeros((max_steps, NEV), np.int32)
+        psi = np.zeros(max_steps)
+        P.events = ptr(ev, PI)
+        P.psi = ptr(psi)
+        react = np.zeros((max_steps, self.nj, 7)) if keep_react else None
+        P.react = ptr(react) if keep_react else None
         be = None
         if backend == "ref":
             be = Backend()
@@ -214,9 +235,11 @@ class OracleModel:
                "lpf": lpf[:n].copy(), "itecnt": ite[:n].copy(), "d": dmat[:n].copy(),
                "t_tangent": P.t_tangent, "t_factor": P.t_factor, "t_solve": P.t_solve,
                "t_forces": P.t_forces, "n_factor": P.n_factor, "n_solve": P.n_solve,
-               "n_newton": P.n_newton}
+               "n_newton": P.n_newton, "events": ev[:n].copy(), "psi": psi[:n].copy()}
         if keep_f:
             out["f"] = fmat[:n].copy()
+        if keep_react:
+            out["react"] = react[:n].copy()
         return out
 
 

@@ -150,6 +150,8 @@ typedef struct {
     double *farea_w, *slen_w;          /* face areas / side lengths the elements read */
     double *keff, *uc, *vc, *ac, *uc_i, *vc_i, *ac_i, *um, *vm, *am, *dyn;
     orc_path *out;
+    int ev[ORC_NEV];                   /* events since the last recorded step */
+    double ev_psi;                     /* psi of the step being taken */
 } run_t;
 
 static double *dz(long n) { return calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
@@ -158,6 +160,7 @@ static double *dz(long n) { return calloc((size_t)(n > 0 ? n : 1), sizeof(double
 static void run_init(run_t *R, orc_model *m, const orc_backend *be, orc_path *out)
 {
     memset(R, 0, sizeof *R);
+    R->ev_psi = 1;
     const orc_dims *d = &m->dm;
     R->m = m; R->be = be; R->out = out;
     R->neq = d->NEQ; R->nj = d->NJ; R->nsh = d->NE_SH; R->nfr = d->NE_FR;
@@ -297,6 +300,7 @@ static int factor(run_t *R, int *det)
 {
     double t0 = now_s();
     int e = orc_skyfact(R->neq, R->m->maxa, R->ss, R->ssd, R->m->dm.ALGFLAG == 3, det);
+    R->ev[ORC_EV_DET] = *det;
     if (R->out) { R->out->t_factor += now_s() - t0; R->out->n_factor++; }
     return e;
 }
@@ -317,7 +321,12 @@ static void record(run_t *R, double lpf, int itecnt, const double *d, const doub
     if (o->itecnt) o->itecnt[o->nsteps] = itecnt;
     if (o->d) CP(o->d + (long)o->nsteps * R->neq, d, R->neq);
     if (o->f && f) CP(o->f + (long)o->nsteps * R->neq, f, R->neq);
+    if (o->events) memcpy(o->events + (long)o->nsteps * ORC_NEV, R->ev, sizeof R->ev);
+    if (o->psi) o->psi[o->nsteps] = R->ev_psi;
+    if (o->react)
+        orc_reactions(R->m, R->ef, R->c1, R->c2, R->c3, o->react + (long)o->nsteps * R->nj * 7);
     o->nsteps++;
+    memset(R->ev, 0, sizeof R->ev);
 }
 
 static double vdot(long n, const double *a, const double *b)
@@ -394,6 +403,7 @@ static int run_nr(run_t *R)
             lpf = lpf - dlpfp + dlpf;
             subcnt++;
             solcnt = 0;
+            R->ev[ORC_EV_HALVE]++;
         } else if (subcnt > m->submax) {
             FAIL(1, "maximum number of subdivisions exceeded");
         } else {
@@ -510,6 +520,7 @@ static int run_msal(run_t *R)
 
     int errchk2 = 0, subcnt = 0, imagcnt = 0, negcnt = 0;
     double arc = 0, a = 0, b, c, lpf_temp;
+    R->ev_psi = psi;
     while (lpfc <= m->lpfmax && dkc <= m->dkimax) {
         CP(R->d_temp, R->d, n);
         lpf_temp = lpf;
@@ -531,6 +542,7 @@ static int run_msal(run_t *R)
             R->ddq[0] = m->q[0] / R->ss[0];
             R->ssd[0] = R->ss[0];
             det = R->ss[0] > 0 ? 0 : 1;
+            R->ev[ORC_EV_DET] = det;
         } else {
             factor(R, &det);
             backsolve(R, m->q, R->ddq);
@@ -603,12 +615,14 @@ static int run_msal(run_t *R)
                         arc *= beta;
                         CP(R->f_temp, R->f, n);
                         errchk2 = 1;
+                        R->ev[ORC_EV_DNORM]++;
                     }
                 } else if (frc != 0 && subcnt <= m->submax) {
                     arc *= 0.5;
                     CP(R->f_temp, R->f, n);
                     subcnt++;
                     errchk2 = 1;
+                    R->ev[ORC_EV_HALVE]++;
                 } else {
                     itecnt++;
                     if (itecnt > m->itemax) {
@@ -616,6 +630,7 @@ static int run_msal(run_t *R)
                         CP(R->f_temp, R->f, n);
                         subcnt++;
                         errchk2 = 1;
+                        R->ev[ORC_EV_HALVE]++;
                     }
                     shift_ip(R, 1);
                 }
@@ -623,10 +638,12 @@ static int run_msal(run_t *R)
                 arc *= 0.5;
                 CP(R->f_temp, R->f, n);
                 imagcnt++;
+                R->ev[ORC_EV_IMAG]++;
             } else if (errchk2 == 3) {
                 arc *= 0.5;
                 CP(R->f_temp, R->f, n);
                 negcnt++;
+                R->ev[ORC_EV_NEG]++;
             }
         } while (convchk != 0 && errchk2 == 0 && subcnt <= m->submax &&
                  imagcnt <= m->imagmax && negcnt <= m->negmax);
@@ -652,6 +669,7 @@ static int run_msal(run_t *R)
             }
             subcnt = imagcnt = negcnt = 0;
             record(R, lpf, itecnt, R->d, R->f);
+            R->ev_psi = psi;
             if (R->out && R->out->nsteps >= R->out->max_steps) return 0;
         }
     }
@@ -844,6 +862,7 @@ static int run_newmark(run_t *R)
                     if (dlpf < m->dlpfmin) dlpf = m->dlpfmin;
                     lpf = lpf - dlpfp + dlpf;
                     subcnt++;
+                    R->ev[ORC_EV_HALVE]++;
                     solcnt = 0;
                 } else if (subcnt > m->submax) {
                     break;
@@ -858,6 +877,7 @@ static int run_newmark(run_t *R)
                 /* halve the time increment (main.c:3826-3846) */
                 ddt = ddt / 2;
                 dt_temp = ddt * dt;
+                R->ev[ORC_EV_HALVE]++;
                 if (tsflag == 0) sub_dt = ddt;
                 else if (tsflag == 1) sub_dt = sub_dt - ddt;
             } else {
