@@ -1,6 +1,7 @@
 // C ABI over the host model, device state and drivers (include/cubens.h).
 #include "../../include/cubens.h"
 
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -58,6 +59,8 @@ struct cb_model {
     std::vector<long> step_newton;
     std::vector<std::vector<double>> step_kt_ms;
     std::vector<std::vector<long>> step_kt_n;
+    std::vector<std::array<int, cb::EV_N>> step_ev;
+    std::vector<double> step_psi;
 };
 
 struct cb_batch {
@@ -227,8 +230,6 @@ int cb_open(const char* path, cb_model** out) {
     std::string err;
     if (!cb::load_model(path, h->m, err)) return fail(-4, err);
     cb::build_pattern(h->m);
-    if (h->m.col_ptr_sh.size() > 65 || h->m.col_ptr_tr.size() > 65)
-        return fail(-4, "element colouring exceeded 64 classes");
     cb::dev_create(h->d, h->m);
     h->opt = default_options();
     *out = h.release();
@@ -420,6 +421,8 @@ int cb_run(cb_model* h, const cb_options* opt, cb_stats* st) {
     h->step_newton = an.step_newton;
     h->step_kt_ms = an.step_kt_ms;
     h->step_kt_n = an.step_kt_n;
+    h->step_ev = an.step_ev;
+    h->step_psi = an.step_psi;
     if (st) {
         st->t_total = rs.t_total; st->t_tangent = rs.t_tangent; st->t_factor = rs.t_factor;
         st->t_solve = rs.t_solve; st->t_forces = rs.t_forces; st->t_judge = rs.t_judge;
@@ -431,7 +434,7 @@ int cb_run(cb_model* h, const cb_options* opt, cb_stats* st) {
         st->nsteps = int(an.lpf.size());
     }
     if (rc == 4) return fail(-2, rs.msg);
-    if (rc) g_err = rs.msg;
+    if (rc) return fail(rc, rs.msg);   // the driver stopped; its path so far is kept
     return 0;
     GUARD_END
 }
@@ -450,6 +453,30 @@ int cb_get_step_stats(const cb_model* h, double* t_wall, double* t_judge, long* 
     if (t_wall) std::memcpy(t_wall, h->step_t.data(), sizeof(double) * n);
     if (t_judge) std::memcpy(t_judge, h->step_judge.data(), sizeof(double) * n);
     if (newton) std::memcpy(newton, h->step_newton.data(), sizeof(long) * n);
+    return 0;
+}
+
+int cb_get_reactions(cb_model* h, int gen, double* r) {
+    if (!h || !r) return fail(-1, "null argument");
+    if (gen < cb::G_BASE || gen > cb::G_IP) return fail(-1, "generation must be 0, 1 or 2");
+    GUARD_BEGIN
+    const size_t n = size_t(h->m.nj) * 7;
+    double* rx = cb::dalloc<double>(n);
+    cb::launch_reactions(h->d, gen, rx);
+    cb::d2h(r, rx, n, h->d.s);
+    CB_CHECK(hipStreamSynchronize(h->d.s));
+    (void)hipFree(rx);
+    return 0;
+    GUARD_END
+}
+
+int cb_get_step_events(const cb_model* h, int* events, double* psi) {
+    if (!h) return fail(-1, "null handle");
+    const size_t n = h->step_ev.size();
+    if (events)
+        for (size_t s = 0; s < n; ++s)
+            std::memcpy(events + s * cb::EV_N, h->step_ev[s].data(), sizeof(int) * cb::EV_N);
+    if (psi) std::memcpy(psi, h->step_psi.data(), sizeof(double) * n);
     return 0;
 }
 
@@ -474,19 +501,20 @@ int cb_get_kernel_times(const cb_model* h, int precision, int step, double* ms, 
 
 int cb_write_results2(const cb_model* h, const char* path) {
     if (!h || !path) return fail(-1, "null argument");
+    const long n = h->m.neq;
+    const size_t ns = h->path_lpf.size();
+    if (h->path_d.size() < ns * size_t(n))
+        return fail(-1, "the last run kept no displacement path (keep_d = 0)");
     FILE* fp = std::fopen(path, "w");
     if (!fp) return fail(-1, std::string("cannot write ") + path);
-    const long n = h->m.neq;
     std::fprintf(fp, "Model Displacements:\n\tLambda\t\tIterations");
     for (long i = 0; i < n; ++i) {
         if (i + 1 <= 1000) std::fprintf(fp, "\tDOF %ld\t", i + 1);
         else std::fprintf(fp, "DOF %ld\t", i + 1);
     }
-    const size_t ns = h->path_lpf.size();
     for (size_t s = 0; s < ns; ++s) {
         std::fprintf(fp, "\n\t%e\t%d\t", h->path_lpf[s], h->path_ite[s]);
-        if (h->path_d.size() >= (s + 1) * size_t(n))
-            for (long i = 0; i < n; ++i) std::fprintf(fp, "\t%e", h->path_d[s * size_t(n) + size_t(i)]);
+        for (long i = 0; i < n; ++i) std::fprintf(fp, "\t%e", h->path_d[s * size_t(n) + size_t(i)]);
     }
     std::fclose(fp);
     return 0;

@@ -79,6 +79,9 @@ void Analysis::record(double lam, int ite, const double* dvec) {
     const KernelClock& kc = (o_.solver == SOLVER_BAND32_FGMRES) ? b32_.clock : b64_.clock;
     step_kt_ms.emplace_back(kc.ms, kc.ms + KT_N);
     step_kt_n.emplace_back(kc.launches, kc.launches + KT_N);
+    step_ev.push_back(ev_);
+    step_psi.push_back(ev_psi_);
+    ev_.fill(0);
     if (o_.keep_d) {
         size_t off = dpath.size();
         dpath.resize(off + size_t(m_.neq));
@@ -138,6 +141,7 @@ int Analysis::factor_current(int* det) {
         if (have_piv)
             for (long i = 0; i < m_.neq; ++i)
                 if (!(pivots[i] > 0)) { *det = 1; break; }
+        ev_[EV_DET] = *det;
     }
     return rc;
 }
@@ -225,6 +229,10 @@ int Analysis::run(RunStats& st) {
     step_newton.clear();
     step_kt_ms.clear();
     step_kt_n.clear();
+    step_ev.clear();
+    step_psi.clear();
+    ev_.fill(0);
+    ev_psi_ = 1;
     dev_reset_state(d_, m_);
     for (double* v : {q, dv, dp, dpp, dt, dd, ddq, ddr, f, fp, ft, fip, r})
         CB_CHECK(hipMemsetAsync(v, 0, sizeof(double) * size_t(m_.neq), d_.s));
@@ -326,6 +334,7 @@ int Analysis::run_nr() {
             lam = lam - dlamp + dlam;
             subcnt++;
             solcnt = 0;
+            ev_[EV_HALVE]++;
         } else if (subcnt > m_.submax) {
             FAIL(1, "maximum number of subdivisions exceeded");
         } else {
@@ -431,6 +440,7 @@ int Analysis::run_msal() {
     double arc = 0, a = 0, b, c, lamt;
     double qq;
     dots(1, {q}, {q}, &qq);
+    ev_psi_ = psi;
     while (lamc <= m_.lpfmax && dkc <= m_.dkimax) {
         vcopy(d_, dt, dv, n);
         lamt = lam;
@@ -524,6 +534,7 @@ int Analysis::run_msal() {
                         arc *= beta;
                         vcopy(d_, ft, f, n);
                         errchk2 = 1;
+                        ev_[EV_DNORM]++;
                     }
                 } else {
                     itecnt_++;
@@ -532,6 +543,7 @@ int Analysis::run_msal() {
                         vcopy(d_, ft, f, n);
                         subcnt++;
                         errchk2 = 1;
+                        ev_[EV_HALVE]++;
                     }
                     shift_ip(true);
                 }
@@ -539,10 +551,12 @@ int Analysis::run_msal() {
                 arc *= 0.5;
                 vcopy(d_, ft, f, n);
                 imagcnt++;
+                ev_[EV_IMAG]++;
             } else if (errchk2 == 3) {
                 arc *= 0.5;
                 vcopy(d_, ft, f, n);
                 negcnt++;
+                ev_[EV_NEG]++;
             }
         } while (convchk != 0 && errchk2 == 0 && subcnt <= m_.submax && imagcnt <= m_.imagmax &&
                  negcnt <= m_.negmax);
@@ -563,6 +577,7 @@ int Analysis::run_msal() {
             psi = psi_now();
             subcnt = imagcnt = negcnt = 0;
             record(lam, itecnt_, dv);
+            ev_psi_ = psi;
             if (long(lpf.size()) >= o_.max_steps) return 0;
         }
     }
@@ -685,6 +700,7 @@ int Analysis::run_newmark() {
                     lam = lam - dlamp + dlam;
                     subcnt++;
                     solcnt = 0;
+                    ev_[EV_HALVE]++;
                 } else if (subcnt > m_.submax) {
                     break;
                 } else {
@@ -697,6 +713,7 @@ int Analysis::run_newmark() {
             if (convchk != 0) {
                 ddt = ddt / 2;                          // main.c:3826-3846
                 dt_temp = ddt * dtime;
+                ev_[EV_HALVE]++;
                 if (tsflag == 0) sub_dt = ddt;
                 else if (tsflag == 1) sub_dt = sub_dt - ddt;
             } else {

@@ -4,6 +4,7 @@
 // over the device element kernels and linear solvers.
 #pragma once
 
+#include <array>
 #include <functional>
 #include <string>
 #include <vector>
@@ -21,6 +22,14 @@ enum SolverKind {
     SOLVER_BJ_FGMRES = 2,       // FGMRES with joint-block Jacobi
     SOLVER_LEARNED_FGMRES = 3,  // FGMRES with the learned block approximate inverse
 };
+
+// Driver events of one converged step: the pivot-sign flag of the step's last
+// factorization (det, solve.c:608-621), and how often within the step quad
+// found imaginary roots (arc.c:145) or no root with a positive gamma
+// (arc.c:134), the dnorm test restarted the increment (main.c:2936-2953), and
+// the arc, load step or time step was halved after itemax iterations
+// (main.c:3000-3012, 2096-2110, 3826-3846).
+enum { EV_DET = 0, EV_IMAG = 1, EV_NEG = 2, EV_DNORM = 3, EV_HALVE = 4, EV_N = 5 };
 
 struct RunOptions {
     int solver = SOLVER_BAND64;
@@ -71,6 +80,10 @@ public:
     // cumulative band-kernel clocks at each converged step: [step][family]
     std::vector<std::vector<double>> step_kt_ms;
     std::vector<std::vector<long>> step_kt_n;
+    // per converged step: driver events (EV_* slots) and the arc-length psi
+    // the step was taken with
+    std::vector<std::array<int, EV_N>> step_ev;
+    std::vector<double> step_psi;
     PrecHook learned;
     // one-shot operations used by the C ABI
     int tangent_and_factor(int gx, int g, int* det);
@@ -96,6 +109,8 @@ private:
     double *ddq = nullptr, *ddr = nullptr, *f = nullptr, *fp = nullptr, *ft = nullptr, *fip = nullptr;
     double *r = nullptr, *eb = nullptr, *ex = nullptr, *tmp = nullptr;
     std::vector<double> piv0_;
+    std::array<int, EV_N> ev_{};   // events since the last recorded step
+    double ev_psi_ = 1;
 
     int run_nr();
     int run_msal();

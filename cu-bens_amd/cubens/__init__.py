@@ -49,6 +49,10 @@ class Stats(C.Structure):
 
 
 KT_COLUMN, KT_DIAG, KT_PANEL, KT_UPDATE, KT_FWD, KT_BWD = range(6)
+# Per-step driver events (cb_get_step_events): det flag, quad imaginary-root and
+# negative-gamma retries, dnorm restarts, itemax halvings.
+EV_DET, EV_IMAG, EV_NEG, EV_DNORM, EV_HALVE = range(5)
+NEV = 5
 KT_NAMES = ("band_column", "band_diag", "band_panel", "band_update", "band_fwd_sweep", "band_bwd_sweep")
 FACTOR_FUSED, FACTOR_SPLIT = 0, 1
 
@@ -101,6 +105,8 @@ ABI = {
     "cb_factor": (C.c_int, [C.c_void_p, C.POINTER(Options), PI, PD]),
     "cb_solve": (C.c_int, [C.c_void_p, PD, PD]),
     "cb_run": (C.c_int, [C.c_void_p, C.POINTER(Options), C.POINTER(Stats)]),
+    "cb_get_step_events": (C.c_int, [C.c_void_p, PI, PD]),
+    "cb_get_reactions": (C.c_int, [C.c_void_p, C.c_int, PD]),
     "cb_get_path": (C.c_int, [C.c_void_p, PD, PI, PD]),
     "cb_get_step_stats": (C.c_int, [C.c_void_p, PD, PD, PL]),
     "cb_get_kernel_times": (C.c_int, [C.c_void_p, C.c_int, C.c_int, PD, PL]),
@@ -249,8 +255,15 @@ class GpuModel:
         return x
 
     def run(self, opt: Options | None = None) -> dict:
+        """Runs the driver; a driver that stops with an error (cb_run's positive
+        return) gives its status and message, with the path up to there."""
         st = Stats()
-        _check(lib().cb_run(self.h, C.byref(opt) if opt is not None else None, C.byref(st)))
+        rc = lib().cb_run(self.h, C.byref(opt) if opt is not None else None, C.byref(st))
+        msg = lib().cb_last_error().decode(errors="replace") if rc else ""
+        if rc < 0:
+            _check(rc)
+        if rc != st.status:
+            raise CubensError(f"cb_run returned {rc} with driver status {st.status}")
         n = st.nsteps
         lpf = np.zeros(max(n, 1))
         ite = np.zeros(max(n, 1), np.int32)
@@ -261,11 +274,19 @@ class GpuModel:
         tj = np.zeros(max(n, 1))
         nw = np.zeros(max(n, 1), np.int64)
         _check(lib().cb_get_step_stats(self.h, _p(tw), _p(tj), _p(nw, PL)))
+        ev = np.zeros((max(n, 1), NEV), np.int32)
+        psi = np.zeros(max(n, 1))
+        _check(lib().cb_get_step_events(self.h, _p(ev, PI), _p(psi)))
         out = st.as_dict()
-        out.update(step_t=tw[:n], step_judge=tj[:n], step_newton=nw[:n])
-        out.update(lpf=lpf[:n], itecnt=ite[:n], d=d[:n] if keep else None,
-                   msg=lib().cb_last_error().decode(errors="replace") if st.status else "")
+        out.update(step_t=tw[:n], step_judge=tj[:n], step_newton=nw[:n], events=ev[:n], psi=psi[:n])
+        out.update(lpf=lpf[:n], itecnt=ite[:n], d=d[:n] if keep else None, msg=msg)
         return out
+
+    def reactions(self, gen: int = GEN_BASE) -> np.ndarray:
+        """Support reactions [NJ, 7] of element-state generation gen (cb_get_reactions)."""
+        r = np.zeros(self.nj * 7)
+        _check(lib().cb_get_reactions(self.h, gen, _p(r)))
+        return r.reshape(self.nj, 7)
 
     def kernel_times(self, precision: int = 64, step: int = -1) -> dict:
         nf = len(KT_NAMES)

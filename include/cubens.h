@@ -104,8 +104,8 @@ int cb_get_load(const cb_model *m, double *q);
 int cb_state_set(cb_model *m, const char *name, const double *host, long n);
 int cb_state_get(cb_model *m, const char *name, double *host, long n);
 
-/* Tangent assembly: replaces ss = 0; stiff_tr/stiff_sh (truss.c:82,
- * shell.c:110).  gen_x / gen_state select the coordinates and element state:
+/* Tangent assembly: replaces ss = 0; stiff_tr/stiff_fr/stiff_sh (truss.c:82,
+ * frame.c:226, shell.c:110).  gen_x / gen_state select the coordinates and element state:
  * 0 converged, 1 current (_i / x_temp), 2 previous (_ip).  The MSAL call of
  * main.c:2717-2734 is (0,0); the NR call of main.c:1903-1921 is (1,2). */
 int cb_tangent(cb_model *m, int gen_x, int gen_state);
@@ -113,7 +113,8 @@ int cb_tangent(cb_model *m, int gen_x, int gen_state);
  * fixed DOFs carry identity). */
 int cb_get_tangent(cb_model *m, double *vals);
 
-/* Internal force: replaces forces_tr/forces_sh (truss.c:231, shell.c:1593).
+/* Internal force: replaces forces_tr/forces_fr/forces_sh (truss.c:231,
+ * frame.c:902, shell.c:1593).
  * f_host [NEQ] receives the force (accumulated from zero); dd_host [NEQ] is the
  * increment; reads ef_ip (clears its membrane entries as shell.c:1773 does) and
  * writes ef_i; coordinates of generation gen_x. */
@@ -137,8 +138,15 @@ int cb_factor(cb_model *m, const cb_options *opt, int *det, double *pivots);
 /* Solve with the last factorization / preconditioner (skysolve, solve.c:655). */
 int cb_solve(cb_model *m, const double *rhs_host, double *x_host);
 
-/* Runs the driver of the model's ALGFLAG: NR / modified NR (main.c:1800-2157)
- * or modified spherical arc length (main.c:2158-3141). */
+/* Runs the driver of the model's ALGFLAG: NR / modified NR (main.c:1800-2157),
+ * modified spherical arc length (main.c:2158-3141) or nonlinear Newmark
+ * (main.c:3305-3960).  Returns 0 when the driver ran to its end or to
+ * max_steps; a positive driver status when it stopped as the reference's
+ * driver stops (1: an error the reference prints to results1.txt and exits on,
+ * e.g. main.c:3041 "maximum allowable number of subdivisions exceeded";
+ * 2: no driver for this ALGFLAG; 3: minimum time increment reached,
+ * main.c:3841), with cb_last_error() the message and the path up to there
+ * kept; a negative code on an argument or device error.  st may be NULL. */
 int cb_run(cb_model *m, const cb_options *opt, cb_stats *st);
 /* Load-displacement path of the last run, the triple misc.c:414-417 writes
  * to results2.txt: lpf [nsteps], itecnt [nsteps], d [nsteps*NEQ] (full
@@ -154,7 +162,31 @@ int cb_get_step_stats(const cb_model *m, double *t_wall, double *t_judge, long *
  * 64) or FP32 (32) factor; step >= 0 returns the totals at that converged step
  * of the last run, step < 0 the totals now (no reference counterpart). */
 int cb_get_kernel_times(const cb_model *m, int precision, int step, double *ms, long *launches);
-/* Writes the results2.txt rows of the last run (format of misc.c:350-417). */
+/* Support reactions r [NJ*7] (joint-major, directions 1..7 as jcode) of the
+ * element state of generation gen (CB_GEN_BASE after a run: the last converged
+ * step): the element forces T_i^T ef_i that forces_tr/forces_fr/forces_sh
+ * accumulate into f (truss.c:349-372, frame.c:1273-1309, shell.c:1728-1785),
+ * summed at the constrained slots (jcode 0, where those routines skip the
+ * scatter); zero at free slots.  In the reference's sign convention of f, so
+ * at equilibrium sum(r) + lambda sum(q) = 0 per direction.  No reference
+ * counterpart: the reference computes no reactions. */
+int cb_get_reactions(cb_model *m, int gen, double *r);
+
+/* Per converged step of the last run: driver events [nsteps*CB_NEV] in the
+ * CB_EV_* slots and the arc-length psi the step was taken with [nsteps]
+ * (either pointer may be NULL).  The events are the branches of the
+ * reference's drivers taken within the step: the pivot-sign flag det of the
+ * step's last factorization (solve.c:608-621, which sets the predictor's sign,
+ * main.c:2765-2769), quad's imaginary roots (arc.c:145) and its no-positive-
+ * gamma return (arc.c:134), the dnorm restart (main.c:2936-2953) and the
+ * halvings after itemax iterations (main.c:3000-3012; the load step of
+ * main.c:2096-2110 and the time step of main.c:3826-3846 for the other
+ * drivers).  No reference counterpart: the reference keeps no such record. */
+enum { CB_EV_DET = 0, CB_EV_IMAG = 1, CB_EV_NEG = 2, CB_EV_DNORM = 3, CB_EV_HALVE = 4,
+       CB_NEV = 5 };
+int cb_get_step_events(const cb_model *m, int *events, double *psi);
+/* Writes the results2.txt rows of the last run (format of misc.c:350-417);
+ * fails when the run kept no displacement path (keep_d = 0). */
 int cb_write_results2(const cb_model *m, const char *path);
 
 /* ---- learned preconditioner (no reference counterpart) ------------------
