@@ -280,7 +280,6 @@ Batch::Batch(const std::vector<std::string>& paths) {
         }
     }
     build_pattern(A);
-    if (A.col_ptr_fr.size() > 65) throw std::runtime_error("member colouring exceeded 64 classes");
     dev_create(d_, A);
     // owners and the equations of each model's last tile that no slot maps to
     std::vector<int32_t> oq(static_cast<size_t>(n_)), oj(static_cast<size_t>(A.nj)),

@@ -463,19 +463,29 @@ void build_pattern(Model& m) {
     fill_blocks(m.nsh, 3, m.ntr * 2 + m.nfr * 2, m.blk_sh);
 
     // Greedy colouring in element order: an element takes the smallest colour not
-    // used by any element already coloured at one of its joints.
+    // used by any element already coloured at one of its joints.  The colour set
+    // of a joint is a bitset of as many 64-bit words as the colours need, so a
+    // joint of any valence (a hub of many bars) gets distinct colours for all
+    // its elements.
     auto colour = [&](long ne, int nn, long ioff, std::vector<int32_t>& ptr, std::vector<int32_t>& els) {
-        std::vector<uint64_t> used(size_t(m.nj), 0);   // colour bitmask per joint
+        std::vector<std::vector<uint64_t>> used(size_t(m.nj));   // colour bitset per joint
         std::vector<std::vector<int32_t>> cls;
+        std::vector<uint64_t> busy;
         for (long e = 0; e < ne; ++e) {
-            uint64_t busy = 0;
-            for (int a = 0; a < nn; ++a) busy |= used[m.minc[ioff + e * nn + a] - 1];
-            int c = 0;
-            while (c < 64 && (busy >> c & 1)) ++c;
-            if (c >= 64) c = 63;   // never reached for the meshes built here
-            if (size_t(c) >= cls.size()) cls.resize(size_t(c) + 1);
+            busy.assign(cls.size() / 64 + 1, 0);
+            for (int a = 0; a < nn; ++a) {
+                const auto& u = used[size_t(m.minc[ioff + e * nn + a] - 1)];
+                for (size_t w = 0; w < u.size(); ++w) busy[w] |= u[w];
+            }
+            size_t c = 0;
+            while (busy[c / 64] >> (c % 64) & 1) ++c;   // busy has a free bit: cls.size() is free
+            if (c >= cls.size()) cls.resize(c + 1);
             cls[c].push_back(int32_t(e));
-            for (int a = 0; a < nn; ++a) used[m.minc[ioff + e * nn + a] - 1] |= uint64_t(1) << c;
+            for (int a = 0; a < nn; ++a) {
+                auto& u = used[size_t(m.minc[ioff + e * nn + a] - 1)];
+                if (u.size() <= c / 64) u.resize(c / 64 + 1, 0);
+                u[c / 64] |= uint64_t(1) << (c % 64);
+            }
         }
         ptr.assign(1, 0);
         els.clear();
