@@ -116,6 +116,11 @@ void launch_frame_tangent_band(Dev& d, int g, double* tiles, int w, int* bad);
 void launch_frame_forces(Dev& d, double* f, const double* dd);
 void launch_frame_update(Dev& d, int gx);
 
+// ---- reactions (reactions.hip) ----------------------------------------------
+// rx[NJ*7] <- the element forces T^T ef of generation g summed at every
+// constrained joint slot (jcode == 0), zero at the free slots.
+void launch_reactions(Dev& d, int g, double* rx);
+
 // ---- nonlinear Newmark (newmark.hip) -------------------------------------------
 // sm += lumped masses of element type (0 truss, 1 frame, 2 shell) at the
 // converged coordinates; overwrites the working lengths / areas.

@@ -1,0 +1,121 @@
+// Support reactions of the converged state.
+//
+// The reference computes none.  They are the internal force vector its
+// element routines accumulate into f, T_i^T ef_i summed over the elements
+// (truss.c:349-372, frame.c:1273-1309, shell.c:1728-1785), read at the slots
+// of the constrained DOFs, where forces_* skip the scatter (mcode == 0).  Every
+// element's end forces are self-equilibrated, so at an equilibrium state the
+// reactions balance the applied load: sum(reactions) + lambda sum(q) = 0 per
+// translational direction, up to the residual the convergence test allowed.
+//
+// One thread per element, one launch per colour class (elements of a class
+// share no joint), so the joint sums are atomic-free and deterministic.
+#include "dev.h"
+
+namespace cb {
+namespace {
+
+constexpr int TPB = 256;
+inline int nblk(long n) { return int((n + TPB - 1) / TPB); }
+
+// truss.c:349-372: f -= ef_end * c at both ends.
+__global__ void k_react_truss(const int32_t* __restrict__ els, int n, const int32_t* __restrict__ minc,
+                              const double* __restrict__ c1, const double* __restrict__ c2,
+                              const double* __restrict__ c3, const double* __restrict__ ef,
+                              double* __restrict__ rx) {
+    const int it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= n) return;
+    const long e = els[it];
+    const double c[3] = {c1[e], c2[e], c3[e]};
+    for (int a = 0; a < 2; ++a) {
+        const long j = minc[e * 2 + a] - 1;
+        const double v = ef[e * 2 + a];
+        for (int k = 0; k < 3; ++k) rx[j * 7 + k] -= v * c[k];
+    }
+}
+
+// Frame end forces in the member triad, 14 per member: per end the force
+// triple, the moment triple and the warping bimoment; T is block diagonal in
+// the triad with 1 on the warping entries (frame.c:286-295).
+__global__ void k_react_frame(const int32_t* __restrict__ els, int n, const int32_t* __restrict__ minc,
+                              const double* __restrict__ c1, const double* __restrict__ c2,
+                              const double* __restrict__ c3, const double* __restrict__ ef,
+                              double* __restrict__ rx, long ntr) {
+    const int it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= n) return;
+    const long e = els[it];
+    const long t = ntr + e * 3;
+    const double R[3][3] = {{c1[t], c1[t + 1], c1[t + 2]},
+                            {c2[t], c2[t + 1], c2[t + 2]},
+                            {c3[t], c3[t + 1], c3[t + 2]}};
+    const double* fe = ef + ntr * 2 + e * 14;
+    for (int a = 0; a < 2; ++a) {
+        const long j = minc[ntr * 2 + e * 2 + a] - 1;
+        for (int h = 0; h < 2; ++h)
+            for (int m = 0; m < 3; ++m) {
+                const int b = 7 * a + 3 * h;
+                rx[j * 7 + 3 * h + m] += R[0][m] * fe[b] + R[1][m] * fe[b + 1] + R[2][m] * fe[b + 2];
+            }
+        rx[j * 7 + 6] += fe[7 * a + 6];
+    }
+}
+
+// Shell forces in the element triad, 6 per vertex (force and moment triples),
+// T block diagonal in the triad (shell.c:1775-1785).
+__global__ void k_react_shell(const int32_t* __restrict__ els, int n, const int32_t* __restrict__ minc,
+                              const double* __restrict__ c1, const double* __restrict__ c2,
+                              const double* __restrict__ c3, const double* __restrict__ ef,
+                              double* __restrict__ rx, long ioff, long toff, long eoff) {
+    const int it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= n) return;
+    const long e = els[it];
+    const long t = toff + e * 3;
+    const double R[3][3] = {{c1[t], c1[t + 1], c1[t + 2]},
+                            {c2[t], c2[t + 1], c2[t + 2]},
+                            {c3[t], c3[t + 1], c3[t + 2]}};
+    const double* fe = ef + eoff + e * 18;
+    for (int a = 0; a < 3; ++a) {
+        const long j = minc[ioff + e * 3 + a] - 1;
+        for (int h = 0; h < 2; ++h)
+            for (int m = 0; m < 3; ++m) {
+                const int b = 6 * a + 3 * h;
+                rx[j * 7 + 3 * h + m] += R[0][m] * fe[b] + R[1][m] * fe[b + 1] + R[2][m] * fe[b + 2];
+            }
+    }
+}
+
+// keep the constrained slots (jcode == 0), clear the others
+__global__ void k_react_mask(const int32_t* __restrict__ jcode7, double* __restrict__ rx, long n) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n && jcode7[i] != 0) rx[i] = 0.0;
+}
+
+}  // namespace
+
+void launch_reactions(Dev& d, int g, double* rx) {
+    const long ns = d.nj * 7;
+    CB_CHECK(hipMemsetAsync(rx, 0, sizeof(double) * size_t(ns), d.s));
+    for (size_t c = 0; c + 1 < d.col_ptr_tr.size(); ++c) {
+        const int n = d.col_ptr_tr[c + 1] - d.col_ptr_tr[c];
+        if (n)
+            hipLaunchKernelGGL(k_react_truss, dim3(nblk(n)), dim3(TPB), 0, d.s, d.col_el_tr + d.col_ptr_tr[c], n,
+                               d.minc, d.c1[g], d.c2[g], d.c3[g], d.ef[g], rx);
+    }
+    for (size_t c = 0; c + 1 < d.col_ptr_fr.size(); ++c) {
+        const int n = d.col_ptr_fr[c + 1] - d.col_ptr_fr[c];
+        if (n)
+            hipLaunchKernelGGL(k_react_frame, dim3(nblk(n)), dim3(TPB), 0, d.s, d.col_el_fr + d.col_ptr_fr[c], n,
+                               d.minc, d.c1[g], d.c2[g], d.c3[g], d.ef[g], rx, d.ntr);
+    }
+    const long io = d.ntr * 2 + d.nfr * 2, to = d.ntr + d.nfr * 3, eo = d.ntr * 2 + d.nfr * 14;
+    for (size_t c = 0; c + 1 < d.col_ptr_sh.size(); ++c) {
+        const int n = d.col_ptr_sh[c + 1] - d.col_ptr_sh[c];
+        if (n)
+            hipLaunchKernelGGL(k_react_shell, dim3(nblk(n)), dim3(TPB), 0, d.s, d.col_el_sh + d.col_ptr_sh[c], n,
+                               d.minc, d.c1[g], d.c2[g], d.c3[g], d.ef[g], rx, io, to, eo);
+    }
+    hipLaunchKernelGGL(k_react_mask, dim3(nblk(ns)), dim3(TPB), 0, d.s, d.jcode, rx, ns);
+    CB_CHECK(hipGetLastError());
+}
+
+}  // namespace cb

@@ -1,0 +1,62 @@
+"""Support reactions on the GPU (cb_get_reactions) against the oracle's
+restatement (orc_reactions) at the last converged step of the same path, and
+against statics: the reactions plus lambda q sum to zero per translational
+direction, up to the residual of the converged step."""
+import numpy as np
+import pytest
+
+import cubens
+import oracle as O
+from cubens import inputs
+from helpers import model_file, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _last_step(name, mt, steps):
+    path = model_file(name, mt.text)
+    print(f"[react] {name}: {steps} steps", flush=True)
+    with cubens.GpuModel(path) as g:
+        gp = g.run(cubens.options(cubens.SOLVER_BAND64, max_steps=steps))
+        rg = g.reactions()
+        # the same reactions from the GPU's own converged element state
+        nef = g.ntr * 2 + g.nfr * 14 + g.nsh * 18
+        ntri = g.ntr + 3 * g.nfr + 3 * g.nsh
+        om = O.OracleModel(path)
+        st = [g.get(k, n) for k, n in (("ef", nef), ("c1", ntri), ("c2", ntri), ("c3", ntri))]
+        r_state = om.reactions(*st)
+    op = om.run(steps, keep_react=True)
+    assert gp["nsteps"] == op["nsteps"] and np.array_equal(gp["itecnt"], op["itecnt"])
+    return om, gp, op, rg, r_state
+
+
+def _balance(om, lam, R):
+    jc = om.jcode.reshape(om.nj, 7)
+    out = []
+    for k in range(3):
+        eq = jc[:, k]
+        eq = eq[eq > 0] - 1
+        out.append(R[:, k].sum() + lam * om.q[eq].sum())
+    return np.abs(out).max()
+
+
+@pytest.mark.parametrize("case", ["dome_nr", "star_msal", "panel8_msal", "tower_nr"])
+def test_reactions_match_the_oracle_and_balance_the_load(case):
+    if case == "dome_nr":
+        mt, steps, tol = inputs.truss_dome(), 20, 1e-6
+    elif case == "star_msal":
+        mt, steps, tol = inputs.star_truss(), 30, 1e-9
+    elif case == "panel8_msal":
+        mt, steps, tol = inputs.shell_panel(n=8), 12, 2e-2
+    else:
+        mt, steps, tol = inputs.space_frame(n_elem=200, kind="tower", algflag=1, amp=2e3), 15, 1e-6
+    om, gp, op, rg, r_state = _last_step(f"react_{case}.txt", mt, steps)
+    ro = op["react"][-1]
+    # the kernels against the restatement on the same (GPU) state, bit-for-bit
+    # up to summation order
+    assert relerr(rg, r_state) <= 1e-12
+    # against the oracle's own path
+    assert relerr(rg, ro) <= 1e-9
+    lam = gp["lpf"][-1]
+    scale = abs(lam) * np.abs(om.q).sum()
+    assert _balance(om, lam, rg) <= tol * scale
