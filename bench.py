@@ -3,14 +3,22 @@ cylindrical shell panel (131072 triangles, 66049 joints, 394752 equations) on
 one MI355X.
 
 A step is one arc-length increment with all its Newton iterations (predictor
-included), tangent assembly and solver set-up included.  The first W
-increments (the displacement-controlled first increment among them) are warmup;
-the next K increments are timed.  Time spent in the exact FP64 pivot judge of
-a path that is not the FP64 factorization itself is subtracted, as it would be
-from every variant.
+included), tangent assembly and solver set-up included.  W warm-up increments
+run first as an analysis of their own (untimed); then a fresh analysis from the
+undeformed state runs and its first K increments are timed -- with the default
+K = 40 that is the window the metric is defined on (increments 1..40,
+the displacement-controlled first increment among them).  Time spent in the
+exact FP64 pivot judge of a path that is not the FP64 factorization itself is
+subtracted, as it would be from every variant.
 
-Paths: "fp64" = FP64 band LDL^T on the GPU (the plain FP64 GPU path);
-"mixed" = FGMRES with FP64 residual preconditioned by an FP32 band LDL^T.
+Paths: "fp64" = FP64 band LDL^T on the GPU (the plain FP64 GPU path, the
+baseline of the metric); "mixed" = FGMRES with FP64 residual preconditioned by
+an FP32 band LDL^T (the best non-plain path at this size).  The mixed path is
+timed over its own first --compare-steps increments beside the plain path's
+rate over the same increments, and their ratio is reported.  The learned
+preconditioner is not run at this size: its FGMRES needs thousands of
+iterations per solve here (it is measured on held-out panels up to 32x32 by
+--workload learned).
 """
 from __future__ import annotations
 
@@ -96,10 +104,25 @@ TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, KiB per dispa
                   "copy (x2.000, x1.000)")
 
 
-def roofline(info, ktimes: dict, default_workload: bool) -> dict:
+def skyline_work(kht: np.ndarray) -> dict:
+    """Algorithmic work of the reference's skyline solver on this profile:
+    skyfact sum(kht^2) flops (one multiply-add per column pair below the
+    skyline, counted as 2 flops over the half-square, solve.c:539-653) and
+    skysolve's read of the lss = sum(kht+1) factor entries once per sweep
+    (solve.c:655-698)."""
+    k = kht.astype(np.float64)
+    return {"factor_flops": float((k * k).sum()), "sweep_bytes": 8.0 * float((k + 1).sum())}
+
+
+def roofline(info, ktimes: dict, default_workload: bool, sky: dict | None = None) -> dict:
     name = max(ktimes, key=lambda k: ktimes[k][0])
     ms, launches = ktimes[name]
     bound, work, unit = kernel_model(info)[name]
+    nt = (info.neq + TILE - 1) // TILE
+    if sky is not None and name == "band_column":
+        work = sky["factor_flops"] / nt          # a factorization is nt column launches
+    elif sky is not None and name in ("band_fwd_sweep", "band_bwd_sweep"):
+        work = sky["sweep_bytes"]
     avg_s = ms / 1000.0 / max(launches, 1)
     if bound == "hbm":
         achieved, peak = work / avg_s / 1e9, PEAK_HBM_GBS
@@ -109,10 +132,34 @@ def roofline(info, ktimes: dict, default_workload: bool) -> dict:
     out = {"kernel": name, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
            "frac": achieved / peak, "traffic": traffic, "avg_launch_us": 1e6 * avg_s, "launches": launches,
            "share_of_kernel_time": ms / sum(v[0] for v in ktimes.values())}
+    if sky is not None and name == "band_column":
+        out["work_basis"] = "sum(kht^2) flops of the skyline factorization / nt launches"
     if traffic is not None:
         out["traffic_unit"] = "B per launch"
         out["traffic_source"] = TRAFFIC_SOURCE
+    if sky is not None:
+        # the two sweeps of every solve: algorithmic bytes (the skyline factor
+        # read once per sweep) over their average duration, against HBM
+        sw = [ktimes[k] for k in ("band_fwd_sweep", "band_bwd_sweep") if ktimes.get(k, (0, 0))[1]]
+        if sw:
+            per = sum(v[0] for v in sw) / 1000.0 / sum(v[1] for v in sw)
+            out["sweeps"] = {"avg_sweep_ms": 1e3 * per, "bytes": sky["sweep_bytes"],
+                             "achieved": sky["sweep_bytes"] / per / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                             "frac": sky["sweep_bytes"] / per / 1e9 / PEAK_HBM_GBS}
     return out
+
+
+def host_cpu() -> dict:
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
 
 
 def cpu_baseline(path: str, newton: int, increments: int, budget: float) -> dict:
@@ -125,7 +172,7 @@ def cpu_baseline(path: str, newton: int, increments: int, budget: float) -> dict
     per_incr = smp["t_tangent"] + smp["t_factor_est"] + smp["t_solve"]
     per_newton = smp["t_solve"] + smp["t_forces"]
     t = increments * per_incr + newton * per_newton
-    return {"value": newton / t, "unit": "Newton iters/s", "cores": 1, "kind": "port",
+    return {"value": newton / t, "unit": "Newton iters/s", "cores": 1, "kind": "port", "host": host_cpu(),
             "sample": (f"oracle (C restatement of the serial skyline path) on 1 core, same model: one tangent "
                        f"{smp['t_tangent']:.2f}s, one internal-force pass {smp['t_forces']:.2f}s, one skyline "
                        f"back-substitution {smp['t_solve']:.2f}s, skyline LDL^T of the first {smp['cols_done']} "
@@ -167,7 +214,7 @@ def frame_cpu_baseline(paths: list[str], steps: int, stride: int = 32) -> dict:
         r = om.run(steps)
         t += time.perf_counter() - t0
         newton += int(r["n_newton"])
-    return {"value": newton / t, "unit": "Newton iters/s", "cores": 1, "kind": "port",
+    return {"value": newton / t, "unit": "Newton iters/s", "cores": 1, "kind": "port", "host": host_cpu(),
             "sample": (f"oracle (C restatement of the serial Newmark driver with the skyline solver) on 1 core: "
                        f"{len(sample)} of the 512 models (one in {stride}), first {steps} time steps each, "
                        f"{newton} Newton iterations in {t:.1f}s")}
@@ -282,42 +329,53 @@ def run_learned(args) -> None:
 
 def run_path(path: str, solver: int, warmup: int, steps: int, tol: float, timing: int = 0,
              schedule: int = 0) -> dict:
+    """warmup increments of an untimed analysis, then the first `steps`
+    increments of a fresh analysis, timed from its start."""
     import cubens
     t0 = time.time()
     with cubens.GpuModel(path) as g:
         t_open = time.time() - t0
-        opt = cubens.options(solver, krylov_tol=tol, restart=30, maxit=400,
-                             judge=0 if solver == cubens.SOLVER_BAND64 else 1,
-                             max_steps=warmup + steps, keep_d=1, kernel_timing=timing,
-                             kernel_timing_every=TIMING_EVERY,
-                             factor_schedule=schedule)
+        _, _, maxa = g.dofmaps()
+        kht = np.diff(maxa) - 1
+        common = dict(krylov_tol=tol, restart=30, maxit=400,
+                      judge=0 if solver == cubens.SOLVER_BAND64 else 1, factor_schedule=schedule)
+        if warmup > 0:
+            g.run(cubens.options(solver, max_steps=warmup, keep_d=0, **common))
+        opt = cubens.options(solver, max_steps=steps, keep_d=1, kernel_timing=timing,
+                             kernel_timing_every=TIMING_EVERY, **common)
         r = g.run(opt)
         info = g.info
         kt = None
-        if timing and r["nsteps"] >= warmup + steps:
+        if timing and r["nsteps"] >= steps:
             prec = 64 if solver == cubens.SOLVER_BAND64 else 32
-            end = g.kernel_times(prec, warmup + steps - 1)
-            start = g.kernel_times(prec, warmup - 1) if warmup > 0 else {k: (0.0, 0) for k in end}
-            kt = {k: (end[k][0] - start[k][0], end[k][1] - start[k][1]) for k in end}
+            kt = g.kernel_times(prec, steps - 1)
     n = r["nsteps"]
-    if n < warmup + steps:
+    if n < steps:
         raise RuntimeError(f"run stopped after {n} steps: {r['msg']}")
     tw, tj, nw = r["step_t"], r["step_judge"], r["step_newton"]
-    a = warmup - 1
-    t = (tw[-1] - tj[-1]) - ((tw[a] - tj[a]) if a >= 0 else 0.0)
-    newton = int(nw[-1] - (nw[a] if a >= 0 else 0))
+    t = tw[steps - 1] - tj[steps - 1]
+    newton = int(nw[steps - 1])
     return {"rate": newton / t, "seconds": t, "newton": newton, "open_s": t_open, "stats": r,
             "neq": info.neq, "bandwidth": info.bandwidth, "lpf": r["lpf"], "d": r["d"], "info": info,
-            "ktimes": kt}
+            "ktimes": kt, "kht": kht, "step_t": tw - tj, "step_newton": nw}
+
+
+def window_rate(res: dict, k: int) -> float:
+    """Newton iterations/s of a timed path over its first k increments."""
+    return float(res["step_newton"][k - 1]) / float(res["step_t"][k - 1])
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40,
+                    help="timed arc-length increments from the start of the analysis (the metric: 40)")
+    ap.add_argument("--warmup", type=int, default=2, help="increments of an untimed warm-up analysis")
     ap.add_argument("--n", type=int, default=256, help="panel mesh (256 = the 396k-DOF workload)")
-    ap.add_argument("--paths", default="fp64", help="fp64 (headline) and/or mixed")
+    ap.add_argument("--paths", default="fp64,mixed",
+                    help="fp64 (the plain FP64 path, the headline) and/or mixed (FP32-factor FGMRES)")
+    ap.add_argument("--compare-steps", type=int, default=6,
+                    help="increments over which the mixed path is timed and compared with the plain one")
     ap.add_argument("--cpu-budget", type=float, default=8.0, help="seconds of partial CPU factorization")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
@@ -358,8 +416,10 @@ def main():
     schedule = cubens.FACTOR_SPLIT if args.schedule == "split" else cubens.FACTOR_FUSED
     for name in args.paths.split(","):
         solver = {"fp64": cubens.SOLVER_BAND64, "mixed": cubens.SOLVER_BAND32_FGMRES}[name]
-        log(f"[bench] path {name}: {args.warmup} warmup + {args.steps} timed increments")
-        res = run_path(path, solver, args.warmup, args.steps, args.tol, timing=63, schedule=schedule)
+        k = args.steps if name == "fp64" else min(args.compare_steps, args.steps)
+        w = args.warmup if name == "fp64" else min(args.warmup, 1)
+        log(f"[bench] path {name}: {w} warm-up increments, then increments 1..{k} timed")
+        res = run_path(path, solver, w, k, args.tol, timing=63, schedule=schedule)
         st = res["stats"]
         log(f"[bench] {name}: {res['rate']:.2f} Newton it/s ({res['newton']} its in {res['seconds']:.2f}s); "
             f"tangent {st['t_tangent']:.2f}s factor {st['t_factor']:.2f}s solve {st['t_solve']:.2f}s "
@@ -371,27 +431,38 @@ def main():
     extra = {}
     if plain is not None and "mixed" in results:
         m = results["mixed"]
-        extra["mixed_fp32_factor_value"] = m["rate"]
-        extra["mixed_vs_plain_fp64_ratio"] = m["rate"] / plain["rate"]
-        extra["mixed_vs_fp64_rel_disp_err"] = float(
-            np.abs(m["d"] - plain["d"]).max() / np.abs(plain["d"]).max())
+        km = len(m["step_t"])
+        base = window_rate(plain, km)
+        extra["compare"] = {
+            "path": "mixed: FGMRES with FP64 residual, FP32 band LDL^T preconditioner",
+            "increments": km, "value": m["rate"], "plain_fp64_same_increments": base,
+            "ratio_vs_plain_fp64": m["rate"] / base,
+            "rel_disp_err_vs_fp64": float(np.abs(m["d"] - plain["d"][:km]).max() / np.abs(plain["d"][:km]).max()),
+            "krylov_iters": int(m["stats"]["krylov_iters"]), "fallbacks": int(m["stats"]["krylov_fallbacks"]),
+            "judge_s_excluded": float(m["stats"]["t_judge"]),
+            "learned": "not run at this size (thousands of FGMRES iterations per solve); see --workload learned"}
+        log(f"[bench] mixed vs plain over increments 1..{km}: {m['rate']:.2f} / {base:.2f} = "
+            f"{m['rate'] / base:.3f}")
+    sky = skyline_work(head["kht"]) if head is plain else None
     # the counters were read on the 256x256 panel's FP64 band with the fused schedule
     roof = roofline(head["info"], head["ktimes"],
-                    default_workload=(args.n == 256 and plain is not None and args.schedule == "fused"))
+                    default_workload=(args.n == 256 and plain is not None and args.schedule == "fused"),
+                    sky=sky)
     log(f"[bench] kernel times (ms, launches): {head['ktimes']}")
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline: oracle sample on 1 core")
-        cpu = cpu_baseline(path, head["newton"], args.steps, args.cpu_budget)
+        cpu = cpu_baseline(path, head["newton"], len(head["step_t"]), args.cpu_budget)
         log(f"[bench] cpu: {cpu['value']:.4f} Newton it/s; {cpu['sample']}")
+    k = len(head["step_t"])
     out = {
         "metric": METRIC,
         "value": head["rate"],
         "unit": "Newton iters/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": k,
         "warmup": args.warmup,
-        "ms_per_step": 1000.0 * head["seconds"] / args.steps,
+        "ms_per_step": 1000.0 * head["seconds"] / k,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -399,8 +470,10 @@ def main():
         "data": "synthetic (seeded SplitMix64 thickness field and imperfection)",
         "config": {"workload": f"cylindrical shell panel {args.n}x{args.n} MSAL, "
                                f"{2 * args.n * args.n} CST+DKT triangles, NEQ {head['neq']}",
-                   "path": "fp64 band LDL^T" if head is plain else "fp32-factor FGMRES",
-                   "increments_timed": args.steps, "factor_schedule": args.schedule},
+                   "path": "plain fp64 band LDL^T (the metric's baseline path)" if head is plain
+                           else "fp32-factor FGMRES",
+                   "increments_timed": f"1..{k} of a fresh analysis after a {args.warmup}-increment warm-up run",
+                   "newton_iterations": head["newton"], "factor_schedule": args.schedule},
         "roofline": roof,
         "cpu_baseline": cpu,
         **extra,
