@@ -24,6 +24,11 @@ int fail(int code, const std::string& msg) {
 
 }  // namespace
 
+namespace cb {
+// the message cb_last_error() reports, for entry points defined in other files
+int set_last_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace cb
+
 struct cb_learned {
     std::unique_ptr<cb::Learned> net;
     // online training state
@@ -466,6 +471,21 @@ int cb_get_reactions(cb_model* h, int gen, double* r) {
     cb::d2h(r, rx, n, h->d.s);
     CB_CHECK(hipStreamSynchronize(h->d.s));
     (void)hipFree(rx);
+    return 0;
+    GUARD_END
+}
+
+int cb_dots(int n, const double* const* a, const double* const* b, long len, double* out) {
+    if (n < 1 || n > cb::DOTS_MAX) return fail(-1, "cb_dots: 1 to 8 products per call");
+    if (!a || !b || !out || len < 0) return fail(-1, "null argument or negative length");
+    for (int k = 0; k < n; ++k)
+        if (len > 0 && (!a[k] || !b[k])) return fail(-1, "null vector");
+    GUARD_BEGIN
+    double* buf = cb::dalloc<double>(size_t(cb::DOTS_PARTS + cb::DOTS_MAX));
+    cb::launch_dots(nullptr, n, a, b, len, buf, buf + cb::DOTS_PARTS);
+    cb::d2h(out, buf + cb::DOTS_PARTS, size_t(n), nullptr);
+    CB_CHECK(hipStreamSynchronize(nullptr));
+    (void)hipFree(buf);
     return 0;
     GUARD_END
 }

@@ -141,7 +141,16 @@ struct NewmarkArgs {
 void newmark_vec(Dev& d, NewmarkVec op, const NewmarkArgs& a);
 
 // ---- vectors -----------------------------------------------------------------
-// out[k] = sum_i a_k[i]*b_k[i], k < n (n <= 8), deterministic two-level reduction.
+// Fused dot products, deterministic two-level reduction: at most DOTS_MAX
+// products per pass, partial sums of at most DOTS_BLOCKS workgroups each.
+constexpr int DOTS_MAX = 8;
+constexpr int DOTS_BLOCKS = 512;
+constexpr long DOTS_PARTS = long(DOTS_MAX) * DOTS_BLOCKS;
+// out[k] (device) = sum_i a_k[i]*b_k[i], k < n <= DOTS_MAX, over device
+// vectors of length len, queued on stream s; part: DOTS_PARTS device doubles.
+void launch_dots(hipStream_t s, int n, const double* const* a, const double* const* b, long len, double* part,
+                 double* out);
+// the same on the model's stream, results to the host
 void multi_dot(Dev& d, int n, const double* const* a, const double* const* b, long len, double* out);
 void vfill(Dev& d, double* y, double v, long n);
 void vcopy(Dev& d, double* y, const double* x, long n);

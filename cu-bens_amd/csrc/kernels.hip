@@ -499,8 +499,8 @@ __global__ void k_cast32(float* __restrict__ y, const double* __restrict__ x, lo
 }
 
 // ---------------------------------------------------------------- vectors
-constexpr int DOT_BLOCKS = 512;
-constexpr int MAXDOT = 8;
+constexpr int DOT_BLOCKS = DOTS_BLOCKS;
+constexpr int MAXDOT = DOTS_MAX;
 
 struct DotArgs {
     const double* a[MAXDOT];
@@ -689,15 +689,19 @@ void copy_gen(Dev& d, int dst, int src, bool with_ef, bool with_x) {
     if (with_x) d2d(d.x[dst], d.x[src], d.nj * 3, d.s);
 }
 
-void multi_dot(Dev& d, int n, const double* const* a, const double* const* b, long len, double* out) {
+void launch_dots(hipStream_t s, int n, const double* const* a, const double* const* b, long len, double* part,
+                 double* out) {
     DotArgs args{};
     for (int k = 0; k < n; ++k) { args.a[k] = a[k]; args.b[k] = b[k]; }
     int nbk = int(std::min<long>(DOT_BLOCKS, std::max<long>(1, (len + TPB - 1) / TPB)));
-    hipLaunchKernelGGL(k_dot_partial, dim3(nbk), dim3(TPB), 0, d.s, args, n, len, d.red);
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(TPB), 0, d.s, d.red, nbk, n, d.red + MAXDOT * DOT_BLOCKS);
+    hipLaunchKernelGGL(k_dot_partial, dim3(nbk), dim3(TPB), 0, s, args, n, len, part);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(TPB), 0, s, part, nbk, n, out);
     CB_CHECK(hipGetLastError());
-    CB_CHECK(hipMemcpyAsync(d.red_host, d.red + MAXDOT * DOT_BLOCKS, sizeof(double) * n,
-                            hipMemcpyDeviceToHost, d.s));
+}
+
+void multi_dot(Dev& d, int n, const double* const* a, const double* const* b, long len, double* out) {
+    launch_dots(d.s, n, a, b, len, d.red, d.red + DOTS_PARTS);
+    CB_CHECK(hipMemcpyAsync(d.red_host, d.red + DOTS_PARTS, sizeof(double) * n, hipMemcpyDeviceToHost, d.s));
     CB_CHECK(hipStreamSynchronize(d.s));
     for (int k = 0; k < n; ++k) out[k] = d.red_host[k];
 }
@@ -810,7 +814,7 @@ void dev_create(Dev& d, const Model& m) {
     }
     d.K = dalloc<double>(size_t(d.nnzb * d.bs * d.bs));
     d.K32 = dalloc<float>(size_t(d.nnzb * d.bs * d.bs));
-    d.red = dalloc<double>(size_t(MAXDOT * DOT_BLOCKS + 64));
+    d.red = dalloc<double>(size_t(DOTS_PARTS + 64));
     CB_CHECK(hipHostMalloc(&d.red_host, sizeof(double) * 64, hipHostMallocDefault));
     dev_reset_state(d, m);
 }

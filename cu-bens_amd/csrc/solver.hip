@@ -217,9 +217,9 @@ template <class Op, int NOUT>
 void fused(Dev& d, const Op& op, long n, double* out) {
     int nb = int(std::min<long>(RB, std::max<long>(1, (n + TPB - 1) / TPB)));
     hipLaunchKernelGGL((k_fused_sums<Op, NOUT>), dim3(nb), dim3(TPB), 0, d.s, op, n, d.red);
-    hipLaunchKernelGGL(k_reduce_parts, dim3(NOUT), dim3(64), 0, d.s, d.red, nb, NOUT, d.red + 8 * 512);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(NOUT), dim3(64), 0, d.s, d.red, nb, NOUT, d.red + DOTS_PARTS);
     CB_CHECK(hipGetLastError());
-    CB_CHECK(hipMemcpyAsync(d.red_host, d.red + 8 * 512, sizeof(double) * NOUT, hipMemcpyDeviceToHost, d.s));
+    CB_CHECK(hipMemcpyAsync(d.red_host, d.red + DOTS_PARTS, sizeof(double) * NOUT, hipMemcpyDeviceToHost, d.s));
     CB_CHECK(hipStreamSynchronize(d.s));
     for (int k = 0; k < NOUT; ++k) out[k] = d.red_host[k];
 }

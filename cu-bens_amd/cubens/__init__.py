@@ -107,6 +107,7 @@ ABI = {
     "cb_run": (C.c_int, [C.c_void_p, C.POINTER(Options), C.POINTER(Stats)]),
     "cb_get_step_events": (C.c_int, [C.c_void_p, PI, PD]),
     "cb_get_reactions": (C.c_int, [C.c_void_p, C.c_int, PD]),
+    "cb_dots": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_long, PD]),
     "cb_get_path": (C.c_int, [C.c_void_p, PD, PI, PD]),
     "cb_get_step_stats": (C.c_int, [C.c_void_p, PD, PD, PL]),
     "cb_get_kernel_times": (C.c_int, [C.c_void_p, C.c_int, C.c_int, PD, PL]),
@@ -157,6 +158,17 @@ class CubensError(RuntimeError):
 def _check(rc: int):
     if rc != 0:
         raise CubensError(lib().cb_last_error().decode(errors="replace") or f"error {rc}")
+
+
+def dots(a_ptrs, b_ptrs, length: int) -> np.ndarray:
+    """cb_dots over device vectors given by their addresses (e.g. a torch
+    tensor's data_ptr()); returns the n dot products."""
+    n = len(a_ptrs)
+    A = (C.c_void_p * n)(*a_ptrs)
+    B = (C.c_void_p * n)(*b_ptrs)
+    out = np.zeros(n)
+    _check(lib().cb_dots(n, A, B, length, _p(out)))
+    return out
 
 
 class GpuModel:

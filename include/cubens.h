@@ -162,6 +162,15 @@ int cb_get_step_stats(const cb_model *m, double *t_wall, double *t_judge, long *
  * 64) or FP32 (32) factor; step >= 0 returns the totals at that converged step
  * of the last run, step < 0 the totals now (no reference counterpart). */
 int cb_get_kernel_times(const cb_model *m, int precision, int step, double *ms, long *launches);
+/* Fused dot products out[k] = sum_i a[k][i] * b[k][i], 1 <= n <= 8, over
+ * device vectors of length len on the current device: replaces dot
+ * (misc.c:252-262), which the arc-length driver calls about a dozen times per
+ * iteration (main.c:2873-2889) and test() per convergence check
+ * (misc.c:187-250).  One pass over the vectors with a deterministic two-level
+ * reduction (partial sums in a device buffer this call allocates); runs on
+ * the null stream and returns the results to the host. */
+int cb_dots(int n, const double *const *a, const double *const *b, long len, double *out);
+
 /* Support reactions r [NJ*7] (joint-major, directions 1..7 as jcode) of the
  * element state of generation gen (CB_GEN_BASE after a run: the last converged
  * step): the element forces T_i^T ef_i that forces_tr/forces_fr/forces_sh
