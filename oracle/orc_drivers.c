@@ -609,6 +609,8 @@ static int run_msal(run_t *R)
                     for (long i = 0; i < n; ++i)
                         dnorm += (R->d_temp[i] - R->dp[i]) * (R->d_temp[i] - R->dp[i]);
                     dnorm = sqrt(dnorm);
+                    if (getenv("ORC_VERBOSE"))
+                        fprintf(stderr, "dnorm ratio %.12e\n", dnorm / (100 * dnormallow));
                     if (dnorm > 100 * dnormallow) {
                         arc /= beta;
                         beta = dnormallow / dnorm;

@@ -157,7 +157,7 @@ int orc_test(long neq, const double *d_temp, const double *dd, const double *f_t
             up += (qtot[i] - fp[i]) * (qtot[i] - fp[i]);
             e += dd[i] * (qtot[i] - f_ip[i]);
         }
-        fprintf(stderr, "test: disp %.3e force %.3e energy %.3e\n", sqrt(dn) / sqrt(tn),
+        fprintf(stderr, "test: disp %.12e force %.12e energy %.12e\n", sqrt(dn) / sqrt(tn),
                 sqrt(ui) / sqrt(up), fabs(e / intener1));
     }
     if (toldisp < 1) {

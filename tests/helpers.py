@@ -55,3 +55,59 @@ def skyline_dense(ss: np.ndarray, maxa: np.ndarray) -> np.ndarray:
     K[r, c] = v
     K[c, r] = v
     return K
+
+
+class Hip:
+    """Device buffers and streams through the HIP runtime the library itself
+    links (libamdhip64.so.7 under /opt/rocm); torch bundles a runtime of its
+    own, which cannot share a process with this one."""
+
+    def __init__(self):
+        import ctypes as C
+        self.C = C
+        self.rt = C.CDLL("/opt/rocm/lib/libamdhip64.so.7")
+        self.rt.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.rt.hipFree.argtypes = [C.c_void_p]
+        self.rt.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.rt.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.rt.hipStreamDestroy.argtypes = [C.c_void_p]
+        self.rt.hipStreamSynchronize.argtypes = [C.c_void_p]
+        self.rt.hipDeviceSynchronize.argtypes = []
+        self.live = []
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed: hipError {rc}")
+
+    def alloc(self, nbytes: int) -> int:
+        p = self.C.c_void_p()
+        self._ok(self.rt.hipMalloc(self.C.byref(p), max(int(nbytes), 8)), "hipMalloc")
+        self.live.append(p.value)
+        return p.value
+
+    def upload(self, a: np.ndarray) -> int:
+        a = np.ascontiguousarray(a)
+        p = self.alloc(a.nbytes)
+        self._ok(self.rt.hipMemcpy(p, a.ctypes.data, a.nbytes, 1), "hipMemcpy H2D")
+        return p
+
+    def download(self, p: int, n: int, dtype=np.float64) -> np.ndarray:
+        out = np.empty(n, dtype)
+        self._ok(self.rt.hipMemcpy(out.ctypes.data, p, out.nbytes, 2), "hipMemcpy D2H")
+        return out
+
+    def stream(self) -> int:
+        s = self.C.c_void_p()
+        self._ok(self.rt.hipStreamCreate(self.C.byref(s)), "hipStreamCreate")
+        return s.value
+
+    def sync(self, s=None):
+        if s is None:
+            self._ok(self.rt.hipDeviceSynchronize(), "hipDeviceSynchronize")
+        else:
+            self._ok(self.rt.hipStreamSynchronize(s), "hipStreamSynchronize")
+
+    def free_all(self):
+        for p in self.live:
+            self.rt.hipFree(p)
+        self.live = []

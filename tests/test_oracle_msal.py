@@ -3,8 +3,10 @@ arc.c:41-157, by what the mathematics fixes rather than by the restatement:
 
 * a shallow star truss traced through both limit points and the flat
   position: the load factor at every converged increment equals the closed
-  form lambda(w) of the bar equilibrium (truss.c:327-331 for the bar force),
-  the apex stays on the axis and the path never reverses;
+  form lambda(w) of the bar equilibrium (truss.c:327-331 for the bar force)
+  to the force tolerance times the step's load increment (the criterion of
+  test(), misc.c:215-232), the apex stays on the axis and the path never
+  reverses;
 * the pivot-sign flag det (solve.c:608-621) of every increment equals the sign
   pattern of the closed-form apex tangent (truss.c:106-166) at the previous
   converged state, and the recorded psi (main.c:2632-2642) equals the
@@ -26,6 +28,12 @@ import pytest
 
 import oracle as O
 from cubens import inputs
+
+
+def closed_form_gap(lam, lc):
+    """|lambda - lambda_closed| over each step's load increment."""
+    dl = np.abs(np.diff(np.concatenate([[0.0], lam])))
+    return np.abs(lam - lc) / dl
 
 
 def _run(mt, steps=400):
@@ -76,19 +84,21 @@ def arc_residuals(om_params, r, q):
 
 
 STAR = dict(bars=4, radius=1.0, rise=0.1, emod=2e11, area=1e-4)
+# convergence tolerances (displacement, force, energy) with every test decision
+# of the traced paths well clear of round-off (see the module docstring)
+TOL = (1e-8, 1e-6, 1.0)
 
 
 def test_star_truss_follows_the_closed_form_through_both_limit_points():
-    mt = inputs.star_truss(**STAR)
+    mt = inputs.star_truss(**STAR, tol=TOL)
     om, r = _run(mt)
     assert r["status"] == 0, r["msg"]
     n = r["nsteps"]
     w = r["d"][:, 2]
     lam = r["lpf"]
     lc = inputs.star_truss_lambda(w, **STAR)
-    scale = np.abs(lc).max()
     # equilibrium along the whole path, to the force tolerance
-    assert np.abs(lam - lc).max() <= 1e-9 * scale
+    assert closed_form_gap(lam, lc).max() <= TOL[1]
     # the apex stays on the axis
     assert np.abs(r["d"][:, :2]).max() <= 1e-12
     # first increment: the control DOF is held at dk (arc.c:41-68)
@@ -107,7 +117,7 @@ def test_star_truss_follows_the_closed_form_through_both_limit_points():
 
 
 def test_star_truss_det_and_psi_match_the_closed_form_tangent():
-    mt = inputs.star_truss(**STAR)
+    mt = inputs.star_truss(**STAR, tol=TOL)
     om, r = _run(mt)
     w = r["d"][:, 2]
     ev, psi = r["events"], r["psi"]
@@ -133,7 +143,7 @@ def test_star_truss_det_and_psi_match_the_closed_form_tangent():
 
 
 def test_star_truss_converged_increments_satisfy_the_arc_constraint():
-    mt = inputs.star_truss(**STAR)
+    mt = inputs.star_truss(**STAR, tol=TOL)
     om, r = _run(mt)
     res = arc_residuals((1.0, 6, 0.1), r, om.q)
     assert len(res) == r["nsteps"] - 1
@@ -150,16 +160,19 @@ def test_soft_star_truss_dnorm_restart_and_subdivision_stop():
     iteopt lets a converged increment exceed 100 alpha |d_0|, which restarts it
     with a smaller beta (main.c:2936-2953); the run later stops with the
     reference's 'maximum allowable number of subdivisions exceeded'
-    (main.c:3038-3043)."""
+    (main.c:3038-3043).  In a displacement-only arc a converged increment is
+    sqrt(iteopt/itecnt) alpha |d_0| long, so iteopt is kept off the multiples
+    of 1e4 itecnt at which that length would sit exactly on the dnorm test's
+    threshold."""
     kw = dict(STAR, emod=2e4)
-    mt = inputs.star_truss(**kw, alpha=0.01, iteopt=30000)
+    mt = inputs.star_truss(**kw, alpha=0.01, iteopt=25000, dk=-0.005, tol=TOL)
     om, r = _run(mt)
     assert r["status"] == 1 and "subdivisions" in r["msg"]
     ev = r["events"]
-    assert ev[:, O.EV_DNORM].sum() >= 2
+    assert ev[:, O.EV_DNORM].sum() >= 2 and ev[:, O.EV_DET].sum() >= 1
     lc = inputs.star_truss_lambda(r["d"][:, 2], **kw)
-    assert np.abs(r["lpf"] - lc).max() <= 1e-9 * np.abs(lc).max()
-    res = arc_residuals((0.01, 30000, 0.1), r, om.q)
+    assert closed_form_gap(r["lpf"], lc).max() <= TOL[1]
+    res = arc_residuals((0.01, 25000, 0.1), r, om.q)
     assert res and max(res.values()) <= 1e-9
 
 
@@ -169,7 +182,7 @@ def test_truss_dome_crosses_its_limit_point_under_arc_length():
     negative (det = 1) and the load factor falls; every increment whose arc the
     path determines satisfies the constraint."""
     mt = inputs.truss_dome(algflag=3, load=-1.0, tol=(1e-6, 1e-4, 1.0),
-                           msal=dict(alpha=0.3, dk=-0.002))
+                           msal=dict(alpha=0.4, dk=-0.0015))
     om, r = _run(mt, steps=60)
     assert r["nsteps"] == 60, r["msg"]
     ev = r["events"]
@@ -177,7 +190,7 @@ def test_truss_dome_crosses_its_limit_point_under_arc_length():
     assert ev[:, O.EV_DET].sum() >= 5 and ev[:, O.EV_IMAG].sum() >= 1
     first_neg = int(np.argmax(ev[:, O.EV_DET] > 0))
     assert lam[:first_neg].max() > lam[first_neg:first_neg + 3].min()
-    res = arc_residuals((0.3, 6, 0.1), r, om.q)
+    res = arc_residuals((0.4, 6, 0.1), r, om.q)
     assert len(res) >= 50 and max(res.values()) <= 1e-8
     # residual equilibrium at each converged increment: the force criterion
     # of test() holds to its tolerance relative to the applied load
@@ -203,8 +216,8 @@ def test_shell_panel_stops_at_its_limit_point(n):
 
 def test_star_hub_of_70_bars_follows_the_closed_form():
     kw = dict(STAR, bars=70)
-    om, r = _run(inputs.star_truss(**kw))
+    om, r = _run(inputs.star_truss(**kw, tol=TOL))
     assert r["status"] == 0 and r["nsteps"] > 40
     lc = inputs.star_truss_lambda(r["d"][:, 2], **kw)
-    assert np.abs(r["lpf"] - lc).max() <= 1e-9 * np.abs(lc).max()
+    assert closed_form_gap(r["lpf"], lc).max() <= TOL[1]
     assert r["events"][:, O.EV_DET].sum() > 10
