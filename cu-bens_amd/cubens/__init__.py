@@ -108,6 +108,8 @@ ABI = {
     "cb_get_step_events": (C.c_int, [C.c_void_p, PI, PD]),
     "cb_get_reactions": (C.c_int, [C.c_void_p, C.c_int, PD]),
     "cb_dots": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_long, PD]),
+    "cb_dots_noalloc": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_long,
+                                  C.c_void_p, C.c_void_p]),
     "cb_get_path": (C.c_int, [C.c_void_p, PD, PI, PD]),
     "cb_get_step_stats": (C.c_int, [C.c_void_p, PD, PD, PL]),
     "cb_get_kernel_times": (C.c_int, [C.c_void_p, C.c_int, C.c_int, PD, PL]),
@@ -161,8 +163,8 @@ def _check(rc: int):
 
 
 def dots(a_ptrs, b_ptrs, length: int) -> np.ndarray:
-    """cb_dots over device vectors given by their addresses (e.g. a torch
-    tensor's data_ptr()); returns the n dot products."""
+    """cb_dots over device vectors given by their addresses (hipMalloc'd
+    through the HIP runtime this library links); returns the n dot products."""
     n = len(a_ptrs)
     A = (C.c_void_p * n)(*a_ptrs)
     B = (C.c_void_p * n)(*b_ptrs)

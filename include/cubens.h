@@ -171,6 +171,18 @@ int cb_get_kernel_times(const cb_model *m, int precision, int step, double *ms, 
  * the null stream and returns the results to the host. */
 int cb_dots(int n, const double *const *a, const double *const *b, long len, double *out);
 
+/* The fused dot products of cb_dots without allocating anything, for callers
+ * that cannot call hipMalloc (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid allocation).  No counterpart in the reference.
+ * a[k], b[k] and out [n] are device pointers on the current device; the work
+ * is queued on `stream` (a hipStream_t, NULL for the null stream) and the
+ * call returns without waiting.  At most 8 products per call (larger n is
+ * rejected): the partial sums live in a static 33 KB device array that each
+ * GPU the library runs on holds until the process ends and that every call on
+ * that GPU shares, so calls that overlap on one GPU race. */
+int cb_dots_noalloc(int n, const double *const *a, const double *const *b, long len, double *out,
+                    void *stream);
+
 /* Support reactions r [NJ*7] (joint-major, directions 1..7 as jcode) of the
  * element state of generation gen (CB_GEN_BASE after a run: the last converged
  * step): the element forces T_i^T ef_i that forces_tr/forces_fr/forces_sh
