@@ -73,6 +73,8 @@ def kernel_model(info) -> dict:
                period (the nt launches of a factorization timed as one span)
       sweeps : every off-diagonal factor tile and every Linv tile read once (8 B
                entries), the right-hand side read and the solution written
+      prep   : the sweeps' chain matrices, two tile products of 2 T^3 per
+               tile row
     """
     nt = (info.neq + TILE - 1) // TILE
     w = min((info.bandwidth + TILE - 1) // TILE, nt - 1)
@@ -87,6 +89,7 @@ def kernel_model(info) -> dict:
         "band_update": ("mfma", 2.0 * t3 * w * (w + 1) / 2, "TFLOP/s"),
         "band_fwd_sweep": ("hbm", float(sweep_bytes), "GB/s"),
         "band_bwd_sweep": ("hbm", float(sweep_bytes), "GB/s"),
+        "band_sweep_prep": ("mfma", 2 * 2.0 * t3 * nt, "TFLOP/s"),
     }
 
 
@@ -419,7 +422,7 @@ def main():
         k = args.steps if name == "fp64" else min(args.compare_steps, args.steps)
         w = args.warmup if name == "fp64" else min(args.warmup, 1)
         log(f"[bench] path {name}: {w} warm-up increments, then increments 1..{k} timed")
-        res = run_path(path, solver, w, k, args.tol, timing=63, schedule=schedule)
+        res = run_path(path, solver, w, k, args.tol, timing=127, schedule=schedule)
         st = res["stats"]
         log(f"[bench] {name}: {res['rate']:.2f} Newton it/s ({res['newton']} its in {res['seconds']:.2f}s); "
             f"tangent {st['t_tangent']:.2f}s factor {st['t_factor']:.2f}s solve {st['t_solve']:.2f}s "

@@ -10,7 +10,9 @@ namespace cb {
 // Kernel families of the band solver whose launches can be timed with HIP
 // events: KT_COLUMN is the fused per-column factorization launch; KT_DIAG,
 // KT_PANEL and KT_UPDATE are the launches of the split schedule.
-enum { KT_COLUMN = 0, KT_DIAG = 1, KT_PANEL = 2, KT_UPDATE = 3, KT_FWD = 4, KT_BWD = 5, KT_N = 6 };
+// KT_PREP times the launch that forms the sweeps' chain matrices after a
+// factorization.
+enum { KT_COLUMN = 0, KT_DIAG = 1, KT_PANEL = 2, KT_UPDATE = 3, KT_FWD = 4, KT_BWD = 5, KT_PREP = 6, KT_N = 7 };
 
 // Launches of a selected family are bracketed by two events.  The fused
 // factorization's nt launches are bracketed as one span (its launch period:
@@ -35,7 +37,7 @@ struct KernelClock {
     bool take(int f) {
         if (!on(f)) return false;
         const long k = launches[f]++;
-        const long e = (f == KT_FWD || f == KT_BWD || every < 1) ? 1 : every;
+        const long e = (f == KT_FWD || f == KT_BWD || f == KT_PREP || every < 1) ? 1 : every;
         return k % e == 0;
     }
     void begin(int f, hipStream_t s, long n = 1);
@@ -54,6 +56,11 @@ struct BandLDLT {
     R* tiles_t = nullptr;    // the factor's off-diagonal tiles transposed (row-major), for the backward sweep
     R* linv = nullptr;
     double* piv = nullptr;   // pivots D (padded to nt*T)
+    // chain matrices of the sweeps, formed after each factorization: M_I =
+    // Linv_I L_{I,I-1} and G_I = L_{I+1,I} Linv_I, [nt][T*T] each
+    R* mch = nullptr;
+    R* gch = nullptr;
+    bool chained = true;     // sweeps with the chain matrices (default) or the two-product row step
     double* xw = nullptr;    // work vectors (padded)
     double* yw = nullptr;
     int* flag = nullptr;       // error bits: 1 a wait bound hit, 2 an entry outside the band
@@ -71,7 +78,7 @@ struct BandLDLT {
     int factor(const Dev& d);
     // out = (L D L^T)^{-1} rhs, both device NEQ vectors (may alias).
     void solve(const double* rhs, double* out);
-    size_t bytes() const { return 2 * sizeof(R) * size_t(nt) * size_t(w + 1) * 64 * 64; }
+    size_t bytes() const { return sizeof(R) * size_t(nt) * size_t(2 * (w + 1) + 2) * 64 * 64; }
 };
 
 // A batch of independent models sharing one FP64 band: model b owns the tile

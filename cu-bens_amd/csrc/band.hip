@@ -962,6 +962,198 @@ __global__ void __launch_bounds__(256) k_bwd_sweep(const R* __restrict__ tiles_t
     if (tid < T) put_value(&x[I * T + tid], part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid]);
 }
 
+// ---- sweeps with a precomputed chain matrix ----------------------------------
+// In the sweeps above the last step of a tile row waits for its predecessor
+// and then applies two dependent 64x64 products (L_{I,I-1} y_{I-1}, then Linv_I).
+// After each factorization k_sweep_chain forms per tile row
+//   M_I = Linv_I L_{I,I-1}            (column-major, for the forward sweep)
+//   G_I = L_{I+1,I} Linv_I            (row-major, for the backward sweep)
+// so that y_I = U_I - M_I y_{I-1}, U_I = Linv_I (b_I - sum_{J <= I-2} L_IJ y_J),
+// and x_I = U'_I - G_I^T x_{I+1}, U'_I = Linv_I^T (y_I / D_I - sum_{J >= I+2} L_JI^T x_J).
+// U_I needs the rows up to I-2 only, which arrive one chain step before
+// y_{I-1}; once y_{I-1} lands the row takes one 64x64 product and one
+// reduction.  Each workgroup still reads only its own tile row (the older
+// contributions' bandwidth per workgroup is that of the sweeps above).
+
+// tile (I, J) of the factor, I > J, read as doubles; zero outside the band
+template <class R> struct TileRd {
+    const R* p;
+    __device__ double operator()(int r, int c) const { return p ? double(p[c * T + r]) : 0.0; }
+};
+template <class R> __device__ inline TileRd<R> factor_tile(const R* tiles, long I, long J, long nt, int w) {
+    if (J < 0 || I >= nt || I - J > w || I - J < 1) return TileRd<R>{nullptr};
+    return TileRd<R>{tiles + tile_index(I, J, w) * TT};
+}
+template <class R> __device__ inline TileRd<R> linv_tile(const R* linv, long I, long nt) {
+    return TileRd<R>{(I >= 0 && I < nt) ? linv + I * TT : nullptr};
+}
+
+// acc[j] += sum_k A(r, k) B(k, c0 + j), j < 16 (B's reads are uniform over a wavefront)
+template <class FA, class FB> __device__ inline void mm64(const FA& A, const FB& B, int r, int c0, double* acc) {
+    for (int k = 0; k < T; ++k) {
+        const double a = A(r, k);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) acc[j] += a * B(k, c0 + j);
+    }
+}
+
+// M_I and G_I for every tile row: one workgroup per row, thread (r, c0)
+// forming entries (r, c0..c0+15) of each product.
+template <class R>
+__global__ void __launch_bounds__(256) k_sweep_chain(const R* __restrict__ tiles, const R* __restrict__ linv,
+                                                     R* __restrict__ mch, R* __restrict__ gch, long nt, int w) {
+    const long I = blockIdx.x;
+    const int r = threadIdx.x % T, c0 = 16 * (threadIdx.x / T);
+    double acc[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = 0.0;
+    mm64(linv_tile(linv, I, nt), factor_tile(tiles, I, I - 1, nt, w), r, c0, acc);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) mch[I * TT + (c0 + j) * T + r] = R(acc[j]);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = 0.0;
+    mm64(factor_tile(tiles, I + 1, I, nt, w), linv_tile(linv, I, nt), r, c0, acc);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) gch[I * TT + r * T + c0 + j] = R(acc[j]);
+}
+
+// y_I = U_I - M_I y_{I-1}.  r is read from xr, whose slot I is set to SENT for
+// the backward sweep; y must hold SENT.
+template <class R>
+__global__ void __launch_bounds__(256) k_fwd_sweep_m(const R* __restrict__ tiles, const R* __restrict__ linv,
+                                                     const R* __restrict__ mch, double* __restrict__ xr, double* y,
+                                                     long nt, int w, long spin_limit, int* err) {
+    __shared__ double vin[T];
+    __shared__ double vch[NCH][T];
+    __shared__ double part[4][T];
+    const long I = blockIdx.x;
+    const int tid = threadIdx.x, row = tid % T, q = tid / T, k0 = 16 * q;
+    R lv[16], lm[16];
+    const R* lvp = linv + I * (long)TT;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) lv[k] = lvp[(k0 + k) * T + row];
+    const bool has_prev = I >= 1;
+    if (has_prev) {
+        const R* lmp = mch + I * (long)TT;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) lm[k] = lmp[(k0 + k) * T + row];
+    }
+    double acc = 0.0;
+    if (q == 0) {
+        acc = xr[I * T + row];
+        xr[I * T + row] = __builtin_bit_cast(double, SENT);
+    }
+    // the older rows J <= I - 2 in chunks of NCH, wavefront c polling row J + c
+    const long J0 = I - w > 0 ? I - w : 0;
+    for (long J = J0; J < I - 1; J += NCH) {
+        const int nc = (I - 1 - J < NCH) ? int(I - 1 - J) : NCH;
+        R lx[NCH][16];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            if (c < nc) {
+                const R* l = tiles + (I * (w + 1) + (J + c - I + w)) * (long)TT;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) lx[c][k] = l[(k0 + k) * T + row];
+            }
+        fetch_rows(y, J, 1, nc, vch, spin_limit, err);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            if (c < nc) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) acc -= double(lx[c][k]) * vch[c][k0 + k];
+            }
+        __syncthreads();
+    }
+    // U_I = Linv_I S_I, formed before y_{I-1} is needed
+    part[q][row] = acc;
+    __syncthreads();
+    if (tid < T) vin[tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+    __syncthreads();
+    double u = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) u += double(lv[k]) * vin[k0 + k];
+    __syncthreads();
+    part[q][row] = u;
+    __syncthreads();
+    double yi = (q == 0) ? part[0][row] + part[1][row] + part[2][row] + part[3][row] : 0.0;
+    if (has_prev) {
+        fetch_row(y, I - 1, vin, spin_limit, err);
+        double m = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) m += double(lm[k]) * vin[k0 + k];
+        part[q][row] = m;
+        __syncthreads();
+        if (q == 0) yi -= part[0][row] + part[1][row] + part[2][row] + part[3][row];
+    }
+    if (q == 0) put_value(&y[I * T + row], yi);
+}
+
+// x_I = U'_I - G_I^T x_{I+1}; workgroup g handles I = nt-1-g.  x must hold SENT.
+template <class R>
+__global__ void __launch_bounds__(256) k_bwd_sweep_m(const R* __restrict__ tiles_t, const R* __restrict__ linv,
+                                                     const R* __restrict__ gch, const double* __restrict__ y,
+                                                     const double* __restrict__ piv, double* x, long nt, int w,
+                                                     long spin_limit, int* err) {
+    __shared__ double vin[T];
+    __shared__ double vch[NCH][T];
+    __shared__ double part[4][T];
+    const long I = nt - 1 - blockIdx.x;
+    const int tid = threadIdx.x, col = tid % T, q = tid / T, k0 = 16 * q;
+    R lv[16], lm[16];
+    const R* lvp = linv + I * (long)TT;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) lv[k] = lvp[col * T + k0 + k];   // Linv[k][col]
+    const bool has_next = I + 1 < nt;
+    if (has_next) {
+        const R* lgp = gch + I * (long)TT;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) lm[k] = lgp[(k0 + k) * T + col];   // G_I[k][col]
+    }
+    double acc = (q == 0) ? y[I * T + col] / piv[I * T + col] : 0.0;
+    // the later rows J >= I + 2 in chunks of NCH, wavefront c polling row J - c
+    const long J1 = (I + w < nt - 1) ? I + w : nt - 1;
+    for (long J = J1; J > I + 1; J -= NCH) {
+        const int nc = (J - (I + 1) < NCH) ? int(J - (I + 1)) : NCH;
+        R lx[NCH][16];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            if (c < nc) {
+                const R* l = tiles_t + ((J - c) * (w + 1) + (I - (J - c) + w)) * (long)TT;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) lx[c][k] = l[(k0 + k) * T + col];
+            }
+        fetch_rows(x, J, -1, nc, vch, spin_limit, err);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            if (c < nc) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) acc -= double(lx[c][k]) * vch[c][k0 + k];
+            }
+        __syncthreads();
+    }
+    part[q][col] = acc;
+    __syncthreads();
+    if (tid < T) vin[tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+    __syncthreads();
+    double u = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) u += double(lv[k]) * vin[k0 + k];
+    __syncthreads();
+    part[q][col] = u;
+    __syncthreads();
+    double xi = (q == 0) ? part[0][col] + part[1][col] + part[2][col] + part[3][col] : 0.0;
+    if (has_next) {
+        fetch_row(x, I + 1, vin, spin_limit, err);
+        double m = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) m += double(lm[k]) * vin[k0 + k];
+        part[q][col] = m;
+        __syncthreads();
+        if (q == 0) xi -= part[0][col] + part[1][col] + part[2][col] + part[3][col];
+    }
+    if (q == 0) put_value(&x[I * T + col], xi);
+}
+
 // dst = rhs padded with zeros to ntot; y = SENT
 __global__ void k_sweep_prep(double* __restrict__ dst, const double* __restrict__ src, double* __restrict__ y,
                              long n, long ntot) {
@@ -1416,6 +1608,8 @@ void BandLDLT<R>::setup(long n_, long bandwidth, hipStream_t s_) {
     tiles = dalloc<R>(size_t(nt) * size_t(w + 1) * TT);
     tiles_t = dalloc<R>(size_t(nt) * size_t(w + 1) * TT);
     linv = dalloc<R>(size_t(nt) * TT);
+    mch = dalloc<R>(size_t(nt) * TT);
+    gch = dalloc<R>(size_t(nt) * TT);
     piv = dalloc<double>(size_t(nt) * T);
     xw = dalloc<double>(size_t(nt) * T);
     yw = dalloc<double>(size_t(nt) * T);
@@ -1430,10 +1624,11 @@ void BandLDLT<R>::setup(long n_, long bandwidth, hipStream_t s_) {
 
 template <class R>
 void BandLDLT<R>::release() {
-    void* ps[] = {tiles, tiles_t, linv, piv, xw, yw, flag, pflags};
+    void* ps[] = {tiles, tiles_t, linv, mch, gch, piv, xw, yw, flag, pflags};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     tiles = nullptr; tiles_t = nullptr; linv = nullptr; piv = nullptr; xw = nullptr; yw = nullptr; flag = nullptr;
+    mch = nullptr; gch = nullptr;
     pflags = nullptr;
     clock.release();
 }
@@ -1492,6 +1687,13 @@ int BandLDLT<R>::factor(const Dev& d) {
         }
     }
     CB_CHECK(hipGetLastError());
+    if (chained) {
+        const bool tk_prep = clock.take(KT_PREP);
+        if (tk_prep) clock.begin(KT_PREP, s);
+        hipLaunchKernelGGL((k_sweep_chain<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, mch, gch, nt, w);
+        if (tk_prep) clock.end(s);
+        CB_CHECK(hipGetLastError());
+    }
     int bad = 0;
     CB_CHECK(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, s));
     CB_CHECK(hipMemcpyAsync(piv_host.data(), piv, sizeof(double) * size_t(n), hipMemcpyDeviceToHost, s));
@@ -1511,13 +1713,21 @@ void BandLDLT<R>::solve(const double* rhs, double* out) {
     const long spin_limit = 1L << 22;
     const bool tk_fwd = clock.take(KT_FWD);
     if (tk_fwd) clock.begin(KT_FWD, s);
-    hipLaunchKernelGGL((k_fwd_sweep<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, xw, yw, nt, w, spin_limit,
-                       flag);
+    if (chained)
+        hipLaunchKernelGGL((k_fwd_sweep_m<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, mch, xw, yw, nt, w,
+                           spin_limit, flag);
+    else
+        hipLaunchKernelGGL((k_fwd_sweep<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, xw, yw, nt, w, spin_limit,
+                           flag);
     if (tk_fwd) clock.end(s);
     const bool tk_bwd = clock.take(KT_BWD);
     if (tk_bwd) clock.begin(KT_BWD, s);
-    hipLaunchKernelGGL((k_bwd_sweep<R>), dim3(int(nt)), dim3(256), 0, s, tiles_t, linv, yw, piv, xw, nt, w,
-                       spin_limit, flag);
+    if (chained)
+        hipLaunchKernelGGL((k_bwd_sweep_m<R>), dim3(int(nt)), dim3(256), 0, s, tiles_t, linv, gch, yw, piv, xw, nt, w,
+                           spin_limit, flag);
+    else
+        hipLaunchKernelGGL((k_bwd_sweep<R>), dim3(int(nt)), dim3(256), 0, s, tiles_t, linv, yw, piv, xw, nt, w,
+                           spin_limit, flag);
     if (tk_bwd) clock.end(s);
     CB_CHECK(hipGetLastError());
     CB_CHECK(hipMemcpyAsync(out, xw, sizeof(double) * size_t(n), hipMemcpyDeviceToDevice, s));

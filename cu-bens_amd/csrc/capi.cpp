@@ -65,7 +65,7 @@ struct cb_model {
     std::vector<std::vector<double>> step_kt_ms;
     std::vector<std::vector<long>> step_kt_n;
     std::vector<std::array<int, cb::EV_N>> step_ev;
-    std::vector<double> step_psi;
+    std::vector<double> step_psi, step_res, step_load;
 };
 
 struct cb_batch {
@@ -86,6 +86,7 @@ cb_options default_options() {
     o.kernel_timing = 0;
     o.factor_schedule = CB_FACTOR_FUSED;
     o.kernel_timing_every = 1;
+    o.sweep_schedule = CB_SWEEP_CHAIN;
     return o;
 }
 
@@ -101,6 +102,7 @@ cb::RunOptions to_run(const cb_options& o) {
     r.kernel_timing = o.kernel_timing;
     r.factor_schedule = o.factor_schedule;
     r.kernel_timing_every = o.kernel_timing_every;
+    r.sweep_schedule = o.sweep_schedule;
     return r;
 }
 
@@ -108,7 +110,7 @@ bool same_opts(const cb_options& a, const cb_options& b) {
     return a.solver == b.solver && a.krylov_tol == b.krylov_tol && a.restart == b.restart &&
            a.maxit == b.maxit && a.judge == b.judge && a.max_steps == b.max_steps && a.keep_d == b.keep_d &&
            a.kernel_timing == b.kernel_timing && a.factor_schedule == b.factor_schedule &&
-           a.kernel_timing_every == b.kernel_timing_every;
+           a.kernel_timing_every == b.kernel_timing_every && a.sweep_schedule == b.sweep_schedule;
 }
 
 double normal01(uint64_t& st) {
@@ -428,6 +430,8 @@ int cb_run(cb_model* h, const cb_options* opt, cb_stats* st) {
     h->step_kt_n = an.step_kt_n;
     h->step_ev = an.step_ev;
     h->step_psi = an.step_psi;
+    h->step_res = an.step_res;
+    h->step_load = an.step_load;
     if (st) {
         st->t_total = rs.t_total; st->t_tangent = rs.t_tangent; st->t_factor = rs.t_factor;
         st->t_solve = rs.t_solve; st->t_forces = rs.t_forces; st->t_judge = rs.t_judge;
@@ -497,6 +501,14 @@ int cb_get_step_events(const cb_model* h, int* events, double* psi) {
         for (size_t s = 0; s < n; ++s)
             std::memcpy(events + s * cb::EV_N, h->step_ev[s].data(), sizeof(int) * cb::EV_N);
     if (psi) std::memcpy(psi, h->step_psi.data(), sizeof(double) * n);
+    return 0;
+}
+
+int cb_get_step_residuals(const cb_model* h, double* res, double* load) {
+    if (!h) return fail(-1, "null handle");
+    const size_t n = h->step_res.size();
+    if (res) std::memcpy(res, h->step_res.data(), sizeof(double) * n);
+    if (load) std::memcpy(load, h->step_load.data(), sizeof(double) * n);
     return 0;
 }
 

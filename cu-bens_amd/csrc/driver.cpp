@@ -49,6 +49,8 @@ Analysis::Analysis(Model& m, Dev& d, const RunOptions& o) : m_(m), d_(d), o_(o) 
     b32_.clock.every = o.kernel_timing_every;
     b64_.split = o.factor_schedule == 1;
     b32_.split = o.factor_schedule == 1;
+    b64_.chained = o.sweep_schedule == 0;
+    b32_.chained = o.sweep_schedule == 0;
     if (o.solver != SOLVER_BAND64) gm_.setup(d.nslot, o.restart);
     CB_CHECK(hipStreamSynchronize(d.s));
 }
@@ -82,6 +84,18 @@ void Analysis::record(double lam, int ite, const double* dvec) {
     step_ev.push_back(ev_);
     step_psi.push_back(ev_psi_);
     ev_.fill(0);
+    if (m_.algflag <= 3) {
+        // equilibrium of the converged step: |lambda q - f| and |lambda q|
+        vresidual(d_, tmp, lam, q, f, m_.neq);
+        const double* a[2] = {tmp, q};
+        double s[2];
+        multi_dot(d_, 2, a, a, m_.neq, s);
+        step_res.push_back(std::sqrt(s[0]));
+        step_load.push_back(std::fabs(lam) * std::sqrt(s[1]));
+    } else {
+        step_res.push_back(0.0);
+        step_load.push_back(0.0);
+    }
     if (o_.keep_d) {
         size_t off = dpath.size();
         dpath.resize(off + size_t(m_.neq));
@@ -231,6 +245,8 @@ int Analysis::run(RunStats& st) {
     step_kt_n.clear();
     step_ev.clear();
     step_psi.clear();
+    step_res.clear();
+    step_load.clear();
     ev_.fill(0);
     ev_psi_ = 1;
     dev_reset_state(d_, m_);

@@ -42,6 +42,7 @@ struct RunOptions {
     int kernel_timing = 0;  // KT_* bitmask of band kernels timed with HIP events
     int factor_schedule = 0;  // 0 fused launch per tile column, 1 split diag/panel/update launches
     int kernel_timing_every = 1;  // time every n-th launch of a timed family
+    int sweep_schedule = 0;       // 0 sweeps with the chain matrices, 1 the two-product row step
 };
 
 struct RunStats {
@@ -84,6 +85,8 @@ public:
     // the step was taken with
     std::vector<std::array<int, EV_N>> step_ev;
     std::vector<double> step_psi;
+    // per converged step of a static driver: |lambda q - f| and |lambda q|
+    std::vector<double> step_res, step_load;
     PrecHook learned;
     // one-shot operations used by the C ABI
     int tangent_and_factor(int gx, int g, int* det);

@@ -33,7 +33,7 @@ class Options(C.Structure):
     _fields_ = [("solver", C.c_int), ("krylov_tol", C.c_double), ("restart", C.c_int),
                 ("maxit", C.c_int), ("judge", C.c_int), ("max_steps", C.c_int),
                 ("keep_d", C.c_int), ("kernel_timing", C.c_int), ("factor_schedule", C.c_int),
-                ("kernel_timing_every", C.c_int)]
+                ("kernel_timing_every", C.c_int), ("sweep_schedule", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -48,20 +48,23 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-KT_COLUMN, KT_DIAG, KT_PANEL, KT_UPDATE, KT_FWD, KT_BWD = range(6)
+KT_COLUMN, KT_DIAG, KT_PANEL, KT_UPDATE, KT_FWD, KT_BWD, KT_PREP = range(7)
 # Per-step driver events (cb_get_step_events): det flag, quad imaginary-root and
 # negative-gamma retries, dnorm restarts, itemax halvings.
 EV_DET, EV_IMAG, EV_NEG, EV_DNORM, EV_HALVE = range(5)
 NEV = 5
-KT_NAMES = ("band_column", "band_diag", "band_panel", "band_update", "band_fwd_sweep", "band_bwd_sweep")
+KT_NAMES = ("band_column", "band_diag", "band_panel", "band_update", "band_fwd_sweep", "band_bwd_sweep",
+            "band_sweep_prep")
 FACTOR_FUSED, FACTOR_SPLIT = 0, 1
+SWEEP_CHAIN, SWEEP_ROWS = 0, 1
 
 
 def options(solver: int = SOLVER_BAND64, krylov_tol: float = 1e-8, restart: int = 30,
             maxit: int = 20000, judge: int = 1, max_steps: int = 1 << 30, keep_d: int = 1,
-            kernel_timing: int = 0, factor_schedule: int = 0, kernel_timing_every: int = 1) -> Options:
+            kernel_timing: int = 0, factor_schedule: int = 0, kernel_timing_every: int = 1,
+            sweep_schedule: int = 0) -> Options:
     return Options(solver, krylov_tol, restart, maxit, judge, max_steps, keep_d, kernel_timing,
-                   factor_schedule, kernel_timing_every)
+                   factor_schedule, kernel_timing_every, sweep_schedule)
 
 
 _lib = None
@@ -107,6 +110,7 @@ ABI = {
     "cb_run": (C.c_int, [C.c_void_p, C.POINTER(Options), C.POINTER(Stats)]),
     "cb_get_step_events": (C.c_int, [C.c_void_p, PI, PD]),
     "cb_get_reactions": (C.c_int, [C.c_void_p, C.c_int, PD]),
+    "cb_get_step_residuals": (C.c_int, [C.c_void_p, PD, PD]),
     "cb_dots": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_long, PD]),
     "cb_dots_noalloc": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_long,
                                   C.c_void_p, C.c_void_p]),
@@ -291,8 +295,12 @@ class GpuModel:
         ev = np.zeros((max(n, 1), NEV), np.int32)
         psi = np.zeros(max(n, 1))
         _check(lib().cb_get_step_events(self.h, _p(ev, PI), _p(psi)))
+        res = np.zeros(max(n, 1))
+        load = np.zeros(max(n, 1))
+        _check(lib().cb_get_step_residuals(self.h, _p(res), _p(load)))
         out = st.as_dict()
-        out.update(step_t=tw[:n], step_judge=tj[:n], step_newton=nw[:n], events=ev[:n], psi=psi[:n])
+        out.update(step_t=tw[:n], step_judge=tj[:n], step_newton=nw[:n], events=ev[:n], psi=psi[:n],
+                   res=res[:n], load=load[:n])
         out.update(lpf=lpf[:n], itecnt=ite[:n], d=d[:n] if keep else None, msg=msg)
         return out
 

@@ -53,6 +53,7 @@ typedef struct {
     int kernel_timing;   /* bitmask of CB_KT_* kernel families timed with HIP events */
     int factor_schedule; /* CB_FACTOR_* schedule of the band factorization  */
     int kernel_timing_every; /* time every n-th launch of a timed family (<= 1: all) */
+    int sweep_schedule;  /* CB_SWEEP_* schedule of the band solves' sweeps    */
 } cb_options;
 
 /* Schedules of the band LDL^T factorization (same tile arithmetic, bitwise
@@ -61,6 +62,12 @@ typedef struct {
  * launch per tile column (for timing the tile tasks one by one). */
 enum { CB_FACTOR_FUSED = 0, CB_FACTOR_SPLIT = 1 };
 
+/* Schedules of the band solves' forward and backward sweeps (the same
+ * solution up to rounding), both one tile row per workgroup: a chain step of
+ * one 64x64 product with the matrix Linv_I L_{I,I-1} formed after each
+ * factorization (default), or of two dependent 64x64 products. */
+enum { CB_SWEEP_CHAIN = 0, CB_SWEEP_ROWS = 1 };
+
 /* Generations of the per-element state (base = last converged, _i = current
  * iterate, _ip = previous iterate), as the reference's x/x_temp/x_ip, c1/c1_i/
  * c1_ip, ef/ef_i/ef_ip arrays (main.c:497-510, 760-790). */
@@ -68,9 +75,10 @@ enum { CB_GEN_BASE = 0, CB_GEN_I = 1, CB_GEN_IP = 2 };
 
 /* Kernel families of the band solver (timed with HIP events on their stream):
  * the fused per-column factorization launch, the diag/panel/update launches
- * of the split schedule, the forward and backward sweeps. */
+ * of the split schedule, the forward and backward sweeps, and the launch that
+ * forms the sweeps' chain matrices after a factorization. */
 enum { CB_KT_COLUMN = 0, CB_KT_DIAG = 1, CB_KT_PANEL = 2, CB_KT_UPDATE = 3, CB_KT_FWD = 4, CB_KT_BWD = 5,
-       CB_KT_N = 6 };
+       CB_KT_PREP = 6, CB_KT_N = 7 };
 
 typedef struct {
     double t_total, t_tangent, t_factor, t_solve, t_forces, t_judge, t_other;
@@ -206,6 +214,11 @@ int cb_get_reactions(cb_model *m, int gen, double *r);
 enum { CB_EV_DET = 0, CB_EV_IMAG = 1, CB_EV_NEG = 2, CB_EV_DNORM = 3, CB_EV_HALVE = 4,
        CB_NEV = 5 };
 int cb_get_step_events(const cb_model *m, int *events, double *psi);
+/* Per converged step of the last NR/MNR/MSAL run: the 2-norms of the
+ * out-of-balance force lambda q - f and of the applied load lambda q at
+ * convergence [nsteps] each (zero for the Newmark driver; either pointer may
+ * be NULL).  No reference counterpart: test() prints nothing of them. */
+int cb_get_step_residuals(const cb_model *m, double *res, double *load);
 /* Writes the results2.txt rows of the last run (format of misc.c:350-417);
  * fails when the run kept no displacement path (keep_d = 0). */
 int cb_write_results2(const cb_model *m, const char *path);

@@ -222,6 +222,8 @@ typedef struct {
     int *events;
     double *psi;
     double *react;       /* optional [max_steps*NJ*7] reactions of each converged step */
+    double *resid;       /* optional [max_steps*2]: |lpf q - f| and |lpf q| at convergence
+                            (NR/MNR/MSAL; zero under Newmark) */
 } orc_path;
 
 /* Driver events of one converged step: the pivot-sign flag of the step's last

@@ -54,7 +54,7 @@ class Path(C.Structure):
                 ("t_tangent", C.c_double), ("t_factor", C.c_double), ("t_solve", C.c_double),
                 ("t_forces", C.c_double), ("t_other", C.c_double),
                 ("n_factor", C.c_long), ("n_solve", C.c_long), ("n_newton", C.c_long),
-                ("events", PI), ("psi", PD), ("react", PD)]
+                ("events", PI), ("psi", PD), ("react", PD), ("resid", PD)]
 
 # Slots of the per-step driver events (ORC_EV_* in cb_oracle.h).
 EV_DET, EV_IMAG, EV_NEG, EV_DNORM, EV_HALVE = range(5)
@@ -225,6 +225,8 @@ class OracleModel:
         P.psi = ptr(psi)
         react = np.zeros((max_steps, self.nj, 7)) if keep_react else None
         P.react = ptr(react) if keep_react else None
+        resid = np.zeros((max_steps, 2))
+        P.resid = ptr(resid)
         be = None
         if backend == "ref":
             be = Backend()
@@ -235,7 +237,8 @@ class OracleModel:
                "lpf": lpf[:n].copy(), "itecnt": ite[:n].copy(), "d": dmat[:n].copy(),
                "t_tangent": P.t_tangent, "t_factor": P.t_factor, "t_solve": P.t_solve,
                "t_forces": P.t_forces, "n_factor": P.n_factor, "n_solve": P.n_solve,
-               "n_newton": P.n_newton, "events": ev[:n].copy(), "psi": psi[:n].copy()}
+               "n_newton": P.n_newton, "events": ev[:n].copy(), "psi": psi[:n].copy(),
+               "res": resid[:n, 0].copy(), "load": resid[:n, 1].copy()}
         if keep_f:
             out["f"] = fmat[:n].copy()
         if keep_react:

@@ -325,6 +325,17 @@ static void record(run_t *R, double lpf, int itecnt, const double *d, const doub
     if (o->psi) o->psi[o->nsteps] = R->ev_psi;
     if (o->react)
         orc_reactions(R->m, R->ef, R->c1, R->c2, R->c3, o->react + (long)o->nsteps * R->nj * 7);
+    if (o->resid) {
+        double rr = 0, qq = 0;
+        if (R->m->dm.ALGFLAG <= 3 && f)
+            for (long i = 0; i < R->neq; ++i) {
+                double t = lpf * R->m->q[i] - f[i];
+                rr += t * t;
+                qq += R->m->q[i] * R->m->q[i];
+            }
+        o->resid[2 * o->nsteps] = sqrt(rr);
+        o->resid[2 * o->nsteps + 1] = fabs(lpf) * sqrt(qq);
+    }
     o->nsteps++;
     memset(R->ev, 0, sizeof R->ev);
 }

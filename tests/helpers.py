@@ -6,6 +6,8 @@ import tempfile
 
 import numpy as np
 
+import math
+
 from cubens import inputs
 
 _TMP = tempfile.mkdtemp(prefix="cubens_tests_")
@@ -111,3 +113,34 @@ class Hip:
         for p in self.live:
             self.rt.hipFree(p)
         self.live = []
+
+
+def arc_residuals(om_params, r, q):
+    """Relative residual of the spherical arc-length constraint
+    |d_s - d_{s-1}|^2 + [psi_s >= psi_thresh] (lambda_s - lambda_{s-1})^2 q.q = arc_s^2
+    of each converged increment s >= 1 whose arc the recorded path determines
+    (no dnorm restart within the step), with arc_s rebuilt from the path alone:
+    beta = sqrt(iteopt/itecnt) alpha |d_0| / |d_{s-1} - d_{s-2}| times the
+    previous increment (main.c:2625-2630, 2655-2664, 3048-3049), halved once per
+    quad retry or itemax exceedance (main.c:2995-3036).  r: a path with keys d,
+    lpf, itecnt, events (EV_* slots), psi; returns {s: residual}."""
+    alpha, iteopt, thr = om_params
+    d, lam, ite, ev, psi = r["d"], r["lpf"], r["itecnt"], r["events"], r["psi"]
+    EV_IMAG, EV_NEG, EV_DNORM, EV_HALVE = 1, 2, 3, 4
+    qq = float(q @ q)
+    dn0 = np.linalg.norm(d[0])
+    out = {}
+    for s in range(1, r["nsteps"]):
+        if ev[s, EV_DNORM]:
+            continue
+        dprev = d[s - 1] - (d[s - 2] if s >= 2 else 0.0)
+        lprev = lam[s - 1] - (lam[s - 2] if s >= 2 else 0.0)
+        beta = math.sqrt(iteopt / ite[s - 1]) * alpha * dn0 / np.linalg.norm(dprev)
+        load = psi[s] >= thr
+        arc = beta * math.sqrt(dprev @ dprev + (lprev * lprev * qq if load else 0.0))
+        arc *= 0.5 ** int(ev[s, EV_IMAG] + ev[s, EV_NEG] + ev[s, EV_HALVE])
+        dd = d[s] - d[s - 1]
+        dl = lam[s] - lam[s - 1]
+        lhs = dd @ dd + (dl * dl * qq if load else 0.0)
+        out[s] = abs(lhs - arc * arc) / (arc * arc)
+    return out

@@ -4,11 +4,26 @@
   model: the factorization of the initial tangent has only positive pivots
   (the hinged panel's tangent is positive definite); the solve has a normwise
   backward error at the FP64 round-off level (K applied on the host from the
-  tangent the device assembled); the fused and split schedules give bitwise
+  tangent the device assembled) with the chain-matrix and with the two-product
+  sweeps; the fused and split factorization schedules give bitwise
   identical pivots and solutions.
 - 64x64 panel (8192 triangles, NEQ 24960): the arc-length path of the plain
-  FP64 GPU path follows the oracle's to 1e-9 with identical iteration counts.
+  FP64 GPU path follows the oracle's to 1e-9 with identical iteration counts,
+  live over 6 increments and against the oracle's fixture over the 60
+  increments of the workload (tests/golden/panel64_path.npz);
+- 256x256 panel: the first increment against the oracle's fixture
+  (tests/golden/panel256_first.npz: one skyline factorization of the serial
+  path, 3 minutes on one core), and over the 40 benchmarked increments the
+  size-independent invariants of every converged step: the spherical
+  arc-length constraint rebuilt from the path, and the out-of-balance force
+  within the force criterion of test() (misc.c:215-232).
+
+Fixture comparisons cover the whole displacement vector through its 2-norm
+and a weighted checksum, and every 97th entry directly.
 """
+import os
+import sys
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -16,7 +31,7 @@ import scipy.sparse as sp
 import cubens
 import oracle as O
 from cubens import inputs
-from helpers import model_file, panel, relerr
+from helpers import arc_residuals, model_file, panel, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -58,6 +73,13 @@ def test_full_size_factor_and_solve():
         assert eta < 1e-13
         assert out[1][0] == det
         assert np.array_equal(out[1][1], piv) and np.array_equal(out[1][2], x)
+        # the two-product sweeps on the same factor: also at round-off
+        g.factor(cubens.options(cubens.SOLVER_BAND64, sweep_schedule=cubens.SWEEP_ROWS))
+        x2 = g.solve(b)
+        r2 = K @ x2 - b
+        eta2 = np.abs(r2).max() / (knorm * np.abs(x2).max() + np.abs(b).max())
+        print(f"two-product sweeps: backward error {eta2:.2e}")
+        assert eta2 < 1e-13
 
 
 def test_panel64_msal_path_matches_oracle():
@@ -72,3 +94,69 @@ def test_panel64_msal_path_matches_oracle():
     assert relerr(gp["lpf"], op["lpf"]) < 1e-9
     assert relerr(gp["d"], op["d"]) < 1e-9
     g.close()
+
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _weights(neq):
+    sys.path.insert(0, GOLD)
+    from make_fullsize import make_weights
+    return make_weights(neq)
+
+
+def _against_fixture(gp, fx):
+    n = len(fx["lpf"])
+    assert gp["status"] == int(fx["status"]) and gp["nsteps"] == n
+    assert np.array_equal(gp["itecnt"], fx["itecnt"])
+    assert np.array_equal(gp["events"], fx["events"])
+    assert np.abs(gp["psi"] - fx["psi"]).max() <= 1e-6
+    assert relerr(gp["lpf"], fx["lpf"]) <= 1e-9
+    d = gp["d"]
+    assert relerr(d[:, fx["sel"]], fx["d_sel"]) <= 1e-9
+    assert relerr(np.linalg.norm(d, axis=1), fx["d_norm"]) <= 1e-9
+    assert relerr(d @ _weights(d.shape[1]), fx["d_check"]) <= 1e-9
+
+
+def _invariants(gp, q, tolforc=1e-3, msal=(1.0, 6, 0.1)):
+    res = arc_residuals(msal, gp, q)
+    assert len(res) >= gp["nsteps"] - 1 - int((gp["events"][:, 3] > 0).sum())
+    worst_arc = max(res.values()) if res else 0.0
+    lam = gp["lpf"]
+    dl = np.abs(np.diff(np.concatenate([[0.0], lam])))
+    # |lambda q - f| within the force criterion |qtot - f| <= tolforc |qtot - fp|
+    # (qtot is formed before the last load-factor correction, main.c:2845-2855)
+    ratio = gp["res"] / (dl * np.linalg.norm(q))
+    print(f"[fullsize] worst arc-constraint residual {worst_arc:.2e}, "
+          f"worst |lq-f|/(|dl||q|) {ratio.max():.3e}", flush=True)
+    assert worst_arc <= 1e-8
+    assert ratio.max() <= 2 * tolforc
+
+
+def test_panel64_sixty_increments_against_fixture():
+    fx = np.load(os.path.join(GOLD, "panel64_path.npz"))
+    with cubens.GpuModel(panel(64)) as g:
+        gp = g.run(cubens.options(cubens.SOLVER_BAND64, max_steps=len(fx["lpf"])))
+        q = g.load()
+    _against_fixture(gp, fx)
+    _invariants(gp, q)
+
+
+def _panel256():
+    return model_file("panel256_c3.txt",
+                      inputs.shell_panel(n=256, thick_sigma=0.1, imperfection=0.1, config_index=3).text)
+
+
+def test_panel256_first_increment_against_fixture():
+    fx = np.load(os.path.join(GOLD, "panel256_first.npz"))
+    with cubens.GpuModel(_panel256()) as g:
+        gp = g.run(cubens.options(cubens.SOLVER_BAND64, max_steps=1))
+    _against_fixture(gp, fx)
+
+
+def test_panel256_forty_increments_invariants():
+    with cubens.GpuModel(_panel256()) as g:
+        gp = g.run(cubens.options(cubens.SOLVER_BAND64, max_steps=40))
+        q = g.load()
+    assert gp["status"] == 0 and gp["nsteps"] == 40
+    _invariants(gp, q)

@@ -241,3 +241,23 @@ def test_panel_msal_mixed_precision_path():
     assert relerr(gp["lpf"], op["lpf"]) < 1e-6
     assert relerr(gp["d"], op["d"]) < 1e-6
     g.close()
+
+
+def test_reference_sample_5a_mnr_on_the_gpu():
+    """The reference's own sample input (Sample_Input_Files/model_def_5a_truss.txt,
+    modified Newton-Raphson, ALGFLAG 2) on the GPU against the path the
+    reference's element routines give under the restated driver
+    (tests/golden/sample_5a.npz)."""
+    import os
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "sample_5a.npz"))
+    path = os.path.join(os.path.dirname(__file__), "golden", "model_def_5a_truss.txt")
+    with cubens.GpuModel(path) as g:
+        assert g.info.algflag == 2
+        jc, _, _ = g.dofmaps()
+        assert np.array_equal(jc, gold["jcode"])
+        gp = g.run(cubens.options(cubens.SOLVER_BAND64, max_steps=50))
+    assert gp["status"] == 0
+    assert gp["nsteps"] == len(gold["path_lpf"])
+    assert np.array_equal(gp["itecnt"], gold["path_itecnt"])
+    assert relerr(gp["lpf"], gold["path_lpf"]) <= 1e-12
+    assert relerr(gp["d"], gold["path_d"]) <= 1e-9

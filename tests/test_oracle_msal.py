@@ -28,6 +28,7 @@ import pytest
 
 import oracle as O
 from cubens import inputs
+from helpers import arc_residuals
 
 
 def closed_form_gap(lam, lc):
@@ -56,31 +57,6 @@ def star_tangent(w, bars, radius, rise, emod, area):
     kxx = ka * (bars / 2.0) * radius * radius / (L * L) + bars * N / L
     kzz = ka * bars * H * H / (L * L) + bars * N / L
     return np.array([kxx, kxx, kzz])
-
-
-def arc_residuals(om_params, r, q):
-    """Relative residual of the spherical constraint of each converged
-    increment s >= 1 whose arc the recorded path determines (no dnorm restart
-    within the step); returns {s: residual}."""
-    alpha, iteopt, thr = om_params
-    d, lam, ite, ev, psi = r["d"], r["lpf"], r["itecnt"], r["events"], r["psi"]
-    qq = float(q @ q)
-    dn0 = np.linalg.norm(d[0])
-    out = {}
-    for s in range(1, r["nsteps"]):
-        if ev[s, O.EV_DNORM]:
-            continue
-        dprev = d[s - 1] - (d[s - 2] if s >= 2 else 0.0)
-        lprev = lam[s - 1] - (lam[s - 2] if s >= 2 else 0.0)
-        beta = math.sqrt(iteopt / ite[s - 1]) * alpha * dn0 / np.linalg.norm(dprev)
-        load = psi[s] >= thr
-        arc = beta * math.sqrt(dprev @ dprev + (lprev * lprev * qq if load else 0.0))
-        arc *= 0.5 ** int(ev[s, O.EV_IMAG] + ev[s, O.EV_NEG] + ev[s, O.EV_HALVE])
-        dd = d[s] - d[s - 1]
-        dl = lam[s] - lam[s - 1]
-        lhs = dd @ dd + (dl * dl * qq if load else 0.0)
-        out[s] = abs(lhs - arc * arc) / (arc * arc)
-    return out
 
 
 STAR = dict(bars=4, radius=1.0, rise=0.1, emod=2e11, area=1e-4)
@@ -221,3 +197,22 @@ def test_star_hub_of_70_bars_follows_the_closed_form():
     lc = inputs.star_truss_lambda(r["d"][:, 2], **kw)
     assert closed_form_gap(r["lpf"], lc).max() <= TOL[1]
     assert r["events"][:, O.EV_DET].sum() > 10
+
+
+def test_shell_panel_increments_satisfy_the_invariants():
+    """The invariants the GPU full-size tests apply at 256x256, here on the
+    oracle's 16x16 panel with the same thickness field and imperfection: the
+    rebuilt arc-length constraint at every increment, and the out-of-balance
+    force within the force criterion of test() (tolforc 1e-3 of the load
+    increment, up to the last load-factor correction)."""
+    mt = inputs.shell_panel(n=16, thick_sigma=0.1, imperfection=0.1, config_index=3)
+    om, r = _run(mt, steps=30)
+    assert r["nsteps"] == 30
+    res = arc_residuals((1.0, 6, 0.1), r, om.q)
+    assert len(res) == 29 and max(res.values()) <= 1e-8
+    lam = r["lpf"]
+    dl = np.abs(np.diff(np.concatenate([[0.0], lam])))
+    ratio = r["res"] / (dl * np.linalg.norm(om.q))
+    assert ratio.max() <= 2e-3
+    # the residual norm the driver records is |lambda q - f| of its converged f
+    assert np.allclose(r["res"], np.linalg.norm(lam[:, None] * om.q[None, :] - r["f"], axis=1), rtol=1e-12)
