@@ -59,7 +59,7 @@ struct cb_model {
     cb_learned* learned = nullptr;
     std::unique_ptr<cb::Analysis> an;
     cb_options opt{};
-    std::vector<double> path_lpf, path_d, step_t, step_judge;
+    std::vector<double> path_lpf, path_d, path_f, step_t, step_judge;
     std::vector<int> path_ite;
     std::vector<long> step_newton;
     std::vector<std::vector<double>> step_kt_ms;
@@ -87,6 +87,7 @@ cb_options default_options() {
     o.factor_schedule = CB_FACTOR_FUSED;
     o.kernel_timing_every = 1;
     o.sweep_schedule = CB_SWEEP_CHAIN;
+    o.keep_forces = 0;
     return o;
 }
 
@@ -103,6 +104,7 @@ cb::RunOptions to_run(const cb_options& o) {
     r.factor_schedule = o.factor_schedule;
     r.kernel_timing_every = o.kernel_timing_every;
     r.sweep_schedule = o.sweep_schedule;
+    r.keep_forces = o.keep_forces;
     return r;
 }
 
@@ -110,7 +112,8 @@ bool same_opts(const cb_options& a, const cb_options& b) {
     return a.solver == b.solver && a.krylov_tol == b.krylov_tol && a.restart == b.restart &&
            a.maxit == b.maxit && a.judge == b.judge && a.max_steps == b.max_steps && a.keep_d == b.keep_d &&
            a.kernel_timing == b.kernel_timing && a.factor_schedule == b.factor_schedule &&
-           a.kernel_timing_every == b.kernel_timing_every && a.sweep_schedule == b.sweep_schedule;
+           a.kernel_timing_every == b.kernel_timing_every && a.sweep_schedule == b.sweep_schedule &&
+           a.keep_forces == b.keep_forces;
 }
 
 double normal01(uint64_t& st) {
@@ -423,6 +426,7 @@ int cb_run(cb_model* h, const cb_options* opt, cb_stats* st) {
     h->path_lpf = an.lpf;
     h->path_ite = an.itecnt;
     h->path_d = an.dpath;
+    h->path_f = an.fpath;
     h->step_t = an.step_t;
     h->step_judge = an.step_judge;
     h->step_newton = an.step_newton;
@@ -531,24 +535,110 @@ int cb_get_kernel_times(const cb_model* h, int precision, int step, double* ms, 
     return 0;
 }
 
+namespace {
+
+// The rows output() (misc.c:345-451) writes over a run: headers (flag 0),
+// then one row per converged step (flag 1), to results2.txt (f[0]) and
+// results3-5.txt (f[1..3], any of them may be null when not wanted).
+void write_results(const cb_model* h, FILE* const f[4]) {
+    const cb::Model& m = h->m;
+    const long n = m.neq, ntr = m.ntr, nfr = m.nfr, nsh = m.nsh;
+    const size_t ns = h->path_lpf.size(), nf = size_t(ntr + 7 * nfr + 6 * nsh);
+    if (f[0]) {
+        std::fprintf(f[0], "Model Displacements:\n\tLambda\t\tIterations");
+        for (long i = 0; i < n; ++i) {
+            if (i + 1 <= 1000) std::fprintf(f[0], "\tDOF %ld\t", i + 1);
+            else std::fprintf(f[0], "DOF %ld\t", i + 1);
+        }
+    }
+    if (ntr > 0 && f[1]) {
+        std::fprintf(f[1], "Truss Element Forces (averaged):\n\tLambda\t\tIterations\t");
+        for (long i = 0; i < ntr; ++i) std::fprintf(f[1], "Element %ld\t", i + 1);
+    }
+    if (nfr > 0) {
+        if (f[2]) std::fprintf(f[2], "Frame Element Forces (averaged):\n\tLambda\t\tIterations");
+        if (f[1]) std::fprintf(f[1], "\t");   // output() writes this tab to results3.txt
+        if (f[2]) {
+            for (long i = 0; i < nfr; ++i) std::fprintf(f[2], "Element %ld\t\t\t\t\t\t\t\t\t\t\t\t\t", i + 1);
+            std::fprintf(f[2], "\n\t\t\t\t\t");
+            for (long i = 0; i < nfr; ++i)
+                std::fprintf(f[2], "X-Force\t\tY-Force\t\tZ-Force\t\tX-Moment\tY-Moment\tZ-Moment\tBi-Moment\t");
+        }
+    }
+    if (nsh > 0) {
+        if (f[3]) std::fprintf(f[3], "Shell Element Forces (averaged):\n\tLambda\t\tIterations");
+        if (f[1]) std::fprintf(f[1], "\t");   // likewise
+        if (f[3]) {
+            for (long i = 0; i < nsh; ++i) std::fprintf(f[3], "Element %ld\t\t\t\t\t\t\t\t\t\t\t", i + 1);
+            std::fprintf(f[3], "\n\t\t\t\t\t");
+            for (long i = 0; i < nsh; ++i)
+                std::fprintf(f[3], "X-Force\t\tY-Force\t\tZ-Force\t\tX-Moment\tY-Moment\tZ-Moment\t");
+        }
+    }
+    for (size_t s = 0; s < ns; ++s) {
+        const double lam = h->path_lpf[s];
+        const int ite = h->path_ite[s];
+        if (f[0]) {
+            std::fprintf(f[0], "\n\t%e\t%d\t", lam, ite);
+            for (long i = 0; i < n; ++i) std::fprintf(f[0], "\t%e", h->path_d[s * size_t(n) + size_t(i)]);
+        }
+        const double* fa = h->path_f.size() >= (s + 1) * nf ? h->path_f.data() + s * nf : nullptr;
+        if (!fa) continue;
+        if (ntr > 0 && f[1]) {
+            std::fprintf(f[1], "\n\t%e\t%d\t", lam, ite);
+            for (long i = 0; i < ntr; ++i) std::fprintf(f[1], "\t%e", fa[i]);
+        }
+        if (nfr > 0 && f[2]) {
+            std::fprintf(f[2], "\n\t%e\t%d\t", lam, ite);
+            for (long i = 0; i < 7 * nfr; ++i) std::fprintf(f[2], "\t%e", fa[ntr + i]);
+        }
+        if (nsh > 0 && f[3]) {
+            std::fprintf(f[3], "\n\t%e\t%d\t", lam, ite);
+            for (long i = 0; i < 6 * nsh; ++i) std::fprintf(f[3], "\t%e", fa[ntr + 7 * nfr + i]);
+        }
+    }
+}
+
+}  // namespace
+
 int cb_write_results2(const cb_model* h, const char* path) {
     if (!h || !path) return fail(-1, "null argument");
-    const long n = h->m.neq;
     const size_t ns = h->path_lpf.size();
-    if (h->path_d.size() < ns * size_t(n))
+    if (h->path_d.size() < ns * size_t(h->m.neq))
         return fail(-1, "the last run kept no displacement path (keep_d = 0)");
     FILE* fp = std::fopen(path, "w");
     if (!fp) return fail(-1, std::string("cannot write ") + path);
-    std::fprintf(fp, "Model Displacements:\n\tLambda\t\tIterations");
-    for (long i = 0; i < n; ++i) {
-        if (i + 1 <= 1000) std::fprintf(fp, "\tDOF %ld\t", i + 1);
-        else std::fprintf(fp, "DOF %ld\t", i + 1);
-    }
-    for (size_t s = 0; s < ns; ++s) {
-        std::fprintf(fp, "\n\t%e\t%d\t", h->path_lpf[s], h->path_ite[s]);
-        for (long i = 0; i < n; ++i) std::fprintf(fp, "\t%e", h->path_d[s * size_t(n) + size_t(i)]);
-    }
+    FILE* const f[4] = {fp, nullptr, nullptr, nullptr};
+    write_results(h, f);
     std::fclose(fp);
+    return 0;
+}
+
+int cb_get_path_forces(const cb_model* h, double* forces) {
+    if (!h || !forces) return fail(-1, "null argument");
+    std::memcpy(forces, h->path_f.data(), sizeof(double) * h->path_f.size());
+    return 0;
+}
+
+int cb_write_results(const cb_model* h, const char* dir) {
+    if (!h || !dir) return fail(-1, "null argument");
+    const cb::Model& m = h->m;
+    const size_t ns = h->path_lpf.size(), nf = size_t(m.ntr + 7 * m.nfr + 6 * m.nsh);
+    if (h->path_d.size() < ns * size_t(m.neq))
+        return fail(-1, "the last run kept no displacement path (keep_d = 0)");
+    if (nf > 0 && h->path_f.size() < ns * nf)
+        return fail(-1, "the last run kept no element forces (keep_forces = 0)");
+    FILE* f[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int k = 0; k < 4; ++k) {
+        const std::string p = std::string(dir) + "/results" + std::to_string(k + 2) + ".txt";
+        f[k] = std::fopen(p.c_str(), "w");
+        if (!f[k]) {
+            for (int j = 0; j < k; ++j) std::fclose(f[j]);
+            return fail(-1, "cannot write " + p);
+        }
+    }
+    write_results(h, f);
+    for (FILE* fp : f) std::fclose(fp);
     return 0;
 }
 

@@ -116,10 +116,13 @@ void launch_frame_tangent_band(Dev& d, int g, double* tiles, int w, int* bad);
 void launch_frame_forces(Dev& d, double* f, const double* dd);
 void launch_frame_update(Dev& d, int gx);
 
-// ---- reactions (reactions.hip) ----------------------------------------------
+// ---- results of a converged state (results.hip) --------------------------------
 // rx[NJ*7] <- the element forces T^T ef of generation g summed at every
 // constrained joint slot (jcode == 0), zero at the free slots.
 void launch_reactions(Dev& d, int g, double* rx);
+// out[NE_TR + 7 NE_FR + 6 NE_SH] <- the averaged element forces of generation g
+// that the reference writes to results3-5.txt (misc.c:419-449).
+void launch_avg_forces(Dev& d, int g, double* out);
 
 // ---- nonlinear Newmark (newmark.hip) -------------------------------------------
 // sm += lumped masses of element type (0 truss, 1 frame, 2 shell) at the

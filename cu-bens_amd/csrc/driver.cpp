@@ -39,6 +39,7 @@ Analysis::Analysis(Model& m, Dev& d, const RunOptions& o) : m_(m), d_(d), o_(o) 
     }
     eb = dalloc<double>(size_t(d.nslot));
     ex = dalloc<double>(size_t(d.nslot));
+    if (o.keep_forces) favg = dalloc<double>(size_t(m.ntr + 7 * m.nfr + 6 * m.nsh));
     h2d(q, m.q.data(), size_t(n), d.s);
     const bool need64 = o.solver == SOLVER_BAND64 || o.judge || o.solver != SOLVER_BAND32_FGMRES;
     if (need64) b64_.setup(n, m.bandwidth, d.s);
@@ -56,7 +57,7 @@ Analysis::Analysis(Model& m, Dev& d, const RunOptions& o) : m_(m), d_(d), o_(o) 
 }
 
 Analysis::~Analysis() {
-    double* vs[] = {q, dv, dp, dpp, dt, dd, ddq, ddr, f, fp, ft, fip, r, tmp, eb, ex,
+    double* vs[] = {q, dv, dp, dpp, dt, dd, ddq, ddr, f, fp, ft, fip, r, tmp, eb, ex, favg,
                     sm, qtot, uc, vc, ac, vci, aci, um, vm, amv, dyn, hist};
     for (double* p : vs)
         if (p) (void)hipFree(p);
@@ -95,6 +96,14 @@ void Analysis::record(double lam, int ite, const double* dvec) {
     } else {
         step_res.push_back(0.0);
         step_load.push_back(0.0);
+    }
+    if (o_.keep_forces) {
+        const size_t nf = size_t(m_.ntr + 7 * m_.nfr + 6 * m_.nsh);
+        const size_t off = fpath.size();
+        fpath.resize(off + nf);
+        launch_avg_forces(d_, G_BASE, favg);
+        d2h(fpath.data() + off, favg, nf, d_.s);
+        CB_CHECK(hipStreamSynchronize(d_.s));
     }
     if (o_.keep_d) {
         size_t off = dpath.size();
@@ -238,6 +247,7 @@ int Analysis::run(RunStats& st) {
     lpf.clear();
     itecnt.clear();
     dpath.clear();
+    fpath.clear();
     step_t.clear();
     step_judge.clear();
     step_newton.clear();

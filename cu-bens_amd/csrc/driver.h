@@ -43,6 +43,7 @@ struct RunOptions {
     int factor_schedule = 0;  // 0 fused launch per tile column, 1 split diag/panel/update launches
     int kernel_timing_every = 1;  // time every n-th launch of a timed family
     int sweep_schedule = 0;       // 0 sweeps with the chain matrices, 1 the two-product row step
+    int keep_forces = 0;          // record the averaged element forces of every converged step
 };
 
 struct RunStats {
@@ -74,6 +75,7 @@ public:
     std::vector<double> lpf;
     std::vector<int> itecnt;
     std::vector<double> dpath;   // nsteps * NEQ
+    std::vector<double> fpath;   // nsteps * (NE_TR + 7 NE_FR + 6 NE_SH), with keep_forces
     // per converged step: wall time since the run started, judge time and
     // Newton iterations so far
     std::vector<double> step_t, step_judge;
@@ -111,6 +113,7 @@ private:
     double *q = nullptr, *dv = nullptr, *dp = nullptr, *dpp = nullptr, *dt = nullptr, *dd = nullptr;
     double *ddq = nullptr, *ddr = nullptr, *f = nullptr, *fp = nullptr, *ft = nullptr, *fip = nullptr;
     double *r = nullptr, *eb = nullptr, *ex = nullptr, *tmp = nullptr;
+    double* favg = nullptr;   // averaged element forces of one step (keep_forces)
     std::vector<double> piv0_;
     std::array<int, EV_N> ev_{};   // events since the last recorded step
     double ev_psi_ = 1;

@@ -1,5 +1,7 @@
-// Support reactions of the converged state.
+// Results of a converged state: support reactions, and the averaged element
+// forces of the reference's results3-5.txt.
 //
+// Support reactions.
 // The reference computes none.  They are the internal force vector its
 // element routines accumulate into f, T_i^T ef_i summed over the elements
 // (truss.c:349-372, frame.c:1273-1309, shell.c:1728-1785), read at the slots
@@ -84,6 +86,27 @@ __global__ void k_react_shell(const int32_t* __restrict__ els, int n, const int3
     }
 }
 
+// Averaged element forces as output() writes them (misc.c:419-449): per truss
+// (|N_1| + |N_2|)/2, per frame member the 7 entries (|f_j| + |f_{7+j}|)/2, per
+// shell the 6 entries (|f_j| + |f_{6+j}| + |f_{12+j}|)/3; out [NE_TR + 7 NE_FR + 6 NE_SH].
+__global__ void k_avg_forces(const double* __restrict__ ef, double* __restrict__ out, long ntr, long nfr,
+                             long nsh) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long n1 = ntr, n2 = n1 + 7 * nfr, n3 = n2 + 6 * nsh;
+    if (i >= n3) return;
+    if (i < n1) {
+        out[i] = (fabs(ef[i * 2]) + fabs(ef[i * 2 + 1])) / 2;
+    } else if (i < n2) {
+        const long e = (i - n1) / 7, j = (i - n1) % 7;
+        const double* f = ef + ntr * 2 + e * 14;
+        out[i] = (fabs(f[j]) + fabs(f[7 + j])) / 2;
+    } else {
+        const long e = (i - n2) / 6, j = (i - n2) % 6;
+        const double* f = ef + ntr * 2 + nfr * 14 + e * 18;
+        out[i] = (fabs(f[j]) + fabs(f[6 + j]) + fabs(f[12 + j])) / 3;
+    }
+}
+
 // keep the constrained slots (jcode == 0), clear the others
 __global__ void k_react_mask(const int32_t* __restrict__ jcode7, double* __restrict__ rx, long n) {
     const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -115,6 +138,13 @@ void launch_reactions(Dev& d, int g, double* rx) {
                                d.minc, d.c1[g], d.c2[g], d.c3[g], d.ef[g], rx, io, to, eo);
     }
     hipLaunchKernelGGL(k_react_mask, dim3(nblk(ns)), dim3(TPB), 0, d.s, d.jcode, rx, ns);
+    CB_CHECK(hipGetLastError());
+}
+
+void launch_avg_forces(Dev& d, int g, double* out) {
+    const long n = d.ntr + 7 * d.nfr + 6 * d.nsh;
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_avg_forces, dim3(nblk(n)), dim3(TPB), 0, d.s, d.ef[g], out, d.ntr, d.nfr, d.nsh);
     CB_CHECK(hipGetLastError());
 }
 

@@ -33,7 +33,7 @@ class Options(C.Structure):
     _fields_ = [("solver", C.c_int), ("krylov_tol", C.c_double), ("restart", C.c_int),
                 ("maxit", C.c_int), ("judge", C.c_int), ("max_steps", C.c_int),
                 ("keep_d", C.c_int), ("kernel_timing", C.c_int), ("factor_schedule", C.c_int),
-                ("kernel_timing_every", C.c_int), ("sweep_schedule", C.c_int)]
+                ("kernel_timing_every", C.c_int), ("sweep_schedule", C.c_int), ("keep_forces", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -62,9 +62,9 @@ SWEEP_CHAIN, SWEEP_ROWS = 0, 1
 def options(solver: int = SOLVER_BAND64, krylov_tol: float = 1e-8, restart: int = 30,
             maxit: int = 20000, judge: int = 1, max_steps: int = 1 << 30, keep_d: int = 1,
             kernel_timing: int = 0, factor_schedule: int = 0, kernel_timing_every: int = 1,
-            sweep_schedule: int = 0) -> Options:
+            sweep_schedule: int = 0, keep_forces: int = 0) -> Options:
     return Options(solver, krylov_tol, restart, maxit, judge, max_steps, keep_d, kernel_timing,
-                   factor_schedule, kernel_timing_every, sweep_schedule)
+                   factor_schedule, kernel_timing_every, sweep_schedule, keep_forces)
 
 
 _lib = None
@@ -111,6 +111,8 @@ ABI = {
     "cb_get_step_events": (C.c_int, [C.c_void_p, PI, PD]),
     "cb_get_reactions": (C.c_int, [C.c_void_p, C.c_int, PD]),
     "cb_get_step_residuals": (C.c_int, [C.c_void_p, PD, PD]),
+    "cb_get_path_forces": (C.c_int, [C.c_void_p, PD]),
+    "cb_write_results": (C.c_int, [C.c_void_p, C.c_char_p]),
     "cb_dots": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_long, PD]),
     "cb_dots_noalloc": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_long,
                                   C.c_void_p, C.c_void_p]),
@@ -302,6 +304,11 @@ class GpuModel:
         out.update(step_t=tw[:n], step_judge=tj[:n], step_newton=nw[:n], events=ev[:n], psi=psi[:n],
                    res=res[:n], load=load[:n])
         out.update(lpf=lpf[:n], itecnt=ite[:n], d=d[:n] if keep else None, msg=msg)
+        if opt is not None and opt.keep_forces:
+            nf = self.ntr + 7 * self.nfr + 6 * self.nsh
+            fa = np.zeros(max(n * nf, 1))
+            _check(lib().cb_get_path_forces(self.h, _p(fa)))
+            out["forces"] = fa[: n * nf].reshape(n, nf)
         return out
 
     def reactions(self, gen: int = GEN_BASE) -> np.ndarray:
@@ -319,6 +326,10 @@ class GpuModel:
 
     def write_results2(self, path: str):
         _check(lib().cb_write_results2(self.h, path.encode()))
+
+    def write_results(self, directory: str):
+        """results2..5.txt of the last run in `directory` (cb_write_results)."""
+        _check(lib().cb_write_results(self.h, directory.encode()))
 
 
 class GpuBatch:

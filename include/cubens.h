@@ -54,6 +54,7 @@ typedef struct {
     int factor_schedule; /* CB_FACTOR_* schedule of the band factorization  */
     int kernel_timing_every; /* time every n-th launch of a timed family (<= 1: all) */
     int sweep_schedule;  /* CB_SWEEP_* schedule of the band solves' sweeps    */
+    int keep_forces;     /* keep the averaged element forces of every step  */
 } cb_options;
 
 /* Schedules of the band LDL^T factorization (same tile arithmetic, bitwise
@@ -222,6 +223,18 @@ int cb_get_step_residuals(const cb_model *m, double *res, double *load);
 /* Writes the results2.txt rows of the last run (format of misc.c:350-417);
  * fails when the run kept no displacement path (keep_d = 0). */
 int cb_write_results2(const cb_model *m, const char *path);
+/* Averaged element forces of every converged step of the last run (a run
+ * with keep_forces), as output() writes them to results3-5.txt
+ * (misc.c:419-449): per step NE_TR truss values, 7 per frame member, 6 per
+ * shell [nsteps*(NE_TR + 7*NE_FR + 6*NE_SH)]. */
+int cb_get_path_forces(const cb_model *m, double *forces);
+/* Writes results2.txt .. results5.txt of the last run into directory dir as
+ * output() (misc.c:345-451) does over a run: the displacement path and, for
+ * each element type of the model, its averaged element forces; a model
+ * without trusses gets a results3.txt of the tabs output()'s frame and shell
+ * headers write there.  Needs keep_d, and keep_forces for a model with
+ * elements. */
+int cb_write_results(const cb_model *m, const char *dir);
 
 /* ---- learned preconditioner (no reference counterpart) ------------------
  * A message-passing network on the joint graph of the tangent emits a

@@ -224,6 +224,9 @@ typedef struct {
     double *react;       /* optional [max_steps*NJ*7] reactions of each converged step */
     double *resid;       /* optional [max_steps*2]: |lpf q - f| and |lpf q| at convergence
                             (NR/MNR/MSAL; zero under Newmark) */
+    double *ef;          /* optional [max_steps*(NE_TR*2+NE_FR*14+NE_SH*18)]: the
+                            converged element forces of each step (what output()
+                            averages into results3-5.txt, misc.c:419-449) */
 } orc_path;
 
 /* Driver events of one converged step: the pivot-sign flag of the step's last

@@ -54,7 +54,7 @@ class Path(C.Structure):
                 ("t_tangent", C.c_double), ("t_factor", C.c_double), ("t_solve", C.c_double),
                 ("t_forces", C.c_double), ("t_other", C.c_double),
                 ("n_factor", C.c_long), ("n_solve", C.c_long), ("n_newton", C.c_long),
-                ("events", PI), ("psi", PD), ("react", PD), ("resid", PD)]
+                ("events", PI), ("psi", PD), ("react", PD), ("resid", PD), ("ef", PD)]
 
 # Slots of the per-step driver events (ORC_EV_* in cb_oracle.h).
 EV_DET, EV_IMAG, EV_NEG, EV_DNORM, EV_HALVE = range(5)
@@ -128,6 +128,8 @@ def ref():
         _ref.ref_quad.argtypes = [C.POINTER(Dims), C.c_double, C.c_double, C.c_double, PD, PD,
                                   PD, PD, PD, PD, PD]
         _ref.ref_quad.restype = C.c_int
+        _ref.ref_output_files.argtypes = [C.c_char_p, C.c_int, PD, PI, PD, PD]
+        _ref.ref_output_files.restype = C.c_int
         _ref.ref_test.argtypes = [C.POINTER(Dims), PD, PD, PD, PD, PD, PD, C.c_double, PI,
                                   C.c_double, C.c_double, C.c_double]
         _ref.ref_test.restype = C.c_int
@@ -206,7 +208,7 @@ class OracleModel:
         return r.reshape(self.nj, 7)
 
     def run(self, max_steps: int, backend: str = "oracle", keep_f: bool = False,
-            keep_react: bool = False) -> dict:
+            keep_react: bool = False, keep_ef: bool = False) -> dict:
         """Runs the analysis; returns the converged path and timings (and, with
         keep_react, the reactions [nsteps, NJ, 7] of every converged step)."""
         P = Path()
@@ -227,6 +229,9 @@ class OracleModel:
         P.react = ptr(react) if keep_react else None
         resid = np.zeros((max_steps, 2))
         P.resid = ptr(resid)
+        nef = self.ntr * 2 + self.nfr * 14 + self.nsh * 18
+        efp = np.zeros((max_steps, max(nef, 1))) if keep_ef else None
+        P.ef = ptr(efp) if keep_ef else None
         be = None
         if backend == "ref":
             be = Backend()
@@ -243,6 +248,8 @@ class OracleModel:
             out["f"] = fmat[:n].copy()
         if keep_react:
             out["react"] = react[:n].copy()
+        if keep_ef:
+            out["ef"] = efp[:n].copy()
         return out
 
 

@@ -325,6 +325,7 @@ static void record(run_t *R, double lpf, int itecnt, const double *d, const doub
     if (o->psi) o->psi[o->nsteps] = R->ev_psi;
     if (o->react)
         orc_reactions(R->m, R->ef, R->c1, R->c2, R->c3, o->react + (long)o->nsteps * R->nj * 7);
+    if (o->ef) CP(o->ef + (long)o->nsteps * R->nef, R->ef, R->nef);
     if (o->resid) {
         double rr = 0, qq = 0;
         if (R->m->dm.ALGFLAG <= 3 && f)

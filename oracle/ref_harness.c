@@ -432,3 +432,38 @@ int ref_get_double(const char *name, double *out, long n)
     memcpy(out, src, sizeof(double) * (size_t)n);
     return 0;
 }
+
+/* The reference's own results writer (misc.c:345-451) over a recorded path:
+ * results1..7.txt are opened in dir as main.c:373-379 opens them, output()
+ * writes the headers (flag 0) and then one row per step (flag 1), as the
+ * drivers call it (main.c:2556-2559, 3132).  lpf [nsteps], itecnt [nsteps],
+ * d [nsteps*NEQ], ef [nsteps*(NE_TR*2 + NE_FR*14 + NE_SH*18)].  The model's
+ * sizes and flags must have been set by ref_open. */
+int ref_output_files(const char *dir, int nsteps, const double *lpf, const int *itecnt,
+                     const double *d, const double *ef)
+{
+    FILE *keep[8];
+    char path[512];
+    for (int i = 0; i < 8; ++i) keep[i] = OFP[i];
+    for (int i = 0; i < 7; ++i) {
+        snprintf(path, sizeof path, "%s/results%d.txt", dir, i + 1);
+        OFP[i] = fopen(path, "w");
+        if (!OFP[i]) {
+            for (int k = 0; k < i; ++k) fclose(OFP[k]);
+            for (int k = 0; k < 8; ++k) OFP[k] = keep[k];
+            return 1;
+        }
+    }
+    const long nef = NE_TR * 2 + NE_FR * 14 + NE_SH * 18;
+    double l = lpf[0];
+    int it = itecnt[0];
+    output(&l, &it, (double *)d, (double *)ef, 0);
+    for (int s = 0; s < nsteps; ++s) {
+        l = lpf[s];
+        it = itecnt[s];
+        output(&l, &it, (double *)(d + (long)s * NEQ), (double *)(ef + (long)s * nef), 1);
+    }
+    for (int i = 0; i < 7; ++i) fclose(OFP[i]);
+    for (int k = 0; k < 8; ++k) OFP[k] = keep[k];
+    return 0;
+}

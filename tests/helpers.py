@@ -144,3 +144,38 @@ def arc_residuals(om_params, r, q):
         lhs = dd @ dd + (dl * dl * qq if load else 0.0)
         out[s] = abs(lhs - arc * arc) / (arc * arc)
     return out
+
+
+def parse_results(path: str):
+    """A results file of output() (misc.c:345-451): (header text, rows), each
+    row [lambda, itecnt, values...] from its tab-separated %e / %d fields."""
+    with open(path) as fh:
+        text = fh.read()
+    head, rows = [], []
+    for line in text.split("\n"):
+        fields = [t for t in line.split("\t") if t]
+        try:
+            vals = [float(t) for t in fields]
+            if fields and len(fields) >= 2 and line.startswith("\t"):
+                rows.append(vals)
+                continue
+        except ValueError:
+            pass
+        head.append(line)
+    return "\n".join(head), rows
+
+
+def rows_close(a, b, rel=1.1e-5):
+    """Rows of two results files equal to the 6 significant digits %e keeps:
+    itecnt exactly, every value within one unit of its last printed digit
+    (relative to the file's largest value for entries at round-off level)."""
+    if len(a) != len(b):
+        return False
+    big = max((abs(v) for r in b for v in r[2:]), default=0.0)
+    for ra, rb in zip(a, b):
+        if len(ra) != len(rb) or ra[1] != rb[1]:
+            return False
+        for x, y in zip([ra[0]] + ra[2:], [rb[0]] + rb[2:]):
+            if abs(x - y) > rel * max(abs(y), 1e-9 * big):
+                return False
+    return True
