@@ -257,16 +257,35 @@ void cb_close(cb_model* h) {
     delete h;
 }
 
+static void fill_info(const cb::Model& m, cb_info* out) {
+    out->nj = m.nj; out->ne_tr = m.ntr; out->ne_fr = m.nfr; out->ne_sh = m.nsh;
+    out->neq = m.neq; out->lss = m.lss; out->bandwidth = m.bandwidth;
+    out->nnz_blocks = long(m.colidx.size());
+    out->bs = m.bs;
+    out->anaflag = m.anaflag; out->algflag = m.algflag;
+    out->ncolours_tr = int(m.col_ptr_tr.size()) - 1;
+    out->ncolours_sh = int(m.col_ptr_sh.size()) - 1;
+}
+
 int cb_get_info(const cb_model* h, cb_info* out) {
     if (!h || !out) return fail(-1, "null argument");
-    out->nj = h->m.nj; out->ne_tr = h->m.ntr; out->ne_fr = h->m.nfr; out->ne_sh = h->m.nsh;
-    out->neq = h->m.neq; out->lss = h->m.lss; out->bandwidth = h->m.bandwidth;
-    out->nnz_blocks = long(h->m.colidx.size());
-    out->bs = h->m.bs;
-    out->anaflag = h->m.anaflag; out->algflag = h->m.algflag;
-    out->ncolours_tr = int(h->m.col_ptr_tr.size()) - 1;
-    out->ncolours_sh = int(h->m.col_ptr_sh.size()) - 1;
+    fill_info(h->m, out);
     return 0;
+}
+
+int cb_model_info(const char* path, cb_info* info, long* jcode, double* x, long* jnt) {
+    if (!path) return fail(-1, "null path");
+    GUARD_BEGIN
+    cb::Model m;
+    std::string err;
+    if (!cb::load_model(path, m, err)) return fail(-4, err);
+    cb::build_pattern(m);
+    if (info) fill_info(m, info);
+    if (jcode) std::memcpy(jcode, m.jcode.data(), sizeof(long) * m.jcode.size());
+    if (x) std::memcpy(x, m.x.data(), sizeof(double) * m.x.size());
+    if (jnt) std::memcpy(jnt, m.jnt.data(), sizeof(long) * m.jnt.size());
+    return 0;
+    GUARD_END
 }
 
 int cb_get_dofmaps(const cb_model* h, long* jcode, long* mcode, long* maxa) {
@@ -274,6 +293,12 @@ int cb_get_dofmaps(const cb_model* h, long* jcode, long* mcode, long* maxa) {
     if (jcode) std::memcpy(jcode, h->m.jcode.data(), sizeof(long) * h->m.jcode.size());
     if (mcode) std::memcpy(mcode, h->m.mcode.data(), sizeof(long) * h->m.mcode.size());
     if (maxa) std::memcpy(maxa, h->m.maxa.data(), sizeof(long) * h->m.maxa.size());
+    return 0;
+}
+
+int cb_get_joint_numbers(const cb_model* h, long* jnt) {
+    if (!h || !jnt) return fail(-1, "null argument");
+    std::memcpy(jnt, h->m.jnt.data(), sizeof(long) * h->m.jnt.size());
     return 0;
 }
 
@@ -816,14 +841,7 @@ int cb_batch_count(const cb_batch* b) { return (b && b->bt) ? b->bt->count() : 0
 int cb_batch_get_info(const cb_batch* b, int i, cb_info* out) {
     if (!b || !b->bt || !out) return fail(-1, "null argument");
     if (i < 0 || i >= b->bt->count()) return fail(-1, "model index out of range");
-    const cb::Model& m = b->bt->model(i);
-    out->nj = m.nj; out->ne_tr = m.ntr; out->ne_fr = m.nfr; out->ne_sh = m.nsh;
-    out->neq = m.neq; out->lss = m.lss; out->bandwidth = m.bandwidth;
-    out->nnz_blocks = long(m.colidx.size());
-    out->bs = m.bs;
-    out->anaflag = m.anaflag; out->algflag = m.algflag;
-    out->ncolours_tr = int(m.col_ptr_tr.size()) - 1;
-    out->ncolours_sh = int(m.col_ptr_sh.size()) - 1;
+    fill_info(b->bt->model(i), out);
     return 0;
 }
 

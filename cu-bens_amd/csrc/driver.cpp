@@ -159,6 +159,11 @@ int Analysis::factor_current(int* det) {
         have_piv = true;
         if (rc == 0) rc = rj;
     }
+    // A tangent entry outside the band means the band was sized wrongly; the
+    // reference's skyline holds every entry, so this is never its behaviour.
+    // A zero or non-finite pivot (1) carries on, as solve() ignores skyfact's
+    // status (solve.c:51-89).
+    if (rc == 2) throw DeviceError("tangent entry outside the band of the factorization");
     if (det) {
         *det = 0;
         if (have_piv)

@@ -64,9 +64,10 @@ void cross3(const double* a, const double* b, double* c, bool unit) {
     }
 }
 
-// DOF numbering: joints missing an element type lose the slots only that type
-// uses; then sequential numbering joint by joint, slots 1-6 then warping.
-void number_dofs(Model& m) {
+// struc's last steps (model.c:267-285): per joint the members framing in
+// (warping restraint counts), and joints missing an element type lose the
+// slots only that type uses.
+void autofix_slots(Model& m) {
     std::vector<long> cnt(size_t(m.nj) * 3, 0);
     for (long e = 0; e < m.ntr; ++e)
         for (int k = 0; k < 2; ++k) cnt[(m.minc[e * 2 + k] - 1) * 3 + 0]++;
@@ -87,6 +88,98 @@ void number_dofs(Model& m) {
         if (cnt[j * 3 + 0]) { std::fill(jc + 3, jc + 7, 0L); continue; }
         std::fill(jc, jc + 7, 0L);
     }
+}
+
+// Joint renumbering for a smaller bandwidth, OPTFLAG 2 (graph, optnum and
+// updatenum, model.c:649-935).  Every joint in turn starts a numbering that
+// gives the unnumbered neighbours of each numbered joint the next numbers,
+// in the order of its adjacency list: per joint its trusses, frames and
+// shells in element order (a shell adds its other two vertices).  An attempt
+// stops extending a joint's neighbours once one lands farther than the best
+// bandwidth so far; a completed attempt with a smaller bandwidth, or an equal
+// one and a smaller profile (the running sum of the bandwidth), replaces the
+// numbering.  jcode rows and incidences are renumbered; wrpres keeps the
+// input numbering, as updatenum leaves it (model.c:893-898).
+void renumber_joints(Model& m) {
+    const long nj = m.nj, ntr = m.ntr, nfr = m.nfr, nsh = m.nsh;
+    const size_t nju = static_cast<size_t>(nj);
+    std::vector<std::vector<long>> tr(nju), fr(nju), sh(nju);
+    for (long e = 0; e < ntr; ++e)
+        for (int a = 0; a < 2; ++a) tr[size_t(m.minc[e * 2 + a] - 1)].push_back(e);
+    for (long e = 0; e < nfr; ++e)
+        for (int a = 0; a < 2; ++a) fr[size_t(m.minc[ntr * 2 + e * 2 + a] - 1)].push_back(e);
+    const long so = ntr * 2 + nfr * 2;
+    for (long e = 0; e < nsh; ++e)
+        for (int a = 0; a < 3; ++a) sh[size_t(m.minc[so + e * 3 + a] - 1)].push_back(e);
+    std::vector<std::vector<long>> adj(nju);
+    for (long i = 0; i < nj; ++i) {
+        auto& a = adj[size_t(i)];
+        for (long e : tr[size_t(i)]) {
+            const long l = m.minc[e * 2] - 1, r = m.minc[e * 2 + 1] - 1;
+            a.push_back(l == i ? r : l);
+        }
+        for (long e : fr[size_t(i)]) {
+            const long l = m.minc[ntr * 2 + e * 2] - 1, r = m.minc[ntr * 2 + e * 2 + 1] - 1;
+            a.push_back(l == i ? r : l);
+        }
+        for (long e : sh[size_t(i)]) {
+            const long v0 = m.minc[so + e * 3] - 1, v1 = m.minc[so + e * 3 + 1] - 1, v2 = m.minc[so + e * 3 + 2] - 1;
+            if (v0 == i) { a.push_back(v1); a.push_back(v2); }
+            else if (v1 == i) { a.push_back(v0); a.push_back(v2); }
+            else { a.push_back(v0); a.push_back(v1); }
+        }
+    }
+    long band = nj * 7, prof = band * band;
+    bool changed = false;
+    std::vector<long> newjt(nju), joint(nju);
+    for (long ik = 0; ik < nj; ++ik) {
+        std::fill(newjt.begin(), newjt.end(), -1L);
+        std::fill(joint.begin(), joint.end(), -1L);
+        long mx = 0, nprof = 0, i = 0, k = 0, kmax = 0;
+        newjt[0] = ik;
+        joint[size_t(ik)] = 0;
+        do {
+            if (newjt[size_t(i)] != -1) {
+                for (long k5 : adj[size_t(newjt[size_t(i)])]) {
+                    if (joint[size_t(k5)] != -1) continue;
+                    ++k;
+                    ++kmax;
+                    newjt[size_t(k)] = k5;
+                    joint[size_t(k5)] = k;
+                    const long nb = std::labs(i - k);
+                    if (nb > band) break;
+                    if (nb > mx) mx = nb;
+                }
+            } else if (k != 0) {
+                // a joint no numbered joint reaches: a placeholder entry
+                // that is never visited, as in optnum
+                ++k;
+                newjt[size_t(k)] = newjt[size_t(k - 1)] - 1;
+            } else {
+                break;
+            }
+            nprof += mx;
+            ++i;
+        } while (k < nj - 1);
+        if (k != nj - 1) continue;
+        if (mx < band || (mx == band && nprof < prof)) {
+            changed = true;
+            band = mx;
+            prof = nprof;
+            for (long j = 0; j < nj; ++j) m.jnt[size_t(j)] = joint[size_t(j)] != -1 ? joint[size_t(j)] : ++kmax;
+        }
+    }
+    if (!changed) return;
+    std::vector<long> jc(m.jcode);
+    for (long i = 0; i < nj; ++i)
+        for (int k = 0; k < 7; ++k) m.jcode[size_t(m.jnt[size_t(i)] * 7 + k)] = jc[size_t(i * 7 + k)];
+    for (auto& v : m.minc) v = m.jnt[size_t(v - 1)] + 1;
+}
+
+// codes (model.c:937-1050): sequential numbering joint by joint, slots 1-6
+// then warping, and the element DOF maps.
+void number_dofs(Model& m) {
+    long off = 0;
     long neq = 0;
     for (long j = 0; j < m.nj; ++j) {
         long* jc = &m.jcode[j * 7];
@@ -177,7 +270,7 @@ bool load_model(const char* path, Model& m, std::string& err) {
     long nsbr = t.l(), nfbr = t.l();
     if (t.bad()) FAIL("truncated header");
     if (nsbr || nfbr) FAIL("brick elements are not on this path");
-    if (m.optflag == 2) FAIL("OPTFLAG 2 (bandwidth renumbering) is not on this path");
+    if (m.optflag != 1 && m.optflag != 2) FAIL("OPTFLAG must be 1 or 2");
     if (m.anaflag != 2) FAIL("only ANAFLAG 2 (geometrically nonlinear elastic) runs on the GPU path");
     if (m.algflag != 1 && m.algflag != 2 && m.algflag != 3 && m.algflag != 5)
         FAIL("ALGFLAG must be 1, 2, 3 or 5");
@@ -199,6 +292,10 @@ bool load_model(const char* path, Model& m, std::string& err) {
         if (k == 7) m.wrpres[(j - 1) * 3] = int(t.l());
         m.jcode[(j - 1) * 7 + (k - 1)] = 0;
     }
+    m.jnt.resize(size_t(nj));
+    for (long j = 0; j < nj; ++j) m.jnt[size_t(j)] = j;
+    autofix_slots(m);
+    if (m.optflag == 2) renumber_joints(m);
     number_dofs(m);
     // the device layout carries 6 slots per joint: warping DOFs must be fixed
     for (long j = 0; j < nj; ++j)
@@ -209,7 +306,8 @@ bool load_model(const char* path, Model& m, std::string& err) {
     }
     profile(m);
     m.x.assign(size_t(nj * 3), 0);
-    for (auto& v : m.x) v = t.d();
+    for (long j = 0; j < nj; ++j)   // coordinates of input joint j land at its new number (main.c:1394-1396)
+        for (int k = 0; k < 3; ++k) m.x[size_t(m.jnt[size_t(j)] * 3 + k)] = t.d();
 
     const long np = ntr + nfr + nsh, ntri = ntr + nfr * 3 + nsh * 3;
     m.emod.assign(size_t(np), 0); m.yield.assign(size_t(np), 0);
@@ -344,6 +442,8 @@ bool load_model(const char* path, Model& m, std::string& err) {
             if (t.bad()) FAIL("truncated load histories");
             if (jt == 0) break;
             if (jt < 1 || jt > nj || dir < 1 || dir > 7) FAIL("load history joint out of range");
+            // load histories keep the input joint number even under OPTFLAG 2:
+            // model.c:1524 indexes jcode with it directly
             if (m.algflag == 5 && mag == 0) mag = 1e-30;
             long ks = m.jcode[(jt - 1) * 7 + dir - 1];
             if (ks != prev) i = 0;
@@ -377,6 +477,7 @@ bool load_model(const char* path, Model& m, std::string& err) {
             if (jt == 0) break;
             if (jt < 1 || jt > nj || dir < 1 || dir > 7) FAIL("joint load out of range");
             any = true;
+            jt = m.jnt[size_t(jt - 1)] + 1;   // model.c:1318
             long k = m.jcode[(jt - 1) * 7 + dir - 1];
             if (k) m.q[k - 1] = mag;
         }
@@ -396,6 +497,7 @@ bool load_model(const char* path, Model& m, std::string& err) {
         long dir = t.l();
         m.dk = t.d();
         if (jt < 1 || jt > nj || dir < 1 || dir > 7) FAIL("bad arc-length control DOF");
+        jt = m.jnt[size_t(jt - 1)] + 1;       // arc.c:49
         long k = m.jcode[(jt - 1) * 7 + dir - 1];
         if (!k) FAIL("initially prescribed displacement at fixed degree of freedom");
         m.dkdof = k - 1;

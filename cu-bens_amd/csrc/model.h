@@ -15,6 +15,7 @@ struct Model {
     long nj = 0, ntr = 0, nfr = 0, nsh = 0, neq = 0;
     // topology, 1-based as in the reference (model.c:42-1050)
     std::vector<long> minc, jcode, mcode, maxa, kht;
+    std::vector<long> jnt;  // input joint -> joint number used (0-based; OPTFLAG 2 renumbers)
     std::vector<int> wrpres;
     long lss = 0;
     std::vector<double> x;  // NJ*3

@@ -97,6 +97,8 @@ ABI = {
     "cb_close": (None, [C.c_void_p]),
     "cb_get_info": (C.c_int, [C.c_void_p, C.POINTER(Info)]),
     "cb_get_dofmaps": (C.c_int, [C.c_void_p, PL, PL, PL]),
+    "cb_get_joint_numbers": (C.c_int, [C.c_void_p, PL]),
+    "cb_model_info": (C.c_int, [C.c_char_p, C.c_void_p, PL, PD, PL]),
     "cb_get_pattern": (C.c_int, [C.c_void_p, PI32, PI32]),
     "cb_get_load": (C.c_int, [C.c_void_p, PD]),
     "cb_state_set": (C.c_int, [C.c_void_p, C.c_char_p, PD, C.c_long]),
@@ -179,6 +181,19 @@ def dots(a_ptrs, b_ptrs, length: int) -> np.ndarray:
     return out
 
 
+def model_info(path: str) -> dict:
+    """A model_def.txt read on the host alone (cb_model_info): sizes, the DOF
+    map jcode [NJ*7], coordinates x [NJ*3] and the renumbering jnt [NJ]."""
+    inf = Info()
+    _check(lib().cb_model_info(path.encode(), C.byref(inf), None, None, None))
+    jc = np.zeros(inf.nj * 7, np.int64)
+    x = np.zeros(inf.nj * 3)
+    jnt = np.zeros(inf.nj, np.int64)
+    _check(lib().cb_model_info(path.encode(), C.byref(inf), _p(jc, PL), _p(x), _p(jnt, PL)))
+    return {"info": inf, "nj": inf.nj, "neq": inf.neq, "algflag": inf.algflag, "jcode": jc, "x": x,
+            "jnt": jnt}
+
+
 class GpuModel:
     """A model opened on the GPU (cb_open)."""
 
@@ -217,6 +232,12 @@ class GpuModel:
         mx = np.zeros(self.neq + 1, np.int64)
         _check(lib().cb_get_dofmaps(self.h, _p(jc, PL), _p(mc, PL), _p(mx, PL)))
         return jc, mc[: self.ntr * 6 + self.nfr * 14 + self.nsh * 18], mx
+
+    def joint_numbers(self) -> np.ndarray:
+        """jnt: input joint i is joint jnt[i] of the model (identity unless OPTFLAG 2)."""
+        jnt = np.zeros(self.nj, np.int64)
+        _check(lib().cb_get_joint_numbers(self.h, _p(jnt, PL)))
+        return jnt
 
     def pattern(self):
         rp = np.zeros(self.nj + 1, np.int32)

@@ -54,6 +54,19 @@ def _fmt(v: float) -> str:
     return repr(float(v))
 
 
+def _joint_ids(nj: int, scramble: int | None) -> np.ndarray:
+    """Output id (1-based) of each generator joint: the identity, or a seeded
+    permutation (SplitMix64 keys) that scatters the numbering so that the
+    input's own joint order gives a wide band."""
+    if scramble is None:
+        return np.arange(1, nj + 1)
+    keys = SplitMix64(scramble).next_u64(nj)
+    order = np.argsort(keys, kind="stable")
+    ids = np.empty(nj, np.int64)
+    ids[order] = np.arange(1, nj + 1)
+    return ids
+
+
 @dataclass
 class ModelText:
     """A model_def.txt image plus the facts tests need about it."""
@@ -75,7 +88,8 @@ class ModelText:
 # --------------------------------------------------------------------------
 def truss_dome(n: int = 10, rise: float = 0.5, seed: int | None = None,
                load: float = -80.0, steps: int = 20, tol=(1e-9, 1e-6, 1e-12),
-               algflag: int = 1, msal: dict | None = None) -> ModelText:
+               algflag: int = 1, msal: dict | None = None, optflag: int = 1,
+               scramble: int | None = None) -> ModelText:
     """n x n bays at 1 m, paraboloid crown rise, boundary pinned, centre load.
 
     The crown of a 0.5 m paraboloid over 10 m is locally flat (bar slope 0.01),
@@ -89,7 +103,8 @@ def truss_dome(n: int = 10, rise: float = 0.5, seed: int | None = None,
     algflag 3 runs the arc-length method instead, with ``load`` as the
     reference load and the centre joint's direction 3 as the control DOF;
     ``msal`` overrides its parameters (dk, alpha, psi_thresh, iteopt, lpfmax,
-    dkimax, itemax, submax, imagmax, negmax).
+    dkimax, itemax, submax, imagmax, negmax).  ``scramble`` and ``optflag`` as
+    for shell_panel.
     """
     rng = SplitMix64(workload_seed(1) if seed is None else seed)
     m = n + 1
@@ -99,7 +114,8 @@ def truss_dome(n: int = 10, rise: float = 0.5, seed: int | None = None,
     c = n / 2.0
     r2max = 2 * c * c
     z = rise * (1 - ((x - c) ** 2 + (y - c) ** 2) / r2max) * (1 + rng.uniform(m * m, -0.02, 0.02))
-    node = lambda i, j: j * m + i + 1          # noqa: E731  joint id, i fastest
+    ids = _joint_ids(m * m, scramble)
+    node = lambda i, j: int(ids[j * m + i])     # noqa: E731  joint id, i fastest
     bars = []
     for j in range(m):
         for i in range(n):
@@ -114,7 +130,7 @@ def truss_dome(n: int = 10, rise: float = 0.5, seed: int | None = None,
     area = 1e-4 * (1 + rng.uniform(len(bars), -0.05, 0.05))
     out = io.StringIO()
     w = out.write
-    w(f"2\n{algflag}\n0\n1\n{m * m}\n{len(bars)},0,0,0,0\n")
+    w(f"2\n{algflag}\n0\n{optflag}\n{m * m}\n{len(bars)},0,0,0,0\n")
     for a, b in bars:
         w(f"{a},{b}\n")
     for j in range(m):
@@ -123,7 +139,9 @@ def truss_dome(n: int = 10, rise: float = 0.5, seed: int | None = None,
                 for d in (1, 2, 3):
                     w(f"{node(i, j)},{d}\n")
     w("0,0\n")
-    for k in range(m * m):
+    at = np.empty(m * m, np.int64)
+    at[ids - 1] = np.arange(m * m)
+    for k in at:
         w(f"{_fmt(x[k])},{_fmt(y[k])},{_fmt(z[k])}\n")
     for a in area:
         w(f"200000000000.0,{_fmt(a)},7850.0,1e30\n")
@@ -218,7 +236,8 @@ def shell_panel(n: int = 64, radius: float = 2.54, angle: float = 0.2, length: f
                 alpha: float = 1.0, psi_thresh: float = 0.1, iteopt: int = 6,
                 itemax: int = 40, submax: int = 10, imagmax: int = 10, negmax: int = 10,
                 tol=(1.0, 1e-3, 1.0), load: float = -1.0, lpfmax: float = 1e30,
-                dkimax: float = 1e30) -> ModelText:
+                dkimax: float = 1e30, optflag: int = 1, scramble: int | None = None,
+                edge_beam: float = 0.0, tie_area: float = 0.0) -> ModelText:
     """n x n quads, each split on one fixed diagonal into two CST+DKT triangles.
 
     Straight edges (i = 0 and i = n) are hinged (translations fixed); the curved
@@ -233,6 +252,14 @@ def shell_panel(n: int = 64, radius: float = 2.54, angle: float = 0.2, length: f
     force carries no drilling term while its tangent does (shell.c:482-484), so
     the modified Newton iteration of the arc-length method converges linearly
     (about 0.45 per iteration); the defaults are levels that iteration reaches.
+
+    ``scramble`` (a seed) permutes the joint numbers of the input file;
+    ``optflag`` 2 asks the reader to renumber the joints (model.c:649-935).
+
+    A mixed model: ``edge_beam`` > 0 adds frame members along both curved
+    edges (circular hollow sections of that outer radius, the shell's E, warping
+    fixed at their joints), ``tie_area`` > 0 adds truss bars of that area on the
+    second diagonal of the quads of the middle row.
     """
     rng = SplitMix64(workload_seed(config_index) if seed is None else seed)
     m = n + 1
@@ -244,7 +271,8 @@ def shell_panel(n: int = 64, radius: float = 2.54, angle: float = 0.2, length: f
     x = (radius + w0) * np.sin(phi)
     z = (radius + w0) * np.cos(phi) - radius * math.cos(angle / 2)
     y = length * jj / n
-    node = lambda i, j: j * m + i + 1          # noqa: E731
+    ids = _joint_ids(m * m, scramble)
+    node = lambda i, j: int(ids[j * m + i])     # noqa: E731
     tris = []
     for j in range(n):
         for i in range(n):
@@ -252,33 +280,57 @@ def shell_panel(n: int = 64, radius: float = 2.54, angle: float = 0.2, length: f
             tris.append((a, b, c))
             tris.append((a, c, d))
     ne = len(tris)
+    bars = [(node(i + 1, n // 2), node(i, n // 2 + 1)) for i in range(n)] if tie_area > 0 else []
+    beams = ([(node(i, j), node(i + 1, j)) for j in (0, n) for i in range(n)] if edge_beam > 0 else [])
     if thick_sigma > 0:
         t_e = thick * np.exp(thick_sigma * smooth_field(rng, tris, ii, jj, n))
     else:
         t_e = np.full(ne, thick)
     out = io.StringIO()
     w = out.write
-    w(f"2\n3\n0\n1\n{m * m}\n0,0,{ne},0,0\n")
+    w(f"2\n3\n0\n{optflag}\n{m * m}\n{len(bars)},{len(beams)},{ne},0,0\n")
+    for a, b in bars + beams:
+        w(f"{a},{b}\n")
     for t in tris:
         w(f"{t[0]},{t[1]},{t[2]}\n")
     for j in range(m):
         for i in (0, n):
             for dd in (1, 2, 3):
                 w(f"{node(i, j)},{dd}\n")
+    for jt in sorted({v for b in beams for v in b}):
+        w(f"{jt},7,0\n")
     w("0,0\n")
-    for k in range(m * m):
+    at = np.empty(m * m, np.int64)
+    at[ids - 1] = np.arange(m * m)                # generator joint of each output id
+    for k in at:
         w(f"{_fmt(x[k])},{_fmt(y[k])},{_fmt(z[k])}\n")
+    for _ in bars:
+        w(f"{_fmt(emod)},{_fmt(tie_area)},1200.0,1e30\n")
+    if beams:
+        sec = chs_section(edge_beam, E=emod)
+        for _ in beams:
+            w(f"{_fmt(sec['E'])},{_fmt(sec['G'])},1200.0,{_fmt(sec['A'])},{_fmt(sec['I'])},"
+              f"{_fmt(sec['I'])},{_fmt(sec['J'])},0.0\n")
+        w("0,0,0,0,0,0,0\n")
+        for a, _b in beams:
+            k = at[a - 1]                          # aux point off the member, across the panel
+            w(f"{_fmt(x[k])},{_fmt(y[k] + 1.0)},{_fmt(z[k])}\n")
+        for _ in beams:
+            w("1e+30,1.0,1.0\n")
+        w("0,0,0,0,0\n")
     for t in t_e:
         w(f"{_fmt(emod)},{_fmt(nu)},{_fmt(t)},1200.0,1e30\n")
     ctr = node(n // 2, n // 2)
     w(f"{ctr},3,{_fmt(load)}\n0,0,0\n")
+    if beams:
+        w("0,0,0\n")                              # no distributed member loads
     w(f"{ctr},3,{_fmt(dk)}\n")
     w(f"{_fmt(alpha)}\n{_fmt(psi_thresh)}\n{iteopt}\n")
     w(f"{_fmt(lpfmax)},{_fmt(dkimax)}\n")
     w(f"{itemax},{submax},{imagmax},{negmax}\n")
     w(f"{tol[0]},{tol[1]},{tol[2]}\n")
     neq = 6 * m * m - 3 * 2 * m
-    return ModelText(out.getvalue(), m * m, n_sh=ne,
+    return ModelText(out.getvalue(), m * m, n_tr=len(bars), n_fr=len(beams), n_sh=ne,
                      info={"centre_joint": ctr, "neq": neq, "n": n})
 
 
@@ -314,7 +366,8 @@ def chs_section(r_o: float, E: float = 2e11):
 def space_frame(n_elem: int = 400, kind: str = "tower", seed: int | None = None,
                 instance: int = 0, steps: int = 200, dt: float = 1e-3, amp: float | None = None,
                 tol=(1e-8, 1e-8, 1e-8), algflag: int = 5, numopt: int = 0,
-                spectrds: float = 1.0, itemax: int = 20, slice_shape: tuple | None = None) -> ModelText:
+                spectrds: float = 1.0, itemax: int = 20, slice_shape: tuple | None = None,
+                optflag: int = 1, scramble: int | None = None) -> ModelText:
     """A random tower or lattice girder of about ``n_elem`` frame members.
 
     tower : square storeys of 4 columns, 4 beams and 8 face X-braces; plan
@@ -330,7 +383,11 @@ def space_frame(n_elem: int = 400, kind: str = "tower", seed: int | None = None,
     Joints are numbered level by level so the skyline band stays narrow.
     slice_shape = (ny, nz) replaces the drawn girder cross-section (the draws
     are still made, so the rest of the model is the same); a wide slice gives
-    a wide band.
+    a wide band.  ``scramble`` (a seed) permutes the joint numbers of the
+    input file; ``optflag`` 2 asks the reader to renumber the joints
+    (model.c:649-935).  The reference applies that renumbering to static loads
+    but not to load histories (model.c:1524), so a scrambled Newmark model with
+    optflag 2 loads other joints than the same model with optflag 1.
     """
     rng = SplitMix64(workload_seed(4, instance) if seed is None else seed)
     joints, members, fixed, loaded = [], [], [], []
@@ -385,6 +442,8 @@ def space_frame(n_elem: int = 400, kind: str = "tower", seed: int | None = None,
     if amp is None:
         amp = float(rng.uniform(1, 1e3, 5e3)[0])
     X = np.array(joints)
+    ids = _joint_ids(nj, scramble)
+    jo = lambda j: int(ids[j - 1])              # noqa: E731  output id of generator joint j
     out = io.StringIO()
     w = out.write
     w(f"2\n{algflag}\n")
@@ -392,18 +451,21 @@ def space_frame(n_elem: int = 400, kind: str = "tower", seed: int | None = None,
         # checkpoint interval beyond the last step: no checkpoint is written
         # (CHKPT 0 would divide by zero in the reference's k % CHKPT)
         w(f"{steps + 1},0\n")
-    w(f"0\n1\n{nj}\n0,{ne},0,0,0\n")
+    w(f"0\n{optflag}\n{nj}\n0,{ne},0,0,0\n")
     for a, b_ in members:
-        w(f"{a},{b_}\n")
+        w(f"{jo(a)},{jo(b_)}\n")
     for j in fixed:
         for d in range(1, 7):
-            w(f"{j},{d}\n")
+            w(f"{jo(j)},{d}\n")
     for j in range(1, nj + 1):
         w(f"{j},7,0\n")
     w("0,0\n")
     if algflag > 3:
         w("0,0\n")
-    for p in X:
+    at = np.empty(nj, np.int64)
+    at[ids - 1] = np.arange(nj)
+    for k in at:
+        p = X[k]
         w(f"{_fmt(p[0])},{_fmt(p[1])},{_fmt(p[2])}\n")
     secs = [chs_section(r) for r in r_o]
     for s_ in secs:
@@ -424,18 +486,18 @@ def space_frame(n_elem: int = 400, kind: str = "tower", seed: int | None = None,
         hist = amp * np.sin(2 * np.pi * freq * t)
         for j in loaded:
             for v in hist:
-                w(f"{j},1,{_fmt(v)}\n")
+                w(f"{jo(j)},1,{_fmt(v)}\n")
         w("0,0,0\n0,0,0\n0,0,0,0,0\n")
         w(f"{numopt},{_fmt(spectrds)}\n1.0,1.0,1.0,1.0,1.0\n{itemax},10,1000\n")
         w(f"{tol[0]},{tol[1]},{tol[2]}\n")
     else:
         for j in loaded:
-            w(f"{j},1,{_fmt(amp)}\n")
+            w(f"{jo(j)},1,{_fmt(amp)}\n")
         w("0,0,0\n0,0,0\n")
         w(f"1.0,0.05,0.05,0.05,0.05\n{itemax},10,1000\n{tol[0]},{tol[1]},{tol[2]}\n")
     return ModelText(out.getvalue(), nj, n_fr=ne,
-                     info={"kind": kind, "freq": freq, "amp": amp, "fixed": fixed,
-                           "loaded": loaded, "neq": 6 * (nj - len(fixed))})
+                     info={"kind": kind, "freq": freq, "amp": amp, "fixed": [jo(j) for j in fixed],
+                           "loaded": [jo(j) for j in loaded], "neq": 6 * (nj - len(fixed))})
 
 
 def frame_batch(count: int = 512, lo: int = 200, hi: int = 5000, **kw) -> list[ModelText]:

@@ -102,6 +102,17 @@ int cb_get_info(const cb_model *m, cb_info *out);
 /* DOF maps as produced by codes()/skylin(): jcode [NJ*7], mcode
  * [NE_TR*6 | NE_FR*14 | NE_SH*18], maxa [NEQ+1] (model.c:937-1283). */
 int cb_get_dofmaps(const cb_model *m, long *jcode, long *mcode, long *maxa);
+/* Joint renumbering jnt [NJ]: input joint i (0-based) is joint jnt[i] of the
+ * model (struc, model.c:143; updatenum, model.c:895-925).  The identity under
+ * OPTFLAG 1; under OPTFLAG 2 every per-joint array of this interface (x,
+ * jcode, the results files) is in the renumbered order, as in the reference. */
+int cb_get_joint_numbers(const cb_model *m, long *jnt);
+/* Reads a model_def.txt on the host only (no device needed): the input
+ * sequence of cb_open (main.c:330-1653) without the device set-up, for tools
+ * such as plotting that need the model's sizes, DOF map jcode [NJ*7],
+ * coordinates x [NJ*3] and renumbering jnt [NJ].  Any array may be NULL
+ * (call once with NULL arrays to size them). */
+int cb_model_info(const char *path, cb_info *info, long *jcode, double *x, long *jnt);
 /* Joint-block pattern of the device tangent: rowptr [NJ+1], colidx [nnz_blocks]. */
 int cb_get_pattern(const cb_model *m, int32_t *rowptr, int32_t *colidx);
 /* Reference load vector q [NEQ] (model.c:1331). */
@@ -280,7 +291,8 @@ int cb_learned_blocks(cb_learned *L, cb_model *m, float *gd, float *go, long *nv
  * model_def.txt each, ALGFLAG 5) run side by side on one GPU.  Each round
  * takes one Newton iteration of every running model; each model keeps its own
  * time steps, step halving and convergence tests, and its path is the one
- * cb_run gives for it alone.  The batch has no reference counterpart (the
+ * cb_run gives for it alone (bitwise with CB_SWEEP_ROWS, whose summation
+ * order the batch's solve keeps).  The batch has no reference counterpart (the
  * reference runs one model per process); it replaces running main() once per
  * model. */
 typedef struct cb_batch cb_batch;

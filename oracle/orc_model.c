@@ -83,7 +83,10 @@ static void cross3(const double *a, const double *b, double *c, int unit)
 }
 
 /* Joint numbering rules of struc (model.c:222-285) and codes (model.c:941-1050). */
-static int number_dofs(orc_model *m, char *err, int errlen)
+/* struc's last steps (model.c:146-149, 267-285): the members framing into
+ * each joint (warping restraint counts) and the slots a joint loses when no
+ * element of the types that use them meets it. */
+static void autofix(orc_model *m)
 {
     orc_dims *d = &m->dm;
     long nj = d->NJ;
@@ -111,8 +114,113 @@ static int number_dofs(orc_model *m, char *err, int errlen)
         for (int k = 0; k < 7; ++k) jc[k] = 0;                 /* floating joint     */
     }
     free(cnt);
+}
 
-    /* sequential numbering, joint-major, DOFs 1-6 then warping */
+/* graph, optnum and updatenum (model.c:649-935), OPTFLAG 2.  adjacency of
+ * joint i as graph() lists it: its trusses, frames and shells in element
+ * order (a truss or frame adds its other end, a shell its other two vertices
+ * in vertex order).  Each joint in turn seeds a level numbering; an attempt
+ * stops extending a joint's neighbours at the first one that lands farther
+ * than the best bandwidth so far, and a completed attempt replaces the
+ * numbering when its bandwidth is smaller, or equal with a smaller profile.
+ * jcode rows move to the new numbers and incidences are renamed; wrpres is
+ * left in the input numbering (updatenum copies it onto itself,
+ * model.c:893-898). */
+static int renumber(orc_model *m)
+{
+    orc_dims *d = &m->dm;
+    const long nj = d->NJ, ntr = d->NE_TR, nfr = d->NE_FR, nsh = d->NE_SH;
+    const long fo = ntr * 2, so = ntr * 2 + nfr * 2;
+    long *deg = zalloc((size_t)nj + 1, sizeof(long));
+    for (long e = 0; e < ntr; ++e) { deg[m->minc[e * 2] - 1]++; deg[m->minc[e * 2 + 1] - 1]++; }
+    for (long e = 0; e < nfr; ++e) { deg[m->minc[fo + e * 2] - 1]++; deg[m->minc[fo + e * 2 + 1] - 1]++; }
+    for (long e = 0; e < nsh; ++e)
+        for (int a = 0; a < 3; ++a) deg[m->minc[so + e * 3 + a] - 1] += 2;
+    long *xadj = zalloc((size_t)nj + 1, sizeof(long));
+    for (long i = 0; i < nj; ++i) xadj[i + 1] = xadj[i] + deg[i];
+    long *adj = zalloc((size_t)xadj[nj] + 1, sizeof(long));
+    long *fill = zalloc((size_t)nj + 1, sizeof(long));
+    for (long i = 0; i < nj; ++i) fill[i] = xadj[i];
+    /* three passes keep each joint's list in type order, elements ascending */
+    for (long e = 0; e < ntr; ++e) {
+        long l = m->minc[e * 2] - 1, r = m->minc[e * 2 + 1] - 1;
+        adj[fill[l]++] = r;
+        adj[fill[r]++] = l;
+    }
+    for (long e = 0; e < nfr; ++e) {
+        long l = m->minc[fo + e * 2] - 1, r = m->minc[fo + e * 2 + 1] - 1;
+        adj[fill[l]++] = r;
+        adj[fill[r]++] = l;
+    }
+    for (long e = 0; e < nsh; ++e) {
+        long v[3] = {m->minc[so + e * 3] - 1, m->minc[so + e * 3 + 1] - 1, m->minc[so + e * 3 + 2] - 1};
+        for (int a = 0; a < 3; ++a) {
+            long j = v[a];
+            for (int b = 0; b < 3; ++b)
+                if (b != a) adj[fill[j]++] = v[b];
+        }
+    }
+    long band = nj * 7, prof = band * band;
+    int changed = 0;
+    long *newjt = zalloc((size_t)nj + 1, sizeof(long)), *joint = zalloc((size_t)nj + 1, sizeof(long));
+    for (long ik = 0; ik < nj; ++ik) {
+        for (long i = 0; i < nj; ++i) joint[i] = newjt[i] = -1;
+        long mx = 0, nprof = 0, i = 0, k = 0, kmax = 0;
+        newjt[0] = ik;
+        joint[ik] = 0;
+        do {
+            if (newjt[i] != -1) {
+                long node = newjt[i];
+                for (long jj = xadj[node]; jj < xadj[node + 1]; ++jj) {
+                    long k5 = adj[jj];
+                    if (joint[k5] == -1) {
+                        k++;
+                        kmax++;
+                        newjt[k] = k5;
+                        joint[k5] = k;
+                        long nb = labs(i - k);
+                        if (nb <= band) {
+                            if (nb > mx) mx = nb;
+                        } else {
+                            break;
+                        }
+                    }
+                }
+            } else if (k != 0) {
+                k++;
+                newjt[k] = newjt[k - 1] - 1;
+            } else {
+                break;
+            }
+            nprof += mx;
+            i++;
+        } while (k < nj - 1);
+        if (k == nj - 1 && (mx < band || (mx == band && nprof < prof))) {
+            changed = 1;
+            band = mx;
+            prof = nprof;
+            for (long j = 0; j < nj; ++j) m->jnt[j] = joint[j] != -1 ? joint[j] : ++kmax;
+        }
+    }
+    if (changed) {
+        long *jc = zalloc((size_t)nj * 7, sizeof(long));
+        memcpy(jc, m->jcode, sizeof(long) * (size_t)nj * 7);
+        for (long i = 0; i < nj; ++i)
+            for (int k = 0; k < 7; ++k) m->jcode[m->jnt[i] * 7 + k] = jc[i * 7 + k];
+        free(jc);
+        for (long i = 0; i < ntr * 2 + nfr * 2 + nsh * 3; ++i) m->minc[i] = m->jnt[m->minc[i] - 1] + 1;
+    }
+    free(deg); free(xadj); free(adj); free(fill); free(newjt); free(joint);
+    return 0;
+}
+
+/* codes (model.c:937-1050): sequential numbering, joint-major, DOFs 1-6 then
+ * warping, and the element DOF maps */
+static int number_dofs(orc_model *m, char *err, int errlen)
+{
+    orc_dims *d = &m->dm;
+    long nj = d->NJ;
+    long off;
     long neq = 0;
     for (long j = 0; j < nj; ++j) {
         long *jc = m->jcode + j * 7;
@@ -230,7 +338,7 @@ orc_model *orc_open(const char *path, char *err, int errlen)
     long nsbr = tk_long(&t), nfbr = tk_long(&t);
     if (t.bad) FAIL("truncated header");
     if (nsbr || nfbr) FAIL("brick elements are not on this path");
-    if (d->OPTFLAG == 2) FAIL("OPTFLAG 2 (bandwidth renumbering) is not on this path");
+    if (d->OPTFLAG != 1 && d->OPTFLAG != 2) FAIL("OPTFLAG must be 1 or 2");
     if (d->ANAFLAG != 2 && d->ANAFLAG != 1) FAIL("ANAFLAG %d not supported (elastic only)", d->ANAFLAG);
     if (d->ALGFLAG != 1 && d->ALGFLAG != 2 && d->ALGFLAG != 3 && d->ALGFLAG != 5)
         FAIL("ALGFLAG %d not supported", d->ALGFLAG);
@@ -260,6 +368,8 @@ orc_model *orc_open(const char *path, char *err, int errlen)
         if (k == 7) m->wrpres[(j - 1) * 3] = (int)tk_long(&t);
         m->jcode[(j - 1) * 7 + (k - 1)] = 0;
     }
+    autofix(m);
+    if (d->OPTFLAG == 2) renumber(m);
     number_dofs(m, err, errlen);
     long neq = d->NEQ;
 

@@ -18,6 +18,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #include "prototypes.h"
 #include "cb_oracle.h"
@@ -313,7 +314,10 @@ int ref_open(const char *path, orc_dims *dims_out, long *lss_out)
     if (fscanf(IFP[0], "%ld,%ld,%ld,%ld,%ld\n", &NE_TR, &NE_FR, &NE_SH, &NE_SBR, &NE_FBR) != 5)
         return 1;
     NE_BR = NE_SBR + NE_FBR;
-    if (NE_BR || OPTFLAG == 2) return 1;
+    if (NE_BR) return 1;
+    /* with OPTFLAG 2 the reference also writes the renumbered input to
+     * model_def_OPT.txt through IFP[1] (main.c:446-455) */
+    if (OPTFLAG == 2 && !IFP[1]) IFP[1] = fopen("/dev/null", "w");
     long ntri = NE_TR + NE_FR * 3 + NE_SH * 3, nprop = NE_TR + NE_FR + NE_SH;
     RM.jcode = za(NJ * 7, sizeof(long));
     RM.minc = za(NE_TR * 2 + NE_FR * 2 + NE_SH * 3, sizeof(long));
@@ -396,6 +400,7 @@ int ref_get_long(const char *name, long *out, long n)
     else if (!strcmp(name, "mcode")) src = RM.mcode;
     else if (!strcmp(name, "maxa")) src = RM.maxa;
     else if (!strcmp(name, "kht")) src = RM.kht;
+    else if (!strcmp(name, "jnt")) src = RM.jnt;
     if (!src || !RM.open) return 1;
     memcpy(out, src, sizeof(long) * (size_t)n);
     return 0;
@@ -466,4 +471,41 @@ int ref_output_files(const char *dir, int nsteps, const double *lpf, const int *
     for (int i = 0; i < 7; ++i) fclose(OFP[i]);
     for (int k = 0; k < 8; ++k) OFP[k] = keep[k];
     return 0;
+}
+
+/* The reference's checkpoint writer and reader (misc.c:494-716), for the
+ * golden checkpoint files: checkPoint writes results8.txt into dir (it opens
+ * that name in the working directory); restartStep reads a checkpoint file
+ * after its first line, which main.c reads itself (main.c:3381-3386).  ANAFLAG 2
+ * only: the plasticity arrays chi, efN, efM are passed but not touched. */
+int ref_checkpoint(const char *dir, long tstep, long lss, double *uc, double *vc, double *ac,
+                   double *ss, double *sm, double *d, double *f, double *ef, double *x, double *c1,
+                   double *c2, double *c3, double *defllen, double *llength, double *efFE,
+                   double *xfr, int *yldflag, double *deffarea, double *defslen)
+{
+    char cwd[4096];
+    if (ALGFLAG != 5 || !getcwd(cwd, sizeof cwd) || chdir(dir) != 0) return 1;
+    double one = 0;
+    checkPoint(tstep, lss, uc, vc, ac, ss, sm, d, f, ef, x, c1, c2, c3, defllen, llength, efFE, xfr,
+               yldflag, deffarea, defslen, &one, &one, &one);
+    return chdir(cwd) != 0;
+}
+
+int ref_restart(const char *path, long *tstep, long lss, double *uc, double *vc, double *ac,
+                double *ss, double *sm, double *d, double *f, double *ef, double *x, double *c1,
+                double *c2, double *c3, double *defllen, double *llength, double *efFE,
+                double *xfr, int *yldflag, double *deffarea, double *defslen)
+{
+    if (ALGFLAG != 5) return 1;
+    FILE *keep = IFP[3];
+    IFP[3] = fopen(path, "r");
+    if (!IFP[3]) { IFP[3] = keep; return 1; }
+    int rc = fscanf(IFP[3], "%ld\n", tstep) == 1 ? 0 : 2;
+    double one = 0;
+    if (rc == 0)
+        restartStep(lss, uc, vc, ac, ss, sm, d, f, ef, x, c1, c2, c3, defllen, llength, efFE, xfr,
+                    yldflag, deffarea, defslen, &one, &one, &one);
+    fclose(IFP[3]);
+    IFP[3] = keep;
+    return rc;
 }
