@@ -15,7 +15,7 @@ OUT = os.path.join(HERE, "libcubens_amd.so")
 OBJ = os.path.join(HERE, "_build")
 SOURCES = ["model.cpp", "kernels.hip", "frame.hip", "newmark.hip", "band.hip", "solver.hip", "driver.cpp",
            "capi.cpp", "learned.hip", "batch.hip", "batch_driver.cpp", "results.hip",
-           "noalloc.hip"]
+           "noalloc.hip", "checkpoint.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

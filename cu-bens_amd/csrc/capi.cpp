@@ -88,6 +88,8 @@ cb_options default_options() {
     o.kernel_timing_every = 1;
     o.sweep_schedule = CB_SWEEP_CHAIN;
     o.keep_forces = 0;
+    o.checkpoint_path = nullptr;
+    o.checkpoint_exact = 0;
     return o;
 }
 
@@ -105,6 +107,8 @@ cb::RunOptions to_run(const cb_options& o) {
     r.kernel_timing_every = o.kernel_timing_every;
     r.sweep_schedule = o.sweep_schedule;
     r.keep_forces = o.keep_forces;
+    r.checkpoint_path = o.checkpoint_path ? o.checkpoint_path : "";
+    r.checkpoint_exact = o.checkpoint_exact;
     return r;
 }
 
@@ -113,7 +117,9 @@ bool same_opts(const cb_options& a, const cb_options& b) {
            a.maxit == b.maxit && a.judge == b.judge && a.max_steps == b.max_steps && a.keep_d == b.keep_d &&
            a.kernel_timing == b.kernel_timing && a.factor_schedule == b.factor_schedule &&
            a.kernel_timing_every == b.kernel_timing_every && a.sweep_schedule == b.sweep_schedule &&
-           a.keep_forces == b.keep_forces;
+           a.keep_forces == b.keep_forces && a.checkpoint_exact == b.checkpoint_exact &&
+           std::string(a.checkpoint_path ? a.checkpoint_path : "") ==
+               std::string(b.checkpoint_path ? b.checkpoint_path : "");
 }
 
 double normal01(uint64_t& st) {

@@ -5,6 +5,9 @@
 #include "driver.h"
 
 #include <algorithm>
+#include <functional>
+
+#include "checkpoint.h"
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -661,13 +664,27 @@ int Analysis::run_newmark() {
     double ddt = 1, sub_dt = 1, dt_temp = dtime;
     int tsflag = 0, convchk = 1, itecnt_ = 0, det = 0;
     double intener1 = 0;
+    long k = 0;
+    const bool ckpt = !o_.checkpoint_path.empty();
+    // restart (main.c:3378-3398): the state of the checkpoint file, then the
+    // time steps after it; a checkpoint of the last step gives that step's
+    // row alone (main.c:3387-3392) rather than a step past the load history
+    if (m_.rflag == 1) {
+        if (!ckpt) FAIL(1, "restart (RFLAG 1) without a checkpoint file");
+        long tstep = 0;
+        if (load_checkpoint(&tstep, &ddt)) FAIL(1, "unable to read the checkpoint file");
+        k = tstep + 1;
+        if (k >= ntst) {
+            if (k == ntst) record(double(tstep) * dtime, 0, dv);
+            return 0;
+        }
+    }
     copy_gen(d_, G_I, G_BASE, true, true);
     copy_gen(d_, G_IP, G_BASE, true, true);
     NewmarkArgs A;
     A.P = hist; A.ld = m_.ntstpsinpt; A.am = am_; A.af = af_;
     A.qtot = qtot; A.r = r; A.ft = ft; A.fip = fip; A.dt = dt; A.dd = dd; A.sm = sm;
     A.ac = aci; A.vc = vci; A.um = um; A.vm = vm; A.amv = amv;
-    long k = 0;
     do {
         dt_temp = ddt * dtime;
         sub_dt = ddt;
@@ -741,6 +758,7 @@ int Analysis::run_newmark() {
                     lam += dlam;
                 }
             } while (lam <= m_.lpfmax);
+            bool committed = false;
             if (convchk != 0) {
                 ddt = ddt / 2;                          // main.c:3826-3846
                 dt_temp = ddt * dtime;
@@ -752,6 +770,7 @@ int Analysis::run_newmark() {
                 vcopy(d_, f, ft, n);
                 vcopy(d_, uc, dt, n); vcopy(d_, vc, vci, n); vcopy(d_, ac, aci, n);
                 commit();
+                committed = true;
             }
             if (ddt < 1) {
                 if (sub_dt <= 1) { sub_dt = sub_dt + ddt; tsflag = 1; }
@@ -759,12 +778,113 @@ int Analysis::run_newmark() {
             } else if (ddt == 1) {
                 tsflag = 2;
             }
+            // checkPoint at the commit (main.c:3904-3907), written after the
+            // multiplier update, which changes no state, so that the file also
+            // holds the multiplier the next step starts from
+            if (committed && ckpt && k % m_.chkpt == 0 && k != 0 &&
+                !save_checkpoint(k, ddt, A.a0 * (1 - am_), 1 - af_))
+                FAIL(1, "unable to write the checkpoint file");
         } while (ddt >= 0.0001 && tsflag != 2);
         if (convchk != 0) FAIL(3, "minimum time increment interval reached without convergence");
         record(double(k) * dtime, itecnt_, dv);
         if (long(lpf.size()) >= o_.max_steps) return 0;
         ++k;
     } while (k < ntst);
+    return 0;
+}
+
+bool Analysis::save_checkpoint(long k, double ddt, double cnum, double cden) {
+    Checkpoint c;
+    c.size_for(m_);
+    c.tstep = k;
+    c.ddt = ddt;
+    const size_t n = size_t(m_.neq), bs = size_t(d_.bs);
+    std::vector<double> smv(n), K(size_t(d_.nnzb) * bs * bs);
+    d2h(c.uc.data(), uc, n, d_.s);
+    d2h(c.vc.data(), vc, n, d_.s);
+    d2h(c.ac.data(), ac, n, d_.s);
+    d2h(smv.data(), sm, n, d_.s);
+    d2h(c.d.data(), dv, n, d_.s);
+    d2h(c.f.data(), f, n, d_.s);
+    d2h(c.ef.data(), d_.ef[G_BASE], c.ef.size(), d_.s);
+    d2h(c.x.data(), d_.x[G_BASE], c.x.size(), d_.s);
+    d2h(c.c1.data(), d_.c1[G_BASE], c.c1.size(), d_.s);
+    d2h(c.c2.data(), d_.c2[G_BASE], c.c2.size(), d_.s);
+    d2h(c.c3.data(), d_.c3[G_BASE], c.c3.size(), d_.s);
+    d2h(c.defllen.data(), d_.defllen[G_BASE], c.defllen.size(), d_.s);
+    d2h(c.deffarea.data(), d_.deffarea[G_BASE], c.deffarea.size(), d_.s);
+    d2h(c.defslen.data(), d_.defslen[G_BASE], c.defslen.size(), d_.s);
+    d2h(c.llen_w.data(), d_.llen, c.llen_w.size(), d_.s);
+    d2h(c.farea_w.data(), d_.farea, c.farea_w.size(), d_.s);
+    d2h(c.slen_w.data(), d_.slength, c.slen_w.size(), d_.s);
+    d2h(K.data(), d_.K, K.size(), d_.s);
+    CB_CHECK(hipStreamSynchronize(d_.s));
+    c.llength = c.llen_w;      // the frame lengths at the commit (main.c:3880)
+    std::copy(smv.begin(), smv.end(), c.sm.begin());
+    // member end coordinates: the joints' (the device takes no end offsets)
+    for (long e = 0; e < m_.nfr; ++e)
+        for (int end = 0; end < 2; ++end) {
+            const long j = m_.minc[size_t(m_.ntr * 2 + e * 2 + end)] - 1;
+            for (int a = 0; a < 3; ++a) c.xfr[size_t(e * 6 + end * 3 + a)] = c.x[size_t(j * 3 + a)];
+        }
+    // skyline ss from the joint-block tangent: column j holds rows j - kht[j]..j,
+    // the diagonal at maxa[j] - 1 and row i at maxa[j] - 1 + (j - i)
+    std::vector<long> slot(n, -1);
+    for (size_t s = 0; s < m_.eq_of_slot.size(); ++s)
+        if (m_.eq_of_slot[s] >= 0) slot[size_t(m_.eq_of_slot[s])] = long(s);
+    auto entry = [&](long si, long sj) {
+        if (si < 0 || sj < 0) return 0.0;
+        const long a = si / long(bs), b = sj / long(bs);
+        const int32_t* lo = m_.colidx.data() + m_.rowptr[size_t(a)];
+        const int32_t* hi = m_.colidx.data() + m_.rowptr[size_t(a) + 1];
+        const int32_t* it = std::lower_bound(lo, hi, int32_t(b));
+        if (it == hi || *it != b) return 0.0;
+        const size_t blk = size_t(it - m_.colidx.data());
+        return K[blk * bs * bs + size_t(si % long(bs)) * bs + size_t(sj % long(bs))];
+    };
+    for (long j = 0; j < long(n); ++j) {
+        for (long h = 0; h <= m_.kht[size_t(j)]; ++h) {
+            double v = entry(slot[size_t(j - h)], slot[size_t(j)]);
+            if (h == 0) v -= cnum * smv[size_t(j)] / cden;
+            c.ss[size_t(m_.maxa[size_t(j)] - 1 + h)] = v;
+        }
+    }
+    return write_checkpoint(o_.checkpoint_path, m_, c, o_.checkpoint_exact != 0);
+}
+
+int Analysis::load_checkpoint(long* tstep, double* ddt) {
+    Checkpoint c;
+    const int rc = read_checkpoint(o_.checkpoint_path, m_, c);
+    if (rc) return rc;
+    const size_t n = size_t(m_.neq);
+    h2d(uc, c.uc.data(), n, d_.s);
+    h2d(vc, c.vc.data(), n, d_.s);
+    h2d(ac, c.ac.data(), n, d_.s);
+    h2d(sm, c.sm.data(), n, d_.s);
+    h2d(dv, c.d.data(), n, d_.s);
+    h2d(f, c.f.data(), n, d_.s);
+    h2d(d_.ef[G_BASE], c.ef.data(), c.ef.size(), d_.s);
+    h2d(d_.x[G_BASE], c.x.data(), c.x.size(), d_.s);
+    h2d(d_.c1[G_BASE], c.c1.data(), c.c1.size(), d_.s);
+    h2d(d_.c2[G_BASE], c.c2.data(), c.c2.size(), d_.s);
+    h2d(d_.c3[G_BASE], c.c3.data(), c.c3.size(), d_.s);
+    h2d(d_.defllen[G_BASE], c.defllen.data(), c.defllen.size(), d_.s);
+    h2d(d_.deffarea[G_BASE], c.deffarea.data(), c.deffarea.size(), d_.s);
+    h2d(d_.defslen[G_BASE], c.defslen.data(), c.defslen.size(), d_.s);
+    // working lengths: the frames' from the file's llength, as a fresh start
+    // takes them (main.c:3420; the reference's restart leaves them unset), the
+    // rest as the model has them, unless the full-precision extension holds all
+    std::vector<double> llen(m_.llength);
+    for (long e = 0; e < m_.nfr; ++e) llen[size_t(m_.ntr + e)] = c.llength[size_t(m_.ntr + e)];
+    if (c.has_ext) {
+        llen = c.llen_w;
+        h2d(d_.farea, c.farea_w.data(), c.farea_w.size(), d_.s);
+        h2d(d_.slength, c.slen_w.data(), c.slen_w.size(), d_.s);
+    }
+    h2d(d_.llen, llen.data(), llen.size(), d_.s);
+    CB_CHECK(hipStreamSynchronize(d_.s));
+    *tstep = c.tstep;
+    *ddt = c.has_ext ? c.ddt : 1.0;
     return 0;
 }
 

@@ -44,6 +44,11 @@ struct RunOptions {
     int kernel_timing_every = 1;  // time every n-th launch of a timed family
     int sweep_schedule = 0;       // 0 sweeps with the chain matrices, 1 the two-product row step
     int keep_forces = 0;          // record the averaged element forces of every converged step
+    // Newmark checkpoint file (checkPoint / restartStep): written every CHKPT
+    // time steps when set, read first when the model has RFLAG 1; exact 1
+    // writes full precision and the working lengths, areas and multiplier
+    std::string checkpoint_path;
+    int checkpoint_exact = 0;
 };
 
 struct RunStats {
@@ -126,6 +131,11 @@ private:
     double *vci = nullptr, *aci = nullptr, *um = nullptr, *vm = nullptr, *amv = nullptr;
     double *dyn = nullptr, *hist = nullptr;
     void record(double lam, int ite, const double* dvec);
+    // the Newmark checkpoint of the committed state at time step k, with the
+    // mass term cnum * sm / cden taken off the diagonal of d.K (Keff of the
+    // last iteration) to give the tangent the file holds
+    bool save_checkpoint(long k, double ddt, double cnum, double cden);
+    int load_checkpoint(long* tstep, double* ddt);   // 0 ok, 1 cannot open, 2 malformed
     void update_and_forces(int gx, double* fout, const double* ddv);
     int factor_current(int* det);
     double psi_now() const;

@@ -33,7 +33,8 @@ class Options(C.Structure):
     _fields_ = [("solver", C.c_int), ("krylov_tol", C.c_double), ("restart", C.c_int),
                 ("maxit", C.c_int), ("judge", C.c_int), ("max_steps", C.c_int),
                 ("keep_d", C.c_int), ("kernel_timing", C.c_int), ("factor_schedule", C.c_int),
-                ("kernel_timing_every", C.c_int), ("sweep_schedule", C.c_int), ("keep_forces", C.c_int)]
+                ("kernel_timing_every", C.c_int), ("sweep_schedule", C.c_int), ("keep_forces", C.c_int),
+                ("checkpoint_path", C.c_char_p), ("checkpoint_exact", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -62,9 +63,13 @@ SWEEP_CHAIN, SWEEP_ROWS = 0, 1
 def options(solver: int = SOLVER_BAND64, krylov_tol: float = 1e-8, restart: int = 30,
             maxit: int = 20000, judge: int = 1, max_steps: int = 1 << 30, keep_d: int = 1,
             kernel_timing: int = 0, factor_schedule: int = 0, kernel_timing_every: int = 1,
-            sweep_schedule: int = 0, keep_forces: int = 0) -> Options:
+            sweep_schedule: int = 0, keep_forces: int = 0, checkpoint_path: str | None = None,
+            checkpoint_exact: int = 0) -> Options:
+    """cb_options; checkpoint_path is the Newmark checkpoint file (written every
+    CHKPT time steps, read first when the model has RFLAG 1)."""
     return Options(solver, krylov_tol, restart, maxit, judge, max_steps, keep_d, kernel_timing,
-                   factor_schedule, kernel_timing_every, sweep_schedule, keep_forces)
+                   factor_schedule, kernel_timing_every, sweep_schedule, keep_forces,
+                   checkpoint_path.encode() if checkpoint_path else None, checkpoint_exact)
 
 
 _lib = None

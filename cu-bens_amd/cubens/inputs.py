@@ -367,7 +367,8 @@ def space_frame(n_elem: int = 400, kind: str = "tower", seed: int | None = None,
                 instance: int = 0, steps: int = 200, dt: float = 1e-3, amp: float | None = None,
                 tol=(1e-8, 1e-8, 1e-8), algflag: int = 5, numopt: int = 0,
                 spectrds: float = 1.0, itemax: int = 20, slice_shape: tuple | None = None,
-                optflag: int = 1, scramble: int | None = None) -> ModelText:
+                optflag: int = 1, scramble: int | None = None, chkpt: int | None = None,
+                rflag: int = 0) -> ModelText:
     """A random tower or lattice girder of about ``n_elem`` frame members.
 
     tower : square storeys of 4 columns, 4 beams and 8 face X-braces; plan
@@ -388,6 +389,8 @@ def space_frame(n_elem: int = 400, kind: str = "tower", seed: int | None = None,
     (model.c:649-935).  The reference applies that renumbering to static loads
     but not to load histories (model.c:1524), so a scrambled Newmark model with
     optflag 2 loads other joints than the same model with optflag 1.
+    ``chkpt`` and ``rflag`` are the Newmark checkpoint interval and restart flag
+    (main.c:347-371); by default the interval lies beyond the last step.
     """
     rng = SplitMix64(workload_seed(4, instance) if seed is None else seed)
     joints, members, fixed, loaded = [], [], [], []
@@ -448,9 +451,9 @@ def space_frame(n_elem: int = 400, kind: str = "tower", seed: int | None = None,
     w = out.write
     w(f"2\n{algflag}\n")
     if algflag > 3:
-        # checkpoint interval beyond the last step: no checkpoint is written
-        # (CHKPT 0 would divide by zero in the reference's k % CHKPT)
-        w(f"{steps + 1},0\n")
+        # default checkpoint interval beyond the last step: no checkpoint is
+        # written (CHKPT 0 would divide by zero in the reference's k % CHKPT)
+        w(f"{steps + 1 if chkpt is None else chkpt},{rflag}\n")
     w(f"0\n{optflag}\n{nj}\n0,{ne},0,0,0\n")
     for a, b_ in members:
         w(f"{jo(a)},{jo(b_)}\n")

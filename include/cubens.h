@@ -55,6 +55,16 @@ typedef struct {
     int kernel_timing_every; /* time every n-th launch of a timed family (<= 1: all) */
     int sweep_schedule;  /* CB_SWEEP_* schedule of the band solves' sweeps    */
     int keep_forces;     /* keep the averaged element forces of every step  */
+    /* Newmark checkpoint file (checkPoint / restartStep, misc.c:494-716; the
+     * reference's results8.txt): when set, written after every CHKPT-th time
+     * step (main.c:3904-3907) and, for a model with RFLAG 1, read before the
+     * first step, which is then tstep + 1 (main.c:3378-3398).  NULL: none. */
+    const char *checkpoint_path;
+    /* 0: the reference's %e file; 1: the same layout at full precision
+     * (%.17g, which the reference's reader takes) plus, after its closing
+     * line, the working lengths and areas and the time-increment multiplier,
+     * so that a restart continues the run bit for bit */
+    int checkpoint_exact;
 } cb_options;
 
 /* Schedules of the band LDL^T factorization (same tile arithmetic, bitwise

@@ -227,6 +227,12 @@ typedef struct {
     double *ef;          /* optional [max_steps*(NE_TR*2+NE_FR*14+NE_SH*18)]: the
                             converged element forces of each step (what output()
                             averages into results3-5.txt, misc.c:419-449) */
+    /* Newmark checkpoint file (checkPoint / restartStep, misc.c:494-716): when
+       set, written every CHKPT time steps, and read at the start of a run whose
+       model has RFLAG 1; ckpt_exact 1 writes full precision plus the working
+       lengths, areas and time-increment multiplier (orc_checkpoint.c) */
+    const char *ckpt_path;
+    int ckpt_exact;
 } orc_path;
 
 /* Driver events of one converged step: the pivot-sign flag of the step's last
@@ -242,6 +248,31 @@ enum { ORC_EV_DET = 0, ORC_EV_IMAG = 1, ORC_EV_NEG = 2, ORC_EV_DNORM = 3, ORC_EV
  * 3 MSAL main.c:2158-3141, 5 nonlinear Newmark main.c:3305-3960) with the
  * element routines of backend b (NULL = oracle's own). */
 int orc_run(orc_model *m, const orc_backend *b, orc_path *out);
+
+/* ---- checkpoint file of the nonlinear Newmark driver (orc_checkpoint.c) ----
+ * The state checkPoint writes (misc.c:494-603) in the reference's layouts;
+ * ANAFLAG 2 (the shell section without the plasticity arrays). */
+typedef struct {
+    long tstep;
+    double *uc, *vc, *ac;            /* NEQ                                  */
+    double *ss, *sm;                 /* lss: the tangent K of the last iteration, the lumped mass */
+    double *d, *f;                   /* NEQ                                  */
+    double *ef;                      /* NE_TR*2 + NE_FR*14 + NE_SH*18        */
+    double *x;                       /* NJ*3                                 */
+    double *c1, *c2, *c3;            /* NE_TR + NE_FR*3 + NE_SH*3            */
+    double *defllen, *llength;       /* NE_TR + NE_FR (llength: frame part)  */
+    double *efFE, *xfr;              /* NE_FR*14, NE_FR*6                    */
+    int *yldflag;                    /* NE_FR*2                              */
+    double *deffarea, *defslen;      /* NE_SH, NE_SH*3                       */
+    /* the full-precision extension: present or not (read), written or not */
+    int has_ext;
+    double ddt;                      /* time-increment multiplier of the next step */
+    double *llen_w;                  /* NE_TR + NE_FR working lengths        */
+    double *farea_w, *slen_w;        /* NE_SH, NE_SH*3 working areas, side lengths */
+} orc_ckpt;
+int orc_ckpt_write(const char *path, const orc_dims *d, long lss, const orc_ckpt *c, int exact);
+/* 0 ok, 1 cannot open, 2 malformed */
+int orc_ckpt_read(const char *path, const orc_dims *d, long lss, orc_ckpt *c);
 
 #ifdef __cplusplus
 }

@@ -54,7 +54,23 @@ class Path(C.Structure):
                 ("t_tangent", C.c_double), ("t_factor", C.c_double), ("t_solve", C.c_double),
                 ("t_forces", C.c_double), ("t_other", C.c_double),
                 ("n_factor", C.c_long), ("n_solve", C.c_long), ("n_newton", C.c_long),
-                ("events", PI), ("psi", PD), ("react", PD), ("resid", PD), ("ef", PD)]
+                ("events", PI), ("psi", PD), ("react", PD), ("resid", PD), ("ef", PD),
+                ("ckpt_path", C.c_char_p), ("ckpt_exact", C.c_int)]
+
+
+class Ckpt(C.Structure):
+    """orc_ckpt: the state of a Newmark checkpoint file (misc.c:494-716)."""
+    _fields_ = [("tstep", C.c_long), ("uc", PD), ("vc", PD), ("ac", PD), ("ss", PD), ("sm", PD),
+                ("d", PD), ("f", PD), ("ef", PD), ("x", PD), ("c1", PD), ("c2", PD), ("c3", PD),
+                ("defllen", PD), ("llength", PD), ("efFE", PD), ("xfr", PD), ("yldflag", PI),
+                ("deffarea", PD), ("defslen", PD), ("has_ext", C.c_int), ("ddt", C.c_double),
+                ("llen_w", PD), ("farea_w", PD), ("slen_w", PD)]
+
+# Arrays of a checkpoint file in file order, with their lengths for a model
+# (NEQ, lss, NJ, element counts); the extension arrays last.
+CKPT_ARRAYS = ("uc", "vc", "ac", "ss", "sm", "d", "f", "ef", "x", "c1", "c2", "c3", "defllen",
+               "llength", "efFE", "xfr", "yldflag", "deffarea", "defslen", "llen_w", "farea_w",
+               "slen_w")
 
 # Slots of the per-step driver events (ORC_EV_* in cb_oracle.h).
 EV_DET, EV_IMAG, EV_NEG, EV_DNORM, EV_HALVE = range(5)
@@ -93,6 +109,10 @@ def lib():
         _lib.orc_free.argtypes = [C.POINTER(Model)]
         _lib.orc_run.argtypes = [C.POINTER(Model), C.POINTER(Backend), C.POINTER(Path)]
         _lib.orc_run.restype = C.c_int
+        _lib.orc_ckpt_read.argtypes = [C.c_char_p, C.POINTER(Dims), C.c_long, C.POINTER(Ckpt)]
+        _lib.orc_ckpt_read.restype = C.c_int
+        _lib.orc_ckpt_write.argtypes = [C.c_char_p, C.POINTER(Dims), C.c_long, C.POINTER(Ckpt), C.c_int]
+        _lib.orc_ckpt_write.restype = C.c_int
         _lib.orc_skyfact.argtypes = [C.c_long, PL, PD, PD, C.c_int, PI]
         _lib.orc_skyfact.restype = C.c_int
         _lib.orc_skysolve.argtypes = [C.c_long, PL, PD, PD]
@@ -129,6 +149,10 @@ def ref():
                                   PD, PD, PD, PD, PD]
         _ref.ref_quad.restype = C.c_int
         _ref.ref_output_files.argtypes = [C.c_char_p, C.c_int, PD, PI, PD, PD]
+        _ref.ref_checkpoint.argtypes = [C.c_char_p, C.c_long, C.c_long] + [PD] * 16 + [PI, PD, PD]
+        _ref.ref_checkpoint.restype = C.c_int
+        _ref.ref_restart.argtypes = [C.c_char_p, PL, C.c_long] + [PD] * 16 + [PI, PD, PD]
+        _ref.ref_restart.restype = C.c_int
         _ref.ref_output_files.restype = C.c_int
         _ref.ref_test.argtypes = [C.POINTER(Dims), PD, PD, PD, PD, PD, PD, C.c_double, PI,
                                   C.c_double, C.c_double, C.c_double]
@@ -208,11 +232,16 @@ class OracleModel:
         return r.reshape(self.nj, 7)
 
     def run(self, max_steps: int, backend: str = "oracle", keep_f: bool = False,
-            keep_react: bool = False, keep_ef: bool = False) -> dict:
+            keep_react: bool = False, keep_ef: bool = False, ckpt_path: str | None = None,
+            ckpt_exact: bool = False) -> dict:
         """Runs the analysis; returns the converged path and timings (and, with
-        keep_react, the reactions [nsteps, NJ, 7] of every converged step)."""
+        keep_react, the reactions [nsteps, NJ, 7] of every converged step).
+        ckpt_path: the Newmark checkpoint file, written every CHKPT steps and
+        read first when the model has RFLAG 1."""
         P = Path()
         P.max_steps = max_steps
+        P.ckpt_path = ckpt_path.encode() if ckpt_path else None
+        P.ckpt_exact = int(bool(ckpt_exact))
         lpf = np.zeros(max_steps)
         ite = np.zeros(max_steps, np.int32)
         dmat = np.zeros((max_steps, max(self.neq, 1)))
@@ -251,6 +280,49 @@ class OracleModel:
         if keep_ef:
             out["ef"] = efp[:n].copy()
         return out
+
+
+def ckpt_sizes(om) -> dict:
+    """Length of each checkpoint array of model om (orc_ckpt layouts)."""
+    ntri = om.ntr + 3 * om.nfr + 3 * om.nsh
+    return {"uc": om.neq, "vc": om.neq, "ac": om.neq, "ss": om.lss, "sm": om.lss, "d": om.neq,
+            "f": om.neq, "ef": om.ntr * 2 + om.nfr * 14 + om.nsh * 18, "x": om.nj * 3, "c1": ntri,
+            "c2": ntri, "c3": ntri, "defllen": om.ntr + om.nfr, "llength": om.ntr + om.nfr,
+            "efFE": om.nfr * 14, "xfr": om.nfr * 6, "yldflag": om.nfr * 2, "deffarea": om.nsh,
+            "defslen": om.nsh * 3, "llen_w": om.ntr + om.nfr, "farea_w": om.nsh,
+            "slen_w": om.nsh * 3}
+
+
+def _ckpt_struct(om):
+    arrs = {k: np.zeros(max(n, 1), np.int32 if k == "yldflag" else np.float64)
+            for k, n in ckpt_sizes(om).items()}
+    c = Ckpt()
+    for k, a in arrs.items():
+        setattr(c, k, ptr(a, PI) if k == "yldflag" else ptr(a))
+    return c, arrs
+
+
+def read_checkpoint(path: str, om) -> dict:
+    """The oracle's reading of a checkpoint file (orc_ckpt_read) for model om:
+    tstep, every array at its length, and the extension (has_ext, ddt)."""
+    c, arrs = _ckpt_struct(om)
+    rc = lib().orc_ckpt_read(path.encode(), C.byref(om.h.contents.dm), om.lss, C.byref(c))
+    if rc:
+        raise ValueError(f"checkpoint file not readable ({rc})")
+    out = {k: arrs[k][:n].copy() for k, n in ckpt_sizes(om).items()}
+    out.update(tstep=c.tstep, has_ext=c.has_ext, ddt=c.ddt)
+    return out
+
+
+def write_checkpoint(path: str, om, state: dict, exact: bool = False):
+    """orc_ckpt_write of a state dict shaped as read_checkpoint returns it."""
+    c, arrs = _ckpt_struct(om)
+    for k, n in ckpt_sizes(om).items():
+        arrs[k][:n] = state[k][:n]
+    c.tstep = int(state["tstep"])
+    c.ddt = float(state.get("ddt", 1.0))
+    if lib().orc_ckpt_write(path.encode(), C.byref(om.h.contents.dm), om.lss, C.byref(c), int(exact)):
+        raise OSError(f"cannot write {path}")
 
 
 class RefModel:

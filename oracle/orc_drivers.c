@@ -695,7 +695,65 @@ static int run_msal(run_t *R)
  * Without prescribed support motion (NBC = 0; the input reader rejects it) and
  * with the skyline solver.  The reference discards the initial conditions
  * read from the input here (uc = vc = ac = 0, main.c:3443-3445), and so does
- * this restatement.  One path row per time step: (time, itecnt, d). */
+ * this restatement.  One path row per time step: (time, itecnt, d).
+ *
+ * Checkpoints (out->ckpt_path): after a committed time step k with
+ * k % CHKPT == 0 and k != 0 the state goes to the file (main.c:3904-3907); a
+ * model with RFLAG 1 starts from the file instead, at step tstep + 1
+ * (main.c:3378-3398).  The reference leaves the frame lengths the elements
+ * read (llength_temp, malloc'd at main.c:653) unset on that path, since only
+ * a fresh start copies them (main.c:3420); here they come from the file's
+ * llength as a fresh start would take them, and the exact file's extension
+ * restores every working length and area and the time-increment multiplier. */
+static int ckpt_view(run_t *R, orc_ckpt *c, double *sm_l, double *ss_l, int *yld)
+{
+    memset(c, 0, sizeof *c);
+    c->uc = R->uc; c->vc = R->vc; c->ac = R->ac;
+    c->ss = ss_l; c->sm = sm_l;
+    c->d = R->d; c->f = R->f; c->ef = R->ef; c->x = R->x;
+    c->c1 = R->c1; c->c2 = R->c2; c->c3 = R->c3;
+    c->defllen = R->defllen; c->llength = R->llen_ref;
+    c->efFE = R->efFE; c->xfr = R->xfr; c->yldflag = yld;
+    c->deffarea = R->deffarea; c->defslen = R->defslen;
+    c->llen_w = R->llen_ref; c->farea_w = R->farea_w; c->slen_w = R->slen_w;
+    return 0;
+}
+
+static int ckpt_write(run_t *R, long k, double ddt)
+{
+    orc_model *m = R->m;
+    long lss = m->lss;
+    double *sm_l = dz(lss);
+    int *yld = calloc((size_t)(R->nfr * 2 + 1), sizeof(int));
+    orc_ckpt c;
+    ckpt_view(R, &c, sm_l, R->ss, yld);
+    CP(sm_l, R->sm, R->neq);
+    c.tstep = k;
+    c.ddt = ddt;
+    int rc = orc_ckpt_write(R->out->ckpt_path, &m->dm, lss, &c, R->out->ckpt_exact);
+    free(sm_l);
+    free(yld);
+    return rc;
+}
+
+static int ckpt_restart(run_t *R, long *tstep, double *ddt)
+{
+    orc_model *m = R->m;
+    long lss = m->lss;
+    double *sm_l = dz(lss), *ss_l = dz(lss);
+    int *yld = calloc((size_t)(R->nfr * 2 + 1), sizeof(int));
+    orc_ckpt c;
+    ckpt_view(R, &c, sm_l, ss_l, yld);
+    int rc = orc_ckpt_read(R->out->ckpt_path, &m->dm, lss, &c);
+    if (rc == 0) {
+        CP(R->sm, sm_l, R->neq);
+        *tstep = c.tstep;
+        *ddt = c.has_ext ? c.ddt : 1;
+    }
+    free(sm_l); free(ss_l); free(yld);
+    return rc;
+}
+
 static int run_newmark(run_t *R)
 {
     orc_model *m = R->m;
@@ -727,6 +785,17 @@ static int run_newmark(run_t *R)
        read are the run's working copies, overwritten by the mass routines */
     double *llen = R->llen_ref;
     long k = 0;
+    const int ckpt = R->out && R->out->ckpt_path;
+    if (m->rflag == 1) {
+        if (!ckpt) FAIL(1, "restart (RFLAG 1) without a checkpoint file");
+        long tstep = 0;
+        if (ckpt_restart(R, &tstep, &ddt)) FAIL(1, "unable to read the checkpoint file");
+        k = tstep + 1;
+        if (k >= ntst) {                  /* the checkpoint is the last step (main.c:3387-3392) */
+            if (k == ntst) record(R, (double)tstep * dt, 0, R->d, R->f);
+            return 0;
+        }
+    }
     do {
         dt_temp = ddt * dt;
         sub_dt = ddt;
@@ -887,6 +956,7 @@ static int run_newmark(run_t *R)
                     lpf += dlpf;
                 }
             } while (lpf <= m->lpfmax);
+            int committed = 0;
             if (convchk != 0) {
                 /* halve the time increment (main.c:3826-3846) */
                 ddt = ddt / 2;
@@ -899,6 +969,7 @@ static int run_newmark(run_t *R)
                 CP(R->f, R->f_temp, n);
                 CP(R->uc, R->uc_i, n); CP(R->vc, R->vc_i, n); CP(R->ac, R->ac_i, n);
                 commit(R);
+                committed = 1;
             }
             if (!frcchk) {
                 if (ddt < 1) {
@@ -908,6 +979,11 @@ static int run_newmark(run_t *R)
                     tsflag = 2;
                 }
             }
+            /* written after the multiplier update, which changes no array, so the
+               file holds what checkPoint writes at the commit (main.c:3904-3907)
+               and the multiplier the next step starts from */
+            if (committed && ckpt && k % m->chkpt == 0 && k != 0 && ckpt_write(R, k, ddt))
+                FAIL(1, "unable to write the checkpoint file");
         } while (ddt >= 0.0001 && tsflag != 2);
         if (convchk != 0 || frcchk)
             FAIL(3, "minimum time increment interval reached without convergence");
