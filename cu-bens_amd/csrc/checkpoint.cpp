@@ -149,4 +149,33 @@ int read_checkpoint(const std::string& path, const Model& m, Checkpoint& c) {
     return r.bad ? 2 : 0;
 }
 
+bool write_checkpoint_linear(const std::string& path, long k, const std::vector<double>& u,
+                             const std::vector<double>& v, const std::vector<double>& a, bool exact) {
+    std::unique_ptr<FILE, FileCloser> fp(std::fopen(path.c_str(), "w"));
+    if (!fp) return false;
+    FILE* f = fp.get();
+    std::fprintf(f, "%ld\n", k);
+    for (size_t i = 0; i < u.size(); ++i) line(f, exact, {u[i], v[i], a[i]});
+    std::fprintf(f, "0,0,0\n");
+    return std::fflush(f) == 0 && !std::ferror(f);
+}
+
+int read_checkpoint_linear(const std::string& path, long neq, long* k, std::vector<double>& u,
+                           std::vector<double>& v, std::vector<double>& a) {
+    std::unique_ptr<FILE, FileCloser> fp(std::fopen(path.c_str(), "rb"));
+    if (!fp) return 1;
+    std::string text;
+    char buf[1 << 16];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof buf, fp.get())) > 0) text.append(buf, got);
+    Reader r{text.c_str()};
+    u.assign(size_t(neq), 0.0);
+    v.assign(size_t(neq), 0.0);
+    a.assign(size_t(neq), 0.0);
+    *k = long(r.num());
+    for (long i = 0; i < neq; ++i) { u[size_t(i)] = r.num(); v[size_t(i)] = r.num(); a[size_t(i)] = r.num(); }
+    r.zeros(3);
+    return r.bad ? 2 : 0;
+}
+
 }  // namespace cb

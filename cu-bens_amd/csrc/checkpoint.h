@@ -37,4 +37,11 @@ bool write_checkpoint(const std::string& path, const Model& m, const Checkpoint&
 // 0 read, 1 cannot open, 2 malformed.  c is sized for m.
 int read_checkpoint(const std::string& path, const Model& m, Checkpoint& c);
 
+// The linear Newmark driver's checkpoint (solve.c:445-457, read at 257-282):
+// the time step, NEQ lines "u,v,a", "0,0,0".
+bool write_checkpoint_linear(const std::string& path, long k, const std::vector<double>& u,
+                             const std::vector<double>& v, const std::vector<double>& a, bool exact);
+int read_checkpoint_linear(const std::string& path, long neq, long* k, std::vector<double>& u,
+                           std::vector<double>& v, std::vector<double>& a);
+
 }  // namespace cb

@@ -143,6 +143,22 @@ struct NewmarkArgs {
 };
 void newmark_vec(Dev& d, NewmarkVec op, const NewmarkArgs& a);
 
+// linear Newmark time step (solve.c:328-423):
+//   LNM_REFF:   r = P_k [+ af/(1-af) P_{k-1}, k > 0]
+//                   + sm ((1-am)(u a0 + v a2 + a a3) - am a)/(1-af) [- af/(1-af) Ku, Ku set]
+//   LNM_UPDATE: a_new = a0 (u_new - u) - a2 v - a3 a,  v_new = v + a6 a + a7 a_new
+enum LinearNewmarkVec { LNM_REFF = 0, LNM_UPDATE = 1 };
+struct LinearNewmarkArgs {
+    const double* P = nullptr;   // load histories [NEQ][ld]
+    long ld = 0, k = 0;
+    double am = 0, af = 0, a0 = 0, a2 = 0, a3 = 0, a6 = 0, a7 = 0;
+    const double *sm = nullptr, *u = nullptr, *v = nullptr, *a = nullptr, *Ku = nullptr;
+    double* r = nullptr;
+    const double* unew = nullptr;
+    double *vnew = nullptr, *anew = nullptr;
+};
+void linear_newmark_vec(Dev& d, LinearNewmarkVec op, const LinearNewmarkArgs& a);
+
 // ---- vectors -----------------------------------------------------------------
 // Fused dot products, deterministic two-level reduction: at most DOTS_MAX
 // products per pass, partial sums of at most DOTS_BLOCKS workgroups each.

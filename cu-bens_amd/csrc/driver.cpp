@@ -61,7 +61,7 @@ Analysis::Analysis(Model& m, Dev& d, const RunOptions& o) : m_(m), d_(d), o_(o) 
 
 Analysis::~Analysis() {
     double* vs[] = {q, dv, dp, dpp, dt, dd, ddq, ddr, f, fp, ft, fip, r, tmp, eb, ex, favg,
-                    sm, qtot, uc, vc, ac, vci, aci, um, vm, amv, dyn, hist};
+                    sm, qtot, uc, vc, ac, vci, aci, um, vm, amv, dyn, hist, kcopy};
     for (double* p : vs)
         if (p) (void)hipFree(p);
     b64_.release();
@@ -276,8 +276,10 @@ int Analysis::run(RunStats& st) {
     t_start_ = t0;
     int rc = 0;
     try {
-        if (m_.algflag == 1 || m_.algflag == 2) rc = run_nr();
+        if ((m_.algflag == 1 || m_.algflag == 2) && m_.anaflag == 1) rc = run_linear_static();
+        else if (m_.algflag == 1 || m_.algflag == 2) rc = run_nr();
         else if (m_.algflag == 3) rc = run_msal();
+        else if (m_.algflag == 4) rc = run_linear_newmark();
         else if (m_.algflag == 5) rc = run_newmark();
         else { rc = 2; st.msg = "driver not available for this ALGFLAG"; }
     } catch (const DeviceError& e) {
@@ -885,6 +887,136 @@ int Analysis::load_checkpoint(long* tstep, double* ddt) {
     CB_CHECK(hipStreamSynchronize(d_.s));
     *tstep = c.tstep;
     *ddt = c.has_ext ? c.ddt : 1.0;
+    return 0;
+}
+
+// Linear static analysis, ANAFLAG 1 under ALGFLAG 1/2 (main.c:1714-1798): the
+// tangent at the input geometry (no geometric terms), K d = lpfmax q, the
+// linear element forces of d and one path row (lpfmax, 0, d).  The reference
+// prints its iteration counter there without setting it (main.c:1670); 0 here.
+int Analysis::run_linear_static() {
+    const long n = m_.neq;
+    double* qt = tmp;
+    vcopy(d_, qt, q, n);
+    vaxpby(d_, qt, 0.0, q, m_.lpfmax, n);
+    {
+        Timer t(&st_->t_tangent, d_.s);
+        launch_tangent(d_, G_BASE, G_BASE, false);
+    }
+    int det = 0;
+    factor_current(&det);                 // solve() ignores skyfact's status (solve.c:78-89)
+    linear_solve(qt, dv);
+    CB_CHECK(hipMemsetAsync(f, 0, sizeof(double) * size_t(n), d_.s));
+    {
+        Timer t(&st_->t_forces, d_.s);
+        launch_forces(d_, G_BASE, f, dv);
+    }
+    commit();                             // the element forces become the state's
+    st_->newton++;
+    record(m_.lpfmax, 0, dv);
+    return 0;
+}
+
+// Linear Newmark, ALGFLAG 4 (main.c:3143-3303, solve.c:139-245, 246-468): K
+// and the lumped M at the input geometry, Keff = K + a0 (1-am)/(1-af) M on the
+// diagonal factored once, then one solve per time step from the input's
+// initial conditions; row (k dt, 0, u) per step.  After the loop the reference
+// computes the forces of d, which it never sets on this path, and writes
+// (lpfmax, counter, 0): that row ends the path here too.  The checkpoint file
+// (CHKPT) holds u, v, a; a model with RFLAG 1 starts from it.
+int Analysis::run_linear_newmark() {
+    const long n = m_.neq;
+    if (!sm) {
+        double** vs[] = {&sm, &qtot, &uc, &vc, &ac, &vci, &aci, &um, &vm, &amv, &dyn};
+        for (double** p : vs) *p = dalloc<double>(size_t(n));
+        hist = dalloc<double>(size_t(n * m_.ntstpsinpt));
+        h2d(hist, m_.pinpt.data(), size_t(n * m_.ntstpsinpt), d_.s);
+    }
+    if (!kcopy) kcopy = dalloc<double>(size_t(d_.nnzb) * size_t(d_.bs * d_.bs));
+    double am_ = 0, af_ = 0;
+    const double rho = m_.spectrds;
+    switch (m_.numopt) {
+    case 0:
+        if (rho != 1) FAIL(1, "invalid spectral radius for Newmark without dissipation");
+        break;
+    case 1: am_ = (2 * rho - 1) / (rho + 1); af_ = rho / (rho + 1); break;
+    case 2: af_ = (1 - rho) / (1 + rho); break;
+    case 3: am_ = (rho - 1) / (rho + 1); break;
+    default: FAIL(1, "unknown Newmark dissipation option");
+    }
+    const double alpha = std::pow(1 - am_ + af_, 2) / 4, delta = 0.5 - am_ + af_;
+    const double dtime = m_.dt;
+    const long ntst = long(m_.ttot / dtime + 1);
+    if (ntst > m_.ntstpsinpt) FAIL(1, "time steps exceed the load history");
+    {
+        Timer t(&st_->t_tangent, d_.s);
+        launch_tangent(d_, G_BASE, G_BASE, false);
+        d2d(kcopy, d_.K, size_t(d_.nnzb) * size_t(d_.bs * d_.bs), d_.s);
+        CB_CHECK(hipMemsetAsync(sm, 0, sizeof(double) * size_t(n), d_.s));
+        mass(sm);
+    }
+    LinearNewmarkArgs A;
+    A.P = hist; A.ld = m_.ntstpsinpt; A.am = am_; A.af = af_;
+    A.a0 = 1 / (alpha * std::pow(dtime, 2));
+    A.a2 = 1 / (alpha * dtime);
+    A.a3 = 1 / (2 * alpha) - 1;
+    A.a6 = dtime * (1 - delta);
+    A.a7 = delta * dtime;
+    add_mass_diag(d_, sm, A.a0 * (1 - am_), 1 - af_);
+    int det = 0;
+    factor_current(&det);                 // status ignored, as solve() does
+    h2d(um, m_.um.data(), size_t(n), d_.s);
+    h2d(vm, m_.vm.data(), size_t(n), d_.s);
+    h2d(amv, m_.am.data(), size_t(n), d_.s);
+    const bool ckpt = !o_.checkpoint_path.empty();
+    long k = 0;
+    if (m_.rflag == 1) {
+        if (!ckpt) FAIL(1, "restart (RFLAG 1) without a checkpoint file");
+        long tstp = 0;
+        std::vector<double> u0, v0, a0v;
+        if (read_checkpoint_linear(o_.checkpoint_path, n, &tstp, u0, v0, a0v))
+            FAIL(1, "unable to read the checkpoint file");
+        h2d(um, u0.data(), size_t(n), d_.s);
+        h2d(vm, v0.data(), size_t(n), d_.s);
+        h2d(amv, a0v.data(), size_t(n), d_.s);
+        if (tstp + 1 == ntst) record(double(tstp) * dtime, 0, um);
+        k = tstp + 1;
+    }
+    A.sm = sm; A.u = um; A.v = vm; A.a = amv; A.r = r;
+    A.unew = uc; A.vnew = vc; A.anew = ac;
+    for (; k < ntst; ++k) {
+        A.k = k;
+        A.Ku = nullptr;
+        if (af_ != 0) {
+            // K u with the tangent kept before the mass term went onto the diagonal
+            to_slots(d_, eb, um);
+            std::swap(d_.K, kcopy);
+            bsr_spmv(d_, eb, ex);
+            std::swap(d_.K, kcopy);
+            from_slots(d_, dyn, ex);
+            A.Ku = dyn;
+        }
+        linear_newmark_vec(d_, LNM_REFF, A);
+        linear_solve(r, uc);
+        linear_newmark_vec(d_, LNM_UPDATE, A);
+        st_->newton++;
+        record(double(k) * dtime, 0, uc);
+        vcopy(d_, um, uc, n);
+        vcopy(d_, vm, vc, n);
+        vcopy(d_, amv, ac, n);
+        if (ckpt && k % m_.chkpt == 0 && k != 0) {
+            const size_t nu = static_cast<size_t>(n);
+            std::vector<double> hu(nu), hv(nu), ha(nu);
+            d2h(hu.data(), um, size_t(n), d_.s);
+            d2h(hv.data(), vm, size_t(n), d_.s);
+            d2h(ha.data(), amv, size_t(n), d_.s);
+            CB_CHECK(hipStreamSynchronize(d_.s));
+            if (!write_checkpoint_linear(o_.checkpoint_path, k, hu, hv, ha, o_.checkpoint_exact != 0))
+                FAIL(1, "unable to write the checkpoint file");
+        }
+        if (long(lpf.size()) >= o_.max_steps) return 0;
+    }
+    record(m_.lpfmax, 0, dv);             // dv is zero: the reference's d on this path
     return 0;
 }
 

@@ -126,6 +126,9 @@ private:
     int run_nr();
     int run_msal();
     int run_newmark();
+    int run_linear_static();     // ANAFLAG 1 under ALGFLAG 1/2 (main.c:1714-1798)
+    int run_linear_newmark();    // ALGFLAG 4 (main.c:3143-3303, solve.c:139-468)
+    double* kcopy = nullptr;     // the tangent K kept beside Keff (linear Newmark, K u)
     // nonlinear Newmark state (allocated on first use)
     double *sm = nullptr, *qtot = nullptr, *uc = nullptr, *vc = nullptr, *ac = nullptr;
     double *vci = nullptr, *aci = nullptr, *um = nullptr, *vm = nullptr, *amv = nullptr;

@@ -174,7 +174,8 @@ __global__ void __launch_bounds__(128) k_shell_tangent(
     const double* __restrict__ c2, const double* __restrict__ c3, const double* __restrict__ emod,
     const double* __restrict__ nu, const double* __restrict__ thick, const double* __restrict__ xlocal,
     const double* __restrict__ farea, const double* __restrict__ slength,
-    const double* __restrict__ deffarea, double* __restrict__ K, long ioff, long toff, long poff) {
+    const double* __restrict__ deffarea, double* __restrict__ K, long ioff, long toff, long poff,
+    int geo) {
     int it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= n) return;
     const long e = els[it];
@@ -209,13 +210,14 @@ __global__ void __launch_bounds__(128) k_shell_tangent(
     const double gxy = gy[1] * du2 + gy[2] * du3 + gx[2] * dv3;
     const double Nx = t * (c0 * ex + c0 * v * ey), Ny = t * (c0 * v * ex + c0 * ey);
     const double Nxy = t * (c0 * (1 - v) / 2 * gxy);
+    // linear elastic analysis (ANAFLAG 1) has no geometric term (shell.c:150-153)
     double g[3][3];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
             double r0 = gx[a] * Nx + gy[a] * Nxy, r1 = gx[a] * Nxy + gy[a] * Ny;
-            g[a][b] = Aip * (r0 * gx[b] + r1 * gy[b]);
+            g[a][b] = geo ? Aip * (r0 * gx[b] + r1 * gy[b]) : 0.0;
         }
 #pragma unroll
     for (int a = 0; a < 3; ++a)
@@ -357,20 +359,21 @@ __global__ void __launch_bounds__(128) k_shell_forces(
 }
 
 // ---------------------------------------------------------------- truss math
-// truss.c:82-204: EA L^2/L0^3 on the axis plus N/L on the translation blocks.
+// truss.c:82-204: EA L^2/L0^3 on the axis plus N/L on the translation blocks;
+// EA/L0 alone in linear elastic analysis (ANAFLAG 1, truss.c:104-106).
 template <class T>
 __global__ void k_truss_tangent(const int32_t* __restrict__ els, int n, const int32_t* __restrict__ blk,
                                 const double* __restrict__ c1, const double* __restrict__ c2,
                                 const double* __restrict__ c3, const double* __restrict__ emod,
                                 const double* __restrict__ carea, const double* __restrict__ llen,
                                 const double* __restrict__ defllen, const double* __restrict__ ef,
-                                T* __restrict__ K, int bs) {
+                                T* __restrict__ K, int bs, int geo) {
     int it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= n) return;
     const long e = els[it];
     const double L0 = llen[e], L = defllen[e];
-    const double ka = emod[e] * carea[e] * (L * L) / (L0 * L0 * L0);
-    const double gN = ef[e * 2] / L;
+    const double ka = geo ? emod[e] * carea[e] * (L * L) / (L0 * L0 * L0) : emod[e] * carea[e] / L0;
+    const double gN = geo ? ef[e * 2] / L : 0.0;
     const double c[3] = {c1[e], c2[e], c3[e]};
     const int bb = bs * bs;
     for (int a = 0; a < 2; ++a)
@@ -406,6 +409,98 @@ __global__ void k_truss_forces(const int32_t* __restrict__ els, int n, const int
         int32_t q = mcode[e * 6 + j];
         if (q) f[q - 1] -= ((j < 3) ? N : -N) * c[j % 3];
     }
+}
+
+// ---------------------------------------------------------------- linear forces
+// Linear elastic analysis (ANAFLAG 1) from the total displacement D.
+// truss.c:243-326: d = T D, ef = EA/L0 [1 -1; -1 1] d, f -= ef c at each end.
+__global__ void k_truss_forces_lin(const int32_t* __restrict__ els, int n, const int32_t* __restrict__ mcode,
+                                   const double* __restrict__ D, const double* __restrict__ c1,
+                                   const double* __restrict__ c2, const double* __restrict__ c3,
+                                   const double* __restrict__ emod, const double* __restrict__ carea,
+                                   const double* __restrict__ llen, double* __restrict__ ef_i,
+                                   double* __restrict__ f) {
+    int it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= n) return;
+    const long e = els[it];
+    const double c[3] = {c1[e], c2[e], c3[e]};
+    double De[6];
+    for (int j = 0; j < 6; ++j) {
+        int32_t q = mcode[e * 6 + j];
+        De[j] = q ? D[q - 1] : 0.0;
+    }
+    const double d0 = c[0] * De[0] + c[1] * De[1] + c[2] * De[2];
+    const double d1 = c[0] * De[3] + c[1] * De[4] + c[2] * De[5];
+    const double ka = carea[e] * emod[e] / llen[e];
+    const double e0 = ka * d0 + (-ka) * d1, e1 = (-ka) * d0 + ka * d1;
+    ef_i[e * 2] = e0;
+    ef_i[e * 2 + 1] = e1;
+    for (int j = 0; j < 6; ++j) {
+        int32_t q = mcode[e * 6 + j];
+        if (q) f[q - 1] -= ((j < 3) ? e0 : e1) * c[j % 3];
+    }
+}
+
+// shell.c:1695-1727 with the scatter of shell.c:2385-2397: ef = k (T_ip D) with
+// the elastic matrix of stiffe_sh (membrane, bending and the drilling term),
+// f += T_i^T ef.
+__global__ void __launch_bounds__(128) k_shell_forces_lin(
+    const int32_t* __restrict__ els, int n, const int32_t* __restrict__ mcode, const double* __restrict__ D,
+    const double* __restrict__ c1p, const double* __restrict__ c2p, const double* __restrict__ c3p,
+    const double* __restrict__ c1c, const double* __restrict__ c2c, const double* __restrict__ c3c,
+    const double* __restrict__ emod, const double* __restrict__ nu, const double* __restrict__ thick,
+    const double* __restrict__ xlocal, const double* __restrict__ farea,
+    const double* __restrict__ slength, double* __restrict__ ef_i, double* __restrict__ f, long toff,
+    long poff, long moff, long eoff) {
+    int it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= n) return;
+    const long e = els[it];
+    const double E = emod[poff + e], v = nu[e], t = thick[e], A0 = farea[e];
+    const double xl[3] = {xlocal[e * 3], xlocal[e * 3 + 1], xlocal[e * 3 + 2]};
+    const double sl[3] = {slength[e * 3], slength[e * 3 + 1], slength[e * 3 + 2]};
+    double Rp[3][3], Rc[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        Rp[0][k] = c1p[toff + e * 3 + k]; Rp[1][k] = c2p[toff + e * 3 + k]; Rp[2][k] = c3p[toff + e * 3 + k];
+        Rc[0][k] = c1c[toff + e * 3 + k]; Rc[1][k] = c2c[toff + e * 3 + k]; Rc[2][k] = c3c[toff + e * 3 + k];
+    }
+    const int32_t* mc = mcode + moff + e * 18;
+    double dl[18];
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        double Db[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            int32_t q = mc[3 * b + k];
+            Db[k] = q ? D[q - 1] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r) dl[3 * b + r] = Rp[r][0] * Db[0] + Rp[r][1] * Db[1] + Rp[r][2] * Db[2];
+    }
+    ShellLocal L;
+    shell_membrane(E, v, t, xl, A0, L.km);
+    shell_bending(E, v, t, xl, A0, sl, L.kb);
+    const double drill[3] = {L.kb[1][1] / 10000, L.kb[4][4] / 10000, L.kb[7][7] / 10000};
+    const double g0[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double ef[18];
+    for (int i = 0; i < 18; ++i) {
+        const int a = i / 6, r = i % 6;
+        double s = 0;
+        for (int j = 0; j < 18; ++j) s += local_entry(L, g0, drill, a, j / 6, r, j % 6) * dl[j];
+        ef[i] = s;
+    }
+    double* efo = ef_i + eoff + e * 18;
+#pragma unroll
+    for (int i = 0; i < 18; ++i) efo[i] = ef[i];
+#pragma unroll
+    for (int b = 0; b < 6; ++b)
+#pragma unroll
+        for (int col = 0; col < 3; ++col) {
+            int i = 3 * b + col;
+            double s = Rc[0][col] * ef[3 * b] + Rc[1][col] * ef[3 * b + 1] + Rc[2][col] * ef[3 * b + 2];
+            int32_t q = mc[i];
+            if (q) f[q - 1] += s;
+        }
 }
 
 // ---------------------------------------------------------------- updatc
@@ -616,7 +711,8 @@ void launch_tangent(Dev& d, int gx, int g, bool want_fp32) {
         if (!n) continue;
         hipLaunchKernelGGL(k_truss_tangent<double>, dim3(nblk(n, 128)), dim3(128), 0, d.s,
                            d.col_el_tr + d.col_ptr_tr[c], n, d.blk_tr, d.c1[g], d.c2[g], d.c3[g],
-                           d.emod, d.carea, d.llen, d.defllen[g], d.ef[g], d.K, d.bs);
+                           d.emod, d.carea, d.llen, d.defllen[g], d.ef[g], d.K, d.bs,
+                           int(d.anaflag != 1));
     }
     if (d.nfr) launch_frame_tangent(d, g);
     for (size_t c = 0; c + 1 < d.col_ptr_sh.size(); ++c) {
@@ -625,7 +721,7 @@ void launch_tangent(Dev& d, int gx, int g, bool want_fp32) {
         hipLaunchKernelGGL(k_shell_tangent, dim3(nblk(n, 128)), dim3(128), 0, d.s,
                            d.col_el_sh + d.col_ptr_sh[c], n, d.minc, d.blk_sh, d.x[gx], d.c1[g],
                            d.c2[g], d.c3[g], d.emod, d.nu, d.thick, d.xlocal, d.farea, d.slength,
-                           d.deffarea[g], d.K, io, to, po);
+                           d.deffarea[g], d.K, io, to, po, int(d.anaflag != 1));
     }
     if (d.bs == 6)
         hipLaunchKernelGGL((k_fix_slots<double>), dim3(nblk(d.nslot)), dim3(TPB), 0, d.s, d.K,
@@ -643,6 +739,28 @@ void launch_tangent(Dev& d, int gx, int g, bool want_fp32) {
 void launch_forces(Dev& d, int gx, double* f, const double* dd) {
     const long to = d.ntr + d.nfr * 3, io = d.ntr * 2 + d.nfr * 2, po = d.ntr + d.nfr;
     const long mo = d.ntr * 6 + d.nfr * 14, eo = d.ntr * 2 + d.nfr * 14;
+    if (d.anaflag == 1) {
+        // linear elastic: dd is the total displacement; frames take their
+        // elastic matrix alone (the geometric one is off for ANAFLAG 1)
+        for (size_t c = 0; c + 1 < d.col_ptr_tr.size(); ++c) {
+            int n = d.col_ptr_tr[c + 1] - d.col_ptr_tr[c];
+            if (!n) continue;
+            hipLaunchKernelGGL(k_truss_forces_lin, dim3(nblk(n)), dim3(TPB), 0, d.s,
+                               d.col_el_tr + d.col_ptr_tr[c], n, d.mcode, dd, d.c1[G_I], d.c2[G_I],
+                               d.c3[G_I], d.emod, d.carea, d.llen, d.ef[G_I], f);
+        }
+        if (d.nfr) launch_frame_forces(d, f, dd);
+        for (size_t c = 0; c + 1 < d.col_ptr_sh.size(); ++c) {
+            int n = d.col_ptr_sh[c + 1] - d.col_ptr_sh[c];
+            if (!n) continue;
+            hipLaunchKernelGGL(k_shell_forces_lin, dim3(nblk(n, 128)), dim3(128), 0, d.s,
+                               d.col_el_sh + d.col_ptr_sh[c], n, d.mcode, dd, d.c1[G_IP], d.c2[G_IP],
+                               d.c3[G_IP], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.emod, d.nu, d.thick,
+                               d.xlocal, d.farea, d.slength, d.ef[G_I], f, to, po, mo, eo);
+        }
+        CB_CHECK(hipGetLastError());
+        return;
+    }
     for (size_t c = 0; c + 1 < d.col_ptr_tr.size(); ++c) {
         int n = d.col_ptr_tr[c + 1] - d.col_ptr_tr[c];
         if (!n) continue;

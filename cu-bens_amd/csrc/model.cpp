@@ -42,6 +42,13 @@ public:
         p_ = size_t(e - s_.c_str());
         return v;
     }
+    // whether a comma follows the last value, past white space: the literal
+    // "," of a scanf format that continues the line (model.c:252)
+    bool comma_next() const {
+        size_t p = p_;
+        while (p < s_.size() && std::isspace((unsigned char)s_[p])) ++p;
+        return p < s_.size() && s_[p] == ',';
+    }
 
 private:
     void skip() {
@@ -271,9 +278,11 @@ bool load_model(const char* path, Model& m, std::string& err) {
     if (t.bad()) FAIL("truncated header");
     if (nsbr || nfbr) FAIL("brick elements are not on this path");
     if (m.optflag != 1 && m.optflag != 2) FAIL("OPTFLAG must be 1 or 2");
-    if (m.anaflag != 2) FAIL("only ANAFLAG 2 (geometrically nonlinear elastic) runs on the GPU path");
-    if (m.algflag != 1 && m.algflag != 2 && m.algflag != 3 && m.algflag != 5)
-        FAIL("ALGFLAG must be 1, 2, 3 or 5");
+    if (m.anaflag != 1 && m.anaflag != 2)
+        FAIL("only ANAFLAG 1 (linear elastic) and 2 (geometrically nonlinear elastic) run on the GPU path");
+    if (m.algflag < 1 || m.algflag > 5) FAIL("ALGFLAG must be 1 to 5");
+    if (m.anaflag == 1 && (m.algflag == 3 || m.algflag == 5))
+        FAIL("linear analysis (ANAFLAG 1) runs under ALGFLAG 1, 2 (static) or 4 (linear Newmark)");
     if (m.nj <= 0 || m.ntr < 0 || m.nfr < 0 || m.nsh < 0) FAIL("bad model sizes");
     const long nj = m.nj, ntr = m.ntr, nfr = m.nfr, nsh = m.nsh;
     m.minc.assign(size_t(ntr * 2 + nfr * 2 + nsh * 3), 0);
@@ -289,7 +298,9 @@ bool load_model(const char* path, Model& m, std::string& err) {
         if (t.bad()) FAIL("truncated constraints");
         if (j == 0) break;
         if (k < 1 || k > 7 || j < 1 || j > nj) FAIL("joint constraint input not recognized");
-        if (k == 7) m.wrpres[(j - 1) * 3] = int(t.l());
+        // "j,7,restraint"; a bare "j,7" leaves the restraint 0, as the
+        // reference's fscanf(",%d") matches nothing then (model.c:251-254)
+        if (k == 7 && t.comma_next()) m.wrpres[(j - 1) * 3] = int(t.l());
         m.jcode[(j - 1) * 7 + (k - 1)] = 0;
     }
     m.jnt.resize(size_t(nj));
@@ -487,7 +498,12 @@ bool load_model(const char* path, Model& m, std::string& err) {
         }
         if (!any) FAIL("no forces present in input file");
     }
-    if (m.algflag == 1 || m.algflag == 2 || m.algflag == 5) {
+    if ((m.algflag == 1 || m.algflag == 2) && m.anaflag == 1) {
+        m.lpfmax = t.d();                                   // main.c:1717
+    } else if (m.algflag == 4) {
+        m.numopt = int(t.d()); m.spectrds = t.d();          // main.c:3184-3197
+        m.lpfmax = t.d();
+    } else if (m.algflag == 1 || m.algflag == 2 || m.algflag == 5) {
         if (m.algflag == 5) { m.numopt = int(t.d()); m.spectrds = t.d(); }
         m.lpfmax = t.d(); m.lpf0 = t.d(); m.dlpf0 = t.d(); m.dlpfmax = t.d(); m.dlpfmin = t.d();
         m.itemax = int(t.l()); m.submax = int(t.l()); m.solmin = int(t.l());

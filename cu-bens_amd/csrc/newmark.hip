@@ -160,6 +160,23 @@ __global__ void k_nm_vec(NewmarkVec op, NewmarkArgs A, long n) {
     if (i < n) nm_apply(op, A, i);
 }
 
+__global__ void k_linear_nm(LinearNewmarkVec op, LinearNewmarkArgs A, long n) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (op == LNM_REFF) {
+        const double meff = A.sm[i] * ((1 - A.am) * (A.u[i] * A.a0 + A.v[i] * A.a2 + A.a[i] * A.a3) -
+                                       A.am * A.a[i]) / (1 - A.af);
+        const double* p = A.P + i * A.ld;
+        double r = (A.k == 0) ? p[A.k] + meff : p[A.k] + A.af / (1 - A.af) * p[A.k - 1] + meff;
+        if (A.Ku) r -= A.af / (1 - A.af) * A.Ku[i];
+        A.r[i] = r;
+    } else {
+        const double an = A.a0 * (A.unew[i] - A.u[i]) - A.a2 * A.v[i] - A.a3 * A.a[i];
+        A.anew[i] = an;
+        A.vnew[i] = A.v[i] + A.a6 * A.a[i] + A.a7 * an;
+    }
+}
+
 }  // namespace
 
 void launch_mass(Dev& d, double* sm, int type) {
@@ -200,6 +217,12 @@ void launch_mass(Dev& d, double* sm, int type) {
 void add_mass_diag(Dev& d, const double* sm, double cnum, double cden) {
     hipLaunchKernelGGL(k_add_mass_diag, dim3(nblk(d.nslot)), dim3(TPB), 0, d.s, d.K, d.diagblk,
                        d.eq_of_slot, sm, cnum, cden, d.nslot, d.bs);
+    CB_CHECK(hipGetLastError());
+}
+
+void linear_newmark_vec(Dev& d, LinearNewmarkVec op, const LinearNewmarkArgs& a) {
+    if (!d.neq) return;
+    hipLaunchKernelGGL(k_linear_nm, dim3(nblk(d.neq)), dim3(TPB), 0, d.s, op, a, d.neq);
     CB_CHECK(hipGetLastError());
 }
 

@@ -24,14 +24,14 @@ inline int nblk(long n) { return int((n + TPB - 1) / TPB); }
 __global__ void k_react_truss(const int32_t* __restrict__ els, int n, const int32_t* __restrict__ minc,
                               const double* __restrict__ c1, const double* __restrict__ c2,
                               const double* __restrict__ c3, const double* __restrict__ ef,
-                              double* __restrict__ rx) {
+                              double* __restrict__ rx, double sgn) {
     const int it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= n) return;
     const long e = els[it];
     const double c[3] = {c1[e], c2[e], c3[e]};
     for (int a = 0; a < 2; ++a) {
         const long j = minc[e * 2 + a] - 1;
-        const double v = ef[e * 2 + a];
+        const double v = sgn * ef[e * 2 + a];
         for (int k = 0; k < 3; ++k) rx[j * 7 + k] -= v * c[k];
     }
 }
@@ -122,7 +122,10 @@ void launch_reactions(Dev& d, int g, double* rx) {
         const int n = d.col_ptr_tr[c + 1] - d.col_ptr_tr[c];
         if (n)
             hipLaunchKernelGGL(k_react_truss, dim3(nblk(n)), dim3(TPB), 0, d.s, d.col_el_tr + d.col_ptr_tr[c], n,
-                               d.minc, d.c1[g], d.c2[g], d.c3[g], d.ef[g], rx);
+                               d.minc, d.c1[g], d.c2[g], d.c3[g], d.ef[g], rx,
+                               // the linear truss end forces carry the opposite sign
+                               // (truss.c:300-305): EA/L0 (d1 - d2) at end 1
+                               d.anaflag == 1 ? -1.0 : 1.0);
     }
     for (size_t c = 0; c + 1 < d.col_ptr_fr.size(); ++c) {
         const int n = d.col_ptr_fr[c + 1] - d.col_ptr_fr[c];

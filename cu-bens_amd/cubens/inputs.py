@@ -164,11 +164,44 @@ def truss_dome(n: int = 10, rise: float = 0.5, seed: int | None = None,
                      info={"centre_joint": ctr, "neq": 3 * (n - 1) ** 2})
 
 
+def _linear_tail(w, analysis: str, lpfmax: float, load_lines, steps: int, dt: float,
+                 numopt: int, spectrds: float, u0_lines, chkpt: int | None, rflag: int) -> None:
+    """Load section and driver parameters of the linear analyses: static
+    (ANAFLAG 1 under ALGFLAG 1, main.c:1717) or linear Newmark (ALGFLAG 4,
+    model.c:1492-1606, main.c:3184-3197).  load_lines are (joint, dir, value)
+    triples: static loads, or constant histories of the Newmark run."""
+    if analysis == "static":
+        for j, d, v in load_lines:
+            w(f"{j},{d},{_fmt(v)}\n")
+        w("0,0,0\n")
+    else:
+        w(f"{steps},{_fmt(steps * dt)}\n")
+        for j, d, v in load_lines:
+            for _ in range(steps + 1):
+                w(f"{j},{d},{_fmt(v)}\n")
+        w("0,0,0\n0,0,0\n")
+        for j, d, *uva in u0_lines:
+            u, v, a = (list(uva) + [0.0, 0.0])[:3]
+            w(f"{j},{d},{_fmt(u)},{_fmt(v)},{_fmt(a)}\n")
+        w("0,0,0,0,0\n")
+        w(f"{numopt},{_fmt(spectrds)}\n")
+    w(f"{_fmt(lpfmax)}\n")
+
+
+def _linear_head(analysis: str, chkpt: int | None, rflag: int, steps: int) -> str:
+    if analysis == "static":
+        return "1\n1\n"
+    return f"1\n4\n{steps + 1 if chkpt is None else chkpt},{rflag}\n"
+
+
 def star_truss(bars: int = 4, radius: float = 1.0, rise: float = 0.1, emod: float = 2e11,
                area: float = 1e-4, load: float = -1.0, dk: float = -0.005, alpha: float = 1.0,
                psi_thresh: float = 0.1, iteopt: int = 6, dkimax: float | None = None,
                lpfmax: float = 1e30, itemax: int = 30, submax: int = 10, imagmax: int = 10,
-               negmax: int = 10, tol=(1e-12, 1e-10, 1.0), tilt: float = 0.0) -> ModelText:
+               negmax: int = 10, tol=(1e-12, 1e-10, 1.0), tilt: float = 0.0,
+               analysis: str = "msal", steps: int = 20, dt: float = 1e-3, numopt: int = 0,
+               spectrds: float = 1.0, u0: float = 0.0, dens: float = 7850.0,
+               chkpt: int | None = None, rflag: int = 0, a0: float = 0.0) -> ModelText:
     """A shallow star truss under arc-length control: ``bars`` equal bars from
     pinned supports on a circle of ``radius`` (at height 0, angles 2 pi k/bars)
     to one apex at height ``rise``, loaded by ``load`` in direction 3 at the
@@ -183,6 +216,13 @@ def star_truss(bars: int = 4, radius: float = 1.0, rise: float = 0.1, emod: floa
     (H = 0), reaches a minimum and rises again once the apex has snapped
     through.  ``tilt`` > 0 moves the apex sideways by tilt * radius so the
     three DOFs couple (no closed form then).
+
+    ``analysis`` "static" writes a linear elastic analysis (ANAFLAG 1,
+    ALGFLAG 1) of ``load`` times ``lpfmax``; "newmark" a linear Newmark run
+    (ALGFLAG 4) of ``steps`` steps of ``dt`` under the constant history
+    ``load`` from the initial apex displacement ``u0`` (and acceleration
+    ``a0``) in direction 3, with Newmark option ``numopt``/``spectrds`` and
+    checkpoint ``chkpt``/``rflag``.
     """
     h = float(rise)
     if dkimax is None:
@@ -190,19 +230,28 @@ def star_truss(bars: int = 4, radius: float = 1.0, rise: float = 0.1, emod: floa
     out = io.StringIO()
     w = out.write
     nj = bars + 1
-    w(f"2\n3\n0\n1\n{nj}\n{bars},0,0,0,0\n")
+    head = "2\n3\n" if analysis == "msal" else _linear_head(analysis, chkpt, rflag, steps)
+    w(f"{head}0\n1\n{nj}\n{bars},0,0,0,0\n")
     for k in range(bars):
         w(f"{k + 2},1\n")             # support -> apex (apex is joint 1)
     for k in range(bars):
         for d in (1, 2, 3):
             w(f"{k + 2},{d}\n")
     w("0,0\n")
+    if analysis == "newmark":
+        w("0,0\n")                    # no prescribed displacement DOFs
     w(f"{_fmt(tilt * radius)},0.0,{_fmt(h)}\n")
     for k in range(bars):
         t = 2 * math.pi * k / bars
         w(f"{_fmt(radius * math.cos(t))},{_fmt(radius * math.sin(t))},0.0\n")
     for _ in range(bars):
-        w(f"{_fmt(emod)},{_fmt(area)},7850.0,1e30\n")
+        w(f"{_fmt(emod)},{_fmt(area)},{_fmt(dens)},1e30\n")
+    if analysis != "msal":
+        _linear_tail(w, analysis, lpfmax, [(1, 3, load)], steps, dt, numopt, spectrds,
+                     [(1, 3, u0, 0.0, a0)] if (u0 or a0) else [], chkpt, rflag)
+        return ModelText(out.getvalue(), nj, n_tr=bars,
+                         info={"bars": bars, "radius": radius, "rise": h, "emod": emod, "area": area,
+                               "load": load, "neq": 3, "dens": dens})
     w(f"1,3,{_fmt(load)}\n0,0,0\n")
     w(f"1,3,{_fmt(dk)}\n")
     w(f"{_fmt(alpha)}\n{_fmt(psi_thresh)}\n{iteopt}\n")
@@ -237,7 +286,7 @@ def shell_panel(n: int = 64, radius: float = 2.54, angle: float = 0.2, length: f
                 itemax: int = 40, submax: int = 10, imagmax: int = 10, negmax: int = 10,
                 tol=(1.0, 1e-3, 1.0), load: float = -1.0, lpfmax: float = 1e30,
                 dkimax: float = 1e30, optflag: int = 1, scramble: int | None = None,
-                edge_beam: float = 0.0, tie_area: float = 0.0) -> ModelText:
+                edge_beam: float = 0.0, tie_area: float = 0.0, analysis: str = "msal") -> ModelText:
     """n x n quads, each split on one fixed diagonal into two CST+DKT triangles.
 
     Straight edges (i = 0 and i = n) are hinged (translations fixed); the curved
@@ -260,6 +309,9 @@ def shell_panel(n: int = 64, radius: float = 2.54, angle: float = 0.2, length: f
     edges (circular hollow sections of that outer radius, the shell's E, warping
     fixed at their joints), ``tie_area`` > 0 adds truss bars of that area on the
     second diagonal of the quads of the middle row.
+
+    ``analysis`` "static" writes a linear elastic analysis (ANAFLAG 1,
+    ALGFLAG 1) of the centre load times ``lpfmax``.
     """
     rng = SplitMix64(workload_seed(config_index) if seed is None else seed)
     m = n + 1
@@ -288,7 +340,8 @@ def shell_panel(n: int = 64, radius: float = 2.54, angle: float = 0.2, length: f
         t_e = np.full(ne, thick)
     out = io.StringIO()
     w = out.write
-    w(f"2\n3\n0\n{optflag}\n{m * m}\n{len(bars)},{len(beams)},{ne},0,0\n")
+    head = "2\n3\n" if analysis == "msal" else _linear_head(analysis, None, 0, 0)
+    w(f"{head}0\n{optflag}\n{m * m}\n{len(bars)},{len(beams)},{ne},0,0\n")
     for a, b in bars + beams:
         w(f"{a},{b}\n")
     for t in tris:
@@ -324,6 +377,11 @@ def shell_panel(n: int = 64, radius: float = 2.54, angle: float = 0.2, length: f
     w(f"{ctr},3,{_fmt(load)}\n0,0,0\n")
     if beams:
         w("0,0,0\n")                              # no distributed member loads
+    neq = 6 * m * m - 3 * 2 * m
+    if analysis == "static":
+        w(f"{_fmt(lpfmax)}\n")
+        return ModelText(out.getvalue(), m * m, n_tr=len(bars), n_fr=len(beams), n_sh=ne,
+                         info={"centre_joint": ctr, "neq": neq, "n": n})
     w(f"{ctr},3,{_fmt(dk)}\n")
     w(f"{_fmt(alpha)}\n{_fmt(psi_thresh)}\n{iteopt}\n")
     w(f"{_fmt(lpfmax)},{_fmt(dkimax)}\n")
@@ -501,6 +559,50 @@ def space_frame(n_elem: int = 400, kind: str = "tower", seed: int | None = None,
     return ModelText(out.getvalue(), nj, n_fr=ne,
                      info={"kind": kind, "freq": freq, "amp": amp, "fixed": [jo(j) for j in fixed],
                            "loaded": [jo(j) for j in loaded], "neq": 6 * (nj - len(fixed))})
+
+
+def cantilever_frame(n_elem: int = 4, length: float = 3.0, r_o: float = 0.1, load: float = -1000.0,
+                     lpfmax: float = 1.0, analysis: str = "static", steps: int = 20, dt: float = 1e-3,
+                     numopt: int = 0, spectrds: float = 1.0, u0: float = 0.0,
+                     chkpt: int | None = None, rflag: int = 0) -> ModelText:
+    """A straight cantilever of ``n_elem`` frame members along x, clamped at
+    joint 1 (warping fixed at every joint), with ``load`` in direction 2 at the
+    tip: a linear elastic analysis (static or linear Newmark, as star_truss
+    writes them).  The members are Euler-Bernoulli beams, so the static tip
+    deflection is load lpfmax L^3 / (3 E I) exactly for any n_elem."""
+    sec = chs_section(r_o)
+    nj = n_elem + 1
+    out = io.StringIO()
+    w = out.write
+    w(f"{_linear_head(analysis, chkpt, rflag, steps)}0\n1\n{nj}\n0,{n_elem},0,0,0\n")
+    for e in range(n_elem):
+        w(f"{e + 1},{e + 2}\n")
+    for d in range(1, 7):
+        w(f"1,{d}\n")
+    for j in range(1, nj + 1):
+        w(f"{j},7,0\n")
+    w("0,0\n")
+    if analysis == "newmark":
+        w("0,0\n")
+    for j in range(nj):
+        w(f"{_fmt(length * j / n_elem)},0.0,0.0\n")
+    for _ in range(n_elem):
+        w(f"{_fmt(sec['E'])},{_fmt(sec['G'])},7850.0,{_fmt(sec['A'])},{_fmt(sec['I'])},"
+          f"{_fmt(sec['I'])},{_fmt(sec['J'])},0.0\n")
+    w("0,0,0,0,0,0,0\n")
+    for e in range(n_elem):
+        w(f"{_fmt(length * e / n_elem)},0.0,1.0\n")     # aux point above the member
+    for _ in range(n_elem):
+        w("1e+30,1.0,1.0\n")
+    w("0,0,0,0,0\n")
+    if analysis == "static":
+        w(f"{nj},2,{_fmt(load)}\n0,0,0\n0,0,0\n{_fmt(lpfmax)}\n")
+    else:
+        _linear_tail(w, analysis, lpfmax, [(nj, 2, load)], steps, dt, numopt, spectrds,
+                     [(nj, 2, u0)] if u0 else [], chkpt, rflag)
+    return ModelText(out.getvalue(), nj, n_fr=n_elem,
+                     info={"E": sec["E"], "I": sec["I"], "A": sec["A"], "length": length,
+                           "load": load, "neq": 6 * n_elem})
 
 
 def frame_batch(count: int = 512, lo: int = 200, hi: int = 5000, **kw) -> list[ModelText]:

@@ -76,6 +76,15 @@ void orc_forces_tr(const orc_dims *d, double *f, double *ef_i, const double *emo
                    const double *carea, const double *llength, const double *defllen_i,
                    const double *c1_i, const double *c2_i, const double *c3_i,
                    const long *mcode);
+/* ANAFLAG 1 (linear elastic) forces from the total displacement D:
+ * truss.c:243-326 and shell.c:1695-1727 with the scatter of shell.c:2385-2397. */
+void orc_forces_tr_lin(const orc_dims *d, double *f, double *ef_i, const double *D,
+                       const double *emod, const double *carea, const double *llength,
+                       const double *c1, const double *c2, const double *c3, const long *mcode);
+void orc_forces_sh_lin(const orc_dims *d, double *f, double *ef_i, const double *D,
+                       const double *emod, const double *nu, const double *xlocal,
+                       const double *thick, const double *farea, const double *slength,
+                       const double *c1, const double *c2, const double *c3, const long *mcode);
 void orc_stiff_sh(const orc_dims *d, double *ss, const double *emod, const double *nu,
                   const double *x_temp, const double *xlocal, const double *thick,
                   const double *farea, const double *deffarea_ip, const double *slength,
@@ -130,6 +139,8 @@ void orc_reactions(const orc_model *m, const double *ef, const double *c1, const
 /* ---- solver pieces (solve.c, misc.c, arc.c) ------------------------------ */
 int orc_skyfact(long neq, const long *maxa, double *ss, double *ssd, int msal, int *det);
 void orc_skysolve(long neq, const long *maxa, const double *ss, double *rhs);
+/* out = K v for the symmetric skyline K (skymult, solve.c:700-760) */
+void orc_skymult(long neq, const long *maxa, const double *ss, const double *v, double *out);
 int orc_test(long neq, const double *d_temp, const double *dd, const double *f_temp,
              const double *fp, const double *qtot, const double *f_ip, double intener1,
              int *convchk, double toldisp, double tolforc, double tolener);
@@ -159,10 +170,13 @@ typedef struct orc_backend {
                      const double *yield, const double *c1_ip, const double *c2_ip,
                      const double *c3_ip, const double *ef_ip, const long *maxa,
                      const long *mcode);
+    /* d_tot: the total displacement (pd of forces_tr, pd_temp of forces_sh),
+       which the linear-elastic branches (ANAFLAG 1) read */
     void (*forces_tr)(void *ctx, const orc_dims *d, double *f, double *ef_i,
                       const double *emod, const double *carea, const double *llength,
                       const double *defllen_i, const double *yield, const double *c1_i,
-                      const double *c2_i, const double *c3_i, const long *mcode);
+                      const double *c2_i, const double *c3_i, const long *mcode,
+                      const double *d_tot);
     void (*stiff_sh)(void *ctx, const orc_dims *d, double *ss, const double *emod,
                      const double *nu, const double *x_temp, const double *xlocal,
                      const double *thick, const double *farea, const double *deffarea_ip,
@@ -177,7 +191,8 @@ typedef struct orc_backend {
                      const double *slength, const double *defslen_ip, const double *yield,
                      const double *c1_ip, const double *c2_ip, const double *c3_ip,
                      const double *c1_i, const double *c2_i, const double *c3_i,
-                     const long *minc, const long *mcode, const long *jcode);
+                     const long *minc, const long *mcode, const long *jcode,
+                     const double *d_tot);
     void (*updatc)(void *ctx, const orc_dims *d, double *x_temp, double *x_ip,
                    double *xfr_temp, const double *dd, double *defllen_i,
                    double *deffarea_i, double *defslen_i, const double *offset,
@@ -271,6 +286,11 @@ typedef struct {
     double *farea_w, *slen_w;        /* NE_SH, NE_SH*3 working areas, side lengths */
 } orc_ckpt;
 int orc_ckpt_write(const char *path, const orc_dims *d, long lss, const orc_ckpt *c, int exact);
+/* The linear Newmark driver's checkpoint (solve.c:445-457, read at 257-282):
+ * the time step, then NEQ lines "u,v,a", then "0,0,0". */
+int orc_ckpt_write_lin(const char *path, long neq, long k, const double *u, const double *v,
+                       const double *a, int exact);
+int orc_ckpt_read_lin(const char *path, long neq, long *k, double *u, double *v, double *a);
 /* 0 ok, 1 cannot open, 2 malformed */
 int orc_ckpt_read(const char *path, const orc_dims *d, long lss, orc_ckpt *c);
 

@@ -153,3 +153,37 @@ int orc_ckpt_read(const char *path, const orc_dims *d, long lss, orc_ckpt *c)
     free(t.buf);
     return t.bad ? 2 : 0;
 }
+
+int orc_ckpt_write_lin(const char *path, long neq, long k, const double *u, const double *v,
+                       const double *a, int exact)
+{
+    FILE *f = fopen(path, "w");
+    if (!f) return 1;
+    fprintf(f, "%ld\n", k);
+    for (long i = 0; i < neq; ++i) {
+        double w[3] = {u[i], v[i], a[i]};
+        put(f, exact, 3, w);
+    }
+    fprintf(f, "%d,%d,%d\n", 0, 0, 0);
+    return fclose(f) ? 1 : 0;
+}
+
+int orc_ckpt_read_lin(const char *path, long neq, long *k, double *u, double *v, double *a)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) return 1;
+    fseek(f, 0, SEEK_END);
+    long len = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    tok_t t = {malloc((size_t)len + 1), NULL, 0};
+    if (!t.buf) { fclose(f); return 1; }
+    size_t got = fread(t.buf, 1, (size_t)len, f);
+    fclose(f);
+    t.buf[got] = 0;
+    t.p = t.buf;
+    *k = (long)tk(&t);
+    for (long i = 0; i < neq; ++i) { u[i] = tk(&t); v[i] = tk(&t); a[i] = tk(&t); }
+    tk_zeros(&t, 3);
+    free(t.buf);
+    return t.bad ? 2 : 0;
+}

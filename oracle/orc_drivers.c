@@ -37,10 +37,14 @@ static void be_stiff_tr(void *ctx, const orc_dims *d, double *ss, const double *
 static void be_forces_tr(void *ctx, const orc_dims *d, double *f, double *ef_i,
                          const double *emod, const double *carea, const double *llength,
                          const double *defllen_i, const double *yield, const double *c1_i,
-                         const double *c2_i, const double *c3_i, const long *mcode)
+                         const double *c2_i, const double *c3_i, const long *mcode,
+                         const double *d_tot)
 {
     (void)ctx; (void)yield;
-    orc_forces_tr(d, f, ef_i, emod, carea, llength, defllen_i, c1_i, c2_i, c3_i, mcode);
+    if (d->ANAFLAG == 1)
+        orc_forces_tr_lin(d, f, ef_i, d_tot, emod, carea, llength, c1_i, c2_i, c3_i, mcode);
+    else
+        orc_forces_tr(d, f, ef_i, emod, carea, llength, defllen_i, c1_i, c2_i, c3_i, mcode);
 }
 static void be_stiff_sh(void *ctx, const orc_dims *d, double *ss, const double *emod,
                         const double *nu, const double *x_temp, const double *xlocal,
@@ -61,11 +65,16 @@ static int be_forces_sh(void *ctx, const orc_dims *d, double *f, double *ef_ip, 
                         const double *slength, const double *defslen_ip, const double *yield,
                         const double *c1_ip, const double *c2_ip, const double *c3_ip,
                         const double *c1_i, const double *c2_i, const double *c3_i,
-                        const long *minc, const long *mcode, const long *jcode)
+                        const long *minc, const long *mcode, const long *jcode,
+                        const double *d_tot)
 {
     (void)ctx; (void)x_ip; (void)deffarea_ip; (void)defslen_ip; (void)yield; (void)jcode;
-    orc_forces_sh(d, f, ef_ip, ef_i, dd, x_temp, emod, nu, xlocal, thick, farea, slength,
-                  c1_ip, c2_ip, c3_ip, c1_i, c2_i, c3_i, minc, mcode);
+    if (d->ANAFLAG == 1)
+        orc_forces_sh_lin(d, f, ef_i, d_tot, emod, nu, xlocal, thick, farea, slength, c1_ip,
+                          c2_ip, c3_ip, mcode);
+    else
+        orc_forces_sh(d, f, ef_ip, ef_i, dd, x_temp, emod, nu, xlocal, thick, farea, slength,
+                      c1_ip, c2_ip, c3_ip, c1_i, c2_i, c3_i, minc, mcode);
     return 0;
 }
 static void be_updatc(void *ctx, const orc_dims *d, double *x_temp, double *x_ip,
@@ -184,7 +193,7 @@ static void run_init(run_t *R, orc_model *m, const orc_backend *be, orc_path *ou
     R->defslen = dz(R->nsh * 3); R->defslen_i = dz(R->nsh * 3); R->defslen_ip = dz(R->nsh * 3);
     R->efFE = dz(R->nfr * 14); R->efFE_i = dz(R->nfr * 14); R->efFE_ip = dz(R->nfr * 14);
     R->farea_w = dz(R->nsh); R->slen_w = dz(R->nsh * 3);
-    R->keff = dz(m->dm.ALGFLAG == 5 ? m->lss : 1);
+    R->keff = dz(m->dm.ALGFLAG >= 4 ? m->lss : 1);
     R->uc = dz(n); R->vc = dz(n); R->ac = dz(n); R->uc_i = dz(n); R->vc_i = dz(n);
     R->ac_i = dz(n); R->um = dz(n); R->vm = dz(n); R->am = dz(n); R->dyn = dz(n);
     CP(R->farea_w, m->farea, R->nsh);
@@ -278,7 +287,7 @@ static int update_and_forces(run_t *R, double *x_temp, double *xfr_temp, double 
                   m->minc, m->jcode);
     if (d->NE_TR > 0)
         R->be->forces_tr(R->be->ctx, d, f_out, R->ef_i, m->emod, m->carea, R->llen_ref,
-                         R->defllen_i, m->yield, R->c1_i, R->c2_i, R->c3_i, m->mcode);
+                         R->defllen_i, m->yield, R->c1_i, R->c2_i, R->c3_i, m->mcode, R->d_temp);
     int chk = 0;
     if (d->NE_FR > 0) {
         double dl = 0;
@@ -291,7 +300,7 @@ static int update_and_forces(run_t *R, double *x_temp, double *xfr_temp, double 
                                m->emod, m->nu, m->xlocal, m->thick, R->farea_w, R->deffarea_ip,
                                R->slen_w, R->defslen_ip, m->yield, R->c1_ip, R->c2_ip,
                                R->c3_ip, R->c1_i, R->c2_i, R->c3_i, m->minc, m->mcode,
-                               m->jcode);
+                               m->jcode, R->d_temp);
     if (R->out) R->out->t_forces += now_s() - t0;
     return chk;
 }
@@ -906,7 +915,7 @@ static int run_newmark(run_t *R)
                     if (d->NE_TR > 0)
                         R->be->forces_tr(R->be->ctx, d, R->f_temp, R->ef_i, m->emod, m->carea, llen,
                                          R->defllen_i, m->yield, R->c1_i, R->c2_i, R->c3_i,
-                                         m->mcode);
+                                         m->mcode, R->d_temp);
                     int frc_fr = 0, frc_sh = 0;
                     if (d->NE_FR > 0)
                         frc_fr = R->be->forces_fr(R->be->ctx, d, R->f_temp, R->ef_ip, R->ef_i,
@@ -920,7 +929,7 @@ static int run_newmark(run_t *R)
                                                   R->deffarea_ip, R->slen_w, R->defslen_ip,
                                                   m->yield, R->c1_ip, R->c2_ip, R->c3_ip,
                                                   R->c1_i, R->c2_i, R->c3_i, m->minc, m->mcode,
-                                                  m->jcode);
+                                                  m->jcode, R->d_temp);
                     if (R->out) R->out->t_forces += now_s() - t0;
                     frcchk = frc_fr || frc_sh;
                     for (long i = 0; i < n; ++i) R->dyn[i] = R->qtot[i] - R->sm[i] * R->ac_i[i];
@@ -994,6 +1003,128 @@ static int run_newmark(run_t *R)
     return 0;
 }
 
+/* ---- linear static analysis: ANAFLAG 1 under ALGFLAG 1/2 (main.c:1714-1798) --
+ * One tangent at the input geometry, K d = lpfmax q, and the linear element
+ * forces of d; one path row (lpfmax, 0, d).  The reference prints its
+ * iteration counter there without setting it on this path (main.c:1670). */
+static int run_linear_static(run_t *R)
+{
+    orc_model *m = R->m;
+    const orc_dims *d = &m->dm;
+    long n = R->neq;
+    for (long i = 0; i < n; ++i) R->qtot[i] = m->q[i] * m->lpfmax;
+    assemble(R, R->x, R->c1, R->c2, R->c3, R->ef, R->defllen, R->deffarea, R->defslen, R->efFE);
+    if (m->lss == 1) {
+        R->d[0] = R->qtot[0] / R->ss[0];
+    } else {
+        int det = 0;
+        (void)factor(R, &det);             /* solve() ignores skyfact's status (solve.c:78-89) */
+        backsolve(R, R->qtot, R->d);
+    }
+    memset(R->f, 0, sizeof(double) * (size_t)n);
+    if (d->NE_TR > 0)
+        R->be->forces_tr(R->be->ctx, d, R->f, R->ef, m->emod, m->carea, R->llen_ref, R->defllen,
+                         m->yield, R->c1, R->c2, R->c3, m->mcode, R->d);
+    if (d->NE_FR > 0) {
+        double dl = 0;
+        R->be->forces_fr(R->be->ctx, d, R->f, R->ef, R->ef, R->efFE, R->efFE, R->d, m, R->llen_ref,
+                         R->defllen, R->c1, R->c2, R->c3, R->c1, R->c2, R->c3, &dl, 0);
+    }
+    if (d->NE_SH > 0)
+        R->be->forces_sh(R->be->ctx, d, R->f, R->ef, R->ef, R->d, R->x, R->x, m->emod, m->nu,
+                         m->xlocal, m->thick, R->farea_w, R->deffarea, R->slen_w, R->defslen,
+                         m->yield, R->c1, R->c2, R->c3, R->c1, R->c2, R->c3, m->minc, m->mcode,
+                         m->jcode, R->d);
+    record(R, m->lpfmax, 0, R->d, R->f);
+    return 0;
+}
+
+/* ---- linear Newmark: ALGFLAG 4 (main.c:3143-3303, solve.c:139-245, 246-468) --
+ * K and the lumped M at the input geometry, Keff = K + a0 (1-am)/(1-af) M on
+ * the diagonal factored once, then per time step
+ *   Reff = P_k [+ af/(1-af) P_{k-1}] + M ((1-am)(a0 u + a2 v + a3 a) - am a)/(1-af)
+ *          [- af/(1-af) K u  when af != 0]
+ *   Keff u_new = Reff,  a_new = a0 (u_new - u) - a2 v - a3 a,  v_new = v + a6 a + a7 a_new
+ * starting from the input's initial conditions.  Each step's row is
+ * (k dt, 0, u_new); after the loop the reference computes the forces of d,
+ * which this driver never sets, and writes (lpfmax, counter, 0) as the last
+ * row.  The checkpoint (CHKPT) holds u, v, a; RFLAG 1 starts from it. */
+static int run_linear_newmark(run_t *R)
+{
+    orc_model *m = R->m;
+    const orc_dims *d = &m->dm;
+    long n = R->neq;
+    double am_ = 0, af_ = 0, rho = m->spectrds;
+    switch (m->numopt) {
+    case 0:
+        if (rho != 1) FAIL(1, "invalid spectral radius for Newmark without dissipation");
+        break;
+    case 1: am_ = (2 * rho - 1) / (rho + 1); af_ = rho / (rho + 1); break;
+    case 2: af_ = (1 - rho) / (1 + rho); break;
+    case 3: am_ = (rho - 1) / (rho + 1); break;
+    default: FAIL(1, "unknown Newmark dissipation option");
+    }
+    double alpha = pow(1 - am_ + af_, 2) / 4, delta = 0.5 - am_ + af_;
+    double dt = m->dt;
+    long ntst = (long)(m->ttot / dt + 1);
+    long ns_in = m->ntstpsinpt;
+    if (ntst > ns_in) FAIL(1, "time steps exceed the load history");
+    const double *P = m->pinpt;
+    assemble(R, R->x, R->c1, R->c2, R->c3, R->ef, R->defllen, R->deffarea, R->defslen, R->efFE);
+    memset(R->sm, 0, sizeof(double) * (size_t)n);
+    if (d->NE_TR > 0) R->be->mass_tr(R->be->ctx, d, R->sm, m, R->x, R->llen_ref, m->dens);
+    if (d->NE_FR > 0) R->be->mass_fr(R->be->ctx, d, R->sm, m, R->x, R->llen_ref, R->xfr, m->dens);
+    if (d->NE_SH > 0) R->be->mass_sh(R->be->ctx, d, R->sm, m, R->x, R->farea_w, R->slen_w, m->dens);
+    double a0 = 1 / (alpha * pow(dt, 2)), a2 = 1 / (alpha * dt), a3 = 1 / (2 * alpha) - 1;
+    double a6 = dt * (1 - delta), a7 = delta * dt;
+    CP(R->keff, R->ss, m->lss);
+    for (long i = 0; i < n; ++i) R->keff[m->maxa[i] - 1] += a0 * (1 - am_) * R->sm[i] / (1 - af_);
+    int det = 0;
+    double t1 = now_s();
+    orc_skyfact(n, m->maxa, R->keff, R->ssd, 0, &det);
+    if (R->out) { R->out->t_factor += now_s() - t1; R->out->n_factor++; }
+    CP(R->um, m->um, n); CP(R->vm, m->vm, n); CP(R->am, m->am, n);
+    const int ckpt = R->out && R->out->ckpt_path;
+    long k = 0;
+    if (m->rflag == 1) {
+        if (!ckpt) FAIL(1, "restart (RFLAG 1) without a checkpoint file");
+        long tstp = 0;
+        if (orc_ckpt_read_lin(R->out->ckpt_path, n, &tstp, R->um, R->vm, R->am))
+            FAIL(1, "unable to read the checkpoint file");
+        if (tstp + 1 == ntst) record(R, (double)tstp * dt, 0, R->um, NULL);
+        k = tstp + 1;
+    }
+    double *Kum = R->dyn;
+    for (; k < ntst; ++k) {
+        for (long i = 0; i < n; ++i) {
+            double meff = R->sm[i] * ((1 - am_) * (R->um[i] * a0 + R->vm[i] * a2 + R->am[i] * a3) -
+                                      am_ * R->am[i]) / (1 - af_);
+            R->r[i] = (k == 0) ? P[i * ns_in + k] + meff
+                               : P[i * ns_in + k] + af_ / (1 - af_) * P[i * ns_in + k - 1] + meff;
+        }
+        if (af_ != 0) {
+            orc_skymult(n, m->maxa, R->ss, R->um, Kum);
+            for (long i = 0; i < n; ++i) R->r[i] -= af_ / (1 - af_) * Kum[i];
+        }
+        t1 = now_s();
+        orc_skysolve(n, m->maxa, R->keff, R->r);
+        if (R->out) { R->out->t_solve += now_s() - t1; R->out->n_solve++; }
+        for (long i = 0; i < n; ++i) {
+            R->uc[i] = R->r[i];
+            R->ac[i] = a0 * (R->uc[i] - R->um[i]) - a2 * R->vm[i] - a3 * R->am[i];
+            R->vc[i] = R->vm[i] + a6 * R->am[i] + a7 * R->ac[i];
+        }
+        record(R, (double)k * dt, 0, R->uc, NULL);
+        CP(R->um, R->uc, n); CP(R->vm, R->vc, n); CP(R->am, R->ac, n);
+        if (ckpt && k % m->chkpt == 0 && k != 0 &&
+            orc_ckpt_write_lin(R->out->ckpt_path, n, k, R->um, R->vm, R->am, R->out->ckpt_exact))
+            FAIL(1, "unable to write the checkpoint file");
+        if (R->out && R->out->nsteps >= R->out->max_steps) return 0;
+    }
+    record(R, m->lpfmax, 0, R->d, R->f);
+    return 0;
+}
+
 int orc_run(orc_model *m, const orc_backend *b, orc_path *out)
 {
     orc_backend own;
@@ -1003,8 +1134,9 @@ int orc_run(orc_model *m, const orc_backend *b, orc_path *out)
     if (out) { out->nsteps = 0; out->status = 0; out->msg[0] = 0; }
     int rc;
     switch (m->dm.ALGFLAG) {
-    case 1: case 2: rc = run_nr(&R); break;
+    case 1: case 2: rc = m->dm.ANAFLAG == 1 ? run_linear_static(&R) : run_nr(&R); break;
     case 3: rc = run_msal(&R); break;
+    case 4: rc = run_linear_newmark(&R); break;
     case 5: rc = run_newmark(&R); break;
     default: rc = 2; if (out) { out->status = 2; snprintf(out->msg, sizeof out->msg, "unsupported ALGFLAG"); }
     }

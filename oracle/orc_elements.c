@@ -498,3 +498,73 @@ void orc_updatc(const orc_dims *d, double *x_temp, double *x_ip, double *xfr_tem
         }
     }
 }
+
+/* ---- linear elastic forces (ANAFLAG 1) -------------------------------------- */
+
+/* truss.c:243-326: d = T D in the element's triad, ef = EA/L0 [1 -1; -1 1] d,
+ * f -= ef c at each end. */
+void orc_forces_tr_lin(const orc_dims *d, double *f, double *ef_i, const double *D,
+                       const double *emod, const double *carea, const double *llength,
+                       const double *c1, const double *c2, const double *c3, const long *mcode)
+{
+    for (long n = 0; n < d->NE_TR; ++n) {
+        double De[6];
+        for (int i = 0; i < 6; ++i) {
+            long j = mcode[n * 6 + i];
+            De[i] = j ? D[j - 1] : 0.0;
+        }
+        double c[3] = {c1[n], c2[n], c3[n]};
+        double dl[2];
+        for (int i = 0; i < 2; ++i) {
+            double s = 0;
+            for (int j = 0; j < 6; ++j) s += (j / 3 == i ? c[j % 3] : 0.0) * De[j];
+            dl[i] = s;
+        }
+        double ka = carea[n] * emod[n] / llength[n];
+        ef_i[n * 2] = ka * dl[0] + (-ka) * dl[1];
+        ef_i[n * 2 + 1] = (-ka) * dl[0] + ka * dl[1];
+        for (int j = 0; j < 6; ++j) {
+            long k = mcode[n * 6 + j];
+            if (k) f[k - 1] -= ef_i[n * 2 + j / 3] * c[j % 3];
+        }
+    }
+}
+
+/* shell.c:1695-1727: ef = k_sh (T D), the elastic matrix of stiffe_sh with its
+ * drilling term, then f += T^T ef (shell.c:2385-2397). */
+void orc_forces_sh_lin(const orc_dims *d, double *f, double *ef_i, const double *D,
+                       const double *emod, const double *nu, const double *xlocal,
+                       const double *thick, const double *farea, const double *slength,
+                       const double *c1, const double *c2, const double *c3, const long *mcode)
+{
+    long poff = d->NE_TR + d->NE_FR, toff = d->NE_TR + d->NE_FR * 3;
+    long moff = d->NE_TR * 6 + d->NE_FR * 14, eoff = d->NE_TR * 2 + d->NE_FR * 14;
+    static const double dm0[6] = {0, 0, 0, 0, 0, 0};
+    for (long n = 0; n < d->NE_SH; ++n) {
+        double k[18][18];
+        shell_local_tangent(emod[poff + n], nu[n], thick[n], xlocal + n * 3, farea[n],
+                            slength + n * 3, farea[n], dm0, 0, k);
+        const double *R[3] = {c1 + toff + n * 3, c2 + toff + n * 3, c3 + toff + n * 3};
+        double De[18], dl[18];
+        for (int i = 0; i < 18; ++i) {
+            long j = mcode[moff + n * 18 + i];
+            De[i] = j ? D[j - 1] : 0.0;
+        }
+        for (int i = 0; i < 18; ++i) {          /* T rows: c1, c2, c3 per node block */
+            int b = i / 3 * 3, r = i % 3;
+            dl[i] = R[r][0] * De[b] + R[r][1] * De[b + 1] + R[r][2] * De[b + 2];
+        }
+        double *ef = ef_i + eoff + n * 18;
+        for (int i = 0; i < 18; ++i) {
+            double s = 0;
+            for (int j = 0; j < 18; ++j) s += k[i][j] * dl[j];
+            ef[i] = s;
+        }
+        for (int i = 0; i < 18; ++i) {
+            int b = i / 3 * 3, c = i % 3;
+            double s = R[0][c] * ef[b] + R[1][c] * ef[b + 1] + R[2][c] * ef[b + 2];
+            long j = mcode[moff + n * 18 + i];
+            if (j) f[j - 1] += s;
+        }
+    }
+}

@@ -54,6 +54,15 @@ static long tk_long(tokrd *t)
     return v;
 }
 
+/* whether a comma follows the last value, past white space (the literal ","
+   of a scanf format continuing the line) */
+static int tk_comma_next(const tokrd *t)
+{
+    size_t p = t->pos;
+    while (p < t->len && isspace((unsigned char)t->buf[p])) p++;
+    return p < t->len && t->buf[p] == ',';
+}
+
 static void *zalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
 
 void orc_free(orc_model *m)
@@ -340,8 +349,9 @@ orc_model *orc_open(const char *path, char *err, int errlen)
     if (nsbr || nfbr) FAIL("brick elements are not on this path");
     if (d->OPTFLAG != 1 && d->OPTFLAG != 2) FAIL("OPTFLAG must be 1 or 2");
     if (d->ANAFLAG != 2 && d->ANAFLAG != 1) FAIL("ANAFLAG %d not supported (elastic only)", d->ANAFLAG);
-    if (d->ALGFLAG != 1 && d->ALGFLAG != 2 && d->ALGFLAG != 3 && d->ALGFLAG != 5)
-        FAIL("ALGFLAG %d not supported", d->ALGFLAG);
+    if (d->ALGFLAG < 1 || d->ALGFLAG > 5) FAIL("ALGFLAG %d not supported", d->ALGFLAG);
+    if (d->ANAFLAG == 1 && (d->ALGFLAG == 3 || d->ALGFLAG == 5))
+        FAIL("linear analysis (ANAFLAG 1) runs under ALGFLAG 1, 2 (static) or 4 (linear Newmark)");
     if (d->SLVFLAG != 0) FAIL("only SLVFLAG 0 (skyline) is restated");
     long nj = d->NJ, ntr = d->NE_TR, nfr = d->NE_FR, nsh = d->NE_SH;
 
@@ -365,7 +375,8 @@ orc_model *orc_open(const char *path, char *err, int errlen)
         if (t.bad) FAIL("truncated constraints");
         if (j == 0) break;
         if (k < 1 || k > 7 || j > nj) FAIL("joint constraint input not recognized");
-        if (k == 7) m->wrpres[(j - 1) * 3] = (int)tk_long(&t);
+        /* model.c:251-254: a bare "j,7" line leaves the restraint 0 */
+        if (k == 7 && tk_comma_next(&t)) m->wrpres[(j - 1) * 3] = (int)tk_long(&t);
         m->jcode[(j - 1) * 7 + (k - 1)] = 0;
     }
     autofix(m);
@@ -594,7 +605,12 @@ orc_model *orc_open(const char *path, char *err, int errlen)
     }
 
     /* driver parameters */
-    if (d->ALGFLAG == 1 || d->ALGFLAG == 2) {
+    if ((d->ALGFLAG == 1 || d->ALGFLAG == 2) && d->ANAFLAG == 1) {
+        m->lpfmax = tk_dbl(&t);                            /* main.c:1717 */
+    } else if (d->ALGFLAG == 4) {
+        m->numopt = (int)tk_dbl(&t); m->spectrds = tk_dbl(&t);   /* main.c:3184-3197 */
+        m->lpfmax = tk_dbl(&t);
+    } else if (d->ALGFLAG == 1 || d->ALGFLAG == 2) {
         m->lpfmax = tk_dbl(&t); m->lpf0 = tk_dbl(&t); m->dlpf0 = tk_dbl(&t);
         m->dlpfmax = tk_dbl(&t); m->dlpfmin = tk_dbl(&t);
         m->itemax = (int)tk_long(&t); m->submax = (int)tk_long(&t); m->solmin = (int)tk_long(&t);

@@ -26,12 +26,16 @@ void orc_reactions(const orc_model *m, const double *ef, const double *c1, const
     const orc_dims *d = &m->dm;
     const long ntr = d->NE_TR, nfr = d->NE_FR, nsh = d->NE_SH;
     memset(r, 0, sizeof(double) * (size_t)(d->NJ * 7));
-    /* truss: f -= ef[end] * c (truss.c:349-372) */
+    /* truss: f -= ef[end] * c (truss.c:349-372).  The linear branch
+     * (ANAFLAG 1, truss.c:300-305) gives end forces of the opposite sign,
+     * EA/L0 (d1 - d2) at end 1, so its f is -K d: the reactions take its end
+     * forces with the sign flipped back. */
+    const double s_tr = d->ANAFLAG == 1 ? -1.0 : 1.0;
     for (long e = 0; e < ntr; ++e) {
         const double c[3] = {c1[e], c2[e], c3[e]};
         for (int a = 0; a < 2; ++a) {
             long j = m->minc[e * 2 + a] - 1;
-            for (int k = 0; k < 3; ++k) r[j * 7 + k] -= ef[e * 2 + a] * c[k];
+            for (int k = 0; k < 3; ++k) r[j * 7 + k] -= s_tr * ef[e * 2 + a] * c[k];
         }
     }
     /* frame: per end a force triple, a moment triple and the warping

@@ -211,3 +211,27 @@ int orc_quad(long neq, double a, double b, double c, double *dt, const double *d
     *lpft += *dlpf;
     return 0;
 }
+
+/* skymult (solve.c:700-760): the upper triangle, the diagonal and the lower
+ * triangle of the profile applied to v in turn, column by column. */
+void orc_skymult(long neq, const long *maxa, const double *ss, const double *v, double *out)
+{
+    for (long i = 0; i < neq; ++i) out[i] = 0;
+    for (long n = neq; n >= 2; --n) {            /* rows above the diagonal of column n */
+        long k = n;
+        for (long a = maxa[n - 1] + 1; a <= maxa[n] - 1; ++a) {
+            k--;
+            out[k - 1] += ss[a - 1] * v[n - 1];
+        }
+    }
+    for (long n = 0; n < neq; ++n) out[n] += v[n] * ss[maxa[n] - 1];
+    for (long n = neq; n >= 1; --n) {            /* the same entries as rows of the lower part */
+        long k = n;
+        double c = 0;
+        for (long a = maxa[n - 1] + 1; a <= maxa[n] - 1; ++a) {
+            k--;
+            c += ss[a - 1] * v[k - 1];
+        }
+        out[n - 1] += c;
+    }
+}

@@ -71,12 +71,13 @@ static void w_stiff_tr(void *ctx, const orc_dims *d, double *ss, const double *e
 static void w_forces_tr(void *ctx, const orc_dims *d, double *f, double *ef_i,
                         const double *emod, const double *carea, const double *llength,
                         const double *defllen_i, const double *yield, const double *c1_i,
-                        const double *c2_i, const double *c3_i, const long *mcode)
+                        const double *c2_i, const double *c3_i, const long *mcode,
+                        const double *d_tot)
 {
     (void)ctx;
     ref_set_dims(d);
-    forces_tr(f, ef_i, zeros(d->NEQ + 1), U(emod), U(carea), U(llength), U(defllen_i), U(yield),
-              U(c1_i), U(c2_i), U(c3_i), UL(mcode));
+    forces_tr(f, ef_i, d_tot ? U(d_tot) : zeros(d->NEQ + 1), U(emod), U(carea), U(llength),
+              U(defllen_i), U(yield), U(c1_i), U(c2_i), U(c3_i), UL(mcode));
 }
 
 static void w_stiff_sh(void *ctx, const orc_dims *d, double *ss, const double *emod,
@@ -102,12 +103,14 @@ static int w_forces_sh(void *ctx, const orc_dims *d, double *f, double *ef_ip, d
                        const double *slength, const double *defslen_ip, const double *yield,
                        const double *c1_ip, const double *c2_ip, const double *c3_ip,
                        const double *c1_i, const double *c2_i, const double *c3_i,
-                       const long *minc, const long *mcode, const long *jcode)
+                       const long *minc, const long *mcode, const long *jcode,
+                       const double *d_tot)
 {
     (void)ctx;
     ref_set_dims(d);
     double *z = zeros(d->NE_SH * 9 + d->NEQ + 1);
-    return forces_sh(f, ef_ip, ef_i, z, z, U(dd), z, z, U(x_temp), U(x_ip), U(emod), U(nu),
+    return forces_sh(f, ef_ip, ef_i, z, z, U(dd), d_tot ? U(d_tot) : z, z, U(x_temp), U(x_ip),
+                     U(emod), U(nu),
                      U(xlocal), U(thick), U(farea), U(deffarea_ip), U(slength), U(defslen_ip),
                      U(yield), U(c1_ip), U(c2_ip), U(c3_ip), U(c1_i), U(c2_i), U(c3_i), UL(minc),
                      UL(mcode), UL(jcode));
@@ -231,7 +234,7 @@ int ref_forces_sh(const orc_dims *d, double *f, double *ef_ip, double *ef_i, con
     double *xip = calloc((size_t)(d->NJ * 3 + 1), sizeof(double));
     int r = w_forces_sh(NULL, d, f, ef_ip, ef_i, dd, x_temp, xip, emod, nu, xlocal, thick, farea,
                         a, slength, s3, y, c1_ip, c2_ip, c3_ip, c1_i, c2_i, c3_i, minc, mcode,
-                        jcode);
+                        jcode, NULL);
     free(y); free(s3); free(a); free(xip);
     return r;
 }
@@ -253,7 +256,7 @@ void ref_forces_tr(const orc_dims *d, double *f, double *ef_i, const double *emo
                    const long *mcode)
 {
     double *y = calloc((size_t)(d->NE_TR + d->NE_FR + d->NE_SH + 1), sizeof(double));
-    w_forces_tr(NULL, d, f, ef_i, emod, carea, llength, defllen_i, y, c1_i, c2_i, c3_i, mcode);
+    w_forces_tr(NULL, d, f, ef_i, emod, carea, llength, defllen_i, y, c1_i, c2_i, c3_i, mcode, NULL);
     free(y);
 }
 
