@@ -126,9 +126,13 @@ struct BandBatch {
     void factor(const Dev& d);
     // The same in two parts, for a caller that writes the band itself:
     // zero() clears the tiles and the error bits; factor_filled() adds the
-    // identity rows of the unused equations and factors.
+    // identity rows of the unused equations and factors, and copies the error
+    // bits to flag_host unless fetch_flag is false (a caller that captures
+    // the launches in a graph then calls fetch_flag() after it).  All device
+    // work here is kernels: no memset node enters a captured graph.
     void zero();
-    void factor_filled();
+    void factor_filled(bool fetch_flag = true);
+    void fetch_flag();
     // out = (L D L^T)^{-1} rhs for every model (device vectors of n, no alias).
     void solve(const double* rhs, double* out);
     size_t bytes() const { return sizeof(double) * size_t(nt) * size_t(w + 1) * 64 * 64; }

@@ -1167,6 +1167,15 @@ __global__ void k_sweep_prep(double* __restrict__ dst, const double* __restrict_
 // ---- batches of models ---------------------------------------------------------
 // Identity rows for the equations no DOF slot maps to (the tail of each
 // model's last tile).
+// Fills by kernel rather than hipMemsetAsync, so that a captured HIP graph of a
+// batch round holds no memset node.
+__global__ void k_fill_f64(double* __restrict__ y, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = 0.0;
+}
+__global__ void k_fill_i32(int* __restrict__ y, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = 0;
+}
+
 __global__ void k_band_gap_pad(double* __restrict__ tiles, const int32_t* __restrict__ gaps, long ngap, int w) {
     long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (k >= ngap) return;
@@ -1832,8 +1841,10 @@ void BandBatch::release() {
 }
 
 void BandBatch::zero() {
-    CB_CHECK(hipMemsetAsync(tiles, 0, sizeof(double) * size_t(nt) * (w + 1) * TT, s));
-    CB_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
+    const long nv = nt * long(w + 1) * TT;
+    hipLaunchKernelGGL(k_fill_f64, dim3(int(std::min<long>(8192, (nv + 255) / 256))), dim3(256), 0, s, tiles, nv);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, flag, 1L);
+    CB_CHECK(hipGetLastError());
 }
 
 void BandBatch::factor(const Dev& d) {
@@ -1844,17 +1855,22 @@ void BandBatch::factor(const Dev& d) {
     factor_filled();
 }
 
-void BandBatch::factor_filled() {
+void BandBatch::factor_filled(bool fetch_flag) {
     if (!nseg) return;
     if (ngap)
         hipLaunchKernelGGL(k_band_gap_pad, dim3(int((ngap + 255) / 256)), dim3(256), 0, s, tiles, gaps, ngap, w);
-    CB_CHECK(hipMemsetAsync(prog, 0, sizeof(int) * size_t(nseg) * 4 * FLAG_STRIDE, s));
+    const long np = long(nseg) * 4 * FLAG_STRIDE;
+    hipLaunchKernelGGL(k_fill_i32, dim3(int(std::min<long>(1024, (np + 255) / 256))), dim3(256), 0, s, prog, np);
     const long spin_limit = 1L << 22;
     if (timing) CB_CHECK(hipEventRecord(ev[0], s));
     hipLaunchKernelGGL(k_band_factor_models, dim3(nwg), dim3(NTH), 0, s, tiles, linv, piv, w, segs, order, role, prog,
                        spin_limit, flag);
     if (timing) CB_CHECK(hipEventRecord(ev[1], s));
     CB_CHECK(hipGetLastError());
+    if (fetch_flag) this->fetch_flag();
+}
+
+void BandBatch::fetch_flag() {
     CB_CHECK(hipMemcpyAsync(flag_host, flag, sizeof(int), hipMemcpyDeviceToHost, s));
 }
 

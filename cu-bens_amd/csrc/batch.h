@@ -70,6 +70,11 @@ struct BatchStats {
     // launch (flops, T^3 + 2 T^3 m + T^3 m (m+1) per tile column of every model)
     double ms_factor = 0, ms_solve = 0;
     double factor_flops = 0, solve_bytes = 0;
+    // stream operations (kernel launches, copies, graph launches) the run
+    // issued per round on average, and the operations of one round's Newton
+    // iteration (the nodes of its graph)
+    double ops_per_round = 0;
+    long body_ops = 0;
 };
 
 class Batch {
@@ -81,8 +86,10 @@ public:
     int count() const { return int(models_.size()); }
     const Model& model(int b) const { return models_[b]; }
     const Model& all() const { return all_; }
-    // Runs every model to its end (or max_steps recorded steps each).
-    void run(int max_steps, bool keep_path, BatchStats& st, bool timing = false);
+    // Runs every model to its end (or max_steps recorded steps each).  With
+    // graph (and no timing), each round's Newton iteration replays one HIP
+    // graph captured from the first round's launches.
+    void run(int max_steps, bool keep_path, BatchStats& st, bool timing = false, bool graph = false);
     // after run(): per model status (0 ran to its end or to max_steps, 1 zero
     // norm in the convergence test, 3 minimum time increment reached), time and
     // Newton iterations of each recorded step, and the displacement rows
@@ -115,7 +122,10 @@ private:
     double *amv = nullptr, *dyn = nullptr, *hist = nullptr, *pathv = nullptr;
     size_t path_len_ = 0;
     bool kept_path_ = false;
+    hipGraphExec_t gexec_ = nullptr;   // the captured Newton iteration
+    long body_ops_ = 0;                // its operations (graph nodes)
     void build_jobs();
+    void iteration(NewmarkArgs& A);    // one Newton iteration of every model (launches only)
 };
 
 }  // namespace cb

@@ -347,6 +347,7 @@ void Batch::build_jobs() {
 
 Batch::~Batch() {
     try {
+        if (gexec_) (void)hipGraphExecDestroy(gexec_);
         band_.release();
         void* ps[] = {own_eq_, own_joint_, own_el_, segs_dev_, prm_dev_, sums_dev_, commit_jobs_, start_jobs_,
                       dv, dt, dd, f, fp, ft, fip, r, tmp, sm, qtot, uc, vc, ac, vci, aci, um, vm, amv, dyn, hist,
@@ -360,7 +361,33 @@ Batch::~Batch() {
     }
 }
 
-void Batch::run(int max_steps, bool keep_path, BatchStats& st, bool timing) {
+// One Newton iteration of every model (Analysis::run_newmark's L4 body): a
+// fixed sequence of launches whose per-round values (each model's time step,
+// predictor or corrector, Keff coefficient) live in prm_dev_, so that it can
+// be captured once and replayed.
+void Batch::iteration(NewmarkArgs& A) {
+    batch_nm_vec(d_.s, NM_MODEL_OP, A, prm_dev_, own_eq_, n_);   // predictor or corrector
+    // the tangent goes straight into the band (the copy the single-model
+    // solver makes from the joint blocks), then Keff's mass term
+    band_.zero();
+    launch_frame_tangent_band(d_, G_IP, band_.tiles, band_.w, band_.flag);
+    vfill(d_, sm, 0.0, n_);
+    launch_mass(d_, sm, 1);
+    batch_add_mass_band(d_, band_.tiles, band_.w, sm, prm_dev_, own_joint_);
+    band_.factor_filled(false);
+    A.out = tmp;
+    batch_nm_vec(d_.s, NM_REFF, A, prm_dev_, own_eq_, n_);
+    band_.solve(tmp, dd);
+    batch_nm_vec(d_.s, NM_UPDATE, A, prm_dev_, own_eq_, n_);
+    launch_updatc(d_, G_I, dd);
+    launch_forces(d_, G_I, ft, dd);
+    A.out = dyn;
+    batch_nm_vec(d_.s, NM_DYN, A, prm_dev_, own_eq_, n_);
+    batch_test_sums(d_.s, segs_dev_, int(models_.size()), dd, dt, dyn, ft, fp, fip, sums_dev_);
+    copy_gen(d_, G_IP, G_I, true, false);
+}
+
+void Batch::run(int max_steps, bool keep_path, BatchStats& st, bool timing, bool graph) {
     const size_t nb = models_.size();
     const double t0 = now_s();
     band_.timing = timing;
@@ -417,6 +444,24 @@ void Batch::run(int max_steps, bool keep_path, BatchStats& st, bool timing) {
     A.ac = aci; A.vc = vci; A.um = um; A.vm = vm; A.amv = amv;
     std::vector<NmParam> prm(nb);
     long rounds = 0;
+    long ops = 0;
+    if (!body_ops_ || (graph && !gexec_)) {
+        // capture the iteration (without the timing events) once: its node
+        // count, and the executable graph when it is to be replayed
+        const bool keep_timing = band_.timing;
+        band_.timing = false;
+        hipGraph_t g = nullptr;
+        CB_CHECK(hipStreamBeginCapture(d_.s, hipStreamCaptureModeThreadLocal));
+        iteration(A);
+        CB_CHECK(hipStreamEndCapture(d_.s, &g));
+        size_t nn = 0;
+        CB_CHECK(hipGraphGetNodes(g, nullptr, &nn));
+        body_ops_ = long(nn);
+        if (graph && !gexec_) CB_CHECK(hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0));
+        CB_CHECK(hipGraphDestroy(g));
+        band_.timing = keep_timing;
+    }
+    const bool replay = graph && !timing && gexec_;
     for (;;) {
         // every model runs its machine to its next Newton iteration
         bool any = false, any_commit = false, any_rec = false, any_start = false;
@@ -438,32 +483,31 @@ void Batch::run(int max_steps, bool keep_path, BatchStats& st, bool timing) {
         if (!any && !any_commit && !any_rec) break;
         std::memcpy(prm_host_, prm.data(), sizeof(NmParam) * nb);
         h2d(prm_dev_, prm_host_, nb, d_.s);
+        ++ops;
         const long nmax = std::max<long>(n_, all_.nfr * 14);
-        if (any_commit) batch_masked_copy(d_.s, commit_jobs_, ncommit_, nmax, own_eq_, own_joint_, own_el_, prm_dev_, 0);
-        if (any_rec) batch_record(d_.s, dv, pathv, segs_dev_, prm_dev_, own_eq_, n_);
+        if (any_commit) {
+            batch_masked_copy(d_.s, commit_jobs_, ncommit_, nmax, own_eq_, own_joint_, own_el_, prm_dev_, 0);
+            ++ops;
+        }
+        if (any_rec) {
+            batch_record(d_.s, dv, pathv, segs_dev_, prm_dev_, own_eq_, n_);
+            ++ops;
+        }
         if (!any) break;
-        if (any_start) batch_masked_copy(d_.s, start_jobs_, nstart_, nmax, own_eq_, own_joint_, own_el_, prm_dev_, 1);
-        // one Newton iteration of every model (Analysis::run_newmark's L4 body)
-        batch_nm_vec(d_.s, NM_MODEL_OP, A, prm_dev_, own_eq_, n_);   // predictor or corrector
-        // the tangent goes straight into the band (the copy the single-model
-        // solver makes from the joint blocks), then Keff's mass term
-        band_.zero();
-        launch_frame_tangent_band(d_, G_IP, band_.tiles, band_.w, band_.flag);
-        vfill(d_, sm, 0.0, n_);
-        launch_mass(d_, sm, 1);
-        batch_add_mass_band(d_, band_.tiles, band_.w, sm, prm_dev_, own_joint_);
-        band_.factor_filled();
-        A.out = tmp;
-        batch_nm_vec(d_.s, NM_REFF, A, prm_dev_, own_eq_, n_);
-        band_.solve(tmp, dd);
-        batch_nm_vec(d_.s, NM_UPDATE, A, prm_dev_, own_eq_, n_);
-        launch_updatc(d_, G_I, dd);
-        launch_forces(d_, G_I, ft, dd);
-        A.out = dyn;
-        batch_nm_vec(d_.s, NM_DYN, A, prm_dev_, own_eq_, n_);
-        batch_test_sums(d_.s, segs_dev_, int(nb), dd, dt, dyn, ft, fp, fip, sums_dev_);
-        copy_gen(d_, G_IP, G_I, true, false);
+        if (any_start) {
+            batch_masked_copy(d_.s, start_jobs_, nstart_, nmax, own_eq_, own_joint_, own_el_, prm_dev_, 1);
+            ++ops;
+        }
+        if (replay) {
+            CB_CHECK(hipGraphLaunch(gexec_, d_.s));
+            ++ops;
+        } else {
+            iteration(A);
+            ops += body_ops_;
+        }
+        band_.fetch_flag();
         d2h(sums_host_, sums_dev_, nb * 6, d_.s);
+        ops += 2;
         CB_CHECK(hipStreamSynchronize(d_.s));
         if (*band_.flag_host & 1) throw DeviceError("batched band factorization: a wait exceeded its bound");
         if (*band_.flag_host & 2) throw DeviceError("batched band factorization: an entry outside a model's band");
@@ -479,6 +523,8 @@ void Batch::run(int max_steps, bool keep_path, BatchStats& st, bool timing) {
     CB_CHECK(hipStreamSynchronize(d_.s));
     st.t_total = now_s() - t0;
     st.rounds = rounds;
+    st.ops_per_round = rounds ? double(ops) / double(rounds) : 0.0;
+    st.body_ops = body_ops_;
     st.ms_factor = ms_factor;
     st.ms_solve = ms_solve;
     // algorithmic work of one factor / solve launch over all models

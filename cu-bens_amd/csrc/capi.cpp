@@ -90,6 +90,7 @@ cb_options default_options() {
     o.keep_forces = 0;
     o.checkpoint_path = nullptr;
     o.checkpoint_exact = 0;
+    o.graph = 0;
     return o;
 }
 
@@ -117,7 +118,7 @@ bool same_opts(const cb_options& a, const cb_options& b) {
            a.maxit == b.maxit && a.judge == b.judge && a.max_steps == b.max_steps && a.keep_d == b.keep_d &&
            a.kernel_timing == b.kernel_timing && a.factor_schedule == b.factor_schedule &&
            a.kernel_timing_every == b.kernel_timing_every && a.sweep_schedule == b.sweep_schedule &&
-           a.keep_forces == b.keep_forces && a.checkpoint_exact == b.checkpoint_exact &&
+           a.keep_forces == b.keep_forces && a.checkpoint_exact == b.checkpoint_exact && a.graph == b.graph &&
            std::string(a.checkpoint_path ? a.checkpoint_path : "") ==
                std::string(b.checkpoint_path ? b.checkpoint_path : "");
 }
@@ -857,8 +858,10 @@ int cb_batch_run(cb_batch* b, const cb_options* opt, cb_batch_stats* st) {
     if (o.solver != CB_SOLVER_BAND64) return fail(-1, "a batch runs the FP64 band solver only");
     GUARD_BEGIN
     cb::BatchStats bs;
-    b->bt->run(o.max_steps, o.keep_d != 0, bs, o.kernel_timing != 0);
+    b->bt->run(o.max_steps, o.keep_d != 0, bs, o.kernel_timing != 0, o.graph != 0);
     if (st) {
+        st->ops_per_round = bs.ops_per_round;
+        st->body_ops = bs.body_ops;
         st->t_total = bs.t_total;
         st->rounds = bs.rounds;
         st->newton = bs.newton;

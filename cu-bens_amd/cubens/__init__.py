@@ -34,7 +34,7 @@ class Options(C.Structure):
                 ("maxit", C.c_int), ("judge", C.c_int), ("max_steps", C.c_int),
                 ("keep_d", C.c_int), ("kernel_timing", C.c_int), ("factor_schedule", C.c_int),
                 ("kernel_timing_every", C.c_int), ("sweep_schedule", C.c_int), ("keep_forces", C.c_int),
-                ("checkpoint_path", C.c_char_p), ("checkpoint_exact", C.c_int)]
+                ("checkpoint_path", C.c_char_p), ("checkpoint_exact", C.c_int), ("graph", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -64,12 +64,12 @@ def options(solver: int = SOLVER_BAND64, krylov_tol: float = 1e-8, restart: int 
             maxit: int = 20000, judge: int = 1, max_steps: int = 1 << 30, keep_d: int = 1,
             kernel_timing: int = 0, factor_schedule: int = 0, kernel_timing_every: int = 1,
             sweep_schedule: int = 0, keep_forces: int = 0, checkpoint_path: str | None = None,
-            checkpoint_exact: int = 0) -> Options:
+            checkpoint_exact: int = 0, graph: int = 0) -> Options:
     """cb_options; checkpoint_path is the Newmark checkpoint file (written every
     CHKPT time steps, read first when the model has RFLAG 1)."""
     return Options(solver, krylov_tol, restart, maxit, judge, max_steps, keep_d, kernel_timing,
                    factor_schedule, kernel_timing_every, sweep_schedule, keep_forces,
-                   checkpoint_path.encode() if checkpoint_path else None, checkpoint_exact)
+                   checkpoint_path.encode() if checkpoint_path else None, checkpoint_exact, graph)
 
 
 _lib = None
@@ -85,7 +85,8 @@ PF = C.POINTER(C.c_float)
 class BatchStats(C.Structure):
     _fields_ = [("t_total", C.c_double), ("rounds", C.c_long), ("newton", C.c_long),
                 ("steps", C.c_long), ("failed", C.c_int), ("ms_factor", C.c_double),
-                ("ms_solve", C.c_double), ("factor_flops", C.c_double), ("solve_bytes", C.c_double)]
+                ("ms_solve", C.c_double), ("factor_flops", C.c_double), ("solve_bytes", C.c_double),
+                ("ops_per_round", C.c_double), ("body_ops", C.c_long)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -391,9 +392,11 @@ class GpuBatch:
         except Exception:
             pass
 
-    def run(self, max_steps: int = 1 << 30, keep_d: bool = True, timing: bool = False) -> dict:
+    def run(self, max_steps: int = 1 << 30, keep_d: bool = True, timing: bool = False,
+            graph: bool = False) -> dict:
+        """cb_batch_run; graph replays each round's Newton iteration as one HIP graph."""
         st = BatchStats()
-        opt = options(max_steps=max_steps, keep_d=int(keep_d), kernel_timing=int(timing))
+        opt = options(max_steps=max_steps, keep_d=int(keep_d), kernel_timing=int(timing), graph=int(graph))
         _check(lib().cb_batch_run(self.h, C.byref(opt), C.byref(st)))
         status = np.zeros(self.count, np.int32)
         nsteps = np.zeros(self.count, np.int32)

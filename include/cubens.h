@@ -65,6 +65,10 @@ typedef struct {
      * line, the working lengths and areas and the time-increment multiplier,
      * so that a restart continues the run bit for bit */
     int checkpoint_exact;
+    /* batches: 1 replays each round's Newton iteration as one HIP graph
+     * captured from the first round's launches (cb_batch_run without
+     * kernel_timing); 0 issues the launches one by one */
+    int graph;
 } cb_options;
 
 /* Schedules of the band LDL^T factorization (same tile arithmetic, bitwise
@@ -318,6 +322,11 @@ typedef struct {
      * entries the two sweeps read, structurally zero chunks left out) */
     double ms_factor, ms_solve;
     double factor_flops, solve_bytes;
+    /* stream operations (launches, copies, graph launches) issued per round on
+     * average, and the operations of one round's Newton iteration (the nodes of
+     * its graph) */
+    double ops_per_round;
+    long body_ops;
 } cb_batch_stats;
 /* Reads every model (as cb_open does) and builds the batch's device state;
  * frame-only models without member-end offsets or free warping DOFs. */

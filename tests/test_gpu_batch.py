@@ -179,3 +179,32 @@ def test_batch_edge_cases():
             lib = cubens.lib()
             import ctypes as C
             cubens._check(lib.cb_batch_get_path(b.h, 5, None, None, None))
+
+
+def test_graph_replay_equals_the_launches_one_by_one():
+    """The round's Newton iteration captured once and replayed as a HIP graph
+    (no memset node in it; the band, flags and counters are cleared by
+    kernels) gives every model's path bitwise as the launches issued one by
+    one, over every round of a mixed batch that halves time steps, and again
+    on a second graph run of the same batch, with eager runs in between."""
+    steps = 12
+    paths = _mixed_files(steps)
+    with cubens.GpuBatch(paths) as b:
+        eager = b.run(max_steps=steps)
+        ref = [b.path(i) for i in range(b.count)]
+        for rep in range(2):
+            print(f"[batch] graph run {rep}", flush=True)
+            st = b.run(max_steps=steps, graph=True)
+            assert np.array_equal(st["status"], eager["status"])
+            assert np.array_equal(st["nsteps"], eager["nsteps"])
+            assert st["rounds"] == eager["rounds"]
+            for i in range(b.count):
+                p = b.path(i)
+                assert np.array_equal(p["itecnt"], ref[i]["itecnt"])
+                assert np.array_equal(p["lpf"], ref[i]["lpf"])
+                assert np.array_equal(p["d"], ref[i]["d"])
+            print(f"[batch] ops per round: eager {eager['ops_per_round']:.1f}, graph "
+                  f"{st['ops_per_round']:.1f}; iteration {st['body_ops']} operations", flush=True)
+            assert st["body_ops"] == eager["body_ops"] > 10
+            assert st["ops_per_round"] < eager["ops_per_round"] - st["body_ops"] + 2
+            b.run(max_steps=3)                 # eager work between the graph runs
