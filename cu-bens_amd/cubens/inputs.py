@@ -605,6 +605,200 @@ def cantilever_frame(n_elem: int = 4, length: float = 3.0, r_o: float = 0.1, loa
                            "load": load, "neq": 6 * n_elem})
 
 
+# --------------------------------------------------------------------------
+# Inelastic analyses (ANAFLAG 3): plastic hinges at frame member ends, bars
+# capped at their squash load
+# --------------------------------------------------------------------------
+def chs_plastic_modulus(r_o: float) -> float:
+    """Plastic section modulus of the circular hollow section of chs_section
+    (wall r_o/10): Z = 4/3 (r_o^3 - r_i^3)."""
+    r_i = 0.9 * r_o
+    return 4.0 / 3.0 * (r_o ** 3 - r_i ** 3)
+
+
+def _nonlinear_tail(w, analysis: str, load_lines, nr, msal, nm, frames: bool) -> None:
+    """Loads and driver parameters of an NR/MNR (main.c:1808-1811), MSAL
+    (main.c:2172-2186) or nonlinear Newmark (main.c:3312-3330) analysis.
+    load_lines are (joint, dir, value) static loads, or (joint, dir, values)
+    load histories under Newmark."""
+    if analysis == "newmark":
+        steps, dt = nm["steps"], nm["dt"]
+        w(f"{steps},{_fmt(steps * dt)}\n")
+        for j, d, vals in load_lines:
+            for v in vals:
+                w(f"{j},{d},{_fmt(v)}\n")
+        w("0,0,0\n0,0,0\n0,0,0,0,0\n")
+        w(f"{nm['numopt']},{_fmt(nm['spectrds'])}\n")
+        w(f"1.0,{_fmt(nm['dlpf'])},{_fmt(nm['dlpf'])},{_fmt(nm['dlpfmax'])},{_fmt(nm['dlpfmin'])}\n")
+        w(f"{nm['itemax']},{nm['submax']},{nm['solmin']}\n")
+        t = nm["tol"]
+        w(f"{t[0]},{t[1]},{t[2]}\n")
+        return
+    for j, d, v in load_lines:
+        w(f"{j},{d},{_fmt(v)}\n")
+    w("0,0,0\n")
+    if frames:
+        w("0,0,0\n")                 # no member distributed loads
+    if analysis == "msal":
+        j, d, dk = msal["ctrl"]
+        w(f"{j},{d},{_fmt(dk)}\n")
+        w(f"{_fmt(msal['alpha'])}\n{_fmt(msal['psi_thresh'])}\n{msal['iteopt']}\n")
+        w(f"{_fmt(msal['lpfmax'])},{_fmt(msal['dkimax'])}\n")
+        w(f"{msal['itemax']},{msal['submax']},{msal['imagmax']},{msal['negmax']}\n")
+        t = msal["tol"]
+    else:
+        w(f"{_fmt(nr['lpfmax'])},{_fmt(nr['lpf'])},{_fmt(nr['dlpf'])},{_fmt(nr['dlpfmax'])},"
+          f"{_fmt(nr['dlpfmin'])}\n")
+        w(f"{nr['itemax']},{nr['submax']},{nr['solmin']}\n")
+        t = nr["tol"]
+    w(f"{t[0]},{t[1]},{t[2]}\n")
+
+
+def propped_cantilever(n_elem: int = 8, length: float = 4.0, r_o: float = 0.1, fy: float = 345e6,
+                       load: float = -1000.0, axial: float = 0.0, analysis: str = "nr",
+                       algflag: int | None = None, lpfmax: float | None = None,
+                       dlpf: float | None = None, dlpfmin: float | None = None, solmin: int = 1000,
+                       itemax: int = 30, submax: int = 20, tol=(1e-10, 1e-10, 1e-10),
+                       dk: float | None = None, alpha: float = 1.0, iteopt: int = 6,
+                       steps: int = 20, dt: float = 2e-3, history=None, numopt: int = 0,
+                       spectrds: float = 1.0, chkpt: int | None = None, rflag: int = 0) -> ModelText:
+    """A propped cantilever of ``n_elem`` circular hollow members (chs_section
+    with the plastic modulus chs_plastic_modulus and yield stress ``fy``)
+    along x, clamped at joint 1, on a roller at the far end (translations y, z
+    and the twist restrained, the axial slide free), warping fixed
+    everywhere, with ``load`` in direction 2 at the middle joint and
+    ``axial`` in direction 1 at the roller: an inelastic analysis (ANAFLAG 3).
+
+    Small-displacement plastic theory for the central load P: the clamped end
+    carries the largest moment, 3 P L/16, and yields first at
+    P1 = 16 Mp / (3 L); a second hinge under the load makes the mechanism at
+    Pc = 6 Mp / L, with Mp = Z fy.
+
+    ``analysis`` "nr" (ALGFLAG 1, or ``algflag`` 2 for modified NR) steps the
+    load factor by ``dlpf`` up to ``lpfmax``; "msal" drives the loaded joint's
+    direction 2 by ``dk`` first (ALGFLAG 3); "newmark" runs ``steps`` steps of
+    ``dt`` (ALGFLAG 5) under the load factor history ``history`` (default: a
+    ramp to 1 over the first half, then constant) times ``load``.
+    """
+    assert n_elem % 2 == 0
+    sec = chs_section(r_o)
+    Z = chs_plastic_modulus(r_o)
+    Mp = Z * fy
+    P1 = 16 * Mp / (3 * length)
+    Pc = 6 * Mp / length
+    nj = n_elem + 1
+    mid = n_elem // 2 + 1
+    if algflag is None:
+        algflag = {"nr": 1, "msal": 3, "newmark": 5}[analysis]
+    if lpfmax is None:
+        lpfmax = 0.95 * Pc / abs(load) if analysis == "nr" else (10 * Pc / abs(load) if analysis == "msal" else 1.0)
+    if dlpf is None:
+        dlpf = 0.08 * Pc / abs(load) if analysis == "nr" else 0.1
+    if dlpfmin is None:
+        dlpfmin = dlpf / 1024 if analysis == "nr" else 1e-3
+    out = io.StringIO()
+    w = out.write
+    w(f"3\n{algflag}\n")
+    if algflag > 3:
+        w(f"{steps + 1 if chkpt is None else chkpt},{rflag}\n")
+    w(f"0\n1\n{nj}\n0,{n_elem},0,0,0\n")
+    for e in range(n_elem):
+        w(f"{e + 1},{e + 2}\n")
+    for d in range(1, 7):
+        w(f"1,{d}\n")
+    for d in (2, 3, 4):
+        w(f"{nj},{d}\n")
+    for j in range(1, nj + 1):
+        w(f"{j},7,0\n")
+    w("0,0\n")
+    if algflag > 3:
+        w("0,0\n")
+    for j in range(nj):
+        w(f"{_fmt(length * j / n_elem)},0.0,0.0\n")
+    for _ in range(n_elem):
+        w(f"{_fmt(sec['E'])},{_fmt(sec['G'])},7850.0,{_fmt(sec['A'])},{_fmt(sec['I'])},"
+          f"{_fmt(sec['I'])},{_fmt(sec['J'])},0.0\n")
+    w("0,0,0,0,0,0,0\n")
+    for e in range(n_elem):
+        w(f"{_fmt(length * e / n_elem)},0.0,1.0\n")
+    for _ in range(n_elem):
+        w(f"{_fmt(fy)},{_fmt(Z)},{_fmt(Z)}\n")
+    w("0,0,0,0,0\n")
+    if analysis == "newmark":
+        if history is None:
+            t = np.arange(steps + 1)
+            history = np.minimum(1.0, t / max(1, steps // 2))
+        loads = [(mid, 2, load * np.asarray(history, float))]
+        if axial:
+            loads.append((nj, 1, axial * np.asarray(history, float)))
+    else:
+        loads = [(mid, 2, load)] + ([(nj, 1, axial)] if axial else [])
+    nr = dict(lpfmax=lpfmax, lpf=dlpf, dlpf=dlpf, dlpfmax=dlpf, dlpfmin=dlpfmin, itemax=itemax,
+              submax=submax, solmin=solmin, tol=tol)
+    if dk is None:
+        dk = -length / 400 if load < 0 else length / 400
+    msal = dict(ctrl=(mid, 2, dk), alpha=alpha, psi_thresh=0.1, iteopt=iteopt, lpfmax=lpfmax,
+                dkimax=length / 8, itemax=itemax, submax=submax, imagmax=10, negmax=10, tol=tol)
+    nm = dict(steps=steps, dt=dt, numopt=numopt, spectrds=spectrds, dlpf=dlpf, dlpfmax=10.0,
+              dlpfmin=dlpfmin, itemax=itemax, submax=submax, solmin=solmin, tol=tol)
+    _nonlinear_tail(w, analysis, loads, nr, msal, nm, True)
+    return ModelText(out.getvalue(), nj, n_fr=n_elem,
+                     info={"E": sec["E"], "I": sec["I"], "A": sec["A"], "Z": Z, "fy": fy, "Mp": Mp,
+                           "Py": sec["A"] * fy, "P1": P1, "Pc": Pc, "length": length, "load": load,
+                           "mid": mid, "neq": 6 * (nj - 1) - 3})
+
+
+def three_bar_truss(height: float = 1.0, theta: float = math.pi / 4, emod: float = 2e11,
+                    area: float = 1e-4, fy: float = 250e6, load: float = -1000.0,
+                    lpfmax: float | None = None, dlpf: float | None = None,
+                    dlpfmin: float | None = None, itemax: int = 60, submax: int = 20,
+                    tol=(1e-10, 1e-10, 1e-10), algflag: int = 1) -> ModelText:
+    """The classic three-bar truss: a vertical bar of length ``height`` and two
+    bars at +-``theta`` from it, hung from supports in the plane y = 0 and
+    meeting at joint 1, which carries ``load`` in direction 3 (its y
+    translation restrained); inelastic (ANAFLAG 3), NR load control.
+
+    Small-displacement elastic-plastic theory with Py = fy A per bar: the
+    vertical bar carries P / (1 + 2 cos^3 theta) and yields first at
+    P1 = Py (1 + 2 cos^3 theta); the inclined bars then take the rest,
+    (P - Py) / (2 cos theta) each, and the truss collapses at
+    Pc = Py (1 + 2 cos theta).  The reference's bar force is capped at +-Py
+    with no plastic strain (truss.c:336-347), which under a monotonic load
+    follows the same path."""
+    Py = fy * area
+    c = math.cos(theta)
+    P1 = Py * (1 + 2 * c ** 3)
+    Pc = Py * (1 + 2 * c)
+    if lpfmax is None:
+        lpfmax = 0.97 * Pc / abs(load)
+    if dlpf is None:
+        dlpf = 0.1 * Pc / abs(load)
+    if dlpfmin is None:
+        dlpfmin = dlpf / 1024
+    out = io.StringIO()
+    w = out.write
+    w(f"3\n{algflag}\n0\n1\n4\n3,0,0,0,0\n")
+    for k in range(3):
+        w(f"1,{k + 2}\n")
+    w("1,2\n")
+    for k in range(3):
+        for d in (1, 2, 3):
+            w(f"{k + 2},{d}\n")
+    w("0,0\n")
+    s = math.tan(theta) * height
+    w(f"0.0,0.0,{_fmt(-height)}\n")
+    for xk in (0.0, -s, s):
+        w(f"{_fmt(xk)},0.0,0.0\n")
+    for _ in range(3):
+        w(f"{_fmt(emod)},{_fmt(area)},7850.0,{_fmt(fy)}\n")
+    nr = dict(lpfmax=lpfmax, lpf=dlpf, dlpf=dlpf, dlpfmax=dlpf, dlpfmin=dlpfmin, itemax=itemax,
+              submax=submax, solmin=1000, tol=tol)
+    _nonlinear_tail(w, "nr", [(1, 3, load)], nr, None, None, False)
+    return ModelText(out.getvalue(), 4, n_tr=3,
+                     info={"Py": Py, "P1": P1, "Pc": Pc, "height": height, "theta": theta,
+                           "emod": emod, "area": area, "load": load, "neq": 2})
+
+
 def frame_batch(count: int = 512, lo: int = 200, hi: int = 5000, **kw) -> list[ModelText]:
     """``count`` frames of U{lo..hi} members, towers and girders alternating."""
     rng = SplitMix64(workload_seed(4, 100000))

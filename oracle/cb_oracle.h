@@ -69,13 +69,13 @@ void orc_free(orc_model *m);
 
 /* ---- element routines; argument lists mirror prototypes.h ------------- */
 void orc_stiff_tr(const orc_dims *d, double *ss, const double *emod, const double *carea,
-                  const double *llength, const double *defllen_ip, const double *c1_ip,
-                  const double *c2_ip, const double *c3_ip, const double *ef_ip,
-                  const long *maxa, const long *mcode);
+                  const double *llength, const double *defllen_ip, const double *yield,
+                  const double *c1_ip, const double *c2_ip, const double *c3_ip,
+                  const double *ef_ip, const long *maxa, const long *mcode);
 void orc_forces_tr(const orc_dims *d, double *f, double *ef_i, const double *emod,
                    const double *carea, const double *llength, const double *defllen_i,
-                   const double *c1_i, const double *c2_i, const double *c3_i,
-                   const long *mcode);
+                   const double *yield, const double *c1_i, const double *c2_i,
+                   const double *c3_i, const long *mcode);
 /* ANAFLAG 1 (linear elastic) forces from the total displacement D:
  * truss.c:243-326 and shell.c:1695-1727 with the scatter of shell.c:2385-2397. */
 void orc_forces_tr_lin(const orc_dims *d, double *f, double *ef_i, const double *D,
@@ -115,12 +115,36 @@ void orc_frame_ke(double E, double G, double A, double L, double Is, double Iw, 
 void orc_frame_kg(const double f[14], double L, double A, double Ip, double k[14][14]);
 void orc_stiff_fr(const orc_dims *d, double *ss, const orc_model *m, const double *llength,
                   const double *defllen_ip, const double *c1_ip, const double *c2_ip,
-                  const double *c3_ip, const double *ef_ip, const double *efFE_ip);
+                  const double *c3_ip, const double *ef_ip, const double *efFE_ip,
+                  const int *yldflag);
+/* Returns forces_fr's code (frame.c:1184-1265): 0, 1 when a member end has
+ * gone through the yield surface (*dlpf scaled by the regula-falsi fraction,
+ * the end flagged 1, the members after it left as they were), 2 when a
+ * yielded end unloads (flagged 2).  yldflag may be NULL below ANAFLAG 3. */
 int orc_forces_fr(const orc_dims *d, double *f, const double *ef_ip, double *ef_i,
                   const double *efFE_ip, double *efFE_i, const double *dd,
                   const orc_model *m, const double *llength, const double *defllen_ip,
                   const double *c1_ip, const double *c2_ip, const double *c3_ip,
-                  const double *c1_i, const double *c2_i, const double *c3_i);
+                  const double *c1_i, const double *c2_i, const double *c3_i,
+                  int *yldflag, double *dlpf);
+
+/* ---- plastic hinges of the frame member ends (ANAFLAG 3) ------------------
+ * Orbison's yield function of end forces normalised by the squash load and
+ * the plastic moments (frame.c:613-620); its gradient with respect to the end
+ * forces (P, My, Mz) (frame.c:626-637); the plastic reduction of a local
+ * tangent k <- k - (k G)(G^T k G)^-1 (k G)^T for the ends stiffm_fr selects
+ * from the total end forces eftot (frame.c:580-797); the fraction of an
+ * end-force increment at which phi reaches 1 (regula_falsi,
+ * frame.c:1397-1455); and the unloading test (unload, frame.c:1457-1658):
+ * 0, or 1 / 2 / 3 for both ends / end 1 / end 2 unloading. */
+double orc_orbison(double p, double my, double mz);
+void orc_orbison_grad(double p, double my, double mz, double Py, double Mpy, double Mpz,
+                      double g[3]);
+void orc_frame_plastic(double k[14][14], const double eftot[14], const int yld[2], double Py,
+                       double Mpy, double Mpz);
+double orc_regula_falsi(double p, double dp, double my, double dmy, double mz, double dmz);
+int orc_frame_unload(const double phi[2], const double eftot_i[14], double Py, double Mpy,
+                     double Mpz, const double k[14][14], const double dl[14]);
 /* Lumped diagonal masses into sm[NEQ]; overwrite the geometry arrays they are
  * given, as the reference's mass_* do. */
 void orc_mass_tr(const orc_dims *d, double *sm, const orc_model *m, const double *x,
@@ -201,13 +225,13 @@ typedef struct orc_backend {
     void (*stiff_fr)(void *ctx, const orc_dims *d, double *ss, const orc_model *m,
                      const double *llength, const double *defllen_ip, const double *c1_ip,
                      const double *c2_ip, const double *c3_ip, const double *ef_ip,
-                     const double *efFE_ip);
+                     const double *efFE_ip, int *yldflag);
     int (*forces_fr)(void *ctx, const orc_dims *d, double *f, double *ef_ip, double *ef_i,
                      const double *efFE_ip, double *efFE_i, const double *dd,
                      const orc_model *m, const double *llength, const double *defllen_ip,
                      const double *c1_ip, const double *c2_ip, const double *c3_ip,
                      const double *c1_i, const double *c2_i, const double *c3_i,
-                     double *dlpf, int itecnt);
+                     int *yldflag, double *dlpf, int itecnt);
     void (*mass_tr)(void *ctx, const orc_dims *d, double *sm, const orc_model *m,
                     const double *x, double *llength, const double *dens);
     void (*mass_fr)(void *ctx, const orc_dims *d, double *sm, const orc_model *m,
@@ -253,11 +277,15 @@ typedef struct {
 /* Driver events of one converged step: the pivot-sign flag of the step's last
  * factorization (det, solve.c:608-621), and how often within the step quad
  * found imaginary roots (arc.c:145) or no root with a positive gamma
- * (arc.c:134), the dnorm test restarted the increment (main.c:2936-2953), and
+ * (arc.c:134), the dnorm test restarted the increment (main.c:2936-2953),
  * the arc or load step was halved after itemax iterations (main.c:3000-3012,
- * 2096-2110). */
+ * 2096-2110), and, under ANAFLAG 3, how often forces_fr's code 1 (a member
+ * end through the yield surface: load step scaled by the regula-falsi
+ * fraction, main.c:2050-2057, or the arc halved, main.c:2961-2970) and code 2
+ * (a yielded end unloading: the same step again, main.c:2033-2034, 2954-2960)
+ * restarted the step. */
 enum { ORC_EV_DET = 0, ORC_EV_IMAG = 1, ORC_EV_NEG = 2, ORC_EV_DNORM = 3, ORC_EV_HALVE = 4,
-       ORC_NEV = 5 };
+       ORC_EV_YIELD = 5, ORC_EV_UNLOAD = 6, ORC_NEV = 7 };
 
 /* Runs the analysis selected by ALGFLAG (1/2 NR/MNR main.c:1800-2157,
  * 3 MSAL main.c:2158-3141, 5 nonlinear Newmark main.c:3305-3960) with the

@@ -73,8 +73,8 @@ CKPT_ARRAYS = ("uc", "vc", "ac", "ss", "sm", "d", "f", "ef", "x", "c1", "c2", "c
                "slen_w")
 
 # Slots of the per-step driver events (ORC_EV_* in cb_oracle.h).
-EV_DET, EV_IMAG, EV_NEG, EV_DNORM, EV_HALVE = range(5)
-NEV = 5
+EV_DET, EV_IMAG, EV_NEG, EV_DNORM, EV_HALVE, EV_YIELD, EV_UNLOAD = range(7)
+NEV = 7
 
 
 class Sample(C.Structure):
@@ -190,6 +190,10 @@ class OracleModel:
         self.x = _arr(m.x, self.nj * 3)
         self.q = _arr(m.q, self.neq)
         self.emod = _arr(m.emod, self.ntr + self.nfr + self.nsh)
+        self.yield_ = _arr(m.yield_, self.ntr + self.nfr + self.nsh)
+        self.zstrong = _arr(m.zstrong, self.nfr)
+        self.zweak = _arr(m.zweak, self.nfr)
+        self.anaflag = m.dm.ANAFLAG
         self.c1 = _arr(m.c1, ntri)
         self.c2 = _arr(m.c2, ntri)
         self.c3 = _arr(m.c3, ntri)
@@ -369,9 +373,44 @@ def bind_elements():
     L.orc_stiff_sh.argtypes = [D] + [PD] * 12 + [PL] * 3
     L.orc_forces_sh.argtypes = [D, PD, PD, PD] + [PD] * 14 + [PL, PL]
     L.orc_updatc.argtypes = [D, PD, PD, PD, PD, PD, PD, PD, PD, PI, PD, PD, PD, PD, PL, PL]
-    L.orc_stiff_tr.argtypes = [D, PD] + [PD] * 8 + [PL, PL]
-    L.orc_forces_tr.argtypes = [D, PD, PD] + [PD] * 7 + [PL]
+    L.orc_stiff_tr.argtypes = [D, PD] + [PD] * 9 + [PL, PL]
+    L.orc_forces_tr.argtypes = [D, PD, PD] + [PD] * 8 + [PL]
+    L.orc_orbison.argtypes = [C.c_double] * 3
+    L.orc_orbison.restype = C.c_double
+    L.orc_orbison_grad.argtypes = [C.c_double] * 6 + [PD]
+    L.orc_frame_plastic.argtypes = [PD, PD, PI, C.c_double, C.c_double, C.c_double]
+    L.orc_regula_falsi.argtypes = [C.c_double] * 6
+    L.orc_regula_falsi.restype = C.c_double
+    L.orc_frame_unload.argtypes = [PD, PD, C.c_double, C.c_double, C.c_double, PD, PD]
+    L.orc_frame_unload.restype = C.c_int
     return L
+
+
+def orbison(p, my, mz) -> float:
+    """Orbison's yield function of normalised end forces (frame.c:613-620)."""
+    return bind_elements().orc_orbison(p, my, mz)
+
+
+def orbison_grad(p, my, mz, Py, Mpy, Mpz) -> np.ndarray:
+    g = np.zeros(3)
+    bind_elements().orc_orbison_grad(p, my, mz, Py, Mpy, Mpz, ptr(g))
+    return g
+
+
+def frame_plastic(k, eftot, yld, Py, Mpy, Mpz) -> np.ndarray:
+    """stiffm_fr (frame.c:580-797) on a copy of the 14x14 local tangent k."""
+    out = np.array(k, np.float64, copy=True, order="C")
+    y = np.ascontiguousarray(yld, np.int32)
+    bind_elements().orc_frame_plastic(ptr(out), _c(eftot), ptr(y, PI), Py, Mpy, Mpz)
+    return out
+
+
+def regula_falsi(p, dp, my, dmy, mz, dmz) -> float:
+    return bind_elements().orc_regula_falsi(p, dp, my, dmy, mz, dmz)
+
+
+def frame_unload(phi, eftot_i, Py, Mpy, Mpz, k, dl) -> int:
+    return bind_elements().orc_frame_unload(_c(phi), _c(eftot_i), Py, Mpy, Mpz, _c(k), _c(dl))
 
 
 def stiff_sh(om: "OracleModel", x, c1, c2, c3, deffarea) -> np.ndarray:
@@ -389,7 +428,7 @@ def stiff_tr(om: "OracleModel", c1, c2, c3, defllen, ef) -> np.ndarray:
     L = bind_elements()
     ss = np.zeros(om.lss)
     L.orc_stiff_tr(C.byref(om.dims), ptr(ss), ptr(om.emod), ptr(om.carea), ptr(om.llength),
-                   ptr(np.ascontiguousarray(defllen)), ptr(np.ascontiguousarray(c1)),
+                   ptr(np.ascontiguousarray(defllen)), ptr(om.yield_), ptr(np.ascontiguousarray(c1)),
                    ptr(np.ascontiguousarray(c2)), ptr(np.ascontiguousarray(c3)),
                    ptr(np.ascontiguousarray(ef)), ptr(om.maxa, PL), ptr(om.mcode, PL))
     return ss
@@ -414,7 +453,7 @@ def forces_tr(om: "OracleModel", c1, c2, c3, defllen):
     f = np.zeros(om.neq)
     ef = np.zeros(om.ntr * 2)
     L.orc_forces_tr(C.byref(om.dims), ptr(f), ptr(ef), ptr(om.emod), ptr(om.carea), ptr(om.llength),
-                    ptr(np.ascontiguousarray(defllen)), ptr(np.ascontiguousarray(c1)),
+                    ptr(np.ascontiguousarray(defllen)), ptr(om.yield_), ptr(np.ascontiguousarray(c1)),
                     ptr(np.ascontiguousarray(c2)), ptr(np.ascontiguousarray(c3)), ptr(om.mcode, PL))
     return f, ef
 
@@ -485,29 +524,39 @@ def _c(a):
     return ptr(np.ascontiguousarray(a, np.float64))
 
 
-def stiff_fr(om: "OracleModel", llength, defllen_ip, c1, c2, c3, ef_ip) -> np.ndarray:
-    """Skyline array of stiff_fr at the given state (ss zeroed first)."""
+def stiff_fr(om: "OracleModel", llength, defllen_ip, c1, c2, c3, ef_ip, yld=None) -> np.ndarray:
+    """Skyline array of stiff_fr at the given state (ss zeroed first); yld:
+    the yldflag state under ANAFLAG 3."""
     L_ = lib()
-    L_.orc_stiff_fr.argtypes = [C.POINTER(Dims), PD, C.c_void_p] + [PD] * 7
+    L_.orc_stiff_fr.argtypes = [C.POINTER(Dims), PD, C.c_void_p] + [PD] * 7 + [PI]
     ss = np.zeros(om.lss)
     efFE = np.zeros(max(om.nfr * 14, 1))
+    y = np.zeros(max(om.nfr * 2, 1), np.int32) if yld is None else np.ascontiguousarray(yld, np.int32)
     L_.orc_stiff_fr(C.byref(om.dims), ptr(ss), om.h, _c(llength), _c(defllen_ip), _c(c1), _c(c2),
-                    _c(c3), _c(ef_ip), ptr(efFE))
+                    _c(c3), _c(ef_ip), ptr(efFE), ptr(y, PI))
     return ss
 
 
-def forces_fr(om: "OracleModel", dd, llength, defllen_ip, c_ip, c_i, ef_ip):
-    """forces_fr: returns (f, ef_i)."""
+def forces_fr(om: "OracleModel", dd, llength, defllen_ip, c_ip, c_i, ef_ip, yld=None, dlpf=1.0):
+    """forces_fr: returns (f, ef_i), or under ANAFLAG 3 with yld given
+    (f, ef_i, code, yld after the call, dlpf after the call)."""
     L_ = lib()
-    L_.orc_forces_fr.argtypes = [C.POINTER(Dims), PD, PD, PD, PD, PD, PD, C.c_void_p] + [PD] * 8
+    L_.orc_forces_fr.argtypes = ([C.POINTER(Dims), PD, PD, PD, PD, PD, PD, C.c_void_p] + [PD] * 8 +
+                                 [PI, PD])
     L_.orc_forces_fr.restype = C.c_int
     f = np.zeros(om.neq)
     efi = np.zeros(len(ef_ip))
     efFE = np.zeros(max(om.nfr * 14, 1))
     efFE_i = np.zeros_like(efFE)
-    L_.orc_forces_fr(C.byref(om.dims), ptr(f), _c(ef_ip), ptr(efi), ptr(efFE), ptr(efFE_i), _c(dd), om.h,
-                     _c(llength), _c(defllen_ip), *[_c(v) for v in (*c_ip, *c_i)])
-    return f, efi
+    y = (np.zeros(max(om.nfr * 2, 1), np.int32) if yld is None
+         else np.array(yld, np.int32, copy=True))
+    dl = np.array([dlpf], np.float64)
+    code = L_.orc_forces_fr(C.byref(om.dims), ptr(f), _c(ef_ip), ptr(efi), ptr(efFE), ptr(efFE_i),
+                            _c(dd), om.h, _c(llength), _c(defllen_ip), *[_c(v) for v in (*c_ip, *c_i)],
+                            ptr(y, PI), ptr(dl))
+    if yld is None:
+        return f, efi
+    return f, efi, code, y, float(dl[0])
 
 
 def mass(om: "OracleModel", x):

@@ -31,8 +31,9 @@ static void be_stiff_tr(void *ctx, const orc_dims *d, double *ss, const double *
                         const double *c3_ip, const double *ef_ip, const long *maxa,
                         const long *mcode)
 {
-    (void)ctx; (void)yield;
-    orc_stiff_tr(d, ss, emod, carea, llength, defllen_ip, c1_ip, c2_ip, c3_ip, ef_ip, maxa, mcode);
+    (void)ctx;
+    orc_stiff_tr(d, ss, emod, carea, llength, defllen_ip, yield, c1_ip, c2_ip, c3_ip, ef_ip, maxa,
+                 mcode);
 }
 static void be_forces_tr(void *ctx, const orc_dims *d, double *f, double *ef_i,
                          const double *emod, const double *carea, const double *llength,
@@ -40,11 +41,11 @@ static void be_forces_tr(void *ctx, const orc_dims *d, double *f, double *ef_i,
                          const double *c2_i, const double *c3_i, const long *mcode,
                          const double *d_tot)
 {
-    (void)ctx; (void)yield;
+    (void)ctx;
     if (d->ANAFLAG == 1)
         orc_forces_tr_lin(d, f, ef_i, d_tot, emod, carea, llength, c1_i, c2_i, c3_i, mcode);
     else
-        orc_forces_tr(d, f, ef_i, emod, carea, llength, defllen_i, c1_i, c2_i, c3_i, mcode);
+        orc_forces_tr(d, f, ef_i, emod, carea, llength, defllen_i, yield, c1_i, c2_i, c3_i, mcode);
 }
 static void be_stiff_sh(void *ctx, const orc_dims *d, double *ss, const double *emod,
                         const double *nu, const double *x_temp, const double *xlocal,
@@ -91,21 +92,21 @@ static void be_updatc(void *ctx, const orc_dims *d, double *x_temp, double *x_ip
 static void be_stiff_fr(void *ctx, const orc_dims *d, double *ss, const orc_model *m,
                         const double *llength, const double *defllen_ip, const double *c1_ip,
                         const double *c2_ip, const double *c3_ip, const double *ef_ip,
-                        const double *efFE_ip)
+                        const double *efFE_ip, int *yldflag)
 {
     (void)ctx;
-    orc_stiff_fr(d, ss, m, llength, defllen_ip, c1_ip, c2_ip, c3_ip, ef_ip, efFE_ip);
+    orc_stiff_fr(d, ss, m, llength, defllen_ip, c1_ip, c2_ip, c3_ip, ef_ip, efFE_ip, yldflag);
 }
 static int be_forces_fr(void *ctx, const orc_dims *d, double *f, double *ef_ip, double *ef_i,
                         const double *efFE_ip, double *efFE_i, const double *dd,
                         const orc_model *m, const double *llength, const double *defllen_ip,
                         const double *c1_ip, const double *c2_ip, const double *c3_ip,
                         const double *c1_i, const double *c2_i, const double *c3_i,
-                        double *dlpf, int itecnt)
+                        int *yldflag, double *dlpf, int itecnt)
 {
-    (void)ctx; (void)dlpf; (void)itecnt;
+    (void)ctx; (void)itecnt;
     return orc_forces_fr(d, f, ef_ip, ef_i, efFE_ip, efFE_i, dd, m, llength, defllen_ip, c1_ip,
-                         c2_ip, c3_ip, c1_i, c2_i, c3_i);
+                         c2_ip, c3_ip, c1_i, c2_i, c3_i, yldflag, dlpf);
 }
 static void be_mass_tr(void *ctx, const orc_dims *d, double *sm, const orc_model *m,
                        const double *x, double *llength, const double *dens)
@@ -158,6 +159,7 @@ typedef struct {
     double *efFE, *efFE_i, *efFE_ip;
     double *farea_w, *slen_w;          /* face areas / side lengths the elements read */
     double *keff, *uc, *vc, *ac, *uc_i, *vc_i, *ac_i, *um, *vm, *am, *dyn;
+    int *yld;                          /* yldflag: 0, 1 yielded, 2 unloading (per member end) */
     orc_path *out;
     int ev[ORC_NEV];                   /* events since the last recorded step */
     double ev_psi;                     /* psi of the step being taken */
@@ -198,6 +200,7 @@ static void run_init(run_t *R, orc_model *m, const orc_backend *be, orc_path *ou
     R->ac_i = dz(n); R->um = dz(n); R->vm = dz(n); R->am = dz(n); R->dyn = dz(n);
     CP(R->farea_w, m->farea, R->nsh);
     CP(R->slen_w, m->slength, R->nsh * 3);
+    R->yld = calloc((size_t)(R->nfr * 2 + 1), sizeof(int));
 
     CP(R->x, m->x, R->nj * 3);
     CP(R->x_temp, m->x, R->nj * 3);
@@ -214,6 +217,7 @@ static void run_free(run_t *R)
 {
     double **p = &R->ss, **e = &R->dyn;
     for (; p <= e; ++p) free(*p);
+    free(R->yld);
 }
 
 /* _ip <- _i for the per-element generations (main.c:2390-2411). */
@@ -241,9 +245,12 @@ static void reset_from_base(run_t *R)
     CP(R->defslen_i, R->defslen, R->nsh * 3); CP(R->defslen_ip, R->defslen, R->nsh * 3);
 }
 
-/* base <- _i (main.c:3055-3110). */
+/* base <- _i (main.c:3055-3110); an unloading end (2) is elastic again
+   (main.c:3086-3091). */
 static void commit(run_t *R)
 {
+    for (long i = 0; i < R->nfr * 2; ++i)
+        if (R->yld[i] == 2) R->yld[i] = 0;
     CP(R->x, R->x_temp, R->nj * 3);
     CP(R->ef, R->ef_i, R->nef);
     CP(R->c1, R->c1_i, R->ntri); CP(R->c2, R->c2_i, R->ntri); CP(R->c3, R->c3_i, R->ntri);
@@ -267,7 +274,8 @@ static void assemble(run_t *R, const double *x, const double *c1, const double *
         R->be->stiff_tr(R->be->ctx, d, R->ss, m->emod, m->carea, R->llen_ref, defllen, m->yield,
                         c1, c2, c3, ef, m->maxa, m->mcode);
     if (d->NE_FR > 0)
-        R->be->stiff_fr(R->be->ctx, d, R->ss, m, R->llen_ref, defllen, c1, c2, c3, ef, efFE);
+        R->be->stiff_fr(R->be->ctx, d, R->ss, m, R->llen_ref, defllen, c1, c2, c3, ef, efFE,
+                        R->yld);
     if (d->NE_SH > 0)
         R->be->stiff_sh(R->be->ctx, d, R->ss, m->emod, m->nu, x, m->xlocal, m->thick, R->farea_w,
                         deffarea, R->slen_w, defslen, m->yield, c1, c2, c3, ef, m->maxa,
@@ -275,9 +283,11 @@ static void assemble(run_t *R, const double *x, const double *c1, const double *
     if (R->out) { R->out->t_tangent += now_s() - t0; }
 }
 
-/* updatc + forces_* into f_out (zeroed by the caller as in the reference). */
+/* updatc + forces_* into f_out (zeroed by the caller as in the reference);
+   returns forces_fr's code, with *dlpf the load factor (increment) that a
+   code 1 scales. */
 static int update_and_forces(run_t *R, double *x_temp, double *xfr_temp, double *f_out,
-                             const double *dd)
+                             const double *dd, double *dlpf, int itecnt)
 {
     orc_model *m = R->m;
     const orc_dims *d = &m->dm;
@@ -289,14 +299,12 @@ static int update_and_forces(run_t *R, double *x_temp, double *xfr_temp, double 
         R->be->forces_tr(R->be->ctx, d, f_out, R->ef_i, m->emod, m->carea, R->llen_ref,
                          R->defllen_i, m->yield, R->c1_i, R->c2_i, R->c3_i, m->mcode, R->d_temp);
     int chk = 0;
-    if (d->NE_FR > 0) {
-        double dl = 0;
+    if (d->NE_FR > 0)
         chk = R->be->forces_fr(R->be->ctx, d, f_out, R->ef_ip, R->ef_i, R->efFE_ip, R->efFE_i, dd,
                                m, R->llen_ref, R->defllen_ip, R->c1_ip, R->c2_ip, R->c3_ip,
-                               R->c1_i, R->c2_i, R->c3_i, &dl, 1);
-    }
+                               R->c1_i, R->c2_i, R->c3_i, R->yld, dlpf, itecnt);
     if (d->NE_SH > 0)
-        chk = R->be->forces_sh(R->be->ctx, d, f_out, R->ef_ip, R->ef_i, dd, x_temp, R->x_ip,
+        (void)R->be->forces_sh(R->be->ctx, d, f_out, R->ef_ip, R->ef_i, dd, x_temp, R->x_ip,
                                m->emod, m->nu, m->xlocal, m->thick, R->farea_w, R->deffarea_ip,
                                R->slen_w, R->defslen_ip, m->yield, R->c1_ip, R->c2_ip,
                                R->c3_ip, R->c1_i, R->c2_i, R->c3_i, m->minc, m->mcode,
@@ -399,7 +407,7 @@ static int run_nr(run_t *R)
                 R->f_ip[i] = R->f_temp[i];
                 R->f_temp[i] = 0;
             }
-            frc = update_and_forces(R, R->x_temp, R->xfr_temp, R->f_temp, R->dd);
+            frc = update_and_forces(R, R->x_temp, R->xfr_temp, R->f_temp, R->dd, &dlpf, itecnt);
             CP(R->ef_ip, R->ef_i, R->nef);
             if (itecnt == 0) {
                 intener1 = 0;
@@ -416,15 +424,20 @@ static int run_nr(run_t *R)
             if (R->out) R->out->n_newton++;
         } while (convchk != 0 && frc == 0 && itecnt <= m->itemax);
 
-        if ((convchk != 0 || frc != 0) && subcnt <= m->submax) {
+        if (frc == 2) {
+            /* an unloading end: the same increment again (main.c:2033-2034) */
+            dlpf = dlpfp;
+            R->ev[ORC_EV_UNLOAD]++;
+        } else if ((convchk != 0 || frc != 0) && subcnt <= m->submax) {
             if (lpf == m->lpfmax) FAIL(1, "maximum load factor attempted without convergence");
             if (dlpfp == m->dlpfmin) FAIL(1, "minimum load factor increment reached");
-            dlpf = dlpfp / 2;
+            /* code 1 keeps the increment forces_fr scaled (main.c:2050-2053) */
+            if (frc != 1) dlpf = dlpfp / 2;
             if (dlpf < m->dlpfmin) dlpf = m->dlpfmin;
             lpf = lpf - dlpfp + dlpf;
             subcnt++;
             solcnt = 0;
-            R->ev[ORC_EV_HALVE]++;
+            R->ev[frc == 1 ? ORC_EV_YIELD : ORC_EV_HALVE]++;
         } else if (subcnt > m->submax) {
             FAIL(1, "maximum number of subdivisions exceeded");
         } else {
@@ -479,7 +492,9 @@ static int run_msal(run_t *R)
         R->ssd_o[i] = R->ssd[i];
     }
     memset(R->f, 0, sizeof(double) * (size_t)n);
-    frc = update_and_forces(R, R->x, R->xfr, R->f, R->dd);
+    /* the first call hands forces_fr lpf itself as the increment a code 1
+       scales, and its code is dropped (main.c:2365-2369, 2411) */
+    frc = update_and_forces(R, R->x, R->xfr, R->f, R->dd, &lpf, 0);
     itecnt = 1;
     shift_ip(R, 1);
     frc = 0;
@@ -502,7 +517,7 @@ static int run_msal(run_t *R)
             R->f_ip[i] = R->f[i];
             R->f[i] = 0;
         }
-        frc = update_and_forces(R, R->x, R->xfr, R->f, R->dd);
+        frc = update_and_forces(R, R->x, R->xfr, R->f, R->dd, &dlpf, itecnt);
         if (orc_test(n, R->d, R->dd, R->f, R->fp, R->qtot, R->f_ip, intener1, &convchk,
                      m->toldisp, m->tolforc, m->tolener))
             FAIL(1, "zero norm in convergence test");
@@ -581,7 +596,9 @@ static int run_msal(run_t *R)
             intener1 += R->dd[i] * (dlpf * m->q[i]);
             R->f_temp[i] = 0;
         }
-        update_and_forces(R, R->x_temp, R->xfr_temp, R->f_temp, R->dd);
+        /* the predictor's code is not looked at; its flags stay, and a return
+           leaves the later members out of f_temp (main.c:2794-2800) */
+        update_and_forces(R, R->x_temp, R->xfr_temp, R->f_temp, R->dd, &dlpf, itecnt);
         shift_ip(R, 1);
         itecnt = 1;
         frc = 0;
@@ -620,7 +637,8 @@ static int run_msal(run_t *R)
                 fprintf(stderr, "msal arc=%.4e psi=%.3f a=%.3e b=%.3e c=%.3e quad=%d lpf=%.6e it=%d\n",
                         arc, psi, a, b, c, errchk2, lpf_temp, itecnt);
             if (errchk2 == 0) {
-                frc = update_and_forces(R, R->x_temp, R->xfr_temp, R->f_temp, R->dd);
+                frc = update_and_forces(R, R->x_temp, R->xfr_temp, R->f_temp, R->dd, &dlpf,
+                                        itecnt);
                 if (orc_test(n, R->d_temp, R->dd, R->f_temp, R->fp, R->qtot, R->f_ip, intener1,
                              &convchk, m->toldisp, m->tolforc, m->tolener))
                     FAIL(1, "zero norm in convergence test");
@@ -640,12 +658,16 @@ static int run_msal(run_t *R)
                         errchk2 = 1;
                         R->ev[ORC_EV_DNORM]++;
                     }
+                } else if (frc == 2) {
+                    CP(R->f_temp, R->f, n);
+                    errchk2 = 1;
+                    R->ev[ORC_EV_UNLOAD]++;
                 } else if (frc != 0 && subcnt <= m->submax) {
                     arc *= 0.5;
                     CP(R->f_temp, R->f, n);
                     subcnt++;
                     errchk2 = 1;
-                    R->ev[ORC_EV_HALVE]++;
+                    R->ev[ORC_EV_YIELD]++;
                 } else {
                     itecnt++;
                     if (itecnt > m->itemax) {
@@ -714,7 +736,7 @@ static int run_msal(run_t *R)
  * a fresh start copies them (main.c:3420); here they come from the file's
  * llength as a fresh start would take them, and the exact file's extension
  * restores every working length and area and the time-increment multiplier. */
-static int ckpt_view(run_t *R, orc_ckpt *c, double *sm_l, double *ss_l, int *yld)
+static int ckpt_view(run_t *R, orc_ckpt *c, double *sm_l, double *ss_l)
 {
     memset(c, 0, sizeof *c);
     c->uc = R->uc; c->vc = R->vc; c->ac = R->ac;
@@ -722,7 +744,7 @@ static int ckpt_view(run_t *R, orc_ckpt *c, double *sm_l, double *ss_l, int *yld
     c->d = R->d; c->f = R->f; c->ef = R->ef; c->x = R->x;
     c->c1 = R->c1; c->c2 = R->c2; c->c3 = R->c3;
     c->defllen = R->defllen; c->llength = R->llen_ref;
-    c->efFE = R->efFE; c->xfr = R->xfr; c->yldflag = yld;
+    c->efFE = R->efFE; c->xfr = R->xfr; c->yldflag = R->yld;
     c->deffarea = R->deffarea; c->defslen = R->defslen;
     c->llen_w = R->llen_ref; c->farea_w = R->farea_w; c->slen_w = R->slen_w;
     return 0;
@@ -733,15 +755,13 @@ static int ckpt_write(run_t *R, long k, double ddt)
     orc_model *m = R->m;
     long lss = m->lss;
     double *sm_l = dz(lss);
-    int *yld = calloc((size_t)(R->nfr * 2 + 1), sizeof(int));
     orc_ckpt c;
-    ckpt_view(R, &c, sm_l, R->ss, yld);
+    ckpt_view(R, &c, sm_l, R->ss);
     CP(sm_l, R->sm, R->neq);
     c.tstep = k;
     c.ddt = ddt;
     int rc = orc_ckpt_write(R->out->ckpt_path, &m->dm, lss, &c, R->out->ckpt_exact);
     free(sm_l);
-    free(yld);
     return rc;
 }
 
@@ -750,16 +770,15 @@ static int ckpt_restart(run_t *R, long *tstep, double *ddt)
     orc_model *m = R->m;
     long lss = m->lss;
     double *sm_l = dz(lss), *ss_l = dz(lss);
-    int *yld = calloc((size_t)(R->nfr * 2 + 1), sizeof(int));
     orc_ckpt c;
-    ckpt_view(R, &c, sm_l, ss_l, yld);
+    ckpt_view(R, &c, sm_l, ss_l);
     int rc = orc_ckpt_read(R->out->ckpt_path, &m->dm, lss, &c);
     if (rc == 0) {
         CP(R->sm, sm_l, R->neq);
         *tstep = c.tstep;
         *ddt = c.has_ext ? c.ddt : 1;
     }
-    free(sm_l); free(ss_l); free(yld);
+    free(sm_l); free(ss_l);
     return rc;
 }
 
@@ -788,7 +807,7 @@ static int run_newmark(run_t *R)
     double ddt = 1, sub_dt = 1, dt_temp = dt;
     int tsflag = 0;
     double lpfi = m->lpf0, dlpfi = m->dlpf0;
-    int convchk = 1, frcchk = 0, itecnt = 0, det = 0;
+    int convchk = 1, frcchk = 0, frc_fr = 0, itecnt = 0, det = 0;
     double intener1 = 0;
     /* the frame/truss lengths and the shell areas/side lengths the elements
        read are the run's working copies, overwritten by the mass routines */
@@ -866,7 +885,7 @@ static int run_newmark(run_t *R)
                     }
                     if (d->NE_FR > 0) {
                         R->be->stiff_fr(R->be->ctx, d, R->ss, m, llen, R->defllen_ip, R->c1_ip,
-                                        R->c2_ip, R->c3_ip, R->ef_ip, R->efFE_ip);
+                                        R->c2_ip, R->c3_ip, R->ef_ip, R->efFE_ip, R->yld);
                         R->be->mass_fr(R->be->ctx, d, R->sm, m, R->x, llen, R->xfr, m->dens);
                     }
                     if (d->NE_SH > 0) {
@@ -916,12 +935,14 @@ static int run_newmark(run_t *R)
                         R->be->forces_tr(R->be->ctx, d, R->f_temp, R->ef_i, m->emod, m->carea, llen,
                                          R->defllen_i, m->yield, R->c1_i, R->c2_i, R->c3_i,
                                          m->mcode, R->d_temp);
-                    int frc_fr = 0, frc_sh = 0;
+                    int frc_sh = 0;
+                    frc_fr = 0;
                     if (d->NE_FR > 0)
                         frc_fr = R->be->forces_fr(R->be->ctx, d, R->f_temp, R->ef_ip, R->ef_i,
                                                   R->efFE_ip, R->efFE_i, R->dd, m, llen,
                                                   R->defllen_ip, R->c1_ip, R->c2_ip, R->c3_ip,
-                                                  R->c1_i, R->c2_i, R->c3_i, &dlpf, itecnt);
+                                                  R->c1_i, R->c2_i, R->c3_i, R->yld, &dlpf,
+                                                  itecnt);
                     if (d->NE_SH > 0)
                         frc_sh = R->be->forces_sh(R->be->ctx, d, R->f_temp, R->ef_ip, R->ef_i,
                                                   R->dd, R->x_temp, R->x_ip, m->emod, m->nu,
@@ -947,14 +968,17 @@ static int run_newmark(run_t *R)
                     if (R->out) R->out->n_newton++;
                     itecnt++;
                 } while (convchk != 0 && frcchk == 0 && itecnt <= m->itemax);
-                if ((convchk != 0 || frcchk) && subcnt <= m->submax) {
+                if (frc_fr == 2) {
+                    dlpf = dlpfp;                         /* main.c:3773-3774 */
+                    R->ev[ORC_EV_UNLOAD]++;
+                } else if ((convchk != 0 || frcchk) && subcnt <= m->submax) {
                     if (lpf == m->lpfmax) break;          /* main.c:3783-3788 */
                     if (dlpfp == m->dlpfmin) break;
-                    dlpf = dlpfp / 2;
+                    if (frc_fr != 1) dlpf = dlpfp / 2;
                     if (dlpf < m->dlpfmin) dlpf = m->dlpfmin;
                     lpf = lpf - dlpfp + dlpf;
                     subcnt++;
-                    R->ev[ORC_EV_HALVE]++;
+                    R->ev[frc_fr == 1 ? ORC_EV_YIELD : ORC_EV_HALVE]++;
                     solcnt = 0;
                 } else if (subcnt > m->submax) {
                     break;
@@ -1028,7 +1052,7 @@ static int run_linear_static(run_t *R)
     if (d->NE_FR > 0) {
         double dl = 0;
         R->be->forces_fr(R->be->ctx, d, R->f, R->ef, R->ef, R->efFE, R->efFE, R->d, m, R->llen_ref,
-                         R->defllen, R->c1, R->c2, R->c3, R->c1, R->c2, R->c3, &dl, 0);
+                         R->defllen, R->c1, R->c2, R->c3, R->c1, R->c2, R->c3, R->yld, &dl, 0);
     }
     if (d->NE_SH > 0)
         R->be->forces_sh(R->be->ctx, d, R->f, R->ef, R->ef, R->d, R->x, R->x, m->emod, m->nu,
@@ -1163,14 +1187,17 @@ int orc_time_sample(orc_model *m, double factor_budget_s, orc_sample *out)
     if (!ss || !f || !dd || !x || !xip || !efp || !efi || !c1 || !c2 || !c3 || !da || !ds || !dl || !xfr) return 1;
     memcpy(x, m->x, sizeof(double) * (size_t)nj * 3);
     double t0 = now_s();
-    if (d->NE_TR > 0) orc_stiff_tr(d, ss, m->emod, m->carea, m->llength, m->llength, m->c1, m->c2, m->c3, efp, m->maxa, m->mcode);
+    if (d->NE_TR > 0)
+        orc_stiff_tr(d, ss, m->emod, m->carea, m->llength, m->llength, m->yield, m->c1, m->c2, m->c3,
+                     efp, m->maxa, m->mcode);
     if (nsh > 0)
         orc_stiff_sh(d, ss, m->emod, m->nu, m->x, m->xlocal, m->thick, m->farea, m->farea, m->slength,
                      m->c1, m->c2, m->c3, m->maxa, m->minc, m->mcode);
     out->t_tangent = now_s() - t0;
     t0 = now_s();
     orc_updatc(d, x, xip, xfr, dd, dl, da, ds, m->offset, m->osflag, m->auxpt, c1, c2, c3, m->minc, m->jcode);
-    if (d->NE_TR > 0) orc_forces_tr(d, f, efi, m->emod, m->carea, m->llength, dl, c1, c2, c3, m->mcode);
+    if (d->NE_TR > 0)
+        orc_forces_tr(d, f, efi, m->emod, m->carea, m->llength, dl, m->yield, c1, c2, c3, m->mcode);
     if (nsh > 0)
         orc_forces_sh(d, f, efp, efi, dd, x, m->emod, m->nu, m->xlocal, m->thick, m->farea, m->slength,
                       m->c1, m->c2, m->c3, c1, c2, c3, m->minc, m->mcode);

@@ -53,11 +53,16 @@ static void skyline_add(double *ss, const long *maxa, const long *mc, int n, con
 /* ---------------------------------------------------------------- truss --- */
 
 /* truss.c:82-204 (ANAFLAG 2): k = EA L^2/L0^3 [1 -1;-1 1] rotated to 6x6, plus
- * the geometric term N/L on the translation diagonal blocks. */
+ * the geometric term N/L on the translation diagonal blocks.  ANAFLAG 3: the
+ * same k less its plastic part when (N/Py)^2 > 1 + tol (stiffm_tr,
+ * truss.c:206-229: k - (k g)(k g)^T / g^T k g with g = 2 (N1, N2) / Py^2),
+ * and the geometric term only on a bar at its squash load,
+ * (N/Py)^2 >= 1 - tol (truss.c:155-166). */
+#define TR_PHITOL 1e-4
 void orc_stiff_tr(const orc_dims *d, double *ss, const double *emod, const double *carea,
-                  const double *llength, const double *defllen_ip, const double *c1_ip,
-                  const double *c2_ip, const double *c3_ip, const double *ef_ip,
-                  const long *maxa, const long *mcode)
+                  const double *llength, const double *defllen_ip, const double *yield,
+                  const double *c1_ip, const double *c2_ip, const double *c3_ip,
+                  const double *ef_ip, const long *maxa, const long *mcode)
 {
     for (long n = 0; n < d->NE_TR; ++n) {
         double ka;
@@ -65,20 +70,38 @@ void orc_stiff_tr(const orc_dims *d, double *ss, const double *emod, const doubl
             ka = emod[n] * carea[n] / llength[n];
         else
             ka = emod[n] * carea[n] * pow(defllen_ip[n], 2) / pow(llength[n], 3);
+        double k2[2][2] = {{ka, -ka}, {-ka, ka}};
+        double Py = 0;
+        if (d->ANAFLAG == 3) {
+            Py = carea[n] * yield[n];
+            double r = ef_ip[n * 2] / Py;
+            if (r * r > 1 + TR_PHITOL) {
+                double g0 = 2 * ef_ip[n * 2] / (Py * Py), g1 = 2 * ef_ip[n * 2 + 1] / (Py * Py);
+                double u0 = k2[0][0] * g0 + k2[0][1] * g1, u1 = k2[1][0] * g0 + k2[1][1] * g1;
+                double gkg = u0 * g0 + u1 * g1;
+                double r00 = u0 * u0 / gkg, r01 = u0 * u1 / gkg, r11 = u1 * u1 / gkg;
+                k2[0][0] -= r00; k2[0][1] -= r01; k2[1][0] -= r01; k2[1][1] -= r11;
+            }
+        }
         double c[3] = {c1_ip[n], c2_ip[n], c3_ip[n]};
         double K[36];
-        /* T^T k T with T = [c 0; 0 c], k = ka [1 -1; -1 1]; the two-term sums
-           keep the order of truss.c:133-150 */
+        /* T^T k T with T = [c 0; 0 c]; the two-term sums keep the order of
+           truss.c:133-150 */
         for (int a = 0; a < 6; ++a) {
             double ta0 = a < 3 ? c[a] : 0.0, ta1 = a < 3 ? 0.0 : c[a - 3];
-            double tk0 = ta0 * ka + ta1 * (-ka);
-            double tk1 = ta0 * (-ka) + ta1 * ka;
+            double tk0 = ta0 * k2[0][0] + ta1 * k2[1][0];
+            double tk1 = ta0 * k2[0][1] + ta1 * k2[1][1];
             for (int b = 0; b < 6; ++b) {
                 double t0b = (b < 3) ? c[b] : 0.0, t1b = (b < 3) ? 0.0 : c[b - 3];
                 K[a * 6 + b] = tk0 * t0b + tk1 * t1b;
             }
         }
-        if (d->ANAFLAG == 2) {
+        int geo = d->ANAFLAG == 2;
+        if (d->ANAFLAG == 3) {
+            double r = ef_ip[n * 2] / Py;
+            geo = r * r >= 1 - TR_PHITOL;
+        }
+        if (geo) {
             double g = ef_ip[n * 2] / defllen_ip[n];
             for (int a = 0; a < 6; ++a) K[a * 6 + a] += g;
             for (int a = 0; a < 3; ++a) {
@@ -90,15 +113,21 @@ void orc_stiff_tr(const orc_dims *d, double *ss, const double *emod, const doubl
     }
 }
 
-/* truss.c:327-376: N = EA (eps + eps^2/2) L/L0, f -= N c at end 1, += at end 2. */
+/* truss.c:327-376: N = EA (eps + eps^2/2) L/L0, f -= N c at end 1, += at end 2;
+ * under ANAFLAG 3, N at or beyond the squash load ((N/Py)^2 >= 1 - tol) is
+ * set to +-Py (truss.c:336-347): the strain keeps no plastic part. */
 void orc_forces_tr(const orc_dims *d, double *f, double *ef_i, const double *emod,
                    const double *carea, const double *llength, const double *defllen_i,
-                   const double *c1_i, const double *c2_i, const double *c3_i,
-                   const long *mcode)
+                   const double *yield, const double *c1_i, const double *c2_i,
+                   const double *c3_i, const long *mcode)
 {
     for (long n = 0; n < d->NE_TR; ++n) {
         double eps = (defllen_i[n] - llength[n]) / llength[n];
         double N = emod[n] * carea[n] * (eps + 0.5 * pow(eps, 2)) * defllen_i[n] / llength[n];
+        if (d->ANAFLAG == 3) {
+            double Py = yield[n] * carea[n], r = N / Py;
+            if (r * r >= 1 - TR_PHITOL) N = N < 0 ? -Py : Py;
+        }
         ef_i[n * 2] = N;
         ef_i[n * 2 + 1] = -N;
         double c[3] = {c1_i[n], c2_i[n], c3_i[n]};

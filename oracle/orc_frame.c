@@ -1,6 +1,7 @@
 /*
- * orc_frame.c -- oracle: 3-D corotational frame element (elastic, ANAFLAG 2)
- * and the lumped mass routines of truss, frame and shell.
+ * orc_frame.c -- oracle: 3-D corotational frame element (elastic, ANAFLAG 2;
+ * plastic hinges at the member ends, ANAFLAG 3) and the lumped mass routines
+ * of truss, frame and shell.
  *
  * TEST INFRASTRUCTURE ONLY (see cb_oracle.h).
  *
@@ -190,31 +191,218 @@ static void matvec14(const double A[14][14], const double *x, double *y, int tra
     }
 }
 
-/* Element tangent of frame n in the local frame: elastic + geometric. */
+/* ------------------------------------------------------ plastic hinges --- */
+#define PHITOL 1e-4   /* allowed deviation of phi from 1 (frame.c:37) */
+
+double orc_orbison(double p, double my, double mz)
+{
+    double p2 = p * p, y2 = my * my, z2 = mz * mz;
+    return p2 + z2 + y2 * y2 + 3.5 * p2 * z2 + 3 * (p2 * p2 * p2) * y2 + 4.5 * (z2 * z2) * y2;
+}
+
+/* d phi / d(P, My, Mz): the partial derivatives of orc_orbison in p, my, mz
+ * over the normalising forces */
+void orc_orbison_grad(double p, double my, double mz, double Py, double Mpy, double Mpz,
+                      double g[3])
+{
+    double p2 = p * p, y2 = my * my, z2 = mz * mz;
+    g[0] = (2 * p + 7 * p * z2 + 18 * (p2 * p2 * p) * y2) / Py;
+    g[1] = (4 * y2 * my + 6 * (p2 * p2 * p2) * my + 9 * (z2 * z2) * my) / Mpy;
+    g[2] = (2 * mz + 7 * p2 * mz + 18 * (z2 * mz) * y2) / Mpz;
+}
+
+/* phi at both ends of total end forces f (P at 7a, My at 7a+4, Mz at 7a+5) */
+static void end_phis(const double f[14], double Py, double Mpy, double Mpz, double phi[2])
+{
+    for (int a = 0; a < 2; ++a)
+        phi[a] = orc_orbison(f[7 * a] / Py, f[7 * a + 4] / Mpy, f[7 * a + 5] / Mpz);
+}
+
+/* the yield-gradient column of end a: nonzero at P, My, Mz of that end */
+static void end_column(const double f[14], int a, double Py, double Mpy, double Mpz, double col[14])
+{
+    double g[3];
+    memset(col, 0, sizeof(double) * 14);
+    orc_orbison_grad(f[7 * a] / Py, f[7 * a + 4] / Mpy, f[7 * a + 5] / Mpz, Py, Mpy, Mpz, g);
+    col[7 * a] = g[0];
+    col[7 * a + 4] = g[1];
+    col[7 * a + 5] = g[2];
+}
+
+/* H^-1 of the 1x1 or 2x2 system H = G^T k G by the adjugate over the
+ * determinant, as frame.c:677-683 forms it (a zero column leaves det = 0) */
+static void small_inverse(double H[2][2], int nc)
+{
+    if (nc == 1) {
+        H[0][0] = 1 / H[0][0];
+        return;
+    }
+    double det = H[0][0] * H[1][1] - H[0][1] * H[1][0];
+    double h00 = H[0][0];
+    H[0][0] = H[1][1] / det;
+    H[1][1] = h00 / det;
+    H[0][1] *= -1 / det;
+    H[1][0] *= -1 / det;
+}
+
+/* k G (14 x nc) and G^T k G for the columns G */
+static void kg_products(const double k[14][14], const double G[2][14], int nc, double kG[2][14],
+                        double H[2][2])
+{
+    for (int a = 0; a < nc; ++a)
+        for (int i = 0; i < 14; ++i) {
+            double s = 0;
+            for (int j = 0; j < 14; ++j) s += k[i][j] * G[a][j];
+            kG[a][i] = s;
+        }
+    for (int a = 0; a < nc; ++a)
+        for (int b = 0; b < nc; ++b) {
+            double s = 0;
+            for (int j = 0; j < 14; ++j) s += kG[a][j] * G[b][j];
+            H[a][b] = s;
+        }
+}
+
+/* stiffm_fr (frame.c:580-797).  Both ends on the surface: both columns, the
+ * column of an end flagged 2 left zero; otherwise the one end on the surface
+ * that is not flagged 2; otherwise no change. */
+void orc_frame_plastic(double k[14][14], const double eftot[14], const int yld[2], double Py,
+                       double Mpy, double Mpz)
+{
+    double phi[2], G[2][14], kG[2][14], H[2][2];
+    end_phis(eftot, Py, Mpy, Mpz, phi);
+    int nc = 0;
+    if (phi[0] >= 1 - PHITOL && phi[1] >= 1 - PHITOL) {
+        for (int a = 0; a < 2; ++a) {
+            if (yld[a] != 2) end_column(eftot, a, Py, Mpy, Mpz, G[a]);
+            else memset(G[a], 0, sizeof G[a]);
+        }
+        nc = 2;
+    } else if (phi[0] >= 1 - PHITOL && yld[0] != 2) {
+        end_column(eftot, 0, Py, Mpy, Mpz, G[0]);
+        nc = 1;
+    } else if (phi[1] >= 1 - PHITOL && yld[1] != 2) {
+        end_column(eftot, 1, Py, Mpy, Mpz, G[0]);
+        nc = 1;
+    }
+    if (!nc) return;
+    kg_products(k, G, nc, kG, H);
+    small_inverse(H, nc);
+    for (int i = 0; i < 14; ++i)
+        for (int j = 0; j < 14; ++j) {
+            double s = 0;
+            for (int a = 0; a < nc; ++a)
+                for (int b = 0; b < nc; ++b) s += kG[a][i] * H[a][b] * kG[b][j];
+            k[i][j] -= s;
+        }
+}
+
+/* False position for phi(tau) = 1 on [0, 1] of the end forces (p, my, mz) +
+ * tau (dp, dmy, dmz): the estimate from the two end values, then one further
+ * step on the sub-interval whose ends straddle 1 (frame.c:1397-1455; the
+ * reference's loop test phi <= 1 - tol && phi >= 1 + tol never holds, so its
+ * body runs once). */
+double orc_regula_falsi(double p, double dp, double my, double dmy, double mz, double dmz)
+{
+    double tu = 1, tl = 0;
+    double fu = orc_orbison(p + tu * dp, my + tu * dmy, mz + tu * dmz);
+    double fl = orc_orbison(p + tl * dp, my + tl * dmy, mz + tl * dmz);
+    double tr = tu - (fu - 1) * (tl - tu) / (fl - fu);
+    double fr = orc_orbison(p + tr * dp, my + tr * dmy, mz + tr * dmz);
+    if ((fl - 1 > 0 && fr - 1 > 0) || (fl - 1 < 0 && fr - 1 < 0)) {
+        tl = tr;
+        fl = fr;
+    } else {
+        tu = tr;
+        fu = fr;
+    }
+    return tu - (fu - 1) * (tl - tu) / (fl - fu);
+}
+
+/* unload (frame.c:1457-1658): the plastic multipliers lambda = (G^T k G)^-1
+ * G^T k dl of the ends on the surface (gradients at the new end forces,
+ * k = elastic + geometric); an end unloads when its lambda < -1e-8. */
+int orc_frame_unload(const double phi[2], const double eftot_i[14], double Py, double Mpy,
+                     double Mpz, const double k[14][14], const double dl[14])
+{
+    double G[2][14], kG[2][14], H[2][2], lam[2];
+    int nc = 0, which = 0;
+    if (phi[0] >= 1 - PHITOL && phi[1] >= 1 - PHITOL) {
+        end_column(eftot_i, 0, Py, Mpy, Mpz, G[0]);
+        end_column(eftot_i, 1, Py, Mpy, Mpz, G[1]);
+        nc = 2;
+    } else if (phi[0] >= 1 - PHITOL) {
+        end_column(eftot_i, 0, Py, Mpy, Mpz, G[0]);
+        nc = 1;
+        which = 0;
+    } else if (phi[1] >= 1 - PHITOL) {
+        end_column(eftot_i, 1, Py, Mpy, Mpz, G[0]);
+        nc = 1;
+        which = 1;
+    }
+    if (!nc) return 0;
+    kg_products(k, G, nc, kG, H);     /* k symmetric: row a of G^T k is kG[a] */
+    small_inverse(H, nc);
+    double w[2];
+    for (int a = 0; a < nc; ++a) {
+        double s = 0;
+        for (int j = 0; j < 14; ++j) s += kG[a][j] * dl[j];
+        w[a] = s;
+    }
+    for (int a = 0; a < nc; ++a) {
+        double s = 0;
+        for (int b = 0; b < nc; ++b) s += H[a][b] * w[b];
+        lam[a] = s;
+    }
+    if (nc == 2) {
+        if (lam[0] < -1e-8 && lam[1] < -1e-8) return 1;
+        if (lam[0] < -1e-8) return 2;
+        if (lam[1] < -1e-8) return 3;
+        return 0;
+    }
+    return lam[0] < -1e-8 ? 2 + which : 0;
+}
+
+static void frame_ke_of(const orc_model *m, long n, const double *llength, double k[14][14])
+{
+    long ne = m->dm.NE_TR + n;
+    orc_frame_ke(m->emod[ne], m->gmod[n], m->carea[ne], llength[ne], m->istrong[n], m->iweak[n],
+                 m->ipolar[n], m->iwarp[n], k);
+}
+
+/* Element tangent of frame n in the local frame: elastic + geometric, and
+ * under ANAFLAG 3 the plastic reduction unless both ends are flagged 2
+ * (frame.c:266-277).  eftot: the member's total end forces (ef + efFE). */
 static void frame_local_tangent(const orc_model *m, long n, const double *llength,
-                                const double *defllen_ip, const double *ef_ip,
-                                const double *efFE_ip, double k[14][14])
+                                const double *defllen_ip, const double *eftot, const int *yld,
+                                double k[14][14])
 {
     const orc_dims *d = &m->dm;
     long ne = d->NE_TR + n;
-    orc_frame_ke(m->emod[ne], m->gmod[n], m->carea[ne], llength[ne], m->istrong[n], m->iweak[n],
-                 m->ipolar[n], m->iwarp[n], k);
-    if (d->ANAFLAG == 2) {
-        double ft[14];
-        for (int i = 0; i < 14; ++i)
-            ft[i] = ef_ip[d->NE_TR * 2 + n * 14 + i] + efFE_ip[n * 14 + i];
-        orc_frame_kg(ft, defllen_ip[ne], m->carea[ne], m->ipolar[n], k);
-    }
+    frame_ke_of(m, n, llength, k);
+    if (d->ANAFLAG == 2 || d->ANAFLAG == 3)
+        orc_frame_kg(eftot, defllen_ip[ne], m->carea[ne], m->ipolar[n], k);
+    if (d->ANAFLAG == 3 && (yld[n * 2] != 2 || yld[n * 2 + 1] != 2))
+        orc_frame_plastic(k, eftot, yld + n * 2, m->carea[ne] * m->yield[ne],
+                          m->zweak[n] * m->yield[ne], m->zstrong[n] * m->yield[ne]);
+}
+
+static void total_forces(const orc_dims *d, long n, const double *ef, const double *efFE,
+                         double ft[14])
+{
+    for (int i = 0; i < 14; ++i) ft[i] = ef[d->NE_TR * 2 + n * 14 + i] + efFE[n * 14 + i];
 }
 
 /* stiff_fr (frame.c:226-349): K = G^T T^T k T G accumulated into the skyline. */
 void orc_stiff_fr(const orc_dims *d, double *ss, const orc_model *m, const double *llength,
                   const double *defllen_ip, const double *c1_ip, const double *c2_ip,
-                  const double *c3_ip, const double *ef_ip, const double *efFE_ip)
+                  const double *c3_ip, const double *ef_ip, const double *efFE_ip,
+                  const int *yldflag)
 {
-    double k[14][14], T[14][14], K[14][14], G[14][14], K2[14][14];
+    double k[14][14], T[14][14], K[14][14], G[14][14], K2[14][14], ft[14];
     for (long n = 0; n < d->NE_FR; ++n) {
-        frame_local_tangent(m, n, llength, defllen_ip, ef_ip, efFE_ip, k);
+        total_forces(d, n, ef_ip, efFE_ip, ft);
+        frame_local_tangent(m, n, llength, defllen_ip, ft, yldflag, k);
         long t = d->NE_TR + n * 3;
         frame_T(c1_ip + t, c2_ip + t, c3_ip + t, T);
         congruence(T, k, K);
@@ -227,23 +415,34 @@ void orc_stiff_fr(const orc_dims *d, double *ss, const orc_model *m, const doubl
     }
 }
 
-/* forces_fr, elastic branch (frame.c:902-1096, 1273-1309):
+/* forces_fr (frame.c:902-1096, 1153-1309):
  *   dd_loc = T_ip G DD12;  def = k(ef_ip + efFE_ip) dd_loc
  *   ef_i   = T_i T_ip^T (ef_ip + def);  efFE_i = T_i T_ip^T efFE_ip
  *   f     += G^T T_i^T ef_i
  * The fixed-end forces carry no distributed load on this path, so the
- * first-iteration dlpf * efFE_ref term is zero. */
+ * first-iteration dlpf * efFE_ref term of every yldflag branch
+ * (frame.c:1098-1151) is zero.
+ * ANAFLAG 3, between ef_i and the scatter, with phi at the new total end
+ * forces: an end beyond the surface (phi > 1 + tol, the larger of the two,
+ * not flagged 2) scales *dlpf by the fraction of this iteration's end-force
+ * increment that reaches phi = 1, is flagged 1 and returns 1; otherwise the
+ * ends on the surface and not flagged 2 are flagged 1, and when an end is
+ * flagged 1 the unloading test of the elastic + geometric k against dd_loc
+ * flags the unloading ends 2 and returns 2.  A return leaves the member's ef_i
+ * written, its forces out of f, and the members after it untouched. */
 int orc_forces_fr(const orc_dims *d, double *f, const double *ef_ip, double *ef_i,
                   const double *efFE_ip, double *efFE_i, const double *dd,
                   const orc_model *m, const double *llength, const double *defllen_ip,
                   const double *c1_ip, const double *c2_ip, const double *c3_ip,
-                  const double *c1_i, const double *c2_i, const double *c3_i)
+                  const double *c1_i, const double *c2_i, const double *c3_i,
+                  int *yldflag, double *dlpf)
 {
     double k[14][14], Tp[14][14], Ti[14][14], G[14][14], Q[14][14];
-    double D[14], Dj[14], dl[14], def[14], tmp[14], eg[14], eg2[14];
+    double D[14], Dj[14], dl[14], def[14], tmp[14], eg[14], eg2[14], fp[14], fi[14];
     for (long n = 0; n < d->NE_FR; ++n) {
         const long *mc = m->mcode + d->NE_TR * 6 + n * 14;
-        frame_local_tangent(m, n, llength, defllen_ip, ef_ip, efFE_ip, k);
+        total_forces(d, n, ef_ip, efFE_ip, fp);
+        frame_local_tangent(m, n, llength, defllen_ip, fp, yldflag, k);
         for (int i = 0; i < 14; ++i) D[i] = mc[i] ? dd[mc[i] - 1] : 0.0;
         long t = d->NE_TR + n * 3;
         frame_T(c1_ip + t, c2_ip + t, c3_ip + t, Tp);
@@ -267,6 +466,41 @@ int orc_forces_fr(const orc_dims *d, double *f, const double *ef_ip, double *ef_
         for (int i = 0; i < 14; ++i) tmp[i] = ep[i] + def[i];
         matvec14(Q, tmp, ei, 0);
         matvec14(Q, efFE_ip + n * 14, efFE_i + n * 14, 0);
+        if (d->ANAFLAG == 3) {
+            long ne = d->NE_TR + n;
+            double Py = m->carea[ne] * m->yield[ne], Mpy = m->zweak[n] * m->yield[ne];
+            double Mpz = m->zstrong[n] * m->yield[ne], phi[2];
+            int *y = yldflag + n * 2;
+            total_forces(d, n, ef_i, efFE_i, fi);
+            end_phis(fi, Py, Mpy, Mpz, phi);
+            for (int a = 0; a < 2; ++a) {
+                int o = 1 - a;
+                if (phi[a] > phi[o] && phi[a] > 1 + PHITOL && y[a] != 2) {
+                    *dlpf *= orc_regula_falsi(fp[7 * a] / Py, (fi[7 * a] - fp[7 * a]) / Py,
+                                              fp[7 * a + 4] / Mpy,
+                                              (fi[7 * a + 4] - fp[7 * a + 4]) / Mpy,
+                                              fp[7 * a + 5] / Mpz,
+                                              (fi[7 * a + 5] - fp[7 * a + 5]) / Mpz);
+                    y[a] = 1;
+                    return 1;
+                }
+            }
+            int on0 = phi[0] >= 1 - PHITOL && y[0] != 2, on1 = phi[1] >= 1 - PHITOL && y[1] != 2;
+            if (on0 && on1) y[0] = y[1] = 1;
+            else if (on0) y[0] = 1;
+            else if (on1) y[1] = 1;
+            if (y[0] == 1 || y[1] == 1) {
+                double ke[14][14];
+                frame_ke_of(m, n, llength, ke);
+                orc_frame_kg(fp, defllen_ip[ne], m->carea[ne], m->ipolar[n], ke);
+                int u = orc_frame_unload(phi, fi, Py, Mpy, Mpz, ke, dl);
+                if (u) {
+                    if (u == 1 || u == 2) y[0] = 2;
+                    if (u == 1 || u == 3) y[1] = 2;
+                    return 2;
+                }
+            }
+        }
         matvec14(Ti, ei, eg, 1);
         if (m->osflag[n]) {
             matvec14(G, eg, eg2, 1);

@@ -348,11 +348,15 @@ orc_model *orc_open(const char *path, char *err, int errlen)
     if (t.bad) FAIL("truncated header");
     if (nsbr || nfbr) FAIL("brick elements are not on this path");
     if (d->OPTFLAG != 1 && d->OPTFLAG != 2) FAIL("OPTFLAG must be 1 or 2");
-    if (d->ANAFLAG != 2 && d->ANAFLAG != 1) FAIL("ANAFLAG %d not supported (elastic only)", d->ANAFLAG);
+    if (d->ANAFLAG < 1 || d->ANAFLAG > 3) FAIL("ANAFLAG %d not supported", d->ANAFLAG);
     if (d->ALGFLAG < 1 || d->ALGFLAG > 5) FAIL("ALGFLAG %d not supported", d->ALGFLAG);
     if (d->ANAFLAG == 1 && (d->ALGFLAG == 3 || d->ALGFLAG == 5))
         FAIL("linear analysis (ANAFLAG 1) runs under ALGFLAG 1, 2 (static) or 4 (linear Newmark)");
+    if (d->ALGFLAG == 4 && d->ANAFLAG != 1)
+        FAIL("invalid entry for analysis type: ALGFLAG 4 (linear Newmark) takes ANAFLAG 1");
     if (d->SLVFLAG != 0) FAIL("only SLVFLAG 0 (skyline) is restated");
+    if (d->ANAFLAG == 3 && d->NE_SH > 0)
+        FAIL("ANAFLAG 3 shells (Ivanov's yield criterion, shell.c:842-1503) are not restated");
     long nj = d->NJ, ntr = d->NE_TR, nfr = d->NE_FR, nsh = d->NE_SH;
 
     m->minc = zalloc((size_t)(ntr * 2 + nfr * 2 + nsh * 3), sizeof(long));

@@ -139,12 +139,13 @@ static int *zeros_int(long n)
 static void w_stiff_fr(void *ctx, const orc_dims *d, double *ss, const orc_model *m,
                        const double *llength, const double *defllen_ip, const double *c1_ip,
                        const double *c2_ip, const double *c3_ip, const double *ef_ip,
-                       const double *efFE_ip)
+                       const double *efFE_ip, int *yldflag)
 {
     (void)ctx;
     ref_set_dims(d);
     stiff_fr(ss, m->emod, m->gmod, m->carea, m->offset, m->osflag, U(llength), U(defllen_ip),
-             m->istrong, m->iweak, m->ipolar, m->iwarp, zeros_int(d->NE_FR * 2 + 2), m->yield,
+             m->istrong, m->iweak, m->ipolar, m->iwarp,
+             yldflag ? yldflag : zeros_int(d->NE_FR * 2 + 2), m->yield,
              m->zstrong, m->zweak, U(c1_ip), U(c2_ip), U(c3_ip), U(ef_ip), U(efFE_ip),
              m->mendrel, m->maxa, m->mcode);
 }
@@ -154,13 +155,13 @@ static int w_forces_fr(void *ctx, const orc_dims *d, double *f, double *ef_ip, d
                        const orc_model *m, const double *llength, const double *defllen_ip,
                        const double *c1_ip, const double *c2_ip, const double *c3_ip,
                        const double *c1_i, const double *c2_i, const double *c3_i,
-                       double *dlpf, int itecnt)
+                       int *yldflag, double *dlpf, int itecnt)
 {
     (void)ctx;
     ref_set_dims(d);
     int it = itecnt;
     return forces_fr(f, ef_ip, ef_i, zeros(d->NE_FR * 14 + 1), U(efFE_ip), efFE_i,
-                     zeros_int(d->NE_FR * 2 + 2), U(dd), m->emod, m->gmod, m->carea, m->offset,
+                     yldflag ? yldflag : zeros_int(d->NE_FR * 2 + 2), U(dd), m->emod, m->gmod, m->carea, m->offset,
                      m->osflag, U(llength), U(defllen_ip), m->istrong, m->iweak, m->ipolar,
                      m->iwarp, m->yield, m->zstrong, m->zweak, U(c1_ip), U(c2_ip), U(c3_ip),
                      U(c1_i), U(c2_i), U(c3_i), m->mendrel, m->mcode, dlpf, &it);
