@@ -211,6 +211,8 @@ Batch::Batch(const std::vector<std::string>& paths) {
         const Model& m = models_[b];
         if (m.algflag != 5)
             throw std::runtime_error(paths[b] + ": a batch runs the nonlinear Newmark driver (ALGFLAG 5) only");
+        if (m.anaflag != 2)
+            throw std::runtime_error(paths[b] + ": a batch runs elastic members (ANAFLAG 2) only");
         if (m.ntr || m.nsh || !m.nfr) throw std::runtime_error(paths[b] + ": a batch takes frame-only models");
         for (int v : m.osflag)
             if (v) throw std::runtime_error(paths[b] + ": member-end offsets are not supported on the device");

@@ -66,6 +66,7 @@ struct cb_model {
     std::vector<std::vector<long>> step_kt_n;
     std::vector<std::array<int, cb::EV_N>> step_ev;
     std::vector<double> step_psi, step_res, step_load;
+    double last_tau = 1.0;   // regula-falsi fraction of the last force pass with code 1
 };
 
 struct cb_batch {
@@ -381,6 +382,34 @@ int cb_internal_force(cb_model* h, int gen_x, const double* dd_host, double* f_h
     CB_CHECK(hipStreamSynchronize(h->d.s));
     (void)hipFree(dd);
     (void)hipFree(f);
+    double tau = 1.0;
+    const int code = cb::frame_plastic_result(h->d, &tau);
+    h->last_tau = tau;
+    return code;
+    GUARD_END
+}
+
+int cb_get_yield_flags(cb_model* h, int* yld, double* tau) {
+    if (!h) return fail(-1, "null handle");
+    GUARD_BEGIN
+    if (yld && h->m.nfr) {
+        cb::d2h(yld, h->d.yld, size_t(h->m.nfr * 2), h->d.s);
+        CB_CHECK(hipStreamSynchronize(h->d.s));
+    }
+    if (tau) *tau = h->last_tau;
+    return 0;
+    GUARD_END
+}
+
+int cb_set_yield_flags(cb_model* h, const int* yld) {
+    if (!h || !yld) return fail(-1, "null argument");
+    GUARD_BEGIN
+    for (long i = 0; i < h->m.nfr * 2; ++i)
+        if (yld[i] < 0 || yld[i] > 2) return fail(-1, "yield flags are 0, 1 or 2");
+    if (h->m.nfr) {
+        cb::h2d(h->d.yld, yld, size_t(h->m.nfr * 2), h->d.s);
+        CB_CHECK(hipStreamSynchronize(h->d.s));
+    }
     return 0;
     GUARD_END
 }

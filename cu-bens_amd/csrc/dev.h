@@ -38,6 +38,15 @@ struct DeviceError {
 // _i (current iterate), _ip (previous iterate).
 enum Gen { G_BASE = 0, G_I = 1, G_IP = 2 };
 
+// Outcome of a frame force pass under ANAFLAG 3: the first member whose pass
+// returned (NE_FR when none did), its code (1 yield overshoot, 2 unloading)
+// and the regula-falsi fraction of a code 1.
+struct PlasticStatus {
+    int32_t first;
+    int32_t code;
+    double tau;
+};
+
 struct Dev {
     uint64_t uid = 0;   // unique per dev_create: caches keyed on a model use this, never its address
     hipStream_t s = nullptr;
@@ -59,6 +68,16 @@ struct Dev {
     double *gmod = nullptr, *istrong = nullptr, *iweak = nullptr, *ipolar = nullptr, *iwarp = nullptr;
     double *auxpt = nullptr;
     double *nu = nullptr, *thick = nullptr, *xlocal = nullptr, *farea = nullptr, *slength = nullptr;
+    // ANAFLAG 3: yield stresses [NE_TR + NE_FR + NE_SH], plastic moduli
+    // [NE_FR], yldflag (2 per member: 0 elastic, 1 yielded, 2 unloading), and
+    // the scratch of the frame force pass: each member's ef_i (14) and
+    // code / flags (4), its regula-falsi fraction, the pass's outcome
+    double *yield = nullptr, *zstrong = nullptr, *zweak = nullptr;
+    int32_t* yld = nullptr;
+    double *pl_efc = nullptr, *pl_tau = nullptr;
+    int32_t* pl_code = nullptr;
+    PlasticStatus* pl_st = nullptr;
+    PlasticStatus* pl_host = nullptr;   // pinned
     // state
     double *x[3] = {nullptr, nullptr, nullptr};  // [G_BASE]=x, [G_I]=x_temp, [G_IP]=x_ip
     double *c1[3] = {}, *c2[3] = {}, *c3[3] = {};
@@ -115,6 +134,11 @@ void launch_frame_tangent_gather(Dev& d, int g);
 void launch_frame_tangent_band(Dev& d, int g, double* tiles, int w, int* bad);
 void launch_frame_forces(Dev& d, double* f, const double* dd);
 void launch_frame_update(Dev& d, int gx);
+// ANAFLAG 3: the code of the last frame force pass (0, 1 or 2, as forces_fr
+// returns it) and the regula-falsi fraction of a code 1 (waits for the stream)
+int frame_plastic_result(Dev& d, double* tau);
+// ANAFLAG 3: flags 2 (unloading) back to 0 at a commit
+void launch_unloaded_elastic(Dev& d);
 
 // ---- results of a converged state (results.hip) --------------------------------
 // rx[NJ*7] <- the element forces T^T ef of generation g summed at every

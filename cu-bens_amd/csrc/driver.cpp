@@ -116,10 +116,14 @@ void Analysis::record(double lam, int ite, const double* dvec) {
     }
 }
 
-void Analysis::update_and_forces(int gx, double* fout, const double* ddv) {
+int Analysis::update_and_forces(int gx, double* fout, const double* ddv, double* dlpf) {
     Timer t(&st_->t_forces, d_.s);
     launch_updatc(d_, gx, ddv);
     launch_forces(d_, gx, fout, ddv);
+    double tau = 1;
+    const int code = frame_plastic_result(d_, &tau);
+    if (code == 1 && dlpf) *dlpf *= tau;
+    return code;
 }
 
 void Analysis::reset_from_base() {
@@ -127,7 +131,12 @@ void Analysis::reset_from_base() {
     copy_gen(d_, G_IP, G_BASE, true, false);
 }
 
-void Analysis::commit() { copy_gen(d_, G_BASE, G_I, true, true); }
+// base <- _i; an unloading member end (flag 2) is elastic again
+// (main.c:2107-2112, 3086-3091, 3887-3892)
+void Analysis::commit() {
+    copy_gen(d_, G_BASE, G_I, true, true);
+    launch_unloaded_elastic(d_);
+}
 
 // Factorization (or preconditioner set-up) of the tangent in d.K, pivots for
 // the arc-length decisions.  Returns 0 or a nonzero code on a singular factor.
@@ -330,6 +339,7 @@ int Analysis::run_nr() {
         dlamp = dlam;
         reset_from_base();
         itecnt_ = 0;
+        int frc = 0;
         do {
             vresidual(d_, r, lam, q, ft, n);
             const bool full = (m_.algflag == 1 || itecnt_ == 0);
@@ -345,7 +355,7 @@ int Analysis::run_nr() {
             vaxpby(d_, dt, 1.0, dd, 1.0, n);
             vcopy(d_, fip, ft, n);
             vfill(d_, ft, 0.0, n);
-            update_and_forces(G_I, ft, dd);
+            frc = update_and_forces(G_I, ft, dd, &dlam);
             d2d(d_.ef[G_IP], d_.ef[G_I], size_t(d_.ntr * 2 + d_.nfr * 14 + d_.nsh * 18), d_.s);
             double s[5];
             test_sums(d_, dd, dt, q, lam, ft, fp, fip, n, s);
@@ -360,17 +370,22 @@ int Analysis::run_nr() {
             shift_ip(true);
             itecnt_++;
             st_->newton++;
-        } while (convchk != 0 && itecnt_ <= m_.itemax);
+        } while (convchk != 0 && frc == 0 && itecnt_ <= m_.itemax);
 
-        if (convchk != 0 && subcnt <= m_.submax) {
+        if (frc == 2) {
+            // a yielded end unloads: the same increment again (main.c:2033-2034)
+            dlam = dlamp;
+            ev_[EV_UNLOAD]++;
+        } else if ((convchk != 0 || frc != 0) && subcnt <= m_.submax) {
             if (lam == m_.lpfmax) FAIL(1, "maximum load factor attempted without convergence");
             if (dlamp == m_.dlpfmin) FAIL(1, "minimum load factor increment reached");
-            dlam = dlamp / 2;
+            // a code 1 keeps the increment the regula falsi scaled (main.c:2050-2053)
+            if (frc != 1) dlam = dlamp / 2;
             if (dlam < m_.dlpfmin) dlam = m_.dlpfmin;
             lam = lam - dlamp + dlam;
             subcnt++;
             solcnt = 0;
-            ev_[EV_HALVE]++;
+            ev_[frc == 1 ? EV_YIELD : EV_HALVE]++;
         } else if (subcnt > m_.submax) {
             FAIL(1, "maximum number of subdivisions exceeded");
         } else {
@@ -429,9 +444,12 @@ int Analysis::run_msal() {
         intener1 = lam * s;
     }
     vfill(d_, f, 0.0, n);
-    update_and_forces(G_BASE, f, dd);
+    // the first call hands forces_fr the load factor itself as the increment
+    // a code 1 scales, and its code is dropped (main.c:2365-2369, 2411)
+    update_and_forces(G_BASE, f, dd, &lam);
     itecnt_ = 1;
     shift_ip(true);
+    int frc = 0;
     do {
         // qtot is formed before the load factor moves and test() sees that
         // qtot (main.c:2415-2485)
@@ -445,7 +463,7 @@ int Analysis::run_msal() {
         vaxpby(d_, dv, 1.0, dd, 1.0, n);
         vcopy(d_, fip, f, n);
         vfill(d_, f, 0.0, n);
-        update_and_forces(G_BASE, f, dd);
+        frc = update_and_forces(G_BASE, f, dd, &dlam);
         double s[5];
         test_sums(d_, dd, dv, q, lam_q, f, fp, fip, n, s);
         if (test_decide(s, intener1, m_.toldisp, m_.tolforc, m_.tolener, &convchk))
@@ -453,8 +471,8 @@ int Analysis::run_msal() {
         itecnt_++;
         st_->newton++;
         shift_ip(true);
-    } while (convchk != 0 && itecnt_ <= m_.itemax);
-    if (convchk != 0) FAIL(1, "initially prescribed displacement too large");
+    } while (convchk != 0 && frc == 0 && itecnt_ <= m_.itemax);
+    if (convchk != 0 || frc != 0) FAIL(1, "initially prescribed displacement too large");
     copy_gen(d_, G_BASE, G_I, true, false);
     record(lam, itecnt_, dv);
     if (long(lpf.size()) >= o_.max_steps) return 0;
@@ -511,7 +529,9 @@ int Analysis::run_msal() {
             intener1 = dlam * s;
         }
         vfill(d_, ft, 0.0, n);
-        update_and_forces(G_I, ft, dd);
+        // the predictor's code is not looked at; its flags stay, and a return
+        // leaves the later members out of f_temp (main.c:2794-2800)
+        update_and_forces(G_I, ft, dd, &dlam);
         shift_ip(true);
         itecnt_ = 1;
         st_->newton++;
@@ -554,7 +574,7 @@ int Analysis::run_msal() {
                 }
             }
             if (errchk2 == 0) {
-                update_and_forces(G_I, ft, dd);
+                frc = update_and_forces(G_I, ft, dd, &dlam);
                 double s[5];
                 test_sums(d_, dd, dt, q, lam_q, ft, fp, fip, n, s);
                 if (test_decide(s, intener1, m_.toldisp, m_.tolforc, m_.tolener, &convchk))
@@ -572,6 +592,16 @@ int Analysis::run_msal() {
                         errchk2 = 1;
                         ev_[EV_DNORM]++;
                     }
+                } else if (frc == 2) {
+                    vcopy(d_, ft, f, n);
+                    errchk2 = 1;
+                    ev_[EV_UNLOAD]++;
+                } else if (frc != 0 && subcnt <= m_.submax) {
+                    arc *= 0.5;
+                    vcopy(d_, ft, f, n);
+                    subcnt++;
+                    errchk2 = 1;
+                    ev_[EV_YIELD]++;
                 } else {
                     itecnt_++;
                     if (itecnt_ > m_.itemax) {
@@ -664,7 +694,7 @@ int Analysis::run_newmark() {
     const long ntst = long(m_.ttot / dtime + 1);
     if (ntst > m_.ntstpsinpt) FAIL(1, "time steps exceed the load history");
     double ddt = 1, sub_dt = 1, dt_temp = dtime;
-    int tsflag = 0, convchk = 1, itecnt_ = 0, det = 0;
+    int tsflag = 0, convchk = 1, itecnt_ = 0, det = 0, frc = 0;
     double intener1 = 0;
     long k = 0;
     const bool ckpt = !o_.checkpoint_path.empty();
@@ -726,7 +756,7 @@ int Analysis::run_newmark() {
                     linear_solve(tmp, dd);
                     A.flip = itecnt_ > 0;
                     newmark_vec(d_, NM_UPDATE, A);
-                    update_and_forces(G_I, ft, dd);
+                    frc = update_and_forces(G_I, ft, dd, &dlam);
                     A.out = dyn;
                     newmark_vec(d_, NM_DYN, A);
                     double s[5];
@@ -741,16 +771,19 @@ int Analysis::run_newmark() {
                     shift_ip(true);
                     itecnt_++;
                     st_->newton++;
-                } while (convchk != 0 && itecnt_ <= m_.itemax);
-                if (convchk != 0 && subcnt <= m_.submax) {
+                } while (convchk != 0 && frc == 0 && itecnt_ <= m_.itemax);
+                if (frc == 2) {
+                    dlam = dlamp;                       // main.c:3773-3774
+                    ev_[EV_UNLOAD]++;
+                } else if ((convchk != 0 || frc != 0) && subcnt <= m_.submax) {
                     if (lam == m_.lpfmax) break;          // main.c:3783-3788
                     if (dlamp == m_.dlpfmin) break;
-                    dlam = dlamp / 2;
+                    if (frc != 1) dlam = dlamp / 2;
                     if (dlam < m_.dlpfmin) dlam = m_.dlpfmin;
                     lam = lam - dlamp + dlam;
                     subcnt++;
                     solcnt = 0;
-                    ev_[EV_HALVE]++;
+                    ev_[frc == 1 ? EV_YIELD : EV_HALVE]++;
                 } else if (subcnt > m_.submax) {
                     break;
                 } else {
@@ -774,11 +807,15 @@ int Analysis::run_newmark() {
                 commit();
                 committed = true;
             }
-            if (ddt < 1) {
-                if (sub_dt <= 1) { sub_dt = sub_dt + ddt; tsflag = 1; }
-                if (sub_dt > 1) { ddt = ddt * 2; tsflag = 2; }
-            } else if (ddt == 1) {
-                tsflag = 2;
+            // the multiplier moves only after a force pass without a code
+            // (main.c:3922-3935)
+            if (frc == 0) {
+                if (ddt < 1) {
+                    if (sub_dt <= 1) { sub_dt = sub_dt + ddt; tsflag = 1; }
+                    if (sub_dt > 1) { ddt = ddt * 2; tsflag = 2; }
+                } else if (ddt == 1) {
+                    tsflag = 2;
+                }
             }
             // checkPoint at the commit (main.c:3904-3907), written after the
             // multiplier update, which changes no state, so that the file also
@@ -787,7 +824,7 @@ int Analysis::run_newmark() {
                 !save_checkpoint(k, ddt, A.a0 * (1 - am_), 1 - af_))
                 FAIL(1, "unable to write the checkpoint file");
         } while (ddt >= 0.0001 && tsflag != 2);
-        if (convchk != 0) FAIL(3, "minimum time increment interval reached without convergence");
+        if (convchk != 0 || frc != 0) FAIL(3, "minimum time increment interval reached without convergence");
         record(double(k) * dtime, itecnt_, dv);
         if (long(lpf.size()) >= o_.max_steps) return 0;
         ++k;
@@ -819,6 +856,7 @@ bool Analysis::save_checkpoint(long k, double ddt, double cnum, double cden) {
     d2h(c.llen_w.data(), d_.llen, c.llen_w.size(), d_.s);
     d2h(c.farea_w.data(), d_.farea, c.farea_w.size(), d_.s);
     d2h(c.slen_w.data(), d_.slength, c.slen_w.size(), d_.s);
+    d2h(c.yldflag.data(), d_.yld, c.yldflag.size(), d_.s);
     d2h(K.data(), d_.K, K.size(), d_.s);
     CB_CHECK(hipStreamSynchronize(d_.s));
     c.llength = c.llen_w;      // the frame lengths at the commit (main.c:3880)
@@ -873,6 +911,7 @@ int Analysis::load_checkpoint(long* tstep, double* ddt) {
     h2d(d_.defllen[G_BASE], c.defllen.data(), c.defllen.size(), d_.s);
     h2d(d_.deffarea[G_BASE], c.deffarea.data(), c.deffarea.size(), d_.s);
     h2d(d_.defslen[G_BASE], c.defslen.data(), c.defslen.size(), d_.s);
+    h2d(d_.yld, c.yldflag.data(), c.yldflag.size(), d_.s);
     // working lengths: the frames' from the file's llength, as a fresh start
     // takes them (main.c:3420; the reference's restart leaves them unset), the
     // rest as the model has them, unless the full-precision extension holds all

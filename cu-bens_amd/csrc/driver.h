@@ -26,10 +26,14 @@ enum SolverKind {
 // Driver events of one converged step: the pivot-sign flag of the step's last
 // factorization (det, solve.c:608-621), and how often within the step quad
 // found imaginary roots (arc.c:145) or no root with a positive gamma
-// (arc.c:134), the dnorm test restarted the increment (main.c:2936-2953), and
+// (arc.c:134), the dnorm test restarted the increment (main.c:2936-2953),
 // the arc, load step or time step was halved after itemax iterations
-// (main.c:3000-3012, 2096-2110, 3826-3846).
-enum { EV_DET = 0, EV_IMAG = 1, EV_NEG = 2, EV_DNORM = 3, EV_HALVE = 4, EV_N = 5 };
+// (main.c:3000-3012, 2096-2110, 3826-3846), and under ANAFLAG 3 how often
+// forces_fr's code 1 (a member end through the yield surface) and code 2 (a
+// yielded end unloading) restarted the step (main.c:2033-2057, 2954-2970,
+// 3773-3795).
+enum { EV_DET = 0, EV_IMAG = 1, EV_NEG = 2, EV_DNORM = 3, EV_HALVE = 4, EV_YIELD = 5, EV_UNLOAD = 6,
+       EV_N = 7 };
 
 struct RunOptions {
     int solver = SOLVER_BAND64;
@@ -139,7 +143,9 @@ private:
     // last iteration) to give the tangent the file holds
     bool save_checkpoint(long k, double ddt, double cnum, double cden);
     int load_checkpoint(long* tstep, double* ddt);   // 0 ok, 1 cannot open, 2 malformed
-    void update_and_forces(int gx, double* fout, const double* ddv);
+    // updatc + internal forces; returns forces_fr's code (ANAFLAG 3; else 0)
+    // and scales *dlpf by the regula-falsi fraction on a code 1
+    int update_and_forces(int gx, double* fout, const double* ddv, double* dlpf = nullptr);
     int factor_current(int* det);
     double psi_now() const;
     void shift_ip(bool with_ef) { copy_gen(d_, G_IP, G_I, with_ef, false); }

@@ -1,4 +1,5 @@
-// Corotational 3-D frame element (elastic + geometric, ANAFLAG 2) on the device.
+// Corotational 3-D frame element (elastic + geometric, ANAFLAG 2; plastic
+// hinges at the member ends, ANAFLAG 3) on the device.
 //
 // One thread per member.  The 14x14 local matrix is never stored: every entry
 // is a closed-form expression of ~40 per-member scalars (stiffe_fr /
@@ -131,6 +132,122 @@ __device__ __forceinline__ double kf(const FrameCoef& k, int i, int j) {
 // local index of slot s (0..5) of end a
 __device__ __forceinline__ constexpr int lidx(int a, int s) { return 7 * a + s; }
 
+// ---------------------------------------------------------- plastic hinges
+// Orbison's yield function of the end forces over the squash load and the
+// plastic moments (frame.c:613-620) and its gradient in the end forces
+// (P, My, Mz) at local DOFs 7a, 7a+4, 7a+5 (frame.c:626-637).
+constexpr double PHITOL = 1e-4;
+
+__device__ inline double orbison(double p, double my, double mz) {
+    const double p2 = p * p, y2 = my * my, z2 = mz * mz;
+    return p2 + z2 + y2 * y2 + 3.5 * p2 * z2 + 3 * (p2 * p2 * p2) * y2 + 4.5 * (z2 * z2) * y2;
+}
+
+struct YieldRef {
+    double Py, Mpy, Mpz;
+};
+
+__device__ inline double end_phi(const double* f, int a, const YieldRef& y) {
+    return orbison(f[7 * a] / y.Py, f[7 * a + 4] / y.Mpy, f[7 * a + 5] / y.Mpz);
+}
+
+__device__ inline void end_grad(const double* f, int a, const YieldRef& y, double g[3]) {
+    const double p = f[7 * a] / y.Py, my = f[7 * a + 4] / y.Mpy, mz = f[7 * a + 5] / y.Mpz;
+    const double p2 = p * p, y2 = my * my, z2 = mz * mz;
+    g[0] = (2 * p + 7 * p * z2 + 18 * (p2 * p2 * p) * y2) / y.Py;
+    g[1] = (4 * y2 * my + 6 * (p2 * p2 * p2) * my + 9 * (z2 * z2) * my) / y.Mpy;
+    g[2] = (2 * mz + 7 * p2 * mz + 18 * (z2 * mz) * y2) / y.Mpz;
+}
+
+// The plastic reduction k_p = k - U H U^T of a member's local tangent, U = k G
+// for up to two yield-gradient columns G (one per end, zero where unused),
+// H = (G^T k G)^-1 by the adjugate over the determinant as frame.c:677-683
+// forms it (a zero column with the other in use leaves det = 0).
+struct FramePl {
+    double u[2][14];
+    double h[2][2];
+};
+
+template <int A>
+__device__ inline void pl_column(const FrameCoef& k, const double g[3], double u[14]) {
+#pragma unroll
+    for (int i = 0; i < 14; ++i)
+        u[i] = kf(k, i, 7 * A) * g[0] + kf(k, i, 7 * A + 4) * g[1] + kf(k, i, 7 * A + 5) * g[2];
+}
+
+// columns: end of column c (-1: zero column), gradients g[c]; nc columns in use
+__device__ inline void pl_setup(FramePl& pl, const FrameCoef& k, const int end[2], const double g[2][3], int nc) {
+    double H[2][2] = {{0, 0}, {0, 0}};
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        if (end[c] == 0) pl_column<0>(k, g[c], pl.u[c]);
+        else if (end[c] == 1) pl_column<1>(k, g[c], pl.u[c]);
+        else
+#pragma unroll
+            for (int i = 0; i < 14; ++i) pl.u[c][i] = 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+            if (end[b] >= 0) {
+                const int o = 7 * end[b];
+                H[c][b] = pl.u[c][o] * g[b][0] + pl.u[c][o + 4] * g[b][1] + pl.u[c][o + 5] * g[b][2];
+            }
+    pl.h[0][1] = pl.h[1][0] = pl.h[1][1] = 0.0;
+    if (nc == 1) {
+        pl.h[0][0] = 1 / H[0][0];
+    } else if (nc == 2) {
+        const double det = H[0][0] * H[1][1] - H[0][1] * H[1][0];
+        pl.h[0][0] = H[1][1] / det;
+        pl.h[1][1] = H[0][0] / det;
+        pl.h[0][1] = H[0][1] * (-1 / det);
+        pl.h[1][0] = H[1][0] * (-1 / det);
+    } else {
+        pl.h[0][0] = 0.0;
+    }
+}
+
+// stiffm_fr (frame.c:580-797) selecting the ends from the total end forces f
+// of the previous iterate: both ends on the surface (the column of an end
+// flagged 2 zero), else the one end on it not flagged 2; none when both ends
+// are flagged 2 (stiff_fr skips the call, frame.c:270-275).
+__device__ inline void frame_plastic(FramePl& pl, const FrameCoef& k, const double* f, int y0, int y1,
+                                     const YieldRef& yr) {
+    int end[2] = {-1, -1}, nc = 0;
+    double g[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    if (y0 != 2 || y1 != 2) {
+        const double phi0 = end_phi(f, 0, yr), phi1 = end_phi(f, 1, yr);
+        if (phi0 >= 1 - PHITOL && phi1 >= 1 - PHITOL) {
+            nc = 2;
+            if (y0 != 2) { end[0] = 0; end_grad(f, 0, yr, g[0]); }
+            if (y1 != 2) { end[1] = 1; end_grad(f, 1, yr, g[1]); }
+        } else if (phi0 >= 1 - PHITOL && y0 != 2) {
+            nc = 1; end[0] = 0; end_grad(f, 0, yr, g[0]);
+        } else if (phi1 >= 1 - PHITOL && y1 != 2) {
+            nc = 1; end[0] = 1; end_grad(f, 1, yr, g[0]);
+        }
+    }
+    pl_setup(pl, k, end, g, nc);
+}
+
+// entry (i, j) of the local tangent, less its plastic part when PL
+template <bool PL>
+__device__ __forceinline__ double kent(const FrameCoef& k, const FramePl& pl, int i, int j) {
+    double v = kf(k, i, j);
+    if constexpr (PL)
+        v -= pl.u[0][i] * (pl.h[0][0] * pl.u[0][j] + pl.h[0][1] * pl.u[1][j]) +
+             pl.u[1][i] * (pl.h[1][0] * pl.u[0][j] + pl.h[1][1] * pl.u[1][j]);
+    return v;
+}
+
+// Member yield properties: Py = A fy, Mpy = Z_weak fy, Mpz = Z_strong fy.
+__device__ inline YieldRef yield_ref(const double* carea, const double* yield, const double* zstrong,
+                                     const double* zweak, long e, long ntr) {
+    const double fy = yield[ntr + e];
+    return YieldRef{carea[ntr + e] * fy, zweak[e] * fy, zstrong[e] * fy};
+}
+
 __device__ inline void load_triad(const double* c1, const double* c2, const double* c3, long t,
                                   double R[3][3]) {
 #pragma unroll
@@ -144,9 +261,19 @@ __device__ inline void load_triad(const double* c1, const double* c2, const doub
 // local index of element of triple t (0: u1, 1: theta1, 2: u2, 3: theta2)
 __device__ __forceinline__ constexpr int tidx(int t, int m) { return lidx(t / 2, (t % 2) * 3 + m); }
 
+// Yield data of the members under ANAFLAG 3 (null otherwise).
+struct PlasticIn {
+    const int32_t* yld;      // yldflag, 2 per member
+    const double* yield;     // [NE_TR + NE_FR + NE_SH]
+    const double* zstrong;   // [NE_FR]
+    const double* zweak;
+};
+
 // stiff_fr (frame.c:226-349) for the members of one colour: T^T k T with
 // T = blockdiag(R, R, 1, R, R, 1), one 3x3 sub-block (pair of triples) at a
-// time, added into the four joint blocks of the member.
+// time, added into the four joint blocks of the member.  PL: less the plastic
+// part of k (ANAFLAG 3).
+template <bool PL>
 __global__ void __launch_bounds__(TPB) k_frame_tangent(
     const int32_t* __restrict__ els, int n, const int32_t* __restrict__ blk,
     const double* __restrict__ c1, const double* __restrict__ c2, const double* __restrict__ c3,
@@ -154,7 +281,7 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent(
     const double* __restrict__ llen, const double* __restrict__ istrong, const double* __restrict__ iweak,
     const double* __restrict__ ipolar, const double* __restrict__ iwarp,
     const double* __restrict__ defllen, const double* __restrict__ ef, double* __restrict__ K,
-    long ntr, int geometric) {
+    long ntr, int geometric, PlasticIn pin) {
     int it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= n) return;
     const long e = els[it], p = ntr + e;
@@ -162,6 +289,10 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent(
     FrameCoef k;
     frame_coef(k, emod[p], gmod[e], carea[p], llen[p], istrong[e], iweak[e], ipolar[e], iwarp[e], fe,
                defllen[p], geometric != 0);
+    FramePl pl;
+    if constexpr (PL)
+        frame_plastic(pl, k, fe, pin.yld[e * 2], pin.yld[e * 2 + 1],
+                      yield_ref(carea, pin.yield, pin.zstrong, pin.zweak, e, ntr));
     double R[3][3];
     load_triad(c1, c2, c3, ntr + e * 3, R);
     int32_t bk[4];
@@ -176,9 +307,9 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent(
             for (int r = 0; r < 3; ++r)
 #pragma unroll
                 for (int l = 0; l < 3; ++l)
-                    tmp[r][l] = R[0][r] * kf(k, tidx(t1, 0), tidx(t2, l)) +
-                                R[1][r] * kf(k, tidx(t1, 1), tidx(t2, l)) +
-                                R[2][r] * kf(k, tidx(t1, 2), tidx(t2, l));
+                    tmp[r][l] = R[0][r] * kent<PL>(k, pl, tidx(t1, 0), tidx(t2, l)) +
+                                R[1][r] * kent<PL>(k, pl, tidx(t1, 1), tidx(t2, l)) +
+                                R[2][r] * kent<PL>(k, pl, tidx(t1, 2), tidx(t2, l));
             double* Kb = K + (long)bk[(t1 / 2) * 2 + t2 / 2] * 36 + ((t1 % 2) * 3) * 6 + (t2 % 2) * 3;
 #pragma unroll
             for (int r = 0; r < 3; ++r)
@@ -195,8 +326,9 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent(
 // with the rows and columns of fixed slots as identity (k_fix_slots).  No
 // zeroing and no read-modify-write of K; a member's scalars are evaluated by
 // each thread whose block it touches.
-template <int RA, int CB, int U>
-__device__ __forceinline__ void frame_half_add(double acc[3][6], const FrameCoef& k, const double R[3][3]) {
+template <int RA, int CB, int U, bool PL>
+__device__ __forceinline__ void frame_half_add(double acc[3][6], const FrameCoef& k, const FramePl& pl,
+                                               const double R[3][3]) {
 #pragma unroll
     for (int v = 0; v < 2; ++v) {
         const int t1 = 2 * RA + U, t2 = 2 * CB + v;
@@ -205,8 +337,9 @@ __device__ __forceinline__ void frame_half_add(double acc[3][6], const FrameCoef
         for (int r = 0; r < 3; ++r)
 #pragma unroll
             for (int l = 0; l < 3; ++l)
-                tmp[r][l] = R[0][r] * kf(k, tidx(t1, 0), tidx(t2, l)) + R[1][r] * kf(k, tidx(t1, 1), tidx(t2, l)) +
-                            R[2][r] * kf(k, tidx(t1, 2), tidx(t2, l));
+                tmp[r][l] = R[0][r] * kent<PL>(k, pl, tidx(t1, 0), tidx(t2, l)) +
+                            R[1][r] * kent<PL>(k, pl, tidx(t1, 1), tidx(t2, l)) +
+                            R[2][r] * kent<PL>(k, pl, tidx(t1, 2), tidx(t2, l));
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -220,7 +353,7 @@ __device__ __forceinline__ void frame_half_add(double acc[3][6], const FrameCoef
 // BandLDLT / BandBatch make from K, with the same values): tile (I, J) of the
 // equation pair (i, j) is tiles[(I (w + 1) + J - I + w) T^2], entry (j % T, i % T)
 // column-major; an entry more than w tiles below the diagonal sets *bad = 2.
-template <bool TO_BAND>
+template <bool TO_BAND, bool PL>
 __global__ void __launch_bounds__(TPB) k_frame_tangent_gather(
     const int32_t* __restrict__ gptr, const int32_t* __restrict__ gitem, const int32_t* __restrict__ blkrow,
     const int32_t* __restrict__ colidx, const int32_t* __restrict__ eq, long nnzb,
@@ -229,7 +362,7 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent_gather(
     const double* __restrict__ llen, const double* __restrict__ istrong, const double* __restrict__ iweak,
     const double* __restrict__ ipolar, const double* __restrict__ iwarp,
     const double* __restrict__ defllen, const double* __restrict__ ef, double* __restrict__ K, int geometric,
-    int w, int* __restrict__ bad) {
+    int w, int* __restrict__ bad, PlasticIn pin) {
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     const long kb = t >> 1;
     const int u = int(t & 1);
@@ -245,17 +378,21 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent_gather(
         FrameCoef k;
         frame_coef(k, emod[e], gmod[e], carea[e], llen[e], istrong[e], iweak[e], ipolar[e], iwarp[e], ef + e * 14,
                    defllen[e], geometric != 0);
+        FramePl pl;
+        if constexpr (PL)
+            frame_plastic(pl, k, ef + e * 14, pin.yld[e * 2], pin.yld[e * 2 + 1],
+                          yield_ref(carea, pin.yield, pin.zstrong, pin.zweak, e, 0));
         double R[3][3];
         load_triad(c1, c2, c3, e * 3, R);
         switch ((item & 3) * 2 + u) {
-        case 0: frame_half_add<0, 0, 0>(acc, k, R); break;
-        case 1: frame_half_add<0, 0, 1>(acc, k, R); break;
-        case 2: frame_half_add<0, 1, 0>(acc, k, R); break;
-        case 3: frame_half_add<0, 1, 1>(acc, k, R); break;
-        case 4: frame_half_add<1, 0, 0>(acc, k, R); break;
-        case 5: frame_half_add<1, 0, 1>(acc, k, R); break;
-        case 6: frame_half_add<1, 1, 0>(acc, k, R); break;
-        default: frame_half_add<1, 1, 1>(acc, k, R); break;
+        case 0: frame_half_add<0, 0, 0, PL>(acc, k, pl, R); break;
+        case 1: frame_half_add<0, 0, 1, PL>(acc, k, pl, R); break;
+        case 2: frame_half_add<0, 1, 0, PL>(acc, k, pl, R); break;
+        case 3: frame_half_add<0, 1, 1, PL>(acc, k, pl, R); break;
+        case 4: frame_half_add<1, 0, 0, PL>(acc, k, pl, R); break;
+        case 5: frame_half_add<1, 0, 1, PL>(acc, k, pl, R); break;
+        case 6: frame_half_add<1, 1, 0, PL>(acc, k, pl, R); break;
+        default: frame_half_add<1, 1, 1, PL>(acc, k, pl, R); break;
         }
     }
     const long A = blkrow[kb], B = colidx[kb];
@@ -409,22 +546,227 @@ __global__ void __launch_bounds__(TPB) k_frame_forces_members(
 }
 
 // ... then one thread per free slot adds the end forces of the members at its
-// joint in colour order, the order in which the colour launches add them.
+// joint in colour order, the order in which the colour launches add them;
+// with st, only the members before st->first (the members forces_fr went
+// through before it returned).
 __global__ void k_frame_forces_gather(const int32_t* __restrict__ jptr, const int32_t* __restrict__ jitem,
                                       const int32_t* __restrict__ eq, long nslot, const double* __restrict__ h,
-                                      double* __restrict__ f) {
+                                      double* __restrict__ f, const PlasticStatus* __restrict__ st) {
     const long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (r >= nslot) return;
     const int32_t q = eq[r];
     if (q < 0) return;
     const long j = r / 6;
     const int s = int(r % 6);
+    const int32_t lim = st ? st->first : INT32_MAX;
     double acc = f[q];
     for (int32_t it = jptr[j]; it < jptr[j + 1]; ++it) {
         const int32_t item = jitem[it];
+        if ((item >> 1) >= lim) continue;
         acc += h[long(item >> 1) * 12 + 6 * (item & 1) + s];
     }
     f[q] = acc;
+}
+
+// regula_falsi (frame.c:1397-1455): the fraction tau of the end-force
+// increment (dp, dmy, dmz) from (p, my, mz) at which phi = 1, by false
+// position on [0, 1], the estimate from the end values and one further step
+// (the reference's loop test can never hold, so its body runs once).
+__device__ inline double regula_falsi(double p, double dp, double my, double dmy, double mz, double dmz) {
+    double tu = 1, tl = 0;
+    double fu = orbison(p + tu * dp, my + tu * dmy, mz + tu * dmz);
+    double fl = orbison(p + tl * dp, my + tl * dmy, mz + tl * dmz);
+    const double tr = tu - (fu - 1) * (tl - tu) / (fl - fu);
+    const double fr = orbison(p + tr * dp, my + tr * dmy, mz + tr * dmz);
+    if ((fl - 1 > 0 && fr - 1 > 0) || (fl - 1 < 0 && fr - 1 < 0)) {
+        tl = tr;
+        fl = fr;
+    } else {
+        tu = tr;
+        fu = fr;
+    }
+    return tu - (fu - 1) * (tl - tu) / (fl - fu);
+}
+
+// forces_fr under ANAFLAG 3 (frame.c:902-1309) for every member at once, each
+// as if the reference's member loop reached it: the plastic tangent of the
+// previous iterate's end forces (stiffm_fr) gives ef_i, kept in efc with the
+// member's global end forces in h; then the yield checks at the new end
+// forces: an end through the surface (phi > 1 + tol, the larger of the two,
+// not flagged 2) gives code 1 with tau from regula_falsi and that end flagged
+// 1; otherwise the ends on the surface and not flagged 2 are flagged 1 and,
+// when an end is flagged 1, the unloading test of unload (frame.c:1457-1658:
+// lambda = (G^T k G)^-1 G^T k dl with the elastic + geometric k and the
+// gradients at the new end forces, an end unloading when lambda < -1e-8)
+// flags the unloading ends 2 with code 2.  code[4e] = code, code[4e+1..2] =
+// the member's flags after its pass; st->first <- the lowest member with a
+// code.  Warping increments are zero (the device fixes warping DOFs); no
+// member end offsets; the fixed-end forces are zero (no distributed loads).
+__global__ void __launch_bounds__(64) k_frame_forces_plastic(
+    long n, const int32_t* __restrict__ mcode, const double* __restrict__ dd, const double* __restrict__ c1p,
+    const double* __restrict__ c2p, const double* __restrict__ c3p, const double* __restrict__ c1c,
+    const double* __restrict__ c2c, const double* __restrict__ c3c, const double* __restrict__ emod,
+    const double* __restrict__ gmod, const double* __restrict__ carea, const double* __restrict__ llen,
+    const double* __restrict__ istrong, const double* __restrict__ iweak, const double* __restrict__ ipolar,
+    const double* __restrict__ iwarp, const double* __restrict__ defllen_ip, const double* __restrict__ ef_ip,
+    PlasticIn pin, double* __restrict__ efc, double* __restrict__ h, int32_t* __restrict__ code,
+    double* __restrict__ tau, PlasticStatus* __restrict__ st, long ntr) {
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const long p = ntr + e;
+    const int32_t* mc = mcode + ntr * 6 + e * 14;
+    const double* fe = ef_ip + ntr * 2 + e * 14;
+    FrameCoef k;
+    frame_coef(k, emod[p], gmod[e], carea[p], llen[p], istrong[e], iweak[e], ipolar[e], iwarp[e], fe,
+               defllen_ip[p], true);
+    const YieldRef yr = yield_ref(carea, pin.yield, pin.zstrong, pin.zweak, e, ntr);
+    int y[2] = {pin.yld[e * 2], pin.yld[e * 2 + 1]};
+    FramePl pl;
+    frame_plastic(pl, k, fe, y[0], y[1], yr);
+    double Rp[3][3], Ri[3][3];
+    load_triad(c1p, c2p, c3p, ntr + e * 3, Rp);
+    load_triad(c1c, c2c, c3c, ntr + e * 3, Ri);
+    double dl[4][3];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        double D[3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const int32_t q = mc[tidx(t, m)];
+            D[m] = q ? dd[q - 1] : 0.0;
+        }
+#pragma unroll
+        for (int m = 0; m < 3; ++m) dl[t][m] = Rp[m][0] * D[0] + Rp[m][1] * D[1] + Rp[m][2] * D[2];
+    }
+    double Q[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Q[r][c] = Ri[r][0] * Rp[c][0] + Ri[r][1] * Rp[c][1] + Ri[r][2] * Rp[c][2];
+    double fi[14];
+#pragma unroll
+    for (int to = 0; to < 4; ++to) {
+        double v[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            double s = 0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int m = 0; m < 3; ++m) s += kent<true>(k, pl, tidx(to, r), tidx(t, m)) * dl[t][m];
+            v[r] = fe[tidx(to, r)] + s;
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r) fi[tidx(to, r)] = Q[r][0] * v[0] + Q[r][1] * v[1] + Q[r][2] * v[2];
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+            h[e * 12 + 3 * to + m] = Ri[0][m] * fi[tidx(to, 0)] + Ri[1][m] * fi[tidx(to, 1)] + Ri[2][m] * fi[tidx(to, 2)];
+    }
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+        const int i = 7 * w + 6;
+        double s = 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int m = 0; m < 3; ++m) s += kent<true>(k, pl, i, tidx(t, m)) * dl[t][m];
+        fi[i] = fe[i] + s;
+    }
+#pragma unroll
+    for (int i = 0; i < 14; ++i) efc[e * 14 + i] = fi[i];
+
+    // yield checks at the new end forces (frame.c:1157-1228)
+    const double phi[2] = {end_phi(fi, 0, yr), end_phi(fi, 1, yr)};
+    int c = 0;
+    double t = 1.0;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int o = 1 - a, b = 7 * a;
+        if (!c && phi[a] > phi[o] && phi[a] > 1 + PHITOL && y[a] != 2) {
+            t = regula_falsi(fe[b] / yr.Py, (fi[b] - fe[b]) / yr.Py, fe[b + 4] / yr.Mpy,
+                             (fi[b + 4] - fe[b + 4]) / yr.Mpy, fe[b + 5] / yr.Mpz, (fi[b + 5] - fe[b + 5]) / yr.Mpz);
+            y[a] = 1;
+            c = 1;
+        }
+    }
+    if (!c) {
+        const bool on0 = phi[0] >= 1 - PHITOL && y[0] != 2, on1 = phi[1] >= 1 - PHITOL && y[1] != 2;
+        if (on0 && on1) y[0] = y[1] = 1;
+        else if (on0) y[0] = 1;
+        else if (on1) y[1] = 1;
+        if (y[0] == 1 || y[1] == 1) {
+            // unload: columns at the new end forces, k without its plastic part
+            int end[2] = {-1, -1}, nc = 0;
+            double g[2][3] = {{0, 0, 0}, {0, 0, 0}};
+            if (phi[0] >= 1 - PHITOL && phi[1] >= 1 - PHITOL) {
+                nc = 2; end[0] = 0; end[1] = 1;
+                end_grad(fi, 0, yr, g[0]);
+                end_grad(fi, 1, yr, g[1]);
+            } else if (phi[0] >= 1 - PHITOL) {
+                nc = 1; end[0] = 0; end_grad(fi, 0, yr, g[0]);
+            } else if (phi[1] >= 1 - PHITOL) {
+                nc = 1; end[0] = 1; end_grad(fi, 1, yr, g[0]);
+            }
+            if (nc) {
+                FramePl ul;
+                pl_setup(ul, k, end, g, nc);
+                double w[2] = {0, 0};
+#pragma unroll
+                for (int cc = 0; cc < 2; ++cc)
+#pragma unroll
+                    for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+                        for (int m = 0; m < 3; ++m) w[cc] += ul.u[cc][tidx(tt, m)] * dl[tt][m];
+                const double l0 = ul.h[0][0] * w[0] + ul.h[0][1] * w[1];
+                const double l1 = ul.h[1][0] * w[0] + ul.h[1][1] * w[1];
+                int un = 0;
+                if (nc == 2) un = (l0 < -1e-8 && l1 < -1e-8) ? 1 : (l0 < -1e-8) ? 2 : (l1 < -1e-8) ? 3 : 0;
+                else if (l0 < -1e-8) un = 2 + end[0];
+                if (un) {
+                    if (un == 1 || un == 2) y[0] = 2;
+                    if (un == 1 || un == 3) y[1] = 2;
+                    c = 2;
+                }
+            }
+        }
+    }
+    code[e * 4] = c;
+    code[e * 4 + 1] = y[0];
+    code[e * 4 + 2] = y[1];
+    tau[e] = t;
+    if (c) atomicMin(&st->first, int32_t(e));
+}
+
+__global__ void k_plastic_init(PlasticStatus* st, long n) {
+    st->first = int32_t(n);
+    st->code = 0;
+    st->tau = 1.0;
+}
+
+// The members up to the first with a code take their pass: ef_i <- efc and
+// their flags; that member's code and tau go to st.  The members after it
+// keep ef_i and their flags, as the reference's loop never reaches them.
+__global__ void k_frame_plastic_commit(long n, const double* __restrict__ efc, const int32_t* __restrict__ code,
+                                       const double* __restrict__ tau, PlasticStatus* __restrict__ st,
+                                       int32_t* __restrict__ yld, double* __restrict__ ef_i, long ntr) {
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const long first = st->first;
+    if (e > first) return;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) ef_i[ntr * 2 + e * 14 + i] = efc[e * 14 + i];
+    yld[e * 2] = code[e * 4 + 1];
+    yld[e * 2 + 1] = code[e * 4 + 2];
+    if (e == first) {
+        st->code = code[e * 4];
+        st->tau = tau[e];
+    }
+}
+
+// yldflag 2 -> 0 at a commit (main.c:2107-2112, 3086-3091, 3887-3892)
+__global__ void k_unloaded_elastic(int32_t* __restrict__ yld, long n2) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n2 && yld[i] == 2) yld[i] = 0;
 }
 
 // misc.c:122-144: member length and triad from the current chord and the
@@ -457,44 +799,102 @@ __global__ void k_update_frame(const int32_t* __restrict__ minc, const double* _
 
 }  // namespace
 
+namespace {
+PlasticIn plastic_in(const Dev& d) { return PlasticIn{d.yld, d.yield, d.zstrong, d.zweak}; }
+}  // namespace
+
+// The geometric terms are on for ANAFLAG 2 and 3 (frame.c:266-277).
 void launch_frame_tangent(Dev& d, int g) {
+    const int geo = d.anaflag == 2 || d.anaflag == 3;
     for (size_t c = 0; c + 1 < d.col_ptr_fr.size(); ++c) {
         int n = d.col_ptr_fr[c + 1] - d.col_ptr_fr[c];
         if (!n) continue;
-        hipLaunchKernelGGL(k_frame_tangent, dim3(nblk(n)), dim3(TPB), 0, d.s,
-                           d.col_el_fr + d.col_ptr_fr[c], n, d.blk_fr, d.c1[g], d.c2[g], d.c3[g],
-                           d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp,
-                           d.defllen[g], d.ef[g], d.K, d.ntr, int(d.anaflag == 2));
+        if (d.anaflag == 3)
+            hipLaunchKernelGGL(k_frame_tangent<true>, dim3(nblk(n)), dim3(TPB), 0, d.s,
+                               d.col_el_fr + d.col_ptr_fr[c], n, d.blk_fr, d.c1[g], d.c2[g], d.c3[g],
+                               d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp,
+                               d.defllen[g], d.ef[g], d.K, d.ntr, geo, plastic_in(d));
+        else
+            hipLaunchKernelGGL(k_frame_tangent<false>, dim3(nblk(n)), dim3(TPB), 0, d.s,
+                               d.col_el_fr + d.col_ptr_fr[c], n, d.blk_fr, d.c1[g], d.c2[g], d.c3[g],
+                               d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp,
+                               d.defllen[g], d.ef[g], d.K, d.ntr, geo, PlasticIn{});
     }
     CB_CHECK(hipGetLastError());
 }
 
 void launch_frame_tangent_gather(Dev& d, int g) {
     if (d.bs != 6) throw DeviceError("frame tangent gather: frame models carry 6 slots per joint");
-    hipLaunchKernelGGL(k_frame_tangent_gather<false>, dim3(nblk(2 * d.nnzb)), dim3(TPB), 0, d.s, d.gat_ptr_fr,
-                       d.gat_item_fr, d.blkrow, d.colidx, d.eq_of_slot, d.nnzb, d.c1[g], d.c2[g], d.c3[g], d.emod,
-                       d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[g], d.ef[g], d.K,
-                       int(d.anaflag == 2), 0, nullptr);
+    const int geo = d.anaflag == 2 || d.anaflag == 3;
+    if (d.anaflag == 3)
+        hipLaunchKernelGGL((k_frame_tangent_gather<false, true>), dim3(nblk(2 * d.nnzb)), dim3(TPB), 0, d.s,
+                           d.gat_ptr_fr, d.gat_item_fr, d.blkrow, d.colidx, d.eq_of_slot, d.nnzb, d.c1[g], d.c2[g],
+                           d.c3[g], d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp,
+                           d.defllen[g], d.ef[g], d.K, geo, 0, nullptr, plastic_in(d));
+    else
+        hipLaunchKernelGGL((k_frame_tangent_gather<false, false>), dim3(nblk(2 * d.nnzb)), dim3(TPB), 0, d.s,
+                           d.gat_ptr_fr, d.gat_item_fr, d.blkrow, d.colidx, d.eq_of_slot, d.nnzb, d.c1[g], d.c2[g],
+                           d.c3[g], d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp,
+                           d.defllen[g], d.ef[g], d.K, geo, 0, nullptr, PlasticIn{});
     CB_CHECK(hipGetLastError());
 }
 
 void launch_frame_tangent_band(Dev& d, int g, double* tiles, int w, int* bad) {
     if (d.bs != 6 || d.ntr || d.nsh) throw DeviceError("frame tangent into the band: frame-only models");
-    hipLaunchKernelGGL(k_frame_tangent_gather<true>, dim3(nblk(2 * d.nnzb)), dim3(TPB), 0, d.s, d.gat_ptr_fr,
-                       d.gat_item_fr, d.blkrow, d.colidx, d.eq_of_slot, d.nnzb, d.c1[g], d.c2[g], d.c3[g], d.emod,
-                       d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[g], d.ef[g], tiles,
-                       int(d.anaflag == 2), w, bad);
+    if (d.anaflag == 3) throw DeviceError("frame tangent into the band: elastic (ANAFLAG 2) members only");
+    hipLaunchKernelGGL((k_frame_tangent_gather<true, false>), dim3(nblk(2 * d.nnzb)), dim3(TPB), 0, d.s,
+                       d.gat_ptr_fr, d.gat_item_fr, d.blkrow, d.colidx, d.eq_of_slot, d.nnzb, d.c1[g], d.c2[g],
+                       d.c3[g], d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp,
+                       d.defllen[g], d.ef[g], tiles, int(d.anaflag == 2), w, bad, PlasticIn{});
+    CB_CHECK(hipGetLastError());
+}
+
+// forces_fr under ANAFLAG 3: every member's pass at once, then the members up
+// to the first that returns take it and their end forces before that member
+// are gathered into f (the reference's loop stops at a return).
+static void launch_frame_forces_plastic(Dev& d, double* f, const double* dd) {
+    if (d.bs != 6) throw DeviceError("plastic frame forces: frame models carry 6 slots per joint");
+    hipLaunchKernelGGL(k_plastic_init, dim3(1), dim3(1), 0, d.s, d.pl_st, d.nfr);
+    hipLaunchKernelGGL(k_frame_forces_plastic, dim3(nblk(d.nfr, 64)), dim3(64), 0, d.s, d.nfr, d.mcode, dd,
+                       d.c1[G_IP], d.c2[G_IP], d.c3[G_IP], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.emod, d.gmod, d.carea,
+                       d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[G_IP], d.ef[G_IP], plastic_in(d),
+                       d.pl_efc, d.fh, d.pl_code, d.pl_tau, d.pl_st, d.ntr);
+    hipLaunchKernelGGL(k_frame_plastic_commit, dim3(nblk(d.nfr, 256)), dim3(256), 0, d.s, d.nfr, d.pl_efc,
+                       d.pl_code, d.pl_tau, d.pl_st, d.yld, d.ef[G_I], d.ntr);
+    hipLaunchKernelGGL(k_frame_forces_gather, dim3(nblk(d.nslot, 256)), dim3(256), 0, d.s, d.jnt_ptr_fr,
+                       d.jnt_item_fr, d.eq_of_slot, d.nslot, d.fh, f, d.pl_st);
+    CB_CHECK(hipGetLastError());
+}
+
+int frame_plastic_result(Dev& d, double* tau) {
+    if (d.anaflag != 3 || !d.nfr) {
+        *tau = 1.0;
+        return 0;
+    }
+    d2h(d.pl_host, d.pl_st, 1, d.s);
+    CB_CHECK(hipStreamSynchronize(d.s));
+    *tau = d.pl_host->tau;
+    return d.pl_host->code;
+}
+
+void launch_unloaded_elastic(Dev& d) {
+    if (d.anaflag != 3 || !d.nfr) return;
+    hipLaunchKernelGGL(k_unloaded_elastic, dim3(nblk(d.nfr * 2, 256)), dim3(256), 0, d.s, d.yld, d.nfr * 2);
     CB_CHECK(hipGetLastError());
 }
 
 void launch_frame_forces(Dev& d, double* f, const double* dd) {
+    if (d.anaflag == 3) {
+        launch_frame_forces_plastic(d, f, dd);
+        return;
+    }
     if (!d.ntr && !d.nsh && d.bs == 6) {
         hipLaunchKernelGGL(k_frame_forces_members, dim3(nblk(d.nfr)), dim3(TPB), 0, d.s, d.nfr, d.mcode, dd,
                            d.c1[G_IP], d.c2[G_IP], d.c3[G_IP], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.emod, d.gmod,
                            d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[G_IP], d.ef[G_IP],
                            d.ef[G_I], d.fh, int(d.anaflag == 2));
         hipLaunchKernelGGL(k_frame_forces_gather, dim3(nblk(d.nslot, 256)), dim3(256), 0, d.s, d.jnt_ptr_fr,
-                           d.jnt_item_fr, d.eq_of_slot, d.nslot, d.fh, f);
+                           d.jnt_item_fr, d.eq_of_slot, d.nslot, d.fh, f, nullptr);
         CB_CHECK(hipGetLastError());
         return;
     }

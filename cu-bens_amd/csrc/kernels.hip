@@ -367,41 +367,65 @@ __global__ void k_truss_tangent(const int32_t* __restrict__ els, int n, const in
                                 const double* __restrict__ c3, const double* __restrict__ emod,
                                 const double* __restrict__ carea, const double* __restrict__ llen,
                                 const double* __restrict__ defllen, const double* __restrict__ ef,
-                                T* __restrict__ K, int bs, int geo) {
+                                const double* __restrict__ yield, T* __restrict__ K, int bs, int ana) {
     int it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= n) return;
     const long e = els[it];
     const double L0 = llen[e], L = defllen[e];
+    const bool geo = ana != 1;
     const double ka = geo ? emod[e] * carea[e] * (L * L) / (L0 * L0 * L0) : emod[e] * carea[e] / L0;
-    const double gN = geo ? ef[e * 2] / L : 0.0;
+    // local 2x2 [k00 k01; k10 k11] of the bar
+    double kl[2][2] = {{ka, -ka}, {-ka, ka}};
+    double gN = geo ? ef[e * 2] / L : 0.0;
+    if (ana == 3) {
+        // stiffm_tr (truss.c:206-229) beyond the surface, (N/Py)^2 > 1 + tol:
+        // k - (k g)(k g)^T / g^T k g with g = 2 (N1, N2) / Py^2; the geometric
+        // term only at the squash load, (N/Py)^2 >= 1 - tol (truss.c:155-166)
+        const double Py = carea[e] * yield[e], r = ef[e * 2] / Py;
+        if (r * r > 1 + 1e-4) {
+            const double g0 = 2 * ef[e * 2] / (Py * Py), g1 = 2 * ef[e * 2 + 1] / (Py * Py);
+            const double u0 = kl[0][0] * g0 + kl[0][1] * g1, u1 = kl[1][0] * g0 + kl[1][1] * g1;
+            const double gkg = u0 * g0 + u1 * g1;
+            const double r00 = u0 * u0 / gkg, r01 = u0 * u1 / gkg, r11 = u1 * u1 / gkg;
+            kl[0][0] -= r00; kl[0][1] -= r01; kl[1][0] -= r01; kl[1][1] -= r11;
+        }
+        if (!(r * r >= 1 - 1e-4)) gN = 0.0;
+    }
     const double c[3] = {c1[e], c2[e], c3[e]};
     const int bb = bs * bs;
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b) {
             T* Kb = K + (long)blk[e * 4 + a * 2 + b] * bb;
-            double sgn = (a == b) ? 1.0 : -1.0;
+            const double kab = kl[a][b], sgn = (a == b) ? 1.0 : -1.0;
             for (int p = 0; p < 3; ++p)
                 for (int q = 0; q < 3; ++q) {
-                    double vv = sgn * ka * c[p] * c[q];
+                    double vv = kab * c[p] * c[q];
                     if (p == q) vv += sgn * gN;
                     Kb[p * bs + q] += T(vv);
                 }
         }
 }
 
-// truss.c:327-376: N = EA (eps + eps^2/2) L/L0; f -= N c at end 1, += at end 2.
+// truss.c:327-376: N = EA (eps + eps^2/2) L/L0; f -= N c at end 1, += at end 2;
+// under ANAFLAG 3 (yield non-null) N at or beyond the squash load,
+// (N/Py)^2 >= 1 - tol, is set to +-Py with no plastic strain kept
+// (truss.c:336-347).
 __global__ void k_truss_forces(const int32_t* __restrict__ els, int n, const int32_t* __restrict__ mcode,
                                const double* __restrict__ c1, const double* __restrict__ c2,
                                const double* __restrict__ c3, const double* __restrict__ emod,
                                const double* __restrict__ carea, const double* __restrict__ llen,
-                               const double* __restrict__ defllen, double* __restrict__ ef_i,
-                               double* __restrict__ f) {
+                               const double* __restrict__ defllen, const double* __restrict__ yield,
+                               double* __restrict__ ef_i, double* __restrict__ f) {
     int it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= n) return;
     const long e = els[it];
     const double L0 = llen[e], L = defllen[e];
     const double eps = (L - L0) / L0;
-    const double N = emod[e] * carea[e] * (eps + 0.5 * (eps * eps)) * L / L0;
+    double N = emod[e] * carea[e] * (eps + 0.5 * (eps * eps)) * L / L0;
+    if (yield) {
+        const double Py = yield[e] * carea[e], r = N / Py;
+        if (r * r >= 1 - 1e-4) N = N < 0 ? -Py : Py;
+    }
     ef_i[e * 2] = N;
     ef_i[e * 2 + 1] = -N;
     const double c[3] = {c1[e], c2[e], c3[e]};
@@ -711,8 +735,8 @@ void launch_tangent(Dev& d, int gx, int g, bool want_fp32) {
         if (!n) continue;
         hipLaunchKernelGGL(k_truss_tangent<double>, dim3(nblk(n, 128)), dim3(128), 0, d.s,
                            d.col_el_tr + d.col_ptr_tr[c], n, d.blk_tr, d.c1[g], d.c2[g], d.c3[g],
-                           d.emod, d.carea, d.llen, d.defllen[g], d.ef[g], d.K, d.bs,
-                           int(d.anaflag != 1));
+                           d.emod, d.carea, d.llen, d.defllen[g], d.ef[g], d.yield, d.K, d.bs,
+                           d.anaflag);
     }
     if (d.nfr) launch_frame_tangent(d, g);
     for (size_t c = 0; c + 1 < d.col_ptr_sh.size(); ++c) {
@@ -766,7 +790,8 @@ void launch_forces(Dev& d, int gx, double* f, const double* dd) {
         if (!n) continue;
         hipLaunchKernelGGL(k_truss_forces, dim3(nblk(n)), dim3(TPB), 0, d.s,
                            d.col_el_tr + d.col_ptr_tr[c], n, d.mcode, d.c1[G_I], d.c2[G_I], d.c3[G_I],
-                           d.emod, d.carea, d.llen, d.defllen[G_I], d.ef[G_I], f);
+                           d.emod, d.carea, d.llen, d.defllen[G_I], d.anaflag == 3 ? d.yield : nullptr,
+                           d.ef[G_I], f);
     }
     if (d.nfr) launch_frame_forces(d, f, dd);
     for (size_t c = 0; c + 1 < d.col_ptr_sh.size(); ++c) {
@@ -919,6 +944,15 @@ void dev_create(Dev& d, const Model& m) {
     d.xlocal = upd(m.xlocal);
     d.farea = upd(m.farea);
     d.slength = upd(m.slength);
+    d.yield = upd(m.yield);
+    d.zstrong = upd(m.zstrong);
+    d.zweak = upd(m.zweak);
+    d.yld = dalloc<int32_t>(size_t(m.nfr * 2));
+    d.pl_efc = dalloc<double>(size_t(m.nfr * 14));
+    d.pl_tau = dalloc<double>(size_t(m.nfr));
+    d.pl_code = dalloc<int32_t>(size_t(m.nfr * 4));
+    d.pl_st = dalloc<PlasticStatus>(1);
+    CB_CHECK(hipHostMalloc(&d.pl_host, sizeof(PlasticStatus), hipHostMallocDefault));
     const long ntri = m.ntr + m.nfr * 3 + m.nsh * 3, nef = m.ntr * 2 + m.nfr * 14 + m.nsh * 18;
     for (int g = 0; g < 3; ++g) {
         d.x[g] = dalloc<double>(size_t(m.nj * 3));
@@ -952,6 +986,7 @@ void dev_reset_state(Dev& d, const Model& m) {
     h2d(d.llen, m.llength.data(), m.llength.size(), d.s);
     h2d(d.farea, m.farea.data(), m.farea.size(), d.s);
     h2d(d.slength, m.slength.data(), m.slength.size(), d.s);
+    CB_CHECK(hipMemsetAsync(d.yld, 0, sizeof(int32_t) * size_t(m.nfr * 2 > 0 ? m.nfr * 2 : 1), d.s));
     CB_CHECK(hipStreamSynchronize(d.s));
 }
 
@@ -960,7 +995,8 @@ void dev_destroy(Dev& d) {
                   d.blk_sh, d.blk_fr, d.col_el_tr, d.col_el_sh, d.col_el_fr, d.blkrow, d.gat_ptr_fr,
                   d.gat_item_fr, d.jnt_ptr_fr, d.jnt_item_fr, d.fh, d.emod, d.carea,
                   d.llen, d.dens, d.gmod, d.istrong, d.iweak, d.ipolar, d.iwarp, d.auxpt, d.nu,
-                  d.thick, d.xlocal, d.farea, d.slength, d.K, d.K32, d.red};
+                  d.thick, d.xlocal, d.farea, d.slength, d.K, d.K32, d.red, d.yield, d.zstrong,
+                  d.zweak, d.yld, d.pl_efc, d.pl_tau, d.pl_code, d.pl_st};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     for (int g = 0; g < 3; ++g) {
@@ -969,6 +1005,7 @@ void dev_destroy(Dev& d) {
             if (p) (void)hipFree(p);
     }
     if (d.red_host) (void)hipHostFree(d.red_host);
+    if (d.pl_host) (void)hipHostFree(d.pl_host);
     if (d.s) (void)hipStreamDestroy(d.s);
     d = Dev();
 }

@@ -278,8 +278,12 @@ bool load_model(const char* path, Model& m, std::string& err) {
     if (t.bad()) FAIL("truncated header");
     if (nsbr || nfbr) FAIL("brick elements are not on this path");
     if (m.optflag != 1 && m.optflag != 2) FAIL("OPTFLAG must be 1 or 2");
-    if (m.anaflag != 1 && m.anaflag != 2)
-        FAIL("only ANAFLAG 1 (linear elastic) and 2 (geometrically nonlinear elastic) run on the GPU path");
+    if (m.anaflag < 1 || m.anaflag > 3)
+        FAIL("ANAFLAG must be 1 (linear elastic), 2 (nonlinear elastic) or 3 (inelastic)");
+    if (m.anaflag == 3 && m.nsh > 0)
+        FAIL("ANAFLAG 3 shells (Ivanov's yield criterion, shell.c:842-1503) are not on this path");
+    if (m.algflag == 4 && m.anaflag != 1)
+        FAIL("invalid entry for analysis type: ALGFLAG 4 (linear Newmark) takes ANAFLAG 1");
     if (m.algflag < 1 || m.algflag > 5) FAIL("ALGFLAG must be 1 to 5");
     if (m.anaflag == 1 && (m.algflag == 3 || m.algflag == 5))
         FAIL("linear analysis (ANAFLAG 1) runs under ALGFLAG 1, 2 (static) or 4 (linear Newmark)");

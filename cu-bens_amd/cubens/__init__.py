@@ -52,8 +52,8 @@ class Stats(C.Structure):
 KT_COLUMN, KT_DIAG, KT_PANEL, KT_UPDATE, KT_FWD, KT_BWD, KT_PREP = range(7)
 # Per-step driver events (cb_get_step_events): det flag, quad imaginary-root and
 # negative-gamma retries, dnorm restarts, itemax halvings.
-EV_DET, EV_IMAG, EV_NEG, EV_DNORM, EV_HALVE = range(5)
-NEV = 5
+EV_DET, EV_IMAG, EV_NEG, EV_DNORM, EV_HALVE, EV_YIELD, EV_UNLOAD = range(7)
+NEV = 7
 KT_NAMES = ("band_column", "band_diag", "band_panel", "band_update", "band_fwd_sweep", "band_bwd_sweep",
             "band_sweep_prep")
 FACTOR_FUSED, FACTOR_SPLIT = 0, 1
@@ -112,6 +112,8 @@ ABI = {
     "cb_tangent": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "cb_get_tangent": (C.c_int, [C.c_void_p, PD]),
     "cb_internal_force": (C.c_int, [C.c_void_p, C.c_int, PD, PD]),
+    "cb_get_yield_flags": (C.c_int, [C.c_void_p, PI, PD]),
+    "cb_set_yield_flags": (C.c_int, [C.c_void_p, PI]),
     "cb_update_geometry": (C.c_int, [C.c_void_p, C.c_int, PD]),
     "cb_factor": (C.c_int, [C.c_void_p, C.POINTER(Options), PI, PD]),
     "cb_solve": (C.c_int, [C.c_void_p, PD, PD]),
@@ -273,11 +275,28 @@ class GpuModel:
         _check(lib().cb_get_tangent(self.h, _p(v)))
         return v
 
-    def internal_force(self, dd: np.ndarray, gen_x: int = GEN_I) -> np.ndarray:
+    def internal_force(self, dd: np.ndarray, gen_x: int = GEN_I, with_code: bool = False):
+        """f of forces_tr/fr/sh; with_code: (f, forces_fr's code) under ANAFLAG 3."""
         dd = np.ascontiguousarray(dd, np.float64)
         f = np.zeros(self.neq)
-        _check(lib().cb_internal_force(self.h, gen_x, _p(dd), _p(f)))
+        rc = lib().cb_internal_force(self.h, gen_x, _p(dd), _p(f))
+        if rc < 0:
+            _check(rc)
+        if with_code:
+            return f, rc
+        _check(rc)
         return f
+
+    def yield_flags(self):
+        """(yldflag [NE_FR*2], regula-falsi fraction of the last code-1 force pass)."""
+        y = np.zeros(max(self.nfr * 2, 1), np.int32)
+        tau = np.zeros(1)
+        _check(lib().cb_get_yield_flags(self.h, _p(y, PI), _p(tau)))
+        return y[: self.nfr * 2], float(tau[0])
+
+    def set_yield_flags(self, y):
+        y = np.ascontiguousarray(y, np.int32)
+        _check(lib().cb_set_yield_flags(self.h, _p(y, PI)))
 
     def mass(self) -> np.ndarray:
         """Lumped diagonal masses at the converged coordinates (cb_mass)."""

@@ -151,8 +151,20 @@ int cb_get_tangent(cb_model *m, double *vals);
  * frame.c:902, shell.c:1593).
  * f_host [NEQ] receives the force (accumulated from zero); dd_host [NEQ] is the
  * increment; reads ef_ip (clears its membrane entries as shell.c:1773 does) and
- * writes ef_i; coordinates of generation gen_x. */
+ * writes ef_i; coordinates of generation gen_x.  Returns forces_fr's code:
+ * 0, or under ANAFLAG 3 1 when a member end went through the yield surface
+ * (its regula-falsi fraction from cb_get_yield_flags) and 2 when a yielded end
+ * unloads (frame.c:1184-1265); the members after the first with a code are
+ * left out of f_host and keep their ef_i and flags, as the reference's loop
+ * stops there.  Negative on an error. */
 int cb_internal_force(cb_model *m, int gen_x, const double *dd_host, double *f_host);
+/* ANAFLAG 3: the yldflag array of the frame member ends [NE_FR*2] (0 elastic,
+ * 1 yielded, 2 unloading; main.c:802-806, frame.c:1184-1265), which the
+ * tangent and the force pass read and update, and the regula-falsi fraction
+ * of the last cb_internal_force that returned 1 (either pointer may be NULL);
+ * set: replaces the array (values 0..2). */
+int cb_get_yield_flags(cb_model *m, int *yld, double *tau);
+int cb_set_yield_flags(cb_model *m, const int *yld);
 
 /* Lumped diagonal masses at the converged coordinates into sm[NEQ]: replaces
  * mass_tr (truss.c:381-441), mass_fr (frame.c:1314-1393) and mass_sh
@@ -236,9 +248,14 @@ int cb_get_reactions(cb_model *m, int gen, double *r);
  * gamma return (arc.c:134), the dnorm restart (main.c:2936-2953) and the
  * halvings after itemax iterations (main.c:3000-3012; the load step of
  * main.c:2096-2110 and the time step of main.c:3826-3846 for the other
- * drivers).  No reference counterpart: the reference keeps no such record. */
+ * drivers), and under ANAFLAG 3 the restarts on forces_fr's code 1, a member
+ * end through the yield surface (the load step scaled by the regula-falsi
+ * fraction, main.c:2050-2057, 3789-3797, or the arc halved, main.c:2961-2970),
+ * and code 2, a yielded end unloading (the same step again, main.c:2033-2034,
+ * 2954-2960, 3773-3774).  No reference counterpart: the reference keeps no
+ * such record. */
 enum { CB_EV_DET = 0, CB_EV_IMAG = 1, CB_EV_NEG = 2, CB_EV_DNORM = 3, CB_EV_HALVE = 4,
-       CB_NEV = 5 };
+       CB_EV_YIELD = 5, CB_EV_UNLOAD = 6, CB_NEV = 7 };
 int cb_get_step_events(const cb_model *m, int *events, double *psi);
 /* Per converged step of the last NR/MNR/MSAL run: the 2-norms of the
  * out-of-balance force lambda q - f and of the applied load lambda q at

@@ -122,11 +122,12 @@ def arc_residuals(om_params, r, q):
     (no dnorm restart within the step), with arc_s rebuilt from the path alone:
     beta = sqrt(iteopt/itecnt) alpha |d_0| / |d_{s-1} - d_{s-2}| times the
     previous increment (main.c:2625-2630, 2655-2664, 3048-3049), halved once per
-    quad retry or itemax exceedance (main.c:2995-3036).  r: a path with keys d,
-    lpf, itecnt, events (EV_* slots), psi; returns {s: residual}."""
+    quad retry, itemax exceedance or yield restart (main.c:2961-3036).  r: a
+    path with keys d, lpf, itecnt, events (EV_* slots), psi; returns
+    {s: residual}."""
     alpha, iteopt, thr = om_params
     d, lam, ite, ev, psi = r["d"], r["lpf"], r["itecnt"], r["events"], r["psi"]
-    EV_IMAG, EV_NEG, EV_DNORM, EV_HALVE = 1, 2, 3, 4
+    EV_IMAG, EV_NEG, EV_DNORM, EV_HALVE, EV_YIELD = 1, 2, 3, 4, 5
     qq = float(q @ q)
     dn0 = np.linalg.norm(d[0])
     out = {}
@@ -138,7 +139,7 @@ def arc_residuals(om_params, r, q):
         beta = math.sqrt(iteopt / ite[s - 1]) * alpha * dn0 / np.linalg.norm(dprev)
         load = psi[s] >= thr
         arc = beta * math.sqrt(dprev @ dprev + (lprev * lprev * qq if load else 0.0))
-        arc *= 0.5 ** int(ev[s, EV_IMAG] + ev[s, EV_NEG] + ev[s, EV_HALVE])
+        arc *= 0.5 ** int(ev[s, EV_IMAG] + ev[s, EV_NEG] + ev[s, EV_HALVE] + ev[s, EV_YIELD])
         dd = d[s] - d[s - 1]
         dl = lam[s] - lam[s - 1]
         lhs = dd @ dd + (dl * dl * qq if load else 0.0)

@@ -109,7 +109,10 @@ def _against_fixture(gp, fx):
     n = len(fx["lpf"])
     assert gp["status"] == int(fx["status"]) and gp["nsteps"] == n
     assert np.array_equal(gp["itecnt"], fx["itecnt"])
-    assert np.array_equal(gp["events"], fx["events"])
+    # fixtures of elastic runs keep the first five event slots; the yield and
+    # unload slots after them stay zero
+    ne = fx["events"].shape[1]
+    assert np.array_equal(gp["events"][:, :ne], fx["events"]) and not gp["events"][:, ne:].any()
     assert np.abs(gp["psi"] - fx["psi"]).max() <= 1e-6
     assert relerr(gp["lpf"], fx["lpf"]) <= 1e-9
     d = gp["d"]
