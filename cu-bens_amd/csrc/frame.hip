@@ -4,13 +4,16 @@
 // One thread per member.  The 14x14 local matrix is never stored: every entry
 // is a closed-form expression of ~40 per-member scalars (stiffe_fr /
 // stiffg_fr, frame.c:354-579), evaluated where the fully unrolled transform
-// loops ask for it, so the compiler folds the zero pattern away.  The device
-// layout carries 6 slots per joint; warping DOFs must be fixed (every member
-// then has 12 active DOF).  Members of one colour share no joint, so the
-// block adds and force scatters of one launch never collide.
+// loops ask for it, so the compiler folds the zero pattern away.  Joints carry
+// BS = 6 slots (warping fixed everywhere: 12 active DOF per member) or BS = 7
+// (the warping DOF in slot 6, shared by the members at the joint as codes()
+// numbers it, model.c:951-957; the transformation is 1 on it).  Members of one
+// colour share no joint, so the block adds and force scatters of one launch
+// never collide.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "dev.h"
 
@@ -273,7 +276,7 @@ struct PlasticIn {
 // T = blockdiag(R, R, 1, R, R, 1), one 3x3 sub-block (pair of triples) at a
 // time, added into the four joint blocks of the member.  PL: less the plastic
 // part of k (ANAFLAG 3).
-template <bool PL>
+template <bool PL, int BS>
 __global__ void __launch_bounds__(TPB) k_frame_tangent(
     const int32_t* __restrict__ els, int n, const int32_t* __restrict__ blk,
     const double* __restrict__ c1, const double* __restrict__ c2, const double* __restrict__ c3,
@@ -310,13 +313,36 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent(
                     tmp[r][l] = R[0][r] * kent<PL>(k, pl, tidx(t1, 0), tidx(t2, l)) +
                                 R[1][r] * kent<PL>(k, pl, tidx(t1, 1), tidx(t2, l)) +
                                 R[2][r] * kent<PL>(k, pl, tidx(t1, 2), tidx(t2, l));
-            double* Kb = K + (long)bk[(t1 / 2) * 2 + t2 / 2] * 36 + ((t1 % 2) * 3) * 6 + (t2 % 2) * 3;
+            double* Kb = K + (long)bk[(t1 / 2) * 2 + t2 / 2] * (BS * BS) + ((t1 % 2) * 3) * BS + (t2 % 2) * 3;
 #pragma unroll
             for (int r = 0; r < 3; ++r)
 #pragma unroll
                 for (int s = 0; s < 3; ++s)
-                    Kb[r * 6 + s] += tmp[r][0] * R[0][s] + tmp[r][1] * R[1][s] + tmp[r][2] * R[2][s];
+                    Kb[r * BS + s] += tmp[r][0] * R[0][s] + tmp[r][1] * R[1][s] + tmp[r][2] * R[2][s];
         }
+    if constexpr (BS == 7) {
+        // warping rows and columns: T is 1 on them
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                double* Kb = K + (long)bk[a * 2 + b] * 49;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int t = 2 * a + h, tb = 2 * b + h;
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                        Kb[(h * 3 + r) * 7 + 6] += R[0][r] * kent<PL>(k, pl, tidx(t, 0), 7 * b + 6) +
+                                                   R[1][r] * kent<PL>(k, pl, tidx(t, 1), 7 * b + 6) +
+                                                   R[2][r] * kent<PL>(k, pl, tidx(t, 2), 7 * b + 6);
+                        Kb[6 * 7 + h * 3 + r] += kent<PL>(k, pl, 7 * a + 6, tidx(tb, 0)) * R[0][r] +
+                                                 kent<PL>(k, pl, 7 * a + 6, tidx(tb, 1)) * R[1][r] +
+                                                 kent<PL>(k, pl, 7 * a + 6, tidx(tb, 2)) * R[2][r];
+                    }
+                }
+                Kb[48] += kent<PL>(k, pl, 7 * a + 6, 7 * b + 6);
+            }
+    }
 }
 
 // The same tangent as k_frame_tangent, gathered: two threads per joint block,
@@ -326,9 +352,32 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent(
 // with the rows and columns of fixed slots as identity (k_fix_slots).  No
 // zeroing and no read-modify-write of K; a member's scalars are evaluated by
 // each thread whose block it touches.
-template <int RA, int CB, int U, bool PL>
-__device__ __forceinline__ void frame_half_add(double acc[3][6], const FrameCoef& k, const FramePl& pl,
+// U 0 / 1: rows 3U..3U+2 of the block (a triple of end RA) over the columns
+// of end CB; U 2: the warping row of end RA.  BS 7 adds the warping column.
+template <int RA, int CB, int U, bool PL, int BS>
+__device__ __forceinline__ void frame_half_add(double acc[3][7], const FrameCoef& k, const FramePl& pl,
                                                const double R[3][3]) {
+    if constexpr (U == 2) {
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+            const int t2 = 2 * CB + v;
+#pragma unroll
+            for (int s = 0; s < 3; ++s)
+                acc[0][v * 3 + s] += kent<PL>(k, pl, 7 * RA + 6, tidx(t2, 0)) * R[0][s] +
+                                     kent<PL>(k, pl, 7 * RA + 6, tidx(t2, 1)) * R[1][s] +
+                                     kent<PL>(k, pl, 7 * RA + 6, tidx(t2, 2)) * R[2][s];
+        }
+        acc[0][6] += kent<PL>(k, pl, 7 * RA + 6, 7 * CB + 6);
+        return;
+    } else {
+    if constexpr (BS == 7) {
+        const int t1 = 2 * RA + U;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            acc[r][6] += R[0][r] * kent<PL>(k, pl, tidx(t1, 0), 7 * CB + 6) +
+                         R[1][r] * kent<PL>(k, pl, tidx(t1, 1), 7 * CB + 6) +
+                         R[2][r] * kent<PL>(k, pl, tidx(t1, 2), 7 * CB + 6);
+    }
 #pragma unroll
     for (int v = 0; v < 2; ++v) {
         const int t1 = 2 * RA + U, t2 = 2 * CB + v;
@@ -346,6 +395,7 @@ __device__ __forceinline__ void frame_half_add(double acc[3][6], const FrameCoef
             for (int s = 0; s < 3; ++s)
                 acc[r][v * 3 + s] += tmp[r][0] * R[0][s] + tmp[r][1] * R[1][s] + tmp[r][2] * R[2][s];
     }
+    }
 }
 
 // TO_BAND: instead of the joint block, the entries of free rows and columns on
@@ -353,7 +403,7 @@ __device__ __forceinline__ void frame_half_add(double acc[3][6], const FrameCoef
 // BandLDLT / BandBatch make from K, with the same values): tile (I, J) of the
 // equation pair (i, j) is tiles[(I (w + 1) + J - I + w) T^2], entry (j % T, i % T)
 // column-major; an entry more than w tiles below the diagonal sets *bad = 2.
-template <bool TO_BAND, bool PL>
+template <bool TO_BAND, bool PL, int BS>
 __global__ void __launch_bounds__(TPB) k_frame_tangent_gather(
     const int32_t* __restrict__ gptr, const int32_t* __restrict__ gitem, const int32_t* __restrict__ blkrow,
     const int32_t* __restrict__ colidx, const int32_t* __restrict__ eq, long nnzb,
@@ -363,15 +413,17 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent_gather(
     const double* __restrict__ ipolar, const double* __restrict__ iwarp,
     const double* __restrict__ defllen, const double* __restrict__ ef, double* __restrict__ K, int geometric,
     int w, int* __restrict__ bad, PlasticIn pin) {
+    // NU threads per block: rows 0-2 and 3-5, and with BS 7 the warping row
+    constexpr int NU = BS == 7 ? 3 : 2;
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    const long kb = t >> 1;
-    const int u = int(t & 1);
+    const long kb = t / NU;
+    const int u = int(t % NU);
     if (kb >= nnzb) return;
-    double acc[3][6];
+    double acc[3][7];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 6; ++j) acc[i][j] = 0.0;
+        for (int j = 0; j < 7; ++j) acc[i][j] = 0.0;
     for (int32_t it = gptr[kb]; it < gptr[kb + 1]; ++it) {
         const int32_t item = gitem[it];
         const long e = item >> 2;
@@ -384,27 +436,39 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent_gather(
                           yield_ref(carea, pin.yield, pin.zstrong, pin.zweak, e, 0));
         double R[3][3];
         load_triad(c1, c2, c3, e * 3, R);
-        switch ((item & 3) * 2 + u) {
-        case 0: frame_half_add<0, 0, 0, PL>(acc, k, pl, R); break;
-        case 1: frame_half_add<0, 0, 1, PL>(acc, k, pl, R); break;
-        case 2: frame_half_add<0, 1, 0, PL>(acc, k, pl, R); break;
-        case 3: frame_half_add<0, 1, 1, PL>(acc, k, pl, R); break;
-        case 4: frame_half_add<1, 0, 0, PL>(acc, k, pl, R); break;
-        case 5: frame_half_add<1, 0, 1, PL>(acc, k, pl, R); break;
-        case 6: frame_half_add<1, 1, 0, PL>(acc, k, pl, R); break;
-        default: frame_half_add<1, 1, 1, PL>(acc, k, pl, R); break;
+        switch ((item & 3) * NU + u) {
+        case 0 * NU + 0: frame_half_add<0, 0, 0, PL, BS>(acc, k, pl, R); break;
+        case 0 * NU + 1: frame_half_add<0, 0, 1, PL, BS>(acc, k, pl, R); break;
+        case 1 * NU + 0: frame_half_add<0, 1, 0, PL, BS>(acc, k, pl, R); break;
+        case 1 * NU + 1: frame_half_add<0, 1, 1, PL, BS>(acc, k, pl, R); break;
+        case 2 * NU + 0: frame_half_add<1, 0, 0, PL, BS>(acc, k, pl, R); break;
+        case 2 * NU + 1: frame_half_add<1, 0, 1, PL, BS>(acc, k, pl, R); break;
+        case 3 * NU + 0: frame_half_add<1, 1, 0, PL, BS>(acc, k, pl, R); break;
+        case 3 * NU + 1: frame_half_add<1, 1, 1, PL, BS>(acc, k, pl, R); break;
+        default:
+            if constexpr (BS == 7) {
+                switch (item & 3) {
+                case 0: frame_half_add<0, 0, 2, PL, BS>(acc, k, pl, R); break;
+                case 1: frame_half_add<0, 1, 2, PL, BS>(acc, k, pl, R); break;
+                case 2: frame_half_add<1, 0, 2, PL, BS>(acc, k, pl, R); break;
+                default: frame_half_add<1, 1, 2, PL, BS>(acc, k, pl, R); break;
+                }
+            }
+            break;
         }
     }
     const long A = blkrow[kb], B = colidx[kb];
+    const int rows = u < 2 ? 3 : 1;
     if constexpr (TO_BAND) {
         constexpr long BT = 64;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            const int32_t qi = eq[A * 6 + u * 3 + i];
+            if (i >= rows) break;
+            const int32_t qi = eq[A * BS + u * 3 + i];
             if (qi < 0) continue;
 #pragma unroll
-            for (int j = 0; j < 6; ++j) {
-                const int32_t qj = eq[B * 6 + j];
+            for (int j = 0; j < BS; ++j) {
+                const int32_t qj = eq[B * BS + j];
                 if (qj < 0 || qj > qi) continue;
                 const long I = qi / BT, J = qj / BT;
                 if (I - J > w) {
@@ -415,14 +479,15 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent_gather(
             }
         }
     } else {
-        double* Kb = K + kb * 36 + u * 18;
+        double* Kb = K + kb * (BS * BS) + u * 3 * BS;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            const bool rowfix = eq[A * 6 + u * 3 + i] < 0;
+            if (i >= rows) break;
+            const bool rowfix = eq[A * BS + u * 3 + i] < 0;
 #pragma unroll
-            for (int j = 0; j < 6; ++j) {
-                const bool colfix = eq[B * 6 + j] < 0;
-                Kb[i * 6 + j] = (rowfix || colfix) ? ((A == B && u * 3 + i == j) ? 1.0 : 0.0) : acc[i][j];
+            for (int j = 0; j < BS; ++j) {
+                const bool colfix = eq[B * BS + j] < 0;
+                Kb[i * BS + j] = (rowfix || colfix) ? ((A == B && u * 3 + i == j) ? 1.0 : 0.0) : acc[i][j];
             }
         }
     }
@@ -432,8 +497,9 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent_gather(
 //   dl = T_ip DD;  def = k(ef_ip) dl;  ef_i = T_i T_ip^T (ef_ip + def);  f += T_i^T ef_i,
 // one output triple at a time.
 // One member of forces_fr: ef_i, and the member's end forces in global axes
-// (T_i^T ef_i) handed to out(triple, m, value).
-template <class Out>
+// (T_i^T ef_i) handed to out(local DOF, value); with BS 7 the warping
+// increments enter and the bimoments are handed out too.
+template <int BS, class Out>
 __device__ __forceinline__ void frame_member_forces(
     long e, const int32_t* __restrict__ mc, const double* __restrict__ dd, const double* __restrict__ c1p,
     const double* __restrict__ c2p, const double* __restrict__ c3p, const double* __restrict__ c1c,
@@ -450,7 +516,8 @@ __device__ __forceinline__ void frame_member_forces(
     double Rp[3][3], Ri[3][3];
     load_triad(c1p, c2p, c3p, ntr + e * 3, Rp);
     load_triad(c1c, c2c, c3c, ntr + e * 3, Ri);
-    // local increments of the four triples; warping increments stay 0
+    // local increments of the four triples, and of the warping DOFs (T is 1
+    // there; zero with BS 6, where they are fixed)
     double dl[4][3];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
@@ -463,6 +530,13 @@ __device__ __forceinline__ void frame_member_forces(
 #pragma unroll
         for (int m = 0; m < 3; ++m) dl[t][m] = Rp[m][0] * D[0] + Rp[m][1] * D[1] + Rp[m][2] * D[2];
     }
+    double dw[2] = {0.0, 0.0};
+    if constexpr (BS == 7)
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const int32_t q = mc[7 * a + 6];
+            dw[a] = q ? dd[q - 1] : 0.0;
+        }
     double Q[3][3];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
@@ -479,6 +553,7 @@ __device__ __forceinline__ void frame_member_forces(
             for (int t = 0; t < 4; ++t)
 #pragma unroll
                 for (int m = 0; m < 3; ++m) s += kf(k, tidx(to, r), tidx(t, m)) * dl[t][m];
+            if constexpr (BS == 7) s += kf(k, tidx(to, r), 6) * dw[0] + kf(k, tidx(to, r), 13) * dw[1];
             v[r] = fe[tidx(to, r)] + s;
         }
         double g[3];
@@ -489,7 +564,7 @@ __device__ __forceinline__ void frame_member_forces(
         }
         // T_i^T ef_i on this triple
 #pragma unroll
-        for (int m = 0; m < 3; ++m) out(to, m, Ri[0][m] * g[0] + Ri[1][m] * g[1] + Ri[2][m] * g[2]);
+        for (int m = 0; m < 3; ++m) out(tidx(to, m), Ri[0][m] * g[0] + Ri[1][m] * g[1] + Ri[2][m] * g[2]);
     }
     // warping rows (bimoments): T_i T_ip^T is 1 there
 #pragma unroll
@@ -500,12 +575,15 @@ __device__ __forceinline__ void frame_member_forces(
         for (int t = 0; t < 4; ++t)
 #pragma unroll
             for (int m = 0; m < 3; ++m) s += kf(k, i, tidx(t, m)) * dl[t][m];
+        if constexpr (BS == 7) s += kf(k, i, 6) * dw[0] + kf(k, i, 13) * dw[1];
         ei[i] = fe[i] + s;
+        if constexpr (BS == 7) out(i, ei[i]);
     }
 }
 
 // forces_fr, elastic branch (frame.c:902-1096, 1273-1309), for the members of
 // one colour, scattered by mcode into f.
+template <int BS>
 __global__ void __launch_bounds__(TPB) k_frame_forces(
     const int32_t* __restrict__ els, int n, const int32_t* __restrict__ mcode,
     const double* __restrict__ dd, const double* __restrict__ c1p, const double* __restrict__ c2p,
@@ -520,15 +598,16 @@ __global__ void __launch_bounds__(TPB) k_frame_forces(
     if (it >= n) return;
     const long e = els[it];
     const int32_t* mc = mcode + ntr * 6 + e * 14;
-    frame_member_forces(e, mc, dd, c1p, c2p, c3p, c1c, c2c, c3c, emod, gmod, carea, llen, istrong, iweak, ipolar,
-                        iwarp, defllen_ip, ef_ip, ef_i, ntr, geometric, [&](int to, int m, double val) {
-                            const int32_t q = mc[tidx(to, m)];
-                            if (q) f[q - 1] += val;
-                        });
+    frame_member_forces<BS>(e, mc, dd, c1p, c2p, c3p, c1c, c2c, c3c, emod, gmod, carea, llen, istrong, iweak,
+                            ipolar, iwarp, defllen_ip, ef_ip, ef_i, ntr, geometric, [&](int i, double val) {
+                                const int32_t q = mc[i];
+                                if (q) f[q - 1] += val;
+                            });
 }
 
 // Frame-only models: every member in one launch, its global end forces into
-// h[e * 12 + 3 * triple + m] ...
+// h[e * 2 BS + BS * end + slot] ...
+template <int BS>
 __global__ void __launch_bounds__(TPB) k_frame_forces_members(
     long n, const int32_t* __restrict__ mcode, const double* __restrict__ dd, const double* __restrict__ c1p,
     const double* __restrict__ c2p, const double* __restrict__ c3p, const double* __restrict__ c1c,
@@ -539,16 +618,17 @@ __global__ void __launch_bounds__(TPB) k_frame_forces_members(
     double* __restrict__ ef_i, double* __restrict__ h, int geometric) {
     const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (e >= n) return;
-    double* he = h + e * 12;
-    frame_member_forces(e, mcode + e * 14, dd, c1p, c2p, c3p, c1c, c2c, c3c, emod, gmod, carea, llen, istrong,
-                        iweak, ipolar, iwarp, defllen_ip, ef_ip, ef_i, 0, geometric,
-                        [&](int to, int m, double val) { he[3 * to + m] = val; });
+    double* he = h + e * (2 * BS);
+    frame_member_forces<BS>(e, mcode + e * 14, dd, c1p, c2p, c3p, c1c, c2c, c3c, emod, gmod, carea, llen,
+                            istrong, iweak, ipolar, iwarp, defllen_ip, ef_ip, ef_i, 0, geometric,
+                            [&](int i, double val) { he[BS * (i / 7) + i % 7] = val; });
 }
 
 // ... then one thread per free slot adds the end forces of the members at its
 // joint in colour order, the order in which the colour launches add them;
 // with st, only the members before st->first (the members forces_fr went
 // through before it returned).
+template <int BS>
 __global__ void k_frame_forces_gather(const int32_t* __restrict__ jptr, const int32_t* __restrict__ jitem,
                                       const int32_t* __restrict__ eq, long nslot, const double* __restrict__ h,
                                       double* __restrict__ f, const PlasticStatus* __restrict__ st) {
@@ -556,14 +636,14 @@ __global__ void k_frame_forces_gather(const int32_t* __restrict__ jptr, const in
     if (r >= nslot) return;
     const int32_t q = eq[r];
     if (q < 0) return;
-    const long j = r / 6;
-    const int s = int(r % 6);
+    const long j = r / BS;
+    const int s = int(r % BS);
     const int32_t lim = st ? st->first : INT32_MAX;
     double acc = f[q];
     for (int32_t it = jptr[j]; it < jptr[j + 1]; ++it) {
         const int32_t item = jitem[it];
         if ((item >> 1) >= lim) continue;
-        acc += h[long(item >> 1) * 12 + 6 * (item & 1) + s];
+        acc += h[long(item >> 1) * (2 * BS) + BS * (item & 1) + s];
     }
     f[q] = acc;
 }
@@ -600,8 +680,10 @@ __device__ inline double regula_falsi(double p, double dp, double my, double dmy
 // gradients at the new end forces, an end unloading when lambda < -1e-8)
 // flags the unloading ends 2 with code 2.  code[4e] = code, code[4e+1..2] =
 // the member's flags after its pass; st->first <- the lowest member with a
-// code.  Warping increments are zero (the device fixes warping DOFs); no
-// member end offsets; the fixed-end forces are zero (no distributed loads).
+// code.  With BS 7 the warping increments and bimoments enter as in the
+// elastic pass; no member end offsets; the fixed-end forces are zero (no
+// distributed loads).
+template <int BS>
 __global__ void __launch_bounds__(64) k_frame_forces_plastic(
     long n, const int32_t* __restrict__ mcode, const double* __restrict__ dd, const double* __restrict__ c1p,
     const double* __restrict__ c2p, const double* __restrict__ c3p, const double* __restrict__ c1c,
@@ -638,12 +720,20 @@ __global__ void __launch_bounds__(64) k_frame_forces_plastic(
 #pragma unroll
         for (int m = 0; m < 3; ++m) dl[t][m] = Rp[m][0] * D[0] + Rp[m][1] * D[1] + Rp[m][2] * D[2];
     }
+    double dw[2] = {0.0, 0.0};
+    if constexpr (BS == 7)
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const int32_t q = mc[7 * a + 6];
+            dw[a] = q ? dd[q - 1] : 0.0;
+        }
     double Q[3][3];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int c = 0; c < 3; ++c) Q[r][c] = Ri[r][0] * Rp[c][0] + Ri[r][1] * Rp[c][1] + Ri[r][2] * Rp[c][2];
     double fi[14];
+    double* he = h + e * (2 * BS);
 #pragma unroll
     for (int to = 0; to < 4; ++to) {
         double v[3];
@@ -654,13 +744,18 @@ __global__ void __launch_bounds__(64) k_frame_forces_plastic(
             for (int t = 0; t < 4; ++t)
 #pragma unroll
                 for (int m = 0; m < 3; ++m) s += kent<true>(k, pl, tidx(to, r), tidx(t, m)) * dl[t][m];
+            if constexpr (BS == 7)
+                s += kent<true>(k, pl, tidx(to, r), 6) * dw[0] + kent<true>(k, pl, tidx(to, r), 13) * dw[1];
             v[r] = fe[tidx(to, r)] + s;
         }
 #pragma unroll
         for (int r = 0; r < 3; ++r) fi[tidx(to, r)] = Q[r][0] * v[0] + Q[r][1] * v[1] + Q[r][2] * v[2];
 #pragma unroll
-        for (int m = 0; m < 3; ++m)
-            h[e * 12 + 3 * to + m] = Ri[0][m] * fi[tidx(to, 0)] + Ri[1][m] * fi[tidx(to, 1)] + Ri[2][m] * fi[tidx(to, 2)];
+        for (int m = 0; m < 3; ++m) {
+            const int i = tidx(to, m);
+            he[BS * (i / 7) + i % 7] =
+                Ri[0][m] * fi[tidx(to, 0)] + Ri[1][m] * fi[tidx(to, 1)] + Ri[2][m] * fi[tidx(to, 2)];
+        }
     }
 #pragma unroll
     for (int w = 0; w < 2; ++w) {
@@ -670,7 +765,9 @@ __global__ void __launch_bounds__(64) k_frame_forces_plastic(
         for (int t = 0; t < 4; ++t)
 #pragma unroll
             for (int m = 0; m < 3; ++m) s += kent<true>(k, pl, i, tidx(t, m)) * dl[t][m];
+        if constexpr (BS == 7) s += kent<true>(k, pl, i, 6) * dw[0] + kent<true>(k, pl, i, 13) * dw[1];
         fi[i] = fe[i] + s;
+        if constexpr (BS == 7) he[BS * w + 6] = fi[i];
     }
 #pragma unroll
     for (int i = 0; i < 14; ++i) efc[e * 14 + i] = fi[i];
@@ -717,6 +814,9 @@ __global__ void __launch_bounds__(64) k_frame_forces_plastic(
                     for (int tt = 0; tt < 4; ++tt)
 #pragma unroll
                         for (int m = 0; m < 3; ++m) w[cc] += ul.u[cc][tidx(tt, m)] * dl[tt][m];
+                if constexpr (BS == 7)
+#pragma unroll
+                    for (int cc = 0; cc < 2; ++cc) w[cc] += ul.u[cc][6] * dw[0] + ul.u[cc][13] * dw[1];
                 const double l0 = ul.h[0][0] * w[0] + ul.h[0][1] * w[1];
                 const double l1 = ul.h[1][0] * w[0] + ul.h[1][1] * w[1];
                 int un = 0;
@@ -801,48 +901,63 @@ __global__ void k_update_frame(const int32_t* __restrict__ minc, const double* _
 
 namespace {
 PlasticIn plastic_in(const Dev& d) { return PlasticIn{d.yld, d.yield, d.zstrong, d.zweak}; }
+
+// f(std::integral_constant<int, BS>) for the joint slots of the model: 7 when
+// a warping DOF is free, else 6 (frames never run with 3)
+template <class F>
+void with_bs(const Dev& d, F&& f) {
+    if (d.bs == 7) f(std::integral_constant<int, 7>{});
+    else if (d.bs == 6) f(std::integral_constant<int, 6>{});
+    else throw DeviceError("frame members need 6 or 7 slots per joint");
+}
 }  // namespace
 
 // The geometric terms are on for ANAFLAG 2 and 3 (frame.c:266-277).
 void launch_frame_tangent(Dev& d, int g) {
     const int geo = d.anaflag == 2 || d.anaflag == 3;
-    for (size_t c = 0; c + 1 < d.col_ptr_fr.size(); ++c) {
-        int n = d.col_ptr_fr[c + 1] - d.col_ptr_fr[c];
-        if (!n) continue;
-        if (d.anaflag == 3)
-            hipLaunchKernelGGL(k_frame_tangent<true>, dim3(nblk(n)), dim3(TPB), 0, d.s,
-                               d.col_el_fr + d.col_ptr_fr[c], n, d.blk_fr, d.c1[g], d.c2[g], d.c3[g],
-                               d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp,
-                               d.defllen[g], d.ef[g], d.K, d.ntr, geo, plastic_in(d));
-        else
-            hipLaunchKernelGGL(k_frame_tangent<false>, dim3(nblk(n)), dim3(TPB), 0, d.s,
-                               d.col_el_fr + d.col_ptr_fr[c], n, d.blk_fr, d.c1[g], d.c2[g], d.c3[g],
-                               d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp,
-                               d.defllen[g], d.ef[g], d.K, d.ntr, geo, PlasticIn{});
-    }
+    with_bs(d, [&](auto B) {
+        constexpr int BS = decltype(B)::value;
+        for (size_t c = 0; c + 1 < d.col_ptr_fr.size(); ++c) {
+            int n = d.col_ptr_fr[c + 1] - d.col_ptr_fr[c];
+            if (!n) continue;
+            if (d.anaflag == 3)
+                hipLaunchKernelGGL((k_frame_tangent<true, BS>), dim3(nblk(n)), dim3(TPB), 0, d.s,
+                                   d.col_el_fr + d.col_ptr_fr[c], n, d.blk_fr, d.c1[g], d.c2[g], d.c3[g],
+                                   d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp,
+                                   d.defllen[g], d.ef[g], d.K, d.ntr, geo, plastic_in(d));
+            else
+                hipLaunchKernelGGL((k_frame_tangent<false, BS>), dim3(nblk(n)), dim3(TPB), 0, d.s,
+                                   d.col_el_fr + d.col_ptr_fr[c], n, d.blk_fr, d.c1[g], d.c2[g], d.c3[g],
+                                   d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp,
+                                   d.defllen[g], d.ef[g], d.K, d.ntr, geo, PlasticIn{});
+        }
+    });
     CB_CHECK(hipGetLastError());
 }
 
 void launch_frame_tangent_gather(Dev& d, int g) {
-    if (d.bs != 6) throw DeviceError("frame tangent gather: frame models carry 6 slots per joint");
     const int geo = d.anaflag == 2 || d.anaflag == 3;
-    if (d.anaflag == 3)
-        hipLaunchKernelGGL((k_frame_tangent_gather<false, true>), dim3(nblk(2 * d.nnzb)), dim3(TPB), 0, d.s,
-                           d.gat_ptr_fr, d.gat_item_fr, d.blkrow, d.colidx, d.eq_of_slot, d.nnzb, d.c1[g], d.c2[g],
-                           d.c3[g], d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp,
-                           d.defllen[g], d.ef[g], d.K, geo, 0, nullptr, plastic_in(d));
-    else
-        hipLaunchKernelGGL((k_frame_tangent_gather<false, false>), dim3(nblk(2 * d.nnzb)), dim3(TPB), 0, d.s,
-                           d.gat_ptr_fr, d.gat_item_fr, d.blkrow, d.colidx, d.eq_of_slot, d.nnzb, d.c1[g], d.c2[g],
-                           d.c3[g], d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp,
-                           d.defllen[g], d.ef[g], d.K, geo, 0, nullptr, PlasticIn{});
+    with_bs(d, [&](auto B) {
+        constexpr int BS = decltype(B)::value;
+        const long nth = (BS == 7 ? 3 : 2) * d.nnzb;
+        if (d.anaflag == 3)
+            hipLaunchKernelGGL((k_frame_tangent_gather<false, true, BS>), dim3(nblk(nth)), dim3(TPB), 0, d.s,
+                               d.gat_ptr_fr, d.gat_item_fr, d.blkrow, d.colidx, d.eq_of_slot, d.nnzb, d.c1[g],
+                               d.c2[g], d.c3[g], d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar,
+                               d.iwarp, d.defllen[g], d.ef[g], d.K, geo, 0, nullptr, plastic_in(d));
+        else
+            hipLaunchKernelGGL((k_frame_tangent_gather<false, false, BS>), dim3(nblk(nth)), dim3(TPB), 0, d.s,
+                               d.gat_ptr_fr, d.gat_item_fr, d.blkrow, d.colidx, d.eq_of_slot, d.nnzb, d.c1[g],
+                               d.c2[g], d.c3[g], d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar,
+                               d.iwarp, d.defllen[g], d.ef[g], d.K, geo, 0, nullptr, PlasticIn{});
+    });
     CB_CHECK(hipGetLastError());
 }
 
 void launch_frame_tangent_band(Dev& d, int g, double* tiles, int w, int* bad) {
-    if (d.bs != 6 || d.ntr || d.nsh) throw DeviceError("frame tangent into the band: frame-only models");
+    if (d.bs != 6 || d.ntr || d.nsh) throw DeviceError("frame tangent into the band: frame-only models, warping fixed");
     if (d.anaflag == 3) throw DeviceError("frame tangent into the band: elastic (ANAFLAG 2) members only");
-    hipLaunchKernelGGL((k_frame_tangent_gather<true, false>), dim3(nblk(2 * d.nnzb)), dim3(TPB), 0, d.s,
+    hipLaunchKernelGGL((k_frame_tangent_gather<true, false, 6>), dim3(nblk(2 * d.nnzb)), dim3(TPB), 0, d.s,
                        d.gat_ptr_fr, d.gat_item_fr, d.blkrow, d.colidx, d.eq_of_slot, d.nnzb, d.c1[g], d.c2[g],
                        d.c3[g], d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp,
                        d.defllen[g], d.ef[g], tiles, int(d.anaflag == 2), w, bad, PlasticIn{});
@@ -853,16 +968,18 @@ void launch_frame_tangent_band(Dev& d, int g, double* tiles, int w, int* bad) {
 // to the first that returns take it and their end forces before that member
 // are gathered into f (the reference's loop stops at a return).
 static void launch_frame_forces_plastic(Dev& d, double* f, const double* dd) {
-    if (d.bs != 6) throw DeviceError("plastic frame forces: frame models carry 6 slots per joint");
-    hipLaunchKernelGGL(k_plastic_init, dim3(1), dim3(1), 0, d.s, d.pl_st, d.nfr);
-    hipLaunchKernelGGL(k_frame_forces_plastic, dim3(nblk(d.nfr, 64)), dim3(64), 0, d.s, d.nfr, d.mcode, dd,
-                       d.c1[G_IP], d.c2[G_IP], d.c3[G_IP], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.emod, d.gmod, d.carea,
-                       d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[G_IP], d.ef[G_IP], plastic_in(d),
-                       d.pl_efc, d.fh, d.pl_code, d.pl_tau, d.pl_st, d.ntr);
-    hipLaunchKernelGGL(k_frame_plastic_commit, dim3(nblk(d.nfr, 256)), dim3(256), 0, d.s, d.nfr, d.pl_efc,
-                       d.pl_code, d.pl_tau, d.pl_st, d.yld, d.ef[G_I], d.ntr);
-    hipLaunchKernelGGL(k_frame_forces_gather, dim3(nblk(d.nslot, 256)), dim3(256), 0, d.s, d.jnt_ptr_fr,
-                       d.jnt_item_fr, d.eq_of_slot, d.nslot, d.fh, f, d.pl_st);
+    with_bs(d, [&](auto B) {
+        constexpr int BS = decltype(B)::value;
+        hipLaunchKernelGGL(k_plastic_init, dim3(1), dim3(1), 0, d.s, d.pl_st, d.nfr);
+        hipLaunchKernelGGL((k_frame_forces_plastic<BS>), dim3(nblk(d.nfr, 64)), dim3(64), 0, d.s, d.nfr, d.mcode,
+                           dd, d.c1[G_IP], d.c2[G_IP], d.c3[G_IP], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.emod, d.gmod,
+                           d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[G_IP], d.ef[G_IP],
+                           plastic_in(d), d.pl_efc, d.fh, d.pl_code, d.pl_tau, d.pl_st, d.ntr);
+        hipLaunchKernelGGL(k_frame_plastic_commit, dim3(nblk(d.nfr, 256)), dim3(256), 0, d.s, d.nfr, d.pl_efc,
+                           d.pl_code, d.pl_tau, d.pl_st, d.yld, d.ef[G_I], d.ntr);
+        hipLaunchKernelGGL((k_frame_forces_gather<BS>), dim3(nblk(d.nslot, 256)), dim3(256), 0, d.s,
+                           d.jnt_ptr_fr, d.jnt_item_fr, d.eq_of_slot, d.nslot, d.fh, f, d.pl_st);
+    });
     CB_CHECK(hipGetLastError());
 }
 
@@ -888,25 +1005,27 @@ void launch_frame_forces(Dev& d, double* f, const double* dd) {
         launch_frame_forces_plastic(d, f, dd);
         return;
     }
-    if (!d.ntr && !d.nsh && d.bs == 6) {
-        hipLaunchKernelGGL(k_frame_forces_members, dim3(nblk(d.nfr)), dim3(TPB), 0, d.s, d.nfr, d.mcode, dd,
-                           d.c1[G_IP], d.c2[G_IP], d.c3[G_IP], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.emod, d.gmod,
-                           d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[G_IP], d.ef[G_IP],
-                           d.ef[G_I], d.fh, int(d.anaflag == 2));
-        hipLaunchKernelGGL(k_frame_forces_gather, dim3(nblk(d.nslot, 256)), dim3(256), 0, d.s, d.jnt_ptr_fr,
-                           d.jnt_item_fr, d.eq_of_slot, d.nslot, d.fh, f, nullptr);
-        CB_CHECK(hipGetLastError());
-        return;
-    }
-    for (size_t c = 0; c + 1 < d.col_ptr_fr.size(); ++c) {
-        int n = d.col_ptr_fr[c + 1] - d.col_ptr_fr[c];
-        if (!n) continue;
-        hipLaunchKernelGGL(k_frame_forces, dim3(nblk(n)), dim3(TPB), 0, d.s,
-                           d.col_el_fr + d.col_ptr_fr[c], n, d.mcode, dd, d.c1[G_IP], d.c2[G_IP],
-                           d.c3[G_IP], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.emod, d.gmod, d.carea,
-                           d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[G_IP],
-                           d.ef[G_IP], d.ef[G_I], f, d.ntr, int(d.anaflag == 2));
-    }
+    with_bs(d, [&](auto B) {
+        constexpr int BS = decltype(B)::value;
+        if (!d.ntr && !d.nsh) {
+            hipLaunchKernelGGL((k_frame_forces_members<BS>), dim3(nblk(d.nfr)), dim3(TPB), 0, d.s, d.nfr, d.mcode,
+                               dd, d.c1[G_IP], d.c2[G_IP], d.c3[G_IP], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.emod,
+                               d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[G_IP],
+                               d.ef[G_IP], d.ef[G_I], d.fh, int(d.anaflag == 2));
+            hipLaunchKernelGGL((k_frame_forces_gather<BS>), dim3(nblk(d.nslot, 256)), dim3(256), 0, d.s,
+                               d.jnt_ptr_fr, d.jnt_item_fr, d.eq_of_slot, d.nslot, d.fh, f, nullptr);
+            return;
+        }
+        for (size_t c = 0; c + 1 < d.col_ptr_fr.size(); ++c) {
+            int n = d.col_ptr_fr[c + 1] - d.col_ptr_fr[c];
+            if (!n) continue;
+            hipLaunchKernelGGL((k_frame_forces<BS>), dim3(nblk(n)), dim3(TPB), 0, d.s,
+                               d.col_el_fr + d.col_ptr_fr[c], n, d.mcode, dd, d.c1[G_IP], d.c2[G_IP],
+                               d.c3[G_IP], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.emod, d.gmod, d.carea,
+                               d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[G_IP],
+                               d.ef[G_IP], d.ef[G_I], f, d.ntr, int(d.anaflag == 2));
+        }
+    });
     CB_CHECK(hipGetLastError());
 }
 

@@ -867,7 +867,10 @@ void from_slots(Dev& d, double* v, const double* ext) {
 }
 
 void bsr_spmv(Dev& d, const double* x, double* y) {
-    if (d.bs == 6)
+    if (d.bs == 7)
+        hipLaunchKernelGGL((k_bsr_spmv<double, 7>), dim3(nblk(d.nslot)), dim3(TPB), 0, d.s, d.K,
+                           d.rowptr, d.colidx, x, y, d.nj);
+    else if (d.bs == 6)
         hipLaunchKernelGGL((k_bsr_spmv<double, 6>), dim3(nblk(d.nslot)), dim3(TPB), 0, d.s, d.K,
                            d.rowptr, d.colidx, x, y, d.nj);
     else
@@ -875,7 +878,10 @@ void bsr_spmv(Dev& d, const double* x, double* y) {
                            d.rowptr, d.colidx, x, y, d.nj);
 }
 void bsr_spmv32(Dev& d, const double* x, double* y) {
-    if (d.bs == 6)
+    if (d.bs == 7)
+        hipLaunchKernelGGL((k_bsr_spmv<float, 7>), dim3(nblk(d.nslot)), dim3(TPB), 0, d.s, d.K32,
+                           d.rowptr, d.colidx, x, y, d.nj);
+    else if (d.bs == 6)
         hipLaunchKernelGGL((k_bsr_spmv<float, 6>), dim3(nblk(d.nslot)), dim3(TPB), 0, d.s, d.K32,
                            d.rowptr, d.colidx, x, y, d.nj);
     else
@@ -928,7 +934,7 @@ void dev_create(Dev& d, const Model& m) {
     d.gat_item_fr = upi(m.gat_item_fr);
     d.jnt_ptr_fr = upi(m.jnt_ptr_fr);
     d.jnt_item_fr = upi(m.jnt_item_fr);
-    d.fh = dalloc<double>(size_t(m.nfr * 12));
+    d.fh = dalloc<double>(size_t(m.nfr * 14));   // 2 bs end forces per member
     d.emod = upd(m.emod);
     d.carea = upd(m.carea);
     d.llen = upd(m.llength);

@@ -312,9 +312,15 @@ bool load_model(const char* path, Model& m, std::string& err) {
     autofix_slots(m);
     if (m.optflag == 2) renumber_joints(m);
     number_dofs(m);
-    // the device layout carries 6 slots per joint: warping DOFs must be fixed
-    for (long j = 0; j < nj; ++j)
-        if (m.jcode[j * 7 + 6] != 0) FAIL("free warping DOFs are not carried by the device layout");
+    // a free warping DOF shared by the members at its joint is slot 6 of a
+    // 7-slot joint; per-member warping DOFs (restraint 1, model.c:951-957,
+    // 1017-1031) have no joint slot
+    for (long j = 0; j < nj; ++j) {
+        if (m.wrpres[j * 3] == 1 && m.jcode[j * 7 + 6] != 0)
+            FAIL("per-member warping DOFs (warping restraint 1) are not carried by the device layout");
+        if (m.jcode[j * 7 + 6] != 0 && nsh > 0)
+            FAIL("free warping DOFs in a model with shell elements are not carried by the device layout");
+    }
     if (m.algflag > 3) {
         long j = t.l(), k = t.l();
         if (j != 0 || k != 0) FAIL("prescribed displacement DOFs (NBC > 0) are not on this path");
@@ -533,9 +539,11 @@ bool load_model(const char* path, Model& m, std::string& err) {
 }
 
 void build_pattern(Model& m) {
-    // Warping DOFs are not carried by the device layout (the frame workloads fix
-    // them); joints carry 3 translation slots in truss-only models, else 6.
-    m.bs = (m.nfr == 0 && m.nsh == 0) ? 3 : 6;
+    // Joints carry 3 translation slots in truss-only models, 7 in frame models
+    // with a free (shared) warping DOF, else 6.
+    bool warping = false;
+    for (long j = 0; j < m.nj; ++j) warping = warping || m.jcode[j * 7 + 6] != 0;
+    m.bs = (m.nfr == 0 && m.nsh == 0) ? 3 : (warping ? 7 : 6);
     const int bs = m.bs;
     m.eq_of_slot.assign(size_t(m.nj * bs), -1);
     for (long j = 0; j < m.nj; ++j)

@@ -359,7 +359,9 @@ int Fgmres::solve(Dev& d, const std::function<void(const double*, double*)>& pre
 // ------------------------------------------------------------------ block Jacobi
 void BlockJacobi::setup(Dev& d) {
     if (!inv) inv = dalloc<double>(size_t(d.nj * d.bs * d.bs));
-    if (d.bs == 6)
+    if (d.bs == 7)
+        hipLaunchKernelGGL(k_bj_setup<7>, dim3(nblk(d.nj)), dim3(TPB), 0, d.s, d.K, d.diagblk, inv, d.nj);
+    else if (d.bs == 6)
         hipLaunchKernelGGL(k_bj_setup<6>, dim3(nblk(d.nj)), dim3(TPB), 0, d.s, d.K, d.diagblk, inv, d.nj);
     else
         hipLaunchKernelGGL(k_bj_setup<3>, dim3(nblk(d.nj)), dim3(TPB), 0, d.s, d.K, d.diagblk, inv, d.nj);
@@ -367,7 +369,9 @@ void BlockJacobi::setup(Dev& d) {
 }
 
 void BlockJacobi::apply(Dev& d, const double* v, double* z) {
-    if (d.bs == 6)
+    if (d.bs == 7)
+        hipLaunchKernelGGL(k_bj_apply<7>, dim3(nblk(d.nslot)), dim3(TPB), 0, d.s, inv, v, z, d.nj);
+    else if (d.bs == 6)
         hipLaunchKernelGGL(k_bj_apply<6>, dim3(nblk(d.nslot)), dim3(TPB), 0, d.s, inv, v, z, d.nj);
     else
         hipLaunchKernelGGL(k_bj_apply<3>, dim3(nblk(d.nslot)), dim3(TPB), 0, d.s, inv, v, z, d.nj);

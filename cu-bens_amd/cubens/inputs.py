@@ -426,7 +426,7 @@ def space_frame(n_elem: int = 400, kind: str = "tower", seed: int | None = None,
                 tol=(1e-8, 1e-8, 1e-8), algflag: int = 5, numopt: int = 0,
                 spectrds: float = 1.0, itemax: int = 20, slice_shape: tuple | None = None,
                 optflag: int = 1, scramble: int | None = None, chkpt: int | None = None,
-                rflag: int = 0) -> ModelText:
+                rflag: int = 0, warping: bool = False) -> ModelText:
     """A random tower or lattice girder of about ``n_elem`` frame members.
 
     tower : square storeys of 4 columns, 4 beams and 8 face X-braces; plan
@@ -449,6 +449,9 @@ def space_frame(n_elem: int = 400, kind: str = "tower", seed: int | None = None,
     optflag 2 loads other joints than the same model with optflag 1.
     ``chkpt`` and ``rflag`` are the Newmark checkpoint interval and restart flag
     (main.c:347-371); by default the interval lies beyond the last step.
+    ``warping`` leaves the warping DOFs free (shared by the members at each
+    joint, model.c:951-957) except at the fixed joints, and gives the sections
+    a warping constant Cw = I r_o^2 / 4.
     """
     rng = SplitMix64(workload_seed(4, instance) if seed is None else seed)
     joints, members, fixed, loaded = [], [], [], []
@@ -518,7 +521,7 @@ def space_frame(n_elem: int = 400, kind: str = "tower", seed: int | None = None,
     for j in fixed:
         for d in range(1, 7):
             w(f"{jo(j)},{d}\n")
-    for j in range(1, nj + 1):
+    for j in (sorted(jo(f) for f in fixed) if warping else range(1, nj + 1)):
         w(f"{j},7,0\n")
     w("0,0\n")
     if algflag > 3:
@@ -529,9 +532,10 @@ def space_frame(n_elem: int = 400, kind: str = "tower", seed: int | None = None,
         p = X[k]
         w(f"{_fmt(p[0])},{_fmt(p[1])},{_fmt(p[2])}\n")
     secs = [chs_section(r) for r in r_o]
-    for s_ in secs:
+    for s_, r in zip(secs, r_o):
+        cw = s_["I"] * r * r / 4 if warping else 0.0
         w(f"{_fmt(s_['E'])},{_fmt(s_['G'])},7850.0,{_fmt(s_['A'])},{_fmt(s_['I'])},"
-          f"{_fmt(s_['I'])},{_fmt(s_['J'])},0.0\n")
+          f"{_fmt(s_['I'])},{_fmt(s_['J'])},{_fmt(cw)}\n")
     w("0,0,0,0,0,0,0\n")
     for a, b_ in members:
         pa, pb = X[a - 1], X[b_ - 1]
@@ -797,6 +801,57 @@ def three_bar_truss(height: float = 1.0, theta: float = math.pi / 4, emod: float
     return ModelText(out.getvalue(), 4, n_tr=3,
                      info={"Py": Py, "P1": P1, "Pc": Pc, "height": height, "theta": theta,
                            "emod": emod, "area": area, "load": load, "neq": 2})
+
+
+def torsion_cantilever(n_elem: int = 16, length: float = 2.0, E: float = 2e11, G: float = 7.7e10,
+                       J: float = 2e-7, Cw: float = 4e-9, A: float = 5e-3, I: float = 2e-5,
+                       torque: float = 100.0, root_warping: str = "fixed", analysis: str = "static",
+                       lpfmax: float = 1.0, dlpf: float | None = None,
+                       tol=(1e-10, 1e-10, 1e-10)) -> ModelText:
+    """A straight cantilever of ``n_elem`` thin-walled members along x under a
+    tip torque (direction 4), clamped at joint 1 with its warping DOF
+    ``root_warping`` ("fixed" or "free") and free to warp at every other
+    joint (the warping DOFs shared by the members at a joint, model.c:951-957):
+    Vlasov's non-uniform torsion.  With k^2 = GJ / (E Cw), the tip twist of a
+    cantilever whose root cannot warp is (Vlasov, Timoshenko & Gere)
+        theta(L) = T / (GJ) (L - tanh(kL) / k),
+    and with the root free to warp it is Saint-Venant's T L / (GJ).
+    ``analysis`` "static" is linear (ANAFLAG 1); "nr" geometrically nonlinear
+    NR load control (ANAFLAG 2) to ``lpfmax`` in steps of ``dlpf``."""
+    nj = n_elem + 1
+    out = io.StringIO()
+    w = out.write
+    w("1\n1\n" if analysis == "static" else "2\n1\n")
+    w(f"0\n1\n{nj}\n0,{n_elem},0,0,0\n")
+    for e in range(n_elem):
+        w(f"{e + 1},{e + 2}\n")
+    for d in range(1, 7):
+        w(f"1,{d}\n")
+    if root_warping == "fixed":
+        w("1,7,0\n")
+    w("0,0\n")
+    for j in range(nj):
+        w(f"{_fmt(length * j / n_elem)},0.0,0.0\n")
+    for _ in range(n_elem):
+        w(f"{_fmt(E)},{_fmt(G)},7850.0,{_fmt(A)},{_fmt(I)},{_fmt(I)},{_fmt(J)},{_fmt(Cw)}\n")
+    w("0,0,0,0,0,0,0\n")
+    for e in range(n_elem):
+        w(f"{_fmt(length * e / n_elem)},0.0,1.0\n")
+    for _ in range(n_elem):
+        w("1e+30,1.0,1.0\n")
+    w("0,0,0,0,0\n")
+    if analysis == "static":
+        w(f"{nj},4,{_fmt(torque)}\n0,0,0\n0,0,0\n{_fmt(lpfmax)}\n")
+    else:
+        step = lpfmax / 5 if dlpf is None else dlpf
+        nr = dict(lpfmax=lpfmax, lpf=step, dlpf=step, dlpfmax=step, dlpfmin=step / 64, itemax=30,
+                  submax=10, solmin=1000, tol=tol)
+        _nonlinear_tail(w, "nr", [(nj, 4, torque)], nr, None, None, True)
+    k = math.sqrt(G * J / (E * Cw))
+    return ModelText(out.getvalue(), nj, n_fr=n_elem,
+                     info={"E": E, "G": G, "J": J, "Cw": Cw, "k": k, "length": length, "torque": torque,
+                           "theta_fixed": torque / (G * J) * (length - math.tanh(k * length) / k),
+                           "theta_free": torque * length / (G * J)})
 
 
 def frame_batch(count: int = 512, lo: int = 200, hi: int = 5000, **kw) -> list[ModelText]:

@@ -27,8 +27,8 @@ def _opts(steps=250, **kw):
     return cubens.options(cubens.SOLVER_BAND64, max_steps=steps, **kw)
 
 
-# the reference's sample leaves the warping DOFs free, which the device
-# layout (six slots per joint) does not carry: it runs below with them fixed
+# the reference's sample leaves the warping DOFs free (7 slots per joint); it
+# runs as given in test_gpu_warping.py, and here with the warping fixed
 GPU_CASES = tuple(c for c in CASES if c != "sample_5b")
 
 
@@ -72,12 +72,6 @@ def test_sample_5b_with_warping_fixed_matches_the_oracle():
     assert r["nsteps"] == o["nsteps"] == 17
     assert np.array_equal(r["itecnt"], o["itecnt"]) and np.array_equal(r["events"], o["events"])
     assert relerr(r["lpf"], o["lpf"]) < 1e-12 and relerr(r["d"], o["d"]) < 1e-9
-
-
-def test_sample_5b_free_warping_is_refused_with_its_reason():
-    path = model_file("gpu_pl_5b.txt", str(gold()["sample_5b_model"]))
-    with pytest.raises(cubens.CubensError, match="free warping"):
-        cubens.GpuModel(path)
 
 
 def _cantilever_state():
