@@ -67,6 +67,9 @@ struct Dev {
     double *emod = nullptr, *carea = nullptr, *llen = nullptr, *dens = nullptr;
     double *gmod = nullptr, *istrong = nullptr, *iweak = nullptr, *ipolar = nullptr, *iwarp = nullptr;
     double *auxpt = nullptr;
+    // member-end offsets [NE_FR * 6] in global axes (zero for a member without
+    // them); null when no member has offsets
+    double *ofs = nullptr;
     double *nu = nullptr, *thick = nullptr, *xlocal = nullptr, *farea = nullptr, *slength = nullptr;
     // ANAFLAG 3: yield stresses [NE_TR + NE_FR + NE_SH], plastic moduli
     // [NE_FR], yldflag (2 per member: 0 elastic, 1 yielded, 2 unloading), and

@@ -861,11 +861,13 @@ bool Analysis::save_checkpoint(long k, double ddt, double cnum, double cden) {
     CB_CHECK(hipStreamSynchronize(d_.s));
     c.llength = c.llen_w;      // the frame lengths at the commit (main.c:3880)
     std::copy(smv.begin(), smv.end(), c.sm.begin());
-    // member end coordinates: the joints' (the device takes no end offsets)
+    // member end coordinates: joint + offset (misc.c:122-131)
     for (long e = 0; e < m_.nfr; ++e)
         for (int end = 0; end < 2; ++end) {
             const long j = m_.minc[size_t(m_.ntr * 2 + e * 2 + end)] - 1;
-            for (int a = 0; a < 3; ++a) c.xfr[size_t(e * 6 + end * 3 + a)] = c.x[size_t(j * 3 + a)];
+            for (int a = 0; a < 3; ++a)
+                c.xfr[size_t(e * 6 + end * 3 + a)] =
+                    c.x[size_t(j * 3 + a)] + (m_.osflag[size_t(e)] ? m_.offset[size_t(e * 6 + end * 3 + a)] : 0.0);
         }
     // skyline ss from the joint-block tangent: column j holds rows j - kht[j]..j,
     // the diagonal at maxa[j] - 1 and row i at maxa[j] - 1 + (j - i)

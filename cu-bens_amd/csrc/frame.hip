@@ -264,6 +264,15 @@ __device__ inline void load_triad(const double* c1, const double* c2, const doub
 // local index of element of triple t (0: u1, 1: theta1, 2: u2, 3: theta2)
 __device__ __forceinline__ constexpr int tidx(int t, int m) { return lidx(t / 2, (t % 2) * 3 + m); }
 
+// Member-end offsets (rigid links, frame.c:297-324): the member end sits at
+// joint + o (o fixed, in global axes), so u_end = u + S theta with
+// S = -[o]x, and the joint takes the end forces back as (F_u, F_t + S^T F_u).
+__device__ inline void link_S(const double* o, double S[3][3]) {
+    S[0][0] = 0.0;   S[0][1] = o[2];  S[0][2] = -o[1];
+    S[1][0] = -o[2]; S[1][1] = 0.0;   S[1][2] = o[0];
+    S[2][0] = o[1];  S[2][1] = -o[0]; S[2][2] = 0.0;
+}
+
 // Yield data of the members under ANAFLAG 3 (null otherwise).
 struct PlasticIn {
     const int32_t* yld;      // yldflag, 2 per member
@@ -275,8 +284,10 @@ struct PlasticIn {
 // stiff_fr (frame.c:226-349) for the members of one colour: T^T k T with
 // T = blockdiag(R, R, 1, R, R, 1), one 3x3 sub-block (pair of triples) at a
 // time, added into the four joint blocks of the member.  PL: less the plastic
-// part of k (ANAFLAG 3).
-template <bool PL, int BS>
+// part of k (ANAFLAG 3).  OFS: members with end offsets, K_joint =
+// G^T K_end G with G = [I S; 0 I] per end (frame.c:297-324), the four 3x3
+// blocks of each end pair combined before they are added.
+template <bool PL, int BS, bool OFS>
 __global__ void __launch_bounds__(TPB) k_frame_tangent(
     const int32_t* __restrict__ els, int n, const int32_t* __restrict__ blk,
     const double* __restrict__ c1, const double* __restrict__ c2, const double* __restrict__ c3,
@@ -284,7 +295,7 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent(
     const double* __restrict__ llen, const double* __restrict__ istrong, const double* __restrict__ iweak,
     const double* __restrict__ ipolar, const double* __restrict__ iwarp,
     const double* __restrict__ defllen, const double* __restrict__ ef, double* __restrict__ K,
-    long ntr, int geometric, PlasticIn pin) {
+    long ntr, int geometric, PlasticIn pin, const double* __restrict__ ofs) {
     int it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= n) return;
     const long e = els[it], p = ntr + e;
@@ -301,6 +312,110 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent(
     int32_t bk[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) bk[q] = blk[e * 4 + q];
+    if constexpr (OFS) {
+        double S[2][3][3];
+        link_S(ofs + e * 6, S[0]);
+        link_S(ofs + e * 6 + 3, S[1]);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                // the end blocks E[h1][h2] = R^T k R of (triple h1 of end a, triple h2 of end b)
+                double E[2][2][3][3];
+#pragma unroll
+                for (int h1 = 0; h1 < 2; ++h1)
+#pragma unroll
+                    for (int h2 = 0; h2 < 2; ++h2) {
+                        const int t1 = 2 * a + h1, t2 = 2 * b + h2;
+                        double tmp[3][3];
+#pragma unroll
+                        for (int r = 0; r < 3; ++r)
+#pragma unroll
+                            for (int l = 0; l < 3; ++l)
+                                tmp[r][l] = R[0][r] * kent<PL>(k, pl, tidx(t1, 0), tidx(t2, l)) +
+                                            R[1][r] * kent<PL>(k, pl, tidx(t1, 1), tidx(t2, l)) +
+                                            R[2][r] * kent<PL>(k, pl, tidx(t1, 2), tidx(t2, l));
+#pragma unroll
+                        for (int r = 0; r < 3; ++r)
+#pragma unroll
+                            for (int s = 0; s < 3; ++s)
+                                E[h1][h2][r][s] = tmp[r][0] * R[0][s] + tmp[r][1] * R[1][s] + tmp[r][2] * R[2][s];
+                    }
+                // joint blocks: J_uu = E00, J_ut = E00 S_b + E01, J_tu = S_a^T E00 + E10,
+                // J_tt = S_a^T (E00 S_b + E01) + E10 S_b + E11
+                double J[2][2][3][3];
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int s = 0; s < 3; ++s) {
+                        double x = E[0][1][r][s], y = E[1][1][r][s];
+#pragma unroll
+                        for (int m = 0; m < 3; ++m) {
+                            x += E[0][0][r][m] * S[b][m][s];
+                            y += E[1][0][r][m] * S[b][m][s];
+                        }
+                        J[0][0][r][s] = E[0][0][r][s];
+                        J[0][1][r][s] = x;
+                        J[1][1][r][s] = y;   // E10 S_b + E11, S_a^T J01 added below
+                    }
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int s = 0; s < 3; ++s) {
+                        double u = E[1][0][r][s], v = J[1][1][r][s];
+#pragma unroll
+                        for (int m = 0; m < 3; ++m) {
+                            u += S[a][m][r] * E[0][0][m][s];
+                            v += S[a][m][r] * J[0][1][m][s];
+                        }
+                        J[1][0][r][s] = u;
+                        J[1][1][r][s] = v;
+                    }
+                double* Kb = K + (long)bk[a * 2 + b] * (BS * BS);
+#pragma unroll
+                for (int h1 = 0; h1 < 2; ++h1)
+#pragma unroll
+                    for (int h2 = 0; h2 < 2; ++h2)
+#pragma unroll
+                        for (int r = 0; r < 3; ++r)
+#pragma unroll
+                            for (int s = 0; s < 3; ++s) Kb[(h1 * 3 + r) * BS + h2 * 3 + s] += J[h1][h2][r][s];
+                if constexpr (BS == 7) {
+                    // warping column of end b and row of end a through G
+                    double cu[3], ct[3], ru[3], rt[3];
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                        cu[r] = R[0][r] * kent<PL>(k, pl, tidx(2 * a, 0), 7 * b + 6) +
+                                R[1][r] * kent<PL>(k, pl, tidx(2 * a, 1), 7 * b + 6) +
+                                R[2][r] * kent<PL>(k, pl, tidx(2 * a, 2), 7 * b + 6);
+                        ct[r] = R[0][r] * kent<PL>(k, pl, tidx(2 * a + 1, 0), 7 * b + 6) +
+                                R[1][r] * kent<PL>(k, pl, tidx(2 * a + 1, 1), 7 * b + 6) +
+                                R[2][r] * kent<PL>(k, pl, tidx(2 * a + 1, 2), 7 * b + 6);
+                        ru[r] = kent<PL>(k, pl, 7 * a + 6, tidx(2 * b, 0)) * R[0][r] +
+                                kent<PL>(k, pl, 7 * a + 6, tidx(2 * b, 1)) * R[1][r] +
+                                kent<PL>(k, pl, 7 * a + 6, tidx(2 * b, 2)) * R[2][r];
+                        rt[r] = kent<PL>(k, pl, 7 * a + 6, tidx(2 * b + 1, 0)) * R[0][r] +
+                                kent<PL>(k, pl, 7 * a + 6, tidx(2 * b + 1, 1)) * R[1][r] +
+                                kent<PL>(k, pl, 7 * a + 6, tidx(2 * b + 1, 2)) * R[2][r];
+                    }
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                        double x = ct[r], y = rt[r];
+#pragma unroll
+                        for (int m = 0; m < 3; ++m) {
+                            x += S[a][m][r] * cu[m];
+                            y += ru[m] * S[b][m][r];
+                        }
+                        Kb[r * 7 + 6] += cu[r];
+                        Kb[(3 + r) * 7 + 6] += x;
+                        Kb[6 * 7 + r] += ru[r];
+                        Kb[6 * 7 + 3 + r] += y;
+                    }
+                    Kb[48] += kent<PL>(k, pl, 7 * a + 6, 7 * b + 6);
+                }
+            }
+        return;
+    } else {
 #pragma unroll
     for (int t1 = 0; t1 < 4; ++t1)
 #pragma unroll
@@ -342,6 +457,7 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent(
                 }
                 Kb[48] += kent<PL>(k, pl, 7 * a + 6, 7 * b + 6);
             }
+    }
     }
 }
 
@@ -498,8 +614,10 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent_gather(
 // one output triple at a time.
 // One member of forces_fr: ef_i, and the member's end forces in global axes
 // (T_i^T ef_i) handed to out(local DOF, value); with BS 7 the warping
-// increments enter and the bimoments are handed out too.
-template <int BS, class Out>
+// increments enter and the bimoments are handed out too; with end offsets
+// (OFS) the joint increments reach the ends through u + S theta and the
+// joints take (F_u, F_t + S^T F_u).
+template <int BS, bool OFS, class Out>
 __device__ __forceinline__ void frame_member_forces(
     long e, const int32_t* __restrict__ mc, const double* __restrict__ dd, const double* __restrict__ c1p,
     const double* __restrict__ c2p, const double* __restrict__ c3p, const double* __restrict__ c1c,
@@ -507,7 +625,7 @@ __device__ __forceinline__ void frame_member_forces(
     const double* __restrict__ gmod, const double* __restrict__ carea, const double* __restrict__ llen,
     const double* __restrict__ istrong, const double* __restrict__ iweak, const double* __restrict__ ipolar,
     const double* __restrict__ iwarp, const double* __restrict__ defllen_ip, const double* __restrict__ ef_ip,
-    double* __restrict__ ef_i, long ntr, int geometric, Out&& out) {
+    double* __restrict__ ef_i, long ntr, int geometric, const double* __restrict__ ofs, Out&& out) {
     const long p = ntr + e;
     const double* fe = ef_ip + ntr * 2 + e * 14;
     FrameCoef k;
@@ -516,20 +634,33 @@ __device__ __forceinline__ void frame_member_forces(
     double Rp[3][3], Ri[3][3];
     load_triad(c1p, c2p, c3p, ntr + e * 3, Rp);
     load_triad(c1c, c2c, c3c, ntr + e * 3, Ri);
+    // the joint increments of the four triples
+    double Dj[4][3];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            int32_t q = mc[tidx(t, m)];
+            Dj[t][m] = q ? dd[q - 1] : 0.0;
+        }
+    double S[2][3][3];
+    if constexpr (OFS) {
+        link_S(ofs + e * 6, S[0]);
+        link_S(ofs + e * 6 + 3, S[1]);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+                Dj[2 * a][m] += S[a][m][0] * Dj[2 * a + 1][0] + S[a][m][1] * Dj[2 * a + 1][1] +
+                                S[a][m][2] * Dj[2 * a + 1][2];
+    }
     // local increments of the four triples, and of the warping DOFs (T is 1
     // there; zero with BS 6, where they are fixed)
     double dl[4][3];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        double D[3];
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int m = 0; m < 3; ++m) {
-            int32_t q = mc[tidx(t, m)];
-            D[m] = q ? dd[q - 1] : 0.0;
-        }
-#pragma unroll
-        for (int m = 0; m < 3; ++m) dl[t][m] = Rp[m][0] * D[0] + Rp[m][1] * D[1] + Rp[m][2] * D[2];
-    }
+        for (int m = 0; m < 3; ++m) dl[t][m] = Rp[m][0] * Dj[t][0] + Rp[m][1] * Dj[t][1] + Rp[m][2] * Dj[t][2];
     double dw[2] = {0.0, 0.0};
     if constexpr (BS == 7)
 #pragma unroll
@@ -543,6 +674,7 @@ __device__ __forceinline__ void frame_member_forces(
 #pragma unroll
         for (int c = 0; c < 3; ++c) Q[r][c] = Ri[r][0] * Rp[c][0] + Ri[r][1] * Rp[c][1] + Ri[r][2] * Rp[c][2];
     double* ei = ef_i + ntr * 2 + e * 14;
+    double Fu[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int to = 0; to < 4; ++to) {
         double v[3];
@@ -562,9 +694,22 @@ __device__ __forceinline__ void frame_member_forces(
             g[r] = Q[r][0] * v[0] + Q[r][1] * v[1] + Q[r][2] * v[2];
             ei[tidx(to, r)] = g[r];
         }
-        // T_i^T ef_i on this triple
+        // T_i^T ef_i on this triple (G^T with offsets)
+        double F[3];
 #pragma unroll
-        for (int m = 0; m < 3; ++m) out(tidx(to, m), Ri[0][m] * g[0] + Ri[1][m] * g[1] + Ri[2][m] * g[2]);
+        for (int m = 0; m < 3; ++m) F[m] = Ri[0][m] * g[0] + Ri[1][m] * g[1] + Ri[2][m] * g[2];
+        if constexpr (OFS) {
+            if (to % 2 == 0) {
+#pragma unroll
+                for (int m = 0; m < 3; ++m) Fu[m] = F[m];
+            } else {
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+                    F[m] += S[to / 2][0][m] * Fu[0] + S[to / 2][1][m] * Fu[1] + S[to / 2][2][m] * Fu[2];
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 3; ++m) out(tidx(to, m), F[m]);
     }
     // warping rows (bimoments): T_i T_ip^T is 1 there
 #pragma unroll
@@ -583,7 +728,7 @@ __device__ __forceinline__ void frame_member_forces(
 
 // forces_fr, elastic branch (frame.c:902-1096, 1273-1309), for the members of
 // one colour, scattered by mcode into f.
-template <int BS>
+template <int BS, bool OFS>
 __global__ void __launch_bounds__(TPB) k_frame_forces(
     const int32_t* __restrict__ els, int n, const int32_t* __restrict__ mcode,
     const double* __restrict__ dd, const double* __restrict__ c1p, const double* __restrict__ c2p,
@@ -593,13 +738,13 @@ __global__ void __launch_bounds__(TPB) k_frame_forces(
     const double* __restrict__ istrong, const double* __restrict__ iweak,
     const double* __restrict__ ipolar, const double* __restrict__ iwarp,
     const double* __restrict__ defllen_ip, const double* __restrict__ ef_ip,
-    double* __restrict__ ef_i, double* __restrict__ f, long ntr, int geometric) {
+    double* __restrict__ ef_i, double* __restrict__ f, long ntr, int geometric, const double* __restrict__ ofs) {
     int it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= n) return;
     const long e = els[it];
     const int32_t* mc = mcode + ntr * 6 + e * 14;
-    frame_member_forces<BS>(e, mc, dd, c1p, c2p, c3p, c1c, c2c, c3c, emod, gmod, carea, llen, istrong, iweak,
-                            ipolar, iwarp, defllen_ip, ef_ip, ef_i, ntr, geometric, [&](int i, double val) {
+    frame_member_forces<BS, OFS>(e, mc, dd, c1p, c2p, c3p, c1c, c2c, c3c, emod, gmod, carea, llen, istrong, iweak,
+                            ipolar, iwarp, defllen_ip, ef_ip, ef_i, ntr, geometric, ofs, [&](int i, double val) {
                                 const int32_t q = mc[i];
                                 if (q) f[q - 1] += val;
                             });
@@ -607,7 +752,7 @@ __global__ void __launch_bounds__(TPB) k_frame_forces(
 
 // Frame-only models: every member in one launch, its global end forces into
 // h[e * 2 BS + BS * end + slot] ...
-template <int BS>
+template <int BS, bool OFS>
 __global__ void __launch_bounds__(TPB) k_frame_forces_members(
     long n, const int32_t* __restrict__ mcode, const double* __restrict__ dd, const double* __restrict__ c1p,
     const double* __restrict__ c2p, const double* __restrict__ c3p, const double* __restrict__ c1c,
@@ -615,12 +760,12 @@ __global__ void __launch_bounds__(TPB) k_frame_forces_members(
     const double* __restrict__ gmod, const double* __restrict__ carea, const double* __restrict__ llen,
     const double* __restrict__ istrong, const double* __restrict__ iweak, const double* __restrict__ ipolar,
     const double* __restrict__ iwarp, const double* __restrict__ defllen_ip, const double* __restrict__ ef_ip,
-    double* __restrict__ ef_i, double* __restrict__ h, int geometric) {
+    double* __restrict__ ef_i, double* __restrict__ h, int geometric, const double* __restrict__ ofs) {
     const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (e >= n) return;
     double* he = h + e * (2 * BS);
-    frame_member_forces<BS>(e, mcode + e * 14, dd, c1p, c2p, c3p, c1c, c2c, c3c, emod, gmod, carea, llen,
-                            istrong, iweak, ipolar, iwarp, defllen_ip, ef_ip, ef_i, 0, geometric,
+    frame_member_forces<BS, OFS>(e, mcode + e * 14, dd, c1p, c2p, c3p, c1c, c2c, c3c, emod, gmod, carea, llen,
+                            istrong, iweak, ipolar, iwarp, defllen_ip, ef_ip, ef_i, 0, geometric, ofs,
                             [&](int i, double val) { he[BS * (i / 7) + i % 7] = val; });
 }
 
@@ -681,9 +826,9 @@ __device__ inline double regula_falsi(double p, double dp, double my, double dmy
 // flags the unloading ends 2 with code 2.  code[4e] = code, code[4e+1..2] =
 // the member's flags after its pass; st->first <- the lowest member with a
 // code.  With BS 7 the warping increments and bimoments enter as in the
-// elastic pass; no member end offsets; the fixed-end forces are zero (no
-// distributed loads).
-template <int BS>
+// elastic pass, and end offsets (ofs non-null) as in frame_member_forces; the
+// fixed-end forces are zero (no distributed loads).
+template <int BS, bool OFS>
 __global__ void __launch_bounds__(64) k_frame_forces_plastic(
     long n, const int32_t* __restrict__ mcode, const double* __restrict__ dd, const double* __restrict__ c1p,
     const double* __restrict__ c2p, const double* __restrict__ c3p, const double* __restrict__ c1c,
@@ -692,7 +837,7 @@ __global__ void __launch_bounds__(64) k_frame_forces_plastic(
     const double* __restrict__ istrong, const double* __restrict__ iweak, const double* __restrict__ ipolar,
     const double* __restrict__ iwarp, const double* __restrict__ defllen_ip, const double* __restrict__ ef_ip,
     PlasticIn pin, double* __restrict__ efc, double* __restrict__ h, int32_t* __restrict__ code,
-    double* __restrict__ tau, PlasticStatus* __restrict__ st, long ntr) {
+    double* __restrict__ tau, PlasticStatus* __restrict__ st, long ntr, const double* __restrict__ ofs) {
     const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (e >= n) return;
     const long p = ntr + e;
@@ -708,18 +853,30 @@ __global__ void __launch_bounds__(64) k_frame_forces_plastic(
     double Rp[3][3], Ri[3][3];
     load_triad(c1p, c2p, c3p, ntr + e * 3, Rp);
     load_triad(c1c, c2c, c3c, ntr + e * 3, Ri);
-    double dl[4][3];
+    double Dj[4][3];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        double D[3];
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
             const int32_t q = mc[tidx(t, m)];
-            D[m] = q ? dd[q - 1] : 0.0;
+            Dj[t][m] = q ? dd[q - 1] : 0.0;
         }
+    double S[2][3][3];
+    if constexpr (OFS) {
+        link_S(ofs + e * 6, S[0]);
+        link_S(ofs + e * 6 + 3, S[1]);
 #pragma unroll
-        for (int m = 0; m < 3; ++m) dl[t][m] = Rp[m][0] * D[0] + Rp[m][1] * D[1] + Rp[m][2] * D[2];
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+                Dj[2 * a][m] += S[a][m][0] * Dj[2 * a + 1][0] + S[a][m][1] * Dj[2 * a + 1][1] +
+                                S[a][m][2] * Dj[2 * a + 1][2];
     }
+    double dl[4][3];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) dl[t][m] = Rp[m][0] * Dj[t][0] + Rp[m][1] * Dj[t][1] + Rp[m][2] * Dj[t][2];
     double dw[2] = {0.0, 0.0};
     if constexpr (BS == 7)
 #pragma unroll
@@ -755,6 +912,16 @@ __global__ void __launch_bounds__(64) k_frame_forces_plastic(
             const int i = tidx(to, m);
             he[BS * (i / 7) + i % 7] =
                 Ri[0][m] * fi[tidx(to, 0)] + Ri[1][m] * fi[tidx(to, 1)] + Ri[2][m] * fi[tidx(to, 2)];
+        }
+        if (OFS && to % 2 == 1) {
+            // joint moments: F_t + S^T F_u of this end
+            const int a = to / 2;
+            double Fu[3];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) Fu[m] = he[BS * a + m];
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+                he[BS * a + 3 + m] += S[a][0][m] * Fu[0] + S[a][1][m] * Fu[1] + S[a][2][m] * Fu[2];
         }
     }
 #pragma unroll
@@ -870,15 +1037,22 @@ __global__ void k_unloaded_elastic(int32_t* __restrict__ yld, long n2) {
 }
 
 // misc.c:122-144: member length and triad from the current chord and the
-// fixed input auxiliary point.
+// fixed input auxiliary point; the member ends sit at joint + offset (fixed
+// in global axes; ofs null when no member has offsets).
 __global__ void k_update_frame(const int32_t* __restrict__ minc, const double* __restrict__ x,
                                const double* __restrict__ auxpt, double* __restrict__ defllen,
                                double* __restrict__ c1, double* __restrict__ c2,
-                               double* __restrict__ c3, long n, long ntr) {
+                               double* __restrict__ c3, long n, long ntr, const double* __restrict__ ofs) {
     long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (e >= n) return;
-    const double* a = x + (minc[ntr * 2 + e * 2] - 1) * 3;
-    const double* b = x + (minc[ntr * 2 + e * 2 + 1] - 1) * 3;
+    const double* xa = x + (minc[ntr * 2 + e * 2] - 1) * 3;
+    const double* xb = x + (minc[ntr * 2 + e * 2 + 1] - 1) * 3;
+    double a[3] = {xa[0], xa[1], xa[2]}, b[3] = {xb[0], xb[1], xb[2]};
+    if (ofs)
+        for (int k = 0; k < 3; ++k) {
+            a[k] += ofs[e * 6 + k];
+            b[k] += ofs[e * 6 + 3 + k];
+        }
     double el[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
     double L = sqrt(el[0] * el[0] + el[1] * el[1] + el[2] * el[2]);
     defllen[ntr + e] = L;
@@ -920,22 +1094,23 @@ void launch_frame_tangent(Dev& d, int g) {
         for (size_t c = 0; c + 1 < d.col_ptr_fr.size(); ++c) {
             int n = d.col_ptr_fr[c + 1] - d.col_ptr_fr[c];
             if (!n) continue;
-            if (d.anaflag == 3)
-                hipLaunchKernelGGL((k_frame_tangent<true, BS>), dim3(nblk(n)), dim3(TPB), 0, d.s,
-                                   d.col_el_fr + d.col_ptr_fr[c], n, d.blk_fr, d.c1[g], d.c2[g], d.c3[g],
-                                   d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp,
-                                   d.defllen[g], d.ef[g], d.K, d.ntr, geo, plastic_in(d));
-            else
-                hipLaunchKernelGGL((k_frame_tangent<false, BS>), dim3(nblk(n)), dim3(TPB), 0, d.s,
-                                   d.col_el_fr + d.col_ptr_fr[c], n, d.blk_fr, d.c1[g], d.c2[g], d.c3[g],
-                                   d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp,
-                                   d.defllen[g], d.ef[g], d.K, d.ntr, geo, PlasticIn{});
+            auto go = [&](auto kern, PlasticIn pin) {
+                hipLaunchKernelGGL(kern, dim3(nblk(n)), dim3(TPB), 0, d.s, d.col_el_fr + d.col_ptr_fr[c], n,
+                                   d.blk_fr, d.c1[g], d.c2[g], d.c3[g], d.emod, d.gmod, d.carea, d.llen,
+                                   d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[g], d.ef[g], d.K, d.ntr, geo,
+                                   pin, d.ofs);
+            };
+            if (d.anaflag == 3 && d.ofs) go(k_frame_tangent<true, BS, true>, plastic_in(d));
+            else if (d.anaflag == 3) go(k_frame_tangent<true, BS, false>, plastic_in(d));
+            else if (d.ofs) go(k_frame_tangent<false, BS, true>, PlasticIn{});
+            else go(k_frame_tangent<false, BS, false>, PlasticIn{});
         }
     });
     CB_CHECK(hipGetLastError());
 }
 
 void launch_frame_tangent_gather(Dev& d, int g) {
+    if (d.ofs) throw DeviceError("gathered frame tangent: members with end offsets take the colour launches");
     const int geo = d.anaflag == 2 || d.anaflag == 3;
     with_bs(d, [&](auto B) {
         constexpr int BS = decltype(B)::value;
@@ -957,6 +1132,7 @@ void launch_frame_tangent_gather(Dev& d, int g) {
 void launch_frame_tangent_band(Dev& d, int g, double* tiles, int w, int* bad) {
     if (d.bs != 6 || d.ntr || d.nsh) throw DeviceError("frame tangent into the band: frame-only models, warping fixed");
     if (d.anaflag == 3) throw DeviceError("frame tangent into the band: elastic (ANAFLAG 2) members only");
+    if (d.ofs) throw DeviceError("frame tangent into the band: members without end offsets only");
     hipLaunchKernelGGL((k_frame_tangent_gather<true, false, 6>), dim3(nblk(2 * d.nnzb)), dim3(TPB), 0, d.s,
                        d.gat_ptr_fr, d.gat_item_fr, d.blkrow, d.colidx, d.eq_of_slot, d.nnzb, d.c1[g], d.c2[g],
                        d.c3[g], d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp,
@@ -971,10 +1147,14 @@ static void launch_frame_forces_plastic(Dev& d, double* f, const double* dd) {
     with_bs(d, [&](auto B) {
         constexpr int BS = decltype(B)::value;
         hipLaunchKernelGGL(k_plastic_init, dim3(1), dim3(1), 0, d.s, d.pl_st, d.nfr);
-        hipLaunchKernelGGL((k_frame_forces_plastic<BS>), dim3(nblk(d.nfr, 64)), dim3(64), 0, d.s, d.nfr, d.mcode,
-                           dd, d.c1[G_IP], d.c2[G_IP], d.c3[G_IP], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.emod, d.gmod,
-                           d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[G_IP], d.ef[G_IP],
-                           plastic_in(d), d.pl_efc, d.fh, d.pl_code, d.pl_tau, d.pl_st, d.ntr);
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(nblk(d.nfr, 64)), dim3(64), 0, d.s, d.nfr, d.mcode, dd, d.c1[G_IP],
+                               d.c2[G_IP], d.c3[G_IP], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.emod, d.gmod, d.carea,
+                               d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[G_IP], d.ef[G_IP],
+                               plastic_in(d), d.pl_efc, d.fh, d.pl_code, d.pl_tau, d.pl_st, d.ntr, d.ofs);
+        };
+        if (d.ofs) go(k_frame_forces_plastic<BS, true>);
+        else go(k_frame_forces_plastic<BS, false>);
         hipLaunchKernelGGL(k_frame_plastic_commit, dim3(nblk(d.nfr, 256)), dim3(256), 0, d.s, d.nfr, d.pl_efc,
                            d.pl_code, d.pl_tau, d.pl_st, d.yld, d.ef[G_I], d.ntr);
         hipLaunchKernelGGL((k_frame_forces_gather<BS>), dim3(nblk(d.nslot, 256)), dim3(256), 0, d.s,
@@ -1008,10 +1188,14 @@ void launch_frame_forces(Dev& d, double* f, const double* dd) {
     with_bs(d, [&](auto B) {
         constexpr int BS = decltype(B)::value;
         if (!d.ntr && !d.nsh) {
-            hipLaunchKernelGGL((k_frame_forces_members<BS>), dim3(nblk(d.nfr)), dim3(TPB), 0, d.s, d.nfr, d.mcode,
-                               dd, d.c1[G_IP], d.c2[G_IP], d.c3[G_IP], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.emod,
-                               d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[G_IP],
-                               d.ef[G_IP], d.ef[G_I], d.fh, int(d.anaflag == 2));
+            auto go = [&](auto kern) {
+                hipLaunchKernelGGL(kern, dim3(nblk(d.nfr)), dim3(TPB), 0, d.s, d.nfr, d.mcode, dd, d.c1[G_IP],
+                                   d.c2[G_IP], d.c3[G_IP], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.emod, d.gmod,
+                                   d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[G_IP],
+                                   d.ef[G_IP], d.ef[G_I], d.fh, int(d.anaflag == 2), d.ofs);
+            };
+            if (d.ofs) go(k_frame_forces_members<BS, true>);
+            else go(k_frame_forces_members<BS, false>);
             hipLaunchKernelGGL((k_frame_forces_gather<BS>), dim3(nblk(d.nslot, 256)), dim3(256), 0, d.s,
                                d.jnt_ptr_fr, d.jnt_item_fr, d.eq_of_slot, d.nslot, d.fh, f, nullptr);
             return;
@@ -1019,11 +1203,15 @@ void launch_frame_forces(Dev& d, double* f, const double* dd) {
         for (size_t c = 0; c + 1 < d.col_ptr_fr.size(); ++c) {
             int n = d.col_ptr_fr[c + 1] - d.col_ptr_fr[c];
             if (!n) continue;
-            hipLaunchKernelGGL((k_frame_forces<BS>), dim3(nblk(n)), dim3(TPB), 0, d.s,
-                               d.col_el_fr + d.col_ptr_fr[c], n, d.mcode, dd, d.c1[G_IP], d.c2[G_IP],
-                               d.c3[G_IP], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.emod, d.gmod, d.carea,
-                               d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[G_IP],
-                               d.ef[G_IP], d.ef[G_I], f, d.ntr, int(d.anaflag == 2));
+            auto go = [&](auto kern) {
+                hipLaunchKernelGGL(kern, dim3(nblk(n)), dim3(TPB), 0, d.s, d.col_el_fr + d.col_ptr_fr[c], n,
+                                   d.mcode, dd, d.c1[G_IP], d.c2[G_IP], d.c3[G_IP], d.c1[G_I], d.c2[G_I],
+                                   d.c3[G_I], d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar,
+                                   d.iwarp, d.defllen[G_IP], d.ef[G_IP], d.ef[G_I], f, d.ntr,
+                                   int(d.anaflag == 2), d.ofs);
+            };
+            if (d.ofs) go(k_frame_forces<BS, true>);
+            else go(k_frame_forces<BS, false>);
         }
     });
     CB_CHECK(hipGetLastError());
@@ -1032,7 +1220,7 @@ void launch_frame_forces(Dev& d, double* f, const double* dd) {
 void launch_frame_update(Dev& d, int gx) {
     if (!d.nfr) return;
     hipLaunchKernelGGL(k_update_frame, dim3(nblk(d.nfr, 256)), dim3(256), 0, d.s, d.minc, d.x[gx],
-                       d.auxpt, d.defllen[G_I], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.nfr, d.ntr);
+                       d.auxpt, d.defllen[G_I], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.nfr, d.ntr, d.ofs);
     CB_CHECK(hipGetLastError());
 }
 

@@ -363,7 +363,6 @@ bool load_model(const char* path, Model& m, std::string& err) {
             if (t.bad()) FAIL("truncated frame offsets");
             if (j == 0) break;
             if (j < 1 || j > nfr) FAIL("frame offset member out of range");
-            FAIL("frame member-end offsets are not carried by the device kernels");
             m.osflag[j - 1] = 1;
             std::copy(o, o + 6, &m.offset[(j - 1) * 6]);
         }

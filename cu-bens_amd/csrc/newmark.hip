@@ -50,20 +50,33 @@ __global__ void k_mass_truss(const int32_t* __restrict__ els, int n, const int32
     }
 }
 
+// The chord length of frame member e between its end points, joint + offset
+// (frame.c:1331-1345; ofs null when no member has offsets).
+__device__ inline double frame_chord(const int32_t* __restrict__ minc, const double* __restrict__ x,
+                                     const double* __restrict__ ofs, long e) {
+    const double* xa = x + (minc[e * 2] - 1) * 3;
+    const double* xb = x + (minc[e * 2 + 1] - 1) * 3;
+    double a[3] = {xa[0], xa[1], xa[2]}, b[3] = {xb[0], xb[1], xb[2]};
+    if (ofs)
+        for (int k = 0; k < 3; ++k) {
+            a[k] += ofs[e * 6 + k];
+            b[k] += ofs[e * 6 + 3 + k];
+        }
+    const double el0 = b[0] - a[0], el1 = b[1] - a[1], el2 = b[2] - a[2];
+    return sqrt(el0 * el0 + el1 * el1 + el2 * el2);
+}
+
 // frame: rho A L/2 per translation, rho A L^3/24 per rotation, 0 on warping
 // (frame.c:1357-1375); single-type models only (the reference indexes the
 // per-type arrays from 0 here)
 __global__ void k_mass_frame(const int32_t* __restrict__ els, int n, const int32_t* __restrict__ minc,
                              const int32_t* __restrict__ mcode, const double* __restrict__ x,
                              const double* __restrict__ dens, const double* __restrict__ carea,
-                             double* __restrict__ llen, double* __restrict__ sm) {
+                             double* __restrict__ llen, double* __restrict__ sm, const double* __restrict__ ofs) {
     int it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= n) return;
     const long e = els[it];
-    const double* a = x + (minc[e * 2] - 1) * 3;
-    const double* b = x + (minc[e * 2 + 1] - 1) * 3;
-    double el0 = b[0] - a[0], el1 = b[1] - a[1], el2 = b[2] - a[2];
-    const double L = sqrt(el0 * el0 + el1 * el1 + el2 * el2);
+    const double L = frame_chord(minc, x, ofs, e);
     llen[e] = L;
     const double mt = dens[e] * carea[e] * L / 24 * 12;
     const double mr = dens[e] * carea[e] * L / 24 * (L * L);
@@ -78,13 +91,11 @@ __global__ void k_mass_frame(const int32_t* __restrict__ els, int n, const int32
 // launch (its length, and its translational and rotational masses into fm)...
 __global__ void k_mass_frame_members(long n, const int32_t* __restrict__ minc, const double* __restrict__ x,
                                      const double* __restrict__ dens, const double* __restrict__ carea,
-                                     double* __restrict__ llen, double* __restrict__ fm) {
+                                     double* __restrict__ llen, double* __restrict__ fm,
+                                     const double* __restrict__ ofs) {
     const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (e >= n) return;
-    const double* a = x + (minc[e * 2] - 1) * 3;
-    const double* b = x + (minc[e * 2 + 1] - 1) * 3;
-    double el0 = b[0] - a[0], el1 = b[1] - a[1], el2 = b[2] - a[2];
-    const double L = sqrt(el0 * el0 + el1 * el1 + el2 * el2);
+    const double L = frame_chord(minc, x, ofs, e);
     llen[e] = L;
     fm[e * 2] = dens[e] * carea[e] * L / 24 * 12;
     fm[e * 2 + 1] = dens[e] * carea[e] * L / 24 * (L * L);
@@ -191,7 +202,7 @@ void launch_mass(Dev& d, double* sm, int type) {
         }
     } else if (type == 1 && !d.ntr && !d.nsh && d.bs == 6) {
         hipLaunchKernelGGL(k_mass_frame_members, dim3(nblk(d.nfr)), dim3(TPB), 0, d.s, d.nfr, d.minc, xb, d.dens,
-                           d.carea, d.llen, d.fh);
+                           d.carea, d.llen, d.fh, d.ofs);
         hipLaunchKernelGGL(k_mass_frame_gather, dim3(nblk(d.nslot)), dim3(TPB), 0, d.s, d.jnt_ptr_fr, d.jnt_item_fr,
                            d.eq_of_slot, d.nslot, d.fh, sm);
     } else if (type == 1) {
@@ -200,7 +211,7 @@ void launch_mass(Dev& d, double* sm, int type) {
             if (!n) continue;
             hipLaunchKernelGGL(k_mass_frame, dim3(nblk(n)), dim3(TPB), 0, d.s,
                                d.col_el_fr + d.col_ptr_fr[c], n, d.minc, d.mcode, xb, d.dens,
-                               d.carea, d.llen, sm);
+                               d.carea, d.llen, sm, d.ofs);
         }
     } else {
         for (size_t c = 0; c + 1 < d.col_ptr_sh.size(); ++c) {

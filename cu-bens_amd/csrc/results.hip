@@ -38,11 +38,13 @@ __global__ void k_react_truss(const int32_t* __restrict__ els, int n, const int3
 
 // Frame end forces in the member triad, 14 per member: per end the force
 // triple, the moment triple and the warping bimoment; T is block diagonal in
-// the triad with 1 on the warping entries (frame.c:286-295).
+// the triad with 1 on the warping entries (frame.c:286-295).  A member end
+// offset o carries the end force F to the joint with the moment o x F added
+// (G^T, frame.c:297-324); ofs is null when no member has offsets.
 __global__ void k_react_frame(const int32_t* __restrict__ els, int n, const int32_t* __restrict__ minc,
                               const double* __restrict__ c1, const double* __restrict__ c2,
                               const double* __restrict__ c3, const double* __restrict__ ef,
-                              double* __restrict__ rx, long ntr) {
+                              double* __restrict__ rx, long ntr, const double* __restrict__ ofs) {
     const int it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= n) return;
     const long e = els[it];
@@ -53,11 +55,20 @@ __global__ void k_react_frame(const int32_t* __restrict__ els, int n, const int3
     const double* fe = ef + ntr * 2 + e * 14;
     for (int a = 0; a < 2; ++a) {
         const long j = minc[ntr * 2 + e * 2 + a] - 1;
+        double F[2][3];
         for (int h = 0; h < 2; ++h)
             for (int m = 0; m < 3; ++m) {
                 const int b = 7 * a + 3 * h;
-                rx[j * 7 + 3 * h + m] += R[0][m] * fe[b] + R[1][m] * fe[b + 1] + R[2][m] * fe[b + 2];
+                F[h][m] = R[0][m] * fe[b] + R[1][m] * fe[b + 1] + R[2][m] * fe[b + 2];
             }
+        if (ofs) {
+            const double* o = ofs + e * 6 + 3 * a;
+            F[1][0] += o[1] * F[0][2] - o[2] * F[0][1];
+            F[1][1] += o[2] * F[0][0] - o[0] * F[0][2];
+            F[1][2] += o[0] * F[0][1] - o[1] * F[0][0];
+        }
+        for (int h = 0; h < 2; ++h)
+            for (int m = 0; m < 3; ++m) rx[j * 7 + 3 * h + m] += F[h][m];
         rx[j * 7 + 6] += fe[7 * a + 6];
     }
 }
@@ -131,7 +142,7 @@ void launch_reactions(Dev& d, int g, double* rx) {
         const int n = d.col_ptr_fr[c + 1] - d.col_ptr_fr[c];
         if (n)
             hipLaunchKernelGGL(k_react_frame, dim3(nblk(n)), dim3(TPB), 0, d.s, d.col_el_fr + d.col_ptr_fr[c], n,
-                               d.minc, d.c1[g], d.c2[g], d.c3[g], d.ef[g], rx, d.ntr);
+                               d.minc, d.c1[g], d.c2[g], d.c3[g], d.ef[g], rx, d.ntr, d.ofs);
     }
     const long io = d.ntr * 2 + d.nfr * 2, to = d.ntr + d.nfr * 3, eo = d.ntr * 2 + d.nfr * 14;
     for (size_t c = 0; c + 1 < d.col_ptr_sh.size(); ++c) {

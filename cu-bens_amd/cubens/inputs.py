@@ -82,6 +82,18 @@ class ModelText:
             fh.write(self.text)
         return path
 
+    def with_offsets(self, rows: dict) -> "ModelText":
+        """The same frame model with member-end offsets: ``rows`` maps a member
+        (1-based) to its six offsets (end 1 x, y, z, end 2 x, y, z, global
+        axes), written before the zero line that ends the offsets, the first
+        all-zero seven-field line of every frame model written here."""
+        assert self.n_fr > 0 and self.n_tr == 0
+        lines = "".join(f"{m}," + ",".join(_fmt(v) for v in o) + "\n" for m, o in sorted(rows.items()))
+        end = "0,0,0,0,0,0,0\n"
+        assert end in self.text
+        return ModelText(self.text.replace(end, lines + end, 1), self.nj, self.n_tr, self.n_fr, self.n_sh,
+                         dict(self.info, offsets=dict(rows)))
+
 
 # --------------------------------------------------------------------------
 # C1: shallow space-truss dome, Newton-Raphson load control

@@ -39,14 +39,24 @@ void orc_reactions(const orc_model *m, const double *ef, const double *c1, const
         }
     }
     /* frame: per end a force triple, a moment triple and the warping
-     * bimoment; T_i is block diagonal in the triad, 1 on warping (frame.c:286-295) */
+     * bimoment; T_i is block diagonal in the triad, 1 on warping
+     * (frame.c:286-295); an end offset o moves the end force F to the joint
+     * with the moment o x F added (G^T, frame.c:297-324) */
     for (long e = 0; e < nfr; ++e) {
         const long t = ntr + e * 3;
         const double *fe = ef + ntr * 2 + e * 14;
         for (int a = 0; a < 2; ++a) {
             long j = m->minc[ntr * 2 + e * 2 + a] - 1;
-            rotate_back(c1 + t, c2 + t, c3 + t, fe + 7 * a, r + j * 7);
+            double F[3] = {0, 0, 0};
+            rotate_back(c1 + t, c2 + t, c3 + t, fe + 7 * a, F);
+            for (int k = 0; k < 3; ++k) r[j * 7 + k] += F[k];
             rotate_back(c1 + t, c2 + t, c3 + t, fe + 7 * a + 3, r + j * 7 + 3);
+            if (m->osflag[e]) {
+                const double *o = m->offset + e * 6 + 3 * a;
+                r[j * 7 + 3] += o[1] * F[2] - o[2] * F[1];
+                r[j * 7 + 4] += o[2] * F[0] - o[0] * F[2];
+                r[j * 7 + 5] += o[0] * F[1] - o[1] * F[0];
+            }
             r[j * 7 + 6] += fe[7 * a + 6];
         }
     }

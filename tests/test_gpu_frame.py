@@ -135,12 +135,3 @@ def test_frame_static_nr_path_against_oracle():
     assert r["status"] == 0, r["msg"]
     assert np.array_equal(r["itecnt"], ro["itecnt"])
     assert relerr(r["d"], ro["d"]) < 1e-9
-
-
-def test_member_end_offsets_are_rejected_loudly():
-    """Offsets are restated by the oracle but not carried by the device
-    kernels: opening such a model fails with a message, never silently."""
-    txt = inputs.space_frame(40, kind="tower", algflag=1, amp=1e4).text
-    txt = txt.replace("0,0,0,0,0,0,0\n", "3,0.05,0.0,0.0,0.0,0.0,0.0\n0,0,0,0,0,0,0\n", 1)
-    with pytest.raises(cubens.CubensError, match="offset"):
-        cubens.GpuModel(model_file("offs.txt", txt))
