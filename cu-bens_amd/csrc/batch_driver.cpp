@@ -216,6 +216,8 @@ Batch::Batch(const std::vector<std::string>& paths) {
         if (m.ntr || m.nsh || !m.nfr) throw std::runtime_error(paths[b] + ": a batch takes frame-only models");
         for (int v : m.osflag)
             if (v) throw std::runtime_error(paths[b] + ": member-end offsets are not carried by the batched frame kernels");
+        for (int32_t v : m.relmask)
+            if (v) throw std::runtime_error(paths[b] + ": member end releases are not carried by the batched frame kernels");
         for (long j = 0; j < m.nj; ++j)
             if (m.jcode[j * 7 + 6] > 0)
                 throw std::runtime_error(paths[b] + ": free warping DOFs are not carried by the device layout");

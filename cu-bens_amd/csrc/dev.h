@@ -70,6 +70,9 @@ struct Dev {
     // member-end offsets [NE_FR * 6] in global axes (zero for a member without
     // them); null when no member has offsets
     double *ofs = nullptr;
+    // member end bending releases [NE_FR], the four flags as a mask (bit a:
+    // flag a); null when no member has releases
+    int32_t *relmask = nullptr;
     double *nu = nullptr, *thick = nullptr, *xlocal = nullptr, *farea = nullptr, *slength = nullptr;
     // ANAFLAG 3: yield stresses [NE_TR + NE_FR + NE_SH], plastic moduli
     // [NE_FR], yldflag (2 per member: 0 elastic, 1 yielded, 2 unloading), and

@@ -273,6 +273,146 @@ __device__ inline void link_S(const double* o, double S[3][3]) {
     S[2][0] = o[1];  S[2][1] = -o[0]; S[2][2] = 0.0;
 }
 
+// The members' offsets and end releases, each null when no member has any:
+// ofs [NE_FR * 6] (global axes, zero for a member without), rel [NE_FR] the
+// release flags as a mask (bit a: flag a of the input line).
+struct MemberXtr {
+    const double* ofs;
+    const int32_t* rel;
+};
+
+// S of both ends of member e, zero without offsets (G = I then, and every
+// term it adds is an exact zero).
+__device__ inline void member_links(const MemberXtr& x, long e, double S[2][3][3]) {
+    if (x.ofs) {
+        link_S(x.ofs + e * 6, S[0]);
+        link_S(x.ofs + e * 6 + 3, S[1]);
+    } else {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) S[a][r][c] = 0.0;
+    }
+}
+
+// Member end bending releases (release, frame.c:798-896): the released local
+// rotations are condensed out of the member's tangent, k <- k - V W V^T with
+// V = k G the columns of k at the released DOFs and W = (G^T k G)^-1.  The
+// four candidates in the input's flag order: strong axis end 1 (local 5),
+// weak axis end 1 (4), strong axis end 2 (12), weak axis end 2 (11).  V and W
+// are kept over all four, zero where a DOF is not released, so a member
+// without releases subtracts exact zeros.
+constexpr int REL_DOF[4] = {5, 4, 12, 11};
+
+struct FrameRel {
+    double v[4][14];
+    double w[4][4];
+};
+
+// inverse (misc.c:284-342) of a 4 x 4 matrix in place: Gauss-Jordan on
+// [H | I] without normalising the pivot rows, divided out at the end; the
+// pivot search scans row i from the diagonal for its first non-zero entry and
+// swaps rows i and that column's index.  Rows and columns of identity among
+// the others are exact no-ops, so the inverse of the released block embedded
+// in the identity equals the reference's inverse of the compact block.
+__device__ inline void gj_inverse4(double H[4][4]) {
+    double aug[4][8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) aug[i][j] = j < 4 ? H[i][j] : (j - 4 == i ? 1.0 : 0.0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        int kk = i;
+#pragma unroll
+        for (int j = 3; j >= i; --j)
+            if (aug[i][j] != 0) kk = j;
+#pragma unroll
+        for (int r = i + 1; r < 4; ++r)
+            if (r == kk)
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const double t = aug[i][q];
+                    aug[i][q] = aug[r][q];
+                    aug[r][q] = t;
+                }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j == i) continue;
+            const double m = aug[j][i] / aug[i][i];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) aug[j][q] -= m * aug[i][q];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) H[i][j] = aug[i][4 + j] / aug[i][i];
+}
+
+// V and W of the release mask on the local tangent kent<PL>; one release
+// inverts by 1/x, two by the adjugate over the determinant, three or four by
+// gj_inverse4, as release does.
+template <bool PL>
+__device__ inline void rel_setup(FrameRel& rl, const FrameCoef& k, const FramePl& pl, int mask) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int i = 0; i < 14; ++i) rl.v[a][i] = (mask >> a & 1) ? kent<PL>(k, pl, i, REL_DOF[a]) : 0.0;
+    // (G^T k G)[a][b] = k[R_b][R_a] = v[a][R_b] on the released DOFs
+    double H[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            H[a][b] = ((mask >> a & 1) && (mask >> b & 1)) ? rl.v[a][REL_DOF[b]] : (a == b ? 1.0 : 0.0);
+            rl.w[a][b] = 0.0;
+        }
+    const int n = __popc(mask);
+    if (n == 1) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+            if (mask == (1 << a)) rl.w[a][a] = 1 / H[a][a];
+    } else if (n == 2) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int q = p + 1; q < 4; ++q)
+                if (mask == ((1 << p) | (1 << q))) {
+                    const double det = H[p][p] * H[q][q] - H[p][q] * H[q][p];
+                    rl.w[p][p] = H[q][q] / det;
+                    rl.w[q][q] = H[p][p] / det;
+                    rl.w[p][q] = H[p][q] * (-1 / det);
+                    rl.w[q][p] = H[q][p] * (-1 / det);
+                }
+    } else if (n > 2) {
+        gj_inverse4(H);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                if ((mask >> a & 1) && (mask >> b & 1)) rl.w[a][b] = H[a][b];
+    }
+}
+
+// entry (i, j) of the local tangent, less its plastic part when PL, with the
+// member's releases condensed out when REL
+template <bool PL, bool REL>
+__device__ __forceinline__ double kx(const FrameCoef& k, const FramePl& pl, const FrameRel& rl, int i, int j) {
+    double v = kent<PL>(k, pl, i, j);
+    if constexpr (REL) {
+        double s = 0.0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+            s += rl.v[a][i] * (rl.w[a][0] * rl.v[0][j] + rl.w[a][1] * rl.v[1][j] + rl.w[a][2] * rl.v[2][j] +
+                               rl.w[a][3] * rl.v[3][j]);
+        v -= s;
+    }
+    return v;
+}
+
 // Yield data of the members under ANAFLAG 3 (null otherwise).
 struct PlasticIn {
     const int32_t* yld;      // yldflag, 2 per member
@@ -284,10 +424,11 @@ struct PlasticIn {
 // stiff_fr (frame.c:226-349) for the members of one colour: T^T k T with
 // T = blockdiag(R, R, 1, R, R, 1), one 3x3 sub-block (pair of triples) at a
 // time, added into the four joint blocks of the member.  PL: less the plastic
-// part of k (ANAFLAG 3).  OFS: members with end offsets, K_joint =
+// part of k (ANAFLAG 3).  XTR: members with end offsets, K_joint =
 // G^T K_end G with G = [I S; 0 I] per end (frame.c:297-324), the four 3x3
-// blocks of each end pair combined before they are added.
-template <bool PL, int BS, bool OFS>
+// blocks of each end pair combined before they are added, and end releases
+// condensed out of k (frame.c:280-283).
+template <bool PL, int BS, bool XTR>
 __global__ void __launch_bounds__(TPB) k_frame_tangent(
     const int32_t* __restrict__ els, int n, const int32_t* __restrict__ blk,
     const double* __restrict__ c1, const double* __restrict__ c2, const double* __restrict__ c3,
@@ -295,7 +436,7 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent(
     const double* __restrict__ llen, const double* __restrict__ istrong, const double* __restrict__ iweak,
     const double* __restrict__ ipolar, const double* __restrict__ iwarp,
     const double* __restrict__ defllen, const double* __restrict__ ef, double* __restrict__ K,
-    long ntr, int geometric, PlasticIn pin, const double* __restrict__ ofs) {
+    long ntr, int geometric, PlasticIn pin, MemberXtr xtr) {
     int it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= n) return;
     const long e = els[it], p = ntr + e;
@@ -312,10 +453,11 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent(
     int32_t bk[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) bk[q] = blk[e * 4 + q];
-    if constexpr (OFS) {
+    if constexpr (XTR) {
         double S[2][3][3];
-        link_S(ofs + e * 6, S[0]);
-        link_S(ofs + e * 6 + 3, S[1]);
+        member_links(xtr, e, S);
+        FrameRel rl;
+        rel_setup<PL>(rl, k, pl, xtr.rel ? xtr.rel[e] : 0);
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -332,9 +474,9 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent(
                         for (int r = 0; r < 3; ++r)
 #pragma unroll
                             for (int l = 0; l < 3; ++l)
-                                tmp[r][l] = R[0][r] * kent<PL>(k, pl, tidx(t1, 0), tidx(t2, l)) +
-                                            R[1][r] * kent<PL>(k, pl, tidx(t1, 1), tidx(t2, l)) +
-                                            R[2][r] * kent<PL>(k, pl, tidx(t1, 2), tidx(t2, l));
+                                tmp[r][l] = R[0][r] * kx<PL, true>(k, pl, rl, tidx(t1, 0), tidx(t2, l)) +
+                                            R[1][r] * kx<PL, true>(k, pl, rl, tidx(t1, 1), tidx(t2, l)) +
+                                            R[2][r] * kx<PL, true>(k, pl, rl, tidx(t1, 2), tidx(t2, l));
 #pragma unroll
                         for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -385,18 +527,18 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent(
                     double cu[3], ct[3], ru[3], rt[3];
 #pragma unroll
                     for (int r = 0; r < 3; ++r) {
-                        cu[r] = R[0][r] * kent<PL>(k, pl, tidx(2 * a, 0), 7 * b + 6) +
-                                R[1][r] * kent<PL>(k, pl, tidx(2 * a, 1), 7 * b + 6) +
-                                R[2][r] * kent<PL>(k, pl, tidx(2 * a, 2), 7 * b + 6);
-                        ct[r] = R[0][r] * kent<PL>(k, pl, tidx(2 * a + 1, 0), 7 * b + 6) +
-                                R[1][r] * kent<PL>(k, pl, tidx(2 * a + 1, 1), 7 * b + 6) +
-                                R[2][r] * kent<PL>(k, pl, tidx(2 * a + 1, 2), 7 * b + 6);
-                        ru[r] = kent<PL>(k, pl, 7 * a + 6, tidx(2 * b, 0)) * R[0][r] +
-                                kent<PL>(k, pl, 7 * a + 6, tidx(2 * b, 1)) * R[1][r] +
-                                kent<PL>(k, pl, 7 * a + 6, tidx(2 * b, 2)) * R[2][r];
-                        rt[r] = kent<PL>(k, pl, 7 * a + 6, tidx(2 * b + 1, 0)) * R[0][r] +
-                                kent<PL>(k, pl, 7 * a + 6, tidx(2 * b + 1, 1)) * R[1][r] +
-                                kent<PL>(k, pl, 7 * a + 6, tidx(2 * b + 1, 2)) * R[2][r];
+                        cu[r] = R[0][r] * kx<PL, true>(k, pl, rl, tidx(2 * a, 0), 7 * b + 6) +
+                                R[1][r] * kx<PL, true>(k, pl, rl, tidx(2 * a, 1), 7 * b + 6) +
+                                R[2][r] * kx<PL, true>(k, pl, rl, tidx(2 * a, 2), 7 * b + 6);
+                        ct[r] = R[0][r] * kx<PL, true>(k, pl, rl, tidx(2 * a + 1, 0), 7 * b + 6) +
+                                R[1][r] * kx<PL, true>(k, pl, rl, tidx(2 * a + 1, 1), 7 * b + 6) +
+                                R[2][r] * kx<PL, true>(k, pl, rl, tidx(2 * a + 1, 2), 7 * b + 6);
+                        ru[r] = kx<PL, true>(k, pl, rl, 7 * a + 6, tidx(2 * b, 0)) * R[0][r] +
+                                kx<PL, true>(k, pl, rl, 7 * a + 6, tidx(2 * b, 1)) * R[1][r] +
+                                kx<PL, true>(k, pl, rl, 7 * a + 6, tidx(2 * b, 2)) * R[2][r];
+                        rt[r] = kx<PL, true>(k, pl, rl, 7 * a + 6, tidx(2 * b + 1, 0)) * R[0][r] +
+                                kx<PL, true>(k, pl, rl, 7 * a + 6, tidx(2 * b + 1, 1)) * R[1][r] +
+                                kx<PL, true>(k, pl, rl, 7 * a + 6, tidx(2 * b + 1, 2)) * R[2][r];
                     }
 #pragma unroll
                     for (int r = 0; r < 3; ++r) {
@@ -411,7 +553,7 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent(
                         Kb[6 * 7 + r] += ru[r];
                         Kb[6 * 7 + 3 + r] += y;
                     }
-                    Kb[48] += kent<PL>(k, pl, 7 * a + 6, 7 * b + 6);
+                    Kb[48] += kx<PL, true>(k, pl, rl, 7 * a + 6, 7 * b + 6);
                 }
             }
         return;
@@ -614,10 +756,10 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent_gather(
 // one output triple at a time.
 // One member of forces_fr: ef_i, and the member's end forces in global axes
 // (T_i^T ef_i) handed to out(local DOF, value); with BS 7 the warping
-// increments enter and the bimoments are handed out too; with end offsets
-// (OFS) the joint increments reach the ends through u + S theta and the
-// joints take (F_u, F_t + S^T F_u).
-template <int BS, bool OFS, class Out>
+// increments enter and the bimoments are handed out too; XTR: with end
+// offsets the joint increments reach the ends through u + S theta and the
+// joints take (F_u, F_t + S^T F_u), and end releases are condensed out of k.
+template <int BS, bool XTR, class Out>
 __device__ __forceinline__ void frame_member_forces(
     long e, const int32_t* __restrict__ mc, const double* __restrict__ dd, const double* __restrict__ c1p,
     const double* __restrict__ c2p, const double* __restrict__ c3p, const double* __restrict__ c1c,
@@ -625,7 +767,7 @@ __device__ __forceinline__ void frame_member_forces(
     const double* __restrict__ gmod, const double* __restrict__ carea, const double* __restrict__ llen,
     const double* __restrict__ istrong, const double* __restrict__ iweak, const double* __restrict__ ipolar,
     const double* __restrict__ iwarp, const double* __restrict__ defllen_ip, const double* __restrict__ ef_ip,
-    double* __restrict__ ef_i, long ntr, int geometric, const double* __restrict__ ofs, Out&& out) {
+    double* __restrict__ ef_i, long ntr, int geometric, MemberXtr xtr, Out&& out) {
     const long p = ntr + e;
     const double* fe = ef_ip + ntr * 2 + e * 14;
     FrameCoef k;
@@ -644,9 +786,11 @@ __device__ __forceinline__ void frame_member_forces(
             Dj[t][m] = q ? dd[q - 1] : 0.0;
         }
     double S[2][3][3];
-    if constexpr (OFS) {
-        link_S(ofs + e * 6, S[0]);
-        link_S(ofs + e * 6 + 3, S[1]);
+    FrameRel rl;
+    FramePl pl;
+    if constexpr (XTR) {
+        member_links(xtr, e, S);
+        rel_setup<false>(rl, k, pl, xtr.rel ? xtr.rel[e] : 0);
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -684,8 +828,10 @@ __device__ __forceinline__ void frame_member_forces(
 #pragma unroll
             for (int t = 0; t < 4; ++t)
 #pragma unroll
-                for (int m = 0; m < 3; ++m) s += kf(k, tidx(to, r), tidx(t, m)) * dl[t][m];
-            if constexpr (BS == 7) s += kf(k, tidx(to, r), 6) * dw[0] + kf(k, tidx(to, r), 13) * dw[1];
+                for (int m = 0; m < 3; ++m) s += kx<false, XTR>(k, pl, rl, tidx(to, r), tidx(t, m)) * dl[t][m];
+            if constexpr (BS == 7)
+                s += kx<false, XTR>(k, pl, rl, tidx(to, r), 6) * dw[0] +
+                     kx<false, XTR>(k, pl, rl, tidx(to, r), 13) * dw[1];
             v[r] = fe[tidx(to, r)] + s;
         }
         double g[3];
@@ -698,7 +844,7 @@ __device__ __forceinline__ void frame_member_forces(
         double F[3];
 #pragma unroll
         for (int m = 0; m < 3; ++m) F[m] = Ri[0][m] * g[0] + Ri[1][m] * g[1] + Ri[2][m] * g[2];
-        if constexpr (OFS) {
+        if constexpr (XTR) {
             if (to % 2 == 0) {
 #pragma unroll
                 for (int m = 0; m < 3; ++m) Fu[m] = F[m];
@@ -719,8 +865,9 @@ __device__ __forceinline__ void frame_member_forces(
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
-            for (int m = 0; m < 3; ++m) s += kf(k, i, tidx(t, m)) * dl[t][m];
-        if constexpr (BS == 7) s += kf(k, i, 6) * dw[0] + kf(k, i, 13) * dw[1];
+            for (int m = 0; m < 3; ++m) s += kx<false, XTR>(k, pl, rl, i, tidx(t, m)) * dl[t][m];
+        if constexpr (BS == 7)
+            s += kx<false, XTR>(k, pl, rl, i, 6) * dw[0] + kx<false, XTR>(k, pl, rl, i, 13) * dw[1];
         ei[i] = fe[i] + s;
         if constexpr (BS == 7) out(i, ei[i]);
     }
@@ -728,7 +875,7 @@ __device__ __forceinline__ void frame_member_forces(
 
 // forces_fr, elastic branch (frame.c:902-1096, 1273-1309), for the members of
 // one colour, scattered by mcode into f.
-template <int BS, bool OFS>
+template <int BS, bool XTR>
 __global__ void __launch_bounds__(TPB) k_frame_forces(
     const int32_t* __restrict__ els, int n, const int32_t* __restrict__ mcode,
     const double* __restrict__ dd, const double* __restrict__ c1p, const double* __restrict__ c2p,
@@ -738,13 +885,13 @@ __global__ void __launch_bounds__(TPB) k_frame_forces(
     const double* __restrict__ istrong, const double* __restrict__ iweak,
     const double* __restrict__ ipolar, const double* __restrict__ iwarp,
     const double* __restrict__ defllen_ip, const double* __restrict__ ef_ip,
-    double* __restrict__ ef_i, double* __restrict__ f, long ntr, int geometric, const double* __restrict__ ofs) {
+    double* __restrict__ ef_i, double* __restrict__ f, long ntr, int geometric, MemberXtr xtr) {
     int it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= n) return;
     const long e = els[it];
     const int32_t* mc = mcode + ntr * 6 + e * 14;
-    frame_member_forces<BS, OFS>(e, mc, dd, c1p, c2p, c3p, c1c, c2c, c3c, emod, gmod, carea, llen, istrong, iweak,
-                            ipolar, iwarp, defllen_ip, ef_ip, ef_i, ntr, geometric, ofs, [&](int i, double val) {
+    frame_member_forces<BS, XTR>(e, mc, dd, c1p, c2p, c3p, c1c, c2c, c3c, emod, gmod, carea, llen, istrong, iweak,
+                            ipolar, iwarp, defllen_ip, ef_ip, ef_i, ntr, geometric, xtr, [&](int i, double val) {
                                 const int32_t q = mc[i];
                                 if (q) f[q - 1] += val;
                             });
@@ -752,7 +899,7 @@ __global__ void __launch_bounds__(TPB) k_frame_forces(
 
 // Frame-only models: every member in one launch, its global end forces into
 // h[e * 2 BS + BS * end + slot] ...
-template <int BS, bool OFS>
+template <int BS, bool XTR>
 __global__ void __launch_bounds__(TPB) k_frame_forces_members(
     long n, const int32_t* __restrict__ mcode, const double* __restrict__ dd, const double* __restrict__ c1p,
     const double* __restrict__ c2p, const double* __restrict__ c3p, const double* __restrict__ c1c,
@@ -760,12 +907,12 @@ __global__ void __launch_bounds__(TPB) k_frame_forces_members(
     const double* __restrict__ gmod, const double* __restrict__ carea, const double* __restrict__ llen,
     const double* __restrict__ istrong, const double* __restrict__ iweak, const double* __restrict__ ipolar,
     const double* __restrict__ iwarp, const double* __restrict__ defllen_ip, const double* __restrict__ ef_ip,
-    double* __restrict__ ef_i, double* __restrict__ h, int geometric, const double* __restrict__ ofs) {
+    double* __restrict__ ef_i, double* __restrict__ h, int geometric, MemberXtr xtr) {
     const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (e >= n) return;
     double* he = h + e * (2 * BS);
-    frame_member_forces<BS, OFS>(e, mcode + e * 14, dd, c1p, c2p, c3p, c1c, c2c, c3c, emod, gmod, carea, llen,
-                            istrong, iweak, ipolar, iwarp, defllen_ip, ef_ip, ef_i, 0, geometric, ofs,
+    frame_member_forces<BS, XTR>(e, mcode + e * 14, dd, c1p, c2p, c3p, c1c, c2c, c3c, emod, gmod, carea, llen,
+                            istrong, iweak, ipolar, iwarp, defllen_ip, ef_ip, ef_i, 0, geometric, xtr,
                             [&](int i, double val) { he[BS * (i / 7) + i % 7] = val; });
 }
 
@@ -826,9 +973,9 @@ __device__ inline double regula_falsi(double p, double dp, double my, double dmy
 // flags the unloading ends 2 with code 2.  code[4e] = code, code[4e+1..2] =
 // the member's flags after its pass; st->first <- the lowest member with a
 // code.  With BS 7 the warping increments and bimoments enter as in the
-// elastic pass, and end offsets (ofs non-null) as in frame_member_forces; the
-// fixed-end forces are zero (no distributed loads).
-template <int BS, bool OFS>
+// elastic pass, and end offsets and releases (XTR) as in frame_member_forces;
+// the fixed-end forces are zero (no distributed loads).
+template <int BS, bool XTR>
 __global__ void __launch_bounds__(64) k_frame_forces_plastic(
     long n, const int32_t* __restrict__ mcode, const double* __restrict__ dd, const double* __restrict__ c1p,
     const double* __restrict__ c2p, const double* __restrict__ c3p, const double* __restrict__ c1c,
@@ -837,7 +984,7 @@ __global__ void __launch_bounds__(64) k_frame_forces_plastic(
     const double* __restrict__ istrong, const double* __restrict__ iweak, const double* __restrict__ ipolar,
     const double* __restrict__ iwarp, const double* __restrict__ defllen_ip, const double* __restrict__ ef_ip,
     PlasticIn pin, double* __restrict__ efc, double* __restrict__ h, int32_t* __restrict__ code,
-    double* __restrict__ tau, PlasticStatus* __restrict__ st, long ntr, const double* __restrict__ ofs) {
+    double* __restrict__ tau, PlasticStatus* __restrict__ st, long ntr, MemberXtr xtr) {
     const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (e >= n) return;
     const long p = ntr + e;
@@ -862,9 +1009,10 @@ __global__ void __launch_bounds__(64) k_frame_forces_plastic(
             Dj[t][m] = q ? dd[q - 1] : 0.0;
         }
     double S[2][3][3];
-    if constexpr (OFS) {
-        link_S(ofs + e * 6, S[0]);
-        link_S(ofs + e * 6 + 3, S[1]);
+    FrameRel rl;
+    if constexpr (XTR) {
+        member_links(xtr, e, S);
+        rel_setup<true>(rl, k, pl, xtr.rel ? xtr.rel[e] : 0);
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -900,9 +1048,9 @@ __global__ void __launch_bounds__(64) k_frame_forces_plastic(
 #pragma unroll
             for (int t = 0; t < 4; ++t)
 #pragma unroll
-                for (int m = 0; m < 3; ++m) s += kent<true>(k, pl, tidx(to, r), tidx(t, m)) * dl[t][m];
+                for (int m = 0; m < 3; ++m) s += kx<true, XTR>(k, pl, rl, tidx(to, r), tidx(t, m)) * dl[t][m];
             if constexpr (BS == 7)
-                s += kent<true>(k, pl, tidx(to, r), 6) * dw[0] + kent<true>(k, pl, tidx(to, r), 13) * dw[1];
+                s += kx<true, XTR>(k, pl, rl, tidx(to, r), 6) * dw[0] + kx<true, XTR>(k, pl, rl, tidx(to, r), 13) * dw[1];
             v[r] = fe[tidx(to, r)] + s;
         }
 #pragma unroll
@@ -913,7 +1061,7 @@ __global__ void __launch_bounds__(64) k_frame_forces_plastic(
             he[BS * (i / 7) + i % 7] =
                 Ri[0][m] * fi[tidx(to, 0)] + Ri[1][m] * fi[tidx(to, 1)] + Ri[2][m] * fi[tidx(to, 2)];
         }
-        if (OFS && to % 2 == 1) {
+        if (XTR && to % 2 == 1) {
             // joint moments: F_t + S^T F_u of this end
             const int a = to / 2;
             double Fu[3];
@@ -931,8 +1079,8 @@ __global__ void __launch_bounds__(64) k_frame_forces_plastic(
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
-            for (int m = 0; m < 3; ++m) s += kent<true>(k, pl, i, tidx(t, m)) * dl[t][m];
-        if constexpr (BS == 7) s += kent<true>(k, pl, i, 6) * dw[0] + kent<true>(k, pl, i, 13) * dw[1];
+            for (int m = 0; m < 3; ++m) s += kx<true, XTR>(k, pl, rl, i, tidx(t, m)) * dl[t][m];
+        if constexpr (BS == 7) s += kx<true, XTR>(k, pl, rl, i, 6) * dw[0] + kx<true, XTR>(k, pl, rl, i, 13) * dw[1];
         fi[i] = fe[i] + s;
         if constexpr (BS == 7) he[BS * w + 6] = fi[i];
     }
@@ -1075,6 +1223,8 @@ __global__ void k_update_frame(const int32_t* __restrict__ minc, const double* _
 
 namespace {
 PlasticIn plastic_in(const Dev& d) { return PlasticIn{d.yld, d.yield, d.zstrong, d.zweak}; }
+MemberXtr member_xtr(const Dev& d) { return MemberXtr{d.ofs, d.relmask}; }
+bool has_xtr(const Dev& d) { return d.ofs || d.relmask; }
 
 // f(std::integral_constant<int, BS>) for the joint slots of the model: 7 when
 // a warping DOF is free, else 6 (frames never run with 3)
@@ -1098,11 +1248,11 @@ void launch_frame_tangent(Dev& d, int g) {
                 hipLaunchKernelGGL(kern, dim3(nblk(n)), dim3(TPB), 0, d.s, d.col_el_fr + d.col_ptr_fr[c], n,
                                    d.blk_fr, d.c1[g], d.c2[g], d.c3[g], d.emod, d.gmod, d.carea, d.llen,
                                    d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[g], d.ef[g], d.K, d.ntr, geo,
-                                   pin, d.ofs);
+                                   pin, member_xtr(d));
             };
-            if (d.anaflag == 3 && d.ofs) go(k_frame_tangent<true, BS, true>, plastic_in(d));
+            if (d.anaflag == 3 && has_xtr(d)) go(k_frame_tangent<true, BS, true>, plastic_in(d));
             else if (d.anaflag == 3) go(k_frame_tangent<true, BS, false>, plastic_in(d));
-            else if (d.ofs) go(k_frame_tangent<false, BS, true>, PlasticIn{});
+            else if (has_xtr(d)) go(k_frame_tangent<false, BS, true>, PlasticIn{});
             else go(k_frame_tangent<false, BS, false>, PlasticIn{});
         }
     });
@@ -1110,7 +1260,8 @@ void launch_frame_tangent(Dev& d, int g) {
 }
 
 void launch_frame_tangent_gather(Dev& d, int g) {
-    if (d.ofs) throw DeviceError("gathered frame tangent: members with end offsets take the colour launches");
+    if (has_xtr(d))
+        throw DeviceError("gathered frame tangent: members with end offsets or releases take the colour launches");
     const int geo = d.anaflag == 2 || d.anaflag == 3;
     with_bs(d, [&](auto B) {
         constexpr int BS = decltype(B)::value;
@@ -1132,7 +1283,7 @@ void launch_frame_tangent_gather(Dev& d, int g) {
 void launch_frame_tangent_band(Dev& d, int g, double* tiles, int w, int* bad) {
     if (d.bs != 6 || d.ntr || d.nsh) throw DeviceError("frame tangent into the band: frame-only models, warping fixed");
     if (d.anaflag == 3) throw DeviceError("frame tangent into the band: elastic (ANAFLAG 2) members only");
-    if (d.ofs) throw DeviceError("frame tangent into the band: members without end offsets only");
+    if (has_xtr(d)) throw DeviceError("frame tangent into the band: members without end offsets or releases only");
     hipLaunchKernelGGL((k_frame_tangent_gather<true, false, 6>), dim3(nblk(2 * d.nnzb)), dim3(TPB), 0, d.s,
                        d.gat_ptr_fr, d.gat_item_fr, d.blkrow, d.colidx, d.eq_of_slot, d.nnzb, d.c1[g], d.c2[g],
                        d.c3[g], d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp,
@@ -1151,9 +1302,9 @@ static void launch_frame_forces_plastic(Dev& d, double* f, const double* dd) {
             hipLaunchKernelGGL(kern, dim3(nblk(d.nfr, 64)), dim3(64), 0, d.s, d.nfr, d.mcode, dd, d.c1[G_IP],
                                d.c2[G_IP], d.c3[G_IP], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.emod, d.gmod, d.carea,
                                d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[G_IP], d.ef[G_IP],
-                               plastic_in(d), d.pl_efc, d.fh, d.pl_code, d.pl_tau, d.pl_st, d.ntr, d.ofs);
+                               plastic_in(d), d.pl_efc, d.fh, d.pl_code, d.pl_tau, d.pl_st, d.ntr, member_xtr(d));
         };
-        if (d.ofs) go(k_frame_forces_plastic<BS, true>);
+        if (has_xtr(d)) go(k_frame_forces_plastic<BS, true>);
         else go(k_frame_forces_plastic<BS, false>);
         hipLaunchKernelGGL(k_frame_plastic_commit, dim3(nblk(d.nfr, 256)), dim3(256), 0, d.s, d.nfr, d.pl_efc,
                            d.pl_code, d.pl_tau, d.pl_st, d.yld, d.ef[G_I], d.ntr);
@@ -1192,9 +1343,9 @@ void launch_frame_forces(Dev& d, double* f, const double* dd) {
                 hipLaunchKernelGGL(kern, dim3(nblk(d.nfr)), dim3(TPB), 0, d.s, d.nfr, d.mcode, dd, d.c1[G_IP],
                                    d.c2[G_IP], d.c3[G_IP], d.c1[G_I], d.c2[G_I], d.c3[G_I], d.emod, d.gmod,
                                    d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[G_IP],
-                                   d.ef[G_IP], d.ef[G_I], d.fh, int(d.anaflag == 2), d.ofs);
+                                   d.ef[G_IP], d.ef[G_I], d.fh, int(d.anaflag == 2), member_xtr(d));
             };
-            if (d.ofs) go(k_frame_forces_members<BS, true>);
+            if (has_xtr(d)) go(k_frame_forces_members<BS, true>);
             else go(k_frame_forces_members<BS, false>);
             hipLaunchKernelGGL((k_frame_forces_gather<BS>), dim3(nblk(d.nslot, 256)), dim3(256), 0, d.s,
                                d.jnt_ptr_fr, d.jnt_item_fr, d.eq_of_slot, d.nslot, d.fh, f, nullptr);
@@ -1208,9 +1359,9 @@ void launch_frame_forces(Dev& d, double* f, const double* dd) {
                                    d.mcode, dd, d.c1[G_IP], d.c2[G_IP], d.c3[G_IP], d.c1[G_I], d.c2[G_I],
                                    d.c3[G_I], d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar,
                                    d.iwarp, d.defllen[G_IP], d.ef[G_IP], d.ef[G_I], f, d.ntr,
-                                   int(d.anaflag == 2), d.ofs);
+                                   int(d.anaflag == 2), member_xtr(d));
             };
-            if (d.ofs) go(k_frame_forces<BS, true>);
+            if (has_xtr(d)) go(k_frame_forces<BS, true>);
             else go(k_frame_forces<BS, false>);
         }
     });

@@ -719,7 +719,7 @@ __global__ void __launch_bounds__(TPB) k_bsr_spmv(const T* __restrict__ K, const
 // ============================================================== launchers
 void launch_tangent(Dev& d, int gx, int g, bool want_fp32) {
     const long nb = d.nnzb * d.bs * d.bs;
-    if (d.nfr && !d.ntr && !d.nsh && !d.ofs) {
+    if (d.nfr && !d.ntr && !d.nsh && !d.ofs && !d.relmask) {
         // frame-only: every block is summed and written once, fixed slots included
         launch_frame_tangent_gather(d, g);
         if (want_fp32) {
@@ -946,6 +946,8 @@ void dev_create(Dev& d, const Model& m) {
     d.iwarp = upd(m.iwarp);
     d.auxpt = upd(m.auxpt);
     if (std::any_of(m.osflag.begin(), m.osflag.end(), [](int v) { return v != 0; })) d.ofs = upd(m.offset);
+    if (std::any_of(m.relmask.begin(), m.relmask.end(), [](int32_t v) { return v != 0; }))
+        d.relmask = upi(m.relmask);
     d.nu = upd(m.nu);
     d.thick = upd(m.thick);
     d.xlocal = upd(m.xlocal);
@@ -1001,7 +1003,7 @@ void dev_destroy(Dev& d) {
     void* ps[] = {d.minc, d.mcode, d.jcode, d.eq_of_slot, d.rowptr, d.colidx, d.diagblk, d.blk_tr,
                   d.blk_sh, d.blk_fr, d.col_el_tr, d.col_el_sh, d.col_el_fr, d.blkrow, d.gat_ptr_fr,
                   d.gat_item_fr, d.jnt_ptr_fr, d.jnt_item_fr, d.fh, d.emod, d.carea,
-                  d.llen, d.dens, d.gmod, d.istrong, d.iweak, d.ipolar, d.iwarp, d.auxpt, d.ofs, d.nu,
+                  d.llen, d.dens, d.gmod, d.istrong, d.iweak, d.ipolar, d.iwarp, d.auxpt, d.ofs, d.relmask, d.nu,
                   d.thick, d.xlocal, d.farea, d.slength, d.K, d.K32, d.red, d.yield, d.zstrong,
                   d.zweak, d.yld, d.pl_efc, d.pl_tau, d.pl_code, d.pl_st};
     for (void* p : ps)

@@ -391,12 +391,22 @@ bool load_model(const char* path, Model& m, std::string& err) {
         for (long e = 0; e < nfr; ++e) {
             m.yield[ntr + e] = t.d(); m.zstrong[e] = t.d(); m.zweak[e] = t.d();
         }
+        // member end bending releases (frame.c:193-220): strong axis end 1,
+        // weak axis end 1, strong axis end 2, weak axis end 2
+        m.relmask.assign(size_t(nfr), 0);
         for (;;) {
             long j = t.l();
-            for (int k = 0; k < 4; ++k) t.l();
+            long r[4];
+            for (long& v : r) v = t.l();
             if (t.bad()) FAIL("truncated frame releases");
             if (j == 0) break;
-            FAIL("frame end releases are not on this path");
+            if (j < 1 || j > nfr) FAIL("frame release member out of range");
+            int32_t mask = 0;
+            for (int k = 0; k < 4; ++k) {
+                if (r[k] != 0 && r[k] != 1) FAIL("frame release flags are 0 or 1");
+                mask |= int32_t(r[k]) << k;
+            }
+            m.relmask[size_t(j - 1)] = mask;
         }
     }
     m.nu.assign(size_t(nsh), 0); m.thick.assign(size_t(nsh), 0); m.dens_sh.assign(size_t(nsh), 0);

@@ -23,6 +23,7 @@ struct Model {
     std::vector<double> emod, yield, carea, llength, dens_tr;
     std::vector<double> gmod, dens_fr, istrong, iweak, ipolar, iwarp, zstrong, zweak;
     std::vector<double> offset, auxpt, xfr;
+    std::vector<int32_t> relmask;   // per frame member: its release flags as a mask (bit a: flag a)
     std::vector<int> osflag;
     std::vector<double> nu, thick, dens_sh, xlocal, farea, slength;
     std::vector<double> dens;        // the reference's one density array (see load_model)

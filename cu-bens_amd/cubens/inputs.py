@@ -94,6 +94,18 @@ class ModelText:
         return ModelText(self.text.replace(end, lines + end, 1), self.nj, self.n_tr, self.n_fr, self.n_sh,
                          dict(self.info, offsets=dict(rows)))
 
+    def with_releases(self, rows: dict) -> "ModelText":
+        """The same frame model with member end bending releases: ``rows`` maps
+        a member (1-based) to its four flags (strong axis end 1, weak axis end
+        1, strong axis end 2, weak axis end 2; 1 released), written before the
+        zero line that ends the releases, the first line "0,0,0,0,0"."""
+        assert self.n_fr > 0
+        lines = self.text.split("\n")
+        at = lines.index("0,0,0,0,0")
+        add = [f"{m}," + ",".join(str(int(v)) for v in f) for m, f in sorted(rows.items())]
+        return ModelText("\n".join(lines[:at] + add + lines[at:]), self.nj, self.n_tr, self.n_fr, self.n_sh,
+                         dict(self.info, releases=dict(rows)))
+
 
 # --------------------------------------------------------------------------
 # C1: shallow space-truss dome, Newton-Raphson load control
@@ -619,6 +631,49 @@ def cantilever_frame(n_elem: int = 4, length: float = 3.0, r_o: float = 0.1, loa
     return ModelText(out.getvalue(), nj, n_fr=n_elem,
                      info={"E": sec["E"], "I": sec["I"], "A": sec["A"], "length": length,
                            "load": load, "neq": 6 * n_elem})
+
+
+def clamped_beam(n_elem: int = 2, length: float = 4.0, r_o: float = 0.1, load: float = -1000.0,
+                 direction: int = 2) -> ModelText:
+    """A straight beam of ``n_elem`` frame members along x, clamped at both end
+    joints (warping fixed everywhere), with ``load`` in ``direction`` at the
+    middle joint: a linear static analysis.  The auxiliary points above the
+    members make local y = global z and local z = -global y, so a load in
+    direction 3 bends the members about their local z (strong) axis and one
+    in direction 2 about local y (weak).  Euler-Bernoulli members and a
+    circular section (equal second moments), so with end releases the
+    mid-span deflection takes the textbook forms, e.g. P L^3/(192 E I)
+    without releases and P L^3/(48 E I) with both supports released in the
+    load's plane."""
+    assert n_elem % 2 == 0 and direction in (2, 3)
+    sec = chs_section(r_o)
+    nj = n_elem + 1
+    out = io.StringIO()
+    w = out.write
+    w(f"{_linear_head('static', None, 0, 1)}0\n1\n{nj}\n0,{n_elem},0,0,0\n")
+    for e in range(n_elem):
+        w(f"{e + 1},{e + 2}\n")
+    for j in (1, nj):
+        for d in range(1, 7):
+            w(f"{j},{d}\n")
+    for j in range(1, nj + 1):
+        w(f"{j},7,0\n")
+    w("0,0\n")
+    for j in range(nj):
+        w(f"{_fmt(length * j / n_elem)},0.0,0.0\n")
+    for _ in range(n_elem):
+        w(f"{_fmt(sec['E'])},{_fmt(sec['G'])},7850.0,{_fmt(sec['A'])},{_fmt(sec['I'])},"
+          f"{_fmt(sec['I'])},{_fmt(sec['J'])},0.0\n")
+    w("0,0,0,0,0,0,0\n")
+    for e in range(n_elem):
+        w(f"{_fmt(length * e / n_elem)},0.0,1.0\n")
+    for _ in range(n_elem):
+        w("1e+30,1.0,1.0\n")
+    w("0,0,0,0,0\n")
+    w(f"{n_elem // 2 + 1},{direction},{_fmt(load)}\n0,0,0\n0,0,0\n1.0\n")
+    return ModelText(out.getvalue(), nj, n_fr=n_elem,
+                     info={"E": sec["E"], "I": sec["I"], "length": length, "load": load,
+                           "direction": direction})
 
 
 # --------------------------------------------------------------------------

@@ -113,6 +113,8 @@ void orc_shell_elem_tangent(double E, double nu, double t, const double xl[3], d
 void orc_frame_ke(double E, double G, double A, double L, double Is, double Iw, double J,
                   double Cw, double k[14][14]);
 void orc_frame_kg(const double f[14], double L, double A, double Ip, double k[14][14]);
+void orc_gj_inverse(double *a, int n);
+void orc_frame_release(double k[14][14], const int flags[4]);
 void orc_stiff_fr(const orc_dims *d, double *ss, const orc_model *m, const double *llength,
                   const double *defllen_ip, const double *c1_ip, const double *c2_ip,
                   const double *c3_ip, const double *ef_ip, const double *efFE_ip,

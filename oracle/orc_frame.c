@@ -363,6 +363,85 @@ int orc_frame_unload(const double phi[2], const double eftot_i[14], double Py, d
     return lam[0] < -1e-8 ? 2 + which : 0;
 }
 
+/* inverse (misc.c:284-342): Gauss-Jordan on [A | I] without normalising the
+ * pivot rows, which are divided out at the end.  Its pivot search scans row i
+ * (not column i) from the diagonal for the first non-zero entry and swaps
+ * rows i and that column's index; with a non-zero diagonal, as the released
+ * blocks of the frame tangent have, no swap happens. */
+void orc_gj_inverse(double *a, int n)
+{
+    double aug[4][8];
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < 2 * n; ++j)
+            aug[i][j] = j < n ? a[i * n + j] : (j - n == i ? 1.0 : 0.0);
+    for (int i = 0; i < n; ++i) {
+        int k = i;
+        for (int j = i; j < n; ++j)
+            if (aug[i][j] != 0) {
+                k = j;
+                break;
+            }
+        if (k != i)
+            for (int j = 0; j < 2 * n; ++j) {
+                double t = aug[i][j];
+                aug[i][j] = aug[k][j];
+                aug[k][j] = t;
+            }
+        for (int j = 0; j < n; ++j) {
+            if (j == i) continue;
+            double f = aug[j][i] / aug[i][i];
+            for (int q = 0; q < 2 * n; ++q) aug[j][q] -= f * aug[i][q];
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) a[i * n + j] = aug[i][n + j] / aug[i][i];
+}
+
+/* release (frame.c:798-896): member end bending releases by static
+ * condensation, k <- k - k G (G^T k G)^-1 G^T k, where G selects the released
+ * local DOFs in the order strong axis end 1 (5), weak axis end 1 (4), strong
+ * axis end 2 (12), weak axis end 2 (11); flags[0..3] say which are released.
+ * G^T k is taken as (k G)^T, the reference's order.  One release inverts by
+ * 1/x, two by the adjugate over the determinant, three or four by
+ * orc_gj_inverse. */
+void orc_frame_release(double k[14][14], const int flags[4])
+{
+    static const int dof[4] = {5, 4, 12, 11};
+    int r[4], n = 0;
+    for (int a = 0; a < 4; ++a)
+        if (flags[a]) r[n++] = dof[a];
+    if (!n) return;
+    double kG[14][4], H[16];
+    for (int i = 0; i < 14; ++i)
+        for (int a = 0; a < n; ++a) kG[i][a] = k[i][r[a]];
+    for (int a = 0; a < n; ++a)            /* (G^T k G)[a][b] = k[r_b][r_a] */
+        for (int b = 0; b < n; ++b) H[a * n + b] = kG[r[b]][a];
+    if (n == 1) {
+        H[0] = 1 / H[0];
+    } else if (n == 2) {
+        double det = H[0] * H[3] - H[1] * H[2], t = H[0];
+        H[0] = H[3] / det;
+        H[3] = t / det;
+        H[1] *= -1 / det;
+        H[2] *= -1 / det;
+    } else {
+        orc_gj_inverse(H, n);
+    }
+    double kGH[14][4];
+    for (int i = 0; i < 14; ++i)
+        for (int b = 0; b < n; ++b) {
+            double s = 0;
+            for (int a = 0; a < n; ++a) s += kG[i][a] * H[a * n + b];
+            kGH[i][b] = s;
+        }
+    for (int i = 0; i < 14; ++i)
+        for (int j = 0; j < 14; ++j) {
+            double s = 0;
+            for (int b = 0; b < n; ++b) s += kGH[i][b] * kG[j][b];
+            k[i][j] -= s;
+        }
+}
+
 static void frame_ke_of(const orc_model *m, long n, const double *llength, double k[14][14])
 {
     long ne = m->dm.NE_TR + n;
@@ -370,9 +449,10 @@ static void frame_ke_of(const orc_model *m, long n, const double *llength, doubl
                  m->ipolar[n], m->iwarp[n], k);
 }
 
-/* Element tangent of frame n in the local frame: elastic + geometric, and
- * under ANAFLAG 3 the plastic reduction unless both ends are flagged 2
- * (frame.c:266-277).  eftot: the member's total end forces (ef + efFE). */
+/* Element tangent of frame n in the local frame: elastic + geometric, under
+ * ANAFLAG 3 the plastic reduction unless both ends are flagged 2
+ * (frame.c:266-277), then the member's end releases (frame.c:280-283).
+ * eftot: the member's total end forces (ef + efFE). */
 static void frame_local_tangent(const orc_model *m, long n, const double *llength,
                                 const double *defllen_ip, const double *eftot, const int *yld,
                                 double k[14][14])
@@ -385,6 +465,7 @@ static void frame_local_tangent(const orc_model *m, long n, const double *llengt
     if (d->ANAFLAG == 3 && (yld[n * 2] != 2 || yld[n * 2 + 1] != 2))
         orc_frame_plastic(k, eftot, yld + n * 2, m->carea[ne] * m->yield[ne],
                           m->zweak[n] * m->yield[ne], m->zstrong[n] * m->yield[ne]);
+    if (m->mendrel[n * 5]) orc_frame_release(k, m->mendrel + n * 5 + 1);
 }
 
 static void total_forces(const orc_dims *d, long n, const double *ef, const double *efFE,

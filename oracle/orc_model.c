@@ -487,14 +487,17 @@ orc_model *orc_open(const char *path, char *err, int errlen)
             m->zstrong[e] = tk_dbl(&t);
             m->zweak[e] = tk_dbl(&t);
         }
-        for (;;) {   /* member end bending releases */
+        for (;;) {   /* member end bending releases (frame.c:193-220) */
             long j = tk_long(&t);
             int r[4];
             for (int k = 0; k < 4; ++k) r[k] = (int)tk_long(&t);
-            (void)r;
             if (t.bad) FAIL("truncated frame releases");
             if (j == 0) break;
-            FAIL("frame end releases are not on this path");
+            if (j < 1 || j > nfr) FAIL("frame release member out of range");
+            for (int k = 0; k < 4; ++k)
+                if (r[k] != 0 && r[k] != 1) FAIL("frame release flags are 0 or 1");
+            m->mendrel[(j - 1) * 5] = 1;
+            for (int k = 0; k < 4; ++k) m->mendrel[(j - 1) * 5 + 1 + k] = r[k];
         }
     }
 
