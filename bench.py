@@ -12,10 +12,14 @@ exact FP64 pivot judge of a path that is not the FP64 factorization itself is
 subtracted, as it would be from every variant.
 
 Paths: "fp64" = FP64 band LDL^T on the GPU (the plain FP64 GPU path, the
-baseline of the metric); "mixed" = FGMRES with FP64 residual preconditioned by
-an FP32 band LDL^T (the best non-plain path at this size).  The mixed path is
-timed over its own first --compare-steps increments beside the plain path's
-rate over the same increments, and their ratio is reported.  The learned
+baseline of the metric); "fp32" = the FP32 precision mode (the shell element
+stiffness and the band LDL^T in FP32, one refinement step per solve against
+an FP64 residual; internal forces, residual, arc-length constraint and
+convergence tests in FP64), timed over the same increments with its
+displacement error against the FP64 path; "mixed" = FGMRES with FP64 residual
+preconditioned by an FP32 band LDL^T, timed over its own first
+--compare-steps increments beside the plain path's rate over the same
+increments.  Each reports its ratio to the plain path.  The learned
 preconditioner is not run at this size: its FGMRES needs thousands of
 iterations per solve here (it is measured on held-out panels up to 32x32 by
 --workload learned).
@@ -347,7 +351,7 @@ def run_learned(args) -> None:
 
 
 def run_path(path: str, solver: int, warmup: int, steps: int, tol: float, timing: int = 0,
-             schedule: int = 0) -> dict:
+             schedule: int = 0, refine: int = 0) -> dict:
     """warmup increments of an untimed analysis, then the first `steps`
     increments of a fresh analysis, timed from its start."""
     import cubens
@@ -356,8 +360,10 @@ def run_path(path: str, solver: int, warmup: int, steps: int, tol: float, timing
         t_open = time.time() - t0
         _, _, maxa = g.dofmaps()
         kht = np.diff(maxa) - 1
-        common = dict(krylov_tol=tol, restart=30, maxit=400,
-                      judge=0 if solver == cubens.SOLVER_BAND64 else 1, factor_schedule=schedule)
+        # the FP32 precision mode decides det and psi on its own FP32 pivots
+        judge = 0 if solver in (cubens.SOLVER_BAND64, cubens.SOLVER_BAND32) else 1
+        common = dict(krylov_tol=tol, restart=30, maxit=400, judge=judge, factor_schedule=schedule,
+                      refine=refine)
         if warmup > 0:
             g.run(cubens.options(solver, max_steps=warmup, keep_d=0, **common))
         opt = cubens.options(solver, max_steps=steps, keep_d=1, kernel_timing=timing,
@@ -379,6 +385,37 @@ def run_path(path: str, solver: int, warmup: int, steps: int, tol: float, timing
             "ktimes": kt, "kht": kht, "step_t": tw - tj, "step_newton": nw}
 
 
+def path_error(ref: dict, got: dict) -> dict:
+    """How far the converged states of a path lie from the FP64 one's
+    equilibrium path.  The increments of two paths are not the same points (the
+    arc length adapts to each increment's iteration count), so the load factor
+    is compared at the same displacement of the DOF that moves most, on a
+    cubic spline through the FP64 path's states, over the range both cover
+    (before a limit point, where that displacement is monotone), with the
+    spline's own error on the FP64 states (fitted through every other state,
+    evaluated at the ones left out) for scale; and each converged
+    state's FP64 equilibrium residual |lambda q - f| / |lambda q| is given
+    beside the FP64 path's own."""
+    k = int(np.argmax(np.abs(ref["d"][-1])))
+    w_ref, l_ref = ref["d"][:, k], ref["lpf"]
+    w_got, l_got = got["d"][:, k], got["lpf"]
+    from scipy.interpolate import CubicSpline
+    order = np.argsort(w_ref)
+    ws, ls = w_ref[order], l_ref[order]
+    keep = np.concatenate([[True], np.diff(ws) > 0])
+    inside = (w_got >= ws.min()) & (w_got <= ws.max())
+    ws, ls = ws[keep], ls[keep]
+    lam_at = CubicSpline(ws, ls)(w_got[inside])
+    scale = float(np.abs(l_ref).max())
+    half = CubicSpline(ws[::2], ls[::2]) if len(ws) >= 8 else None
+    self_err = (float(np.abs(half(ws[1:-1:2]) - ls[1:-1:2]).max() / scale) if half is not None else None)
+    rel = lambda r: float(np.max(r["res"] / np.maximum(r["load"], 1e-300)))      # noqa: E731
+    return {"lpf_err_at_same_displacement": float(np.abs(l_got[inside] - lam_at).max() / scale)
+            if inside.any() else None,
+            "states_compared": int(inside.sum()), "spline_self_error": self_err,
+            "equilibrium_rel_residual": rel(got["stats"]), "equilibrium_rel_residual_fp64": rel(ref["stats"])}
+
+
 def window_rate(res: dict, k: int) -> float:
     """Newton iterations/s of a timed path over its first k increments."""
     return float(res["step_newton"][k - 1]) / float(res["step_t"][k - 1])
@@ -391,8 +428,9 @@ def main():
                     help="timed arc-length increments from the start of the analysis (the metric: 40)")
     ap.add_argument("--warmup", type=int, default=2, help="increments of an untimed warm-up analysis")
     ap.add_argument("--n", type=int, default=256, help="panel mesh (256 = the 396k-DOF workload)")
-    ap.add_argument("--paths", default="fp64,mixed",
-                    help="fp64 (the plain FP64 path, the headline) and/or mixed (FP32-factor FGMRES)")
+    ap.add_argument("--paths", default="fp64,fp32",
+                    help="fp64 (the plain FP64 path, the headline), fp32 (the FP32 precision mode) "
+                         "and/or mixed (FP32-factor FGMRES)")
     ap.add_argument("--compare-steps", type=int, default=6,
                     help="increments over which the mixed path is timed and compared with the plain one")
     ap.add_argument("--cpu-budget", type=float, default=8.0, help="seconds of partial CPU factorization")
@@ -434,11 +472,13 @@ def main():
     results = {}
     schedule = cubens.FACTOR_SPLIT if args.schedule == "split" else cubens.FACTOR_FUSED
     for name in args.paths.split(","):
-        solver = {"fp64": cubens.SOLVER_BAND64, "mixed": cubens.SOLVER_BAND32_FGMRES}[name]
-        k = args.steps if name == "fp64" else min(args.compare_steps, args.steps)
-        w = args.warmup if name == "fp64" else min(args.warmup, 1)
+        solver = {"fp64": cubens.SOLVER_BAND64, "fp32": cubens.SOLVER_BAND32,
+                  "mixed": cubens.SOLVER_BAND32_FGMRES}[name]
+        k = args.steps if name != "mixed" else min(args.compare_steps, args.steps)
+        w = args.warmup if name != "mixed" else min(args.warmup, 1)
         log(f"[bench] path {name}: {w} warm-up increments, then increments 1..{k} timed")
-        res = run_path(path, solver, w, k, args.tol, timing=127, schedule=schedule)
+        res = run_path(path, solver, w, k, args.tol, timing=127 if name == "fp64" else 0, schedule=schedule,
+                       refine=1 if name == "fp32" else 0)
         st = res["stats"]
         log(f"[bench] {name}: {res['rate']:.2f} Newton it/s ({res['newton']} its in {res['seconds']:.2f}s); "
             f"tangent {st['t_tangent']:.2f}s factor {st['t_factor']:.2f}s solve {st['t_solve']:.2f}s "
@@ -446,8 +486,21 @@ def main():
             f"fallbacks {st['krylov_fallbacks']} worst relres {st['worst_relres']:.2e}")
         results[name] = res
     plain = results.get("fp64")
-    head = plain if plain is not None else results["mixed"]
+    head = plain if plain is not None else results[next(iter(results))]
     extra = {}
+    if plain is not None and "fp32" in results:
+        m = results["fp32"]
+        km = len(m["step_t"])
+        base = window_rate(plain, km)
+        err = path_error(plain, m)
+        extra["precision"] = {
+            "path": "fp32: FP32 shell stiffness and band LDL^T, one FP64-residual refinement step per "
+                    "solve; FP64 forces, residual, arc-length constraint and convergence tests",
+            "increments": km, "value": m["rate"], "plain_fp64_same_increments": base,
+            "ratio_vs_plain_fp64": m["rate"] / base,
+            "newton": int(m["newton"]), "newton_fp64": int(plain["step_newton"][km - 1]), **err}
+        log(f"[bench] fp32 vs plain over increments 1..{km}: {m['rate']:.2f} / {base:.2f} = "
+            f"{m['rate'] / base:.3f}; {err}")
     if plain is not None and "mixed" in results:
         m = results["mixed"]
         km = len(m["step_t"])

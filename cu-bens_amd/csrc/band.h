@@ -69,11 +69,13 @@ struct BandLDLT {
     // factorization schedule: one fused launch per tile column (default), or
     // separate diag/panel/update launches
     bool split = false;
+    // the tangent comes from d.K32 (the FP32 joint blocks) instead of d.K
+    bool src32 = false;
     std::vector<double> piv_host;
 
     void setup(long n_, long bandwidth, hipStream_t s_);
     void release();
-    // Copies the lower band of d.K (equation order) and factorizes.
+    // Copies the lower band of d.K (d.K32 with src32; equation order) and factorizes.
     // 0 ok, 1 zero or non-finite pivot, 2 entry outside the band.
     int factor(const Dev& d);
     // out = (L D L^T)^{-1} rhs, both device NEQ vectors (may alias).

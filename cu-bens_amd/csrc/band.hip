@@ -29,8 +29,8 @@ namespace {
 constexpr int T = 64;
 constexpr int TT = T * T;
 
-template <class R>
-__global__ void k_band_from_bsr(R* __restrict__ tiles, const double* __restrict__ K,
+template <class R, class KS = double>
+__global__ void k_band_from_bsr(R* __restrict__ tiles, const KS* __restrict__ K,
                                 const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
                                 const int32_t* __restrict__ eq, long nj, int bs, int w, int* __restrict__ bad) {
     long r = blockIdx.x * (long)blockDim.x + threadIdx.x;   // slot row
@@ -1646,8 +1646,12 @@ template <class R>
 int BandLDLT<R>::factor(const Dev& d) {
     CB_CHECK(hipMemsetAsync(tiles, 0, sizeof(R) * size_t(nt) * (w + 1) * TT, s));
     CB_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
-    hipLaunchKernelGGL((k_band_from_bsr<R>), dim3(int((d.nslot + 255) / 256)), dim3(256), 0, s, tiles,
-                       d.K, d.rowptr, d.colidx, d.eq_of_slot, d.nj, d.bs, w, flag);
+    if (src32)
+        hipLaunchKernelGGL((k_band_from_bsr<R, float>), dim3(int((d.nslot + 255) / 256)), dim3(256), 0, s, tiles,
+                           d.K32, d.rowptr, d.colidx, d.eq_of_slot, d.nj, d.bs, w, flag);
+    else
+        hipLaunchKernelGGL((k_band_from_bsr<R>), dim3(int((d.nslot + 255) / 256)), dim3(256), 0, s, tiles,
+                           d.K, d.rowptr, d.colidx, d.eq_of_slot, d.nj, d.bs, w, flag);
     if (nt * T > n)
         hipLaunchKernelGGL((k_band_pad<R>), dim3(1), dim3(T), 0, s, tiles, n, nt, w);
     CB_CHECK(hipGetLastError());

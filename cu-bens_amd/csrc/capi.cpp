@@ -92,10 +92,13 @@ cb_options default_options() {
     o.checkpoint_path = nullptr;
     o.checkpoint_exact = 0;
     o.graph = 0;
+    o.refine = 0;
     return o;
 }
 
 cb::RunOptions to_run(const cb_options& o) {
+    if (o.solver < CB_SOLVER_BAND64 || o.solver > CB_SOLVER_BAND32)
+        throw cb::DeviceError("unknown solver " + std::to_string(o.solver));
     cb::RunOptions r;
     r.solver = o.solver;
     r.krylov_tol = o.krylov_tol;
@@ -111,6 +114,7 @@ cb::RunOptions to_run(const cb_options& o) {
     r.keep_forces = o.keep_forces;
     r.checkpoint_path = o.checkpoint_path ? o.checkpoint_path : "";
     r.checkpoint_exact = o.checkpoint_exact;
+    r.refine = o.refine;
     return r;
 }
 
@@ -120,6 +124,7 @@ bool same_opts(const cb_options& a, const cb_options& b) {
            a.kernel_timing == b.kernel_timing && a.factor_schedule == b.factor_schedule &&
            a.kernel_timing_every == b.kernel_timing_every && a.sweep_schedule == b.sweep_schedule &&
            a.keep_forces == b.keep_forces && a.checkpoint_exact == b.checkpoint_exact && a.graph == b.graph &&
+           a.refine == b.refine &&
            std::string(a.checkpoint_path ? a.checkpoint_path : "") ==
                std::string(b.checkpoint_path ? b.checkpoint_path : "");
 }
@@ -363,6 +368,17 @@ int cb_get_tangent(cb_model* h, double* vals) {
     if (!h || !vals) return fail(-1, "null argument");
     GUARD_BEGIN
     cb::d2h(vals, h->d.K, size_t(h->d.nnzb * h->d.bs * h->d.bs), h->d.s);
+    CB_CHECK(hipStreamSynchronize(h->d.s));
+    return 0;
+    GUARD_END
+}
+
+int cb_tangent_fp32(cb_model* h, int gen_x, int gen_state, float* vals) {
+    if (!h || !vals) return fail(-1, "null argument");
+    if (gen_x < 0 || gen_x > 2 || gen_state < 0 || gen_state > 2) return fail(-1, "bad generation");
+    GUARD_BEGIN
+    cb::launch_tangent_fp32(h->d, gen_x, gen_state);
+    cb::d2h(vals, h->d.K32, size_t(h->d.nnzb * h->d.bs * h->d.bs), h->d.s);
     CB_CHECK(hipStreamSynchronize(h->d.s));
     return 0;
     GUARD_END

@@ -120,6 +120,7 @@ template <class T> void d2d(T* dst, const T* src, size_t n, hipStream_t s) {
 // ---- element kernels (kernels.hip) -----------------------------------------
 // K <- tangent at generation g (coords x[gx], triads c[g], forces ef[g], ...).
 void launch_tangent(Dev& d, int gx, int g, bool want_fp32);
+void launch_tangent_fp32(Dev& d, int gx, int g);   // the tangent in K32 only
 // f += internal forces; reads ef_ip (clears its membrane entries, shell.c:1773),
 // writes ef_i; triads c_ip (previous) and c_i (current), coordinates x[gx].
 void launch_forces(Dev& d, int gx, double* f, const double* dd);

@@ -43,10 +43,13 @@ Analysis::Analysis(Model& m, Dev& d, const RunOptions& o) : m_(m), d_(d), o_(o) 
     eb = dalloc<double>(size_t(d.nslot));
     ex = dalloc<double>(size_t(d.nslot));
     if (o.keep_forces) favg = dalloc<double>(size_t(m.ntr + 7 * m.nfr + 6 * m.nsh));
+    if (o.solver == SOLVER_BAND32 && o.refine > 0) rf = dalloc<double>(size_t(n));
     h2d(q, m.q.data(), size_t(n), d.s);
-    const bool need64 = o.solver == SOLVER_BAND64 || o.judge || o.solver != SOLVER_BAND32_FGMRES;
+    const bool b32 = o.solver == SOLVER_BAND32_FGMRES || o.solver == SOLVER_BAND32;
+    const bool need64 = o.solver == SOLVER_BAND64 || o.judge || !b32;
     if (need64) b64_.setup(n, m.bandwidth, d.s);
-    if (o.solver == SOLVER_BAND32_FGMRES) b32_.setup(n, m.bandwidth, d.s);
+    if (b32) b32_.setup(n, m.bandwidth, d.s);
+    b32_.src32 = o.solver == SOLVER_BAND32;
     b64_.clock.mask = o.kernel_timing;
     b32_.clock.mask = o.kernel_timing;
     b64_.clock.every = o.kernel_timing_every;
@@ -55,12 +58,12 @@ Analysis::Analysis(Model& m, Dev& d, const RunOptions& o) : m_(m), d_(d), o_(o) 
     b32_.split = o.factor_schedule == 1;
     b64_.chained = o.sweep_schedule == 0;
     b32_.chained = o.sweep_schedule == 0;
-    if (o.solver != SOLVER_BAND64) gm_.setup(d.nslot, o.restart);
+    if (o.solver != SOLVER_BAND64 && o.solver != SOLVER_BAND32) gm_.setup(d.nslot, o.restart);
     CB_CHECK(hipStreamSynchronize(d.s));
 }
 
 Analysis::~Analysis() {
-    double* vs[] = {q, dv, dp, dpp, dt, dd, ddq, ddr, f, fp, ft, fip, r, tmp, eb, ex, favg,
+    double* vs[] = {q, dv, dp, dpp, dt, dd, ddq, ddr, f, fp, ft, fip, r, tmp, eb, ex, favg, rf,
                     sm, qtot, uc, vc, ac, vci, aci, um, vm, amv, dyn, hist, kcopy};
     for (double* p : vs)
         if (p) (void)hipFree(p);
@@ -82,7 +85,8 @@ void Analysis::record(double lam, int ite, const double* dvec) {
     step_t.push_back(now_s() - t_start_);
     step_judge.push_back(st_->t_judge);
     step_newton.push_back(st_->newton);
-    const KernelClock& kc = (o_.solver == SOLVER_BAND32_FGMRES) ? b32_.clock : b64_.clock;
+    const KernelClock& kc =
+        (o_.solver == SOLVER_BAND32_FGMRES || o_.solver == SOLVER_BAND32) ? b32_.clock : b64_.clock;
     step_kt_ms.emplace_back(kc.ms, kc.ms + KT_N);
     step_kt_n.emplace_back(kc.launches, kc.launches + KT_N);
     step_ev.push_back(ev_);
@@ -149,7 +153,7 @@ int Analysis::factor_current(int* det) {
             rc = b64_.factor(d_);
             pivots = b64_.piv_host;
             have_piv = true;
-        } else if (o_.solver == SOLVER_BAND32_FGMRES) {
+        } else if (o_.solver == SOLVER_BAND32_FGMRES || o_.solver == SOLVER_BAND32) {
             rc = b32_.factor(d_);
             pivots = b32_.piv_host;
             have_piv = true;
@@ -196,14 +200,22 @@ double Analysis::psi_now() const {
     return psi;
 }
 
+void Analysis::tangent(int gx, int g) {
+    Timer t(&st_->t_tangent, d_.s);
+    if (o_.solver == SOLVER_BAND32) {
+        launch_tangent_fp32(d_, gx, g);
+        // bars or frame members: launch_tangent_fp32 formed d.K on the way
+        if (o_.judge && !d_.ntr && !d_.nfr) launch_tangent(d_, gx, g, false);
+    } else {
+        launch_tangent(d_, gx, g, false);
+    }
+}
+
 int Analysis::tangent_and_factor(int gx, int g, int* det) {
     RunStats tmpst;
     RunStats* keep = st_;
     if (!st_) st_ = &tmpst;
-    {
-        Timer t(&st_->t_tangent, d_.s);
-        launch_tangent(d_, gx, g, false);
-    }
+    tangent(gx, g);
     int rc = factor_current(det);
     st_ = keep;
     return rc;
@@ -228,6 +240,17 @@ int Analysis::linear_solve(const double* rhs, double* x) {
     if (o_.solver == SOLVER_BAND64) {
         b64_.solve(rhs, x);
         if (learned.observe) learned.observe(d_, rhs, x);
+    } else if (o_.solver == SOLVER_BAND32) {
+        b32_.solve(rhs, x);
+        for (int k = 0; k < o_.refine; ++k) {
+            // rf = rhs - K32 x in FP64, then x += (LDL^T)^-1 rf
+            to_slots(d_, ex, x);
+            bsr_spmv32(d_, ex, eb);
+            from_slots(d_, rf, eb);
+            vaxpby(d_, rf, 1.0, rhs, -1.0, m_.neq);
+            b32_.solve(rf, rf);
+            vaxpby(d_, x, 1.0, rf, 1.0, m_.neq);
+        }
     } else {
         std::function<void(const double*, double*)> prec;
         if (o_.solver == SOLVER_BAND32_FGMRES) {
@@ -285,6 +308,8 @@ int Analysis::run(RunStats& st) {
     t_start_ = t0;
     int rc = 0;
     try {
+        if (o_.solver == SOLVER_BAND32 && m_.algflag >= 4)
+            throw DeviceError("the FP32 precision mode runs the static drivers (ALGFLAG 1-3)");
         if ((m_.algflag == 1 || m_.algflag == 2) && m_.anaflag == 1) rc = run_linear_static();
         else if (m_.algflag == 1 || m_.algflag == 2) rc = run_nr();
         else if (m_.algflag == 3) rc = run_msal();
@@ -344,10 +369,7 @@ int Analysis::run_nr() {
             vresidual(d_, r, lam, q, ft, n);
             const bool full = (m_.algflag == 1 || itecnt_ == 0);
             if (full) {
-                {
-                    Timer t(&st_->t_tangent, d_.s);
-                    launch_tangent(d_, G_I, G_IP, false);
-                }
+                tangent(G_I, G_IP);
                 // solve() discards the factor status (solve.c:86-88, 535)
                 factor_current(&det);
             }
@@ -428,10 +450,7 @@ int Analysis::run_msal() {
     };
 
     // first increment: displacement control at the arc-length DOF
-    {
-        Timer t(&st_->t_tangent, d_.s);
-        launch_tangent(d_, G_BASE, G_BASE, false);
-    }
+    tangent(G_BASE, G_BASE);
     factor_current(&det);
     piv0_ = pivots;
     linear_solve(q, ddq);
@@ -510,10 +529,7 @@ int Analysis::run_msal() {
             reset_from_base();
             errchk2 = 0;
         }
-        {
-            Timer t(&st_->t_tangent, d_.s);
-            launch_tangent(d_, G_BASE, G_BASE, false);
-        }
+        tangent(G_BASE, G_BASE);
         factor_current(&det);
         linear_solve(q, ddq);
         double qd;
@@ -940,10 +956,7 @@ int Analysis::run_linear_static() {
     double* qt = tmp;
     vcopy(d_, qt, q, n);
     vaxpby(d_, qt, 0.0, q, m_.lpfmax, n);
-    {
-        Timer t(&st_->t_tangent, d_.s);
-        launch_tangent(d_, G_BASE, G_BASE, false);
-    }
+    tangent(G_BASE, G_BASE);
     int det = 0;
     factor_current(&det);                 // solve() ignores skyfact's status (solve.c:78-89)
     linear_solve(qt, dv);

@@ -21,6 +21,12 @@ enum SolverKind {
     SOLVER_BAND32_FGMRES = 1,   // FGMRES(FP64 K) preconditioned by an FP32 band LDL^T
     SOLVER_BJ_FGMRES = 2,       // FGMRES with joint-block Jacobi
     SOLVER_LEARNED_FGMRES = 3,  // FGMRES with the learned block approximate inverse
+    // FP32 precision mode: the element tangent in FP32 (shells; bars and frame
+    // members rounded from FP64), its FP32 band LDL^T, one solve per Newton
+    // iteration; the internal forces, the residual, the arc-length constraint
+    // and the convergence tests stay FP64, so the Newton iteration itself
+    // refines each FP32 correction against the FP64 equilibrium
+    SOLVER_BAND32 = 4,
 };
 
 // Driver events of one converged step: the pivot-sign flag of the step's last
@@ -53,6 +59,7 @@ struct RunOptions {
     // writes full precision and the working lengths, areas and multiplier
     std::string checkpoint_path;
     int checkpoint_exact = 0;
+    int refine = 0;   // SOLVER_BAND32: iterative refinement steps per solve
 };
 
 struct RunStats {
@@ -101,6 +108,9 @@ public:
     PrecHook learned;
     // one-shot operations used by the C ABI
     int tangent_and_factor(int gx, int g, int* det);
+    // the tangent the solver factors: d.K, or d.K32 in the FP32 precision
+    // mode (and d.K too when the FP64 judge pivots are wanted)
+    void tangent(int gx, int g);
     int factor_only(int* det);   // factor the tangent already in d.K
     int mass(double* sm_dev);    // lumped masses at the converged coordinates
     int linear_solve(const double* rhs, double* x);   // device NEQ vectors
@@ -122,6 +132,7 @@ private:
     double *q = nullptr, *dv = nullptr, *dp = nullptr, *dpp = nullptr, *dt = nullptr, *dd = nullptr;
     double *ddq = nullptr, *ddr = nullptr, *f = nullptr, *fp = nullptr, *ft = nullptr, *fip = nullptr;
     double *r = nullptr, *eb = nullptr, *ex = nullptr, *tmp = nullptr;
+    double *rf = nullptr;   // refinement residual (SOLVER_BAND32 with refine)
     double* favg = nullptr;   // averaged element forces of one step (keep_forces)
     std::vector<double> piv0_;
     std::array<int, EV_N> ev_{};   // events since the last recorded step

@@ -35,39 +35,43 @@ __device__ inline void local_xy(const double* p1, const double* p2, const double
 
 // CST shape-function gradients / (2A): vertex 1 at the origin, 2 at (x2,0),
 // 3 at (x3,y3).
-__device__ inline void cst_grad(const double xl[3], double twoA, double gx[3], double gy[3]) {
+template <class R>
+__device__ inline void cst_grad(const R xl[3], R twoA, R gx[3], R gy[3]) {
     gx[0] = -(xl[2] / twoA);
     gx[1] = xl[2] / twoA;
-    gx[2] = 0.0;
+    gx[2] = R(0);
     gy[0] = (xl[1] - xl[0]) / twoA;
     gy[1] = -(xl[1] / twoA);
     gy[2] = xl[0] / twoA;
 }
 
+// R: the arithmetic of the element stiffness (double, or float in the
+// FP32 precision mode)
+template <class R>
 struct ShellLocal {
-    double km[6][6];   // membrane over (u1,v1,u2,v2,u3,v3)
-    double kb[9][9];   // bending over (w,tx,ty) per vertex
+    R km[6][6];   // membrane over (u1,v1,u2,v2,u3,v3)
+    R kb[9][9];   // bending over (w,tx,ty) per vertex
 };
 
 // Plane-stress membrane stiffness t A Bm^T C Bm (shell.c:487-531).
-__device__ inline void shell_membrane(double E, double nu, double t, const double xl[3], double A,
-                                      double km[6][6]) {
-    double gx[3], gy[3];
-    cst_grad(xl, 2 * A, gx, gy);
-    double c0 = E / (1 - nu * nu);
-    double C00 = c0, C01 = c0 * nu, C22 = c0 * (1 - nu) / 2;
+template <class R>
+__device__ inline void shell_membrane(R E, R nu, R t, const R xl[3], R A, R km[6][6]) {
+    R gx[3], gy[3];
+    cst_grad<R>(xl, 2 * A, gx, gy);
+    R c0 = E / (1 - nu * nu);
+    R C00 = c0, C01 = c0 * nu, C22 = c0 * (1 - nu) / 2;
     // columns of Bm: u_a -> (gx_a, 0, gy_a), v_a -> (0, gy_a, gx_a)
-    double B[3][6];
+    R B[3][6];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        B[0][2 * a] = gx[a]; B[1][2 * a] = 0.0; B[2][2 * a] = gy[a];
-        B[0][2 * a + 1] = 0.0; B[1][2 * a + 1] = gy[a]; B[2][2 * a + 1] = gx[a];
+        B[0][2 * a] = gx[a]; B[1][2 * a] = R(0); B[2][2 * a] = gy[a];
+        B[0][2 * a + 1] = R(0); B[1][2 * a + 1] = gy[a]; B[2][2 * a + 1] = gx[a];
     }
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
-        double bc0 = B[0][i] * C00 + B[1][i] * C01;
-        double bc1 = B[0][i] * C01 + B[1][i] * C00;
-        double bc2 = B[2][i] * C22;
+        R bc0 = B[0][i] * C00 + B[1][i] * C01;
+        R bc1 = B[0][i] * C01 + B[1][i] * C00;
+        R bc2 = B[2][i] * C22;
 #pragma unroll
         for (int j = 0; j < 6; ++j) km[i][j] = t * A * (bc0 * B[0][j] + bc1 * B[1][j] + bc2 * B[2][j]);
     }
@@ -75,56 +79,56 @@ __device__ inline void shell_membrane(double E, double nu, double t, const doubl
 
 // DKT bending stiffness by exact mid-side quadrature of the Batoz-Bathe-Ho
 // curvature field; squared side lengths from the 3-D side lengths sl.
-__device__ inline void shell_bending(double E, double nu, double t, const double xl[3], double A,
-                                     const double sl[3], double kb[9][9]) {
-    const double x2 = xl[0], x3 = xl[1], y3 = xl[2];
-    const double xs[3] = {x2 - x3, x3, -x2};
-    const double ys[3] = {-y3, y3, 0.0};
-    const double l2[3] = {sl[1] * sl[1], sl[2] * sl[2], sl[0] * sl[0]};
-    double a[3], b[3], c[3], dd[3], e[3];
+template <class R>
+__device__ inline void shell_bending(R E, R nu, R t, const R xl[3], R A, const R sl[3], R kb[9][9]) {
+    const R x2 = xl[0], x3 = xl[1], y3 = xl[2];
+    const R xs[3] = {x2 - x3, x3, -x2};
+    const R ys[3] = {-y3, y3, R(0)};
+    const R l2[3] = {sl[1] * sl[1], sl[2] * sl[2], sl[0] * sl[0]};
+    R a[3], b[3], c[3], dd[3], e[3];
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
         a[s] = -xs[s] / l2[s];
-        b[s] = 0.75 * xs[s] * ys[s] / l2[s];
-        c[s] = (0.25 * xs[s] * xs[s] - 0.5 * ys[s] * ys[s]) / l2[s];
+        b[s] = R(0.75) * xs[s] * ys[s] / l2[s];
+        c[s] = (R(0.25) * xs[s] * xs[s] - R(0.5) * ys[s] * ys[s]) / l2[s];
         dd[s] = -ys[s] / l2[s];
-        e[s] = (0.25 * ys[s] * ys[s] - 0.5 * xs[s] * xs[s]) / l2[s];
+        e[s] = (R(0.25) * ys[s] * ys[s] - R(0.5) * xs[s] * xs[s]) / l2[s];
     }
-    const double D = E * t * t * t / (12 * (1 - nu * nu));
-    const double D01 = D * nu, D22 = D * (1 - nu) / 2;
-    const double w = 1.0 / (6.0 * 2 * A);
+    const R D = E * t * t * t / (12 * (1 - nu * nu));
+    const R D01 = D * nu, D22 = D * (1 - nu) / 2;
+    const R w = R(1.0) / (R(6.0) * 2 * A);
 #pragma unroll
     for (int i = 0; i < 9; ++i)
 #pragma unroll
-        for (int j = 0; j < 9; ++j) kb[i][j] = 0.0;
+        for (int j = 0; j < 9; ++j) kb[i][j] = R(0);
 #pragma unroll
     for (int g = 0; g < 3; ++g) {
-        const double xi = (g == 2) ? 0.0 : 0.5, eta = (g == 0) ? 0.0 : 0.5, L = 1 - xi - eta;
-        double n1[2] = {-(4 * L - 1), -(4 * L - 1)}, n2[2] = {4 * xi - 1, 0.0}, n3[2] = {0.0, 4 * eta - 1};
-        double n4[2] = {4 * eta, 4 * xi}, n5[2] = {-4 * eta, 4 * L - 4 * eta}, n6[2] = {4 * L - 4 * xi, -4 * xi};
-        double Hx[9][2], Hy[9][2];
+        const R xi = (g == 2) ? R(0) : R(0.5), eta = (g == 0) ? R(0) : R(0.5), L = 1 - xi - eta;
+        R n1[2] = {-(4 * L - 1), -(4 * L - 1)}, n2[2] = {4 * xi - 1, R(0)}, n3[2] = {R(0), 4 * eta - 1};
+        R n4[2] = {4 * eta, 4 * xi}, n5[2] = {-4 * eta, 4 * L - 4 * eta}, n6[2] = {4 * L - 4 * xi, -4 * xi};
+        R Hx[9][2], Hy[9][2];
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
-            Hx[0][r] = 1.5 * (a[2] * n6[r] - a[1] * n5[r]);
+            Hx[0][r] = R(1.5) * (a[2] * n6[r] - a[1] * n5[r]);
             Hx[1][r] = b[1] * n5[r] + b[2] * n6[r];
             Hx[2][r] = n1[r] - c[1] * n5[r] - c[2] * n6[r];
-            Hx[3][r] = 1.5 * (a[0] * n4[r] - a[2] * n6[r]);
+            Hx[3][r] = R(1.5) * (a[0] * n4[r] - a[2] * n6[r]);
             Hx[4][r] = b[2] * n6[r] + b[0] * n4[r];
             Hx[5][r] = n2[r] - c[2] * n6[r] - c[0] * n4[r];
-            Hx[6][r] = 1.5 * (a[1] * n5[r] - a[0] * n4[r]);
+            Hx[6][r] = R(1.5) * (a[1] * n5[r] - a[0] * n4[r]);
             Hx[7][r] = b[0] * n4[r] + b[1] * n5[r];
             Hx[8][r] = n3[r] - c[0] * n4[r] - c[1] * n5[r];
-            Hy[0][r] = 1.5 * (dd[2] * n6[r] - dd[1] * n5[r]);
+            Hy[0][r] = R(1.5) * (dd[2] * n6[r] - dd[1] * n5[r]);
             Hy[1][r] = -n1[r] + e[1] * n5[r] + e[2] * n6[r];
             Hy[2][r] = -Hx[1][r];
-            Hy[3][r] = 1.5 * (dd[0] * n4[r] - dd[2] * n6[r]);
+            Hy[3][r] = R(1.5) * (dd[0] * n4[r] - dd[2] * n6[r]);
             Hy[4][r] = -n2[r] + e[2] * n6[r] + e[0] * n4[r];
             Hy[5][r] = -Hx[4][r];
-            Hy[6][r] = 1.5 * (dd[1] * n5[r] - dd[0] * n4[r]);
+            Hy[6][r] = R(1.5) * (dd[1] * n5[r] - dd[0] * n4[r]);
             Hy[7][r] = -n3[r] + e[0] * n4[r] + e[1] * n5[r];
             Hy[8][r] = -Hx[7][r];
         }
-        double B0[9], B1[9], B2[9];
+        R B0[9], B1[9], B2[9];
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             B0[i] = y3 * Hx[i][0];
@@ -133,7 +137,7 @@ __device__ inline void shell_bending(double E, double nu, double t, const double
         }
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
-            double d0 = D * B0[i] + D01 * B1[i], d1 = D01 * B0[i] + D * B1[i], d2 = D22 * B2[i];
+            R d0 = D * B0[i] + D01 * B1[i], d1 = D01 * B0[i] + D * B1[i], d2 = D22 * B2[i];
 #pragma unroll
             for (int j = i; j < 9; ++j) kb[i][j] += w * (d0 * B0[j] + d1 * B1[j] + d2 * B2[j]);
         }
@@ -145,9 +149,10 @@ __device__ inline void shell_bending(double E, double nu, double t, const double
 }
 
 // Entry (r,s) of the local 6x6 block for vertices (a,b): slots (u,v,w,tx,ty,tz).
-__device__ inline double local_entry(const ShellLocal& L, const double g[3][3], const double drill[3],
-                                     int a, int b, int r, int s) {
-    double v = 0.0;
+template <class R>
+__device__ inline R local_entry(const ShellLocal<R>& L, const R g[3][3], const R drill[3], int a, int b, int r,
+                                int s) {
+    R v = R(0);
     if (r < 2 && s < 2) v += L.km[2 * a + r][2 * b + s];
     if (r >= 2 && r < 5 && s >= 2 && s < 5) v += L.kb[3 * a + r - 2][3 * b + s - 2];
     if (r == 5 && s == 5 && a == b) v += drill[a];
@@ -156,25 +161,31 @@ __device__ inline double local_entry(const ShellLocal& L, const double g[3][3], 
 }
 
 template <class T>
-__device__ inline void add_block(T* Kb, const double Kab[6][6], bool transpose, bool upper_only) {
+__device__ inline void add_block(T* Kb, const T Kab[6][6], bool transpose, bool upper_only) {
 #pragma unroll
     for (int p = 0; p < 6; ++p)
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            double v;
+            T v;
             if (upper_only) v = (p <= q) ? Kab[p][q] : Kab[q][p];
             else v = transpose ? Kab[q][p] : Kab[p][q];
             Kb[p * 6 + q] += T(v);
         }
 }
 
+// stiff_sh (shell.c:110-840) for the shells of one colour, with the
+// element stiffness in the arithmetic T: double, or float for the FP32
+// precision mode.  The geometry stays FP64 in both: the deformed local
+// coordinates, the strains of the deformed area and the membrane forces are
+// formed in double and rounded once to T.
+template <class T>
 __global__ void __launch_bounds__(128) k_shell_tangent(
     const int32_t* __restrict__ els, int n, const int32_t* __restrict__ minc,
     const int32_t* __restrict__ blk, const double* __restrict__ x, const double* __restrict__ c1,
     const double* __restrict__ c2, const double* __restrict__ c3, const double* __restrict__ emod,
     const double* __restrict__ nu, const double* __restrict__ thick, const double* __restrict__ xlocal,
     const double* __restrict__ farea, const double* __restrict__ slength,
-    const double* __restrict__ deffarea, double* __restrict__ K, long ioff, long toff, long poff,
+    const double* __restrict__ deffarea, T* __restrict__ K, long ioff, long toff, long poff,
     int geo) {
     int it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= n) return;
@@ -182,57 +193,67 @@ __global__ void __launch_bounds__(128) k_shell_tangent(
     const double E = emod[poff + e], v = nu[e], t = thick[e], A0 = farea[e], Aip = deffarea[e];
     const double xl[3] = {xlocal[e * 3], xlocal[e * 3 + 1], xlocal[e * 3 + 2]};
     const double sl[3] = {slength[e * 3], slength[e * 3 + 1], slength[e * 3 + 2]};
-    double R[3][3];
+    double Rd[3][3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        R[0][k] = c1[toff + e * 3 + k];
-        R[1][k] = c2[toff + e * 3 + k];
-        R[2][k] = c3[toff + e * 3 + k];
+        Rd[0][k] = c1[toff + e * 3 + k];
+        Rd[1][k] = c2[toff + e * 3 + k];
+        Rd[2][k] = c3[toff + e * 3 + k];
     }
     const int32_t v0 = minc[ioff + e * 3] - 1, v1 = minc[ioff + e * 3 + 1] - 1, v2 = minc[ioff + e * 3 + 2] - 1;
     double P[3][3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) { P[0][k] = x[v0 * 3 + k]; P[1][k] = x[v1 * 3 + k]; P[2][k] = x[v2 * 3 + k]; }
     double xc[3];
-    local_xy(P[0], P[1], P[2], R[0], R[1], xc);
+    local_xy(P[0], P[1], P[2], Rd[0], Rd[1], xc);
     const double du2 = xc[0] - xl[0], du3 = xc[1] - xl[1], dv3 = xc[2] - xl[2];
 
-    ShellLocal L;
-    shell_membrane(E, v, t, xl, A0, L.km);
-    shell_bending(E, v, t, xl, A0, sl, L.kb);
-    const double drill[3] = {L.kb[1][1] / 10000, L.kb[4][4] / 10000, L.kb[7][7] / 10000};
+    const T Et = T(E), vt = T(v), tt = T(t), A0t = T(A0);
+    const T xlt[3] = {T(xl[0]), T(xl[1]), T(xl[2])};
+    const T slt[3] = {T(sl[0]), T(sl[1]), T(sl[2])};
+    ShellLocal<T> L;
+    shell_membrane<T>(Et, vt, tt, xlt, A0t, L.km);
+    shell_bending<T>(Et, vt, tt, xlt, A0t, slt, L.kb);
+    const T drill[3] = {L.kb[1][1] / 10000, L.kb[4][4] / 10000, L.kb[7][7] / 10000};
     // membrane force of the deformed-area strain (stiffg_sh, shell.c:660-840)
-    double gx[3], gy[3];
-    cst_grad(xl, 2 * Aip, gx, gy);
+    double gxd[3], gyd[3];
+    cst_grad<double>(xl, 2 * Aip, gxd, gyd);
     const double c0 = E / (1 - v * v);
-    const double ex = gx[1] * du2 + gx[2] * du3;
-    const double ey = gy[2] * dv3;
-    const double gxy = gy[1] * du2 + gy[2] * du3 + gx[2] * dv3;
-    const double Nx = t * (c0 * ex + c0 * v * ey), Ny = t * (c0 * v * ex + c0 * ey);
-    const double Nxy = t * (c0 * (1 - v) / 2 * gxy);
+    const double ex = gxd[1] * du2 + gxd[2] * du3;
+    const double ey = gyd[2] * dv3;
+    const double gxy = gyd[1] * du2 + gyd[2] * du3 + gxd[2] * dv3;
+    const T Nx = T(t * (c0 * ex + c0 * v * ey)), Ny = T(t * (c0 * v * ex + c0 * ey));
+    const T Nxy = T(t * (c0 * (1 - v) / 2 * gxy));
+    const T gx[3] = {T(gxd[0]), T(gxd[1]), T(gxd[2])}, gy[3] = {T(gyd[0]), T(gyd[1]), T(gyd[2])};
+    const T Aipt = T(Aip);
     // linear elastic analysis (ANAFLAG 1) has no geometric term (shell.c:150-153)
-    double g[3][3];
+    T g[3][3];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-            double r0 = gx[a] * Nx + gy[a] * Nxy, r1 = gx[a] * Nxy + gy[a] * Ny;
-            g[a][b] = geo ? Aip * (r0 * gx[b] + r1 * gy[b]) : 0.0;
+            T r0 = gx[a] * Nx + gy[a] * Nxy, r1 = gx[a] * Nxy + gy[a] * Ny;
+            g[a][b] = geo ? Aipt * (r0 * gx[b] + r1 * gy[b]) : T(0);
         }
+    T R[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) R[r][c] = T(Rd[r][c]);
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int b = a; b < 3; ++b) {
             // K_ab = blockdiag(R,R)^T k_ab blockdiag(R,R)
-            double tmp[6][6], Kab[6][6];
+            T tmp[6][6], Kab[6][6];
 #pragma unroll
             for (int p = 0; p < 6; ++p)
 #pragma unroll
                 for (int s = 0; s < 6; ++s) {
                     int bp = p / 3 * 3, cp = p % 3;
-                    tmp[p][s] = R[0][cp] * local_entry(L, g, drill, a, b, bp, s) +
-                                R[1][cp] * local_entry(L, g, drill, a, b, bp + 1, s) +
-                                R[2][cp] * local_entry(L, g, drill, a, b, bp + 2, s);
+                    tmp[p][s] = R[0][cp] * local_entry<T>(L, g, drill, a, b, bp, s) +
+                                R[1][cp] * local_entry<T>(L, g, drill, a, b, bp + 1, s) +
+                                R[2][cp] * local_entry<T>(L, g, drill, a, b, bp + 2, s);
                 }
 #pragma unroll
             for (int p = 0; p < 6; ++p)
@@ -242,10 +263,10 @@ __global__ void __launch_bounds__(128) k_shell_tangent(
                     Kab[p][q] = tmp[p][bq] * R[0][cq] + tmp[p][bq + 1] * R[1][cq] + tmp[p][bq + 2] * R[2][cq];
                 }
             if (a == b) {
-                add_block(K + (long)blk[e * 9 + a * 3 + a] * 36, Kab, false, true);
+                add_block<T>(K + (long)blk[e * 9 + a * 3 + a] * 36, Kab, false, true);
             } else {
-                add_block(K + (long)blk[e * 9 + a * 3 + b] * 36, Kab, false, false);
-                add_block(K + (long)blk[e * 9 + b * 3 + a] * 36, Kab, true, false);
+                add_block<T>(K + (long)blk[e * 9 + a * 3 + b] * 36, Kab, false, false);
+                add_block<T>(K + (long)blk[e * 9 + b * 3 + a] * 36, Kab, true, false);
             }
         }
 }
@@ -501,7 +522,7 @@ __global__ void __launch_bounds__(128) k_shell_forces_lin(
 #pragma unroll
         for (int r = 0; r < 3; ++r) dl[3 * b + r] = Rp[r][0] * Db[0] + Rp[r][1] * Db[1] + Rp[r][2] * Db[2];
     }
-    ShellLocal L;
+    ShellLocal<double> L;
     shell_membrane(E, v, t, xl, A0, L.km);
     shell_bending(E, v, t, xl, A0, sl, L.kb);
     const double drill[3] = {L.kb[1][1] / 10000, L.kb[4][4] / 10000, L.kb[7][7] / 10000};
@@ -742,7 +763,7 @@ void launch_tangent(Dev& d, int gx, int g, bool want_fp32) {
     for (size_t c = 0; c + 1 < d.col_ptr_sh.size(); ++c) {
         int n = d.col_ptr_sh[c + 1] - d.col_ptr_sh[c];
         if (!n) continue;
-        hipLaunchKernelGGL(k_shell_tangent, dim3(nblk(n, 128)), dim3(128), 0, d.s,
+        hipLaunchKernelGGL(k_shell_tangent<double>, dim3(nblk(n, 128)), dim3(128), 0, d.s,
                            d.col_el_sh + d.col_ptr_sh[c], n, d.minc, d.blk_sh, d.x[gx], d.c1[g],
                            d.c2[g], d.c3[g], d.emod, d.nu, d.thick, d.xlocal, d.farea, d.slength,
                            d.deffarea[g], d.K, io, to, po, int(d.anaflag != 1));
@@ -758,6 +779,31 @@ void launch_tangent(Dev& d, int gx, int g, bool want_fp32) {
         hipLaunchKernelGGL(k_cast32, dim3(nblk(nb)), dim3(TPB), 0, d.s, d.K32, d.K, nb);
         CB_CHECK(hipGetLastError());
     }
+}
+
+// The tangent in d.K32 for the FP32 precision mode: shell-only models take
+// the element stiffness in FP32 (k_shell_tangent<float>) straight into the
+// FP32 joint blocks; models with bars or frame members form the FP64 tangent
+// and round it once.
+void launch_tangent_fp32(Dev& d, int gx, int g) {
+    if (d.ntr || d.nfr) {
+        launch_tangent(d, gx, g, true);
+        return;
+    }
+    const long nb = d.nnzb * d.bs * d.bs;
+    CB_CHECK(hipMemsetAsync(d.K32, 0, sizeof(float) * nb, d.s));
+    const long to = d.ntr + d.nfr * 3, io = d.ntr * 2 + d.nfr * 2, po = d.ntr + d.nfr;
+    for (size_t c = 0; c + 1 < d.col_ptr_sh.size(); ++c) {
+        int n = d.col_ptr_sh[c + 1] - d.col_ptr_sh[c];
+        if (!n) continue;
+        hipLaunchKernelGGL(k_shell_tangent<float>, dim3(nblk(n, 128)), dim3(128), 0, d.s,
+                           d.col_el_sh + d.col_ptr_sh[c], n, d.minc, d.blk_sh, d.x[gx], d.c1[g],
+                           d.c2[g], d.c3[g], d.emod, d.nu, d.thick, d.xlocal, d.farea, d.slength,
+                           d.deffarea[g], d.K32, io, to, po, int(d.anaflag != 1));
+    }
+    hipLaunchKernelGGL((k_fix_slots<float>), dim3(nblk(d.nslot)), dim3(TPB), 0, d.s, d.K32, d.rowptr, d.colidx,
+                       d.eq_of_slot, d.nj, d.bs);
+    CB_CHECK(hipGetLastError());
 }
 
 void launch_forces(Dev& d, int gx, double* f, const double* dd) {

@@ -18,6 +18,7 @@ SOLVER_BAND64 = 0
 SOLVER_BAND32_FGMRES = 1
 SOLVER_BJ_FGMRES = 2
 SOLVER_LEARNED_FGMRES = 3
+SOLVER_BAND32 = 4
 
 GEN_BASE, GEN_I, GEN_IP = 0, 1, 2
 
@@ -34,7 +35,8 @@ class Options(C.Structure):
                 ("maxit", C.c_int), ("judge", C.c_int), ("max_steps", C.c_int),
                 ("keep_d", C.c_int), ("kernel_timing", C.c_int), ("factor_schedule", C.c_int),
                 ("kernel_timing_every", C.c_int), ("sweep_schedule", C.c_int), ("keep_forces", C.c_int),
-                ("checkpoint_path", C.c_char_p), ("checkpoint_exact", C.c_int), ("graph", C.c_int)]
+                ("checkpoint_path", C.c_char_p), ("checkpoint_exact", C.c_int), ("graph", C.c_int),
+                ("refine", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -64,12 +66,13 @@ def options(solver: int = SOLVER_BAND64, krylov_tol: float = 1e-8, restart: int 
             maxit: int = 20000, judge: int = 1, max_steps: int = 1 << 30, keep_d: int = 1,
             kernel_timing: int = 0, factor_schedule: int = 0, kernel_timing_every: int = 1,
             sweep_schedule: int = 0, keep_forces: int = 0, checkpoint_path: str | None = None,
-            checkpoint_exact: int = 0, graph: int = 0) -> Options:
+            checkpoint_exact: int = 0, graph: int = 0, refine: int = 0) -> Options:
     """cb_options; checkpoint_path is the Newmark checkpoint file (written every
-    CHKPT time steps, read first when the model has RFLAG 1)."""
+    CHKPT time steps, read first when the model has RFLAG 1); refine the
+    iterative refinement steps of the FP32 precision mode's solves."""
     return Options(solver, krylov_tol, restart, maxit, judge, max_steps, keep_d, kernel_timing,
                    factor_schedule, kernel_timing_every, sweep_schedule, keep_forces,
-                   checkpoint_path.encode() if checkpoint_path else None, checkpoint_exact, graph)
+                   checkpoint_path.encode() if checkpoint_path else None, checkpoint_exact, graph, refine)
 
 
 _lib = None
@@ -111,6 +114,7 @@ ABI = {
     "cb_state_get": (C.c_int, [C.c_void_p, C.c_char_p, PD, C.c_long]),
     "cb_tangent": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "cb_get_tangent": (C.c_int, [C.c_void_p, PD]),
+    "cb_tangent_fp32": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float)]),
     "cb_internal_force": (C.c_int, [C.c_void_p, C.c_int, PD, PD]),
     "cb_get_yield_flags": (C.c_int, [C.c_void_p, PI, PD]),
     "cb_set_yield_flags": (C.c_int, [C.c_void_p, PI]),
@@ -273,6 +277,12 @@ class GpuModel:
         _check(lib().cb_tangent(self.h, gen_x, gen_state))
         v = np.zeros(self.info.nnz_blocks * self.info.bs * self.info.bs)
         _check(lib().cb_get_tangent(self.h, _p(v)))
+        return v
+
+    def tangent_fp32(self, gen_x: int = GEN_BASE, gen_state: int = GEN_BASE) -> np.ndarray:
+        """The FP32 precision mode's tangent (cb_tangent_fp32), float32 joint blocks."""
+        v = np.zeros(self.info.nnz_blocks * self.info.bs * self.info.bs, np.float32)
+        _check(lib().cb_tangent_fp32(self.h, gen_x, gen_state, v.ctypes.data_as(C.POINTER(C.c_float))))
         return v
 
     def internal_force(self, dd: np.ndarray, gen_x: int = GEN_I, with_code: bool = False):

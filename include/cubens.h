@@ -39,7 +39,11 @@ enum {
     CB_SOLVER_BAND64 = 0,         /* FP64 band LDL^T on the GPU                  */
     CB_SOLVER_BAND32_FGMRES = 1,  /* FGMRES, FP32 band LDL^T preconditioner      */
     CB_SOLVER_BJ_FGMRES = 2,      /* FGMRES, 6x6 joint-block Jacobi              */
-    CB_SOLVER_LEARNED_FGMRES = 3  /* FGMRES, learned block approximate inverse   */
+    CB_SOLVER_LEARNED_FGMRES = 3, /* FGMRES, learned block approximate inverse   */
+    CB_SOLVER_BAND32 = 4          /* FP32 precision mode: FP32 element tangent and
+                                     band LDL^T, one solve per Newton iteration;
+                                     forces, residual, arc-length constraint and
+                                     convergence tests in FP64 (static drivers) */
 };
 
 typedef struct {
@@ -69,6 +73,10 @@ typedef struct {
      * captured from the first round's launches (cb_batch_run without
      * kernel_timing); 0 issues the launches one by one */
     int graph;
+    /* CB_SOLVER_BAND32: iterative refinement steps per solve, each an FP64
+     * residual of the FP32 tangent (rhs - K32 x, FP64 accumulation) solved
+     * again with the FP32 factor and added; 0: the FP32 solve alone */
+    int refine;
 } cb_options;
 
 /* Schedules of the band LDL^T factorization (same tile arithmetic, bitwise
@@ -146,6 +154,10 @@ int cb_tangent(cb_model *m, int gen_x, int gen_state);
 /* Copies the assembled joint-block values [nnz_blocks*bs*bs] (rows/columns of
  * fixed DOFs carry identity). */
 int cb_get_tangent(cb_model *m, double *vals);
+/* The same tangent as the FP32 precision mode (CB_SOLVER_BAND32) forms it:
+ * stiff_sh (shell.c:110-840) with the element stiffness in FP32 for shell-only
+ * models, the FP64 tangent rounded once otherwise; vals [nnz_blocks*bs*bs]. */
+int cb_tangent_fp32(cb_model *m, int gen_x, int gen_state, float *vals);
 
 /* Internal force: replaces forces_tr/forces_fr/forces_sh (truss.c:231,
  * frame.c:902, shell.c:1593).
