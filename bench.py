@@ -238,8 +238,13 @@ def run_frames(args) -> None:
     neq = sum(i.neq for i in b.info)
     nfr = sum(i.ne_fr for i in b.info)
     log(f"[bench] frames: {b.count} models, {nfr} members, NEQ {neq}, opened in {t_open:.1f}s")
-    if args.warmup > 0:
-        b.run(max_steps=args.warmup, keep_d=False)
+    # the warm-up run tunes the band factor's helper pairs on the initial
+    # tangent; the timed runs use the count it keeps
+    wst = b.run(max_steps=max(args.warmup, 1), keep_d=False, autotune=True)
+    tuning = {"pairs_kept": int(wst["pairs"]), "pairs_max": int(wst["pairs_max"]),
+              "trials_ms": [[p, round(ms, 3)] for p, ms in b.tuning()], "seconds": round(wst["tune_s"], 3)}
+    log(f"[bench] frames: tuner kept {tuning['pairs_kept']} of {tuning['pairs_max']} helper pairs; "
+        f"trials (pairs, factor ms) {tuning['trials_ms']} in {tuning['seconds']}s")
     k = min(args.steps, steps_total)
     # the same rounds three ways: launches one by one, each round's Newton
     # iteration replayed as one HIP graph (the timed value), and one by one
@@ -290,7 +295,8 @@ def run_frames(args) -> None:
                    "rounds": g["rounds"],
                    "launches": {"ops_per_round_graph": g["ops_per_round"], "ops_per_round_one_by_one":
                                 e["ops_per_round"], "iteration_ops": g["body_ops"],
-                                "newton_per_s_one_by_one": rate_e}},
+                                "newton_per_s_one_by_one": rate_e},
+                   "autotune": tuning},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

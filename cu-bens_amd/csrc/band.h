@@ -1,6 +1,7 @@
 // Device band LDL^T of the tangent in the reference's equation order.
 #pragma once
 
+#include <utility>
 #include <vector>
 
 #include "dev.h"
@@ -113,6 +114,11 @@ struct BandBatch {
     int* sorder = nullptr;          // solve workgroup -> model, most work first
     int* prog = nullptr;            // progress counters of the paired models
     int nwg = 0;                    // factor workgroups
+    // models in decreasing order of their tile work; the helper pairs the
+    // factor may use (models with two tiles below the diagonal, as many as
+    // keep main and helper resident together) and the pairs it uses
+    std::vector<int> work_order;
+    int max_pairs = 0, npair = 0;
     int32_t* gaps = nullptr;        // equations that no DOF slot maps to
     long ngap = 0;
     std::vector<BandSeg> segs_host;
@@ -135,6 +141,15 @@ struct BandBatch {
     void zero();
     void factor_filled(bool fetch_flag = true);
     void fetch_flag();
+    // The factor workgroups with np helper pairs (clamped to max_pairs): the
+    // pairs first, then the other models, each part in decreasing tile work.
+    // Every model's factor is the same whatever np is.
+    void set_pairs(int np);
+    // Times the factor of the band as it stands (filled, not yet factored) for
+    // each candidate pair count, restoring the band from a copy before each
+    // trial and taking the better of two runs, and keeps the fastest; returns
+    // the (pairs, ms) of every trial (ms < 0: a wait bound was hit).
+    std::vector<std::pair<int, double>> tune_pairs(const std::vector<int>& cand);
     // out = (L D L^T)^{-1} rhs for every model (device vectors of n, no alias).
     void solve(const double* rhs, double* out);
     size_t bytes() const { return sizeof(double) * size_t(nt) * size_t(w + 1) * 64 * 64; }

@@ -1775,30 +1775,15 @@ void BandBatch::setup(const std::vector<BandSeg>& sg, long n_total, int w_band, 
     for (size_t i = 0; i < mo.size(); ++i) mo[i] = int(i);
     auto work = [&](int i) { return double(sg[i].nt) * double((sg[i].w + 1) * (sg[i].w + 2) / 2); };
     std::stable_sort(mo.begin(), mo.end(), [&](int a, int b) { return work(a) > work(b); });
-    std::vector<int> ord, rol;
-    std::vector<char> paired(sg.size(), 0);
+    work_order = mo;
     // the factor kernel's LDS allows one workgroup per CU: the pairs stay
     // within the first CUs - 32 workgroups of the grid
     int dev = 0, ncu = 0;
     CB_CHECK(hipGetDevice(&dev));
     CB_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const int max_pairs = std::max(0, std::min(96, (ncu - 32) / 2));
-    int npair = 0;
-    for (int i : mo)
-        if (sg[size_t(i)].w == 2 && sg[size_t(i)].nt >= 3 && npair < max_pairs) {
-            ord.push_back(i);
-            rol.push_back(ROLE_MAIN);
-            ord.push_back(i);
-            rol.push_back(ROLE_HELPER);
-            paired[size_t(i)] = 1;
-            ++npair;
-        }
-    for (int i : mo)
-        if (!paired[size_t(i)]) {
-            ord.push_back(i);
-            rol.push_back(ROLE_SINGLE);
-        }
-    nwg = int(ord.size());
+    int eligible = 0;
+    for (const BandSeg& g : sg) eligible += (g.w == 2 && g.nt >= 3);
+    max_pairs = std::max(0, std::min({96, (ncu - 32) / 2, eligible}));
     tiles = dalloc<double>(size_t(nt) * size_t(w + 1) * TT);
     linv = dalloc<double>(size_t(nt) * TT);
     piv = dalloc<double>(size_t(nt) * T);
@@ -1807,15 +1792,14 @@ void BandBatch::setup(const std::vector<BandSeg>& sg, long n_total, int w_band, 
     CB_CHECK(hipHostMalloc(&flag_host, sizeof(int), hipHostMallocDefault));
     *flag_host = 0;
     segs = dalloc<BandSeg>(sg.size());
-    order = dalloc<int>(ord.size());
-    role = dalloc<int>(rol.size());
+    order = dalloc<int>(sg.size() + size_t(max_pairs));
+    role = dalloc<int>(sg.size() + size_t(max_pairs));
     sorder = dalloc<int>(mo.size());
     prog = dalloc<int>(sg.size() * 4 * FLAG_STRIDE);
     ngap = long(gap_eqs.size());
     gaps = dalloc<int32_t>(size_t(ngap));
     h2d(segs, sg.data(), sg.size(), s);
-    h2d(order, ord.data(), ord.size(), s);
-    h2d(role, rol.data(), rol.size(), s);
+    set_pairs(max_pairs);
     h2d(sorder, mo.data(), mo.size(), s);
     h2d(gaps, gap_eqs.data(), gap_eqs.size(), s);
     CB_CHECK(hipMemsetAsync(piv, 0, sizeof(double) * size_t(nt) * T, s));
@@ -1876,6 +1860,77 @@ void BandBatch::factor_filled(bool fetch_flag) {
 
 void BandBatch::fetch_flag() {
     CB_CHECK(hipMemcpyAsync(flag_host, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+}
+
+void BandBatch::set_pairs(int np) {
+    np = std::max(0, std::min(np, max_pairs));
+    std::vector<int> ord, rol;
+    std::vector<char> paired(segs_host.size(), 0);
+    npair = 0;
+    for (int i : work_order)
+        if (segs_host[size_t(i)].w == 2 && segs_host[size_t(i)].nt >= 3 && npair < np) {
+            ord.push_back(i);
+            rol.push_back(ROLE_MAIN);
+            ord.push_back(i);
+            rol.push_back(ROLE_HELPER);
+            paired[size_t(i)] = 1;
+            ++npair;
+        }
+    for (int i : work_order)
+        if (!paired[size_t(i)]) {
+            ord.push_back(i);
+            rol.push_back(ROLE_SINGLE);
+        }
+    nwg = int(ord.size());
+    h2d(order, ord.data(), ord.size(), s);
+    h2d(role, rol.data(), rol.size(), s);
+    CB_CHECK(hipStreamSynchronize(s));
+}
+
+std::vector<std::pair<int, double>> BandBatch::tune_pairs(const std::vector<int>& cand) {
+    std::vector<std::pair<int, double>> out;
+    if (!nseg || cand.empty()) return out;
+    const size_t nbytes = sizeof(double) * size_t(nt) * size_t(w + 1) * TT;
+    double* save = dalloc<double>(nbytes / sizeof(double));
+    CB_CHECK(hipMemcpyAsync(save, tiles, nbytes, hipMemcpyDeviceToDevice, s));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    CB_CHECK(hipEventCreate(&e0));
+    CB_CHECK(hipEventCreate(&e1));
+    const bool keep_timing = timing;
+    timing = false;
+    int best = max_pairs;
+    double best_ms = 1e300;
+    for (int c : cand) {
+        double ms = 1e300;
+        bool bad = false;
+        for (int rep = 0; rep < 2 && !bad; ++rep) {
+            CB_CHECK(hipMemcpyAsync(tiles, save, nbytes, hipMemcpyDeviceToDevice, s));
+            CB_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
+            set_pairs(c);
+            CB_CHECK(hipEventRecord(e0, s));
+            factor_filled(true);
+            CB_CHECK(hipEventRecord(e1, s));
+            CB_CHECK(hipStreamSynchronize(s));
+            float t = 0;
+            CB_CHECK(hipEventElapsedTime(&t, e0, e1));
+            if (*flag_host & 1) bad = true;
+            ms = std::min(ms, double(t));
+        }
+        out.emplace_back(npair, bad ? -1.0 : ms);
+        if (!bad && ms < best_ms) {
+            best_ms = ms;
+            best = npair;
+        }
+    }
+    timing = keep_timing;
+    CB_CHECK(hipMemcpyAsync(tiles, save, nbytes, hipMemcpyDeviceToDevice, s));
+    CB_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
+    CB_CHECK(hipStreamSynchronize(s));
+    (void)hipFree(save);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    set_pairs(best);
+    return out;
 }
 
 void BandBatch::solve(const double* rhs, double* out) {

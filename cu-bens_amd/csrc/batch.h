@@ -75,6 +75,10 @@ struct BatchStats {
     // iteration (the nodes of its graph)
     double ops_per_round = 0;
     long body_ops = 0;
+    // helper pairs of the band factor (the tuned count with autotune) and
+    // the most it may use; seconds spent tuning in this run
+    int pairs = 0, pairs_max = 0;
+    double tune_s = 0;
 };
 
 class Batch {
@@ -89,7 +93,16 @@ public:
     // Runs every model to its end (or max_steps recorded steps each).  With
     // graph (and no timing), each round's Newton iteration replays one HIP
     // graph captured from the first round's launches.
-    void run(int max_steps, bool keep_path, BatchStats& st, bool timing = false, bool graph = false);
+    // With autotune, the first run times the band factor for a few helper-pair
+    // counts on the initial tangent and keeps the fastest (the factor, and so
+    // every path, is the same for each count); the choice is remembered for
+    // batches of the same shape in this process, and in the file named by
+    // CUBENS_TUNE_FILE when that is set.
+    void run(int max_steps, bool keep_path, BatchStats& st, bool timing = false, bool graph = false,
+             bool autotune = false);
+    // the pair counts the tuner timed and their factor times (ms; < 0 a wait
+    // bound was hit), empty when it did not run or the choice came from a cache
+    std::vector<std::pair<int, double>> tuning;
     // after run(): per model status (0 ran to its end or to max_steps, 1 zero
     // norm in the convergence test, 3 minimum time increment reached), time and
     // Newton iterations of each recorded step, and the displacement rows
@@ -123,6 +136,8 @@ private:
     size_t path_len_ = 0;
     bool kept_path_ = false;
     hipGraphExec_t gexec_ = nullptr;   // the captured Newton iteration
+    bool tuned_ = false;
+    void tune_band();
     long body_ops_ = 0;                // its operations (graph nodes)
     void build_jobs();
     void iteration(NewmarkArgs& A);    // one Newton iteration of every model (launches only)

@@ -14,7 +14,12 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
 #include <stdexcept>
 #include <thread>
 
@@ -391,7 +396,65 @@ void Batch::iteration(NewmarkArgs& A) {
     copy_gen(d_, G_IP, G_I, true, false);
 }
 
-void Batch::run(int max_steps, bool keep_path, BatchStats& st, bool timing, bool graph) {
+namespace {
+// Tuned helper-pair counts by batch shape: the device's CU count and every
+// model's tile rows and band width.
+std::mutex tune_mu;
+std::map<std::string, int> tune_cache;
+bool tune_file_read = false;
+
+std::string band_shape_key(const BandBatch& b) {
+    int dev = 0, ncu = 0;
+    CB_CHECK(hipGetDevice(&dev));
+    CB_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    uint64_t h = 1469598103934665603ull;   // FNV-1a over (nt, w) of every model
+    for (const BandSeg& g : b.segs_host)
+        for (long v : {g.nt, long(g.w)}) {
+            h ^= uint64_t(v);
+            h *= 1099511628211ull;
+        }
+    std::ostringstream k;
+    k << "batch-pairs cu" << ncu << " models" << b.nseg << " nt" << b.nt << " w" << b.w << " h" << std::hex << h;
+    return k.str();
+}
+}  // namespace
+
+// The band of the initial tangent (the first round's, without the mass term:
+// the factor's time does not depend on the values) is factored for 0, 1/4,
+// 1/2, 3/4 and all of the allowed helper pairs; the fastest count stays.
+void Batch::tune_band() {
+    const std::string key = band_shape_key(band_);
+    const char* file = std::getenv("CUBENS_TUNE_FILE");
+    std::lock_guard<std::mutex> lk(tune_mu);
+    if (file && !tune_file_read) {
+        std::ifstream in(file);
+        std::string line;
+        while (std::getline(in, line)) {
+            const size_t sp = line.rfind(' ');
+            if (sp != std::string::npos) tune_cache[line.substr(0, sp)] = std::atoi(line.c_str() + sp + 1);
+        }
+        tune_file_read = true;
+    }
+    auto it = tune_cache.find(key);
+    if (it != tune_cache.end()) {
+        band_.set_pairs(it->second);
+        return;
+    }
+    band_.zero();
+    launch_frame_tangent_band(d_, G_IP, band_.tiles, band_.w, band_.flag);
+    const int mp = band_.max_pairs;
+    std::vector<int> cand = {0, mp / 4, mp / 2, 3 * mp / 4, mp};
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    tuning = band_.tune_pairs(cand);
+    band_.zero();
+    tune_cache[key] = band_.npair;
+    if (file) {
+        std::ofstream out(file, std::ios::app);
+        out << key << ' ' << band_.npair << '\n';
+    }
+}
+
+void Batch::run(int max_steps, bool keep_path, BatchStats& st, bool timing, bool graph, bool autotune) {
     const size_t nb = models_.size();
     const double t0 = now_s();
     band_.timing = timing;
@@ -449,6 +512,20 @@ void Batch::run(int max_steps, bool keep_path, BatchStats& st, bool timing, bool
     std::vector<NmParam> prm(nb);
     long rounds = 0;
     long ops = 0;
+    if (autotune && !tuned_) {
+        const double tt = now_s();
+        tune_band();
+        tuned_ = true;
+        st.tune_s = now_s() - tt;
+        // the factor's grid changed: capture the iteration again
+        if (gexec_) {
+            CB_CHECK(hipGraphExecDestroy(gexec_));
+            gexec_ = nullptr;
+        }
+        body_ops_ = 0;
+    }
+    st.pairs = band_.npair;
+    st.pairs_max = band_.max_pairs;
     if (!body_ops_ || (graph && !gexec_)) {
         // capture the iteration (without the timing events) once: its node
         // count, and the executable graph when it is to be replayed

@@ -93,6 +93,7 @@ cb_options default_options() {
     o.checkpoint_exact = 0;
     o.graph = 0;
     o.refine = 0;
+    o.autotune = 0;
     return o;
 }
 
@@ -124,7 +125,7 @@ bool same_opts(const cb_options& a, const cb_options& b) {
            a.kernel_timing == b.kernel_timing && a.factor_schedule == b.factor_schedule &&
            a.kernel_timing_every == b.kernel_timing_every && a.sweep_schedule == b.sweep_schedule &&
            a.keep_forces == b.keep_forces && a.checkpoint_exact == b.checkpoint_exact && a.graph == b.graph &&
-           a.refine == b.refine &&
+           a.refine == b.refine && a.autotune == b.autotune &&
            std::string(a.checkpoint_path ? a.checkpoint_path : "") ==
                std::string(b.checkpoint_path ? b.checkpoint_path : "");
 }
@@ -903,8 +904,11 @@ int cb_batch_run(cb_batch* b, const cb_options* opt, cb_batch_stats* st) {
     if (o.solver != CB_SOLVER_BAND64) return fail(-1, "a batch runs the FP64 band solver only");
     GUARD_BEGIN
     cb::BatchStats bs;
-    b->bt->run(o.max_steps, o.keep_d != 0, bs, o.kernel_timing != 0, o.graph != 0);
+    b->bt->run(o.max_steps, o.keep_d != 0, bs, o.kernel_timing != 0, o.graph != 0, o.autotune != 0);
     if (st) {
+        st->pairs = bs.pairs;
+        st->pairs_max = bs.pairs_max;
+        st->tune_s = bs.tune_s;
         st->ops_per_round = bs.ops_per_round;
         st->body_ops = bs.body_ops;
         st->t_total = bs.t_total;
@@ -919,6 +923,17 @@ int cb_batch_run(cb_batch* b, const cb_options* opt, cb_batch_stats* st) {
     }
     return 0;
     GUARD_END
+}
+
+int cb_batch_get_tuning(const cb_batch* b, int* pairs, double* ms, int cap) {
+    if (!b || !b->bt) return fail(-1, "null handle");
+    const auto& t = b->bt->tuning;
+    const int n = int(t.size());
+    for (int i = 0; i < n && i < cap; ++i) {
+        if (pairs) pairs[i] = t[size_t(i)].first;
+        if (ms) ms[i] = t[size_t(i)].second;
+    }
+    return n;
 }
 
 int cb_batch_get_status(const cb_batch* b, int* status, int* nsteps) {

@@ -36,7 +36,7 @@ class Options(C.Structure):
                 ("keep_d", C.c_int), ("kernel_timing", C.c_int), ("factor_schedule", C.c_int),
                 ("kernel_timing_every", C.c_int), ("sweep_schedule", C.c_int), ("keep_forces", C.c_int),
                 ("checkpoint_path", C.c_char_p), ("checkpoint_exact", C.c_int), ("graph", C.c_int),
-                ("refine", C.c_int)]
+                ("refine", C.c_int), ("autotune", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -66,13 +66,14 @@ def options(solver: int = SOLVER_BAND64, krylov_tol: float = 1e-8, restart: int 
             maxit: int = 20000, judge: int = 1, max_steps: int = 1 << 30, keep_d: int = 1,
             kernel_timing: int = 0, factor_schedule: int = 0, kernel_timing_every: int = 1,
             sweep_schedule: int = 0, keep_forces: int = 0, checkpoint_path: str | None = None,
-            checkpoint_exact: int = 0, graph: int = 0, refine: int = 0) -> Options:
+            checkpoint_exact: int = 0, graph: int = 0, refine: int = 0, autotune: int = 0) -> Options:
     """cb_options; checkpoint_path is the Newmark checkpoint file (written every
     CHKPT time steps, read first when the model has RFLAG 1); refine the
     iterative refinement steps of the FP32 precision mode's solves."""
     return Options(solver, krylov_tol, restart, maxit, judge, max_steps, keep_d, kernel_timing,
                    factor_schedule, kernel_timing_every, sweep_schedule, keep_forces,
-                   checkpoint_path.encode() if checkpoint_path else None, checkpoint_exact, graph, refine)
+                   checkpoint_path.encode() if checkpoint_path else None, checkpoint_exact, graph, refine,
+                   autotune)
 
 
 _lib = None
@@ -89,7 +90,8 @@ class BatchStats(C.Structure):
     _fields_ = [("t_total", C.c_double), ("rounds", C.c_long), ("newton", C.c_long),
                 ("steps", C.c_long), ("failed", C.c_int), ("ms_factor", C.c_double),
                 ("ms_solve", C.c_double), ("factor_flops", C.c_double), ("solve_bytes", C.c_double),
-                ("ops_per_round", C.c_double), ("body_ops", C.c_long)]
+                ("ops_per_round", C.c_double), ("body_ops", C.c_long), ("pairs", C.c_int),
+                ("pairs_max", C.c_int), ("tune_s", C.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -151,6 +153,7 @@ ABI = {
     "cb_batch_get_info": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Info)]),
     "cb_batch_run": (C.c_int, [C.c_void_p, C.POINTER(Options), C.POINTER(BatchStats)]),
     "cb_batch_get_status": (C.c_int, [C.c_void_p, PI, PI]),
+    "cb_batch_get_tuning": (C.c_int, [C.c_void_p, PI, PD, C.c_int]),
     "cb_batch_get_path": (C.c_int, [C.c_void_p, C.c_int, PD, PI, PD]),
 }
 
@@ -422,10 +425,12 @@ class GpuBatch:
             pass
 
     def run(self, max_steps: int = 1 << 30, keep_d: bool = True, timing: bool = False,
-            graph: bool = False) -> dict:
-        """cb_batch_run; graph replays each round's Newton iteration as one HIP graph."""
+            graph: bool = False, autotune: bool = False) -> dict:
+        """cb_batch_run; graph replays each round's Newton iteration as one HIP
+        graph; autotune tunes the band factor's helper pairs on the first run."""
         st = BatchStats()
-        opt = options(max_steps=max_steps, keep_d=int(keep_d), kernel_timing=int(timing), graph=int(graph))
+        opt = options(max_steps=max_steps, keep_d=int(keep_d), kernel_timing=int(timing), graph=int(graph),
+                      autotune=int(autotune))
         _check(lib().cb_batch_run(self.h, C.byref(opt), C.byref(st)))
         status = np.zeros(self.count, np.int32)
         nsteps = np.zeros(self.count, np.int32)
@@ -433,6 +438,15 @@ class GpuBatch:
         out = st.as_dict()
         out.update(status=status, nsteps=nsteps)
         return out
+
+    def tuning(self) -> list:
+        """The tuner's trials: [(helper pairs, factor ms)] (cb_batch_get_tuning)."""
+        pairs = np.zeros(16, np.int32)
+        ms = np.zeros(16)
+        n = lib().cb_batch_get_tuning(self.h, _p(pairs, PI), _p(ms), 16)
+        if n < 0:
+            _check(n)
+        return [(int(pairs[i]), float(ms[i])) for i in range(min(n, 16))]
 
     def path(self, i: int, with_d: bool = True) -> dict:
         """Model i's recorded steps: time, itecnt and (with_d) d [nsteps, NEQ_i]."""

@@ -77,6 +77,12 @@ typedef struct {
      * residual of the FP32 tangent (rhs - K32 x, FP64 accumulation) solved
      * again with the FP32 factor and added; 0: the FP32 solve alone */
     int refine;
+    /* batches: 1 has the first run time the band factor for a few helper-pair
+     * counts on the initial tangent and keep the fastest (every path is the
+     * same for each count; the choice is remembered per batch shape in the
+     * process, and in the file named by the environment variable
+     * CUBENS_TUNE_FILE when set); 0 uses the most pairs that fit */
+    int autotune;
 } cb_options;
 
 /* Schedules of the band LDL^T factorization (same tile arithmetic, bitwise
@@ -356,6 +362,10 @@ typedef struct {
      * its graph) */
     double ops_per_round;
     long body_ops;
+    /* helper pairs of the band factor in this run and the most it may use;
+     * seconds spent tuning (autotune) */
+    int pairs, pairs_max;
+    double tune_s;
 } cb_batch_stats;
 /* Reads every model (as cb_open does) and builds the batch's device state;
  * frame-only models without member-end offsets or free warping DOFs. */
@@ -371,6 +381,10 @@ int cb_batch_run(cb_batch *b, const cb_options *opt, cb_batch_stats *st);
  * norm in the convergence test, 3 minimum time increment reached without
  * convergence) and recorded steps. */
 int cb_batch_get_status(const cb_batch *b, int *status, int *nsteps);
+/* The tuner's trials (no counterpart in the reference): up to cap (helper
+ * pairs, band factor ms; ms < 0 when a wait bound was hit) pairs; returns the
+ * number of trials, 0 when it has not run or its choice came from a cache. */
+int cb_batch_get_tuning(const cb_batch *b, int *pairs, double *ms, int cap);
 /* Model i's path as cb_get_path gives it: time [nsteps], itecnt [nsteps],
  * d [nsteps*NEQ_i] (any pointer may be NULL). */
 int cb_batch_get_path(const cb_batch *b, int i, double *t, int *itecnt, double *d);

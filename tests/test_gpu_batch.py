@@ -208,3 +208,38 @@ def test_graph_replay_equals_the_launches_one_by_one():
             assert st["body_ops"] == eager["body_ops"] > 10
             assert st["ops_per_round"] < eager["ops_per_round"] - st["body_ops"] + 2
             b.run(max_steps=3)                 # eager work between the graph runs
+
+
+def test_autotuned_helper_pairs_keep_every_path(tmp_path, monkeypatch):
+    """The tuner times the band factor for 0, 1 and 2 helper pairs (the two
+    girders with two tiles below the diagonal), keeps the fastest, and every
+    model's path stays bitwise the same, launched one by one and replayed as a
+    graph captured after the tuning; the choice goes to CUBENS_TUNE_FILE and a
+    second batch of the same shape takes it without timing again."""
+    steps = 8
+    paths = _mixed_files(steps)
+    tune = tmp_path / "tune.txt"
+    monkeypatch.setenv("CUBENS_TUNE_FILE", str(tune))
+    with cubens.GpuBatch(paths) as b:
+        plain = b.run(max_steps=steps)
+        assert plain["pairs"] == plain["pairs_max"] == 2 and plain["tune_s"] == 0
+        ref = [b.path(i) for i in range(b.count)]
+        print("[batch] autotune", flush=True)
+        st = b.run(max_steps=steps, autotune=True)
+        trials = b.tuning()
+        print(f"[batch] tuner trials (pairs, ms): {trials}; kept {st['pairs']} in {st['tune_s']:.3f}s", flush=True)
+        assert [p for p, _ in trials] == [0, 1, 2]
+        assert all(ms > 0 for _, ms in trials)
+        assert st["pairs"] == min(trials, key=lambda t: t[1])[0] and st["tune_s"] > 0
+        for run in (st, b.run(max_steps=steps, graph=True)):
+            assert np.array_equal(run["nsteps"], plain["nsteps"]) and run["rounds"] == plain["rounds"]
+            for i in range(b.count):
+                p = b.path(i)
+                assert np.array_equal(p["itecnt"], ref[i]["itecnt"])
+                assert np.array_equal(p["d"], ref[i]["d"])
+        kept = st["pairs"]
+    lines = tune.read_text().splitlines()
+    assert len(lines) == 1 and lines[0].startswith("batch-pairs ") and lines[0].endswith(f" {kept}")
+    with cubens.GpuBatch(paths) as b2:
+        st2 = b2.run(max_steps=2, autotune=True)
+        assert b2.tuning() == [] and st2["pairs"] == kept
