@@ -325,7 +325,7 @@ def run_learned(args) -> None:
         t0 = time.time()
         hist = learned.fit(L, train, panel(10, 998), rounds=12, lr=2e-4)
         log(f"[bench] learned: fit in {time.time() - t0:.1f}s, validation iterations {hist['val_iters']}")
-        for n in (10, 16, 24, 32):
+        for n in (int(v) for v in args.learned_sizes.split(",")):
             path = panel(n, 999)
             opt = dict(krylov_tol=1e-8, restart=30, maxit=40000, judge=1, max_steps=2)
             with cubens.GpuModel(path) as g:
@@ -350,7 +350,8 @@ def run_learned(args) -> None:
                       "value": last["bj_iters"] / last["learned_iters"], "unit": "ratio", "n_gpus": 1,
                       "higher_is_better": True, "dtype": "bf16 network, f64 Krylov",
                       "data": "synthetic (seeded shell panels)",
-                      "config": {"workload": "held-out shell panels 10x10 to 32x32, 2 arc-length steps",
+                      "config": {"workload": f"held-out shell panels {args.learned_sizes} (n x n), 2 arc-length "
+                                             "steps",
                                  "training": "four 8x8 and two 12x12 panels, 12 rounds, lr 2e-4",
                                  "selected_round": hist["best_round"]},
                       "panels": rows}))
@@ -449,6 +450,8 @@ def main():
                     help="panel: the 256x256 shell panel MSAL (the headline); frames: the 512-frame "
                          "nonlinear Newmark batch (--steps time steps); learned: the learned "
                          "preconditioner against block-Jacobi on held-out panels")
+    ap.add_argument("--learned-sizes", default="10,16,24,32",
+                    help="held-out panel sizes n (n x n) of --workload learned")
     ap.add_argument("--schedule", default="fused", choices=("fused", "split"),
                     help="band factorization schedule: one fused launch per tile column, or "
                          "diag/panel/update launches")
