@@ -358,7 +358,7 @@ def run_learned(args) -> None:
 
 
 def run_path(path: str, solver: int, warmup: int, steps: int, tol: float, timing: int = 0,
-             schedule: int = 0, refine: int = 0) -> dict:
+             schedule: int = 0, refine: int = 0, sweeps: int = 0) -> dict:
     """warmup increments of an untimed analysis, then the first `steps`
     increments of a fresh analysis, timed from its start."""
     import cubens
@@ -370,7 +370,7 @@ def run_path(path: str, solver: int, warmup: int, steps: int, tol: float, timing
         # the FP32 precision mode decides det and psi on its own FP32 pivots
         judge = 0 if solver in (cubens.SOLVER_BAND64, cubens.SOLVER_BAND32) else 1
         common = dict(krylov_tol=tol, restart=30, maxit=400, judge=judge, factor_schedule=schedule,
-                      refine=refine)
+                      refine=refine, sweep_schedule=sweeps)
         if warmup > 0:
             g.run(cubens.options(solver, max_steps=warmup, keep_d=0, **common))
         opt = cubens.options(solver, max_steps=steps, keep_d=1, kernel_timing=timing,
@@ -450,6 +450,9 @@ def main():
                     help="panel: the 256x256 shell panel MSAL (the headline); frames: the 512-frame "
                          "nonlinear Newmark batch (--steps time steps); learned: the learned "
                          "preconditioner against block-Jacobi on held-out panels")
+    ap.add_argument("--sweeps", default="chain", choices=("chain", "rows", "look2"),
+                    help="band solve sweeps: chain matrices, the two-product row step, or the chain "
+                         "matrices with the forward sweep two rows ahead")
     ap.add_argument("--learned-sizes", default="10,16,24,32",
                     help="held-out panel sizes n (n x n) of --workload learned")
     ap.add_argument("--schedule", default="fused", choices=("fused", "split"),
@@ -487,7 +490,8 @@ def main():
         w = args.warmup if name != "mixed" else min(args.warmup, 1)
         log(f"[bench] path {name}: {w} warm-up increments, then increments 1..{k} timed")
         res = run_path(path, solver, w, k, args.tol, timing=127 if name == "fp64" else 0, schedule=schedule,
-                       refine=1 if name == "fp32" else 0)
+                       refine=1 if name == "fp32" else 0,
+                       sweeps={"chain": 0, "rows": 1, "look2": 2}[args.sweeps])
         st = res["stats"]
         log(f"[bench] {name}: {res['rate']:.2f} Newton it/s ({res['newton']} its in {res['seconds']:.2f}s); "
             f"tangent {st['t_tangent']:.2f}s factor {st['t_factor']:.2f}s solve {st['t_solve']:.2f}s "
@@ -554,7 +558,8 @@ def main():
                    "path": "plain fp64 band LDL^T (the metric's baseline path)" if head is plain
                            else "fp32-factor FGMRES",
                    "increments_timed": f"1..{k} of a fresh analysis after a {args.warmup}-increment warm-up run",
-                   "newton_iterations": head["newton"], "factor_schedule": args.schedule},
+                   "newton_iterations": head["newton"], "factor_schedule": args.schedule,
+                   "sweeps": args.sweeps},
         "roofline": roof,
         "cpu_baseline": cpu,
         **extra,

@@ -62,6 +62,13 @@ struct BandLDLT {
     R* mch = nullptr;
     R* gch = nullptr;
     bool chained = true;     // sweeps with the chain matrices (default) or the two-product row step
+    // the forward sweep two rows ahead (with chained): L' and W of the
+    // look-ahead system [nt][w] and [nt] tiles, b' [nt * 64]; allocated at the
+    // first factorization that needs them
+    bool look2 = false;
+    R* l2 = nullptr;
+    R* w2 = nullptr;
+    double* bp = nullptr;
     double* xw = nullptr;    // work vectors (padded)
     double* yw = nullptr;
     int* flag = nullptr;       // error bits: 1 a wait bound hit, 2 an entry outside the band

@@ -810,7 +810,7 @@ __device__ __forceinline__ double poll_value(const double* p, long spin_limit, i
             atomicOr(err, 1);
             return __builtin_nan("");
         }
-        if (n >= 1024) __builtin_amdgcn_s_sleep(32);   // far from the front: poll less often
+        if (n >= 1024) __builtin_amdgcn_s_sleep(4);   // far from the front: poll less often
     }
 }
 
@@ -1152,6 +1152,150 @@ __global__ void __launch_bounds__(256) k_bwd_sweep_m(const R* __restrict__ tiles
         if (q == 0) xi -= part[0][col] + part[1][col] + part[2][col] + part[3][col];
     }
     if (q == 0) put_value(&x[I * T + col], xi);
+}
+
+// ---- look-ahead forward sweep -------------------------------------------------
+// Row I of the block system L y = b, multiplied by Linv_I, less W_I = M_I
+// Linv_{I-1} times row I-1 (multiplied by Linv_{I-1}), has no coupling to row
+// I-1 any more (Linv_I L_{I,I-1} - M_I = 0):
+//   y_I = b'_I - sum_{J=I-w-1}^{I-2} L'_IJ y_J,
+//   b'_I = Linv_I b_I - W_I b_{I-1},  L'_IJ = Linv_I L_IJ - W_I L_{I-1,J}.
+// The newest row a row waits for is then I-2, so the even and the odd rows
+// form two chains that run side by side.  L' is kept per row in w slots, slot
+// s for J = I-2-s (column-major tiles), formed after each factorization.
+
+// W_I = M_I Linv_{I-1} (I >= 1), column-major.  Y^T = Linv_{I-1}: ys[k][col] =
+// Linv_{I-1}[k][col], a transposed staging of the column-major tile.
+template <class R>
+__global__ void __launch_bounds__(NTH) k_look2_w(const R* __restrict__ mch, const R* __restrict__ linv,
+                                                 R* __restrict__ w2, long nt) {
+    __shared__ R xs[T][T];
+    __shared__ R ys[T][T];
+    const long I = blockIdx.x + 1;
+    stage<false>(xs, mch + I * (long)TT);
+    const R* lv = linv + (I - 1) * (long)TT;
+    for (int idx = threadIdx.x; idx < TT; idx += NTH) ys[idx % T][idx / T] = lv[idx];
+    __syncthreads();
+    TileAcc<R> acc;
+    acc.product(xs, ys);
+    acc.each([&](int row, int col, R v) { w2[I * (long)TT + col * T + row] = v; });
+}
+
+// L'_{I, I-2-s} for workgroup (I, s): Linv_I L_IJ (when I - J <= w) less
+// W_I L_{I-1,J}; the L tiles are staged from their row-major transposed
+// copies, so that ys[k][col] = L[k][col] reads straight.
+template <class R>
+__global__ void __launch_bounds__(NTH) k_look2_band(const R* __restrict__ tiles_t, const R* __restrict__ linv,
+                                                    const R* __restrict__ w2, R* __restrict__ l2, long nt, int w) {
+    __shared__ R xs[T][T];
+    __shared__ R ys[T][T];
+    const long I = blockIdx.x / w;
+    const int s = int(blockIdx.x % w);
+    const long J = I - 2 - s;
+    if (J < 0) return;
+    TileAcc<R> a, b;
+    const bool direct = I - J <= w;
+    if (direct) {
+        stage<false>(xs, linv + I * (long)TT);
+        stage<false>(ys, tiles_t + tile_index(I, J, w) * TT);
+        __syncthreads();
+        a.product(xs, ys);
+        __syncthreads();
+    }
+    stage<false>(xs, w2 + I * (long)TT);
+    stage<false>(ys, tiles_t + tile_index(I - 1, J, w) * TT);
+    __syncthreads();
+    b.product(xs, ys);
+    R* out = l2 + (I * w + s) * (long)TT;
+    if (direct) {
+        R av[TileAcc<R>::N];
+        a.each_i([&](int, int, R v, int e) { av[e] = v; });
+        b.each_i([&](int row, int col, R v, int e) { out[col * T + row] = av[e] - v; });
+    } else {
+        b.each([&](int row, int col, R v) { out[col * T + row] = -v; });
+    }
+}
+
+// b'_I = Linv_I b_I - W_I b_{I-1}, one workgroup per tile row, thread (row, q)
+// summing columns 16q..16q+15
+template <class R>
+__global__ void __launch_bounds__(256) k_look2_rhs(const R* __restrict__ linv, const R* __restrict__ w2,
+                                                   const double* __restrict__ b, double* __restrict__ bp, long nt) {
+    __shared__ double part[4][T];
+    const long I = blockIdx.x;
+    const int tid = threadIdx.x, row = tid % T, q = tid / T, k0 = 16 * q;
+    const R* lv = linv + I * (long)TT;
+    double u = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) u += double(lv[(k0 + k) * T + row]) * b[I * T + k0 + k];
+    if (I >= 1) {
+        const R* wv = w2 + I * (long)TT;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) u -= double(wv[(k0 + k) * T + row]) * b[(I - 1) * T + k0 + k];
+    }
+    part[q][row] = u;
+    __syncthreads();
+    if (tid < T) bp[I * T + tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+}
+
+// y_I = b'_I - sum_{J=I-w-1}^{I-3} L'_IJ y_J - L'_{I,I-2} y_{I-2}: the older
+// rows in chunks of NCH as in k_fwd_sweep_m, then the newest, two rows back.
+// y must hold SENT.
+template <class R>
+__global__ void __launch_bounds__(256) k_fwd_sweep_2(const R* __restrict__ l2, const double* __restrict__ bp,
+                                                     double* y, long nt, int w, long spin_limit, int* err) {
+    __shared__ double vin[T];
+    __shared__ double vch[NCH][T];
+    __shared__ double part[4][T];
+    const long I = blockIdx.x;
+    const int tid = threadIdx.x, row = tid % T, q = tid / T, k0 = 16 * q;
+    R ln[16];
+    const bool has2 = I >= 2;
+    if (has2) {
+        const R* np = l2 + (I * w) * (long)TT;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) ln[k] = np[(k0 + k) * T + row];
+    }
+    double acc = (q == 0) ? bp[I * T + row] : 0.0;
+    const long J0 = I - w - 1 > 0 ? I - w - 1 : 0;
+    for (long J = J0; J < I - 2; J += NCH) {
+        const int nc = (I - 2 - J < NCH) ? int(I - 2 - J) : NCH;
+        R lx[NCH][16];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            if (c < nc) {
+                const R* l = l2 + (I * w + (I - 2 - (J + c))) * (long)TT;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) lx[c][k] = l[(k0 + k) * T + row];
+            }
+        fetch_rows(y, J, 1, nc, vch, spin_limit, err);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            if (c < nc) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) acc -= double(lx[c][k]) * vch[c][k0 + k];
+            }
+        __syncthreads();
+    }
+    part[q][row] = acc;
+    __syncthreads();
+    double yi = (q == 0) ? part[0][row] + part[1][row] + part[2][row] + part[3][row] : 0.0;
+    if (has2) {
+        fetch_row(y, I - 2, vin, spin_limit, err);
+        double m = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) m += double(ln[k]) * vin[k0 + k];
+        part[q][row] = m;
+        __syncthreads();
+        if (q == 0) yi -= part[0][row] + part[1][row] + part[2][row] + part[3][row];
+    }
+    if (q == 0) put_value(&y[I * T + row], yi);
+}
+
+// v = SENT
+__global__ void k_fill_sent(double* __restrict__ v, long n) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) v[i] = __builtin_bit_cast(double, SENT);
 }
 
 // dst = rhs padded with zeros to ntot; y = SENT
@@ -1633,11 +1777,12 @@ void BandLDLT<R>::setup(long n_, long bandwidth, hipStream_t s_) {
 
 template <class R>
 void BandLDLT<R>::release() {
-    void* ps[] = {tiles, tiles_t, linv, mch, gch, piv, xw, yw, flag, pflags};
+    void* ps[] = {tiles, tiles_t, linv, mch, gch, piv, xw, yw, flag, pflags, l2, w2, bp};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     tiles = nullptr; tiles_t = nullptr; linv = nullptr; piv = nullptr; xw = nullptr; yw = nullptr; flag = nullptr;
     mch = nullptr; gch = nullptr;
+    l2 = nullptr; w2 = nullptr; bp = nullptr;
     pflags = nullptr;
     clock.release();
 }
@@ -1704,6 +1849,15 @@ int BandLDLT<R>::factor(const Dev& d) {
         const bool tk_prep = clock.take(KT_PREP);
         if (tk_prep) clock.begin(KT_PREP, s);
         hipLaunchKernelGGL((k_sweep_chain<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, mch, gch, nt, w);
+        if (look2 && nt >= 2) {
+            if (!l2) {
+                l2 = dalloc<R>(size_t(nt) * size_t(w) * TT);
+                w2 = dalloc<R>(size_t(nt) * TT);
+                bp = dalloc<double>(size_t(nt) * T);
+            }
+            hipLaunchKernelGGL((k_look2_w<R>), dim3(int(nt - 1)), dim3(NTH), 0, s, mch, linv, w2, nt);
+            hipLaunchKernelGGL((k_look2_band<R>), dim3(int(nt * w)), dim3(NTH), 0, s, tiles_t, linv, w2, l2, nt, w);
+        }
         if (tk_prep) clock.end(s);
         CB_CHECK(hipGetLastError());
     }
@@ -1724,9 +1878,16 @@ void BandLDLT<R>::solve(const double* rhs, double* out) {
     CB_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_sweep_prep, dim3(int((nt * T + 255) / 256)), dim3(256), 0, s, xw, rhs, yw, n, nt * T);
     const long spin_limit = 1L << 22;
+    const bool ahead = chained && look2 && l2;
+    if (ahead) {
+        hipLaunchKernelGGL((k_look2_rhs<R>), dim3(int(nt)), dim3(256), 0, s, linv, w2, xw, bp, nt);
+        hipLaunchKernelGGL(k_fill_sent, dim3(int((nt * T + 255) / 256)), dim3(256), 0, s, xw, nt * T);
+    }
     const bool tk_fwd = clock.take(KT_FWD);
     if (tk_fwd) clock.begin(KT_FWD, s);
-    if (chained)
+    if (ahead)
+        hipLaunchKernelGGL((k_fwd_sweep_2<R>), dim3(int(nt)), dim3(256), 0, s, l2, bp, yw, nt, w, spin_limit, flag);
+    else if (chained)
         hipLaunchKernelGGL((k_fwd_sweep_m<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, mch, xw, yw, nt, w,
                            spin_limit, flag);
     else

@@ -56,8 +56,10 @@ Analysis::Analysis(Model& m, Dev& d, const RunOptions& o) : m_(m), d_(d), o_(o) 
     b32_.clock.every = o.kernel_timing_every;
     b64_.split = o.factor_schedule == 1;
     b32_.split = o.factor_schedule == 1;
-    b64_.chained = o.sweep_schedule == 0;
-    b32_.chained = o.sweep_schedule == 0;
+    b64_.chained = o.sweep_schedule != 1;
+    b32_.chained = o.sweep_schedule != 1;
+    b64_.look2 = o.sweep_schedule == 2;
+    b32_.look2 = o.sweep_schedule == 2;
     if (o.solver != SOLVER_BAND64 && o.solver != SOLVER_BAND32) gm_.setup(d.nslot, o.restart);
     CB_CHECK(hipStreamSynchronize(d.s));
 }

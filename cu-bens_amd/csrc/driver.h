@@ -52,7 +52,8 @@ struct RunOptions {
     int kernel_timing = 0;  // KT_* bitmask of band kernels timed with HIP events
     int factor_schedule = 0;  // 0 fused launch per tile column, 1 split diag/panel/update launches
     int kernel_timing_every = 1;  // time every n-th launch of a timed family
-    int sweep_schedule = 0;       // 0 sweeps with the chain matrices, 1 the two-product row step
+    int sweep_schedule = 0;       // 0 sweeps with the chain matrices, 1 the two-product row step,
+                                  // 2 chain matrices with the forward sweep two rows ahead
     int keep_forces = 0;          // record the averaged element forces of every converged step
     // Newmark checkpoint file (checkPoint / restartStep): written every CHKPT
     // time steps when set, read first when the model has RFLAG 1; exact 1

@@ -51,7 +51,7 @@ def test_chain_and_row_sweeps_solve_the_tangent(case):
         rng = np.random.default_rng(nt * 31 + w)
         b = rng.standard_normal(g.neq)
         xs = {}
-        for sch in (cubens.SWEEP_CHAIN, cubens.SWEEP_ROWS):
+        for sch in (cubens.SWEEP_CHAIN, cubens.SWEEP_ROWS, cubens.SWEEP_LOOK2):
             det, piv = g.factor(cubens.options(cubens.SOLVER_BAND64, sweep_schedule=sch))
             xs[sch] = g.solve(b)
         knorm = np.abs(K).sum(axis=1).max()
@@ -60,12 +60,13 @@ def test_chain_and_row_sweeps_solve_the_tangent(case):
             eta = np.abs(r).max() / (knorm * np.abs(x).max() + np.abs(b).max())
             print(f"[sweeps] {case} nt {nt} w {w} schedule {sch}: backward error {eta:.2e}", flush=True)
             assert eta < 1e-13
-        # the two schedules differ only in rounding
+        # the schedules differ only in rounding
         assert relerr(xs[cubens.SWEEP_CHAIN], xs[cubens.SWEEP_ROWS]) < 1e-8
+        assert relerr(xs[cubens.SWEEP_LOOK2], xs[cubens.SWEEP_CHAIN]) < 1e-8
 
 
-@pytest.mark.parametrize("sch", [0, 1])
-def test_panel_path_of_both_sweep_schedules_matches_the_oracle(sch):
+@pytest.mark.parametrize("sch", [0, 1, 2])
+def test_panel_path_of_every_sweep_schedule_matches_the_oracle(sch):
     path = model_file("sweep_panel16_path.txt", inputs.shell_panel(n=16).text)
     with cubens.GpuModel(path) as g:
         gp = g.run(cubens.options(cubens.SOLVER_BAND64, max_steps=6, sweep_schedule=sch))
