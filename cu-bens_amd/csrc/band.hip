@@ -775,8 +775,9 @@ __global__ void __launch_bounds__(NTH) k_band_column(R* __restrict__ tiles, R* _
 // the 64 values with system-scope stores (written through to memory) and a
 // consumer polls them with system-scope loads until none is SENT.  There is
 // no flag and no cache maintenance on the critical path.  Every lane polls
-// its own value with several loads in flight, so a row is seen about one
-// memory round trip after it lands; polling slows down once a wait has
+// its own value, one load at a time (several in flight, which sees a value
+// sooner, made the sweeps slower: about 26 consumers poll each row, so the
+// polls crowd the row's memory channel); polling slows down once a wait has
 // lasted long, so the workgroups far ahead of the front load memory little.
 // Each wait is bounded: past spin_limit polls the value reads as
 // a quiet NaN and *err is raised, so every workgroup still finishes and the
@@ -792,25 +793,14 @@ constexpr unsigned long long SENT = 0x7FF4C0DE5E471E11ull;
 __device__ __forceinline__ double poll_value(const double* p, long spin_limit, int* err) {
     const unsigned long long* q = reinterpret_cast<const unsigned long long*>(p);
     auto ld = [q] { return __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM); };
-    unsigned long long a = ld();
-    if (a != SENT) return __builtin_bit_cast(double, a);
-    // four loads in flight: each check waits for the oldest one only and
-    // issues the next, so a value is seen about one round trip after it lands
-    unsigned long long b = ld(), c = ld(), d = ld();
-    for (long n = 0;; n += 4) {
+    for (long n = 0;; ++n) {
+        const unsigned long long a = ld();
         if (a != SENT) return __builtin_bit_cast(double, a);
-        a = ld();
-        if (b != SENT) return __builtin_bit_cast(double, b);
-        b = ld();
-        if (c != SENT) return __builtin_bit_cast(double, c);
-        c = ld();
-        if (d != SENT) return __builtin_bit_cast(double, d);
-        d = ld();
         if (n >= spin_limit) {
             atomicOr(err, 1);
             return __builtin_nan("");
         }
-        if (n >= 1024) __builtin_amdgcn_s_sleep(4);   // far from the front: poll less often
+        if (n >= 256) __builtin_amdgcn_s_sleep(4);   // far from the front: poll less often
     }
 }
 
@@ -1238,28 +1228,23 @@ __global__ void __launch_bounds__(256) k_look2_rhs(const R* __restrict__ linv, c
     if (tid < T) bp[I * T + tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
 }
 
-// y_I = b'_I - sum_{J=I-w-1}^{I-3} L'_IJ y_J - L'_{I,I-2} y_{I-2}: the older
-// rows in chunks of NCH as in k_fwd_sweep_m, then the newest, two rows back.
-// y must hold SENT.
+// y_I = b'_I - sum_{J=I-w-1}^{I-2} L'_IJ y_J over the rows in chunks of NCH,
+// polled side by side, aligned so that the last chunk ends at row I-2: a row
+// waits once for the newest rows it needs and the two chains of the
+// look-ahead system (even and odd rows) run side by side.  y must hold SENT.
 template <class R>
 __global__ void __launch_bounds__(256) k_fwd_sweep_2(const R* __restrict__ l2, const double* __restrict__ bp,
                                                      double* y, long nt, int w, long spin_limit, int* err) {
-    __shared__ double vin[T];
     __shared__ double vch[NCH][T];
     __shared__ double part[4][T];
     const long I = blockIdx.x;
     const int tid = threadIdx.x, row = tid % T, q = tid / T, k0 = 16 * q;
-    R ln[16];
-    const bool has2 = I >= 2;
-    if (has2) {
-        const R* np = l2 + (I * w) * (long)TT;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) ln[k] = np[(k0 + k) * T + row];
-    }
     double acc = (q == 0) ? bp[I * T + row] : 0.0;
     const long J0 = I - w - 1 > 0 ? I - w - 1 : 0;
-    for (long J = J0; J < I - 2; J += NCH) {
-        const int nc = (I - 2 - J < NCH) ? int(I - 2 - J) : NCH;
+    const long nrow = I - 1 - J0;   // rows J0 .. I-2
+    int nc = int(nrow % NCH);
+    if (nc == 0) nc = NCH;
+    for (long J = J0; J <= I - 2; J += nc, nc = NCH) {
         R lx[NCH][16];
 #pragma unroll
         for (int c = 0; c < NCH; ++c)
@@ -1279,17 +1264,7 @@ __global__ void __launch_bounds__(256) k_fwd_sweep_2(const R* __restrict__ l2, c
     }
     part[q][row] = acc;
     __syncthreads();
-    double yi = (q == 0) ? part[0][row] + part[1][row] + part[2][row] + part[3][row] : 0.0;
-    if (has2) {
-        fetch_row(y, I - 2, vin, spin_limit, err);
-        double m = 0.0;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) m += double(ln[k]) * vin[k0 + k];
-        part[q][row] = m;
-        __syncthreads();
-        if (q == 0) yi -= part[0][row] + part[1][row] + part[2][row] + part[3][row];
-    }
-    if (q == 0) put_value(&y[I * T + row], yi);
+    if (q == 0) put_value(&y[I * T + row], part[0][row] + part[1][row] + part[2][row] + part[3][row]);
 }
 
 // v = SENT
