@@ -252,6 +252,11 @@ def run_frames(args) -> None:
     e = b.run(max_steps=k, keep_d=False)
     g = b.run(max_steps=k, keep_d=False, graph=True)
     r = b.run(max_steps=k, keep_d=False, timing=True)
+    # the member tangents in FP32 (rounded into the FP64 band), one graph per round
+    g32 = b.run(max_steps=k, keep_d=False, graph=True, fp32_elements=True)
+    rate32 = g32["newton"] / g32["t_total"]
+    log(f"[bench] frames: {rate32:.0f} Newton it/s with FP32 member tangents ({g32['newton']} its, "
+        f"{g32['rounds']} rounds in {g32['t_total']:.2f}s)")
     for x in (e, g, r):
         if x["failed"] or int(x["nsteps"].min()) < k:
             raise RuntimeError(f"frame batch stopped early: {x}")
@@ -296,7 +301,10 @@ def run_frames(args) -> None:
                    "launches": {"ops_per_round_graph": g["ops_per_round"], "ops_per_round_one_by_one":
                                 e["ops_per_round"], "iteration_ops": g["body_ops"],
                                 "newton_per_s_one_by_one": rate_e},
-                   "autotune": tuning},
+                   "autotune": tuning,
+                   "fp32_element_tangents": {"newton_per_s": rate32, "newton": int(g32["newton"]),
+                                             "rounds": int(g32["rounds"]), "seconds": g32["t_total"],
+                                             "failed": int(g32["failed"])}},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

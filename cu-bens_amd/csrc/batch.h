@@ -98,8 +98,10 @@ public:
     // every path, is the same for each count); the choice is remembered for
     // batches of the same shape in this process, and in the file named by
     // CUBENS_TUNE_FILE when that is set.
+    // With fp32_elements, the member tangents are formed in FP32 and rounded
+    // into the FP64 band; internal forces, residual, factor and solve stay FP64.
     void run(int max_steps, bool keep_path, BatchStats& st, bool timing = false, bool graph = false,
-             bool autotune = false);
+             bool autotune = false, bool fp32_elements = false);
     // the pair counts the tuner timed and their factor times (ms; < 0 a wait
     // bound was hit), empty when it did not run or the choice came from a cache
     std::vector<std::pair<int, double>> tuning;
@@ -137,6 +139,7 @@ private:
     bool kept_path_ = false;
     hipGraphExec_t gexec_ = nullptr;   // the captured Newton iteration
     bool tuned_ = false;
+    bool fp32_el_ = false;     // the element tangent of iteration() (and of the captured graph)
     void tune_band();
     long body_ops_ = 0;                // its operations (graph nodes)
     void build_jobs();

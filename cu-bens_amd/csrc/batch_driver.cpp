@@ -379,7 +379,7 @@ void Batch::iteration(NewmarkArgs& A) {
     // the tangent goes straight into the band (the copy the single-model
     // solver makes from the joint blocks), then Keff's mass term
     band_.zero();
-    launch_frame_tangent_band(d_, G_IP, band_.tiles, band_.w, band_.flag);
+    launch_frame_tangent_band(d_, G_IP, band_.tiles, band_.w, band_.flag, fp32_el_);
     vfill(d_, sm, 0.0, n_);
     launch_mass(d_, sm, 1);
     batch_add_mass_band(d_, band_.tiles, band_.w, sm, prm_dev_, own_joint_);
@@ -454,7 +454,8 @@ void Batch::tune_band() {
     }
 }
 
-void Batch::run(int max_steps, bool keep_path, BatchStats& st, bool timing, bool graph, bool autotune) {
+void Batch::run(int max_steps, bool keep_path, BatchStats& st, bool timing, bool graph, bool autotune,
+                bool fp32_elements) {
     const size_t nb = models_.size();
     const double t0 = now_s();
     band_.timing = timing;
@@ -518,6 +519,15 @@ void Batch::run(int max_steps, bool keep_path, BatchStats& st, bool timing, bool
         tuned_ = true;
         st.tune_s = now_s() - tt;
         // the factor's grid changed: capture the iteration again
+        if (gexec_) {
+            CB_CHECK(hipGraphExecDestroy(gexec_));
+            gexec_ = nullptr;
+        }
+        body_ops_ = 0;
+    }
+    if (fp32_elements != fp32_el_) {
+        // another element tangent: capture the iteration again
+        fp32_el_ = fp32_elements;
         if (gexec_) {
             CB_CHECK(hipGraphExecDestroy(gexec_));
             gexec_ = nullptr;

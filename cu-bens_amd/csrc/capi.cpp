@@ -94,6 +94,7 @@ cb_options default_options() {
     o.graph = 0;
     o.refine = 0;
     o.autotune = 0;
+    o.fp32_elements = 0;
     return o;
 }
 
@@ -125,7 +126,7 @@ bool same_opts(const cb_options& a, const cb_options& b) {
            a.kernel_timing == b.kernel_timing && a.factor_schedule == b.factor_schedule &&
            a.kernel_timing_every == b.kernel_timing_every && a.sweep_schedule == b.sweep_schedule &&
            a.keep_forces == b.keep_forces && a.checkpoint_exact == b.checkpoint_exact && a.graph == b.graph &&
-           a.refine == b.refine && a.autotune == b.autotune &&
+           a.refine == b.refine && a.autotune == b.autotune && a.fp32_elements == b.fp32_elements &&
            std::string(a.checkpoint_path ? a.checkpoint_path : "") ==
                std::string(b.checkpoint_path ? b.checkpoint_path : "");
 }
@@ -904,7 +905,8 @@ int cb_batch_run(cb_batch* b, const cb_options* opt, cb_batch_stats* st) {
     if (o.solver != CB_SOLVER_BAND64) return fail(-1, "a batch runs the FP64 band solver only");
     GUARD_BEGIN
     cb::BatchStats bs;
-    b->bt->run(o.max_steps, o.keep_d != 0, bs, o.kernel_timing != 0, o.graph != 0, o.autotune != 0);
+    b->bt->run(o.max_steps, o.keep_d != 0, bs, o.kernel_timing != 0, o.graph != 0, o.autotune != 0,
+               o.fp32_elements != 0);
     if (st) {
         st->pairs = bs.pairs;
         st->pairs_max = bs.pairs_max;

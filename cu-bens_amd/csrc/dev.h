@@ -138,7 +138,8 @@ void launch_frame_tangent_gather(Dev& d, int g);
 // frame-only models: the same tangent written straight into band tiles of
 // tile width 64 (free entries on or below the diagonal; the tiles must be zero
 // elsewhere), w tiles below the diagonal; *bad = 2 for an entry outside
-void launch_frame_tangent_band(Dev& d, int g, double* tiles, int w, int* bad);
+// (fp32: the member tangents in FP32, rounded once into the FP64 band)
+void launch_frame_tangent_band(Dev& d, int g, double* tiles, int w, int* bad, bool fp32 = false);
 void launch_frame_forces(Dev& d, double* f, const double* dd);
 void launch_frame_update(Dev& d, int gx);
 // ANAFLAG 3: the code of the last frame force pass (0, 1 or 2, as forces_fr

@@ -27,18 +27,25 @@ inline int nblk(long n, int tpb = TPB) { return int((n + tpb - 1) / tpb); }
 // I_strong, I_weak, J, C_w and geometric terms from the previous iterate's end
 // forces (P = f[7], My1 = f[4], Mz1 = f[5], T = f[10], My2 = f[11],
 // Mz2 = f[12]) over its deformed length L.
-struct FrameCoef {
-    double ax, s12, s6, s4, s2, w12, w6, w4, w2, t33, t63, t66, t136;
-    double gP, pv, pt, pr, pw, p10, pq, pl, pw30, tl, th;
-    double a, b, my1, my2, c, e, mz1, mz2;
-    double x34, x35, x311, x312, x410, x510, x1011, x1012;
-    double w46, w56, w413, w513, w611, w612, w1113, w1213;
+// T: the arithmetic of the local tangent (double; float for the FP32 element
+// tangent of frame batches)
+template <class T>
+struct FrameCoefT {
+    T ax, s12, s6, s4, s2, w12, w6, w4, w2, t33, t63, t66, t136;
+    T gP, pv, pt, pr, pw, p10, pq, pl, pw30, tl, th;
+    T a, b, my1, my2, c, e, mz1, mz2;
+    T x34, x35, x311, x312, x410, x510, x1011, x1012;
+    T w46, w56, w413, w513, w611, w612, w1113, w1213;
 };
+using FrameCoef = FrameCoefT<double>;
 
-__device__ inline void frame_coef(FrameCoef& k, double E, double G, double A, double L0,
-                                  double Is, double Iw, double J, double Cw, const double* f,
-                                  double L, bool geometric) {
-    const double L2 = L0 * L0, L3 = L0 * L0 * L0;
+template <class T>
+__device__ inline void frame_coef(FrameCoefT<T>& k, double E_, double G_, double A_, double L0_,
+                                  double Is_, double Iw_, double J_, double Cw_, const double* f,
+                                  double L_, bool geometric) {
+    const T E = T(E_), G = T(G_), A = T(A_), L0 = T(L0_), Is = T(Is_), Iw = T(Iw_), J = T(J_), Cw = T(Cw_);
+    const T L = T(L_);
+    const T L2 = L0 * L0, L3 = L0 * L0 * L0;
     k.ax = E * A / L0;
     k.s12 = 12 * E * Is / L3; k.s6 = 6 * E * Is / L2; k.s4 = 4 * E * Is / L0; k.s2 = 2 * E * Is / L0;
     k.w12 = 12 * E * Iw / L3; k.w6 = 6 * E * Iw / L2; k.w4 = 4 * E * Iw / L0; k.w2 = 2 * E * Iw / L0;
@@ -46,13 +53,13 @@ __device__ inline void frame_coef(FrameCoef& k, double E, double G, double A, do
     k.t63 = G * J / 10 + 6 * E * Cw / L2;
     k.t66 = 2 * G * J * L0 / 15 + 4 * E * Cw / L0;
     k.t136 = -(G * J * L0 / 30 - 2 * E * Cw / L0);
-    const double P = geometric ? f[7] : 0.0, My1 = geometric ? f[4] : 0.0;
-    const double Mz1 = geometric ? f[5] : 0.0, T = geometric ? f[10] : 0.0;
-    const double My2 = geometric ? f[11] : 0.0, Mz2 = geometric ? f[12] : 0.0;
+    const T P = T(geometric ? f[7] : 0.0), My1 = T(geometric ? f[4] : 0.0);
+    const T Mz1 = T(geometric ? f[5] : 0.0), Tq = T(geometric ? f[10] : 0.0);
+    const T My2 = T(geometric ? f[11] : 0.0), Mz2 = T(geometric ? f[12] : 0.0);
     k.gP = P / L; k.pv = 6 * P / (5 * L); k.pt = 6 * P * J / (5 * A * L);
     k.pr = 2 * P * L / 15; k.pw = 2 * P * J / (15 * A); k.p10 = P / 10;
     k.pq = P * J / (10 * A); k.pl = P * L / 30; k.pw30 = P * J / (30 * A);
-    k.tl = T / L; k.th = T / 2;
+    k.tl = Tq / L; k.th = Tq / 2;
     k.a = (11 * My1 - My2) / (10 * L); k.b = (My1 - 11 * My2) / (10 * L);
     k.my1 = My1 / 10; k.my2 = My2 / 10;
     k.c = (11 * Mz1 - Mz2) / (10 * L); k.e = (Mz1 - 11 * Mz2) / (10 * L);
@@ -69,7 +76,8 @@ __device__ inline void frame_coef(FrameCoef& k, double E, double G, double A, do
 
 // Entry (i, j) of the symmetric local tangent, DOF order per end u, v, w,
 // theta_x, theta_y, theta_z, warping (end 2 at +7).
-__device__ __forceinline__ double kf(const FrameCoef& k, int i, int j) {
+template <class T>
+__device__ __forceinline__ T kf(const FrameCoefT<T>& k, int i, int j) {
     if (i > j) { int t = i; i = j; j = t; }
     switch (i * 14 + j) {
     case 0 * 14 + 0: case 7 * 14 + 7: return k.ax + k.gP;
@@ -128,7 +136,7 @@ __device__ __forceinline__ double kf(const FrameCoef& k, int i, int j) {
     case 6 * 14 + 12: return k.w612;
     case 11 * 14 + 13: return k.w1113;
     case 12 * 14 + 13: return k.w1213;
-    default: return 0.0;
+    default: return T(0);
     }
 }
 
@@ -235,9 +243,9 @@ __device__ inline void frame_plastic(FramePl& pl, const FrameCoef& k, const doub
 }
 
 // entry (i, j) of the local tangent, less its plastic part when PL
-template <bool PL>
-__device__ __forceinline__ double kent(const FrameCoef& k, const FramePl& pl, int i, int j) {
-    double v = kf(k, i, j);
+template <bool PL, class T = double>
+__device__ __forceinline__ T kent(const FrameCoefT<T>& k, const FramePl& pl, int i, int j) {
+    T v = kf(k, i, j);
     if constexpr (PL)
         v -= pl.u[0][i] * (pl.h[0][0] * pl.u[0][j] + pl.h[0][1] * pl.u[1][j]) +
              pl.u[1][i] * (pl.h[1][0] * pl.u[0][j] + pl.h[1][1] * pl.u[1][j]);
@@ -612,9 +620,9 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent(
 // each thread whose block it touches.
 // U 0 / 1: rows 3U..3U+2 of the block (a triple of end RA) over the columns
 // of end CB; U 2: the warping row of end RA.  BS 7 adds the warping column.
-template <int RA, int CB, int U, bool PL, int BS>
-__device__ __forceinline__ void frame_half_add(double acc[3][7], const FrameCoef& k, const FramePl& pl,
-                                               const double R[3][3]) {
+template <int RA, int CB, int U, bool PL, int BS, class T>
+__device__ __forceinline__ void frame_half_add(T acc[3][7], const FrameCoefT<T>& k, const FramePl& pl,
+                                               const T R[3][3]) {
     if constexpr (U == 2) {
 #pragma unroll
         for (int v = 0; v < 2; ++v) {
@@ -639,7 +647,7 @@ __device__ __forceinline__ void frame_half_add(double acc[3][7], const FrameCoef
 #pragma unroll
     for (int v = 0; v < 2; ++v) {
         const int t1 = 2 * RA + U, t2 = 2 * CB + v;
-        double tmp[3][3];
+        T tmp[3][3];
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -661,7 +669,9 @@ __device__ __forceinline__ void frame_half_add(double acc[3][7], const FrameCoef
 // BandLDLT / BandBatch make from K, with the same values): tile (I, J) of the
 // equation pair (i, j) is tiles[(I (w + 1) + J - I + w) T^2], entry (j % T, i % T)
 // column-major; an entry more than w tiles below the diagonal sets *bad = 2.
-template <bool TO_BAND, bool PL, int BS>
+// T: the arithmetic of the member tangents (float: the FP32 element tangent of
+// frame batches, summed in FP32 and rounded once into K).
+template <bool TO_BAND, bool PL, int BS, class T = double>
 __global__ void __launch_bounds__(TPB) k_frame_tangent_gather(
     const int32_t* __restrict__ gptr, const int32_t* __restrict__ gitem, const int32_t* __restrict__ blkrow,
     const int32_t* __restrict__ colidx, const int32_t* __restrict__ eq, long nnzb,
@@ -677,39 +687,44 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent_gather(
     const long kb = t / NU;
     const int u = int(t % NU);
     if (kb >= nnzb) return;
-    double acc[3][7];
+    T acc[3][7];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 7; ++j) acc[i][j] = 0.0;
+        for (int j = 0; j < 7; ++j) acc[i][j] = T(0);
     for (int32_t it = gptr[kb]; it < gptr[kb + 1]; ++it) {
         const int32_t item = gitem[it];
         const long e = item >> 2;
-        FrameCoef k;
-        frame_coef(k, emod[e], gmod[e], carea[e], llen[e], istrong[e], iweak[e], ipolar[e], iwarp[e], ef + e * 14,
-                   defllen[e], geometric != 0);
+        FrameCoefT<T> k;
+        frame_coef<T>(k, emod[e], gmod[e], carea[e], llen[e], istrong[e], iweak[e], ipolar[e], iwarp[e],
+                      ef + e * 14, defllen[e], geometric != 0);
         FramePl pl;
         if constexpr (PL)
             frame_plastic(pl, k, ef + e * 14, pin.yld[e * 2], pin.yld[e * 2 + 1],
                           yield_ref(carea, pin.yield, pin.zstrong, pin.zweak, e, 0));
-        double R[3][3];
-        load_triad(c1, c2, c3, e * 3, R);
+        double Rd[3][3];
+        load_triad(c1, c2, c3, e * 3, Rd);
+        T R[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) R[r][c] = T(Rd[r][c]);
         switch ((item & 3) * NU + u) {
-        case 0 * NU + 0: frame_half_add<0, 0, 0, PL, BS>(acc, k, pl, R); break;
-        case 0 * NU + 1: frame_half_add<0, 0, 1, PL, BS>(acc, k, pl, R); break;
-        case 1 * NU + 0: frame_half_add<0, 1, 0, PL, BS>(acc, k, pl, R); break;
-        case 1 * NU + 1: frame_half_add<0, 1, 1, PL, BS>(acc, k, pl, R); break;
-        case 2 * NU + 0: frame_half_add<1, 0, 0, PL, BS>(acc, k, pl, R); break;
-        case 2 * NU + 1: frame_half_add<1, 0, 1, PL, BS>(acc, k, pl, R); break;
-        case 3 * NU + 0: frame_half_add<1, 1, 0, PL, BS>(acc, k, pl, R); break;
-        case 3 * NU + 1: frame_half_add<1, 1, 1, PL, BS>(acc, k, pl, R); break;
+        case 0 * NU + 0: frame_half_add<0, 0, 0, PL, BS, T>(acc, k, pl, R); break;
+        case 0 * NU + 1: frame_half_add<0, 0, 1, PL, BS, T>(acc, k, pl, R); break;
+        case 1 * NU + 0: frame_half_add<0, 1, 0, PL, BS, T>(acc, k, pl, R); break;
+        case 1 * NU + 1: frame_half_add<0, 1, 1, PL, BS, T>(acc, k, pl, R); break;
+        case 2 * NU + 0: frame_half_add<1, 0, 0, PL, BS, T>(acc, k, pl, R); break;
+        case 2 * NU + 1: frame_half_add<1, 0, 1, PL, BS, T>(acc, k, pl, R); break;
+        case 3 * NU + 0: frame_half_add<1, 1, 0, PL, BS, T>(acc, k, pl, R); break;
+        case 3 * NU + 1: frame_half_add<1, 1, 1, PL, BS, T>(acc, k, pl, R); break;
         default:
             if constexpr (BS == 7) {
                 switch (item & 3) {
-                case 0: frame_half_add<0, 0, 2, PL, BS>(acc, k, pl, R); break;
-                case 1: frame_half_add<0, 1, 2, PL, BS>(acc, k, pl, R); break;
-                case 2: frame_half_add<1, 0, 2, PL, BS>(acc, k, pl, R); break;
-                default: frame_half_add<1, 1, 2, PL, BS>(acc, k, pl, R); break;
+                case 0: frame_half_add<0, 0, 2, PL, BS, T>(acc, k, pl, R); break;
+                case 1: frame_half_add<0, 1, 2, PL, BS, T>(acc, k, pl, R); break;
+                case 2: frame_half_add<1, 0, 2, PL, BS, T>(acc, k, pl, R); break;
+                default: frame_half_add<1, 1, 2, PL, BS, T>(acc, k, pl, R); break;
                 }
             }
             break;
@@ -733,7 +748,7 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent_gather(
                     *bad = 2;
                     continue;
                 }
-                K[(I * (w + 1) + (J - I + w)) * BT * BT + (qj % BT) * BT + (qi % BT)] = acc[i][j];
+                K[(I * (w + 1) + (J - I + w)) * BT * BT + (qj % BT) * BT + (qi % BT)] = double(acc[i][j]);
             }
         }
     } else {
@@ -745,7 +760,7 @@ __global__ void __launch_bounds__(TPB) k_frame_tangent_gather(
 #pragma unroll
             for (int j = 0; j < BS; ++j) {
                 const bool colfix = eq[B * BS + j] < 0;
-                Kb[i * BS + j] = (rowfix || colfix) ? ((A == B && u * 3 + i == j) ? 1.0 : 0.0) : acc[i][j];
+                Kb[i * BS + j] = (rowfix || colfix) ? ((A == B && u * 3 + i == j) ? 1.0 : 0.0) : double(acc[i][j]);
             }
         }
     }
@@ -1280,14 +1295,18 @@ void launch_frame_tangent_gather(Dev& d, int g) {
     CB_CHECK(hipGetLastError());
 }
 
-void launch_frame_tangent_band(Dev& d, int g, double* tiles, int w, int* bad) {
+void launch_frame_tangent_band(Dev& d, int g, double* tiles, int w, int* bad, bool fp32) {
     if (d.bs != 6 || d.ntr || d.nsh) throw DeviceError("frame tangent into the band: frame-only models, warping fixed");
     if (d.anaflag == 3) throw DeviceError("frame tangent into the band: elastic (ANAFLAG 2) members only");
     if (has_xtr(d)) throw DeviceError("frame tangent into the band: members without end offsets or releases only");
-    hipLaunchKernelGGL((k_frame_tangent_gather<true, false, 6>), dim3(nblk(2 * d.nnzb)), dim3(TPB), 0, d.s,
-                       d.gat_ptr_fr, d.gat_item_fr, d.blkrow, d.colidx, d.eq_of_slot, d.nnzb, d.c1[g], d.c2[g],
-                       d.c3[g], d.emod, d.gmod, d.carea, d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp,
-                       d.defllen[g], d.ef[g], tiles, int(d.anaflag == 2), w, bad, PlasticIn{});
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(nblk(2 * d.nnzb)), dim3(TPB), 0, d.s, d.gat_ptr_fr, d.gat_item_fr, d.blkrow,
+                           d.colidx, d.eq_of_slot, d.nnzb, d.c1[g], d.c2[g], d.c3[g], d.emod, d.gmod, d.carea,
+                           d.llen, d.istrong, d.iweak, d.ipolar, d.iwarp, d.defllen[g], d.ef[g], tiles,
+                           int(d.anaflag == 2), w, bad, PlasticIn{});
+    };
+    if (fp32) go(k_frame_tangent_gather<true, false, 6, float>);
+    else go(k_frame_tangent_gather<true, false, 6, double>);
     CB_CHECK(hipGetLastError());
 }
 

@@ -36,7 +36,7 @@ class Options(C.Structure):
                 ("keep_d", C.c_int), ("kernel_timing", C.c_int), ("factor_schedule", C.c_int),
                 ("kernel_timing_every", C.c_int), ("sweep_schedule", C.c_int), ("keep_forces", C.c_int),
                 ("checkpoint_path", C.c_char_p), ("checkpoint_exact", C.c_int), ("graph", C.c_int),
-                ("refine", C.c_int), ("autotune", C.c_int)]
+                ("refine", C.c_int), ("autotune", C.c_int), ("fp32_elements", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -66,14 +66,15 @@ def options(solver: int = SOLVER_BAND64, krylov_tol: float = 1e-8, restart: int 
             maxit: int = 20000, judge: int = 1, max_steps: int = 1 << 30, keep_d: int = 1,
             kernel_timing: int = 0, factor_schedule: int = 0, kernel_timing_every: int = 1,
             sweep_schedule: int = 0, keep_forces: int = 0, checkpoint_path: str | None = None,
-            checkpoint_exact: int = 0, graph: int = 0, refine: int = 0, autotune: int = 0) -> Options:
+            checkpoint_exact: int = 0, graph: int = 0, refine: int = 0, autotune: int = 0,
+            fp32_elements: int = 0) -> Options:
     """cb_options; checkpoint_path is the Newmark checkpoint file (written every
     CHKPT time steps, read first when the model has RFLAG 1); refine the
     iterative refinement steps of the FP32 precision mode's solves."""
     return Options(solver, krylov_tol, restart, maxit, judge, max_steps, keep_d, kernel_timing,
                    factor_schedule, kernel_timing_every, sweep_schedule, keep_forces,
                    checkpoint_path.encode() if checkpoint_path else None, checkpoint_exact, graph, refine,
-                   autotune)
+                   autotune, fp32_elements)
 
 
 _lib = None
@@ -425,12 +426,13 @@ class GpuBatch:
             pass
 
     def run(self, max_steps: int = 1 << 30, keep_d: bool = True, timing: bool = False,
-            graph: bool = False, autotune: bool = False) -> dict:
+            graph: bool = False, autotune: bool = False, fp32_elements: bool = False) -> dict:
         """cb_batch_run; graph replays each round's Newton iteration as one HIP
-        graph; autotune tunes the band factor's helper pairs on the first run."""
+        graph; autotune tunes the band factor's helper pairs on the first run;
+        fp32_elements forms the member tangents in FP32."""
         st = BatchStats()
         opt = options(max_steps=max_steps, keep_d=int(keep_d), kernel_timing=int(timing), graph=int(graph),
-                      autotune=int(autotune))
+                      autotune=int(autotune), fp32_elements=int(fp32_elements))
         _check(lib().cb_batch_run(self.h, C.byref(opt), C.byref(st)))
         status = np.zeros(self.count, np.int32)
         nsteps = np.zeros(self.count, np.int32)

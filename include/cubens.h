@@ -83,6 +83,9 @@ typedef struct {
      * process, and in the file named by the environment variable
      * CUBENS_TUNE_FILE when set); 0 uses the most pairs that fit */
     int autotune;
+    /* batches: 1 forms the frame member tangents in FP32 (rounded once into
+     * the FP64 band); internal forces, residual, factor and solve stay FP64 */
+    int fp32_elements;
 } cb_options;
 
 /* Schedules of the band LDL^T factorization (same tile arithmetic, bitwise

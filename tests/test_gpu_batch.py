@@ -243,3 +243,38 @@ def test_autotuned_helper_pairs_keep_every_path(tmp_path, monkeypatch):
     with cubens.GpuBatch(paths) as b2:
         st2 = b2.run(max_steps=2, autotune=True)
         assert b2.tuning() == [] and st2["pairs"] == kept
+
+
+def test_fp32_element_tangents_follow_the_fp64_paths():
+    """The member tangents in FP32 (rounded once into the FP64 band; internal
+    forces, residual, factor and solve FP64): every model runs its steps with
+    the FP64 batch's step count and times, Newton iterations within 10%, and
+    the models whose Newton loops converge keep the FP64 path's displacements
+    to 1e-9 (measured 2e-12; the error is printed), launched one by one and as
+    a graph captured for the FP32 tangent.  The models with itemax 1 accept
+    the state after a single correction, which follows the tangent itself:
+    for them only the steps and times are compared."""
+    steps = 8
+    paths = _mixed_files(steps)
+    with cubens.GpuBatch(paths) as b:
+        ref_st = b.run(max_steps=steps)
+        ref = [b.path(i) for i in range(b.count)]
+        for graph in (False, True):
+            st = b.run(max_steps=steps, fp32_elements=True, graph=graph)
+            assert np.array_equal(st["status"], ref_st["status"])
+            assert np.array_equal(st["nsteps"], ref_st["nsteps"])
+            errs = []
+            for i in range(b.count):
+                p = b.path(i)
+                assert np.array_equal(p["lpf"], ref[i]["lpf"])          # the step times
+                ni, nr = int(p["itecnt"].sum()), int(ref[i]["itecnt"].sum())
+                assert abs(ni - nr) <= max(1, nr // 10), (i, ni, nr)
+                if MIXED[i].get("itemax", 20) > 1:
+                    errs.append(float(np.abs(p["d"] - ref[i]["d"]).max() / np.abs(ref[i]["d"]).max()))
+            print(f"[batch] fp32 elements (graph {graph}): Newton {st['newton']} (FP64 {ref_st['newton']}), "
+                  f"worst displacement error of the converging models {max(errs):.2e}", flush=True)
+            assert len(errs) == 5 and max(errs) < 1e-9
+        # and back to the FP64 tangent: the FP64 paths bitwise
+        b.run(max_steps=steps, graph=True)
+        for i in range(b.count):
+            assert np.array_equal(b.path(i)["d"], ref[i]["d"])
