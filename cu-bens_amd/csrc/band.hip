@@ -1231,7 +1231,9 @@ __global__ void __launch_bounds__(256) k_look2_rhs(const R* __restrict__ linv, c
 // y_I = b'_I - sum_{J=I-w-1}^{I-2} L'_IJ y_J over the rows in chunks of NCH,
 // polled side by side, aligned so that the last chunk ends at row I-2: a row
 // waits once for the newest rows it needs and the two chains of the
-// look-ahead system (even and odd rows) run side by side.  y must hold SENT.
+// look-ahead system (even and odd rows) run side by side.  The tiles of the
+// next chunk are loaded while the current one is polled, so that the products
+// after a chunk's rows land read registers only.  y must hold SENT.
 template <class R>
 __global__ void __launch_bounds__(256) k_fwd_sweep_2(const R* __restrict__ l2, const double* __restrict__ bp,
                                                      double* y, long nt, int w, long spin_limit, int* err) {
@@ -1244,15 +1246,20 @@ __global__ void __launch_bounds__(256) k_fwd_sweep_2(const R* __restrict__ l2, c
     const long nrow = I - 1 - J0;   // rows J0 .. I-2
     int nc = int(nrow % NCH);
     if (nc == 0) nc = NCH;
-    for (long J = J0; J <= I - 2; J += nc, nc = NCH) {
-        R lx[NCH][16];
+    auto load = [&](long J, int n, R (*dst)[16]) {
 #pragma unroll
         for (int c = 0; c < NCH; ++c)
-            if (c < nc) {
+            if (c < n) {
                 const R* l = l2 + (I * w + (I - 2 - (J + c))) * (long)TT;
 #pragma unroll
-                for (int k = 0; k < 16; ++k) lx[c][k] = l[(k0 + k) * T + row];
+                for (int k = 0; k < 16; ++k) dst[c][k] = l[(k0 + k) * T + row];
             }
+    };
+    R lx[NCH][16], nx[NCH][16];
+    if (nrow > 0) load(J0, nc, lx);
+    for (long J = J0; J <= I - 2; J += nc, nc = NCH) {
+        const long Jn = J + nc;
+        if (Jn <= I - 2) load(Jn, NCH, nx);
         fetch_rows(y, J, 1, nc, vch, spin_limit, err);
 #pragma unroll
         for (int c = 0; c < NCH; ++c)
@@ -1261,6 +1268,10 @@ __global__ void __launch_bounds__(256) k_fwd_sweep_2(const R* __restrict__ l2, c
                 for (int k = 0; k < 16; ++k) acc -= double(lx[c][k]) * vch[c][k0 + k];
             }
         __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+#pragma unroll
+            for (int k = 0; k < 16; ++k) lx[c][k] = nx[c][k];
     }
     part[q][row] = acc;
     __syncthreads();
