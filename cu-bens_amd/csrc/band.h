@@ -69,6 +69,10 @@ struct BandLDLT {
     R* l2 = nullptr;
     R* w2 = nullptr;
     double* bp = nullptr;
+    // the backward sweep's look-ahead system: B [nt][w] and V [nt] tiles, z' [nt * 64]
+    R* b2 = nullptr;
+    R* v2 = nullptr;
+    double* zp = nullptr;
     double* xw = nullptr;    // work vectors (padded)
     double* yw = nullptr;
     int* flag = nullptr;       // error bits: 1 a wait bound hit, 2 an entry outside the band

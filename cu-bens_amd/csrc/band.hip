@@ -1278,6 +1278,142 @@ __global__ void __launch_bounds__(256) k_fwd_sweep_2(const R* __restrict__ l2, c
     if (q == 0) put_value(&y[I * T + row], part[0][row] + part[1][row] + part[2][row] + part[3][row]);
 }
 
+// The backward sweep x_I = Linv_I^T (z_I - sum_{J>I} L_JI^T x_J), z = y / d,
+// on the same look-ahead system: with V_I = G_I^T Linv_{I+1}^T the coupling
+// to x_{I+1} cancels and
+//   x_I = z'_I - sum_{J=I+2}^{I+w+1} B_IJ x_J,
+//   z'_I = Linv_I^T z_I - V_I z_{I+1},  B_IJ = Linv_I^T L_JI^T - V_I L_{J,I+1}^T,
+// B kept per row in w slots, slot s for J = I+2+s (column-major tiles).
+
+// V_I (I <= nt-2): X = G_I^T (G_I row-major: xs[k][row] = G_I[k][row]),
+// Y = Linv_{I+1} (ys[k][col] = Linv_{I+1}[col][k], column-major storage)
+template <class R>
+__global__ void __launch_bounds__(NTH) k_look2_v(const R* __restrict__ gch, const R* __restrict__ linv,
+                                                 R* __restrict__ v2, long nt) {
+    __shared__ R xs[T][T];
+    __shared__ R ys[T][T];
+    const long I = blockIdx.x;
+    stage<false>(xs, gch + I * (long)TT);
+    stage<false>(ys, linv + (I + 1) * (long)TT);
+    __syncthreads();
+    TileAcc<R> acc;
+    acc.product(xs, ys);
+    acc.each([&](int row, int col, R v) { v2[I * (long)TT + col * T + row] = v; });
+}
+
+// B_{I, I+2+s} for workgroup (I, s): X = Linv_I^T (a transposed staging of the
+// column-major tile) with Y = L_JI, less X = V_I with Y = L_{J,I+1}; the L
+// tiles are column-major, so ys[k][col] = L[col][k] stages straight.
+template <class R>
+__global__ void __launch_bounds__(NTH) k_look2_bband(const R* __restrict__ tiles, const R* __restrict__ linv,
+                                                     const R* __restrict__ v2, R* __restrict__ b2, long nt,
+                                                     int w) {
+    __shared__ R xs[T][T];
+    __shared__ R ys[T][T];
+    const long I = blockIdx.x / w;
+    const int s = int(blockIdx.x % w);
+    const long J = I + 2 + s;
+    if (J >= nt) return;
+    TileAcc<R> a, b;
+    const bool direct = J - I <= w;
+    if (direct) {
+        const R* lv = linv + I * (long)TT;
+        for (int idx = threadIdx.x; idx < TT; idx += NTH) xs[idx % T][idx / T] = lv[idx];
+        stage<false>(ys, tiles + tile_index(J, I, w) * TT);
+        __syncthreads();
+        a.product(xs, ys);
+        __syncthreads();
+    }
+    stage<false>(xs, v2 + I * (long)TT);
+    stage<false>(ys, tiles + tile_index(J, I + 1, w) * TT);
+    __syncthreads();
+    b.product(xs, ys);
+    R* out = b2 + (I * w + s) * (long)TT;
+    if (direct) {
+        R av[TileAcc<R>::N];
+        a.each_i([&](int, int, R v, int e) { av[e] = v; });
+        b.each_i([&](int row, int col, R v, int e) { out[col * T + row] = av[e] - v; });
+    } else {
+        b.each([&](int row, int col, R v) { out[col * T + row] = -v; });
+    }
+}
+
+// z'_I = Linv_I^T z_I - V_I z_{I+1}, z = y / d, one workgroup per tile row
+template <class R>
+__global__ void __launch_bounds__(256) k_look2_brhs(const R* __restrict__ linv, const R* __restrict__ v2,
+                                                    const double* __restrict__ y, const double* __restrict__ piv,
+                                                    double* __restrict__ zp, long nt) {
+    __shared__ double part[4][T];
+    const long I = blockIdx.x;
+    const int tid = threadIdx.x, row = tid % T, q = tid / T, k0 = 16 * q;
+    const R* lv = linv + I * (long)TT;
+    double u = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const long e = I * T + k0 + k;
+        u += double(lv[row * T + k0 + k]) * (y[e] / piv[e]);   // Linv_I[k][row]
+    }
+    if (I + 1 < nt) {
+        const R* vv = v2 + I * (long)TT;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const long e = (I + 1) * T + k0 + k;
+            u -= double(vv[(k0 + k) * T + row]) * (y[e] / piv[e]);
+        }
+    }
+    part[q][row] = u;
+    __syncthreads();
+    if (tid < T) zp[I * T + tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+}
+
+// x_I = z'_I - sum_{J=I+2}^{I+w+1} B_IJ x_J, I = nt-1-blockIdx.x: the later
+// rows in chunks of NCH polled side by side, farthest first, aligned so that
+// the last chunk ends at row I+2; the next chunk's tiles load while the
+// current one is polled.  x must hold SENT.
+template <class R>
+__global__ void __launch_bounds__(256) k_bwd_sweep_2(const R* __restrict__ b2, const double* __restrict__ zp,
+                                                     double* x, long nt, int w, long spin_limit, int* err) {
+    __shared__ double vch[NCH][T];
+    __shared__ double part[4][T];
+    const long I = nt - 1 - blockIdx.x;
+    const int tid = threadIdx.x, row = tid % T, q = tid / T, k0 = 16 * q;
+    double acc = (q == 0) ? zp[I * T + row] : 0.0;
+    const long J1 = (I + w + 1 < nt - 1) ? I + w + 1 : nt - 1;
+    const long nrow = J1 - (I + 1);   // rows J1 down to I+2
+    int nc = int(nrow % NCH);
+    if (nc == 0) nc = NCH;
+    auto load = [&](long J, int n, R (*dst)[16]) {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            if (c < n) {
+                const R* l = b2 + (I * w + (J - c - I - 2)) * (long)TT;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) dst[c][k] = l[(k0 + k) * T + row];
+            }
+    };
+    R lx[NCH][16], nx[NCH][16];
+    if (nrow > 0) load(J1, nc, lx);
+    for (long J = J1; J >= I + 2; J -= nc, nc = NCH) {
+        const long Jn = J - nc;
+        if (Jn >= I + 2) load(Jn, NCH, nx);
+        fetch_rows(x, J, -1, nc, vch, spin_limit, err);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+            if (c < nc) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) acc -= double(lx[c][k]) * vch[c][k0 + k];
+            }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+#pragma unroll
+            for (int k = 0; k < 16; ++k) lx[c][k] = nx[c][k];
+    }
+    part[q][row] = acc;
+    __syncthreads();
+    if (q == 0) put_value(&x[I * T + row], part[0][row] + part[1][row] + part[2][row] + part[3][row]);
+}
+
 // v = SENT
 __global__ void k_fill_sent(double* __restrict__ v, long n) {
     const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -1763,12 +1899,13 @@ void BandLDLT<R>::setup(long n_, long bandwidth, hipStream_t s_) {
 
 template <class R>
 void BandLDLT<R>::release() {
-    void* ps[] = {tiles, tiles_t, linv, mch, gch, piv, xw, yw, flag, pflags, l2, w2, bp};
+    void* ps[] = {tiles, tiles_t, linv, mch, gch, piv, xw, yw, flag, pflags, l2, w2, bp, b2, v2, zp};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     tiles = nullptr; tiles_t = nullptr; linv = nullptr; piv = nullptr; xw = nullptr; yw = nullptr; flag = nullptr;
     mch = nullptr; gch = nullptr;
     l2 = nullptr; w2 = nullptr; bp = nullptr;
+    b2 = nullptr; v2 = nullptr; zp = nullptr;
     pflags = nullptr;
     clock.release();
 }
@@ -1840,9 +1977,14 @@ int BandLDLT<R>::factor(const Dev& d) {
                 l2 = dalloc<R>(size_t(nt) * size_t(w) * TT);
                 w2 = dalloc<R>(size_t(nt) * TT);
                 bp = dalloc<double>(size_t(nt) * T);
+                b2 = dalloc<R>(size_t(nt) * size_t(w) * TT);
+                v2 = dalloc<R>(size_t(nt) * TT);
+                zp = dalloc<double>(size_t(nt) * T);
             }
             hipLaunchKernelGGL((k_look2_w<R>), dim3(int(nt - 1)), dim3(NTH), 0, s, mch, linv, w2, nt);
             hipLaunchKernelGGL((k_look2_band<R>), dim3(int(nt * w)), dim3(NTH), 0, s, tiles_t, linv, w2, l2, nt, w);
+            hipLaunchKernelGGL((k_look2_v<R>), dim3(int(nt - 1)), dim3(NTH), 0, s, gch, linv, v2, nt);
+            hipLaunchKernelGGL((k_look2_bband<R>), dim3(int(nt * w)), dim3(NTH), 0, s, tiles, linv, v2, b2, nt, w);
         }
         if (tk_prep) clock.end(s);
         CB_CHECK(hipGetLastError());
@@ -1880,9 +2022,13 @@ void BandLDLT<R>::solve(const double* rhs, double* out) {
         hipLaunchKernelGGL((k_fwd_sweep<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, xw, yw, nt, w, spin_limit,
                            flag);
     if (tk_fwd) clock.end(s);
+    if (ahead)
+        hipLaunchKernelGGL((k_look2_brhs<R>), dim3(int(nt)), dim3(256), 0, s, linv, v2, yw, piv, zp, nt);
     const bool tk_bwd = clock.take(KT_BWD);
     if (tk_bwd) clock.begin(KT_BWD, s);
-    if (chained)
+    if (ahead)
+        hipLaunchKernelGGL((k_bwd_sweep_2<R>), dim3(int(nt)), dim3(256), 0, s, b2, zp, xw, nt, w, spin_limit, flag);
+    else if (chained)
         hipLaunchKernelGGL((k_bwd_sweep_m<R>), dim3(int(nt)), dim3(256), 0, s, tiles_t, linv, gch, yw, piv, xw, nt, w,
                            spin_limit, flag);
     else

@@ -53,7 +53,7 @@ struct RunOptions {
     int factor_schedule = 0;  // 0 fused launch per tile column, 1 split diag/panel/update launches
     int kernel_timing_every = 1;  // time every n-th launch of a timed family
     int sweep_schedule = 0;       // 0 sweeps with the chain matrices, 1 the two-product row step,
-                                  // 2 chain matrices with the forward sweep two rows ahead
+                                  // 2 both sweeps on the look-ahead system (rows two back)
     int keep_forces = 0;          // record the averaged element forces of every converged step
     // Newmark checkpoint file (checkPoint / restartStep): written every CHKPT
     // time steps when set, read first when the model has RFLAG 1; exact 1

@@ -97,9 +97,11 @@ enum { CB_FACTOR_FUSED = 0, CB_FACTOR_SPLIT = 1 };
 /* Schedules of the band solves' forward and backward sweeps (the same
  * solution up to rounding), all one tile row per workgroup: a chain step of
  * one 64x64 product with the matrix Linv_I L_{I,I-1} formed after each
- * factorization (default), of two dependent 64x64 products, or the chain
- * matrices with the forward sweep on a look-ahead system whose rows depend on
- * the row two back (its band formed after each factorization). */
+ * factorization (default), of two dependent 64x64 products, or both sweeps on
+ * a look-ahead system whose rows depend on the rows two and more back, its
+ * band (twice the memory of the factor's) formed after each factorization:
+ * faster sweeps for the drivers that solve many times per factorization
+ * (arc length, modified NR). */
 enum { CB_SWEEP_CHAIN = 0, CB_SWEEP_ROWS = 1, CB_SWEEP_LOOK2 = 2 };
 
 /* Generations of the per-element state (base = last converged, _i = current
