@@ -64,7 +64,7 @@ PEAK_HBM_GBS = 8000.0
 TILE = 64
 
 
-def kernel_model(info) -> dict:
+def kernel_model(info, sweeps: str = "chain") -> dict:
     """Algorithmic work of one launch of each band kernel family at this model.
 
     nt tile rows of 64, w sub-diagonal tiles (w = ceil(bandwidth/64)); column K
@@ -78,7 +78,9 @@ def kernel_model(info) -> dict:
       sweeps : every off-diagonal factor tile and every Linv tile read once (8 B
                entries), the right-hand side read and the solution written
       prep   : the sweeps' chain matrices, two tile products of 2 T^3 per
-               tile row
+               tile row; with the look-ahead sweeps also W_I and V_I (one
+               product each) and the two look-ahead bands (two products per
+               sub-diagonal tile each)
     """
     nt = (info.neq + TILE - 1) // TILE
     w = min((info.bandwidth + TILE - 1) // TILE, nt - 1)
@@ -93,7 +95,8 @@ def kernel_model(info) -> dict:
         "band_update": ("mfma", 2.0 * t3 * w * (w + 1) / 2, "TFLOP/s"),
         "band_fwd_sweep": ("hbm", float(sweep_bytes), "GB/s"),
         "band_bwd_sweep": ("hbm", float(sweep_bytes), "GB/s"),
-        "band_sweep_prep": ("mfma", 2 * 2.0 * t3 * nt, "TFLOP/s"),
+        "band_sweep_prep": ("mfma", 2.0 * t3 * (2 * nt + (2 * nt + 4 * offd if sweeps == "look2" else 0)),
+                            "TFLOP/s"),
     }
 
 
@@ -121,10 +124,10 @@ def skyline_work(kht: np.ndarray) -> dict:
     return {"factor_flops": float((k * k).sum()), "sweep_bytes": 8.0 * float((k + 1).sum())}
 
 
-def roofline(info, ktimes: dict, default_workload: bool, sky: dict | None = None) -> dict:
+def roofline(info, ktimes: dict, default_workload: bool, sky: dict | None = None, sweeps: str = "chain") -> dict:
     name = max(ktimes, key=lambda k: ktimes[k][0])
     ms, launches = ktimes[name]
-    bound, work, unit = kernel_model(info)[name]
+    bound, work, unit = kernel_model(info, sweeps)[name]
     nt = (info.neq + TILE - 1) // TILE
     if sky is not None and name == "band_column":
         work = sky["factor_flops"] / nt          # a factorization is nt column launches
@@ -542,7 +545,7 @@ def main():
     # the counters were read on the 256x256 panel's FP64 band with the fused schedule
     roof = roofline(head["info"], head["ktimes"],
                     default_workload=(args.n == 256 and plain is not None and args.schedule == "fused"),
-                    sky=sky)
+                    sky=sky, sweeps=args.sweeps)
     log(f"[bench] kernel times (ms, launches): {head['ktimes']}")
     cpu = None
     if not args.no_cpu:

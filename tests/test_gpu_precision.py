@@ -85,6 +85,23 @@ def test_fp32_precision_mode_panel_msal_path(n, steps, refine):
     assert relerr(r32["d"], o["d"]) < 1e-5
 
 
+@pytest.mark.parametrize("sweeps", [1, 2])
+def test_fp32_precision_mode_under_every_sweep_schedule(sweeps):
+    """The FP32 band's solves on the two-product row step and on the
+    look-ahead system (both sweeps coupled two rows back) against the chain
+    matrices: the same converged steps, the displacements to 1e-5 (the bar of
+    the FP32 paths above; the schedules differ only in rounding)."""
+    path = panel(8)
+    with cubens.GpuModel(path) as g:
+        a = g.run(cubens.options(cubens.SOLVER_BAND32, max_steps=12, judge=0, refine=1, sweep_schedule=0))
+        b = g.run(cubens.options(cubens.SOLVER_BAND32, max_steps=12, judge=0, refine=1, sweep_schedule=sweeps))
+    e = relerr(b["d"], a["d"])
+    print(f"[precision] FP32 sweeps {sweeps} against the chain matrices: d error {e:.2e}", flush=True)
+    assert a["status"] == b["status"] == 0, b["msg"]
+    assert a["nsteps"] == b["nsteps"] == 12
+    assert e < 1e-5 and relerr(b["lpf"], a["lpf"]) < 1e-5
+
+
 def test_fp32_precision_mode_with_the_fp64_judge_pivots():
     """judge 1 takes det and psi from an FP64 factor of the FP64 tangent, as
     the other reduced-precision solvers do, and gives the same path."""
