@@ -95,7 +95,8 @@ def kernel_model(info, sweeps: str = "chain") -> dict:
         "band_update": ("mfma", 2.0 * t3 * w * (w + 1) / 2, "TFLOP/s"),
         "band_fwd_sweep": ("hbm", float(sweep_bytes), "GB/s"),
         "band_bwd_sweep": ("hbm", float(sweep_bytes), "GB/s"),
-        "band_sweep_prep": ("mfma", 2.0 * t3 * (2 * nt + (2 * nt + 4 * offd if sweeps == "look2" else 0)),
+        "band_sweep_prep": ("mfma", 2.0 * t3 * (2 * nt + (2 * nt + 4 * offd if sweeps in ("look2", "look3") else 0)
+                                                + (2 * (offd + nt) if sweeps == "look3" else 0)),
                             "TFLOP/s"),
     }
 
@@ -461,7 +462,7 @@ def main():
                     help="panel: the 256x256 shell panel MSAL (the headline); frames: the 512-frame "
                          "nonlinear Newmark batch (--steps time steps); learned: the learned "
                          "preconditioner against block-Jacobi on held-out panels")
-    ap.add_argument("--sweeps", default="look2", choices=("chain", "rows", "look2"),
+    ap.add_argument("--sweeps", default="look2", choices=("chain", "rows", "look2", "look3"),
                     help="band solve sweeps: chain matrices (the library's default), the two-product "
                          "row step, or both sweeps on the look-ahead system (rows coupled two back; "
                          "its band formed after each factorization pays back over the ~13 solves "
@@ -504,7 +505,7 @@ def main():
         log(f"[bench] path {name}: {w} warm-up increments, then increments 1..{k} timed")
         res = run_path(path, solver, w, k, args.tol, timing=127 if name == "fp64" else 0, schedule=schedule,
                        refine=1 if name == "fp32" else 0,
-                       sweeps={"chain": 0, "rows": 1, "look2": 2}[args.sweeps])
+                       sweeps={"chain": 0, "rows": 1, "look2": 2, "look3": 3}[args.sweeps])
         st = res["stats"]
         log(f"[bench] {name}: {res['rate']:.2f} Newton it/s ({res['newton']} its in {res['seconds']:.2f}s); "
             f"tangent {st['t_tangent']:.2f}s factor {st['t_factor']:.2f}s solve {st['t_solve']:.2f}s "

@@ -65,7 +65,7 @@ struct BandLDLT {
     // the forward sweep two rows ahead (with chained): L' and W of the
     // look-ahead system [nt][w] and [nt] tiles, b' [nt * 64]; allocated at the
     // first factorization that needs them
-    bool look2 = false;
+    int look = 0;            // look-ahead depth of the sweeps (with chained): 0 none, 2 or 3
     R* l2 = nullptr;
     R* w2 = nullptr;
     double* bp = nullptr;
@@ -73,6 +73,12 @@ struct BandLDLT {
     R* b2 = nullptr;
     R* v2 = nullptr;
     double* zp = nullptr;
+    // depth 3: both systems deepened by one row ([nt][w+1] tiles each) and
+    // their right-hand sides
+    R* l3 = nullptr;
+    R* b3 = nullptr;
+    double* bq = nullptr;
+    double* zq = nullptr;
     double* xw = nullptr;    // work vectors (padded)
     double* yw = nullptr;
     int* flag = nullptr;       // error bits: 1 a wait bound hit, 2 an entry outside the band

@@ -1228,38 +1228,39 @@ __global__ void __launch_bounds__(256) k_look2_rhs(const R* __restrict__ linv, c
     if (tid < T) bp[I * T + tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
 }
 
-// y_I = b'_I - sum_{J=I-w-1}^{I-2} L'_IJ y_J over the rows in chunks of NCH,
-// polled side by side, aligned so that the last chunk ends at row I-2: a row
-// waits once for the newest rows it needs and the two chains of the
-// look-ahead system (even and odd rows) run side by side.  The tiles of the
-// next chunk are loaded while the current one is polled, so that the products
+// y_I = b'_I - sum_{J=I-w-D+1}^{I-D} L'_IJ y_J on a look-ahead system of
+// depth D (slot s of row I for J = I-D-s, w slots), over the rows in chunks of
+// NCH, polled side by side, aligned so that the last chunk ends at row I-D: a
+// row waits once for the newest rows it needs and the D chains of the system
+// (the rows of each residue mod D) run side by side.  The tiles of the next
+// chunk are loaded while the current one is polled, so that the products
 // after a chunk's rows land read registers only.  y must hold SENT.
 template <class R>
-__global__ void __launch_bounds__(256) k_fwd_sweep_2(const R* __restrict__ l2, const double* __restrict__ bp,
-                                                     double* y, long nt, int w, long spin_limit, int* err) {
+__global__ void __launch_bounds__(256) k_fwd_sweep_la(const R* __restrict__ l2, const double* __restrict__ bp,
+                                                      double* y, long nt, int w, int D, long spin_limit, int* err) {
     __shared__ double vch[NCH][T];
     __shared__ double part[4][T];
     const long I = blockIdx.x;
     const int tid = threadIdx.x, row = tid % T, q = tid / T, k0 = 16 * q;
     double acc = (q == 0) ? bp[I * T + row] : 0.0;
-    const long J0 = I - w - 1 > 0 ? I - w - 1 : 0;
-    const long nrow = I - 1 - J0;   // rows J0 .. I-2
+    const long J0 = I - w - D + 1 > 0 ? I - w - D + 1 : 0;
+    const long nrow = I - D + 1 - J0;   // rows J0 .. I-D
     int nc = int(nrow % NCH);
     if (nc == 0) nc = NCH;
     auto load = [&](long J, int n, R (*dst)[16]) {
 #pragma unroll
         for (int c = 0; c < NCH; ++c)
             if (c < n) {
-                const R* l = l2 + (I * w + (I - 2 - (J + c))) * (long)TT;
+                const R* l = l2 + (I * w + (I - D - (J + c))) * (long)TT;
 #pragma unroll
                 for (int k = 0; k < 16; ++k) dst[c][k] = l[(k0 + k) * T + row];
             }
     };
     R lx[NCH][16], nx[NCH][16];
     if (nrow > 0) load(J0, nc, lx);
-    for (long J = J0; J <= I - 2; J += nc, nc = NCH) {
+    for (long J = J0; J <= I - D; J += nc, nc = NCH) {
         const long Jn = J + nc;
-        if (Jn <= I - 2) load(Jn, NCH, nx);
+        if (Jn <= I - D) load(Jn, NCH, nx);
         fetch_rows(y, J, 1, nc, vch, spin_limit, err);
 #pragma unroll
         for (int c = 0; c < NCH; ++c)
@@ -1366,36 +1367,37 @@ __global__ void __launch_bounds__(256) k_look2_brhs(const R* __restrict__ linv, 
     if (tid < T) zp[I * T + tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
 }
 
-// x_I = z'_I - sum_{J=I+2}^{I+w+1} B_IJ x_J, I = nt-1-blockIdx.x: the later
-// rows in chunks of NCH polled side by side, farthest first, aligned so that
-// the last chunk ends at row I+2; the next chunk's tiles load while the
-// current one is polled.  x must hold SENT.
+// x_I = z'_I - sum_{J=I+D}^{I+D+w-1} B_IJ x_J on a look-ahead system of depth
+// D (slot s of row I for J = I+D+s), I = nt-1-blockIdx.x: the later rows in
+// chunks of NCH polled side by side, farthest first, aligned so that the last
+// chunk ends at row I+D; the next chunk's tiles load while the current one is
+// polled.  x must hold SENT.
 template <class R>
-__global__ void __launch_bounds__(256) k_bwd_sweep_2(const R* __restrict__ b2, const double* __restrict__ zp,
-                                                     double* x, long nt, int w, long spin_limit, int* err) {
+__global__ void __launch_bounds__(256) k_bwd_sweep_la(const R* __restrict__ b2, const double* __restrict__ zp,
+                                                      double* x, long nt, int w, int D, long spin_limit, int* err) {
     __shared__ double vch[NCH][T];
     __shared__ double part[4][T];
     const long I = nt - 1 - blockIdx.x;
     const int tid = threadIdx.x, row = tid % T, q = tid / T, k0 = 16 * q;
     double acc = (q == 0) ? zp[I * T + row] : 0.0;
-    const long J1 = (I + w + 1 < nt - 1) ? I + w + 1 : nt - 1;
-    const long nrow = J1 - (I + 1);   // rows J1 down to I+2
+    const long J1 = (I + D + w - 1 < nt - 1) ? I + D + w - 1 : nt - 1;
+    const long nrow = J1 - (I + D) + 1;   // rows J1 down to I+D
     int nc = int(nrow % NCH);
     if (nc == 0) nc = NCH;
     auto load = [&](long J, int n, R (*dst)[16]) {
 #pragma unroll
         for (int c = 0; c < NCH; ++c)
             if (c < n) {
-                const R* l = b2 + (I * w + (J - c - I - 2)) * (long)TT;
+                const R* l = b2 + (I * w + (J - c - I - D)) * (long)TT;
 #pragma unroll
                 for (int k = 0; k < 16; ++k) dst[c][k] = l[(k0 + k) * T + row];
             }
     };
     R lx[NCH][16], nx[NCH][16];
     if (nrow > 0) load(J1, nc, lx);
-    for (long J = J1; J >= I + 2; J -= nc, nc = NCH) {
+    for (long J = J1; J >= I + D; J -= nc, nc = NCH) {
         const long Jn = J - nc;
-        if (Jn >= I + 2) load(Jn, NCH, nx);
+        if (Jn >= I + D) load(Jn, NCH, nx);
         fetch_rows(x, J, -1, nc, vch, spin_limit, err);
 #pragma unroll
         for (int c = 0; c < NCH; ++c)
@@ -1412,6 +1414,66 @@ __global__ void __launch_bounds__(256) k_bwd_sweep_2(const R* __restrict__ b2, c
     part[q][row] = acc;
     __syncthreads();
     if (q == 0) put_value(&x[I * T + row], part[0][row] + part[1][row] + part[2][row] + part[3][row]);
+}
+
+// ---- deeper look-ahead -------------------------------------------------------
+// A look-ahead system of depth D (row I coupled to row I-D and rows further
+// back, slot s of row I holding the tile for J = I-D-s, ws slots) is deepened
+// by substituting row I-D's own equation for the coupling P_I (slot 0) to it:
+//   L+_IJ = L_IJ - P_I L_{I-D,J},  b+_I = b_I - P_I b_{I-D},
+// which couples row I to rows I-D-1 and further back in ws + D - 1 slots: slot
+// s (J = I-D-1-s) is slot s+1 of row I less P_I times slot s+1-D of row I-D.
+// The backward system deepens the same way towards the later rows (J =
+// I+D+1+s, row I+D): dir = +1 forward, -1 backward.  Every slot whose row J
+// exists is written (zero where neither term has a tile).
+template <class R>
+__global__ void __launch_bounds__(NTH) k_look_deepen(const R* __restrict__ src, int ws, R* __restrict__ dst, int wd,
+                                                     long nt, int D, int dir) {
+    __shared__ R xs[T][T];
+    __shared__ R ys[T][T];
+    const long I = blockIdx.x / wd;
+    const int s = int(blockIdx.x % wd);
+    const long J = I - long(dir) * (D + 1 + s);
+    if (J < 0 || J >= nt) return;
+    R* out = dst + (I * wd + s) * (long)TT;
+    const R* keep = (s + 1 <= ws - 1) ? src + (I * ws + s + 1) * (long)TT : nullptr;
+    const int sr = s + 1 - D;   // <= ws - 1 for every s < wd
+    if (sr < 0) {
+        for (int idx = threadIdx.x; idx < TT; idx += NTH) out[idx] = keep ? keep[idx] : R(0);
+        return;
+    }
+    const long Ir = I - long(dir) * D;
+    stage<false>(xs, src + I * ws * (long)TT);   // P_I, column-major: xs[k][row] = P_I[row][k]
+    const R* y = src + (Ir * ws + sr) * (long)TT;
+    for (int idx = threadIdx.x; idx < TT; idx += NTH) ys[idx % T][idx / T] = y[idx];
+    __syncthreads();
+    TileAcc<R> acc;
+    acc.product(xs, ys);
+    if (keep)
+        acc.each([&](int row, int col, R v) { out[col * T + row] = keep[col * T + row] - v; });
+    else
+        acc.each([&](int row, int col, R v) { out[col * T + row] = -v; });
+}
+
+// b+_I = b_I - P_I b_{I-dir*D} with P_I slot 0 of row I of the depth-D
+// system, one workgroup per tile row
+template <class R>
+__global__ void __launch_bounds__(256) k_look_deepen_rhs(const R* __restrict__ src, int ws,
+                                                         const double* __restrict__ b, double* __restrict__ bo,
+                                                         long nt, int D, int dir) {
+    __shared__ double part[4][T];
+    const long I = blockIdx.x;
+    const int tid = threadIdx.x, row = tid % T, q = tid / T, k0 = 16 * q;
+    const long Ir = I - long(dir) * D;
+    double u = (q == 0) ? b[I * T + row] : 0.0;
+    if (Ir >= 0 && Ir < nt) {
+        const R* pv = src + I * ws * (long)TT;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) u -= double(pv[(k0 + k) * T + row]) * b[Ir * T + k0 + k];
+    }
+    part[q][row] = u;
+    __syncthreads();
+    if (tid < T) bo[I * T + tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
 }
 
 // v = SENT
@@ -1899,13 +1961,14 @@ void BandLDLT<R>::setup(long n_, long bandwidth, hipStream_t s_) {
 
 template <class R>
 void BandLDLT<R>::release() {
-    void* ps[] = {tiles, tiles_t, linv, mch, gch, piv, xw, yw, flag, pflags, l2, w2, bp, b2, v2, zp};
+    void* ps[] = {tiles, tiles_t, linv, mch, gch, piv, xw, yw, flag, pflags, l2, w2, bp, b2, v2, zp, l3, b3, bq, zq};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     tiles = nullptr; tiles_t = nullptr; linv = nullptr; piv = nullptr; xw = nullptr; yw = nullptr; flag = nullptr;
     mch = nullptr; gch = nullptr;
     l2 = nullptr; w2 = nullptr; bp = nullptr;
     b2 = nullptr; v2 = nullptr; zp = nullptr;
+    l3 = nullptr; b3 = nullptr; bq = nullptr; zq = nullptr;
     pflags = nullptr;
     clock.release();
 }
@@ -1972,7 +2035,7 @@ int BandLDLT<R>::factor(const Dev& d) {
         const bool tk_prep = clock.take(KT_PREP);
         if (tk_prep) clock.begin(KT_PREP, s);
         hipLaunchKernelGGL((k_sweep_chain<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, mch, gch, nt, w);
-        if (look2 && nt >= 2) {
+        if (look >= 2 && nt >= 2) {
             if (!l2) {
                 l2 = dalloc<R>(size_t(nt) * size_t(w) * TT);
                 w2 = dalloc<R>(size_t(nt) * TT);
@@ -1985,6 +2048,18 @@ int BandLDLT<R>::factor(const Dev& d) {
             hipLaunchKernelGGL((k_look2_band<R>), dim3(int(nt * w)), dim3(NTH), 0, s, tiles_t, linv, w2, l2, nt, w);
             hipLaunchKernelGGL((k_look2_v<R>), dim3(int(nt - 1)), dim3(NTH), 0, s, gch, linv, v2, nt);
             hipLaunchKernelGGL((k_look2_bband<R>), dim3(int(nt * w)), dim3(NTH), 0, s, tiles, linv, v2, b2, nt, w);
+            if (look >= 3) {
+                if (!l3) {
+                    l3 = dalloc<R>(size_t(nt) * size_t(w + 1) * TT);
+                    b3 = dalloc<R>(size_t(nt) * size_t(w + 1) * TT);
+                    bq = dalloc<double>(size_t(nt) * T);
+                    zq = dalloc<double>(size_t(nt) * T);
+                }
+                hipLaunchKernelGGL((k_look_deepen<R>), dim3(int(nt * (w + 1))), dim3(NTH), 0, s, l2, w, l3, w + 1,
+                                   nt, 2, 1);
+                hipLaunchKernelGGL((k_look_deepen<R>), dim3(int(nt * (w + 1))), dim3(NTH), 0, s, b2, w, b3, w + 1,
+                                   nt, 2, -1);
+            }
         }
         if (tk_prep) clock.end(s);
         CB_CHECK(hipGetLastError());
@@ -2006,15 +2081,22 @@ void BandLDLT<R>::solve(const double* rhs, double* out) {
     CB_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_sweep_prep, dim3(int((nt * T + 255) / 256)), dim3(256), 0, s, xw, rhs, yw, n, nt * T);
     const long spin_limit = 1L << 22;
-    const bool ahead = chained && look2 && l2;
+    const bool ahead = chained && look >= 2 && l2;
+    const bool deep = ahead && look >= 3 && l3;
     if (ahead) {
         hipLaunchKernelGGL((k_look2_rhs<R>), dim3(int(nt)), dim3(256), 0, s, linv, w2, xw, bp, nt);
+        if (deep)
+            hipLaunchKernelGGL((k_look_deepen_rhs<R>), dim3(int(nt)), dim3(256), 0, s, l2, w, bp, bq, nt, 2, 1);
         hipLaunchKernelGGL(k_fill_sent, dim3(int((nt * T + 255) / 256)), dim3(256), 0, s, xw, nt * T);
     }
     const bool tk_fwd = clock.take(KT_FWD);
     if (tk_fwd) clock.begin(KT_FWD, s);
-    if (ahead)
-        hipLaunchKernelGGL((k_fwd_sweep_2<R>), dim3(int(nt)), dim3(256), 0, s, l2, bp, yw, nt, w, spin_limit, flag);
+    if (deep)
+        hipLaunchKernelGGL((k_fwd_sweep_la<R>), dim3(int(nt)), dim3(256), 0, s, l3, bq, yw, nt, w + 1, 3, spin_limit,
+                           flag);
+    else if (ahead)
+        hipLaunchKernelGGL((k_fwd_sweep_la<R>), dim3(int(nt)), dim3(256), 0, s, l2, bp, yw, nt, w, 2, spin_limit,
+                           flag);
     else if (chained)
         hipLaunchKernelGGL((k_fwd_sweep_m<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, mch, xw, yw, nt, w,
                            spin_limit, flag);
@@ -2024,10 +2106,16 @@ void BandLDLT<R>::solve(const double* rhs, double* out) {
     if (tk_fwd) clock.end(s);
     if (ahead)
         hipLaunchKernelGGL((k_look2_brhs<R>), dim3(int(nt)), dim3(256), 0, s, linv, v2, yw, piv, zp, nt);
+    if (deep)
+        hipLaunchKernelGGL((k_look_deepen_rhs<R>), dim3(int(nt)), dim3(256), 0, s, b2, w, zp, zq, nt, 2, -1);
     const bool tk_bwd = clock.take(KT_BWD);
     if (tk_bwd) clock.begin(KT_BWD, s);
-    if (ahead)
-        hipLaunchKernelGGL((k_bwd_sweep_2<R>), dim3(int(nt)), dim3(256), 0, s, b2, zp, xw, nt, w, spin_limit, flag);
+    if (deep)
+        hipLaunchKernelGGL((k_bwd_sweep_la<R>), dim3(int(nt)), dim3(256), 0, s, b3, zq, xw, nt, w + 1, 3, spin_limit,
+                           flag);
+    else if (ahead)
+        hipLaunchKernelGGL((k_bwd_sweep_la<R>), dim3(int(nt)), dim3(256), 0, s, b2, zp, xw, nt, w, 2, spin_limit,
+                           flag);
     else if (chained)
         hipLaunchKernelGGL((k_bwd_sweep_m<R>), dim3(int(nt)), dim3(256), 0, s, tiles_t, linv, gch, yw, piv, xw, nt, w,
                            spin_limit, flag);

@@ -58,8 +58,8 @@ Analysis::Analysis(Model& m, Dev& d, const RunOptions& o) : m_(m), d_(d), o_(o) 
     b32_.split = o.factor_schedule == 1;
     b64_.chained = o.sweep_schedule != 1;
     b32_.chained = o.sweep_schedule != 1;
-    b64_.look2 = o.sweep_schedule == 2;
-    b32_.look2 = o.sweep_schedule == 2;
+    b64_.look = (o.sweep_schedule == 2 || o.sweep_schedule == 3) ? o.sweep_schedule : 0;
+    b32_.look = b64_.look;
     if (o.solver != SOLVER_BAND64 && o.solver != SOLVER_BAND32) gm_.setup(d.nslot, o.restart);
     CB_CHECK(hipStreamSynchronize(d.s));
 }

@@ -59,7 +59,7 @@ NEV = 7
 KT_NAMES = ("band_column", "band_diag", "band_panel", "band_update", "band_fwd_sweep", "band_bwd_sweep",
             "band_sweep_prep")
 FACTOR_FUSED, FACTOR_SPLIT = 0, 1
-SWEEP_CHAIN, SWEEP_ROWS, SWEEP_LOOK2 = 0, 1, 2
+SWEEP_CHAIN, SWEEP_ROWS, SWEEP_LOOK2, SWEEP_LOOK3 = 0, 1, 2, 3
 
 
 def options(solver: int = SOLVER_BAND64, krylov_tol: float = 1e-8, restart: int = 30,

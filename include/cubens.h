@@ -101,8 +101,10 @@ enum { CB_FACTOR_FUSED = 0, CB_FACTOR_SPLIT = 1 };
  * a look-ahead system whose rows depend on the rows two and more back, its
  * band (twice the memory of the factor's) formed after each factorization:
  * faster sweeps for the drivers that solve many times per factorization
- * (arc length, modified NR). */
-enum { CB_SWEEP_CHAIN = 0, CB_SWEEP_ROWS = 1, CB_SWEEP_LOOK2 = 2 };
+ * (arc length, modified NR).  CB_SWEEP_LOOK3 deepens that system once more
+ * (rows three and more back, by substituting row I-2's equation into row I;
+ * one more band each way). */
+enum { CB_SWEEP_CHAIN = 0, CB_SWEEP_ROWS = 1, CB_SWEEP_LOOK2 = 2, CB_SWEEP_LOOK3 = 3 };
 
 /* Generations of the per-element state (base = last converged, _i = current
  * iterate, _ip = previous iterate), as the reference's x/x_temp/x_ip, c1/c1_i/

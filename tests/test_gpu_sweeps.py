@@ -5,7 +5,9 @@ both give a normwise backward error at the FP64 round-off level for random
 right-hand sides, on models whose band shapes cover the kernels' edge cases --
 a single tile row (nt = 1), odd and even numbers of tile rows, and 1, 2 and
 more tiles below the diagonal -- and the arc-length paths of both schedules
-agree with the oracle."""
+agree with the oracle.  The look-ahead systems (CB_SWEEP_LOOK2, rows coupled
+two and more back, and CB_SWEEP_LOOK3, three and more back) are held to the
+same bars."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -51,7 +53,7 @@ def test_chain_and_row_sweeps_solve_the_tangent(case):
         rng = np.random.default_rng(nt * 31 + w)
         b = rng.standard_normal(g.neq)
         xs = {}
-        for sch in (cubens.SWEEP_CHAIN, cubens.SWEEP_ROWS, cubens.SWEEP_LOOK2):
+        for sch in (cubens.SWEEP_CHAIN, cubens.SWEEP_ROWS, cubens.SWEEP_LOOK2, cubens.SWEEP_LOOK3):
             det, piv = g.factor(cubens.options(cubens.SOLVER_BAND64, sweep_schedule=sch))
             xs[sch] = g.solve(b)
         knorm = np.abs(K).sum(axis=1).max()
@@ -63,9 +65,10 @@ def test_chain_and_row_sweeps_solve_the_tangent(case):
         # the schedules differ only in rounding
         assert relerr(xs[cubens.SWEEP_CHAIN], xs[cubens.SWEEP_ROWS]) < 1e-8
         assert relerr(xs[cubens.SWEEP_LOOK2], xs[cubens.SWEEP_CHAIN]) < 1e-8
+        assert relerr(xs[cubens.SWEEP_LOOK3], xs[cubens.SWEEP_CHAIN]) < 1e-8
 
 
-@pytest.mark.parametrize("sch", [0, 1, 2])
+@pytest.mark.parametrize("sch", [0, 1, 2, 3])
 def test_panel_path_of_every_sweep_schedule_matches_the_oracle(sch):
     path = model_file("sweep_panel16_path.txt", inputs.shell_panel(n=16).text)
     with cubens.GpuModel(path) as g:
