@@ -279,64 +279,13 @@ struct LowerAcc {
 };
 
 // ---- D_K --------------------------------------------------------------------
-// Both versions take the 64x64 tile S (lower part; the part above the
-// diagonal is never read) in LDS as xs[col][row], write D to piv and
-// Linv = L^{-1} (zeros above the diagonal) to lout, and overwrite xs.  L_KK
-// itself is read by neither the panels and updates nor the solves and is not
-// stored.
-
-// FP32: unblocked.  Thread (r = tid % 64, g = tid / 64) keeps the entries
-// (r, j), j = g + NWAVE m, in registers: the Schur complement a[r][j] while j
-// is still ahead of the elimination step c, afterwards the entry B[r][j] of
-// B = L^{-1}, which the same elimination applied to the identity produces
-// (B[r][:] -= l_r B[c][:]).  Step c broadcasts one vector through LDS: column
-// c of the Schur complement in its entries j >= c and row c of B in its
-// entries j < c (double-buffered, one barrier per step).  The steps are
-// unrolled, so every register index is static; the multiplier is zero on the
-// rows r <= c, which makes the update branch-free.
+// The task takes the 64x64 tile S (lower part; the part above the diagonal is
+// never read) in LDS as xs[col][row], writes D to piv and Linv = L^{-1} (zeros
+// above the diagonal) to lout, and overwrites xs.  L_KK itself is read by
+// neither the panels and updates nor the solves and is not stored.  FP32
+// tiles go through the FP64 task (below).
 template <class R> struct DiagScratch;
-template <> struct DiagScratch<float> { float bc[2][T]; };
 template <> struct DiagScratch<double> { double tsc[5][256]; double ub[16]; };
-
-__device__ __forceinline__ void diag_task(float (*xs)[T], float* /*ys*/, double* /*dd*/, DiagScratch<float>& sm,
-                                          float* __restrict__ lout, double* __restrict__ piv, long Kt) {
-    constexpr int S = T / NWAVE;
-    const int r = threadIdx.x & 63;
-    const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    float v[S];
-#pragma unroll
-    for (int m = 0; m < S; ++m) {
-        const int j = g + NWAVE * m;
-        const float x = xs[j][r];
-        v[m] = (j <= r) ? x : 0.0f;
-    }
-    float dr = 0.0f;
-#pragma unroll
-    for (int c = 0; c < T; ++c) {
-        const int p = c & 1;
-        if (g == c % NWAVE && r >= c) sm.bc[p][r] = v[c / NWAVE];
-        if (r == c) {
-#pragma unroll
-            for (int m = 0; m < S; ++m)
-                if (g + NWAVE * m < c) sm.bc[p][g + NWAVE * m] = v[m];
-        }
-        __syncthreads();
-        const float d = sm.bc[p][c];
-        const float l = sm.bc[p][r] / d;
-        const float le = (r > c) ? l : 0.0f;
-#pragma unroll
-        for (int m = 0; m < S; ++m) v[m] = __builtin_fmaf(-le, sm.bc[p][g + NWAVE * m], v[m]);
-        if (g == c % NWAVE) v[c / NWAVE] = (r > c) ? -l : ((r == c) ? 1.0f : v[c / NWAVE]);
-        dr = (r == c) ? d : dr;
-    }
-    if (g == 0) piv[Kt * T + r] = double(dr);
-#pragma unroll
-    for (int m = 0; m < S; ++m) {
-        const int j = g + NWAVE * m;
-        lout[j * T + r] = (j <= r) ? v[m] : 0.0f;
-    }
-}
-
 // FP64: blocked by 16.  For each column block k:
 //  - wavefront 0 factorizes the 64x16 panel (rows 16k..63, lane = row):
 //    16 pivot steps, each broadcasting column c of the block through LDS; the
@@ -389,9 +338,9 @@ struct NoHook {
 // tail_hook runs once the pivot steps are done (every thread), where few
 // registers are live: a caller issues loads there that it consumes after the
 // task.
-template <class Hook = NoHook>
+template <class Hook = NoHook, class O = double>
 __device__ __forceinline__ void diag_task(double (*xs)[T], double* ys, double* dd, DiagScratch<double>& sm,
-                                          double* __restrict__ lout, double* __restrict__ piv, long Kt,
+                                          O* __restrict__ lout, double* __restrict__ piv, long Kt,
                                           double* lds_out = nullptr, Hook tail_hook = Hook()) {
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, rr = l & 15, lg = l >> 4;
     double(*tsc)[256] = sm.tsc;
@@ -536,9 +485,22 @@ __device__ __forceinline__ void diag_task(double (*xs)[T], double* ys, double* d
     for (int idx = threadIdx.x; idx < TT; idx += NTH) {
         const int col = idx / T, row = idx % T;
         const double v = (row >= col) ? ys[ysw(col, row)] : 0.0;
-        lout[idx] = v;
+        lout[idx] = O(v);
         if (lds_out) lds_out[idx] = v;
     }
+}
+
+// FP32 tiles: the FP64 task on the tile widened in LDS, with Linv rounded once
+// to FP32 and the pivots kept in FP64.  The FP64 task is blocked by 16 and
+// carries each panel on one wavefront; a 64-step FP32 chain with a workgroup
+// barrier per pivot made the FP32 column launch slower than the FP64 one.
+template <> struct DiagScratch<float> { double xd[T][T]; double yd[TT]; DiagScratch<double> in; };
+
+__device__ __forceinline__ void diag_task(float (*xs)[T], float* /*ys*/, double* dd, DiagScratch<float>& sm,
+                                          float* __restrict__ lout, double* __restrict__ piv, long Kt) {
+    for (int idx = threadIdx.x; idx < TT; idx += NTH) (&sm.xd[0][0])[idx] = double((&xs[0][0])[idx]);
+    __syncthreads();
+    diag_task(sm.xd, sm.yd, dd, sm.in, lout, piv, Kt);
 }
 
 // ---- staging ----------------------------------------------------------------
