@@ -462,6 +462,8 @@ def main():
                     help="panel: the 256x256 shell panel MSAL (the headline); frames: the 512-frame "
                          "nonlinear Newmark batch (--steps time steps); learned: the learned "
                          "preconditioner against block-Jacobi on held-out panels")
+    ap.add_argument("--refine", type=int, default=1,
+                    help="FP64-residual refinement steps per solve of the fp32 path")
     ap.add_argument("--sweeps", default="look2", choices=("chain", "rows", "look2", "look3"),
                     help="band solve sweeps: chain matrices (the library's default), the two-product "
                          "row step, or both sweeps on the look-ahead system (rows coupled two back; "
@@ -504,7 +506,7 @@ def main():
         w = args.warmup if name != "mixed" else min(args.warmup, 1)
         log(f"[bench] path {name}: {w} warm-up increments, then increments 1..{k} timed")
         res = run_path(path, solver, w, k, args.tol, timing=127 if name == "fp64" else 0, schedule=schedule,
-                       refine=1 if name == "fp32" else 0,
+                       refine=args.refine if name == "fp32" else 0,
                        sweeps={"chain": 0, "rows": 1, "look2": 2, "look3": 3}[args.sweeps])
         st = res["stats"]
         log(f"[bench] {name}: {res['rate']:.2f} Newton it/s ({res['newton']} its in {res['seconds']:.2f}s); "
@@ -521,8 +523,9 @@ def main():
         base = window_rate(plain, km)
         err = path_error(plain, m)
         extra["precision"] = {
-            "path": "fp32: FP32 shell stiffness and band LDL^T, one FP64-residual refinement step per "
-                    "solve; FP64 forces, residual, arc-length constraint and convergence tests",
+            "path": f"fp32: FP32 shell stiffness and band LDL^T, {args.refine} FP64-residual refinement "
+                    "step(s) per solve; FP64 forces, residual, arc-length constraint and convergence tests",
+            "refine": args.refine,
             "increments": km, "value": m["rate"], "plain_fp64_same_increments": base,
             "ratio_vs_plain_fp64": m["rate"] / base,
             "newton": int(m["newton"]), "newton_fp64": int(plain["step_newton"][km - 1]), **err}
@@ -544,9 +547,10 @@ def main():
             f"{m['rate'] / base:.3f}")
     sky = skyline_work(head["kht"]) if head is plain else None
     # the counters were read on the 256x256 panel's FP64 band with the fused schedule
+    # kernel timings are taken on the FP64 path only; without it the line has no roofline
     roof = roofline(head["info"], head["ktimes"],
                     default_workload=(args.n == 256 and plain is not None and args.schedule == "fused"),
-                    sky=sky, sweeps=args.sweeps)
+                    sky=sky, sweeps=args.sweeps) if head.get("ktimes") else None
     log(f"[bench] kernel times (ms, launches): {head['ktimes']}")
     cpu = None
     if not args.no_cpu:
