@@ -1190,55 +1190,58 @@ __global__ void __launch_bounds__(256) k_look2_rhs(const R* __restrict__ linv, c
     if (tid < T) bp[I * T + tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
 }
 
+// The look-ahead sweeps take their older rows NCW at a time: a workgroup
+// catching up behind the front consumes one chunk per memory round trip, so
+// the front advances at most NCW rows per round trip whatever the depth of
+// the system.  Eight waves, thread (row, q) holding columns 8q..8q+7 of each
+// tile; a chunk's tile loads are issued before its rows are polled and share
+// that round trip.
+constexpr int NCW = 8;
+constexpr int SWT = 512;
+
 // y_I = b'_I - sum_{J=I-w-D+1}^{I-D} L'_IJ y_J on a look-ahead system of
-// depth D (slot s of row I for J = I-D-s, w slots), over the rows in chunks of
-// NCH, polled side by side, aligned so that the last chunk ends at row I-D: a
-// row waits once for the newest rows it needs and the D chains of the system
-// (the rows of each residue mod D) run side by side.  The tiles of the next
-// chunk are loaded while the current one is polled, so that the products
-// after a chunk's rows land read registers only.  y must hold SENT.
+// depth D (slot s of row I for J = I-D-s, w slots), the older rows in chunks
+// of NCW aligned so that the last chunk ends at row I-D: a row waits once for
+// the newest rows it needs and the D chains of the system (the rows of each
+// residue mod D) run side by side.  y must hold SENT.
 template <class R>
-__global__ void __launch_bounds__(256) k_fwd_sweep_la(const R* __restrict__ l2, const double* __restrict__ bp,
+__global__ void __launch_bounds__(SWT) k_fwd_sweep_la(const R* __restrict__ l2, const double* __restrict__ bp,
                                                       double* y, long nt, int w, int D, long spin_limit, int* err) {
-    __shared__ double vch[NCH][T];
-    __shared__ double part[4][T];
+    __shared__ double vch[NCW][T];
+    __shared__ double part[SWT / T][T];
     const long I = blockIdx.x;
-    const int tid = threadIdx.x, row = tid % T, q = tid / T, k0 = 16 * q;
+    const int tid = threadIdx.x, row = tid % T, q = tid / T, k0 = 8 * q;
     double acc = (q == 0) ? bp[I * T + row] : 0.0;
     const long J0 = I - w - D + 1 > 0 ? I - w - D + 1 : 0;
     const long nrow = I - D + 1 - J0;   // rows J0 .. I-D
-    int nc = int(nrow % NCH);
-    if (nc == 0) nc = NCH;
-    auto load = [&](long J, int n, R (*dst)[16]) {
+    int nc = int(nrow % NCW);
+    if (nc == 0) nc = NCW;
+    for (long J = J0; J <= I - D; J += nc, nc = NCW) {
+        R lx[NCW][8];
 #pragma unroll
-        for (int c = 0; c < NCH; ++c)
-            if (c < n) {
+        for (int c = 0; c < NCW; ++c)
+            if (c < nc) {
                 const R* l = l2 + (I * w + (I - D - (J + c))) * (long)TT;
 #pragma unroll
-                for (int k = 0; k < 16; ++k) dst[c][k] = l[(k0 + k) * T + row];
+                for (int k = 0; k < 8; ++k) lx[c][k] = l[(k0 + k) * T + row];
             }
-    };
-    R lx[NCH][16], nx[NCH][16];
-    if (nrow > 0) load(J0, nc, lx);
-    for (long J = J0; J <= I - D; J += nc, nc = NCH) {
-        const long Jn = J + nc;
-        if (Jn <= I - D) load(Jn, NCH, nx);
         fetch_rows(y, J, 1, nc, vch, spin_limit, err);
 #pragma unroll
-        for (int c = 0; c < NCH; ++c)
+        for (int c = 0; c < NCW; ++c)
             if (c < nc) {
 #pragma unroll
-                for (int k = 0; k < 16; ++k) acc -= double(lx[c][k]) * vch[c][k0 + k];
+                for (int k = 0; k < 8; ++k) acc -= double(lx[c][k]) * vch[c][k0 + k];
             }
         __syncthreads();
-#pragma unroll
-        for (int c = 0; c < NCH; ++c)
-#pragma unroll
-            for (int k = 0; k < 16; ++k) lx[c][k] = nx[c][k];
     }
     part[q][row] = acc;
     __syncthreads();
-    if (q == 0) put_value(&y[I * T + row], part[0][row] + part[1][row] + part[2][row] + part[3][row]);
+    if (q == 0) {
+        double v = part[0][row];
+#pragma unroll
+        for (int g = 1; g < SWT / T; ++g) v += part[g][row];
+        put_value(&y[I * T + row], v);
+    }
 }
 
 // The backward sweep x_I = Linv_I^T (z_I - sum_{J>I} L_JI^T x_J), z = y / d,
@@ -1331,51 +1334,46 @@ __global__ void __launch_bounds__(256) k_look2_brhs(const R* __restrict__ linv, 
 
 // x_I = z'_I - sum_{J=I+D}^{I+D+w-1} B_IJ x_J on a look-ahead system of depth
 // D (slot s of row I for J = I+D+s), I = nt-1-blockIdx.x: the later rows in
-// chunks of NCH polled side by side, farthest first, aligned so that the last
-// chunk ends at row I+D; the next chunk's tiles load while the current one is
-// polled.  x must hold SENT.
+// chunks of NCW, farthest first, aligned so that the last chunk ends at row
+// I+D.  x must hold SENT.
 template <class R>
-__global__ void __launch_bounds__(256) k_bwd_sweep_la(const R* __restrict__ b2, const double* __restrict__ zp,
+__global__ void __launch_bounds__(SWT) k_bwd_sweep_la(const R* __restrict__ b2, const double* __restrict__ zp,
                                                       double* x, long nt, int w, int D, long spin_limit, int* err) {
-    __shared__ double vch[NCH][T];
-    __shared__ double part[4][T];
+    __shared__ double vch[NCW][T];
+    __shared__ double part[SWT / T][T];
     const long I = nt - 1 - blockIdx.x;
-    const int tid = threadIdx.x, row = tid % T, q = tid / T, k0 = 16 * q;
+    const int tid = threadIdx.x, row = tid % T, q = tid / T, k0 = 8 * q;
     double acc = (q == 0) ? zp[I * T + row] : 0.0;
     const long J1 = (I + D + w - 1 < nt - 1) ? I + D + w - 1 : nt - 1;
     const long nrow = J1 - (I + D) + 1;   // rows J1 down to I+D
-    int nc = int(nrow % NCH);
-    if (nc == 0) nc = NCH;
-    auto load = [&](long J, int n, R (*dst)[16]) {
+    int nc = int(nrow % NCW);
+    if (nc == 0) nc = NCW;
+    for (long J = J1; J >= I + D; J -= nc, nc = NCW) {
+        R lx[NCW][8];
 #pragma unroll
-        for (int c = 0; c < NCH; ++c)
-            if (c < n) {
+        for (int c = 0; c < NCW; ++c)
+            if (c < nc) {
                 const R* l = b2 + (I * w + (J - c - I - D)) * (long)TT;
 #pragma unroll
-                for (int k = 0; k < 16; ++k) dst[c][k] = l[(k0 + k) * T + row];
+                for (int k = 0; k < 8; ++k) lx[c][k] = l[(k0 + k) * T + row];
             }
-    };
-    R lx[NCH][16], nx[NCH][16];
-    if (nrow > 0) load(J1, nc, lx);
-    for (long J = J1; J >= I + D; J -= nc, nc = NCH) {
-        const long Jn = J - nc;
-        if (Jn >= I + D) load(Jn, NCH, nx);
         fetch_rows(x, J, -1, nc, vch, spin_limit, err);
 #pragma unroll
-        for (int c = 0; c < NCH; ++c)
+        for (int c = 0; c < NCW; ++c)
             if (c < nc) {
 #pragma unroll
-                for (int k = 0; k < 16; ++k) acc -= double(lx[c][k]) * vch[c][k0 + k];
+                for (int k = 0; k < 8; ++k) acc -= double(lx[c][k]) * vch[c][k0 + k];
             }
         __syncthreads();
-#pragma unroll
-        for (int c = 0; c < NCH; ++c)
-#pragma unroll
-            for (int k = 0; k < 16; ++k) lx[c][k] = nx[c][k];
     }
     part[q][row] = acc;
     __syncthreads();
-    if (q == 0) put_value(&x[I * T + row], part[0][row] + part[1][row] + part[2][row] + part[3][row]);
+    if (q == 0) {
+        double v = part[0][row];
+#pragma unroll
+        for (int g = 1; g < SWT / T; ++g) v += part[g][row];
+        put_value(&x[I * T + row], v);
+    }
 }
 
 // ---- deeper look-ahead -------------------------------------------------------
@@ -2054,10 +2052,10 @@ void BandLDLT<R>::solve(const double* rhs, double* out) {
     const bool tk_fwd = clock.take(KT_FWD);
     if (tk_fwd) clock.begin(KT_FWD, s);
     if (deep)
-        hipLaunchKernelGGL((k_fwd_sweep_la<R>), dim3(int(nt)), dim3(256), 0, s, l3, bq, yw, nt, w + 1, 3, spin_limit,
+        hipLaunchKernelGGL((k_fwd_sweep_la<R>), dim3(int(nt)), dim3(SWT), 0, s, l3, bq, yw, nt, w + 1, 3, spin_limit,
                            flag);
     else if (ahead)
-        hipLaunchKernelGGL((k_fwd_sweep_la<R>), dim3(int(nt)), dim3(256), 0, s, l2, bp, yw, nt, w, 2, spin_limit,
+        hipLaunchKernelGGL((k_fwd_sweep_la<R>), dim3(int(nt)), dim3(SWT), 0, s, l2, bp, yw, nt, w, 2, spin_limit,
                            flag);
     else if (chained)
         hipLaunchKernelGGL((k_fwd_sweep_m<R>), dim3(int(nt)), dim3(256), 0, s, tiles, linv, mch, xw, yw, nt, w,
@@ -2073,10 +2071,10 @@ void BandLDLT<R>::solve(const double* rhs, double* out) {
     const bool tk_bwd = clock.take(KT_BWD);
     if (tk_bwd) clock.begin(KT_BWD, s);
     if (deep)
-        hipLaunchKernelGGL((k_bwd_sweep_la<R>), dim3(int(nt)), dim3(256), 0, s, b3, zq, xw, nt, w + 1, 3, spin_limit,
+        hipLaunchKernelGGL((k_bwd_sweep_la<R>), dim3(int(nt)), dim3(SWT), 0, s, b3, zq, xw, nt, w + 1, 3, spin_limit,
                            flag);
     else if (ahead)
-        hipLaunchKernelGGL((k_bwd_sweep_la<R>), dim3(int(nt)), dim3(256), 0, s, b2, zp, xw, nt, w, 2, spin_limit,
+        hipLaunchKernelGGL((k_bwd_sweep_la<R>), dim3(int(nt)), dim3(SWT), 0, s, b2, zp, xw, nt, w, 2, spin_limit,
                            flag);
     else if (chained)
         hipLaunchKernelGGL((k_bwd_sweep_m<R>), dim3(int(nt)), dim3(256), 0, s, tiles_t, linv, gch, yw, piv, xw, nt, w,
